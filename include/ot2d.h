@@ -1,0 +1,162 @@
+/*
+ * ot2d.h -- C-ABI of the MI355X-native OVERTURNS-2D iteration
+ * (residual + LU-SGS implicit update, SA turbulence, forces), gfx950 HIP.
+ *
+ * The boundary replaces the F77 calls the reference driver makes inside its
+ * time loop (src/Turns_overset.f:475-554):
+ *   time_step(q,xx,xy,yx,yy,ug,vg,tscale,bt,iblank,jd,kd)      src/time.f:2
+ *   step(q,...,s,...,turmu,tscale,bt,Ds,im,jd,kd)               src/step.f:2-7
+ *   monitor(mstop,q,jd,kd)                                      src/step.f:332
+ *   compute_forces(jd,kd,x,y,xv,yv,q,xx,...,cl,cd,...)          src/forces.f:2-3
+ * and the set-up it does once (initia, src/init.f:2-177; metfv,
+ * src/setup.f:440; find_cellcenter, src/setup.f:111; qzero/qdivj,
+ * src/init.f:180-277).
+ *
+ * Conventions (those of the reference, so host arrays can be passed as is):
+ *  - arrays are Fortran column-major, cell-centred (jd,kd) with jd=jmax+1,
+ *    kd=kmax+1 (one halo ring, nhalo=1), variable index slowest:
+ *    q(jd,kd,nspec,nq), s(jd,kd,nspec,nv); vertex arrays (jmax,kmax,nspec);
+ *  - q(:,:,:,1:4) hold conserved variables times cell area, q(:,:,:,nq) is
+ *    1/area (src/init.f:197-203, src/setup.f:489-500); q(:,:,:,5..) are the
+ *    turbulence variables as the reference stores them;
+ *  - reals are IEEE binary64, integers int32_t, logicals int32_t 0/1;
+ *  - every entry point returns an int32_t status: OT2D_OK, or
+ *    OT2D_MSTOP (negative speed of sound, maps to the reference's mstop=1,
+ *    src/step.f:353-385), OT2D_NORM (l2 norm > 1000, the reference's
+ *    "stop 'norm'", src/step.f:105-110), or a negative error code.
+ *  - a case handle is bound to the HIP device current at creation and to one
+ *    HIP stream; calls on one handle are not re-entrant.
+ */
+#ifndef OT2D_H
+#define OT2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OT2D_MAXBC 25
+
+enum {
+  OT2D_OK = 0,
+  OT2D_MSTOP = 1,
+  OT2D_NORM = 2,
+  OT2D_ERR_ARG = -1,
+  OT2D_ERR_HIP = -2,
+  OT2D_ERR_UNSUPPORTED = -3,
+  OT2D_ERR_STATE = -4,
+  OT2D_ERR_NOMEM = -5
+};
+
+/* One &bcinp row (src/setup.f:894-931; index rules src/bc.f:21-35). */
+typedef struct ot2d_bcseg {
+  int32_t ibtyp, ibdir, jbcs, jbce, kbcs, kbce;
+} ot2d_bcseg;
+
+/* Per-mesh description: vertex dimensions and the bc.inp table. */
+typedef struct ot2d_mesh_desc {
+  int32_t jmax, kmax;
+  int32_t nbc;
+  int32_t irot_dir;
+  double xcen, ycen;
+  ot2d_bcseg bc[OT2D_MAXBC];
+} ot2d_mesh_desc;
+
+/* The &inputs namelist values the iteration reads (src/modules.f:170-208)
+ * plus the fixed data of params_global.  Fill with ot2d_default_params()
+ * (the reference's data statements) and override what the deck sets. */
+typedef struct ot2d_params {
+  int32_t iread, nhalo, half;
+  int32_t invisc, lamin, iturb, itrans;
+  int32_t irhsy, ilim, ilhs, iconstant_cfl, idual, iprecon;
+  int32_t ntac, iunst, itnmax, timespectral, nspec, motiontype;
+  int32_t nsteps, npnorm, nrest;
+  double fsmach, alfa, rey, vnuinf, tuinf;
+  double dt, timeac, epse, dualtime, Mp;
+  double rf, angmax, fmtip, rartio, totime;
+  double gamma, pr, rmue, tinf;
+  /* extensions (no reference counterpart): arithmetic of the LU-SGS
+   * operator, 0 = binary64 (reference), 1 = binary32 sweeps */
+  int32_t lu_fp32;
+  int32_t reserved[7];
+} ot2d_params;
+
+typedef struct ot2d_case ot2d_case;
+
+/* Fields readable with ot2d_get_field (all (jd,kd,nspec) unless noted). */
+enum {
+  OT2D_F_X = 0, OT2D_F_Y, OT2D_F_XX, OT2D_F_XY, OT2D_F_YX, OT2D_F_YY,
+  OT2D_F_UG, OT2D_F_VG, OT2D_F_TSCALE, OT2D_F_BT, OT2D_F_TURMU,
+  OT2D_F_S,          /* (jd,kd,nspec,nv) */
+  OT2D_F_XV, OT2D_F_YV, /* (jmax,kmax,nspec) */
+  OT2D_F_UGV, OT2D_F_VGV,
+  OT2D_F_COUNT
+};
+
+/* Defaults of params_global data statements (src/modules.f:170-193). */
+int32_t ot2d_default_params(ot2d_params* p);
+
+/* Derived sizes for mesh im of a case: jd, kd, nq, nv. */
+int32_t ot2d_sizes(const ot2d_params* p, const ot2d_mesh_desc* m,
+                   int32_t* jd, int32_t* kd, int32_t* nq, int32_t* nv);
+
+/* Allocate device state for nmesh meshes (replaces the allocation block,
+ * src/Turns_overset.f:191-237, and read_inputs' derived constants,
+ * src/setup.f:1162-1265). */
+int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh,
+                         const ot2d_mesh_desc* meshes, ot2d_case** out);
+int32_t ot2d_case_destroy(ot2d_case* c);
+
+/* Upload the vertex grid of mesh im (jmax,kmax) and initialise it as
+ * initia does (src/init.f:2-177): cell centres, finite-volume metrics,
+ * free stream (qzero), q/J (qdivj), iblank=1, turmu=ug=vg=0, istep0=0. */
+int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, const double* yv);
+
+/* Internal state q(jd,kd,nspec,nq) of mesh im (as the reference holds it). */
+int32_t ot2d_set_state(ot2d_case* c, int32_t im, const double* q);
+int32_t ot2d_get_state(ot2d_case* c, int32_t im, double* q);
+int32_t ot2d_set_iblank(ot2d_case* c, int32_t im, const int32_t* iblank);
+int32_t ot2d_get_iblank(ot2d_case* c, int32_t im, int32_t* iblank);
+int32_t ot2d_get_field(ot2d_case* c, int32_t im, int32_t field, double* out);
+int32_t ot2d_set_field(ot2d_case* c, int32_t im, int32_t field, const double* in);
+
+/* istep0 counter (src/Turns_overset.f:364; drives the wall-BC ramp,
+ * src/bc.f:1779-1793). */
+int32_t ot2d_set_istep0(ot2d_case* c, int32_t istep0);
+int32_t ot2d_get_istep0(ot2d_case* c, int32_t* istep0);
+
+/* Stage entry points, one per reference subroutine, acting on the
+ * device-resident state of every mesh of the case (used by parity tests
+ * and by drivers that interleave their own work). */
+int32_t ot2d_time_step(ot2d_case* c);                     /* src/time.f:2       */
+int32_t ot2d_bc(ot2d_case* c);                            /* src/bc.f:2         */
+int32_t ot2d_compute_residual(ot2d_case* c);              /* src/step.f:221     */
+int32_t ot2d_rbc(ot2d_case* c);                           /* src/bc.f:88        */
+int32_t ot2d_lhs(ot2d_case* c);    /* s*=tscale, ilu2d   src/step.f:141-159, src/lhs.f:470 */
+int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop);       /* src/step.f:332     */
+
+/* One full step of every mesh (src/step.f:2-218) followed by monitor, as
+ * the driver's inner loop does (src/Turns_overset.f:475-498).  norms
+ * receives rsum,resrho,resmax per mesh and instance (3*nmesh*nspec
+ * doubles, may be NULL); argmax receives j,k,n of resmax (3 ints each, may
+ * be NULL).  istep0 is advanced by one first. */
+int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax);
+
+/* niter steps without host synchronisation in between; the residual
+ * history (rsum per step per mesh/instance) is written to hist
+ * (niter*nmesh*nspec doubles, may be NULL).  Stops early on MSTOP/NORM and
+ * reports the number of steps done in *done. */
+int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32_t* done);
+
+/* Sectional forces of mesh im, instance nsp (0-based): out = cl, cd, cm,
+ * cpower, cfx, cfy (src/forces.f:2-107, force2d src/forces.f:111-398). */
+int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]);
+
+/* Error text of the last failing call on this thread. */
+const char* ot2d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OT2D_H */

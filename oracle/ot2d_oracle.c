@@ -1,0 +1,1772 @@
+/*
+ * ot2d_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * Plain-C restatement of the reference OVERTURNS-2D iteration on host
+ * memory.  Every routine names the reference file:line it follows.  Operation
+ * order follows the Fortran expressions left to right (no contraction:
+ * built with -ffp-contract=off) so that the restatement agrees with the
+ * reference built by oracle/Makefile bit for bit where the reference's own
+ * arithmetic is deterministic; tests check that against oracle/_ref dumps and
+ * the committed fixtures under tests/golden/.
+ *
+ * Layout: Fortran column-major, 1-based indices in the macros below.
+ */
+#include "ot2d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MIN(a, b) ((a) < (b) ? (a) : (b))
+#define MAX(a, b) ((a) > (b) ? (a) : (b))
+
+typedef struct or_mesh {
+  int jmax, kmax, jd, kd, jbeg, jend, kbeg, kend;
+  int jtail1, jtail2, jle, jtail_v;
+  int bodyflag;
+  double rotf, xac, yac;
+  ot2d_mesh_desc d;
+  size_t n2;  /* jd*kd */
+  size_t nv2; /* jmax*kmax */
+  double *q, *s, *x, *y, *xv, *yv, *xx, *xy, *yx, *yy, *ug, *vg, *ugv, *vgv;
+  double *turmu, *tscale, *bt;
+  int32_t* iblank;
+} or_mesh;
+
+struct or_case {
+  ot2d_params p;
+  /* derived (read_inputs, src/setup.f:1162-1248) */
+  double gm1, ggm1, einf, pinf, rinf, ainf, htinf, uinf, vinf, rey, h, pi;
+  double rmue, tkeinf, tomegainf;
+  int lamin, iturb, nq, nv, nmv, nspec, timespectral, iunst;
+  int nmesh, istep0, istep, mdim;
+  double* Ds;
+  double* tspec;
+  or_mesh* m;
+};
+
+/* ---------------------------------------------------------------- access */
+#define IX(M, j, k) ((size_t)((j)-1) + (size_t)((k)-1) * (size_t)(M)->jd)
+#define QA(M, nsp, n) ((M)->q + ((size_t)((n)-1) * c->nspec + (size_t)(nsp)) * (M)->n2)
+#define SA(M, nsp, n) ((M)->s + ((size_t)((n)-1) * c->nspec + (size_t)(nsp)) * (M)->n2)
+#define GA(M, arr, nsp) ((M)->arr + (size_t)(nsp) * (M)->n2)
+#define VA(M, arr, nsp) ((M)->arr + (size_t)(nsp) * (M)->nv2)
+#define VX(M, j, k) ((size_t)((j)-1) + (size_t)((k)-1) * (size_t)(M)->jmax)
+
+static double sq(double a) { return a * a; }
+static double cube(double a) { return a * a * a; }
+/* Fortran SIGN(a,b) */
+static double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+/* ------------------------------------------------------------- defaults */
+int32_t or_default_params(ot2d_params* p) {
+  /* data statements, src/modules.f:170-193 */
+  memset(p, 0, sizeof(*p));
+  p->gamma = 1.4; p->pr = 0.72; p->rmue = 1.0; p->tinf = 400.0;
+  p->iread = 0; p->nhalo = 1; p->half = 0;
+  p->nsteps = 500; p->nrest = 1000; p->npnorm = 25;
+  p->timespectral = 0; p->nspec = 1;
+  p->fsmach = 0.16; p->fmtip = 0.0; p->alfa = 0.0; p->rartio = 1.0; p->rey = 3900000.0;
+  p->invisc = 1; p->lamin = 1; p->iturb = 1; p->itrans = 0;
+  p->vnuinf = 0.1; p->tuinf = 0.2;
+  p->iunst = 0; p->ntac = 1; p->itnmax = 1; p->dt = 0.1; p->timeac = 1.0; p->motiontype = 1;
+  p->epse = 0.01; p->irhsy = -3; p->ilhs = 1; p->ilim = 0; p->totime = 0.0; p->rf = 0.0;
+  p->angmax = 0.0;
+  p->idual = 0; p->dualtime = 1.0; p->iconstant_cfl = 0;
+  p->iprecon = 0; p->Mp = 1.0;
+  return OT2D_OK;
+}
+
+int32_t or_sizes(const ot2d_params* p, const ot2d_mesh_desc* m, int32_t* jd, int32_t* kd,
+                 int32_t* nq, int32_t* nv) {
+  int nadd = 2 * p->nhalo - 1; /* src/setup.f:1263 */
+  int iturb = p->invisc ? -1 : (p->lamin ? -1 : p->iturb);
+  int q = 5, v = 4;
+  if (iturb == 1) { q += 1; v += 1; }
+  if (iturb == 2) { q += 2; v += 2; }
+  if ((iturb == 1 || iturb == 2) && p->itrans == 1) { q += 2; v += 2; }
+  if (jd) *jd = m->jmax + nadd;
+  if (kd) *kd = m->kmax + nadd;
+  if (nq) *nq = q;
+  if (nv) *nv = v;
+  return OT2D_OK;
+}
+
+/* ---------------------------------------------------------------- create */
+int32_t or_case_create(const ot2d_params* p, int32_t nmesh, const ot2d_mesh_desc* md,
+                       or_case** out) {
+  if (!p || !md || nmesh < 1 || !out) return OT2D_ERR_ARG;
+  if (p->nhalo != 1) return OT2D_ERR_UNSUPPORTED;
+  or_case* c = (or_case*)calloc(1, sizeof(or_case));
+  c->p = *p;
+  /* read_inputs, src/setup.f:963-1265 */
+  c->timespectral = p->timespectral;
+  c->iunst = p->timespectral ? 0 : p->iunst;
+  c->nspec = p->timespectral ? p->nspec : 1;
+  c->rmue = p->rmue;
+  c->lamin = p->lamin;
+  c->iturb = p->iturb;
+  if (p->invisc) { c->rmue = 0.0; c->lamin = 1; }
+  if (c->lamin) c->iturb = -1;
+  c->pi = 4.0 * atan(1.0);
+  c->gm1 = p->gamma - 1.0;
+  c->ggm1 = p->gamma * c->gm1;
+  double cs = cos(c->pi * p->alfa / 180.0);
+  double ss = sin(c->pi * p->alfa / 180.0);
+  c->einf = 1.0 / c->ggm1 + 0.5 * sq(p->fsmach);
+  c->pinf = 1.0 / p->gamma;
+  c->rinf = 1.0;
+  c->ainf = p->gamma * c->pinf / c->rinf;
+  c->htinf = p->gamma * c->einf / c->rinf - 0.5 * c->gm1 * sq(p->fsmach);
+  c->nq = 5; c->nv = 4; c->nmv = 4;
+  if (c->iturb == 1) { c->nq += 1; c->nv += 1; }
+  else if (c->iturb == 2) { c->nq += 2; c->nv += 2; }
+  if ((c->iturb == 1 || c->iturb == 2) && p->itrans == 1) { c->nq += 2; c->nv += 2; }
+  c->uinf = p->fsmach * cs;
+  c->vinf = p->fsmach * ss;
+  c->rey = (p->fsmach != 0.0) ? p->rey / p->fsmach : p->rey / p->fmtip;
+  c->h = p->dt;
+  if (p->itrans == 0) {
+    c->tkeinf = 1e-6 * sq(p->fsmach);
+    c->tomegainf = 5 * p->fsmach / c->rey;
+  } else {
+    double tu = p->tuinf;
+    c->tkeinf = sq(tu / 100 * p->fsmach) * 1.5;
+    c->tomegainf = c->tkeinf / p->vnuinf;
+  }
+  c->nmesh = nmesh;
+  c->mdim = 0;
+  c->m = (or_mesh*)calloc((size_t)nmesh, sizeof(or_mesh));
+  for (int im = 0; im < nmesh; ++im) {
+    or_mesh* M = &c->m[im];
+    M->d = md[im];
+    M->jmax = md[im].jmax;
+    M->kmax = md[im].kmax;
+    M->jd = M->jmax + 1;
+    M->kd = M->kmax + 1;
+    c->mdim = MAX(c->mdim, MAX(M->jd, M->kd));
+    /* set_pointers_globals, src/setup.f:55-108 */
+    M->jbeg = 2; M->kbeg = 2; M->jend = M->jd - 1; M->kend = M->kd - 1;
+    M->bodyflag = 0;
+    M->jtail_v = 1; /* initia, src/init.f:59-66 */
+    for (int ib = 0; ib < md[im].nbc; ++ib)
+      if (md[im].bc[ib].ibtyp == 5) { M->bodyflag = 1; M->jtail_v = md[im].bc[ib].jbcs; }
+    M->jtail1 = M->jtail_v + 1;
+    if (p->half == 0) { M->jle = (M->jd + 1) / 2; M->jtail2 = M->jd - M->jtail1 + 1; }
+    else { M->jle = M->jd - 1; M->jtail2 = M->jd - 1; }
+    M->rotf = (p->rf != 0.0) ? md[im].irot_dir * p->rf : 0.0;
+    M->xac = md[im].xcen; M->yac = md[im].ycen;
+    M->n2 = (size_t)M->jd * M->kd;
+    M->nv2 = (size_t)M->jmax * M->kmax;
+    size_t ns = (size_t)c->nspec;
+    M->q = (double*)calloc(M->n2 * ns * c->nq, sizeof(double));
+    M->s = (double*)calloc(M->n2 * ns * c->nv, sizeof(double));
+#define AL(a) M->a = (double*)calloc(M->n2 * ns, sizeof(double))
+    AL(x); AL(y); AL(xx); AL(xy); AL(yx); AL(yy); AL(ug); AL(vg); AL(turmu); AL(tscale); AL(bt);
+#undef AL
+    M->xv = (double*)calloc(M->nv2 * ns, sizeof(double));
+    M->yv = (double*)calloc(M->nv2 * ns, sizeof(double));
+    M->ugv = (double*)calloc(M->nv2 * ns, sizeof(double));
+    M->vgv = (double*)calloc(M->nv2 * ns, sizeof(double));
+    M->iblank = (int32_t*)calloc(M->n2 * ns, sizeof(int32_t));
+    for (size_t i = 0; i < M->n2 * ns; ++i) { M->iblank[i] = 1; M->bt[i] = 1.0; }
+  }
+  /* computeTScoefs, src/setup.f:838-873 */
+  int N = c->nspec;
+  c->Ds = (double*)calloc((size_t)N * N, sizeof(double));
+  c->tspec = (double*)calloc((size_t)N, sizeof(double));
+  for (int i = 1; i <= N; ++i)
+    for (int j = 1; j <= N; ++j) {
+      double v = 0.0;
+      if (i != j) {
+        double sgn = ((i - j) % 2 == 0) ? 1.0 : -1.0;
+        if (N % 2 == 0) v = 0.5 * sgn / tan(c->pi * (i - j) / N);
+        else v = 0.5 * sgn / sin(c->pi * (i - j) / N);
+      }
+      c->Ds[(i - 1) + (size_t)(j - 1) * N] = v;
+    }
+  *out = c;
+  return OT2D_OK;
+}
+
+int32_t or_case_destroy(or_case* c) {
+  if (!c) return OT2D_OK;
+  for (int im = 0; im < c->nmesh; ++im) {
+    or_mesh* M = &c->m[im];
+    free(M->q); free(M->s); free(M->x); free(M->y); free(M->xx); free(M->xy); free(M->yx);
+    free(M->yy); free(M->ug); free(M->vg); free(M->turmu); free(M->tscale); free(M->bt);
+    free(M->xv); free(M->yv); free(M->ugv); free(M->vgv); free(M->iblank);
+  }
+  free(c->m); free(c->Ds); free(c->tspec); free(c);
+  return OT2D_OK;
+}
+
+/* BC index fix-ups, src/bc.f:21-35 */
+static void bc_range(const or_mesh* M, const ot2d_bcseg* b, int* js, int* je, int* ks, int* ke) {
+  int a = b->jbcs, e = b->jbce, f = b->kbcs, g = b->kbce;
+  if (a < 0) a = M->jmax + a + 1;
+  if (f < 0) f = M->kmax + f + 1;
+  if (e < 0) e = M->jmax + e + 1;
+  if (g < 0) g = M->kmax + g + 1;
+  if (a > 1) a = a + 1;
+  if (f > 1) f = f + 1;
+  if (e < M->jmax) e = e + 1 - 1;
+  if (g < M->kmax) g = g + 1 - 1;
+  if (e == M->jmax) e = M->jd;
+  if (g == M->kmax) g = M->kd;
+  *js = a; *je = e; *ks = f; *ke = g;
+}
+
+/* ------------------------------------------------------ geometry (a14) */
+/* find_cellcenter, src/setup.f:111-345 */
+static void find_cellcenter(or_case* c, or_mesh* M, int nsp) {
+  double* x = GA(M, x, nsp);
+  double* y = GA(M, y, nsp);
+  const double* xv = VA(M, xv, nsp);
+  const double* yv = VA(M, yv, nsp);
+  int jd = M->jd, kd = M->kd;
+  (void)kd;
+#define XC(j, k) x[IX(M, j, k)]
+#define YC(j, k) y[IX(M, j, k)]
+#define XV(j, k) xv[VX(M, j, k)]
+#define YV(j, k) yv[VX(M, j, k)]
+  memset(x, 0, M->n2 * sizeof(double));
+  memset(y, 0, M->n2 * sizeof(double));
+  for (int j = M->jbeg; j <= M->jend; ++j)
+    for (int k = M->kbeg; k <= M->kend; ++k) {
+      int jv = j, kv = k;
+      XC(j, k) = 0.25 * (XV(jv, kv) + XV(jv - 1, kv) + XV(jv, kv - 1) + XV(jv - 1, kv - 1));
+      YC(j, k) = 0.25 * (YV(jv, kv) + YV(jv - 1, kv) + YV(jv, kv - 1) + YV(jv - 1, kv - 1));
+    }
+  int j = M->jbeg - 1;
+  for (int k = M->kbeg; k <= M->kend; ++k) {
+    int jv = j, kv = k;
+    XC(j, k) = XC(j + 1, k) - 0.5 * (XV(jv + 1, kv) - XV(jv, kv) + XV(jv + 1, kv - 1) - XV(jv, kv - 1));
+    YC(j, k) = YC(j + 1, k) - 0.5 * (YV(jv + 1, kv) - YV(jv, kv) + YV(jv + 1, kv - 1) - YV(jv, kv - 1));
+  }
+  j = M->jend + 1;
+  for (int k = M->kbeg; k <= M->kend; ++k) {
+    int jv = j, kv = k;
+    XC(j, k) = XC(j - 1, k) - 0.5 * (XV(jv - 2, kv) - XV(jv - 1, kv) + XV(jv - 2, kv - 1) - XV(jv - 1, kv - 1));
+    YC(j, k) = YC(j - 1, k) - 0.5 * (YV(jv - 2, kv) - YV(jv - 1, kv) + YV(jv - 2, kv - 1) - YV(jv - 1, kv - 1));
+  }
+  int k = M->kbeg - 1;
+  for (j = M->jbeg; j <= M->jend; ++j) {
+    int jv = j, kv = k;
+    XC(j, k) = XC(j, k + 1) - 0.5 * (XV(jv, kv + 1) - XV(jv, kv) + XV(jv - 1, kv + 1) - XV(jv - 1, kv));
+    YC(j, k) = YC(j, k + 1) - 0.5 * (YV(jv, kv + 1) - YV(jv, kv) + YV(jv - 1, kv + 1) - YV(jv - 1, kv));
+  }
+  k = M->kend + 1;
+  for (j = M->jbeg; j <= M->jend; ++j) {
+    int jv = j, kv = k;
+    XC(j, k) = XC(j, k - 1) - 0.5 * (XV(jv, kv - 2) - XV(jv, kv - 1) + XV(jv - 1, kv - 2) - XV(jv - 1, kv - 1));
+    YC(j, k) = YC(j, k - 1) - 0.5 * (YV(jv, kv - 2) - YV(jv, kv - 1) + YV(jv - 1, kv - 2) - YV(jv - 1, kv - 1));
+  }
+  j = M->jbeg - 1; k = M->kbeg - 1;
+  XC(j, k) = 2 * XC(j + 1, k) - XC(j + 2, k);
+  YC(j, k) = 2 * YC(j + 1, k) - YC(j + 2, k);
+  k = M->kend + 1;
+  XC(j, k) = 2 * XC(j + 1, k) - XC(j + 2, k);
+  YC(j, k) = 2 * YC(j + 1, k) - YC(j + 2, k);
+  j = M->jend + 1; k = M->kbeg - 1;
+  XC(j, k) = 2 * XC(j, k + 1) - XC(j, k + 2);
+  YC(j, k) = 2 * YC(j, k + 1) - YC(j, k + 2);
+  k = M->kend + 1;
+  XC(j, k) = 2 * XC(j, k - 1) - XC(j, k - 2);
+  YC(j, k) = 2 * YC(j, k - 1) - YC(j, k - 2);
+  for (int ib = 0; ib < M->d.nbc; ++ib) {
+    const ot2d_bcseg* b = &M->d.bc[ib];
+    int js, je, ks, ke;
+    bc_range(M, b, &js, &je, &ks, &ke);
+    int idir = b->ibdir, iadir = abs(idir);
+    if (b->ibtyp == 22) {
+      if (idir == 1) {
+        int jj = je - js + 1;
+        for (j = js; j <= je; ++j) {
+          int jc = jd - 2 * jj + (j - js + 1);
+          for (k = ks; k <= ke; ++k) { XC(j, k) = XC(jc, k); YC(j, k) = YC(jc, k); }
+        }
+      } else if (idir == -1) {
+        int jj = je - js + 1;
+        for (j = js; j <= je; ++j) {
+          int jc = 1 + 2 * jj - (je - j + 1);
+          for (k = ks; k <= ke; ++k) { XC(j, k) = XC(jc, k); YC(j, k) = YC(jc, k); }
+        }
+      } else if (idir == 2) {
+        int kk = ke - ks + 1;
+        for (k = ks; k <= ke; ++k) {
+          int kc = M->kd - 2 * kk + (k - ks + 1);
+          for (j = js; j <= je; ++j) { XC(j, k) = XC(j, kc); YC(j, k) = YC(j, kc); }
+        }
+      } else if (idir == -2) {
+        int kk = ke - ks + 1;
+        for (k = ks; k <= ke; ++k) {
+          int kc = 1 + 2 * kk - (ke - k + 1);
+          for (j = js; j <= je; ++j) { XC(j, k) = XC(j, kc); YC(j, k) = YC(j, kc); }
+        }
+      }
+    } else if (b->ibtyp == 51 && iadir == 2) {
+      for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+        int kk, k1;
+        if (idir == 2) { kk = ke - kc + 1; k1 = ke + kc; }
+        else { kk = ks + kc - 1; k1 = ks - kc; }
+        for (j = js; j <= je; ++j) {
+          int jj = jd - j + 1;
+          XC(j, kk) = XC(jj, k1); YC(j, kk) = YC(jj, k1);
+          XC(jj, kk) = XC(j, k1); YC(jj, kk) = YC(j, k1);
+        }
+      }
+    }
+  }
+#undef XC
+#undef YC
+}
+
+/* metfv, src/setup.f:440-724 */
+static void metfv(or_case* c, or_mesh* M, int nsp) {
+  double* qj = QA(M, nsp, c->nq);
+  double* xx = GA(M, xx, nsp);
+  double* xy = GA(M, xy, nsp);
+  double* yx = GA(M, yx, nsp);
+  double* yy = GA(M, yy, nsp);
+  const double* xv = VA(M, xv, nsp);
+  const double* yv = VA(M, yv, nsp);
+  int jd = M->jd;
+  for (int k = 2; k <= M->kmax; ++k)
+    for (int j = 2; j <= M->jmax; ++j) {
+      double dx1 = XV(j, k) - XV(j - 1, k - 1);
+      double dy1 = YV(j, k) - YV(j - 1, k - 1);
+      double dx2 = XV(j - 1, k) - XV(j, k - 1);
+      double dy2 = YV(j - 1, k) - YV(j, k - 1);
+      double a3 = 0.5 * (dx1 * dy2 - dx2 * dy1);
+      qj[IX(M, j, k)] = 1. / a3;
+    }
+  for (int j = 2; j <= M->jmax; ++j)
+    for (int k = 2; k <= M->kmax; ++k) {
+      xx[IX(M, j, k)] = 0.5 * (YV(j, k) - YV(j, k - 1) + YV(j - 1, k) - YV(j - 1, k - 1));
+      xy[IX(M, j, k)] = -0.5 * (XV(j, k) - XV(j, k - 1) + XV(j - 1, k) - XV(j - 1, k - 1));
+    }
+  for (int k = 2; k <= M->kmax; ++k)
+    for (int j = 2; j <= M->jmax; ++j) {
+      yx[IX(M, j, k)] = -0.5 * (YV(j, k) - YV(j - 1, k) + YV(j, k - 1) - YV(j - 1, k - 1));
+      yy[IX(M, j, k)] = 0.5 * (XV(j, k) - XV(j - 1, k) + XV(j, k - 1) - XV(j - 1, k - 1));
+    }
+  for (int k = M->kbeg; k <= M->kend; ++k)
+    for (int j = M->jbeg; j <= M->jend; ++j) {
+      size_t i = IX(M, j, k);
+      double J = qj[i];
+      xx[i] = J * xx[i]; xy[i] = J * xy[i]; yx[i] = J * yx[i]; yy[i] = J * yy[i];
+    }
+#define CP(dst, src) do { qj[dst] = qj[src]; xx[dst] = xx[src]; xy[dst] = xy[src]; \
+    yx[dst] = yx[src]; yy[dst] = yy[src]; } while (0)
+  {
+    int j = M->jbeg - 1, j1 = M->jbeg;
+    for (int k = M->kbeg; k <= M->kend; ++k) CP(IX(M, j, k), IX(M, j1, k));
+    j = M->jend + 1; j1 = M->jend;
+    for (int k = M->kbeg; k <= M->kend; ++k) CP(IX(M, j, k), IX(M, j1, k));
+    int k = M->kbeg - 1, k1 = M->kbeg;
+    for (j = M->jbeg - 1; j <= M->jend + 1; ++j) CP(IX(M, j, k), IX(M, j, k1));
+    k = M->kend + 1; k1 = M->kend;
+    for (j = M->jbeg - 1; j <= M->jend + 1; ++j) CP(IX(M, j, k), IX(M, j, k1));
+  }
+  for (int ib = 0; ib < M->d.nbc; ++ib) {
+    const ot2d_bcseg* b = &M->d.bc[ib];
+    int js, je, ks, ke;
+    bc_range(M, b, &js, &je, &ks, &ke);
+    int idir = b->ibdir, iadir = abs(idir);
+    if (b->ibtyp == 22) {
+      if (idir == 1) {
+        int jj = je - js + 1;
+        for (int j = js; j <= je; ++j) {
+          int jc = jd - 2 * jj + (j - js + 1);
+          for (int k = ks; k <= ke; ++k) CP(IX(M, j, k), IX(M, jc, k));
+        }
+      } else if (idir == -1) {
+        int jj = je - js + 1;
+        for (int j = js; j <= je; ++j) {
+          int jc = 1 + 2 * jj - (je - j + 1);
+          for (int k = ks; k <= ke; ++k) CP(IX(M, j, k), IX(M, jc, k));
+        }
+      } else if (idir == 2) {
+        int kk = ke - ks + 1;
+        for (int k = ks; k <= ke; ++k) {
+          int kc = M->kd - 2 * kk + (k - ks + 1);
+          for (int j = js; j <= je; ++j) CP(IX(M, j, k), IX(M, j, kc));
+        }
+      } else if (idir == -2) {
+        int kk = ke - ks + 1;
+        for (int k = ks; k <= ke; ++k) {
+          int kc = 1 + 2 * kk - (ke - k + 1);
+          for (int j = js; j <= je; ++j) CP(IX(M, j, k), IX(M, j, kc));
+        }
+      }
+    } else if (b->ibtyp == 51 && iadir == 2) {
+      for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+        int k, k1;
+        if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; }
+        else { k = ks + kc - 1; k1 = ks - kc; }
+        for (int j = js; j <= je; ++j) {
+          int jj = jd - j + 1;
+          size_t a = IX(M, j, k), bb = IX(M, jj, k1), cc = IX(M, jj, k), dd = IX(M, j, k1);
+          qj[a] = qj[bb]; xx[a] = -xx[bb]; xy[a] = -xy[bb]; yx[a] = -yx[bb]; yy[a] = -yy[bb];
+          qj[cc] = qj[dd]; xx[cc] = -xx[dd]; xy[cc] = -xy[dd]; yx[cc] = -yx[dd]; yy[cc] = -yy[dd];
+        }
+      }
+    }
+  }
+#undef CP
+}
+#undef XV
+#undef YV
+
+/* find_coord_TS, src/setup.f:348-393 (motiontype 1 and 2) */
+static void find_coord_TS(or_case* c, or_mesh* M, int nsp) {
+  const double* xv1 = VA(M, xv, 0);
+  const double* yv1 = VA(M, yv, 0);
+  double* xv = VA(M, xv, nsp);
+  double* yv = VA(M, yv, nsp);
+  double tspec = c->tspec[nsp];
+  if (c->p.motiontype == 1) {
+    double theta = c->p.angmax * c->pi / 180. * (1. - cos(M->rotf * tspec));
+    double ss = sin(theta), cs = cos(theta);
+    for (size_t i = 0; i < M->nv2; ++i) {
+      xv[i] = (xv1[i] - M->xac) * cs + (yv1[i] - M->yac) * ss + M->xac;
+      yv[i] = (yv1[i] - M->yac) * cs - (xv1[i] - M->xac) * ss + M->yac;
+    }
+  } else if (c->p.motiontype == 2) {
+    double cs = cos(M->rotf * tspec), ss = sin(M->rotf * tspec);
+    for (size_t i = 0; i < M->nv2; ++i) {
+      xv[i] = (xv1[i] - M->xac) * cs - (yv1[i] - M->yac) * ss + M->xac;
+      yv[i] = (yv1[i] - M->yac) * cs + (xv1[i] - M->xac) * ss + M->yac;
+    }
+  }
+}
+
+/* find_timemetrics_TS, src/setup.f:727-786 (motiontype 1 and 2) */
+static void find_timemetrics_TS(or_case* c, or_mesh* M, int nsp) {
+  double tspec = c->tspec[nsp];
+  double* ug = GA(M, ug, nsp);
+  double* vg = GA(M, vg, nsp);
+  const double* x = GA(M, x, nsp);
+  const double* y = GA(M, y, nsp);
+  double* ugv = VA(M, ugv, nsp);
+  double* vgv = VA(M, vgv, nsp);
+  const double* xv = VA(M, xv, nsp);
+  const double* yv = VA(M, yv, nsp);
+  double rotf = M->rotf, pi = c->pi, am = c->p.angmax;
+  if (c->p.motiontype == 1) {
+    for (size_t i = 0; i < M->n2; ++i) {
+      ug[i] = am * pi / 180. * rotf * (y[i] - M->yac) * sin(rotf * tspec);
+      vg[i] = -am * pi / 180. * rotf * (x[i] - M->xac) * sin(rotf * tspec);
+    }
+    for (size_t i = 0; i < M->nv2; ++i) {
+      ugv[i] = am * pi / 180. * rotf * (yv[i] - M->yac) * sin(rotf * tspec);
+      vgv[i] = -am * pi / 180. * rotf * (xv[i] - M->xac) * sin(rotf * tspec);
+    }
+  } else if (c->p.motiontype == 2) {
+    for (size_t i = 0; i < M->n2; ++i) { ug[i] = -rotf * (y[i] - M->yac); vg[i] = rotf * (x[i] - M->xac); }
+    for (size_t i = 0; i < M->nv2; ++i) { ugv[i] = -rotf * (yv[i] - M->yac); vgv[i] = rotf * (xv[i] - M->xac); }
+  }
+}
+
+/* qzero + qdivj, src/init.f:209-277, 180-206 */
+static void qzero_qdivj(or_case* c, or_mesh* M, int nsp) {
+  double ruinf = c->rinf * c->uinf, rvinf = c->rinf * c->vinf;
+  double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+  double* qj = QA(M, nsp, c->nq);
+  for (size_t i = 0; i < M->n2; ++i) { q1[i] = c->rinf; q2[i] = ruinf; q3[i] = rvinf; q4[i] = c->einf; }
+  if (c->iturb == 1) {
+    double* q5 = QA(M, nsp, 5);
+    for (size_t i = 0; i < M->n2; ++i) q5[i] = c->p.vnuinf;
+  } else if (c->iturb == 2) {
+    double *q5 = QA(M, nsp, 5), *q6 = QA(M, nsp, 6);
+    for (size_t i = 0; i < M->n2; ++i) { q5[i] = c->rinf * c->tkeinf; q6[i] = c->rinf * c->tomegainf; }
+  }
+  for (size_t i = 0; i < M->n2; ++i) {
+    q1[i] = q1[i] / qj[i]; q2[i] = q2[i] / qj[i]; q3[i] = q3[i] / qj[i]; q4[i] = q4[i] / qj[i];
+  }
+}
+
+/* initia, src/init.f:2-177 (iread=0 part) */
+int32_t or_set_grid(or_case* c, int32_t im, const double* xv, const double* yv) {
+  if (!c || im < 0 || im >= c->nmesh) return OT2D_ERR_ARG;
+  or_mesh* M = &c->m[im];
+  size_t ns = (size_t)c->nspec;
+  for (size_t i = 0; i < M->n2 * ns; ++i) { M->turmu[i] = 0.; M->ug[i] = 0.; M->vg[i] = 0.; M->iblank[i] = 1; }
+  memset(M->s, 0, M->n2 * ns * c->nv * sizeof(double));
+  c->istep0 = 0;
+  for (int nsp = 0; nsp < c->nspec; ++nsp)
+    c->tspec[nsp] = (c->p.rf != 0.0) ? 2 * c->pi * nsp / c->nspec / fabs(c->p.rf) : 0.0;
+  memcpy(M->xv, xv, M->nv2 * sizeof(double));
+  memcpy(M->yv, yv, M->nv2 * sizeof(double));
+  for (int nsp = 1; nsp < c->nspec; ++nsp) find_coord_TS(c, M, nsp);
+  for (int nsp = 0; nsp < c->nspec; ++nsp) {
+    find_cellcenter(c, M, nsp);
+    metfv(c, M, nsp);
+    if (c->timespectral) find_timemetrics_TS(c, M, nsp);
+    qzero_qdivj(c, M, nsp);
+  }
+  return OT2D_OK;
+}
+
+/* ------------------------------------------------------------ state I/O */
+int32_t or_set_state(or_case* c, int32_t im, const double* q) {
+  if (!c || im < 0 || im >= c->nmesh) return OT2D_ERR_ARG;
+  or_mesh* M = &c->m[im];
+  memcpy(M->q, q, M->n2 * c->nspec * c->nq * sizeof(double));
+  return OT2D_OK;
+}
+int32_t or_get_state(or_case* c, int32_t im, double* q) {
+  if (!c || im < 0 || im >= c->nmesh) return OT2D_ERR_ARG;
+  or_mesh* M = &c->m[im];
+  memcpy(q, M->q, M->n2 * c->nspec * c->nq * sizeof(double));
+  return OT2D_OK;
+}
+int32_t or_set_iblank(or_case* c, int32_t im, const int32_t* ib) {
+  if (!c || im < 0 || im >= c->nmesh) return OT2D_ERR_ARG;
+  memcpy(c->m[im].iblank, ib, c->m[im].n2 * c->nspec * sizeof(int32_t));
+  return OT2D_OK;
+}
+int32_t or_get_iblank(or_case* c, int32_t im, int32_t* ib) {
+  if (!c || im < 0 || im >= c->nmesh) return OT2D_ERR_ARG;
+  memcpy(ib, c->m[im].iblank, c->m[im].n2 * c->nspec * sizeof(int32_t));
+  return OT2D_OK;
+}
+static double* field_ptr(or_case* c, or_mesh* M, int field, size_t* n) {
+  size_t ns = (size_t)c->nspec;
+  *n = M->n2 * ns;
+  switch (field) {
+    case OT2D_F_X: return M->x;
+    case OT2D_F_Y: return M->y;
+    case OT2D_F_XX: return M->xx;
+    case OT2D_F_XY: return M->xy;
+    case OT2D_F_YX: return M->yx;
+    case OT2D_F_YY: return M->yy;
+    case OT2D_F_UG: return M->ug;
+    case OT2D_F_VG: return M->vg;
+    case OT2D_F_TSCALE: return M->tscale;
+    case OT2D_F_BT: return M->bt;
+    case OT2D_F_TURMU: return M->turmu;
+    case OT2D_F_S: *n = M->n2 * ns * c->nv; return M->s;
+    case OT2D_F_XV: *n = M->nv2 * ns; return M->xv;
+    case OT2D_F_YV: *n = M->nv2 * ns; return M->yv;
+    case OT2D_F_UGV: *n = M->nv2 * ns; return M->ugv;
+    case OT2D_F_VGV: *n = M->nv2 * ns; return M->vgv;
+  }
+  return NULL;
+}
+int32_t or_get_field(or_case* c, int32_t im, int32_t field, double* out) {
+  if (!c || im < 0 || im >= c->nmesh) return OT2D_ERR_ARG;
+  size_t n;
+  double* p = field_ptr(c, &c->m[im], field, &n);
+  if (!p) return OT2D_ERR_ARG;
+  memcpy(out, p, n * sizeof(double));
+  return OT2D_OK;
+}
+int32_t or_set_field(or_case* c, int32_t im, int32_t field, const double* in) {
+  if (!c || im < 0 || im >= c->nmesh) return OT2D_ERR_ARG;
+  size_t n;
+  double* p = field_ptr(c, &c->m[im], field, &n);
+  if (!p) return OT2D_ERR_ARG;
+  memcpy(p, in, n * sizeof(double));
+  return OT2D_OK;
+}
+int32_t or_set_istep0(or_case* c, int32_t istep0) { c->istep0 = istep0; return OT2D_OK; }
+int32_t or_get_istep0(or_case* c, int32_t* istep0) { *istep0 = c->istep0; return OT2D_OK; }
+
+/* ------------------------------------------------------ time step (a8) */
+/* time_step, src/time.f:2-107 */
+static void time_step_mesh(or_case* c, or_mesh* M) {
+  const ot2d_params* p = &c->p;
+  double oat = 1.0;
+  if (p->ntac >= 2 && c->istep > 1) oat = 2. / 3.;
+  if (p->ntac == 3 && c->istep > 2) oat = 6. / 11.;
+  double jac_factor = p->invisc ? 0.01 : 0.002;
+  for (int nsp = 0; nsp < c->nspec; ++nsp) {
+    double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+    double* qj = QA(M, nsp, c->nq);
+    double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+    double *ug = GA(M, ug, nsp), *vg = GA(M, vg, nsp), *ts = GA(M, tscale, nsp), *bt = GA(M, bt, nsp);
+    const int32_t* ib = M->iblank + (size_t)nsp * M->n2;
+    for (size_t i = 0; i < M->n2; ++i) {
+      double ibp = (double)MAX(ib[i], 0);
+      if (p->iconstant_cfl == 0) {
+        if (p->idual == 0) {
+          ts[i] = oat * c->h * ibp * (1.0 + jac_factor * (1. - p->timeac) * sqrt(qj[i])) /
+                  (1.0 + (1. - p->timeac) * sqrt(qj[i]));
+          bt[i] = 1.;
+        } else {
+          double t = ibp * (1.0 + jac_factor * sqrt(qj[i])) / (1.0 + sqrt(qj[i]));
+          if (p->iprecon) t = 100. * ibp * (1.0 + 0.005 * sqrt(qj[i])) / (1.0 + sqrt(qj[i]));
+          t = t * p->dualtime;
+          bt[i] = 1. / (1. + t / c->h / oat);
+          ts[i] = t / (1. + t / c->h / oat);
+        }
+      } else {
+        double u = q2[i] / q1[i], v = q3[i] / q1[i], e = q4[i] / q1[i];
+        double uu = (u - ug[i]) * xx[i] + (v - vg[i]) * xy[i];
+        double vv = (u - ug[i]) * yx[i] + (v - vg[i]) * yy[i];
+        double dxdx = sq(xx[i]) + sq(xy[i]);
+        double dydy = sq(yx[i]) + sq(yy[i]);
+        double asq = c->ggm1 * (e - 0.5 * (sq(u) + sq(v)));
+        double lam_j = fabs(uu) + sqrt(asq * dxdx);
+        double lam_k = fabs(vv) + sqrt(asq * dydy);
+        if (p->idual == 0) {
+          ts[i] = c->h / (lam_j + lam_k);
+          ts[i] = ts[i] * ibp;
+          bt[i] = 1.;
+        } else {
+          double hnew = p->dualtime / (lam_j + lam_k);
+          ts[i] = hnew * ibp;
+          ts[i] = ts[i] / (1. + ts[i] / c->h / oat);
+          bt[i] = 1. / (1. + ts[i] / c->h / oat);
+        }
+      }
+    }
+  }
+}
+
+int32_t or_time_step(or_case* c) {
+  for (int im = 0; im < c->nmesh; ++im) time_step_mesh(c, &c->m[im]);
+  return OT2D_OK;
+}
+
+/* ------------------------------------------------------------- BCs (a6) */
+typedef struct {
+  double *q1, *q2, *q3, *q4, *qj, *xx, *xy, *yx, *yy, *ug, *vg, *bt;
+} qview;
+
+static qview view(or_case* c, or_mesh* M, int nsp) {
+  qview v;
+  v.q1 = QA(M, nsp, 1); v.q2 = QA(M, nsp, 2); v.q3 = QA(M, nsp, 3); v.q4 = QA(M, nsp, 4);
+  v.qj = QA(M, nsp, c->nq);
+  v.xx = GA(M, xx, nsp); v.xy = GA(M, xy, nsp); v.yx = GA(M, yx, nsp); v.yy = GA(M, yy, nsp);
+  v.ug = GA(M, ug, nsp); v.vg = GA(M, vg, nsp); v.bt = GA(M, bt, nsp);
+  return v;
+}
+
+/* bcwall + bctany + uv, src/bc.f:1669-1801, 1328-1415, 2224-2264 */
+static void bcwall(or_case* c, or_mesh* M, qview* v, int js, int je, int ks, int ke, int idir, int invsc) {
+  int iadd = idir >= 0 ? 1 : -1, iadir = abs(idir);
+  if (iadir != 2) return; /* "not implemented in bcwall" */
+  int k = (idir == 2) ? ke : ks;
+  int k1 = k + iadd, k2 = k1 + iadd;
+  double foso = invsc ? 1.0 : 0.0;
+  double gm1 = c->gm1;
+  for (int j = js; j <= je; ++j) {
+    size_t i = IX(M, j, k), i1 = IX(M, j, k1), i2 = IX(M, j, k2);
+    double u, vv;
+    /* bctany */
+    if (invsc) {
+      double f = 1.0;
+      double u1 = (v->q2[i1] * v->xx[i] + v->q3[i1] * v->xy[i]) / v->q1[i1];
+      double u2 = (v->q2[i2] * v->xx[i] + v->q3[i2] * v->xy[i]) / v->q1[i2];
+      u = (1. + f) * u1 - f * u2;
+      double v1 = (v->q2[i1] * v->yx[i] + v->q3[i1] * v->yy[i]) / v->q1[i1];
+      vv = v->ug[i] * v->yx[i] + v->vg[i] * v->yy[i] + v->ug[i1] * v->yx[i1] + v->vg[i1] * v->yy[i1] - v1;
+    } else {
+      double u1 = (v->q2[i1] * v->xx[i] + v->q3[i1] * v->xy[i]) / v->q1[i1];
+      double v1 = (v->q2[i1] * v->yx[i] + v->q3[i1] * v->yy[i]) / v->q1[i1];
+      u = v->ug[i] * v->xx[i] + v->vg[i] * v->xy[i] + v->ug[i1] * v->xx[i1] + v->vg[i1] * v->xy[i1] - u1;
+      vv = v->ug[i] * v->yx[i] + v->vg[i] * v->yy[i] + v->ug[i1] * v->yx[i1] + v->vg[i1] * v->yy[i1] - v1;
+    }
+    /* uv */
+    double t11 = v->yy[i], t12 = -v->xy[i], t21 = -v->yx[i], t22 = v->xx[i];
+    double vol = 1. / (t22 * t11 - t12 * t21);
+    double uwall = (t11 * u + t12 * vv) * vol;
+    double vwall = (t21 * u + t22 * vv) * vol;
+    /* density / pressure extrapolation */
+    double rho1 = v->q1[i1] * v->qj[i1];
+    double rho2 = v->q1[i2] * v->qj[i2];
+    double p1 = gm1 * (v->q4[i1] - 0.5 * (sq(v->q2[i1]) + sq(v->q3[i1])) / v->q1[i1]) * v->qj[i1];
+    double p2 = gm1 * (v->q4[i2] - 0.5 * (sq(v->q2[i2]) + sq(v->q3[i2])) / v->q1[i2]) * v->qj[i2];
+    v->q1[i] = ((1. + foso) * rho1 - foso * rho2) / v->qj[i];
+    v->q2[i] = uwall * v->q1[i];
+    v->q3[i] = vwall * v->q1[i];
+    double press = (1. + foso) * p1 - foso * p2;
+    v->q4[i] = press / (gm1 * v->qj[i]) + 0.5 * (sq(v->q2[i]) + sq(v->q3[i])) / v->q1[i];
+  }
+  /* other halo cells: ke-ks = 0 with one halo ring */
+  for (int kc = 1; kc <= ke - ks; ++kc) {
+    int kk = (idir == 2) ? ke - kc : ks + kc;
+    int kk1 = kk + iadd, kk2 = kk1 + iadd;
+    double f = 1.0;
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, kk), i1 = IX(M, j, kk1), i2 = IX(M, j, kk2);
+      double rho1 = v->q1[i1] * v->qj[i1], rho2 = v->q1[i2] * v->qj[i2];
+      double u1 = v->q2[i1] / v->q1[i1], u2 = v->q2[i2] / v->q1[i2];
+      double v1 = v->q3[i1] / v->q1[i1], v2 = v->q3[i2] / v->q1[i2];
+      double p1 = gm1 * (v->q4[i1] - 0.5 * (sq(v->q2[i1]) + sq(v->q3[i1])) / v->q1[i1]) * v->qj[i1];
+      double p2 = gm1 * (v->q4[i2] - 0.5 * (sq(v->q2[i2]) + sq(v->q3[i2])) / v->q1[i2]) * v->qj[i2];
+      v->q1[i] = ((1. + f) * rho1 - f * rho2) / v->qj[i];
+      v->q2[i] = ((1. + f) * u1 - f * u2) * v->q1[i];
+      v->q3[i] = ((1. + f) * v1 - f * v2) * v->q1[i];
+      double press = (1. + f) * p1 - f * p2;
+      v->q4[i] = press / (gm1 * v->qj[i]) + 0.5 * (sq(v->q2[i]) + sq(v->q3[i])) / v->q1[i];
+    }
+  }
+  /* 30-step ramp, src/bc.f:1779-1793 */
+  double t = (double)c->istep0 / 30.;
+  if (t > 1. || c->p.iread > 0) t = 1.;
+  double scal = (10. - 15. * t + 6. * t * t) * cube(t);
+  double rscal = 1. - scal;
+  for (int kk = ks; kk <= ke; ++kk)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, kk);
+      double ajacinv = 1. / v->qj[i];
+      v->q1[i] = v->q1[i] * scal + rscal * c->rinf * ajacinv;
+      v->q2[i] = v->q2[i] * scal + rscal * c->rinf * c->uinf * ajacinv;
+      v->q3[i] = v->q3[i] * scal + rscal * c->rinf * c->vinf * ajacinv;
+      v->q4[i] = v->q4[i] * scal + rscal * c->einf * ajacinv;
+    }
+}
+
+/* bcextp, src/bc.f:1962-2026 */
+static void bcextp(or_case* c, or_mesh* M, qview* v, int js, int je, int ks, int ke, int idir) {
+  double foso = 0.0;
+  int iadd = idir >= 0 ? 1 : -1, iadir = abs(idir);
+  double* qs[4] = {v->q1, v->q2, v->q3, v->q4};
+  if (iadir == 1) {
+    for (int jc = 1; jc <= je - js + 1; ++jc) {
+      int j = (idir == 1) ? je - jc + 1 : js + jc - 1;
+      int j1 = j + iadd, j2 = j1 + iadd;
+      for (int k = ks; k <= ke; ++k) {
+        size_t i = IX(M, j, k), i1 = IX(M, j1, k), i2 = IX(M, j2, k);
+        double s1 = v->qj[i1] / v->qj[i], s2 = v->qj[i2] / v->qj[i];
+        for (int n = 0; n < 4; ++n) qs[n][i] = (1. + foso) * qs[n][i1] * s1 - foso * qs[n][i2] * s2;
+      }
+    }
+  } else if (iadir == 2) {
+    for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+      int k = (idir == 2) ? ke - kc + 1 : ks + kc - 1;
+      int k1 = k + iadd, k2 = k1 + iadd;
+      for (int j = js; j <= je; ++j) {
+        size_t i = IX(M, j, k), i1 = IX(M, j, k1), i2 = IX(M, j, k2);
+        double s1 = v->qj[i1] / v->qj[i], s2 = v->qj[i2] / v->qj[i];
+        for (int n = 0; n < 4; ++n) qs[n][i] = (1. + foso) * qs[n][i1] * s1 - foso * qs[n][i2] * s2;
+      }
+    }
+  }
+}
+
+/* bcsym, src/bc.f:2096-2157 */
+static void bcsym(or_case* c, or_mesh* M, qview* v, int js, int je, int ks, int ke, int idir) {
+  int iadir = abs(idir);
+  if (iadir == 1) {
+    for (int jc = 1; jc <= je - js + 1; ++jc) {
+      int j, j1;
+      if (idir == 1) { j = je - jc + 1; j1 = je + jc; } else { j = js + jc - 1; j1 = js - jc; }
+      for (int k = ks; k <= ke; ++k) {
+        size_t i = IX(M, j, k), i1 = IX(M, j1, k);
+        double sc = v->qj[i1] / v->qj[i];
+        v->q1[i] = v->q1[i1] * sc; v->q2[i] = -v->q2[i1] * sc;
+        v->q3[i] = v->q3[i1] * sc; v->q4[i] = v->q4[i1] * sc;
+      }
+    }
+  } else if (iadir == 2) {
+    for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+      int k, k1;
+      if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+      for (int j = js; j <= je; ++j) {
+        size_t i = IX(M, j, k), i1 = IX(M, j, k1);
+        double sc = v->qj[i1] / v->qj[i];
+        v->q1[i] = v->q1[i1] * sc; v->q2[i] = v->q2[i1] * sc;
+        v->q3[i] = -v->q3[i1] * sc; v->q4[i] = v->q4[i1] * sc;
+      }
+    }
+  }
+}
+
+/* periodic source index, src/bc.f:192-262 */
+static int periodic_src(int idir, int js, int je, int ks, int ke, int jd, int kd, int jk) {
+  if (idir == 1) { int jj = je - js + 1; return jd - 2 * jj + (jk - js + 1); }
+  if (idir == -1) { int jj = je - js + 1; return 1 + 2 * jj - (je - jk + 1); }
+  if (idir == 2) { int kk = ke - ks + 1; return kd - 2 * kk + (jk - ks + 1); }
+  int kk = ke - ks + 1;
+  return 1 + 2 * kk - (ke - jk + 1);
+}
+
+/* bcperiodic, src/bc.f:180-267 */
+static void bcperiodic(or_case* c, or_mesh* M, qview* v, int js, int je, int ks, int ke, int idir) {
+  if (abs(idir) == 1) {
+    for (int j = js; j <= je; ++j) {
+      int jc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, j);
+      for (int k = ks; k <= ke; ++k) {
+        size_t i = IX(M, j, k), i1 = IX(M, jc, k);
+        double sc = v->qj[i1] / v->qj[i];
+        v->q1[i] = v->q1[i1] * sc; v->q2[i] = v->q2[i1] * sc; v->q3[i] = v->q3[i1] * sc; v->q4[i] = v->q4[i1] * sc;
+      }
+    }
+  } else {
+    for (int k = ks; k <= ke; ++k) {
+      int kc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, k);
+      for (int j = js; j <= je; ++j) {
+        size_t i = IX(M, j, k), i1 = IX(M, j, kc);
+        double sc = v->qj[i1] / v->qj[i];
+        v->q1[i] = v->q1[i1] * sc; v->q2[i] = v->q2[i1] * sc; v->q3[i] = v->q3[i1] * sc; v->q4[i] = v->q4[i1] * sc;
+      }
+    }
+  }
+}
+
+/* bcwake, src/bc.f:1516-1573 */
+static void bcwake(or_case* c, or_mesh* M, qview* v, int js, int je, int ks, int ke, int idir) {
+  if (abs(idir) != 2) return;
+  for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+    int k, k1;
+    if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+    for (int j = js; j <= je; ++j) {
+      int jj = M->jd - j + 1;
+      size_t a = IX(M, j, k), b = IX(M, jj, k1), cc = IX(M, jj, k), d = IX(M, j, k1);
+      double scale1 = v->qj[b] / v->qj[a];
+      double scale2 = v->qj[d] / v->qj[cc];
+      v->q1[a] = v->q1[b] * scale1; v->q2[a] = v->q2[b] * scale1;
+      v->q3[a] = v->q3[b] * scale1; v->q4[a] = v->q4[b] * scale1;
+      v->q1[cc] = v->q1[d] * scale2; v->q2[cc] = v->q2[d] * scale2;
+      v->q3[cc] = v->q3[d] * scale2; v->q4[cc] = v->q4[d] * scale2;
+    }
+  }
+}
+
+/* bcout_inf, src/bc.f:360-407 */
+static void bcout_inf(or_case* c, or_mesh* M, qview* v, int js, int je, int ks, int ke, int idir) {
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      double rj = 1. / v->qj[i];
+      v->q1[i] = c->rinf * rj;
+      v->q2[i] = c->rinf * c->uinf * rj;
+      v->q3[i] = c->rinf * c->vinf * rj;
+      v->q4[i] = c->pinf / c->gm1 * rj + 0.5 * (sq(v->q2[i]) + sq(v->q3[i])) / v->q1[i];
+    }
+  (void)idir;
+}
+
+/* characteristic far field, bcout_trkl src/bc.f:694-983: one boundary cell
+ * i with its first two interior neighbours i1, i2 and outward metric (nx,ny)
+ * already normalised as xxn/xyn (or yxn/yyn). */
+static void trkl_cell(or_case* c, qview* v, size_t i, size_t i1, size_t i2, double xxn, double xyn) {
+  double gm1 = c->gm1, gamma = c->p.gamma, Mp = c->p.Mp;
+  double foso = 1.0;
+  double qext[4], qbar[4], delq[4], delw[4];
+  double* qs[4] = {v->q1, v->q2, v->q3, v->q4};
+  for (int n = 0; n < 4; ++n)
+    qext[n] = (1. + foso) * qs[n][i1] * v->qj[i1] - foso * qs[n][i2] * v->qj[i2];
+  qbar[0] = 0.5 * (qext[0] + c->rinf);
+  qbar[1] = 0.5 * (qext[1] + c->rinf * c->uinf);
+  qbar[2] = 0.5 * (qext[2] + c->rinf * c->vinf);
+  qbar[3] = 0.5 * (qext[3] + c->einf);
+  delq[0] = 0.5 * (c->rinf - qext[0]);
+  delq[1] = 0.5 * (c->rinf * c->uinf - qext[1]);
+  delq[2] = 0.5 * (c->rinf * c->vinf - qext[2]);
+  delq[3] = 0.5 * (c->einf - qext[3]);
+  double rho = qbar[0];
+  double rrho = 1. / rho;
+  double uu = qbar[1] * rrho, vv = qbar[2] * rrho, e = qbar[3] * rrho;
+  double uv2 = 0.5 * (uu * uu + vv * vv);
+  double cjkl = sqrt(c->ggm1 * (e - uv2));
+  double c2i = 1. / (cjkl * cjkl);
+  double ge = gamma * e - gm1 * uv2;
+  double qq = (uu - v->ug[i]) * xxn + (vv - v->vg[i]) * xyn;
+  double bt = v->bt[i];
+  double bSq = sq(Mp) / (bt - sq(Mp) * (bt - 1));
+  double Z = sqrt((1. - bSq) * qq * (1. - bSq) * qq + 4. * bSq * cjkl * cjkl);
+  double Zi = 1. / Z;
+  double R = 0.5 * ((1. - bSq) * qq + Z);
+  double S = 0.5 * ((1. - bSq) * qq - Z);
+  double sq0 = fsign(1., qq);
+  double sR = fsign(1., R + bSq * qq), sR2 = fsign(1., R + qq * bSq);
+  double sS = fsign(1., S + bSq * qq), sS2 = fsign(1., S + qq * bSq);
+  delw[0] = (1. - uv2 * gm1 * c2i) * sq0 * delq[0];
+  delw[0] = delw[0] + uu * c2i * gm1 * sq0 * delq[1];
+  delw[0] = delw[0] + vv * c2i * gm1 * sq0 * delq[2];
+  delw[0] = delw[0] - gm1 * c2i * sq0 * delq[3];
+  delw[1] = (vv * xxn - uu * xyn) * sq0 * delq[0];
+  delw[1] = delw[1] + xyn * sq0 * delq[1];
+  delw[1] = delw[1] - xxn * sq0 * delq[2];
+  delw[2] = -(Zi * (uv2 * gm1 * c2i * S + bSq * qq) * sR * delq[0]);
+  delw[2] = delw[2] + (xxn * bSq + S * uu * gm1 * c2i) * Zi * sR * delq[1];
+  delw[2] = delw[2] + (xyn * bSq + S * vv * gm1 * c2i) * Zi * sR2 * delq[2];
+  delw[2] = delw[2] - S * gm1 * c2i * Zi * sR2 * delq[3];
+  delw[3] = Zi * (uv2 * gm1 * c2i * R + qq * bSq) * sS * delq[0];
+  delw[3] = delw[3] - (xxn * bSq + R * uu * gm1 * c2i) * Zi * sS * delq[1];
+  delw[3] = delw[3] - (xyn * bSq + R * vv * gm1 * c2i) * Zi * sS2 * delq[2];
+  delw[3] = delw[3] + R * gm1 * c2i * Zi * sS2 * delq[3];
+  delq[0] = delw[0] + delw[2] + delw[3];
+  delq[1] = uu * delw[0] + xyn * delw[1] + (uu + xxn * R / bSq) * delw[2];
+  delq[1] = delq[1] + (uu + xxn * S / bSq) * delw[3];
+  delq[2] = vv * delw[0] - xxn * delw[1] + (vv + xyn * R / bSq) * delw[2];
+  delq[2] = delq[2] + (vv + xyn * S / bSq) * delw[3];
+  delq[3] = uv2 * delw[0] + (uu * xyn - vv * xxn) * delw[1];
+  delq[3] = delq[3] + (ge + R * qq / bSq) * delw[2];
+  delq[3] = delq[3] + (ge + S * qq / bSq) * delw[3];
+  for (int n = 0; n < 4; ++n) qs[n][i] = (qbar[n] - delq[n]) / v->qj[i];
+}
+
+/* zeroth-order extrapolation of primitives to further halo layers
+ * (src/bc.f:810-835); empty with one halo ring. */
+static void extrap_prim(or_case* c, qview* v, size_t i, size_t i1, size_t i2) {
+  double gm1 = c->gm1, foso = 0.0;
+  double rho1 = v->q1[i1] * v->qj[i1], rho2 = v->q1[i2] * v->qj[i2];
+  double u1 = v->q2[i1] / v->q1[i1], u2 = v->q2[i2] / v->q1[i2];
+  double v1 = v->q3[i1] / v->q1[i1], v2 = v->q3[i2] / v->q1[i2];
+  double p1 = gm1 * (v->q4[i1] - 0.5 * (sq(v->q2[i1]) + sq(v->q3[i1])) / v->q1[i1]) * v->qj[i1];
+  double p2 = gm1 * (v->q4[i2] - 0.5 * (sq(v->q2[i2]) + sq(v->q3[i2])) / v->q1[i2]) * v->qj[i2];
+  v->q1[i] = ((1. + foso) * rho1 - foso * rho2) / v->qj[i];
+  v->q2[i] = ((1. + foso) * u1 - foso * u2) * v->q1[i];
+  v->q3[i] = ((1. + foso) * v1 - foso * v2) * v->q1[i];
+  double press = (1. + foso) * p1 - foso * p2;
+  v->q4[i] = press / (gm1 * v->qj[i]) + 0.5 * (sq(v->q2[i]) + sq(v->q3[i])) / v->q1[i];
+}
+
+static void bcout_trkl(or_case* c, or_mesh* M, qview* v, int js, int je, int ks, int ke, int idir) {
+  int iadd = idir >= 0 ? 1 : -1, iadir = abs(idir);
+  if (iadir == 1) {
+    int j = (idir == 1) ? je : js;
+    int j1 = j + iadd, j2 = j1 + iadd;
+    for (int k = ks; k <= ke; ++k) {
+      size_t i = IX(M, j, k);
+      double snorm = -1. * iadd / sqrt(sq(v->xx[i]) + sq(v->xy[i]));
+      trkl_cell(c, v, i, IX(M, j1, k), IX(M, j2, k), v->xx[i] * snorm, v->xy[i] * snorm);
+    }
+    for (int jc = 1; jc <= je - js; ++jc) {
+      int jj = (idir == 1) ? je - jc : js + jc;
+      for (int k = ks; k <= ke; ++k)
+        extrap_prim(c, v, IX(M, jj, k), IX(M, jj + iadd, k), IX(M, jj + 2 * iadd, k));
+    }
+  } else if (iadir == 2) {
+    int k = (idir == 2) ? ke : ks;
+    int k1 = k + iadd, k2 = k1 + iadd;
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      double snorm = -1. * iadd / sqrt(sq(v->yx[i]) + sq(v->yy[i]));
+      trkl_cell(c, v, i, IX(M, j, k1), IX(M, j, k2), v->yx[i] * snorm, v->yy[i] * snorm);
+    }
+    for (int kc = 1; kc <= ke - ks; ++kc) {
+      int kk = (idir == 2) ? ke - kc : ks + kc;
+      for (int j = js; j <= je; ++j)
+        extrap_prim(c, v, IX(M, j, kk), IX(M, j, kk + iadd), IX(M, j, kk + 2 * iadd));
+    }
+  }
+}
+
+/* bc dispatch, src/bc.f:2-85 */
+static void bc_mesh(or_case* c, or_mesh* M, int nsp) {
+  qview v = view(c, M, nsp);
+  for (int ib = 0; ib < M->d.nbc; ++ib) {
+    const ot2d_bcseg* b = &M->d.bc[ib];
+    int js, je, ks, ke;
+    bc_range(M, b, &js, &je, &ks, &ke);
+    int idir = b->ibdir;
+    switch (b->ibtyp) {
+      case 3: bcwall(c, M, &v, js, je, ks, ke, idir, 1); break;
+      case 4: bcwall(c, M, &v, js, je, ks, ke, idir, 0); break;
+      case 5: bcwall(c, M, &v, js, je, ks, ke, idir, c->p.invisc); break;
+      case 10: bcextp(c, M, &v, js, je, ks, ke, idir); break;
+      case 11: bcsym(c, M, &v, js, je, ks, ke, idir); break;
+      case 22: bcperiodic(c, M, &v, js, je, ks, ke, idir); break;
+      case 51: bcwake(c, M, &v, js, je, ks, ke, idir); break;
+      case 46: bcout_inf(c, M, &v, js, je, ks, ke, idir); break;
+      case 47: bcout_trkl(c, M, &v, js, je, ks, ke, idir); break;
+      default: break;
+    }
+  }
+}
+
+int32_t or_bc(or_case* c) {
+  for (int im = 0; im < c->nmesh; ++im)
+    for (int nsp = 0; nsp < c->nspec; ++nsp) bc_mesh(c, &c->m[im], nsp);
+  return OT2D_OK;
+}
+
+/* -------------------------------------------------- reconstruction (a2) */
+/* muscld_new, src/eulerrhs.f:821-912 (ibmin = ibmax = 1) */
+static void muscld_new(double* f, double* ql, double* qr, int mdim, int is, int imax, int im,
+                       double th, double qt, double eps, const int* iba, double* f2) {
+  if (qt == 0.0) {
+    for (int n = 0; n < 4; ++n)
+      for (int i = is; i <= imax; ++i) { ql[n * mdim + i] = f[n * mdim + i]; qr[n * mdim + i] = f[n * mdim + i]; }
+    return;
+  }
+  double thm = 1.0 - th, thp = 1.0 + th;
+  for (int n = 0; n < 4; ++n) {
+    double* fn = f + n * mdim;
+    double* l = ql + n * mdim;
+    double* r = qr + n * mdim;
+    for (int i = is; i <= im; ++i) f2[i] = fn[i + 1] - fn[i];
+    for (int i = is + 1; i <= im; ++i) {
+      double f2i = f2[i], f2i1 = f2[i - 1];
+      double a1 = 3.0 * (f2i * f2i1 + eps);
+      double a2 = 2.0 * sq(f2i - f2i1) + a1;
+      double f3i = a1 / a2;
+      double f3qt = qt * f3i * (double)iba[i] * (double)iba[i + 1] * (double)iba[i - 1];
+      l[i] = fn[i] + f3qt * (thm * f2i1 + thp * f2i);
+      r[i] = fn[i] - f3qt * (thp * f2i1 + thm * f2i);
+    }
+    r[is] = fn[is]; l[is] = fn[is];
+    l[imax] = fn[imax]; r[imax] = fn[imax];
+  }
+}
+
+/* ---------------------------------------------------------- Roe (a4) */
+/* roeflx, src/eulerrhs.f:283-446 */
+static void roeflx(or_case* c, double* f, const double* ql, const double* qr, int mdim,
+                   const double* xa, const double* ya, const double* tj, int is, int ie) {
+  double gm1 = c->gm1;
+  double eps = 1.e-6;
+  for (int i = is; i <= ie; ++i) {
+    int i1 = i + 1;
+    double rlft = ql[i], ulft = ql[mdim + i], vlft = ql[2 * mdim + i], plft = ql[3 * mdim + i];
+    double rlfti = 1.0 / rlft;
+    double rulft = rlft * ulft, rvlft = rlft * vlft;
+    double uvl = 0.5 * (ulft * ulft + vlft * vlft);
+    double elft = plft / gm1 + rlft * uvl;
+    double hlft = (elft + plft) * rlfti;
+    double clft = sqrt(gm1 * (hlft - uvl));
+    double rrht = qr[i1], urht = qr[mdim + i1], vrht = qr[2 * mdim + i1], prht = qr[3 * mdim + i1];
+    double rrhti = 1.0 / rrht;
+    double rurht = rrht * urht, rvrht = rrht * vrht;
+    double uvr = 0.5 * (urht * urht + vrht * vrht);
+    double erht = prht / gm1 + rrht * uvr;
+    double hrht = (erht + prht) * rrhti;
+    double crht = sqrt(gm1 * (hrht - uvr));
+    double rat = sqrt(rrht * rlfti);
+    double rati = 1.0 / (rat + 1.);
+    double rav = rat * rlft;
+    double uav = (rat * urht + ulft) * rati;
+    double vav = (rat * vrht + vlft) * rati;
+    double hav = (rat * hrht + hlft) * rati;
+    double uv = 0.5 * (uav * uav + vav * vav);
+    double cav = sqrt(gm1 * (hav - uv));
+    double aq1 = rrht - rlft, aq2 = urht - ulft, aq3 = vrht - vlft, aq4 = prht - plft;
+    double ri1 = xa[i], ri2 = ya[i], ri3 = tj[i];
+    double rr2 = ri1 * ri1 + ri2 * ri2;
+    double rr = sqrt(rr2);
+    double r0 = 1.0 / rr;
+    double r1 = ri1 * r0, r2 = ri2 * r0, r3 = ri3 * r0;
+    double uu = r1 * uav + r2 * vav + r3;
+    double c2 = cav * cav;
+    double c2i = 1.0 / c2;
+    double auu = fabs(uu), aupc = fabs(uu + cav), aumc = fabs(uu - cav);
+    double uulft = r1 * ulft + r2 * vlft + r3;
+    double uurht = r1 * urht + r2 * vrht + r3;
+    double upclft = uulft + clft, upcrht = uurht + crht;
+    double umclft = uulft - clft, umcrht = uurht - crht;
+    double dauu = 4. * (uurht - uulft) + eps;
+    double dauus = MAX(dauu, 0.0);
+    if (auu <= 0.5 * dauus) auu = sq(auu) / dauu + 0.25 * dauu;
+    double daupc = 4. * (upcrht - upclft) + eps;
+    double daupcs = MAX(daupc, 0.0);
+    if (aupc <= 0.5 * daupcs) aupc = sq(aupc) / daupc + 0.25 * daupc;
+    double daumc = 4. * (umcrht - umclft) + eps;
+    double daumcs = MAX(daumc, 0.0);
+    if (aumc <= 0.5 * daumcs) aumc = sq(aumc) / daumc + 0.25 * daumc;
+    double rcav = rav * cav;
+    double aquu = uurht - uulft;
+    double c2ih = 0.5 * c2i;
+    double ruuav = auu * rav;
+    double b1 = auu * (aq1 - c2i * aq4);
+    double b2 = c2ih * aupc * (aq4 + rcav * aquu);
+    double b3 = c2ih * aumc * (aq4 - rcav * aquu);
+    double b4 = b1 + b2 + b3;
+    double b5 = cav * (b2 - b3);
+    double b6 = ruuav * (aq2 - r1 * aquu);
+    double b7 = ruuav * (aq3 - r2 * aquu);
+    aq1 = b4;
+    aq2 = uav * b4 + r1 * b5 + b6;
+    aq3 = vav * b4 + r2 * b5 + b7;
+    aq4 = hav * b4 + (uu - r3) * b5 + uav * b6 + vav * b7 - c2 * b1 / gm1;
+    double aj = 0.5 * rr;
+    double plar = plft + prht;
+    double eplft = elft + plft, eprht = erht + prht;
+    double fssub = 0.0;
+    f[i] = aj * (rlft * uulft + rrht * uurht - aq1) - fssub * c->rinf;
+    f[mdim + i] = aj * (rulft * uulft + rurht * uurht + r1 * plar - aq2) - fssub * c->rinf * c->uinf;
+    f[2 * mdim + i] = aj * (rvlft * uulft + rvrht * uurht + r2 * plar - aq3) - fssub * c->rinf * c->vinf;
+    f[3 * mdim + i] = aj * (eplft * uulft + eprht * uurht - r3 * plar - aq4) - fssub * c->einf;
+  }
+}
+
+/* --------------------------------------------------- inviscid rhs (a1) */
+/* rhsup, src/eulerrhs.f:2-280 (nei = 1) */
+static int rhsup_mesh(or_case* c, or_mesh* M, int nsp) {
+  int ilima = abs(c->p.ilim);
+  if (!(ilima == 1 || c->p.ilim == 4)) return OT2D_ERR_UNSUPPORTED;
+  int irhs = abs(c->p.irhsy);
+  double th = (double)(irhs - 2) / (double)irhs;
+  double qt = 0.25;
+  if (irhs == 1) qt = 0.0;
+  double epsj = cube(10. / (double)M->jmax);
+  double epsk = cube(10. / (double)M->kmax);
+  int mdim = c->mdim + 2;
+  double* f = (double*)calloc((size_t)mdim * 4, sizeof(double));
+  double* ql = (double*)calloc((size_t)mdim * 4, sizeof(double));
+  double* qr = (double*)calloc((size_t)mdim * 4, sizeof(double));
+  double* f2 = (double*)calloc((size_t)mdim, sizeof(double));
+  double* xa = (double*)calloc((size_t)mdim, sizeof(double));
+  double* ya = (double*)calloc((size_t)mdim, sizeof(double));
+  double* tj = (double*)calloc((size_t)mdim, sizeof(double));
+  int* iba = (int*)calloc((size_t)mdim, sizeof(int));
+  double* ppp = (double*)calloc(M->n2, sizeof(double));
+  qview v = view(c, M, nsp);
+  const int32_t* ib = M->iblank + (size_t)nsp * M->n2;
+  const double* xv = VA(M, xv, nsp);
+  const double* yv = VA(M, yv, nsp);
+  const double* ugv = VA(M, ugv, nsp);
+  const double* vgv = VA(M, vgv, nsp);
+  double *s1 = SA(M, nsp, 1), *s2 = SA(M, nsp, 2), *s3 = SA(M, nsp, 3), *s4 = SA(M, nsp, 4);
+  double* ss[4] = {s1, s2, s3, s4};
+  for (size_t i = 0; i < M->n2; ++i)
+    ppp[i] = c->gm1 * (v.q4[i] - 0.5 * (sq(v.q2[i]) + sq(v.q3[i])) / v.q1[i]);
+  /* xi fluxes along k = const lines */
+  for (int k = M->kbeg; k <= M->kend; ++k) {
+    int kv = k;
+    for (int j = 1; j <= M->jd; ++j) {
+      size_t i = IX(M, j, k);
+      double rhoi = 1.0 / v.q1[i];
+      f[j] = v.q1[i] * v.qj[i];
+      f[mdim + j] = v.q2[i] * rhoi;
+      f[2 * mdim + j] = v.q3[i] * rhoi;
+      f[3 * mdim + j] = ppp[i] * v.qj[i];
+      iba[j] = abs(ib[i]);
+    }
+    if (ilima == 1) muscld_new(f, ql, qr, mdim, 1, M->jd, M->jd - 1, th, qt, epsj, iba, f2);
+    for (int j = M->jbeg - 1; j <= M->jend; ++j) {
+      int jv = j;
+      xa[j] = (yv[VX(M, jv, kv)] - yv[VX(M, jv, kv - 1)]);
+      ya[j] = -(xv[VX(M, jv, kv)] - xv[VX(M, jv, kv - 1)]);
+    }
+    if (c->timespectral) {
+      for (int j = M->jbeg - 1; j <= M->jend; ++j) {
+        int jv = j;
+        tj[j] = -0.5 * (ugv[VX(M, jv, kv)] + ugv[VX(M, jv, kv - 1)]) * xa[j]
+                - 0.5 * (vgv[VX(M, jv, kv)] + vgv[VX(M, jv, kv - 1)]) * ya[j];
+      }
+    } else {
+      for (int j = M->jbeg - 1; j <= M->jend; ++j) tj[j] = 0.0;
+    }
+    roeflx(c, f, ql, qr, mdim, xa, ya, tj, M->jbeg - 1, M->jend);
+    for (int j = M->jbeg; j <= M->jend; ++j) {
+      size_t i = IX(M, j, k);
+      for (int n = 0; n < 4; ++n) ss[n][i] = ss[n][i] - (f[n * mdim + j] - f[n * mdim + j - 1]);
+    }
+  }
+  /* eta fluxes along j = const lines */
+  for (int j = M->jbeg; j <= M->jend; ++j) {
+    int jv = j;
+    for (int k = 1; k <= M->kd; ++k) {
+      size_t i = IX(M, j, k);
+      double rhoi = 1.0 / v.q1[i];
+      f[k] = v.q1[i] * v.qj[i];
+      f[mdim + k] = v.q2[i] * rhoi;
+      f[2 * mdim + k] = v.q3[i] * rhoi;
+      f[3 * mdim + k] = ppp[i] * v.qj[i];
+      iba[k] = abs(ib[i]);
+    }
+    if (ilima == 1) muscld_new(f, ql, qr, mdim, 1, M->kd, M->kd - 1, th, qt, epsk, iba, f2);
+    for (int k = M->kbeg - 1; k <= M->kend; ++k) {
+      int kv = k;
+      xa[k] = -(yv[VX(M, jv, kv)] - yv[VX(M, jv - 1, kv)]);
+      ya[k] = (xv[VX(M, jv, kv)] - xv[VX(M, jv - 1, kv)]);
+    }
+    if (c->timespectral) {
+      for (int k = M->kbeg - 1; k <= M->kend; ++k) {
+        int kv = k;
+        tj[k] = -0.5 * (ugv[VX(M, jv, kv)] + ugv[VX(M, jv - 1, kv)]) * xa[k]
+                - 0.5 * (vgv[VX(M, jv, kv)] + vgv[VX(M, jv - 1, kv)]) * ya[k];
+      }
+    } else {
+      for (int k = M->kbeg - 1; k <= M->kend; ++k) tj[k] = 0.;
+    }
+    roeflx(c, f, ql, qr, mdim, xa, ya, tj, M->kbeg - 1, M->kend);
+    for (int k = M->kbeg; k <= M->kend; ++k) {
+      size_t i = IX(M, j, k);
+      for (int n = 0; n < 4; ++n) ss[n][i] = ss[n][i] - (f[n * mdim + k] - f[n * mdim + k - 1]);
+    }
+  }
+  free(f); free(ql); free(qr); free(f2); free(xa); free(ya); free(tj); free(iba); free(ppp);
+  return OT2D_OK;
+}
+
+/* timespectralrhs, src/sources.f:2-27 */
+static void timespectralrhs(or_case* c, or_mesh* M, int nsp) {
+  int N = c->nspec;
+  for (int n = 1; n <= c->nv; ++n) {
+    double* s = SA(M, nsp, n);
+    for (int nsp1 = 0; nsp1 < N; ++nsp1) {
+      double dsv = c->Ds[nsp + (size_t)nsp1 * N];
+      const double* qn = QA(M, nsp1, n);
+      for (int k = M->kbeg; k <= M->kend; ++k)
+        for (int j = M->jbeg; j <= M->jend; ++j) {
+          size_t i = IX(M, j, k);
+          s[i] = s[i] - M->rotf * dsv * qn[i];
+        }
+    }
+  }
+}
+
+/* compute_residual, src/step.f:221-300 */
+int32_t or_compute_residual(or_case* c) {
+  for (int im = 0; im < c->nmesh; ++im) {
+    or_mesh* M = &c->m[im];
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      bc_mesh(c, M, nsp);
+      int rc = rhsup_mesh(c, M, nsp);
+      if (rc) return rc;
+      if (!c->p.invisc) return OT2D_ERR_UNSUPPORTED;
+      if (c->timespectral) timespectralrhs(c, M, nsp);
+    }
+  }
+  return OT2D_OK;
+}
+
+/* ---------------------------------------------------- residual BC (a7) */
+/* rbc + zero_residuals + rbcwake/rbcextp/rbcsym/rbcperiodic,
+ * src/bc.f:88-177, 1636-1666, 1576-1633, 2029-2094, 2160-2222, 270-357 */
+static void rbc_mesh(or_case* c, or_mesh* M, int nsp) {
+  double* qj = QA(M, nsp, c->nq);
+  double* ss[4] = {SA(M, nsp, 1), SA(M, nsp, 2), SA(M, nsp, 3), SA(M, nsp, 4)};
+  for (int ib = 0; ib < M->d.nbc; ++ib) {
+    const ot2d_bcseg* b = &M->d.bc[ib];
+    int js, je, ks, ke;
+    bc_range(M, b, &js, &je, &ks, &ke);
+    int idir = b->ibdir, iadd = idir >= 0 ? 1 : -1, iadir = abs(idir);
+    int t = b->ibtyp;
+    if (t == 3 || t == 4 || t == 5 || t == 46 || t == 47) {
+      for (int j = js; j <= je; ++j)
+        for (int k = ks; k <= ke; ++k)
+          for (int n = 0; n < 4; ++n) ss[n][IX(M, j, k)] = 0.0;
+    } else if (t == 10) {
+      double foso = 0.0;
+      if (iadir == 1) {
+        for (int jc = 1; jc <= je - js + 1; ++jc) {
+          int j = (idir == 1) ? je - jc + 1 : js + jc - 1;
+          int j1 = j + iadd, j2 = j1 + iadd;
+          for (int k = ks; k <= ke; ++k) {
+            size_t i = IX(M, j, k), i1 = IX(M, j1, k), i2 = IX(M, j2, k);
+            double s1 = qj[i1] / qj[i], s2 = qj[i2] / qj[i];
+            for (int n = 0; n < 4; ++n) ss[n][i] = (1. + foso) * ss[n][i1] * s1 - foso * ss[n][i2] * s2;
+          }
+        }
+      } else if (iadir == 2) {
+        for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+          int k = (idir == 2) ? ke - kc + 1 : ks + kc - 1;
+          int k1 = k + iadd, k2 = k1 + iadd;
+          for (int j = js; j <= je; ++j) {
+            size_t i = IX(M, j, k), i1 = IX(M, j, k1), i2 = IX(M, j, k2);
+            double s1 = qj[i1] / qj[i], s2 = qj[i2] / qj[i];
+            for (int n = 0; n < 4; ++n) ss[n][i] = (1. + foso) * ss[n][i1] * s1 - foso * ss[n][i2] * s2;
+          }
+        }
+      }
+    } else if (t == 11) {
+      if (iadir == 1) {
+        for (int jc = 1; jc <= je - js + 1; ++jc) {
+          int j, j1;
+          if (idir == 1) { j = je - jc + 1; j1 = je + jc; } else { j = js + jc - 1; j1 = js - jc; }
+          for (int k = ks; k <= ke; ++k) {
+            size_t i = IX(M, j, k), i1 = IX(M, j1, k);
+            double sc = qj[i1] / qj[i];
+            ss[0][i] = ss[0][i1] * sc; ss[1][i] = -ss[1][i1] * sc;
+            ss[2][i] = ss[2][i1] * sc; ss[3][i] = ss[3][i1] * sc;
+          }
+        }
+      } else if (iadir == 2) {
+        for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+          int k, k1;
+          if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+          for (int j = js; j <= je; ++j) {
+            size_t i = IX(M, j, k), i1 = IX(M, j, k1);
+            double sc = qj[i1] / qj[i];
+            ss[0][i] = ss[0][i1] * sc; ss[1][i] = ss[1][i1] * sc;
+            ss[2][i] = -ss[2][i1] * sc; ss[3][i] = ss[3][i1] * sc;
+          }
+        }
+      }
+    } else if (t == 22) {
+      if (iadir == 1) {
+        for (int j = js; j <= je; ++j) {
+          int jc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, j);
+          for (int k = ks; k <= ke; ++k) {
+            size_t i = IX(M, j, k), i1 = IX(M, jc, k);
+            double sc = qj[i1] / qj[i];
+            for (int n = 0; n < 4; ++n) ss[n][i] = ss[n][i1] * sc;
+          }
+        }
+      } else {
+        for (int k = ks; k <= ke; ++k) {
+          int kc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, k);
+          for (int j = js; j <= je; ++j) {
+            size_t i = IX(M, j, k), i1 = IX(M, j, kc);
+            double sc = qj[i1] / qj[i];
+            for (int n = 0; n < 4; ++n) ss[n][i] = ss[n][i1] * sc;
+          }
+        }
+      }
+    } else if (t == 51 && iadir == 2) {
+      for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+        int k, k1;
+        if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+        for (int j = js; j <= je; ++j) {
+          int jj = M->jd - j + 1;
+          size_t a = IX(M, j, k), bb = IX(M, jj, k1), cc = IX(M, jj, k), d = IX(M, j, k1);
+          double scale1 = qj[bb] / qj[a];
+          double scale2 = qj[d] / qj[cc];
+          for (int n = 0; n < 4; ++n) ss[n][a] = ss[n][bb] * scale1;
+          for (int n = 0; n < 4; ++n) ss[n][cc] = ss[n][d] * scale2;
+        }
+      }
+    }
+  }
+}
+
+int32_t or_rbc(or_case* c) {
+  for (int im = 0; im < c->nmesh; ++im)
+    for (int nsp = 0; nsp < c->nspec; ++nsp) rbc_mesh(c, &c->m[im], nsp);
+  return OT2D_OK;
+}
+
+/* ----------------------------------------------------------- LU-SGS (a11) */
+/* ilu2d, src/lhs.f:470-814 (iunst != 2 branch) */
+static void ilu2d(or_case* c, or_mesh* M, int nsp) {
+  int jd = M->jd, kd = M->kd, js = M->jbeg, je = M->jend, ks = M->kbeg, ke = M->kend;
+  double gm1 = c->gm1, gamma = c->p.gamma, ggm1 = c->ggm1;
+  double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+  double* qj = QA(M, nsp, c->nq);
+  double* s[4] = {SA(M, nsp, 1), SA(M, nsp, 2), SA(M, nsp, 3), SA(M, nsp, 4)};
+  double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+  double *ug = GA(M, ug, nsp), *vg = GA(M, vg, nsp), *turmu = GA(M, turmu, nsp), *ts = GA(M, tscale, nsp);
+  size_t n2 = M->n2;
+  double* d = (double*)calloc(n2, sizeof(double));
+  double* st = (double*)calloc(n2 * 4, sizeof(double));
+  double* uv = (double*)calloc(n2, sizeof(double));
+  double* vn = (double*)calloc(n2, sizeof(double));
+  double* ge = (double*)calloc(n2, sizeof(double));
+  double* qx = (double*)calloc(n2, sizeof(double));
+  double* qy = (double*)calloc(n2, sizeof(double));
+  double* cx = (double*)calloc(n2, sizeof(double));
+  double* cy = (double*)calloc(n2, sizeof(double));
+  int mdim = c->mdim + 2;
+  double* a = (double*)calloc((size_t)mdim * 4, sizeof(double));
+  double* b = (double*)calloc((size_t)mdim * 4, sizeof(double));
+  double eps2 = c->p.epse * 2.;
+  double epsv = 1. + eps2;
+  double rmue = c->rmue, rey = c->rey;
+  (void)jd; (void)kd;
+  for (int k = ks - 1; k <= ke + 1; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k);
+      double dj = 1.0 / q1[i];
+      q2[i] = q2[i] * dj; q3[i] = q3[i] * dj; q4[i] = q4[i] * dj;
+      q1[i] = q1[i] * qj[i];
+    }
+  for (int k = ks - 1; k <= ke + 1; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k);
+      double uu = q2[i], vv = q3[i];
+      double uv2 = 0.5 * (uu * uu + vv * vv);
+      double cjkl = sqrt(ggm1 * (q4[i] - uv2));
+      double rj1 = xx[i], rj2 = xy[i];
+      double qq1 = rj1 * uu + rj2 * vv;
+      double qqx = fabs(-rj1 * ug[i] - rj2 * vg[i] + qq1);
+      double rr1 = sqrt(sq(rj1) + sq(rj2));
+      double rk1 = yx[i], rk2 = yy[i];
+      double qq2 = rk1 * uu + rk2 * vv;
+      double qqy = fabs(-rk1 * ug[i] - rk2 * vg[i] + qq2);
+      double rr2 = sqrt(sq(rk1) + sq(rk2));
+      double vnu = 2.0 * rr2 * rr2 * (rmue + turmu[i]) / (rey * q1[i]);
+      double svt = ts[i];
+      d[i] = 1.0 / (1.0 + svt * ((qqx + qqy + cjkl * (rr1 + rr2)) * epsv + vnu));
+      uv[i] = uv2; qx[i] = qq1; cx[i] = cjkl * rr1; qy[i] = qq2; cy[i] = cjkl * rr2; vn[i] = vnu;
+      ge[i] = gamma * q4[i] - gm1 * uv2;
+    }
+  for (int k = ks; k <= ke; ++k) {
+    size_t i = IX(M, js - 1, k), i2 = IX(M, je + 1, k);
+    for (int n = 0; n < 4; ++n) s[n][i] = s[n][i] * d[i];
+    for (int n = 0; n < 4; ++n) s[n][i2] = s[n][i2] * d[i2];
+  }
+  for (int j = js; j <= je; ++j) {
+    size_t i = IX(M, j, ks - 1), i2 = IX(M, j, ke + 1);
+    for (int n = 0; n < 4; ++n) s[n][i] = s[n][i] * d[i];
+    for (int n = 0; n < 4; ++n) s[n][i2] = s[n][i2] * d[i2];
+  }
+  for (int k = ks - 1; k <= ke + 1; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k);
+      for (int n = 0; n < 4; ++n) st[n * n2 + i] = s[n][i];
+    }
+  /* forward sweep over hyperplanes j+k = m */
+  for (int m = js + ks; m <= je + ke; ++m) {
+    int ms = MAX(js, m - ke), me = MIN(je, m - js);
+    for (int j = ms; j <= me; ++j) {
+      int k = m - j;
+      size_t i1 = IX(M, j - 1, k);
+      double uu = q2[i1], vv = q3[i1], uv2 = uv[i1], ri1 = xx[i1], ri2 = xy[i1], qq = qx[i1], cc = cx[i1];
+      double qqx = -ri1 * ug[i1] - ri2 * vg[i1] + qq;
+      double sp1 = fabs(qqx) + cc;
+      double sm2 = eps2 * sp1;
+      double chkx = 0.5 + fsign(0.5, qqx + cc);
+      double spec = chkx * (qqx + sp1) + sm2;
+      double s1 = s[0][i1], s2 = s[1][i1], s3 = s[2][i1], s4 = s[3][i1];
+      double a4 = chkx * (ri1 * s2 + ri2 * s3 - qq * s1);
+      double a2 = chkx * gm1 * (uv2 * s1 - (uu * s2 + vv * s3) + s4);
+      a[j] = a4 + spec * s1;
+      a[mdim + j] = ri1 * a2 + uu * a4 + spec * s2;
+      a[2 * mdim + j] = ri2 * a2 + vv * a4 + spec * s3;
+      a[3 * mdim + j] = qq * a2 + ge[i1] * a4 + spec * s4;
+    }
+    for (int j = ms; j <= me; ++j) {
+      int k = m - j;
+      size_t i1 = IX(M, j, k - 1);
+      double uu = q2[i1], vv = q3[i1], uv2 = uv[i1], ri1 = yx[i1], ri2 = yy[i1], qq = qy[i1], cc = cy[i1];
+      double vnu = vn[i1];
+      double qqy = -ri1 * ug[i1] - ri2 * vg[i1] + qq;
+      double sp1 = fabs(qqy) + cc;
+      double sm2 = eps2 * sp1;
+      double chky = 0.5 + fsign(0.5, qqy + cc);
+      double spec = chky * (qqy + sp1) + vnu + sm2;
+      double s1 = s[0][i1], s2 = s[1][i1], s3 = s[2][i1], s4 = s[3][i1];
+      double a4 = chky * (ri1 * s2 + ri2 * s3 - qq * s1);
+      double a2 = chky * gm1 * (uv2 * s1 - (uu * s2 + vv * s3) + s4);
+      b[j] = a4 + spec * s1;
+      b[mdim + j] = ri1 * a2 + uu * a4 + spec * s2;
+      b[2 * mdim + j] = ri2 * a2 + vv * a4 + spec * s3;
+      b[3 * mdim + j] = qq * a2 + ge[i1] * a4 + spec * s4;
+    }
+    for (int j = ms; j <= me; ++j) {
+      int k = m - j;
+      size_t i = IX(M, j, k);
+      double sv = ts[i] * 0.5;
+      double di = d[i];
+      for (int n = 0; n < 4; ++n)
+        s[n][i] = (st[n * n2 + i] + sv * (a[n * mdim + j] + b[n * mdim + j])) * di;
+    }
+  }
+  /* backward sweep */
+  for (int m = je + ke; m >= js + ks; --m) {
+    int ms = MAX(js, m - ke), me = MIN(je, m - js);
+    for (int j = ms; j <= me; ++j) {
+      int k = m - j;
+      size_t i1 = IX(M, j + 1, k);
+      double uu = q2[i1], vv = q3[i1], uv2 = uv[i1], ri1 = xx[i1], ri2 = xy[i1], qq = qx[i1], cc = cx[i1];
+      double qqx = -ri1 * ug[i1] - ri2 * vg[i1] + qq;
+      double sp1 = fabs(qqx) + cc;
+      double sm2 = eps2 * sp1;
+      double chkx = 0.5 - fsign(0.5, qqx - cc);
+      double spec = chkx * (qqx - sp1) - sm2;
+      double s1 = s[0][i1], s2 = s[1][i1], s3 = s[2][i1], s4 = s[3][i1];
+      double a4 = chkx * (ri1 * s2 + ri2 * s3 - qq * s1);
+      double a2 = chkx * gm1 * (uv2 * s1 - (uu * s2 + vv * s3) + s4);
+      a[j] = a4 + spec * s1;
+      a[mdim + j] = ri1 * a2 + uu * a4 + spec * s2;
+      a[2 * mdim + j] = ri2 * a2 + vv * a4 + spec * s3;
+      a[3 * mdim + j] = qq * a2 + ge[i1] * a4 + spec * s4;
+    }
+    for (int j = ms; j <= me; ++j) {
+      int k = m - j;
+      size_t i1 = IX(M, j, k + 1);
+      double uu = q2[i1], vv = q3[i1], uv2 = uv[i1], ri1 = yx[i1], ri2 = yy[i1], qq = qy[i1], cc = cy[i1];
+      double vnu = vn[i1];
+      double qqy = -ri1 * ug[i1] - ri2 * vg[i1] + qq;
+      double sp1 = fabs(qqy) + cc;
+      double sm2 = eps2 * sp1;
+      double chky = 0.5 - fsign(0.5, qqy - cc);
+      double spec = chky * (qqy - sp1) - vnu - sm2;
+      double s1 = s[0][i1], s2 = s[1][i1], s3 = s[2][i1], s4 = s[3][i1];
+      double a4 = chky * (ri1 * s2 + ri2 * s3 - qq * s1);
+      double a2 = chky * gm1 * (uv2 * s1 - (uu * s2 + vv * s3) + s4);
+      b[j] = a4 + spec * s1;
+      b[mdim + j] = ri1 * a2 + uu * a4 + spec * s2;
+      b[2 * mdim + j] = ri2 * a2 + vv * a4 + spec * s3;
+      b[3 * mdim + j] = qq * a2 + ge[i1] * a4 + spec * s4;
+    }
+    for (int j = ms; j <= me; ++j) {
+      int k = m - j;
+      size_t i = IX(M, j, k);
+      double sv = ts[i] * 0.5;
+      double di = d[i];
+      for (int n = 0; n < 4; ++n) s[n][i] = s[n][i] - sv * (a[n * mdim + j] + b[n * mdim + j]) * di;
+    }
+  }
+  for (int k = ks - 1; k <= ke + 1; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k);
+      q1[i] = q1[i] / qj[i];
+      q2[i] = q2[i] * q1[i]; q3[i] = q3[i] * q1[i]; q4[i] = q4[i] * q1[i];
+    }
+  free(d); free(st); free(uv); free(vn); free(ge); free(qx); free(qy); free(cx); free(cy);
+  free(a); free(b);
+}
+
+/* s *= tscale then ilu2d, src/step.f:141-159 */
+static void lhs_mesh(or_case* c, or_mesh* M, int nsp) {
+  double* ts = GA(M, tscale, nsp);
+  for (int n = 1; n <= 4; ++n) {
+    double* s = SA(M, nsp, n);
+    for (size_t i = 0; i < M->n2; ++i) s[i] = s[i] * ts[i];
+  }
+  ilu2d(c, M, nsp);
+}
+
+int32_t or_lhs(or_case* c) {
+  if (c->p.ilhs != 1 || c->p.iprecon) return OT2D_ERR_UNSUPPORTED;
+  for (int im = 0; im < c->nmesh; ++im)
+    for (int nsp = 0; nsp < c->nspec; ++nsp) lhs_mesh(c, &c->m[im], nsp);
+  return OT2D_OK;
+}
+
+/* ------------------------------------------------------- monitor (a15) */
+/* monitor, src/step.f:332-392 */
+int32_t or_monitor(or_case* c, int32_t* mstop) {
+  int stop = 0;
+  for (int im = 0; im < c->nmesh; ++im) {
+    or_mesh* M = &c->m[im];
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+      for (size_t i = 0; i < M->n2; ++i) {
+        double rho = q1[i];
+        double qq = sq(q2[i]) + sq(q3[i]);
+        double asq = c->ggm1 * (q4[i] - .5 * qq / rho) / rho;
+        if (asq <= 0.002 || asq + 1 == asq) stop = 1;
+      }
+    }
+  }
+  if (mstop) *mstop = stop;
+  return OT2D_OK;
+}
+
+/* ---------------------------------------------------------- step (a9) */
+/* residual norms, src/step.f:53-115 */
+static int norms_mesh(or_case* c, or_mesh* M, int nsp, double* out, int32_t* am) {
+  const int32_t* ib = M->iblank + (size_t)nsp * M->n2;
+  double rsum = 0., resrho = 0., resmax = 0.;
+  int jd1 = 2, kd1 = 2, nd1 = 1;
+  for (int k = M->kbeg; k <= M->kend; ++k) {
+    for (int n = 1; n <= c->nmv; ++n) {
+      const double* s = SA(M, nsp, n);
+      for (int j = M->jbeg; j <= M->jend; ++j) {
+        size_t i = IX(M, j, k);
+        double smrs = fabs(s[i]) * (double)MAX(ib[i], 0);
+        if (smrs > resmax) { jd1 = j; kd1 = k; nd1 = n; resmax = smrs; }
+      }
+    }
+    const double *s1 = SA(M, nsp, 1), *s2 = SA(M, nsp, 2), *s3 = SA(M, nsp, 3), *s4 = SA(M, nsp, 4);
+    for (int j = M->jbeg; j <= M->jend; ++j) {
+      size_t i = IX(M, j, k);
+      double ibp = (double)MAX(ib[i], 0);
+      double m1 = s1[i] * ibp, m2 = s2[i] * ibp, m3 = s3[i] * ibp, m4 = s4[i] * ibp;
+      resrho = resrho + sq(s1[i]);
+      rsum = rsum + m1 * m1 + m2 * m2 + m3 * m3 + m4 * m4;
+    }
+  }
+  double volum = (double)(M->jd * M->kd);
+  resrho = sqrt(resrho / volum);
+  rsum = sqrt(rsum / volum);
+  if (out) { out[0] = rsum; out[1] = resrho; out[2] = resmax; }
+  if (am) { am[0] = jd1; am[1] = kd1; am[2] = nd1; }
+  return rsum > 1000.0 ? OT2D_NORM : OT2D_OK;
+}
+
+/* step, src/step.f:2-218, with istep0 advanced first
+ * (src/Turns_overset.f:364) and monitor after (src/Turns_overset.f:495) */
+int32_t or_step(or_case* c, double* norms, int32_t* argmax) {
+  c->istep0 += 1;
+  c->istep += 1;
+  int status = OT2D_OK;
+  for (int im = 0; im < c->nmesh; ++im) {
+    or_mesh* M = &c->m[im];
+    time_step_mesh(c, M);
+    memset(M->s, 0, M->n2 * c->nspec * c->nv * sizeof(double));
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      bc_mesh(c, M, nsp);
+      int rc = rhsup_mesh(c, M, nsp);
+      if (rc) return rc;
+      if (!c->p.invisc) return OT2D_ERR_UNSUPPORTED;
+      if (c->timespectral) timespectralrhs(c, M, nsp);
+    }
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      size_t o = (size_t)im * c->nspec + nsp;
+      int rc = norms_mesh(c, M, nsp, norms ? norms + 3 * o : NULL, argmax ? argmax + 3 * o : NULL);
+      if (rc == OT2D_NORM) return OT2D_NORM;
+    }
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      /* s = s + ss with ss = 0 (src/step.f:126-132) */
+      for (int n = 1; n <= c->nv; ++n) {
+        double* s = SA(M, nsp, n);
+        for (int k = M->kbeg; k <= M->kend; ++k)
+          for (int j = M->jbeg; j <= M->jend; ++j) s[IX(M, j, k)] = s[IX(M, j, k)] + 0.0;
+      }
+      rbc_mesh(c, M, nsp);
+      if (c->p.ilhs != 1 || c->p.iprecon) return OT2D_ERR_UNSUPPORTED;
+      lhs_mesh(c, M, nsp);
+    }
+    if (!c->lamin) return OT2D_ERR_UNSUPPORTED;
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      const int32_t* ib = M->iblank + (size_t)nsp * M->n2;
+      for (int n = 1; n <= 4; ++n) {
+        double* q = QA(M, nsp, n);
+        const double* s = SA(M, nsp, n);
+        for (int k = M->kbeg; k <= M->kend; ++k)
+          for (int j = M->jbeg; j <= M->jend; ++j) {
+            size_t i = IX(M, j, k);
+            q[i] = q[i] + s[i] * (double)MAX(ib[i], 0);
+          }
+      }
+    }
+    int32_t ms = 0;
+    /* monitor per mesh, as the driver calls it right after step */
+    {
+      int stop = 0;
+      for (int nsp = 0; nsp < c->nspec; ++nsp) {
+        double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+        for (size_t i = 0; i < M->n2; ++i) {
+          double rho = q1[i];
+          double qq = sq(q2[i]) + sq(q3[i]);
+          double asq = c->ggm1 * (q4[i] - .5 * qq / rho) / rho;
+          if (asq <= 0.002 || asq + 1 == asq) stop = 1;
+        }
+      }
+      ms = stop;
+    }
+    if (ms) { status = OT2D_MSTOP; break; }
+  }
+  return status;
+}
+
+int32_t or_iterate(or_case* c, int32_t niter, double* hist, int32_t* done) {
+  size_t per = (size_t)c->nmesh * c->nspec;
+  double* nb = (double*)calloc(per * 3, sizeof(double));
+  int rc = OT2D_OK, it;
+  for (it = 0; it < niter; ++it) {
+    rc = or_step(c, nb, NULL);
+    if (hist)
+      for (size_t o = 0; o < per; ++o) hist[(size_t)it * per + o] = nb[3 * o];
+    if (rc != OT2D_OK) { ++it; break; }
+  }
+  if (done) *done = it;
+  free(nb);
+  return rc;
+}
+
+/* ------------------------------------------------------- forces (a16) */
+/* compute_forces + force2d, src/forces.f:2-107, 111-398 */
+int32_t or_forces(or_case* c, int32_t im, int32_t nsp, double out[6]) {
+  if (!c || im < 0 || im >= c->nmesh || nsp < 0 || nsp >= c->nspec) return OT2D_ERR_ARG;
+  or_mesh* M = &c->m[im];
+  double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+  double* qj = QA(M, nsp, c->nq);
+  const double* xv = VA(M, xv, nsp);
+  const double* yv = VA(M, yv, nsp);
+  double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+  double gm1 = c->gm1, gamma = c->p.gamma;
+  int k = M->kbeg, k2 = k + 1, k3 = k2 + 1;
+  int j1 = M->jtail1, j2 = M->jtail2;
+  double fmm = c->p.fsmach != 0 ? c->p.fsmach : (c->p.fmtip != 0 ? c->p.fmtip : 1.0);
+  double half = (double)c->p.half;
+  double* zz = (double*)calloc((size_t)M->jd + 2, sizeof(double));
+  double* sxx = (double*)calloc((size_t)M->jd + 2, sizeof(double));
+  double* syy = (double*)calloc((size_t)M->jd + 2, sizeof(double));
+#define XVv(j, k) xv[VX(M, j, k)]
+#define YVv(j, k) yv[VX(M, j, k)]
+  for (int j = j1; j <= j2; ++j) {
+    size_t i = IX(M, j, k), im1 = IX(M, j, k - 1);
+    double rr = 1. / q1[i];
+    double rho = q1[i] * qj[i];
+    double u = q2[i] * rr, v = q3[i] * rr, e = q4[i] * qj[i];
+    double vsq = u * u + v * v;
+    double pp = gm1 * (e - rho * vsq / 2.);
+    double pp1 = pp / c->pinf;
+    double cp = 2. * (pp1 - 1.) / (gamma * sq(fmm));
+    zz[j] = cp;
+    rr = 1. / q1[im1];
+    rho = q1[im1] * qj[im1];
+    u = q2[im1] * rr; v = q3[im1] * rr; e = q4[im1] * qj[im1];
+    vsq = u * u + v * v;
+    pp = gm1 * (e - rho * vsq / 2.);
+    pp1 = pp / c->pinf;
+    cp = 2. * (pp1 - 1.) / (gamma * sq(fmm));
+    zz[j] = 0.5 * (zz[j] + cp);
+  }
+  double cn = 0., cc = 0., cmle = 0., cpinv = 0.;
+  for (int j = j1; j <= j2; ++j) {
+    int jv = j - 1, kv = k - 1;
+    cn = cn - zz[j] * (XVv(jv + 1, kv) - XVv(jv, kv));
+    cc = cc + zz[j] * (YVv(jv + 1, kv) - YVv(jv, kv));
+    cmle = cmle + zz[j] * (0.5 * (XVv(jv + 1, kv) + XVv(jv, kv)) * (XVv(jv + 1, kv) - XVv(jv, kv)) +
+                           0.5 * (YVv(jv + 1, kv) + YVv(jv, kv)) * (YVv(jv + 1, kv) - YVv(jv, kv)));
+    size_t i = IX(M, j, k), im1 = IX(M, j, k - 1);
+    double u = q2[i] / q1[i], v = q3[i] / q1[i];
+    double um = q2[im1] / q1[im1], vm = q3[im1] / q1[im1];
+    cpinv = cpinv + 0.5 * (u + um) * zz[j] * (YVv(jv + 1, kv) - YVv(jv, kv))
+            - 0.5 * (v + vm) * zz[j] * (XVv(jv + 1, kv) - XVv(jv, kv));
+  }
+  cn = cn - half * cn;
+  cmle = cmle - half * cmle;
+  cc = cc + half * cc;
+  double cnv = 0., ccv = 0., cmlev = 0., cpv = 0.;
+  if (!c->p.invisc) {
+    double alngth = 1.;
+    double amu = c->rinf * alngth / c->rey;
+    double uinf2 = sq(fmm);
+#define UQ(j, kk) (q2[IX(M, j, kk)] / q1[IX(M, j, kk)])
+#define VQ(j, kk) (q3[IX(M, j, kk)] / q1[IX(M, j, kk)])
+    for (int j = j1; j <= j2; ++j) {
+      double uxi, vxi;
+      if (j == j1) {
+        uxi = UQ(j + 1, k) - UQ(j, k);
+        vxi = VQ(j + 1, k) - VQ(j, k);
+      } else if (j == j2) {
+        uxi = UQ(j, k) - UQ(j - 1, k);
+        vxi = VQ(j, k) - VQ(j - 1, k);
+      } else {
+        uxi = .5 * (UQ(j + 1, k) - UQ(j - 1, k));
+        vxi = .5 * (VQ(j + 1, k) - VQ(j - 1, k));
+      }
+      double ueta = -1.5 * UQ(j, k) + 2. * UQ(j, k2) - .5 * UQ(j, k3);
+      double veta = -1.5 * VQ(j, k) + 2. * VQ(j, k2) - .5 * VQ(j, k3);
+      size_t i = IX(M, j, k);
+      double xix = xx[i], xiy = xy[i], etax = yx[i], etay = yy[i];
+      double sfdiv = 2. / 3 * (uxi * xix + ueta * etax + vxi * xiy + veta * etay);
+      sxx[j] = amu * (2. * (uxi * xix + ueta * etax) - sfdiv);
+      syy[j] = amu * (2. * (vxi * xiy + veta * etay) - sfdiv);
+      double tauw = amu * ((uxi * xiy + ueta * etay) + (vxi * xix + veta * etax));
+      sxx[j] = sxx[j] / (.5 * c->rinf * uinf2);
+      syy[j] = syy[j] / (.5 * c->rinf * uinf2);
+      zz[j] = tauw / (.5 * c->rinf * uinf2);
+    }
+    for (int j = j1; j <= j2; ++j) {
+      int jv = j - 1, kv = k - 1;
+      double dxv = XVv(jv + 1, kv) - XVv(jv, kv), dyv = YVv(jv + 1, kv) - YVv(jv, kv);
+      ccv = ccv + zz[j] * (XVv(jv + 1, kv) - XVv(jv, kv)) - sxx[j] * (YVv(jv + 1, kv) - YVv(jv, kv));
+      cnv = cnv - zz[j] * (YVv(jv + 1, kv) - YVv(jv, kv)) + syy[j] * (XVv(jv + 1, kv) - XVv(jv, kv));
+      cmlev = cmlev + zz[j] * (0.5 * (XVv(jv + 1, kv) + XVv(jv, kv)) * (YVv(jv + 1, kv) - YVv(jv, kv)) +
+                               0.5 * (YVv(jv + 1, kv) + YVv(jv, kv)) * (XVv(jv + 1, kv) - XVv(jv, kv)))
+              - 0.5 * (XVv(jv + 1, kv) + XVv(jv, kv)) * (XVv(jv + 1, kv) - XVv(jv, kv)) * syy[j]
+              - 0.5 * (YVv(jv + 1, kv) + YVv(jv, kv)) * (YVv(jv + 1, kv) - YVv(jv, kv)) * sxx[j];
+      size_t i = IX(M, j, k), im1 = IX(M, j, k - 1);
+      double u = q2[i] / q1[i], v = q3[i] / q1[i];
+      double um = q2[im1] / q1[im1], vm = q3[im1] / q1[im1];
+      cpv = cpv + 0.5 * (u + um) * zz[j] * dxv - 0.5 * (u + um) * sxx[j] * dyv
+            - 0.5 * (v + vm) * zz[j] * dyv + 0.5 * (v + vm) * syy[j] * dxv;
+    }
+    cnv = cnv - half * cnv;
+    cmlev = cmlev - half * cmlev;
+    ccv = ccv + half * ccv;
+#undef UQ
+#undef VQ
+  }
+#undef XVv
+#undef YVv
+  double cfx = cc + ccv, cfy = cn + cnv, cm0 = cmle + cmlev;
+  double cpower = (cpinv + cpv) / fmm;
+  double ca = cos(c->pi * c->p.alfa / 180.0), sa = sin(c->pi * c->p.alfa / 180.0);
+  double x0 = 0.25, y0 = 0.00;
+  out[0] = -cfx * sa + cfy * ca;
+  out[1] = cfx * ca + cfy * sa;
+  out[2] = cm0 - y0 * cfx + x0 * cfy;
+  out[3] = cpower;
+  out[4] = cfx;
+  out[5] = cfy;
+  free(zz); free(sxx); free(syy);
+  return OT2D_OK;
+}

@@ -1,0 +1,268 @@
+"""ctypes mirror of ``include/ot2d.h`` and a thin host-side case wrapper.
+
+``CaseAPI`` drives any library that exports the ``ot2d.h`` entry points under
+a given prefix.  The product library ``libot2d.so`` uses the prefix
+``ot2d_``; the wrapper is deliberately free of torch types so the same host
+code can feed NumPy arrays straight through the C-ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+MAXBC = 25
+
+OK, MSTOP, NORM = 0, 1, 2
+ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4, -5
+
+(F_X, F_Y, F_XX, F_XY, F_YX, F_YY, F_UG, F_VG, F_TSCALE, F_BT, F_TURMU, F_S,
+ F_XV, F_YV, F_UGV, F_VGV) = range(16)
+
+
+class BCSeg(C.Structure):
+    _fields_ = [("ibtyp", C.c_int32), ("ibdir", C.c_int32), ("jbcs", C.c_int32),
+                ("jbce", C.c_int32), ("kbcs", C.c_int32), ("kbce", C.c_int32)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("jmax", C.c_int32), ("kmax", C.c_int32), ("nbc", C.c_int32),
+                ("irot_dir", C.c_int32), ("xcen", C.c_double), ("ycen", C.c_double),
+                ("bc", BCSeg * MAXBC)]
+
+
+_PARAM_INTS = ["iread", "nhalo", "half", "invisc", "lamin", "iturb", "itrans",
+               "irhsy", "ilim", "ilhs", "iconstant_cfl", "idual", "iprecon",
+               "ntac", "iunst", "itnmax", "timespectral", "nspec", "motiontype",
+               "nsteps", "npnorm", "nrest"]
+_PARAM_DBLS = ["fsmach", "alfa", "rey", "vnuinf", "tuinf", "dt", "timeac", "epse",
+               "dualtime", "Mp", "rf", "angmax", "fmtip", "rartio", "totime",
+               "gamma", "pr", "rmue", "tinf"]
+
+
+class Params(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in _PARAM_INTS]
+                + [(n, C.c_double) for n in _PARAM_DBLS]
+                + [("lu_fp32", C.c_int32), ("reserved", C.c_int32 * 7)])
+
+    def update(self, **kw):
+        for k, v in kw.items():
+            if k in _PARAM_INTS or k == "lu_fp32":
+                setattr(self, k, int(bool(v)) if isinstance(v, bool) else int(v))
+            elif k in _PARAM_DBLS:
+                setattr(self, k, float(v))
+            else:
+                raise KeyError(f"unknown parameter {k}")
+        return self
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n in _PARAM_INTS + _PARAM_DBLS + ["lu_fp32"]}
+
+
+def mesh_desc(jmax: int, kmax: int, segs: Sequence, irot_dir: int = 1,
+              xcen: float = 0.0, ycen: float = 0.0) -> MeshDesc:
+    md = MeshDesc()
+    md.jmax, md.kmax, md.nbc = jmax, kmax, len(segs)
+    md.irot_dir, md.xcen, md.ycen = irot_dir, xcen, ycen
+    for i, s in enumerate(segs):
+        md.bc[i] = BCSeg(s.ibtyp, s.ibdir, s.jbcs, s.jbce, s.kbcs, s.kbce)
+    return md
+
+
+_DP = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_IP = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def bind(lib: C.CDLL, prefix: str):
+    """Declare the ot2d.h signatures on ``lib`` under ``prefix``."""
+    P = C.POINTER
+    sig = {
+        "default_params": (C.c_int32, [P(Params)]),
+        "sizes": (C.c_int32, [P(Params), P(MeshDesc), P(C.c_int32), P(C.c_int32),
+                              P(C.c_int32), P(C.c_int32)]),
+        "case_create": (C.c_int32, [P(Params), C.c_int32, P(MeshDesc), P(C.c_void_p)]),
+        "case_destroy": (C.c_int32, [C.c_void_p]),
+        "set_grid": (C.c_int32, [C.c_void_p, C.c_int32, _DP, _DP]),
+        "set_state": (C.c_int32, [C.c_void_p, C.c_int32, _DP]),
+        "get_state": (C.c_int32, [C.c_void_p, C.c_int32, _DP]),
+        "set_iblank": (C.c_int32, [C.c_void_p, C.c_int32, _IP]),
+        "get_iblank": (C.c_int32, [C.c_void_p, C.c_int32, _IP]),
+        "get_field": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
+        "set_field": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
+        "set_istep0": (C.c_int32, [C.c_void_p, C.c_int32]),
+        "get_istep0": (C.c_int32, [C.c_void_p, P(C.c_int32)]),
+        "time_step": (C.c_int32, [C.c_void_p]),
+        "bc": (C.c_int32, [C.c_void_p]),
+        "compute_residual": (C.c_int32, [C.c_void_p]),
+        "rbc": (C.c_int32, [C.c_void_p]),
+        "lhs": (C.c_int32, [C.c_void_p]),
+        "monitor": (C.c_int32, [C.c_void_p, P(C.c_int32)]),
+        "step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+        "iterate": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_int32)]),
+        "forces": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
+    }
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    return fns
+
+
+class CaseError(RuntimeError):
+    def __init__(self, rc, where, msg=""):
+        super().__init__(f"{where} failed with status {rc}{': ' + msg if msg else ''}")
+        self.rc = rc
+
+
+class CaseAPI:
+    """One case (``nmesh`` meshes x ``nspec`` instances) behind a C-ABI library."""
+
+    def __init__(self, lib: C.CDLL, prefix: str, params: Params, meshes: Sequence[MeshDesc],
+                 errfn=None):
+        self._fns = bind(lib, prefix)
+        self._errfn = errfn
+        self.params = params
+        self.meshes = list(meshes)
+        arr = (MeshDesc * len(meshes))(*meshes)
+        self._arr = arr
+        h = C.c_void_p()
+        self._check(self._fns["case_create"](C.byref(params), len(meshes), arr, C.byref(h)),
+                    "case_create")
+        self.h = h
+        self.dims = []
+        for md in meshes:
+            jd, kd, nq, nv = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+            self._fns["sizes"](C.byref(params), C.byref(md), C.byref(jd), C.byref(kd),
+                               C.byref(nq), C.byref(nv))
+            self.dims.append((jd.value, kd.value, nq.value, nv.value, md.jmax, md.kmax))
+        self.nspec = params.nspec if params.timespectral else 1
+
+    # -------------------------------------------------------------- utils
+    def _check(self, rc, where):
+        if rc < 0:
+            msg = self._errfn() if self._errfn else ""
+            raise CaseError(rc, where, msg)
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self._fns["case_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ state
+    def set_grid(self, im: int, xv: np.ndarray, yv: np.ndarray):
+        xv = np.ascontiguousarray(np.asfortranarray(xv).ravel(order="F"))
+        yv = np.ascontiguousarray(np.asfortranarray(yv).ravel(order="F"))
+        self._check(self._fns["set_grid"](self.h, im, xv, yv), "set_grid")
+
+    def qshape(self, im):
+        jd, kd, nq, nv, _, _ = self.dims[im]
+        return (jd, kd, self.nspec, nq)
+
+    def get_state(self, im: int = 0) -> np.ndarray:
+        jd, kd, nq, nv, _, _ = self.dims[im]
+        buf = np.empty(jd * kd * self.nspec * nq)
+        self._check(self._fns["get_state"](self.h, im, buf), "get_state")
+        return buf.reshape((jd, kd, self.nspec, nq), order="F")
+
+    def set_state(self, im: int, q: np.ndarray):
+        buf = np.ascontiguousarray(np.asarray(q, dtype=np.float64).ravel(order="F"))
+        self._check(self._fns["set_state"](self.h, im, buf), "set_state")
+
+    def get_iblank(self, im: int = 0) -> np.ndarray:
+        jd, kd, *_ = self.dims[im]
+        buf = np.empty(jd * kd * self.nspec, dtype=np.int32)
+        self._check(self._fns["get_iblank"](self.h, im, buf), "get_iblank")
+        return buf.reshape((jd, kd, self.nspec), order="F")
+
+    def set_iblank(self, im: int, ib: np.ndarray):
+        buf = np.ascontiguousarray(np.asarray(ib, dtype=np.int32).ravel(order="F"))
+        self._check(self._fns["set_iblank"](self.h, im, buf), "set_iblank")
+
+    def _field_shape(self, im, field):
+        jd, kd, nq, nv, jmax, kmax = self.dims[im]
+        if field == F_S:
+            return (jd, kd, self.nspec, nv)
+        if field in (F_XV, F_YV, F_UGV, F_VGV):
+            return (jmax, kmax, self.nspec)
+        return (jd, kd, self.nspec)
+
+    def get_field(self, im: int, field: int) -> np.ndarray:
+        shp = self._field_shape(im, field)
+        buf = np.empty(int(np.prod(shp)))
+        self._check(self._fns["get_field"](self.h, im, field, buf), "get_field")
+        return buf.reshape(shp, order="F")
+
+    def set_field(self, im: int, field: int, a: np.ndarray):
+        buf = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+        self._check(self._fns["set_field"](self.h, im, field, buf), "set_field")
+
+    @property
+    def istep0(self) -> int:
+        v = C.c_int32()
+        self._fns["get_istep0"](self.h, C.byref(v))
+        return v.value
+
+    @istep0.setter
+    def istep0(self, v: int):
+        self._fns["set_istep0"](self.h, int(v))
+
+    # ------------------------------------------------------------ stages
+    def time_step(self):
+        return self._check(self._fns["time_step"](self.h), "time_step")
+
+    def bc(self):
+        return self._check(self._fns["bc"](self.h), "bc")
+
+    def compute_residual(self):
+        return self._check(self._fns["compute_residual"](self.h), "compute_residual")
+
+    def rbc(self):
+        return self._check(self._fns["rbc"](self.h), "rbc")
+
+    def lhs(self):
+        return self._check(self._fns["lhs"](self.h), "lhs")
+
+    def monitor(self) -> int:
+        m = C.c_int32()
+        self._check(self._fns["monitor"](self.h, C.byref(m)), "monitor")
+        return m.value
+
+    def step(self):
+        n = len(self.meshes) * self.nspec
+        norms = np.zeros(3 * n)
+        am = np.zeros(3 * n, dtype=np.int32)
+        rc = self._check(self._fns["step"](self.h, norms.ctypes.data, am.ctypes.data), "step")
+        return rc, norms.reshape(n, 3), am.reshape(n, 3)
+
+    def iterate(self, niter: int):
+        n = len(self.meshes) * self.nspec
+        hist = np.zeros(niter * n)
+        done = C.c_int32()
+        rc = self._check(self._fns["iterate"](self.h, int(niter), hist.ctypes.data,
+                                              C.byref(done)), "iterate")
+        return rc, hist.reshape(niter, n)[: done.value], done.value
+
+    def forces(self, im: int = 0, nsp: int = 0):
+        out = np.zeros(6)
+        self._check(self._fns["forces"](self.h, im, nsp, out), "forces")
+        return dict(zip(("cl", "cd", "cm", "cpower", "cfx", "cfy"), out))
+
+
+def default_params(lib: C.CDLL, prefix: str, **overrides) -> Params:
+    p = Params()
+    f = getattr(lib, prefix + "default_params")
+    f.restype = C.c_int32
+    f.argtypes = [C.POINTER(Params)]
+    f(C.byref(p))
+    p.update(**overrides)
+    return p

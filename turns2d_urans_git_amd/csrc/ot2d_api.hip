@@ -1,0 +1,685 @@
+// C-ABI of libot2d (include/ot2d.h): case objects holding device-resident
+// state, stage entry points mirroring the reference subroutines, and the
+// step / iterate drivers that replay one captured HIP graph per step.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ot2d.h"
+#include "ot2d_dev.h"
+
+using namespace ot2d;
+
+static thread_local std::string g_err;
+
+static int32_t fail(int32_t rc, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return rc;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(OT2D_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                              \
+  } while (0)
+
+extern "C" const char* ot2d_last_error(void) { return g_err.c_str(); }
+
+namespace {
+
+struct MeshBuf {
+  ot2d_mesh_desc d;
+  int jd, kd, jmax, kmax;
+  size_t n2, nv2;
+  double *q = nullptr, *s = nullptr;
+  double *x = nullptr, *y = nullptr, *xx = nullptr, *xy = nullptr, *yx = nullptr, *yy = nullptr;
+  double *ug = nullptr, *vg = nullptr, *turmu = nullptr, *tscale = nullptr, *bt = nullptr;
+  double *xv = nullptr, *yv = nullptr, *ugv = nullptr, *vgv = nullptr;
+  int* iblank = nullptr;
+  bool has_grid = false;
+};
+
+}  // namespace
+
+struct ot2d_case {
+  ot2d_params p;
+  DevConst c;
+  int nmesh = 0, nspec = 1, nq = 5, nv = 4, lamin = 1, iturb = -1;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<MeshBuf> m;
+  std::vector<DevMesh> hM;   // nmesh*nspec records (mesh-major)
+  DevMesh* dM = nullptr;
+  std::vector<ResTile> hT;
+  ResTile* dT = nullptr;
+  // bookkeeping (device)
+  int* d_status = nullptr;
+  int* d_istep0 = nullptr;
+  int* d_stepc = nullptr;
+  double* d_partials = nullptr;
+  unsigned int* d_counter = nullptr;
+  double* d_norms = nullptr;
+  double* d_tspec = nullptr;
+  double* d_Ds = nullptr;
+  double* d_hist = nullptr;
+  int hist_cap = 0;
+  double* d_forces = nullptr;
+  int istep0 = 0;
+  // captured step
+  hipGraphExec_t graph = nullptr;
+  int graph_valid = 0;
+};
+
+// ------------------------------------------------------------- constants
+extern "C" int32_t ot2d_default_params(ot2d_params* p) {
+  if (!p) return fail(OT2D_ERR_ARG, "null params");
+  memset(p, 0, sizeof(*p));
+  // data statements, src/modules.f:170-193
+  p->gamma = 1.4; p->pr = 0.72; p->rmue = 1.0; p->tinf = 400.0;
+  p->iread = 0; p->nhalo = 1; p->half = 0;
+  p->nsteps = 500; p->nrest = 1000; p->npnorm = 25;
+  p->timespectral = 0; p->nspec = 1;
+  p->fsmach = 0.16; p->fmtip = 0.0; p->alfa = 0.0; p->rartio = 1.0; p->rey = 3900000.0;
+  p->invisc = 1; p->lamin = 1; p->iturb = 1; p->itrans = 0;
+  p->vnuinf = 0.1; p->tuinf = 0.2;
+  p->iunst = 0; p->ntac = 1; p->itnmax = 1; p->dt = 0.1; p->timeac = 1.0; p->motiontype = 1;
+  p->epse = 0.01; p->irhsy = -3; p->ilhs = 1; p->ilim = 0; p->totime = 0.0; p->rf = 0.0;
+  p->angmax = 0.0;
+  p->idual = 0; p->dualtime = 1.0; p->iconstant_cfl = 0;
+  p->iprecon = 0; p->Mp = 1.0;
+  return OT2D_OK;
+}
+
+static void derive_counts(const ot2d_params* p, int* nq, int* nv, int* lamin, int* iturb) {
+  int lam = p->lamin, it = p->iturb;
+  if (p->invisc) lam = 1;  // src/setup.f:963-967
+  if (lam) it = -1;
+  int q = 5, v = 4;
+  if (it == 1) { q += 1; v += 1; }
+  else if (it == 2) { q += 2; v += 2; }
+  if ((it == 1 || it == 2) && p->itrans == 1) { q += 2; v += 2; }
+  *nq = q; *nv = v; *lamin = lam; *iturb = it;
+}
+
+extern "C" int32_t ot2d_sizes(const ot2d_params* p, const ot2d_mesh_desc* m, int32_t* jd, int32_t* kd,
+                              int32_t* nq, int32_t* nv) {
+  if (!p || !m) return fail(OT2D_ERR_ARG, "null argument");
+  int nadd = 2 * p->nhalo - 1;  // src/setup.f:1263
+  int q, v, lam, it;
+  derive_counts(p, &q, &v, &lam, &it);
+  if (jd) *jd = m->jmax + nadd;
+  if (kd) *kd = m->kmax + nadd;
+  if (nq) *nq = q;
+  if (nv) *nv = v;
+  return OT2D_OK;
+}
+
+// read_inputs' derived quantities, src/setup.f:1162-1248
+static DevConst make_const(const ot2d_params* p, int iturb, int nmv) {
+  DevConst c;
+  memset(&c, 0, sizeof c);
+  double pi = 4.0 * atan(1.0);
+  c.gamma = p->gamma;
+  c.gm1 = p->gamma - 1.0;
+  c.ggm1 = p->gamma * c.gm1;
+  double cs = cos(pi * p->alfa / 180.0);
+  double ss = sin(pi * p->alfa / 180.0);
+  c.einf = 1.0 / c.ggm1 + 0.5 * (p->fsmach * p->fsmach);
+  c.pinf = 1.0 / p->gamma;
+  c.rinf = 1.0;
+  c.uinf = p->fsmach * cs;
+  c.vinf = p->fsmach * ss;
+  c.rey = (p->fsmach != 0.0) ? p->rey / p->fsmach : p->rey / p->fmtip;
+  c.rmue = p->invisc ? 0.0 : p->rmue;
+  c.h = p->dt;
+  int irhs = abs(p->irhsy);
+  c.th = (double)(irhs - 2) / (double)irhs;
+  c.qt = (irhs == 1) ? 0.0 : 0.25;
+  c.epse = p->epse;
+  c.Mp = p->iprecon ? p->Mp : 1.0;
+  c.dualtime = p->dualtime;
+  c.timeac = p->timeac;
+  c.jac_factor = p->invisc ? 0.01 : 0.002;
+  c.pr = p->pr;
+  c.tinf = p->tinf;
+  c.vnuinf = p->vnuinf;
+  c.angmax = p->angmax;
+  c.fmm = p->fsmach != 0 ? p->fsmach : (p->fmtip != 0 ? p->fmtip : 1.0);
+  c.half = (double)p->half;
+  c.ca = cs;  // compute_forces uses the same cos/sin of alfa (src/forces.f:29-30)
+  c.sa = ss;
+  c.invisc = p->invisc;
+  c.iread = p->iread;
+  c.ilim = p->ilim;
+  c.iconstant_cfl = p->iconstant_cfl;
+  c.idual = p->idual;
+  c.iprecon = p->iprecon;
+  c.timespectral = p->timespectral;
+  c.ntac = p->ntac;
+  c.iturb = iturb;
+  c.nmv = nmv;
+  c.motiontype = p->motiontype;
+  return c;
+}
+
+// src/bc.f:21-35
+static DevBC fix_bc(const ot2d_bcseg& b, int jmax, int kmax, int jd, int kd) {
+  int js = b.jbcs, je = b.jbce, ks = b.kbcs, ke = b.kbce;
+  if (js < 0) js = jmax + js + 1;
+  if (ks < 0) ks = kmax + ks + 1;
+  if (je < 0) je = jmax + je + 1;
+  if (ke < 0) ke = kmax + ke + 1;
+  if (js > 1) js = js + 1;
+  if (ks > 1) ks = ks + 1;
+  if (je < jmax) je = je + 1 - 1;
+  if (ke < kmax) ke = ke + 1 - 1;
+  if (je == jmax) je = jd;
+  if (ke == kmax) ke = kd;
+  DevBC d;
+  d.type = b.ibtyp; d.idir = b.ibdir; d.js = js; d.je = je; d.ks = ks; d.ke = ke;
+  return d;
+}
+
+// Every cell a boundary segment touches must lie inside the (jd,kd) array:
+// the kernels trust these ranges.
+static bool bc_in_range(const DevBC& b, int jd, int kd) {
+  auto in = [&](int j, int k) { return j >= 1 && j <= jd && k >= 1 && k <= kd; };
+  if (!(b.js >= 1 && b.js <= b.je && b.je <= jd && b.ks >= 1 && b.ks <= b.ke && b.ke <= kd)) return false;
+  int iadir = b.idir >= 0 ? b.idir : -b.idir;
+  int iadd = b.idir >= 0 ? 1 : -1;
+  switch (b.type) {
+    case 3: case 4: case 5: case 10: case 47:
+      if (iadir == 2) {
+        int k = b.idir == 2 ? b.ke : b.ks;
+        return in(b.js, k + 2 * iadd) && in(b.je, k + 2 * iadd) && b.ks == b.ke;
+      }
+      if (iadir == 1) {
+        if (b.type >= 3 && b.type <= 5) return true;  // not implemented for j walls
+        int j = b.idir == 1 ? b.je : b.js;
+        return in(j + 2 * iadd, b.ks) && in(j + 2 * iadd, b.ke) && b.js == b.je;
+      }
+      return false;
+    case 11:
+      if (iadir == 2) return in(b.js, b.idir == 2 ? b.ke + 1 : b.ks - 1) && b.ks == b.ke;
+      if (iadir == 1) return in(b.idir == 1 ? b.je + 1 : b.js - 1, b.ks) && b.js == b.je;
+      return false;
+    case 22: {
+      int jj = b.je - b.js + 1, kk = b.ke - b.ks + 1;
+      if (b.idir == 1) return jd - 2 * jj + 1 >= 1;
+      if (b.idir == -1) return 1 + 2 * jj <= jd;
+      if (b.idir == 2) return kd - 2 * kk + 1 >= 1;
+      if (b.idir == -2) return 1 + 2 * kk <= kd;
+      return false;
+    }
+    case 51:
+      if (iadir != 2 || b.ks != b.ke) return iadir == 1;
+      return in(b.js, b.idir == 2 ? b.ke + 1 : b.ks - 1);
+    case 46:
+      return true;
+    default:
+      return true;  // unknown types are skipped by every kernel
+  }
+}
+
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+static void free_case(ot2d_case* c) {
+  if (!c) return;
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  for (auto& mb : c->m) {
+    dfree(mb.q); dfree(mb.s); dfree(mb.x); dfree(mb.y); dfree(mb.xx); dfree(mb.xy);
+    dfree(mb.yx); dfree(mb.yy); dfree(mb.ug); dfree(mb.vg); dfree(mb.turmu);
+    dfree(mb.tscale); dfree(mb.bt); dfree(mb.xv); dfree(mb.yv); dfree(mb.ugv);
+    dfree(mb.vgv); dfree(mb.iblank);
+  }
+  dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
+  dfree(c->d_partials); dfree(c->d_counter); dfree(c->d_norms); dfree(c->d_tspec);
+  dfree(c->d_Ds); dfree(c->d_hist); dfree(c->d_forces);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+  hipError_t e = hipMalloc((void**)p, (n ? n : 1) * sizeof(T));
+  if (e == hipSuccess) e = hipMemset(*p, 0, (n ? n : 1) * sizeof(T));
+  return e;
+}
+
+extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const ot2d_mesh_desc* md,
+                                    ot2d_case** out) {
+  if (!p || !md || !out || nmesh < 1) return fail(OT2D_ERR_ARG, "bad arguments to ot2d_case_create");
+  *out = nullptr;
+  if (p->nhalo != 1) return fail(OT2D_ERR_UNSUPPORTED, "nhalo=%d: only one halo ring is supported", p->nhalo);
+  if (p->ilhs != 1) return fail(OT2D_ERR_UNSUPPORTED, "ilhs=%d: only LU-SGS (ilhs=1) is supported", p->ilhs);
+  if (p->iprecon) return fail(OT2D_ERR_UNSUPPORTED, "low-Mach preconditioning is not supported");
+  if (!(abs(p->ilim) == 1)) return fail(OT2D_ERR_UNSUPPORTED, "ilim=%d not supported (1: Koren MUSCL)", p->ilim);
+  if (!p->timespectral && p->iunst != 0)
+    return fail(OT2D_ERR_UNSUPPORTED, "iunst=%d: time-accurate motion is not supported", p->iunst);
+  if (p->itnmax != 1) return fail(OT2D_ERR_UNSUPPORTED, "itnmax=%d: Newton sub-iterations not supported", p->itnmax);
+  if (p->ntac != 1) return fail(OT2D_ERR_UNSUPPORTED, "ntac=%d not supported", p->ntac);
+  if (p->timespectral && (p->nspec < 1 || p->nspec > 64)) return fail(OT2D_ERR_ARG, "nspec=%d", p->nspec);
+  ot2d_case* c = new ot2d_case();
+  c->p = *p;
+  derive_counts(p, &c->nq, &c->nv, &c->lamin, &c->iturb);
+  if (!p->invisc) { free_case(c); return fail(OT2D_ERR_UNSUPPORTED, "viscous flow not supported yet"); }
+  c->nspec = p->timespectral ? p->nspec : 1;
+  c->nmesh = nmesh;
+  c->c = make_const(p, c->iturb, 4);
+  (void)hipGetDevice(&c->device);
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { free_case(c); return fail(OT2D_ERR_HIP, "stream: %s", hipGetErrorString(e)); }
+  int ns = c->nspec;
+  c->m.resize(nmesh);
+  int nmi = nmesh * ns;
+  c->hM.resize(nmi);
+  // per-case bookkeeping
+#define A(ptr, n) do { if ((e = dalloc(&(ptr), (n))) != hipSuccess) { free_case(c); \
+    return fail(OT2D_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e)); } } while (0)
+  A(c->d_status, 1); A(c->d_istep0, 1); A(c->d_stepc, 1);
+  A(c->d_counter, nmi); A(c->d_norms, 6 * nmi); A(c->d_tspec, ns); A(c->d_Ds, ns * ns);
+  A(c->d_forces, 16);
+  // spectral coefficients, src/setup.f:838-873
+  {
+    std::vector<double> Ds(ns * ns, 0.0), tspec(ns, 0.0);
+    double pi = 4. * atan(1.);
+    for (int i = 1; i <= ns; ++i)
+      for (int j = 1; j <= ns; ++j) {
+        double v = 0.0;
+        if (i != j) {
+          double sgn = ((i - j) % 2 == 0) ? 1.0 : -1.0;
+          v = (ns % 2 == 0) ? 0.5 * sgn / tan(pi * (i - j) / ns) : 0.5 * sgn / sin(pi * (i - j) / ns);
+        }
+        Ds[(i - 1) + (size_t)(j - 1) * ns] = v;
+      }
+    for (int n = 0; n < ns; ++n) tspec[n] = (p->rf != 0.0) ? 2 * pi * n / ns / fabs(p->rf) : 0.0;  // src/init.f:71-78
+    (void)hipMemcpy(c->d_Ds, Ds.data(), sizeof(double) * ns * ns, hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->d_tspec, tspec.data(), sizeof(double) * ns, hipMemcpyHostToDevice);
+  }
+  int ntile_total = 0;
+  for (int im = 0; im < nmesh; ++im) {
+    MeshBuf& mb = c->m[im];
+    mb.d = md[im];
+    if (md[im].jmax < 5 || md[im].kmax < 5) { free_case(c); return fail(OT2D_ERR_ARG, "mesh %d too small", im); }
+    if (md[im].nbc < 0 || md[im].nbc > OT2D_MAXBC) { free_case(c); return fail(OT2D_ERR_ARG, "nbc"); }
+    if (md[im].kmax - 1 > 1024) { free_case(c); return fail(OT2D_ERR_UNSUPPORTED, "kmax > 1025"); }
+    mb.jmax = md[im].jmax; mb.kmax = md[im].kmax;
+    mb.jd = mb.jmax + 1; mb.kd = mb.kmax + 1;
+    mb.n2 = (size_t)mb.jd * mb.kd;
+    mb.nv2 = (size_t)mb.jmax * mb.kmax;
+    A(mb.q, mb.n2 * ns * c->nq); A(mb.s, mb.n2 * ns * c->nv);
+    A(mb.x, mb.n2 * ns); A(mb.y, mb.n2 * ns); A(mb.xx, mb.n2 * ns); A(mb.xy, mb.n2 * ns);
+    A(mb.yx, mb.n2 * ns); A(mb.yy, mb.n2 * ns); A(mb.ug, mb.n2 * ns); A(mb.vg, mb.n2 * ns);
+    A(mb.turmu, mb.n2 * ns); A(mb.tscale, mb.n2 * ns); A(mb.bt, mb.n2 * ns);
+    A(mb.xv, mb.nv2 * ns); A(mb.yv, mb.nv2 * ns); A(mb.ugv, mb.nv2 * ns); A(mb.vgv, mb.nv2 * ns);
+    A(mb.iblank, mb.n2 * ns);
+    int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
+    ntile_total += tj * tk * ns;
+  }
+  A(c->d_partials, 4 * (size_t)ntile_total);
+  c->hT.reserve(ntile_total);
+  int pofs = 0;
+  for (int im = 0; im < nmesh; ++im) {
+    MeshBuf& mb = c->m[im];
+    int jtail_v = 1;
+    for (int ib = 0; ib < md[im].nbc; ++ib)
+      if (md[im].bc[ib].ibtyp == 5) jtail_v = md[im].bc[ib].jbcs;  // src/init.f:59-66
+    for (int nsp = 0; nsp < ns; ++nsp) {
+      int mi = im * ns + nsp;
+      DevMesh& M = c->hM[mi];
+      memset(&M, 0, sizeof M);
+      M.jd = mb.jd; M.kd = mb.kd; M.jmax = mb.jmax; M.kmax = mb.kmax;
+      M.js = 2; M.je = mb.jd - 1; M.ks = 2; M.ke = mb.kd - 1;
+      M.nq = c->nq; M.nv = c->nv; M.nspec = ns; M.nsp = nsp;
+      M.jtail1 = jtail_v + 1;
+      M.jtail2 = p->half == 0 ? mb.jd - M.jtail1 + 1 : mb.jd - 1;
+      M.has_gv = p->timespectral ? 1 : 0;
+      M.epsj = pow(10. / (double)mb.jmax, 3);
+      M.epsk = pow(10. / (double)mb.kmax, 3);
+      {
+        double a = 10. / (double)mb.jmax, b = 10. / (double)mb.kmax;
+        M.epsj = a * a * a;  // (10./real(jmax))**3, src/eulerrhs.f:51
+        M.epsk = b * b * b;
+      }
+      M.nbc = md[im].nbc;
+      for (int ib = 0; ib < md[im].nbc; ++ib) {
+        M.bc[ib] = fix_bc(md[im].bc[ib], mb.jmax, mb.kmax, mb.jd, mb.kd);
+        if (!bc_in_range(M.bc[ib], mb.jd, mb.kd)) {
+          free_case(c);
+          return fail(OT2D_ERR_ARG, "mesh %d boundary segment %d (type %d, dir %d) indexes outside the grid",
+                      im, ib + 1, md[im].bc[ib].ibtyp, md[im].bc[ib].ibdir);
+        }
+      }
+      size_t o = mb.n2 * nsp, ov = mb.nv2 * nsp;
+      for (int n = 0; n < c->nq; ++n) M.q[n] = mb.q + mb.n2 * ((size_t)n * ns + nsp);
+      for (int n = 0; n < c->nv; ++n) M.s[n] = mb.s + mb.n2 * ((size_t)n * ns + nsp);
+      for (int n = 0; n < c->nq; ++n) M.qall[n] = mb.q + mb.n2 * ((size_t)n * ns);
+      M.inst_stride = (long long)mb.n2;
+      M.x = mb.x + o; M.y = mb.y + o; M.xx = mb.xx + o; M.xy = mb.xy + o; M.yx = mb.yx + o; M.yy = mb.yy + o;
+      M.ug = mb.ug + o; M.vg = mb.vg + o; M.turmu = mb.turmu + o; M.tscale = mb.tscale + o; M.bt = mb.bt + o;
+      M.xv = mb.xv + ov; M.yv = mb.yv + ov; M.ugv = mb.ugv + ov; M.vgv = mb.vgv + ov;
+      M.xv0 = mb.xv; M.yv0 = mb.yv;
+      M.iblank = mb.iblank + o;
+      M.status = c->d_status;
+      M.istep0 = c->d_istep0;
+      M.counter = c->d_counter + mi;
+      M.norms = c->d_norms + 6 * mi;
+      M.tspec = c->d_tspec + nsp;
+      M.Ds = c->d_Ds;
+      M.c = c->c;
+      M.c.rotf = (p->rf != 0.0) ? md[im].irot_dir * p->rf : 0.0;  // src/setup.f:96-101
+      M.c.xac = md[im].xcen; M.c.yac = md[im].ycen;
+      int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
+      M.res_block0 = pofs;
+      M.res_nblocks = tj * tk;
+      M.partials = c->d_partials + 4 * (size_t)pofs;
+      int b = 0;
+      for (int kt = 0; kt < tk; ++kt)
+        for (int jt = 0; jt < tj; ++jt) {
+          ResTile T;
+          T.mesh = mi; T.j0 = 2 + jt * RES_TJ; T.k0 = 2 + kt * RES_TK; T.blk = b++;
+          c->hT.push_back(T);
+        }
+      pofs += tj * tk;
+    }
+  }
+#undef A
+  if ((e = dalloc(&c->dM, c->hM.size())) != hipSuccess || (e = dalloc(&c->dT, c->hT.size())) != hipSuccess) {
+    free_case(c);
+    return fail(OT2D_ERR_NOMEM, "device allocation failed");
+  }
+  (void)hipMemcpy(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice);
+  (void)hipMemcpy(c->dT, c->hT.data(), sizeof(ResTile) * c->hT.size(), hipMemcpyHostToDevice);
+  *out = c;
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_case_destroy(ot2d_case* c) {
+  if (!c) return OT2D_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_case(c);
+  return OT2D_OK;
+}
+
+static int32_t check_im(ot2d_case* c, int32_t im) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (im < 0 || im >= c->nmesh) return fail(OT2D_ERR_ARG, "mesh index %d out of range", im);
+  return OT2D_OK;
+}
+
+static int32_t sync(ot2d_case* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, const double* yv) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  if (!xv || !yv) return fail(OT2D_ERR_ARG, "null grid");
+  MeshBuf& mb = c->m[im];
+  HIPCHK(hipMemcpyAsync(mb.xv, xv, sizeof(double) * mb.nv2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(mb.yv, yv, sizeof(double) * mb.nv2, hipMemcpyHostToDevice, c->stream));
+  // instance 0 first (the other instances rotate its vertices)
+  HIPCHK(launch_init_geometry(c->dM + (size_t)im * c->nspec, 1, c->stream));
+  if (c->nspec > 1) HIPCHK(launch_init_geometry(c->dM + (size_t)im * c->nspec + 1, c->nspec - 1, c->stream));
+  mb.has_grid = true;
+  c->istep0 = 0;
+  HIPCHK(hipMemsetAsync(c->d_istep0, 0, sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_set_state(ot2d_case* c, int32_t im, const double* q) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  MeshBuf& mb = c->m[im];
+  HIPCHK(hipMemcpyAsync(mb.q, q, sizeof(double) * mb.n2 * c->nspec * c->nq, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_get_state(ot2d_case* c, int32_t im, double* q) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  MeshBuf& mb = c->m[im];
+  HIPCHK(hipMemcpyAsync(q, mb.q, sizeof(double) * mb.n2 * c->nspec * c->nq, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_set_iblank(ot2d_case* c, int32_t im, const int32_t* ib) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  MeshBuf& mb = c->m[im];
+  HIPCHK(hipMemcpyAsync(mb.iblank, ib, sizeof(int) * mb.n2 * c->nspec, hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_get_iblank(ot2d_case* c, int32_t im, int32_t* ib) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  MeshBuf& mb = c->m[im];
+  HIPCHK(hipMemcpyAsync(ib, mb.iblank, sizeof(int) * mb.n2 * c->nspec, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+static double* field_ptr(ot2d_case* c, MeshBuf& mb, int field, size_t* n) {
+  size_t ns = (size_t)c->nspec;
+  *n = mb.n2 * ns;
+  switch (field) {
+    case OT2D_F_X: return mb.x;
+    case OT2D_F_Y: return mb.y;
+    case OT2D_F_XX: return mb.xx;
+    case OT2D_F_XY: return mb.xy;
+    case OT2D_F_YX: return mb.yx;
+    case OT2D_F_YY: return mb.yy;
+    case OT2D_F_UG: return mb.ug;
+    case OT2D_F_VG: return mb.vg;
+    case OT2D_F_TSCALE: return mb.tscale;
+    case OT2D_F_BT: return mb.bt;
+    case OT2D_F_TURMU: return mb.turmu;
+    case OT2D_F_S: *n = mb.n2 * ns * c->nv; return mb.s;
+    case OT2D_F_XV: *n = mb.nv2 * ns; return mb.xv;
+    case OT2D_F_YV: *n = mb.nv2 * ns; return mb.yv;
+    case OT2D_F_UGV: *n = mb.nv2 * ns; return mb.ugv;
+    case OT2D_F_VGV: *n = mb.nv2 * ns; return mb.vgv;
+  }
+  return nullptr;
+}
+
+extern "C" int32_t ot2d_get_field(ot2d_case* c, int32_t im, int32_t field, double* out) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  size_t n;
+  double* p = field_ptr(c, c->m[im], field, &n);
+  if (!p || !out) return fail(OT2D_ERR_ARG, "bad field %d", field);
+  HIPCHK(hipMemcpyAsync(out, p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_set_field(ot2d_case* c, int32_t im, int32_t field, const double* in) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  size_t n;
+  double* p = field_ptr(c, c->m[im], field, &n);
+  if (!p || !in) return fail(OT2D_ERR_ARG, "bad field %d", field);
+  HIPCHK(hipMemcpyAsync(p, in, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_set_istep0(ot2d_case* c, int32_t istep0) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  c->istep0 = istep0;
+  HIPCHK(hipMemcpyAsync(c->d_istep0, &c->istep0, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_get_istep0(ot2d_case* c, int32_t* istep0) {
+  if (!c || !istep0) return fail(OT2D_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(&c->istep0, c->d_istep0, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  *istep0 = c->istep0;
+  return OT2D_OK;
+}
+
+static int nmi(ot2d_case* c) { return c->nmesh * c->nspec; }
+
+static int32_t clear_status(ot2d_case* c) {
+  HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_time_step(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_bc(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(launch_bc(c->dM, nmi(c), 0, c->stream));
+  return sync(c);
+}
+
+static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms) {
+  HIPCHK(launch_bc(c->dM, nmi(c), zero_halo_s, c->stream));
+  HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms, c->stream));
+  if (c->p.timespectral) HIPCHK(launch_tsrhs(c->dM, c->hM.data(), nmi(c), c->stream));
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_compute_residual(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (int32_t rc = clear_status(c)) return rc;
+  if (int32_t rc = enqueue_residual(c, 0, 0)) return rc;
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_rbc(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_lhs(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(launch_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
+  int st = 0;
+  HIPCHK(hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  if (mstop) *mstop = (st == 1);
+  return clear_status(c) ? OT2D_ERR_HIP : sync(c);
+}
+
+// one step (src/step.f:2-218) + monitor, enqueued without synchronisation
+static int32_t enqueue_step(ot2d_case* c) {
+  HIPCHK(launch_step_begin(c->d_istep0, c->d_status, 1, c->stream));
+  HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream));
+  if (int32_t rc = enqueue_residual(c, 1, 1)) return rc;
+  HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));
+  HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, c->stream));
+  return OT2D_OK;
+}
+
+static int32_t read_status(ot2d_case* c, int* st) {
+  HIPCHK(hipMemcpyAsync(st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  for (auto& mb : c->m)
+    if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  if (int32_t rc = clear_status(c)) return rc;
+  if (int32_t rc = enqueue_step(c)) return rc;
+  int st = 0;
+  if (int32_t rc = read_status(c, &st)) return rc;
+  std::vector<double> nb(6 * nmi(c));
+  HIPCHK(hipMemcpy(nb.data(), c->d_norms, sizeof(double) * nb.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nmi(c); ++i) {
+    if (norms) { norms[3 * i] = nb[6 * i]; norms[3 * i + 1] = nb[6 * i + 1]; norms[3 * i + 2] = nb[6 * i + 2]; }
+    if (argmax) {
+      argmax[3 * i] = (int)nb[6 * i + 3]; argmax[3 * i + 1] = (int)nb[6 * i + 4]; argmax[3 * i + 2] = (int)nb[6 * i + 5];
+    }
+  }
+  c->istep0 += 1;
+  if (st == 2) return OT2D_NORM;
+  if (st == 1) return OT2D_MSTOP;
+  return OT2D_OK;
+}
+
+// capture one step into a graph (kernels only: no memset / memcpy nodes)
+static int32_t ensure_graph(ot2d_case* c) {
+  if (c->graph_valid) return OT2D_OK;
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int32_t rc = enqueue_step(c);
+  hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+  if (rc) return rc;
+  if (e2 != hipSuccess) return fail(OT2D_ERR_HIP, "capture: %s", hipGetErrorString(e2));
+  hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) return fail(OT2D_ERR_HIP, "instantiate: %s", hipGetErrorString(e3));
+  c->graph_valid = 1;
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32_t* done) {
+  if (!c || niter < 0) return fail(OT2D_ERR_ARG, "bad arguments");
+  for (auto& mb : c->m)
+    if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  if (int32_t rc = clear_status(c)) return rc;
+  if (int32_t rc = ensure_graph(c)) return rc;
+  int per = nmi(c);
+  int st = 0, it = 0;
+  std::vector<double> nb(6 * per);
+  // replay in chunks; the residual history needs the norms of every step,
+  // so copy them out after each replay (device-to-host, async, pinned-free)
+  for (it = 0; it < niter; ++it) {
+    HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    if (hist) {
+      HIPCHK(hipMemcpyAsync(nb.data(), c->d_norms, sizeof(double) * nb.size(), hipMemcpyDeviceToHost, c->stream));
+      if (int32_t rc = read_status(c, &st)) return rc;
+      for (int o = 0; o < per; ++o) hist[(size_t)it * per + o] = nb[6 * o];
+      if (st) { ++it; break; }
+    }
+  }
+  if (!hist) {
+    if (int32_t rc = read_status(c, &st)) return rc;
+  }
+  int i0 = 0;
+  HIPCHK(hipMemcpy(&i0, c->d_istep0, sizeof(int), hipMemcpyDeviceToHost));
+  if (!hist) it = i0 - c->istep0;
+  c->istep0 = i0;
+  if (done) *done = it;
+  if (st == 2) return OT2D_NORM;
+  if (st == 1) return OT2D_MSTOP;
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]) {
+  if (int32_t rc = check_im(c, im)) return rc;
+  if (nsp < 0 || nsp >= c->nspec || !out) return fail(OT2D_ERR_ARG, "bad instance");
+  HIPCHK(launch_forces(c->dM, im * c->nspec + nsp, c->d_forces, c->stream));
+  double buf[16];
+  HIPCHK(hipMemcpyAsync(buf, c->d_forces, sizeof buf, hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  for (int i = 0; i < 6; ++i) out[i] = buf[i];
+  return OT2D_OK;
+}

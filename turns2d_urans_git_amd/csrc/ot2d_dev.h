@@ -1,0 +1,101 @@
+// Device-side data structures shared by the gfx950 kernels and the C-ABI
+// host code of libot2d.
+//
+// State layout in HBM is the reference's own (src/step.f:17-29): per mesh,
+// Fortran column-major cell arrays (jd,kd) with j fastest, one halo ring,
+// variables of q and s as separate planes q(jd,kd,nspec,nq).  A DevMesh
+// record describes one (mesh, time instance) pair: base pointers already
+// offset to the instance, so every kernel works on flat j-fastest planes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OT2D_MAXQ 8
+#define OT2D_MAXBC_DEV 25
+
+namespace ot2d {
+
+// One boundary segment with the reference's index fix-ups already applied
+// (src/bc.f:21-35): js..je, ks..ke are 1-based cell indices incl. halo.
+struct DevBC {
+  int type, idir, js, je, ks, ke;
+};
+
+// Case constants derived as read_inputs does (src/setup.f:1162-1248) plus
+// the scheme switches the kernels branch on.
+struct DevConst {
+  double gamma, gm1, ggm1, einf, pinf, rinf, uinf, vinf, rey, rmue, h;
+  double th, qt, epse, Mp, dualtime, timeac, jac_factor, pr, tinf;
+  double vnuinf, rotf;
+  double angmax, xac, yac;   // time-spectral pitching (src/setup.f:348-393)
+  double fmm, half, ca, sa;  // force normalisation (src/forces.f:27-31, 166-172)
+  int invisc, iread, ilim, iconstant_cfl, idual, iprecon, timespectral, ntac;
+  int iturb, nmv, motiontype;
+};
+
+struct DevMesh {
+  int jd, kd, jmax, kmax;
+  int js, je, ks, ke;        // jbeg, jend, kbeg, kend (1-based)
+  int nq, nv, nbc, nspec, nsp;
+  int jtail1, jtail2;
+  int has_gv;                // grid velocities present (time-spectral)
+  int res_block0, res_nblocks;  // this instance's slice of the norm partials
+  double epsj, epsk;         // Koren limiter epsilons (src/eulerrhs.f:51-52)
+  DevBC bc[OT2D_MAXBC_DEV];
+  double* q[OT2D_MAXQ];      // q(:,:,nsp,n) planes, n = 1..nq  (index n-1)
+  double* s[OT2D_MAXQ];      // s(:,:,nsp,n) planes
+  double *x, *y, *xx, *xy, *yx, *yy, *ug, *vg, *turmu, *tscale, *bt;
+  double *xv, *yv, *ugv, *vgv;  // vertex arrays (jmax,kmax) of this instance
+  const double *xv0, *yv0;      // instance-1 vertices (time-spectral rotation)
+  int* iblank;
+  double* sdt;               // (jd,kd) scratch: ilu2d forward-sweep result
+  // per-case bookkeeping (shared by all instances of a case)
+  int* status;               // 0 ok, 1 mstop, 2 norm
+  int* istep0;               // device copy of istep0
+  double* partials;          // norm partials (res_nblocks * 4 doubles)
+  unsigned int* counter;     // last-block counter for the norm reduction
+  double* norms;             // rsum, resrho, resmax, jmax, kmax, nmax
+  double* tspec;             // time instant of this instance
+  const double* Ds;          // spectral derivative matrix (nspec x nspec)
+  double* qall[OT2D_MAXQ];   // q planes of instance 0 (TS coupling base)
+  long long inst_stride;     // elements between instances within a plane set
+  DevConst c;
+};
+
+// Residual work item: one TJ x TK tile of interior cells of one instance.
+struct ResTile {
+  int mesh;   // index into the DevMesh array
+  int j0, k0; // first interior cell (1-based)
+  int blk;    // tile index within the instance (norm partial slot)
+};
+
+constexpr int RES_TJ = 32;
+constexpr int RES_TK = 8;
+constexpr int RES_THREADS = RES_TJ * RES_TK;
+
+__host__ __device__ inline long long cell(const DevMesh& M, int j, int k) {
+  return (long long)(j - 1) + (long long)(k - 1) * M.jd;
+}
+__host__ __device__ inline long long vert(const DevMesh& M, int j, int k) {
+  return (long long)(j - 1) + (long long)(k - 1) * M.jmax;
+}
+
+}  // namespace ot2d
+
+// launch wrappers (ot2d_kernels.hip)
+namespace ot2d {
+hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st);
+hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
+hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms,
+                           hipStream_t st);
+hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st);
+hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32,
+                     hipStream_t st);
+hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_step_begin(int* istep0, int* status, int ncase, hipStream_t st);
+hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
+hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+}  // namespace ot2d

@@ -1,0 +1,1514 @@
+// gfx950 kernels of the OVERTURNS-2D iteration (FP64 plain path).
+//
+// Kernel map (reference routine -> kernel):
+//   time_step        src/time.f:2-107          k_time_step      (all cells)
+//   bc               src/bc.f:2-85 (+ bc*)      k_bc             (1 block / instance)
+//   rhsup+roeflx+    src/eulerrhs.f:2-446,       k_residual       (32x8 tiles, LDS)
+//   muscld_new       821-912; norms step.f:53-115
+//   timespectralrhs  src/sources.f:2-27          k_tsrhs
+//   rbc              src/bc.f:88-177             k_rbc            (1 block / instance)
+//   s*tscale, ilu2d  src/step.f:141-159,         k_lu_fwd/k_lu_bwd (1 block / instance,
+//                    src/lhs.f:470-814            one thread per k row, hyperplane sweeps)
+//   q update         src/step.f:205-211          fused in k_lu_bwd or k_update
+//   monitor          src/step.f:332-392          fused / k_monitor
+//   initia geometry  src/init.f:2-177, setup.f   k_init_geometry
+//   force2d          src/forces.f:111-398        k_forces
+//
+// Arithmetic follows the reference expressions operand for operand; the
+// library is built with -ffp-contract=off so the FP64 path reproduces the
+// reference's rounding except where reductions are reordered (norms,
+// forces).
+#include "ot2d_dev.h"
+
+namespace ot2d {
+
+__device__ __forceinline__ double sq(double a) { return a * a; }
+__device__ __forceinline__ double cube(double a) { return a * a * a; }
+__device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+// =================================================================== time_step
+// src/time.f:2-107 (one value per cell, all cells incl. halo)
+__device__ inline void time_step_cell(const DevMesh& M, long long i) {
+  const DevConst& C = M.c;
+  double oat = 1.0;  // ntac = 1 (steady); ntac>=2 handled on the host (rejected)
+  double ibp = (double)imax(M.iblank[i], 0);
+  double qj = M.q[M.nq - 1][i];
+  if (C.iconstant_cfl == 0) {
+    if (C.idual == 0) {
+      M.tscale[i] = oat * C.h * ibp * (1.0 + C.jac_factor * (1. - C.timeac) * sqrt(qj)) /
+                    (1.0 + (1. - C.timeac) * sqrt(qj));
+      M.bt[i] = 1.;
+    } else {
+      double t = ibp * (1.0 + C.jac_factor * sqrt(qj)) / (1.0 + sqrt(qj));
+      if (C.iprecon) t = 100. * ibp * (1.0 + 0.005 * sqrt(qj)) / (1.0 + sqrt(qj));
+      t = t * C.dualtime;
+      M.bt[i] = 1. / (1. + t / C.h / oat);
+      M.tscale[i] = t / (1. + t / C.h / oat);
+    }
+    return;
+  }
+  double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
+  double u = q2 / q1, v = q3 / q1, e = q4 / q1;
+  double ug = M.has_gv ? M.ug[i] : 0.0, vg = M.has_gv ? M.vg[i] : 0.0;
+  double xx = M.xx[i], xy = M.xy[i], yx = M.yx[i], yy = M.yy[i];
+  double uu = (u - ug) * xx + (v - vg) * xy;
+  double vv = (u - ug) * yx + (v - vg) * yy;
+  double dxdx = sq(xx) + sq(xy);
+  double dydy = sq(yx) + sq(yy);
+  double asq = C.ggm1 * (e - 0.5 * (sq(u) + sq(v)));
+  double lam_j = fabs(uu) + sqrt(asq * dxdx);
+  double lam_k = fabs(vv) + sqrt(asq * dydy);
+  if (C.idual == 0) {
+    double t = C.h / (lam_j + lam_k);
+    M.tscale[i] = t * ibp;
+    M.bt[i] = 1.;
+  } else {
+    double hnew = C.dualtime / (lam_j + lam_k);
+    double t = hnew * ibp;
+    t = t / (1. + t / C.h / oat);
+    M.tscale[i] = t;
+    M.bt[i] = 1. / (1. + t / C.h / oat);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_time_step(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  long long n2 = (long long)M.jd * M.kd;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x)
+    time_step_cell(M, i);
+}
+
+// ======================================================================= BCs
+struct QV {
+  double *q1, *q2, *q3, *q4, *qj;
+};
+__device__ inline QV qview(const DevMesh& M) {
+  QV v;
+  v.q1 = M.q[0]; v.q2 = M.q[1]; v.q3 = M.q[2]; v.q4 = M.q[3]; v.qj = M.q[M.nq - 1];
+  return v;
+}
+
+// pressure (times J) of a cell as the reference extrapolates it
+__device__ inline double p_of(const QV& v, double gm1, long long i) {
+  return gm1 * (v.q4[i] - 0.5 * (sq(v.q2[i]) + sq(v.q3[i])) / v.q1[i]) * v.qj[i];
+}
+
+// bcwall + bctany + uv for one j of a k=const wall (src/bc.f:1669-1801,
+// 1328-1415, 2224-2264), including the 30-step ramp of the same cell.
+__device__ inline void bcwall_cell(const DevMesh& M, const QV& v, int j, int k, int k1, int k2,
+                                   int invsc, double scal, double rscal) {
+  const DevConst& C = M.c;
+  long long i = cell(M, j, k), i1 = cell(M, j, k1), i2 = cell(M, j, k2);
+  double ug = M.has_gv ? M.ug[i] : 0.0, vg = M.has_gv ? M.vg[i] : 0.0;
+  double ug1 = M.has_gv ? M.ug[i1] : 0.0, vg1 = M.has_gv ? M.vg[i1] : 0.0;
+  double xx = M.xx[i], xy = M.xy[i], yx = M.yx[i], yy = M.yy[i];
+  double xx1 = M.xx[i1], xy1 = M.xy[i1], yx1 = M.yx[i1], yy1 = M.yy[i1];
+  double u, vv;
+  if (invsc) {
+    double f = 1.0;
+    double u1 = (v.q2[i1] * xx + v.q3[i1] * xy) / v.q1[i1];
+    double u2 = (v.q2[i2] * xx + v.q3[i2] * xy) / v.q1[i2];
+    u = (1. + f) * u1 - f * u2;
+    double v1 = (v.q2[i1] * yx + v.q3[i1] * yy) / v.q1[i1];
+    vv = ug * yx + vg * yy + ug1 * yx1 + vg1 * yy1 - v1;
+  } else {
+    double u1 = (v.q2[i1] * xx + v.q3[i1] * xy) / v.q1[i1];
+    double v1 = (v.q2[i1] * yx + v.q3[i1] * yy) / v.q1[i1];
+    u = ug * xx + vg * xy + ug1 * xx1 + vg1 * xy1 - u1;
+    vv = ug * yx + vg * yy + ug1 * yx1 + vg1 * yy1 - v1;
+  }
+  double t11 = yy, t12 = -xy, t21 = -yx, t22 = xx;
+  double vol = 1. / (t22 * t11 - t12 * t21);
+  double uwall = (t11 * u + t12 * vv) * vol;
+  double vwall = (t21 * u + t22 * vv) * vol;
+  double foso = invsc ? 1.0 : 0.0;
+  double rho1 = v.q1[i1] * v.qj[i1];
+  double rho2 = v.q1[i2] * v.qj[i2];
+  double p1 = p_of(v, C.gm1, i1);
+  double p2 = p_of(v, C.gm1, i2);
+  double qj = v.qj[i];
+  double r = ((1. + foso) * rho1 - foso * rho2) / qj;
+  double m2 = uwall * r;
+  double m3 = vwall * r;
+  double press = (1. + foso) * p1 - foso * p2;
+  double e = press / (C.gm1 * qj) + 0.5 * (sq(m2) + sq(m3)) / r;
+  double ajacinv = 1. / qj;
+  v.q1[i] = r * scal + rscal * C.rinf * ajacinv;
+  v.q2[i] = m2 * scal + rscal * C.rinf * C.uinf * ajacinv;
+  v.q3[i] = m3 * scal + rscal * C.rinf * C.vinf * ajacinv;
+  v.q4[i] = e * scal + rscal * C.einf * ajacinv;
+}
+
+// bcout_trkl first layer (src/bc.f:694-983)
+__device__ inline void trkl_cell(const DevMesh& M, const QV& v, long long i, long long i1,
+                                 long long i2, double xxn, double xyn) {
+  const DevConst& C = M.c;
+  double gm1 = C.gm1, gamma = C.gamma, Mp = C.Mp;
+  double foso = 1.0;
+  double qext[4], qbar[4], delq[4], delw[4];
+  double* qs[4] = {v.q1, v.q2, v.q3, v.q4};
+#pragma unroll
+  for (int n = 0; n < 4; ++n) qext[n] = (1. + foso) * qs[n][i1] * v.qj[i1] - foso * qs[n][i2] * v.qj[i2];
+  qbar[0] = 0.5 * (qext[0] + C.rinf);
+  qbar[1] = 0.5 * (qext[1] + C.rinf * C.uinf);
+  qbar[2] = 0.5 * (qext[2] + C.rinf * C.vinf);
+  qbar[3] = 0.5 * (qext[3] + C.einf);
+  delq[0] = 0.5 * (C.rinf - qext[0]);
+  delq[1] = 0.5 * (C.rinf * C.uinf - qext[1]);
+  delq[2] = 0.5 * (C.rinf * C.vinf - qext[2]);
+  delq[3] = 0.5 * (C.einf - qext[3]);
+  double rho = qbar[0];
+  double rrho = 1. / rho;
+  double uu = qbar[1] * rrho, vv = qbar[2] * rrho, e = qbar[3] * rrho;
+  double uv2 = 0.5 * (uu * uu + vv * vv);
+  double cjkl = sqrt(C.ggm1 * (e - uv2));
+  double c2i = 1. / (cjkl * cjkl);
+  double ge = gamma * e - gm1 * uv2;
+  double ug = M.has_gv ? M.ug[i] : 0.0, vg = M.has_gv ? M.vg[i] : 0.0;
+  double qq = (uu - ug) * xxn + (vv - vg) * xyn;
+  double bt = M.bt[i];
+  double bSq = sq(Mp) / (bt - sq(Mp) * (bt - 1));
+  double Z = sqrt((1. - bSq) * qq * (1. - bSq) * qq + 4. * bSq * cjkl * cjkl);
+  double Zi = 1. / Z;
+  double R = 0.5 * ((1. - bSq) * qq + Z);
+  double S = 0.5 * ((1. - bSq) * qq - Z);
+  double sq0 = fsign(1., qq);
+  double sR = fsign(1., R + bSq * qq), sR2 = fsign(1., R + qq * bSq);
+  double sS = fsign(1., S + bSq * qq), sS2 = fsign(1., S + qq * bSq);
+  delw[0] = (1. - uv2 * gm1 * c2i) * sq0 * delq[0];
+  delw[0] = delw[0] + uu * c2i * gm1 * sq0 * delq[1];
+  delw[0] = delw[0] + vv * c2i * gm1 * sq0 * delq[2];
+  delw[0] = delw[0] - gm1 * c2i * sq0 * delq[3];
+  delw[1] = (vv * xxn - uu * xyn) * sq0 * delq[0];
+  delw[1] = delw[1] + xyn * sq0 * delq[1];
+  delw[1] = delw[1] - xxn * sq0 * delq[2];
+  delw[2] = -(Zi * (uv2 * gm1 * c2i * S + bSq * qq) * sR * delq[0]);
+  delw[2] = delw[2] + (xxn * bSq + S * uu * gm1 * c2i) * Zi * sR * delq[1];
+  delw[2] = delw[2] + (xyn * bSq + S * vv * gm1 * c2i) * Zi * sR2 * delq[2];
+  delw[2] = delw[2] - S * gm1 * c2i * Zi * sR2 * delq[3];
+  delw[3] = Zi * (uv2 * gm1 * c2i * R + qq * bSq) * sS * delq[0];
+  delw[3] = delw[3] - (xxn * bSq + R * uu * gm1 * c2i) * Zi * sS * delq[1];
+  delw[3] = delw[3] - (xyn * bSq + R * vv * gm1 * c2i) * Zi * sS2 * delq[2];
+  delw[3] = delw[3] + R * gm1 * c2i * Zi * sS2 * delq[3];
+  delq[0] = delw[0] + delw[2] + delw[3];
+  delq[1] = uu * delw[0] + xyn * delw[1] + (uu + xxn * R / bSq) * delw[2];
+  delq[1] = delq[1] + (uu + xxn * S / bSq) * delw[3];
+  delq[2] = vv * delw[0] - xxn * delw[1] + (vv + xyn * R / bSq) * delw[2];
+  delq[2] = delq[2] + (vv + xyn * S / bSq) * delw[3];
+  delq[3] = uv2 * delw[0] + (uu * xyn - vv * xxn) * delw[1];
+  delq[3] = delq[3] + (ge + R * qq / bSq) * delw[2];
+  delq[3] = delq[3] + (ge + S * qq / bSq) * delw[3];
+  double qj = v.qj[i];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) qs[n][i] = (qbar[n] - delq[n]) / qj;
+}
+
+// periodic source index (src/bc.f:192-262)
+__device__ inline int periodic_src(int idir, int js, int je, int ks, int ke, int jd, int kd, int jk) {
+  if (idir == 1) { int jj = je - js + 1; return jd - 2 * jj + (jk - js + 1); }
+  if (idir == -1) { int jj = je - js + 1; return 1 + 2 * jj - (je - jk + 1); }
+  if (idir == 2) { int kk = ke - ks + 1; return kd - 2 * kk + (jk - ks + 1); }
+  int kk = ke - ks + 1;
+  return 1 + 2 * kk - (ke - jk + 1);
+}
+
+// One boundary segment, cells distributed over the block.  Every segment
+// type only reads cells outside the rows/columns it writes, so the cells of
+// one segment are independent; segments are separated by barriers to keep
+// the reference's sequential order (src/bc.f:19-82).
+__device__ void bc_segment(const DevMesh& M, const DevBC& b, int istep0) {
+  const DevConst& C = M.c;
+  QV v = qview(M);
+  int js = b.js, je = b.je, ks = b.ks, ke = b.ke, idir = b.idir;
+  int iadd = idir >= 0 ? 1 : -1, iadir = idir >= 0 ? idir : -idir;
+  int t = threadIdx.x, nt = blockDim.x;
+  switch (b.type) {
+    case 3: case 4: case 5: {
+      if (iadir != 2) return;
+      int invsc = b.type == 3 ? 1 : (b.type == 4 ? 0 : C.invisc);
+      int k = (idir == 2) ? ke : ks;
+      double tt = (double)istep0 / 30.;
+      if (tt > 1. || C.iread > 0) tt = 1.;
+      double scal = (10. - 15. * tt + 6. * tt * tt) * cube(tt);
+      double rscal = 1. - scal;
+      // one halo layer: ks == ke, the ramp touches exactly the wall row
+      for (int j = js + t; j <= je; j += nt) bcwall_cell(M, v, j, k, k + iadd, k + 2 * iadd, invsc, scal, rscal);
+      return;
+    }
+    case 10: {
+      double foso = 0.0;
+      double* qs[4] = {v.q1, v.q2, v.q3, v.q4};
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js;  // one layer
+        int j1 = j + iadd, j2 = j1 + iadd;
+        for (int k = ks + t; k <= ke; k += nt) {
+          long long i = cell(M, j, k), i1 = cell(M, j1, k), i2 = cell(M, j2, k);
+          double s1 = v.qj[i1] / v.qj[i], s2 = v.qj[i2] / v.qj[i];
+          for (int n = 0; n < 4; ++n) qs[n][i] = (1. + foso) * qs[n][i1] * s1 - foso * qs[n][i2] * s2;
+        }
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks;
+        int k1 = k + iadd, k2 = k1 + iadd;
+        for (int j = js + t; j <= je; j += nt) {
+          long long i = cell(M, j, k), i1 = cell(M, j, k1), i2 = cell(M, j, k2);
+          double s1 = v.qj[i1] / v.qj[i], s2 = v.qj[i2] / v.qj[i];
+          for (int n = 0; n < 4; ++n) qs[n][i] = (1. + foso) * qs[n][i1] * s1 - foso * qs[n][i2] * s2;
+        }
+      }
+      return;
+    }
+    case 11: {
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = (idir == 1) ? je + 1 : js - 1;
+        for (int k = ks + t; k <= ke; k += nt) {
+          long long i = cell(M, j, k), i1 = cell(M, j1, k);
+          double sc = v.qj[i1] / v.qj[i];
+          v.q1[i] = v.q1[i1] * sc; v.q2[i] = -v.q2[i1] * sc;
+          v.q3[i] = v.q3[i1] * sc; v.q4[i] = v.q4[i1] * sc;
+        }
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+        for (int j = js + t; j <= je; j += nt) {
+          long long i = cell(M, j, k), i1 = cell(M, j, k1);
+          double sc = v.qj[i1] / v.qj[i];
+          v.q1[i] = v.q1[i1] * sc; v.q2[i] = v.q2[i1] * sc;
+          v.q3[i] = -v.q3[i1] * sc; v.q4[i] = v.q4[i1] * sc;
+        }
+      }
+      return;
+    }
+    case 22: {
+      if (iadir == 1) {
+        int ncell = (je - js + 1) * (ke - ks + 1);
+        for (int w = t; w < ncell; w += nt) {
+          int j = js + w % (je - js + 1), k = ks + w / (je - js + 1);
+          int jc = periodic_src(idir, js, je, ks, ke, M.jd, M.kd, j);
+          long long i = cell(M, j, k), i1 = cell(M, jc, k);
+          double sc = v.qj[i1] / v.qj[i];
+          v.q1[i] = v.q1[i1] * sc; v.q2[i] = v.q2[i1] * sc; v.q3[i] = v.q3[i1] * sc; v.q4[i] = v.q4[i1] * sc;
+        }
+      } else {
+        int ncell = (je - js + 1) * (ke - ks + 1);
+        for (int w = t; w < ncell; w += nt) {
+          int j = js + w % (je - js + 1), k = ks + w / (je - js + 1);
+          int kc = periodic_src(idir, js, je, ks, ke, M.jd, M.kd, k);
+          long long i = cell(M, j, k), i1 = cell(M, j, kc);
+          double sc = v.qj[i1] / v.qj[i];
+          v.q1[i] = v.q1[i1] * sc; v.q2[i] = v.q2[i1] * sc; v.q3[i] = v.q3[i1] * sc; v.q4[i] = v.q4[i1] * sc;
+        }
+      }
+      return;
+    }
+    case 51: {
+      if (iadir != 2) return;
+      int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+      for (int j = js + t; j <= je; j += nt) {
+        int jj = M.jd - j + 1;
+        long long a = cell(M, j, k), bb = cell(M, jj, k1), cc = cell(M, jj, k), d = cell(M, j, k1);
+        double scale1 = v.qj[bb] / v.qj[a];
+        double scale2 = v.qj[d] / v.qj[cc];
+        double a1 = v.q1[bb] * scale1, a2 = v.q2[bb] * scale1, a3 = v.q3[bb] * scale1, a4 = v.q4[bb] * scale1;
+        double c1 = v.q1[d] * scale2, c2 = v.q2[d] * scale2, c3 = v.q3[d] * scale2, c4 = v.q4[d] * scale2;
+        v.q1[a] = a1; v.q2[a] = a2; v.q3[a] = a3; v.q4[a] = a4;
+        v.q1[cc] = c1; v.q2[cc] = c2; v.q3[cc] = c3; v.q4[cc] = c4;
+      }
+      return;
+    }
+    case 46: {
+      int ncell = (je - js + 1) * (ke - ks + 1);
+      for (int w = t; w < ncell; w += nt) {
+        int j = js + w % (je - js + 1), k = ks + w / (je - js + 1);
+        long long i = cell(M, j, k);
+        double rj = 1. / v.qj[i];
+        double r = C.rinf * rj, m2 = C.rinf * C.uinf * rj, m3 = C.rinf * C.vinf * rj;
+        v.q1[i] = r; v.q2[i] = m2; v.q3[i] = m3;
+        v.q4[i] = C.pinf / C.gm1 * rj + 0.5 * (sq(m2) + sq(m3)) / r;
+      }
+      return;
+    }
+    case 47: {
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js;
+        for (int k = ks + t; k <= ke; k += nt) {
+          long long i = cell(M, j, k);
+          double snorm = -1. * iadd / sqrt(sq(M.xx[i]) + sq(M.xy[i]));
+          trkl_cell(M, v, i, cell(M, j + iadd, k), cell(M, j + 2 * iadd, k), M.xx[i] * snorm, M.xy[i] * snorm);
+        }
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks;
+        for (int j = js + t; j <= je; j += nt) {
+          long long i = cell(M, j, k);
+          double snorm = -1. * iadd / sqrt(sq(M.yx[i]) + sq(M.yy[i]));
+          trkl_cell(M, v, i, cell(M, j, k + iadd), cell(M, j, k + 2 * iadd), M.yx[i] * snorm, M.yy[i] * snorm);
+        }
+      }
+      return;
+    }
+    default:
+      return;
+  }
+}
+
+// zero s on the halo ring (the reference clears all of s at step start,
+// src/step.f:41; interior s is fully rewritten by k_residual)
+__device__ void zero_halo_s(const DevMesh& M) {
+  int jd = M.jd, kd = M.kd;
+  int nring = 2 * jd + 2 * (kd - 2);
+  for (int w = threadIdx.x; w < nring; w += blockDim.x) {
+    int j, k;
+    if (w < jd) { j = w + 1; k = 1; }
+    else if (w < 2 * jd) { j = w - jd + 1; k = kd; }
+    else if (w < 2 * jd + kd - 2) { j = 1; k = w - 2 * jd + 2; }
+    else { j = jd; k = w - 2 * jd - (kd - 2) + 2; }
+    long long i = cell(M, j, k);
+    for (int n = 0; n < M.nv; ++n) M.s[n][i] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bc(const DevMesh* __restrict__ Ms, int zero_s) {
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
+  int istep0 = *M.istep0;
+  if (zero_s) zero_halo_s(M);
+  for (int ib = 0; ib < M.nbc; ++ib) {
+    bc_segment(M, M.bc[ib], istep0);
+    __syncthreads();
+  }
+}
+
+// =================================================================== residual
+// MUSCL kappa=1/3 with Koren's differentiable limiter for cell i of a line
+// (src/eulerrhs.f:856-872): returns ql(i), qr(i).
+__device__ __forceinline__ void muscl_cell(double fm, double f0, double fp, int ibm, int ib0, int ibp,
+                                           double th, double qt, double eps, double& ql, double& qr) {
+  double thm = 1.0 - th, thp = 1.0 + th;
+  double f2i = fp - f0, f2i1 = f0 - fm;
+  double a1 = 3.0 * (f2i * f2i1 + eps);
+  double a2 = 2.0 * sq(f2i - f2i1) + a1;
+  double f3i = a1 / a2;
+  double f3qt = qt * f3i * (double)ib0 * (double)ibp * (double)ibm;
+  ql = f0 + f3qt * (thm * f2i1 + thp * f2i);
+  qr = f0 - f3qt * (thp * f2i1 + thm * f2i);
+}
+
+// Roe flux with the smooth entropy fix (src/eulerrhs.f:283-446) for one face
+// with left state l, right state r (rho,u,v,p), face vector (xa,ya), tj.
+__device__ __forceinline__ void roe_face(const DevConst& C, const double* l, const double* r, double xa,
+                                         double ya, double tj, double* f) {
+  double gm1 = C.gm1;
+  double eps = 1.e-6;
+  double rlft = l[0], ulft = l[1], vlft = l[2], plft = l[3];
+  double rlfti = 1.0 / rlft;
+  double rulft = rlft * ulft, rvlft = rlft * vlft;
+  double uvl = 0.5 * (ulft * ulft + vlft * vlft);
+  double elft = plft / gm1 + rlft * uvl;
+  double hlft = (elft + plft) * rlfti;
+  double clft = sqrt(gm1 * (hlft - uvl));
+  double rrht = r[0], urht = r[1], vrht = r[2], prht = r[3];
+  double rrhti = 1.0 / rrht;
+  double rurht = rrht * urht, rvrht = rrht * vrht;
+  double uvr = 0.5 * (urht * urht + vrht * vrht);
+  double erht = prht / gm1 + rrht * uvr;
+  double hrht = (erht + prht) * rrhti;
+  double crht = sqrt(gm1 * (hrht - uvr));
+  double rat = sqrt(rrht * rlfti);
+  double rati = 1.0 / (rat + 1.);
+  double rav = rat * rlft;
+  double uav = (rat * urht + ulft) * rati;
+  double vav = (rat * vrht + vlft) * rati;
+  double hav = (rat * hrht + hlft) * rati;
+  double uv = 0.5 * (uav * uav + vav * vav);
+  double cav = sqrt(gm1 * (hav - uv));
+  double aq1 = rrht - rlft, aq2 = urht - ulft, aq3 = vrht - vlft, aq4 = prht - plft;
+  double ri1 = xa, ri2 = ya, ri3 = tj;
+  double rr2 = ri1 * ri1 + ri2 * ri2;
+  double rr = sqrt(rr2);
+  double r0 = 1.0 / rr;
+  double r1 = ri1 * r0, r2 = ri2 * r0, r3 = ri3 * r0;
+  double uu = r1 * uav + r2 * vav + r3;
+  double c2 = cav * cav;
+  double c2i = 1.0 / c2;
+  double auu = fabs(uu), aupc = fabs(uu + cav), aumc = fabs(uu - cav);
+  double uulft = r1 * ulft + r2 * vlft + r3;
+  double uurht = r1 * urht + r2 * vrht + r3;
+  double upclft = uulft + clft, upcrht = uurht + crht;
+  double umclft = uulft - clft, umcrht = uurht - crht;
+  double dauu = 4. * (uurht - uulft) + eps;
+  double dauus = fmax(dauu, 0.0);
+  if (auu <= 0.5 * dauus) auu = sq(auu) / dauu + 0.25 * dauu;
+  double daupc = 4. * (upcrht - upclft) + eps;
+  double daupcs = fmax(daupc, 0.0);
+  if (aupc <= 0.5 * daupcs) aupc = sq(aupc) / daupc + 0.25 * daupc;
+  double daumc = 4. * (umcrht - umclft) + eps;
+  double daumcs = fmax(daumc, 0.0);
+  if (aumc <= 0.5 * daumcs) aumc = sq(aumc) / daumc + 0.25 * daumc;
+  double rcav = rav * cav;
+  double aquu = uurht - uulft;
+  double c2ih = 0.5 * c2i;
+  double ruuav = auu * rav;
+  double b1 = auu * (aq1 - c2i * aq4);
+  double b2 = c2ih * aupc * (aq4 + rcav * aquu);
+  double b3 = c2ih * aumc * (aq4 - rcav * aquu);
+  double b4 = b1 + b2 + b3;
+  double b5 = cav * (b2 - b3);
+  double b6 = ruuav * (aq2 - r1 * aquu);
+  double b7 = ruuav * (aq3 - r2 * aquu);
+  aq1 = b4;
+  aq2 = uav * b4 + r1 * b5 + b6;
+  aq3 = vav * b4 + r2 * b5 + b7;
+  aq4 = hav * b4 + (uu - r3) * b5 + uav * b6 + vav * b7 - c2 * b1 / gm1;
+  double aj = 0.5 * rr;
+  double plar = plft + prht;
+  double eplft = elft + plft, eprht = erht + prht;
+  double fssub = 0.0;
+  f[0] = aj * (rlft * uulft + rrht * uurht - aq1) - fssub * C.rinf;
+  f[1] = aj * (rulft * uulft + rurht * uurht + r1 * plar - aq2) - fssub * C.rinf * C.uinf;
+  f[2] = aj * (rvlft * uulft + rvrht * uurht + r2 * plar - aq3) - fssub * C.rinf * C.vinf;
+  f[3] = aj * (eplft * uulft + eprht * uurht - r3 * plar - aq4) - fssub * C.einf;
+}
+
+// Reconstructed left/right states at the face between line cells i and i+1
+// of a line with n cells (1-based) held in LDS (stride st between cells).
+// First order at the line ends exactly as muscld_new (is=1, imax=n).
+__device__ __forceinline__ void face_states(const double* sf0, int stv, int stc, const int* sib, int stci,
+                                            int c0, int i, int n, double th, double qt, double eps,
+                                            double* l, double* r) {
+  // c0 = LDS column of line cell i
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double* f = sf0 + v * stv;
+    double fl0 = f[c0 * stc], fr0 = f[(c0 + 1) * stc];
+    if (qt == 0.0) { l[v] = fl0; r[v] = fr0; continue; }
+    double dummy;
+    if (i == 1) l[v] = fl0;
+    else muscl_cell(f[(c0 - 1) * stc], fl0, fr0, sib[(c0 - 1) * stci], sib[c0 * stci], sib[(c0 + 1) * stci],
+                    th, qt, eps, l[v], dummy);
+    if (i + 1 == n) r[v] = fr0;
+    else muscl_cell(fl0, fr0, f[(c0 + 2) * stc], sib[c0 * stci], sib[(c0 + 1) * stci], sib[(c0 + 2) * stci],
+                    th, qt, eps, dummy, r[v]);
+  }
+}
+
+// Residual of one 32x8 tile: rhsup for nei=1 (src/eulerrhs.f:2-280), fluxes
+// of the tile's faces computed once into LDS, plus the residual-norm partials
+// of step (src/step.f:60-93).  Writes s(:,:,1:4) of the tile's interior cells.
+__global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restrict__ Ms,
+                                                          const ResTile* __restrict__ tiles, int do_norms) {
+  constexpr int TJ = RES_TJ, TK = RES_TK, LJ = TJ + 4, LK = TK + 4;
+  __shared__ double sf[4][LK][LJ];   // primitives rho, u, v, p
+  __shared__ int sib[LK][LJ];
+  __shared__ double sxv[TK + 1][TJ + 1], syv[TK + 1][TJ + 1];
+  __shared__ double sF[4][TK][TJ + 1];  // j-face fluxes
+  __shared__ double sG[4][TK + 1][TJ];  // k-face fluxes
+  __shared__ double red[4][RES_THREADS];
+  const ResTile T = tiles[blockIdx.x];
+  const DevMesh& M = Ms[T.mesh];
+  const DevConst& C = M.c;
+  const int tid = threadIdx.x;
+  const int jd = M.jd, kd = M.kd;
+  const int j0 = T.j0, k0 = T.k0;
+  QV v = qview(M);
+  // ---- load primitives of the tile + 2-cell halo (corners unused)
+  for (int w = tid; w < LK * LJ; w += RES_THREADS) {
+    int r = w / LJ, cc = w % LJ;
+    int j = j0 - 2 + cc, k = k0 - 2 + r;
+    bool row_in = (r >= 2 && r < 2 + TK), col_in = (cc >= 2 && cc < 2 + TJ);
+    if (!row_in && !col_in) continue;
+    int jj = j < 1 ? 1 : (j > jd ? jd : j);
+    int kk = k < 1 ? 1 : (k > kd ? kd : k);
+    long long i = cell(M, jj, kk);
+    double q1 = v.q1[i], q2 = v.q2[i], q3 = v.q3[i], q4 = v.q4[i], qj = v.qj[i];
+    double rhoi = 1.0 / q1;
+    double ppp = C.gm1 * (q4 - 0.5 * (sq(q2) + sq(q3)) / q1);
+    sf[0][r][cc] = q1 * qj;
+    sf[1][r][cc] = q2 * rhoi;
+    sf[2][r][cc] = q3 * rhoi;
+    sf[3][r][cc] = ppp * qj;
+    int ib = M.iblank[i];
+    sib[r][cc] = ib < 0 ? -ib : ib;
+  }
+  for (int w = tid; w < (TK + 1) * (TJ + 1); w += RES_THREADS) {
+    int r = w / (TJ + 1), cc = w % (TJ + 1);
+    int jv = j0 - 1 + cc, kv = k0 - 1 + r;
+    jv = jv > M.jmax ? M.jmax : jv;
+    kv = kv > M.kmax ? M.kmax : kv;
+    long long iv = vert(M, jv, kv);
+    sxv[r][cc] = M.xv[iv];
+    syv[r][cc] = M.yv[iv];
+  }
+  __syncthreads();
+  const double th = C.th, qt = C.qt;
+  // ---- xi (j-direction) faces: face fi between cells fi and fi+1, rows k0..k0+TK-1
+  for (int w = tid; w < TK * (TJ + 1); w += RES_THREADS) {
+    int r = w / (TJ + 1), cc = w % (TJ + 1);
+    int fi = j0 - 1 + cc, k = k0 + r;
+    if (fi > M.je || k > M.ke) continue;
+    double l[4], rr[4], f[4];
+    // LDS column of cell fi: fi - (j0 - 2)
+    face_states(&sf[0][r + 2][0], LK * LJ, 1, &sib[r + 2][0], 1, fi - (j0 - 2), fi, jd, th, qt, M.epsj, l, rr);
+    // face vector at vertex column jv = fi, rows kv-1..kv with kv = k
+    double xa = syv[r + 1][cc] - syv[r][cc];
+    double ya = -(sxv[r + 1][cc] - sxv[r][cc]);
+    double tj = 0.0;
+    if (M.has_gv) {
+      long long a = vert(M, fi, k), b = vert(M, fi, k - 1);
+      tj = -0.5 * (M.ugv[a] + M.ugv[b]) * xa - 0.5 * (M.vgv[a] + M.vgv[b]) * ya;
+    }
+    roe_face(C, l, rr, xa, ya, tj, f);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) sF[n][r][cc] = f[n];
+  }
+  // ---- eta (k-direction) faces: face fk between cells fk and fk+1, columns j0..j0+TJ-1
+  for (int w = tid; w < (TK + 1) * TJ; w += RES_THREADS) {
+    int r = w / TJ, cc = w % TJ;
+    int fk = k0 - 1 + r, j = j0 + cc;
+    if (fk > M.ke || j > M.je) continue;
+    double l[4], rr[4], f[4];
+    face_states(&sf[0][0][cc + 2], LK * LJ, LJ, &sib[0][cc + 2], LJ, fk - (k0 - 2), fk, kd, th, qt, M.epsk, l, rr);
+    // vertex row kv = fk, columns jv-1..jv with jv = j
+    double xa = -(syv[r][cc + 1] - syv[r][cc]);
+    double ya = (sxv[r][cc + 1] - sxv[r][cc]);
+    double tj = 0.0;
+    if (M.has_gv) {
+      long long a = vert(M, j, fk), b = vert(M, j - 1, fk);
+      tj = -0.5 * (M.ugv[a] + M.ugv[b]) * xa - 0.5 * (M.vgv[a] + M.vgv[b]) * ya;
+    }
+    roe_face(C, l, rr, xa, ya, tj, f);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) sG[n][r][cc] = f[n];
+  }
+  __syncthreads();
+  // ---- flux balance
+  int cx = tid % TJ, ry = tid / TJ;
+  int j = j0 + cx, k = k0 + ry;
+  double rs = 0.0, rr1 = 0.0, rmax = 0.0, key = 0.0;
+  bool valid = (j <= M.je && k <= M.ke);
+  if (valid) {
+    long long i = cell(M, j, k);
+    double ibp = (double)imax(M.iblank[i], 0);
+    double s[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      s[n] = 0.0 - (sF[n][ry][cx + 1] - sF[n][ry][cx]);
+      s[n] = s[n] - (sG[n][ry + 1][cx] - sG[n][ry][cx]);
+      M.s[n][i] = s[n];
+    }
+    if (do_norms) {
+      double m1 = s[0] * ibp, m2 = s[1] * ibp, m3 = s[2] * ibp, m4 = s[3] * ibp;
+      rs = m1 * m1 + m2 * m2 + m3 * m3 + m4 * m4;
+      rr1 = sq(s[0]);
+      // first maximum in (k, n, j) order (src/step.f:67-78)
+      for (int n = 0; n < 4; ++n) {
+        double a = fabs(s[n]) * ibp;
+        if (a > rmax) { rmax = a; key = (double)(((long long)(k - 1) * 4 + n) * jd + (j - 1)); }
+      }
+    }
+  }
+  if (!do_norms) return;
+  red[0][tid] = rs; red[1][tid] = rr1; red[2][tid] = rmax; red[3][tid] = valid && rmax > 0.0 ? key : 1e300;
+  __syncthreads();
+  for (int off = RES_THREADS / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      red[0][tid] = red[0][tid] + red[0][tid + off];
+      red[1][tid] = red[1][tid] + red[1][tid + off];
+      double a = red[2][tid], b = red[2][tid + off];
+      double ka = red[3][tid], kb = red[3][tid + off];
+      if (b > a || (b == a && kb < ka)) { red[2][tid] = b; red[3][tid] = kb; }
+    }
+    __syncthreads();
+  }
+  __shared__ bool last;
+  if (tid == 0) {
+    double* P = M.partials + 4ll * T.blk;
+    P[0] = red[0][0]; P[1] = red[1][0]; P[2] = red[2][0]; P[3] = red[3][0];
+    __threadfence();
+    unsigned int done = atomicAdd(M.counter, 1u);
+    last = (done == (unsigned)M.res_nblocks - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  // last tile of this instance: reduce the partials in tile order
+  if (tid == 0) {
+    __threadfence();
+    double rsum = 0.0, resrho = 0.0, resmax = 0.0, kbest = 1e300;
+    volatile double* P = M.partials;
+    for (int b = 0; b < M.res_nblocks; ++b) {
+      rsum = rsum + P[4 * b + 0];
+      resrho = resrho + P[4 * b + 1];
+      double a = P[4 * b + 2], kb = P[4 * b + 3];
+      if (a > resmax || (a == resmax && kb < kbest)) { resmax = a; kbest = kb; }
+    }
+    double volum = (double)(jd * kd);
+    resrho = sqrt(resrho / volum);
+    rsum = sqrt(rsum / volum);
+    int jm = 2, km = 2, nm = 1;
+    if (resmax > 0.0) {
+      long long kk = (long long)kbest;
+      jm = (int)(kk % jd) + 1;
+      long long rest = kk / jd;
+      nm = (int)(rest % 4) + 1;
+      km = (int)(rest / 4) + 1;
+    }
+    double* N = M.norms;
+    N[0] = rsum; N[1] = resrho; N[2] = resmax; N[3] = jm; N[4] = km; N[5] = nm;
+    *M.counter = 0u;
+    if (rsum > 1000.0) atomicCAS(M.status, 0, 2);
+  }
+}
+
+// ===================================================== time-spectral coupling
+// src/sources.f:2-27: s(nsp) -= rotf*Ds(nsp,nsp1)*q(nsp1) over interior cells
+__global__ __launch_bounds__(256) void k_tsrhs(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
+  long long ncell = (long long)nj * nk;
+  int N = M.nspec, nsp = M.nsp;
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
+    int j = M.js + (int)(w % nj), k = M.ks + (int)(w / nj);
+    long long i = cell(M, j, k);
+    for (int n = 0; n < M.nv; ++n) {
+      double s = M.s[n][i];
+      for (int nsp1 = 0; nsp1 < N; ++nsp1) {
+        double dsv = M.Ds[nsp + (long long)nsp1 * N];
+        s = s - M.c.rotf * dsv * M.qall[n][i + nsp1 * M.inst_stride];
+      }
+      M.s[n][i] = s;
+    }
+  }
+}
+
+// ================================================================ residual BC
+// rbc (src/bc.f:88-177): zero for wall/far-field types, wake copy
+// (rbcwake 1576-1633), extrapolation (2029-2094), symmetry (2160-2222),
+// periodic (270-357).
+__device__ void rbc_segment(const DevMesh& M, const DevBC& b) {
+  double* qj = M.q[M.nq - 1];
+  double* ss[4] = {M.s[0], M.s[1], M.s[2], M.s[3]};
+  int js = b.js, je = b.je, ks = b.ks, ke = b.ke, idir = b.idir;
+  int iadd = idir >= 0 ? 1 : -1, iadir = idir >= 0 ? idir : -idir;
+  int t = threadIdx.x, nt = blockDim.x;
+  int ty = b.type;
+  int nj = je - js + 1, nk = ke - ks + 1;
+  if (ty == 3 || ty == 4 || ty == 5 || ty == 46 || ty == 47) {
+    for (int w = t; w < nj * nk; w += nt) {
+      long long i = cell(M, js + w % nj, ks + w / nj);
+      for (int n = 0; n < 4; ++n) ss[n][i] = 0.0;
+    }
+  } else if (ty == 10) {
+    double foso = 0.0;
+    if (iadir == 1) {
+      int j = (idir == 1) ? je : js, j1 = j + iadd, j2 = j1 + iadd;
+      for (int k = ks + t; k <= ke; k += nt) {
+        long long i = cell(M, j, k), i1 = cell(M, j1, k), i2 = cell(M, j2, k);
+        double s1 = qj[i1] / qj[i], s2 = qj[i2] / qj[i];
+        for (int n = 0; n < 4; ++n) ss[n][i] = (1. + foso) * ss[n][i1] * s1 - foso * ss[n][i2] * s2;
+      }
+    } else if (iadir == 2) {
+      int k = (idir == 2) ? ke : ks, k1 = k + iadd, k2 = k1 + iadd;
+      for (int j = js + t; j <= je; j += nt) {
+        long long i = cell(M, j, k), i1 = cell(M, j, k1), i2 = cell(M, j, k2);
+        double s1 = qj[i1] / qj[i], s2 = qj[i2] / qj[i];
+        for (int n = 0; n < 4; ++n) ss[n][i] = (1. + foso) * ss[n][i1] * s1 - foso * ss[n][i2] * s2;
+      }
+    }
+  } else if (ty == 11) {
+    if (iadir == 1) {
+      int j = (idir == 1) ? je : js, j1 = (idir == 1) ? je + 1 : js - 1;
+      for (int k = ks + t; k <= ke; k += nt) {
+        long long i = cell(M, j, k), i1 = cell(M, j1, k);
+        double sc = qj[i1] / qj[i];
+        ss[0][i] = ss[0][i1] * sc; ss[1][i] = -ss[1][i1] * sc;
+        ss[2][i] = ss[2][i1] * sc; ss[3][i] = ss[3][i1] * sc;
+      }
+    } else if (iadir == 2) {
+      int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+      for (int j = js + t; j <= je; j += nt) {
+        long long i = cell(M, j, k), i1 = cell(M, j, k1);
+        double sc = qj[i1] / qj[i];
+        ss[0][i] = ss[0][i1] * sc; ss[1][i] = ss[1][i1] * sc;
+        ss[2][i] = -ss[2][i1] * sc; ss[3][i] = ss[3][i1] * sc;
+      }
+    }
+  } else if (ty == 22) {
+    for (int w = t; w < nj * nk; w += nt) {
+      int j = js + w % nj, k = ks + w / nj;
+      long long i = cell(M, j, k), i1;
+      if (iadir == 1) i1 = cell(M, periodic_src(idir, js, je, ks, ke, M.jd, M.kd, j), k);
+      else i1 = cell(M, j, periodic_src(idir, js, je, ks, ke, M.jd, M.kd, k));
+      double sc = qj[i1] / qj[i];
+      for (int n = 0; n < 4; ++n) ss[n][i] = ss[n][i1] * sc;
+    }
+  } else if (ty == 51 && iadir == 2) {
+    int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+    for (int j = js + t; j <= je; j += nt) {
+      int jj = M.jd - j + 1;
+      long long a = cell(M, j, k), bb = cell(M, jj, k1), cc = cell(M, jj, k), d = cell(M, j, k1);
+      double scale1 = qj[bb] / qj[a];
+      double scale2 = qj[d] / qj[cc];
+      double x[4], y[4];
+      for (int n = 0; n < 4; ++n) { x[n] = ss[n][bb] * scale1; y[n] = ss[n][d] * scale2; }
+      for (int n = 0; n < 4; ++n) { ss[n][a] = x[n]; ss[n][cc] = y[n]; }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rbc(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
+  for (int ib = 0; ib < M.nbc; ++ib) {
+    rbc_segment(M, M.bc[ib]);
+    __syncthreads();
+  }
+}
+
+// ==================================================================== LU-SGS
+// Per-cell quantities ilu2d derives from the primitive state
+// (src/lhs.f:524-564): u, v, uv2, ge, contravariant velocities, spectral
+// radii, viscous term and the diagonal d.
+template <typename T>
+struct LUCoef {
+  T uu, vv, uv2, ge;   // primitive velocity, kinetic energy, gamma*e - gm1*uv2
+  T ri1, ri2, qq;      // metric pair and contravariant velocity of the direction
+  T chk, spec;         // upwind switch and spectral term of the direction/sweep
+};
+
+struct LUCellF64 {
+  double uu, vv, uv2, ge, qx, qy, cx, cy, vn, d, qqx_s, qqy_s;
+};
+
+__device__ __forceinline__ LUCellF64 lu_cell(const DevConst& C, double q1, double q2, double q3, double q4,
+                                             double qj, double xx, double xy, double yx, double yy, double ug,
+                                             double vg, double turmu, double ts, double epsv) {
+  LUCellF64 o;
+  double dj = 1.0 / q1;
+  double u = q2 * dj, v = q3 * dj, e = q4 * dj, rho = q1 * qj;
+  double uv2 = 0.5 * (u * u + v * v);
+  double cjkl = sqrt(C.ggm1 * (e - uv2));
+  double qq1 = xx * u + xy * v;
+  double qqxs = -xx * ug - xy * vg + qq1;
+  double qqx = fabs(qqxs);
+  double rr1 = sqrt(sq(xx) + sq(xy));
+  double qq2 = yx * u + yy * v;
+  double qqys = -yx * ug - yy * vg + qq2;
+  double qqy = fabs(qqys);
+  double rr2 = sqrt(sq(yx) + sq(yy));
+  double vnu = 2.0 * rr2 * rr2 * (C.rmue + turmu) / (C.rey * rho);
+  o.d = 1.0 / (1.0 + ts * ((qqx + qqy + cjkl * (rr1 + rr2)) * epsv + vnu));
+  o.uu = u; o.vv = v; o.uv2 = uv2;
+  o.qx = qq1; o.cx = cjkl * rr1; o.qy = qq2; o.cy = cjkl * rr2; o.vn = vnu;
+  o.ge = C.gamma * e - C.gm1 * uv2;
+  o.qqx_s = qqxs; o.qqy_s = qqys;
+  return o;
+}
+
+// A(+/-) or B(+/-) operator of a neighbour applied to its s (src/lhs.f:633-651)
+__device__ __forceinline__ void lu_apply(const double gm1, double chk, double spec, double ri1, double ri2,
+                                         double qq, double uu, double vv, double uv2, double ge, const double* s,
+                                         double* a) {
+  double a4 = chk * (ri1 * s[1] + ri2 * s[2] - qq * s[0]);
+  double a2 = chk * gm1 * (uv2 * s[0] - (uu * s[1] + vv * s[2]) + s[3]);
+  a[0] = a4 + spec * s[0];
+  a[1] = ri1 * a2 + uu * a4 + spec * s[1];
+  a[2] = ri2 * a2 + vv * a4 + spec * s[2];
+  a[3] = qq * a2 + ge * a4 + spec * s[3];
+}
+
+struct LUNb {           // what a consumer needs of a neighbour cell
+  double s[4];
+  double ri1, ri2, qq, uu, vv, uv2, ge, chk, spec;
+};
+
+struct CellLoad {
+  double q1, q2, q3, q4, qj, xx, xy, yx, yy, ug, vg, tm, ts;
+};
+
+__device__ __forceinline__ CellLoad load_cell(const DevMesh& M, long long i) {
+  CellLoad c;
+  c.q1 = M.q[0][i]; c.q2 = M.q[1][i]; c.q3 = M.q[2][i]; c.q4 = M.q[3][i]; c.qj = M.q[M.nq - 1][i];
+  c.xx = M.xx[i]; c.xy = M.xy[i]; c.yx = M.yx[i]; c.yy = M.yy[i];
+  c.ug = M.has_gv ? M.ug[i] : 0.0;
+  c.vg = M.has_gv ? M.vg[i] : 0.0;
+  c.tm = M.turmu[i];
+  c.ts = M.tscale[i];
+  return c;
+}
+
+__device__ __forceinline__ LUCellF64 lu_of(const DevConst& C, const CellLoad& c, double epsv) {
+  return lu_cell(C, c.q1, c.q2, c.q3, c.q4, c.qj, c.xx, c.xy, c.yx, c.yy, c.ug, c.vg, c.tm, c.ts, epsv);
+}
+
+// neighbour record for the j direction (A term) of sweep direction dir (+1 fwd, -1 bwd)
+__device__ __forceinline__ void nb_j(const CellLoad& c, const LUCellF64& L, double eps2, int dir, LUNb& o) {
+  double qqx = L.qqx_s;
+  double sp1 = fabs(qqx) + L.cx;
+  double sm2 = eps2 * sp1;
+  if (dir > 0) {
+    o.chk = 0.5 + fsign(0.5, qqx + L.cx);
+    o.spec = o.chk * (qqx + sp1) + sm2;
+  } else {
+    o.chk = 0.5 - fsign(0.5, qqx - L.cx);
+    o.spec = o.chk * (qqx - sp1) - sm2;
+  }
+  o.ri1 = c.xx; o.ri2 = c.xy; o.qq = L.qx; o.uu = L.uu; o.vv = L.vv; o.uv2 = L.uv2; o.ge = L.ge;
+}
+
+// neighbour record for the k direction (B term)
+__device__ __forceinline__ void nb_k(const CellLoad& c, const LUCellF64& L, double eps2, int dir, LUNb& o) {
+  double qqy = L.qqy_s;
+  double sp1 = fabs(qqy) + L.cy;
+  double sm2 = eps2 * sp1;
+  if (dir > 0) {
+    o.chk = 0.5 + fsign(0.5, qqy + L.cy);
+    o.spec = o.chk * (qqy + sp1) + L.vn + sm2;
+  } else {
+    o.chk = 0.5 - fsign(0.5, qqy - L.cy);
+    o.spec = o.chk * (qqy - sp1) - L.vn - sm2;
+  }
+  o.ri1 = c.yx; o.ri2 = c.yy; o.qq = L.qy; o.uu = L.uu; o.vv = L.vv; o.uv2 = L.uv2; o.ge = L.ge;
+}
+
+// Forward sweep of ilu2d.  One block per instance; thread t owns row
+// k = ks + t and visits cell (m-k, k) on hyperplane m = j + k.  The j-1
+// neighbour is the thread's own previous cell (kept in registers), the k-1
+// neighbour comes from thread t-1's previous cell through LDS.  The prologue
+// applies s *= tscale (src/step.f:141-148) to the halo ring and the
+// point-Jacobi update of the four halo sides (src/lhs.f:568-598).
+constexpr int LU_NB_DBL = 13;
+
+__global__ void k_lu_fwd(const DevMesh* __restrict__ Ms) {
+  extern __shared__ double lds[];
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
+  const DevConst& C = M.c;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke, jd = M.jd, kd = M.kd;
+  const int nk = ke - ks + 1;
+  const double eps2 = C.epse * 2.;
+  const double epsv = 1. + eps2;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // ---- prologue: halo ring
+  int nring = 2 * jd + 2 * (kd - 2);
+  for (int w = tid; w < nring; w += nt) {
+    int j, k;
+    if (w < jd) { j = w + 1; k = 1; }
+    else if (w < 2 * jd) { j = w - jd + 1; k = kd; }
+    else if (w < 2 * jd + kd - 2) { j = 1; k = w - 2 * jd + 2; }
+    else { j = jd; k = w - 2 * jd - (kd - 2) + 2; }
+    long long i = cell(M, j, k);
+    double ts = M.tscale[i];
+    bool side = ((j == js - 1 || j == je + 1) && k >= ks && k <= ke) ||
+                ((k == ks - 1 || k == ke + 1) && j >= js && j <= je);
+    double d = 1.0;
+    if (side) {
+      CellLoad c = load_cell(M, i);
+      d = lu_of(C, c, epsv).d;
+    }
+    for (int n = 0; n < 4; ++n) {
+      double s = M.s[n][i] * ts;
+      if (side) s = s * d;
+      M.s[n][i] = s;
+    }
+  }
+  __syncthreads();
+  double* buf0 = lds;
+  double* buf1 = lds + (size_t)LU_NB_DBL * nt;
+  const int k = ks + tid;
+  const bool row = tid < nk;
+  LUNb prevA;  // own previous cell (j-1, k)
+  for (int m = js + ks; m <= je + ke; ++m) {
+    double* wr = ((m & 1) ? buf1 : buf0);
+    double* rd = ((m & 1) ? buf0 : buf1);
+    int j = m - k;
+    bool act = row && j >= js && j <= je;
+    if (act) {
+      long long i = cell(M, j, k);
+      CellLoad c = load_cell(M, i);
+      LUCellF64 L = lu_of(C, c, epsv);
+      // A neighbour (j-1, k)
+      if (j == js) {
+        long long ia = cell(M, js - 1, k);
+        CellLoad ca = load_cell(M, ia);
+        LUCellF64 La = lu_of(C, ca, epsv);
+        nb_j(ca, La, eps2, +1, prevA);
+        for (int n = 0; n < 4; ++n) prevA.s[n] = M.s[n][ia];
+      }
+      // B neighbour (j, k-1)
+      LUNb B;
+      if (k == ks) {
+        long long ib = cell(M, j, ks - 1);
+        CellLoad cb = load_cell(M, ib);
+        LUCellF64 Lb = lu_of(C, cb, epsv);
+        nb_k(cb, Lb, eps2, +1, B);
+        for (int n = 0; n < 4; ++n) B.s[n] = M.s[n][ib];
+      } else {
+        const double* p = rd + (size_t)LU_NB_DBL * (tid - 1);
+        for (int n = 0; n < 4; ++n) B.s[n] = p[n];
+        B.ri1 = p[4]; B.ri2 = p[5]; B.qq = p[6]; B.uu = p[7]; B.vv = p[8]; B.uv2 = p[9]; B.ge = p[10];
+        B.chk = p[11]; B.spec = p[12];
+      }
+      double a[4], b[4];
+      lu_apply(C.gm1, prevA.chk, prevA.spec, prevA.ri1, prevA.ri2, prevA.qq, prevA.uu, prevA.vv, prevA.uv2,
+               prevA.ge, prevA.s, a);
+      lu_apply(C.gm1, B.chk, B.spec, B.ri1, B.ri2, B.qq, B.uu, B.vv, B.uv2, B.ge, B.s, b);
+      double sv = c.ts * 0.5;
+      double di = L.d;
+      double snew[4];
+      for (int n = 0; n < 4; ++n) {
+        double st = M.s[n][i] * c.ts;  // s*tscale (src/step.f:141-148)
+        snew[n] = (st + sv * (a[n] + b[n])) * di;
+        M.s[n][i] = snew[n];
+      }
+      // becomes the A neighbour of (j+1, k) and the B neighbour of (j, k+1)
+      nb_j(c, L, eps2, +1, prevA);
+      LUNb E;
+      nb_k(c, L, eps2, +1, E);
+      for (int n = 0; n < 4; ++n) prevA.s[n] = snew[n];
+      double* p = wr + (size_t)LU_NB_DBL * tid;
+      for (int n = 0; n < 4; ++n) p[n] = snew[n];
+      p[4] = E.ri1; p[5] = E.ri2; p[6] = E.qq; p[7] = E.uu; p[8] = E.vv; p[9] = E.uv2; p[10] = E.ge;
+      p[11] = E.chk; p[12] = E.spec;
+    }
+    __syncthreads();
+  }
+}
+
+// round trip of ilu2d's in-place primitive conversion (src/lhs.f:524-531,
+// 805-811): the reference leaves q(j,k,1:4) with these bits
+__device__ __forceinline__ void q_roundtrip(double& q1, double& q2, double& q3, double& q4, double qj) {
+  double dj = 1.0 / q1;
+  double u = q2 * dj, v = q3 * dj, e = q4 * dj;
+  double r = q1 * qj;
+  q1 = r / qj;
+  q2 = u * q1; q3 = v * q1; q4 = e * q1;
+}
+
+__device__ __forceinline__ bool asq_bad(const DevConst& C, double q1, double q2, double q3, double q4) {
+  double rho = q1;
+  double qq = sq(q2) + sq(q3);
+  double asq = C.ggm1 * (q4 - .5 * qq / rho) / rho;
+  return asq <= 0.002 || asq + 1 == asq;
+}
+
+// Backward sweep of ilu2d (src/lhs.f:719-800) with the conservative
+// restore; with fuse_update the q update (src/step.f:205-211) and monitor
+// (src/step.f:353-385) are applied to each cell as soon as its correction
+// is final.
+__global__ void k_lu_bwd(const DevMesh* __restrict__ Ms, int fuse_update) {
+  extern __shared__ double lds[];
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
+  const DevConst& C = M.c;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke, jd = M.jd, kd = M.kd;
+  const int nk = ke - ks + 1;
+  const double eps2 = C.epse * 2.;
+  const double epsv = 1. + eps2;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double* buf0 = lds;
+  double* buf1 = lds + (size_t)LU_NB_DBL * nt;
+  const int k = ks + tid;
+  const bool row = tid < nk;
+  int bad = 0;
+  LUNb prevA;  // own previous cell (j+1, k)
+  for (int m = je + ke; m >= js + ks; --m) {
+    double* wr = ((m & 1) ? buf1 : buf0);
+    double* rd = ((m & 1) ? buf0 : buf1);
+    int j = m - k;
+    bool act = row && j >= js && j <= je;
+    if (act) {
+      long long i = cell(M, j, k);
+      CellLoad c = load_cell(M, i);
+      LUCellF64 L = lu_of(C, c, epsv);
+      if (j == je) {
+        long long ia = cell(M, je + 1, k);
+        CellLoad ca = load_cell(M, ia);
+        LUCellF64 La = lu_of(C, ca, epsv);
+        nb_j(ca, La, eps2, -1, prevA);
+        for (int n = 0; n < 4; ++n) prevA.s[n] = M.s[n][ia];
+      }
+      LUNb B;
+      if (k == ke) {
+        long long ib = cell(M, j, ke + 1);
+        CellLoad cb = load_cell(M, ib);
+        LUCellF64 Lb = lu_of(C, cb, epsv);
+        nb_k(cb, Lb, eps2, -1, B);
+        for (int n = 0; n < 4; ++n) B.s[n] = M.s[n][ib];
+      } else {
+        const double* p = rd + (size_t)LU_NB_DBL * (tid + 1);
+        for (int n = 0; n < 4; ++n) B.s[n] = p[n];
+        B.ri1 = p[4]; B.ri2 = p[5]; B.qq = p[6]; B.uu = p[7]; B.vv = p[8]; B.uv2 = p[9]; B.ge = p[10];
+        B.chk = p[11]; B.spec = p[12];
+      }
+      double a[4], b[4];
+      lu_apply(C.gm1, prevA.chk, prevA.spec, prevA.ri1, prevA.ri2, prevA.qq, prevA.uu, prevA.vv, prevA.uv2,
+               prevA.ge, prevA.s, a);
+      lu_apply(C.gm1, B.chk, B.spec, B.ri1, B.ri2, B.qq, B.uu, B.vv, B.uv2, B.ge, B.s, b);
+      double sv = c.ts * 0.5;
+      double di = L.d;
+      double snew[4];
+      for (int n = 0; n < 4; ++n) {
+        snew[n] = M.s[n][i] - sv * (a[n] + b[n]) * di;
+        M.s[n][i] = snew[n];
+      }
+      nb_j(c, L, eps2, -1, prevA);
+      LUNb E;
+      nb_k(c, L, eps2, -1, E);
+      for (int n = 0; n < 4; ++n) prevA.s[n] = snew[n];
+      double* p = wr + (size_t)LU_NB_DBL * tid;
+      for (int n = 0; n < 4; ++n) p[n] = snew[n];
+      p[4] = E.ri1; p[5] = E.ri2; p[6] = E.qq; p[7] = E.uu; p[8] = E.vv; p[9] = E.uv2; p[10] = E.ge;
+      p[11] = E.chk; p[12] = E.spec;
+      // conservative restore (+ update)
+      double q1 = c.q1, q2 = c.q2, q3 = c.q3, q4 = c.q4;
+      q_roundtrip(q1, q2, q3, q4, c.qj);
+      if (fuse_update) {
+        double ibp = (double)imax(M.iblank[i], 0);
+        q1 = q1 + snew[0] * ibp; q2 = q2 + snew[1] * ibp; q3 = q3 + snew[2] * ibp; q4 = q4 + snew[3] * ibp;
+        bad |= asq_bad(C, q1, q2, q3, q4);
+      }
+      M.q[0][i] = q1; M.q[1][i] = q2; M.q[2][i] = q3; M.q[3][i] = q4;
+    }
+    __syncthreads();
+  }
+  // halo ring: restore only (the sweep above has read every halo value it needs)
+  int nring = 2 * jd + 2 * (kd - 2);
+  for (int w = tid; w < nring; w += nt) {
+    int j, kk;
+    if (w < jd) { j = w + 1; kk = 1; }
+    else if (w < 2 * jd) { j = w - jd + 1; kk = kd; }
+    else if (w < 2 * jd + kd - 2) { j = 1; kk = w - 2 * jd + 2; }
+    else { j = jd; kk = w - 2 * jd - (kd - 2) + 2; }
+    long long i = cell(M, j, kk);
+    double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
+    q_roundtrip(q1, q2, q3, q4, M.q[M.nq - 1][i]);
+    M.q[0][i] = q1; M.q[1][i] = q2; M.q[2][i] = q3; M.q[3][i] = q4;
+    if (fuse_update) bad |= asq_bad(C, q1, q2, q3, q4);
+  }
+  if (fuse_update && bad) atomicCAS(M.status, 0, 1);
+}
+
+// q update (src/step.f:205-211) when it cannot be fused
+__global__ __launch_bounds__(256) void k_update(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
+  long long ncell = (long long)nj * nk;
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
+    int j = M.js + (int)(w % nj), k = M.ks + (int)(w / nj);
+    long long i = cell(M, j, k);
+    double ibp = (double)imax(M.iblank[i], 0);
+    for (int n = 0; n < 4; ++n) M.q[n][i] = M.q[n][i] + M.s[n][i] * ibp;
+  }
+}
+
+// monitor (src/step.f:332-392) over all cells
+__global__ __launch_bounds__(256) void k_monitor(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  long long n2 = (long long)M.jd * M.kd;
+  int bad = 0;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x)
+    bad |= asq_bad(M.c, M.q[0][i], M.q[1][i], M.q[2][i], M.q[3][i]);
+  if (bad) atomicCAS(M.status, 0, 1);
+}
+
+__global__ void k_step_begin(int* istep0, int* status, int ncase) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncase && status[i] == 0) istep0[i] += 1;
+}
+
+// ================================================================ geometry
+// initia for one instance (src/init.f:2-177): find_coord_TS (setup.f:348),
+// find_cellcenter (setup.f:111-345), metfv (setup.f:440-724),
+// find_timemetrics_TS (setup.f:727-786), qzero (init.f:209-277), qdivj
+// (init.f:180-206).  One block per instance, phases separated by barriers.
+__global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.x];
+  const DevConst& C = M.c;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int jd = M.jd, kd = M.kd, jmax = M.jmax, kmax = M.kmax;
+  const long long nv2 = (long long)jmax * kmax, n2 = (long long)jd * kd;
+  double* x = M.x;
+  double* y = M.y;
+  double* xv = M.xv;
+  double* yv = M.yv;
+  const double pi = 4.0 * atan(1.0);
+  // find_coord_TS for instances 2..nspec (src/setup.f:348-393)
+  if (M.nsp > 0 && C.timespectral) {
+    double tspec = *M.tspec;
+    if (C.motiontype == 1) {
+      double theta = C.angmax * pi / 180. * (1. - cos(C.rotf * tspec));
+      double ss = sin(theta), cs = cos(theta);
+      for (long long w = t; w < nv2; w += nt) {
+        double a = M.xv0[w], b = M.yv0[w];
+        xv[w] = (a - C.xac) * cs + (b - C.yac) * ss + C.xac;
+        yv[w] = (b - C.yac) * cs - (a - C.xac) * ss + C.yac;
+      }
+    } else if (C.motiontype == 2) {
+      double cs = cos(C.rotf * tspec), ss = sin(C.rotf * tspec);
+      for (long long w = t; w < nv2; w += nt) {
+        double a = M.xv0[w], b = M.yv0[w];
+        xv[w] = (a - C.xac) * cs - (b - C.yac) * ss + C.xac;
+        yv[w] = (b - C.yac) * cs + (a - C.xac) * ss + C.yac;
+      }
+    }
+    __syncthreads();
+  }
+  // cell centres
+  for (long long w = t; w < n2; w += nt) { x[w] = 0.; y[w] = 0.; }
+  __syncthreads();
+#define XV(j, k) xv[vert(M, j, k)]
+#define YV(j, k) yv[vert(M, j, k)]
+#define XC(j, k) x[cell(M, j, k)]
+#define YC(j, k) y[cell(M, j, k)]
+  {
+    int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
+    for (long long w = t; w < (long long)nj * nk; w += nt) {
+      int j = M.js + (int)(w % nj), k = M.ks + (int)(w / nj);
+      XC(j, k) = 0.25 * (XV(j, k) + XV(j - 1, k) + XV(j, k - 1) + XV(j - 1, k - 1));
+      YC(j, k) = 0.25 * (YV(j, k) + YV(j - 1, k) + YV(j, k - 1) + YV(j - 1, k - 1));
+    }
+  }
+  __syncthreads();
+  for (int k = M.ks + t; k <= M.ke; k += nt) {
+    int j = M.js - 1;
+    XC(j, k) = XC(j + 1, k) - 0.5 * (XV(j + 1, k) - XV(j, k) + XV(j + 1, k - 1) - XV(j, k - 1));
+    YC(j, k) = YC(j + 1, k) - 0.5 * (YV(j + 1, k) - YV(j, k) + YV(j + 1, k - 1) - YV(j, k - 1));
+    j = M.je + 1;
+    XC(j, k) = XC(j - 1, k) - 0.5 * (XV(j - 2, k) - XV(j - 1, k) + XV(j - 2, k - 1) - XV(j - 1, k - 1));
+    YC(j, k) = YC(j - 1, k) - 0.5 * (YV(j - 2, k) - YV(j - 1, k) + YV(j - 2, k - 1) - YV(j - 1, k - 1));
+  }
+  for (int j = M.js + t; j <= M.je; j += nt) {
+    int k = M.ks - 1;
+    XC(j, k) = XC(j, k + 1) - 0.5 * (XV(j, k + 1) - XV(j, k) + XV(j - 1, k + 1) - XV(j - 1, k));
+    YC(j, k) = YC(j, k + 1) - 0.5 * (YV(j, k + 1) - YV(j, k) + YV(j - 1, k + 1) - YV(j - 1, k));
+    k = M.ke + 1;
+    XC(j, k) = XC(j, k - 1) - 0.5 * (XV(j, k - 2) - XV(j, k - 1) + XV(j - 1, k - 2) - XV(j - 1, k - 1));
+    YC(j, k) = YC(j, k - 1) - 0.5 * (YV(j, k - 2) - YV(j, k - 1) + YV(j - 1, k - 2) - YV(j - 1, k - 1));
+  }
+  __syncthreads();
+  if (t == 0) {
+    int j = M.js - 1, k = M.ks - 1;
+    XC(j, k) = 2 * XC(j + 1, k) - XC(j + 2, k);
+    YC(j, k) = 2 * YC(j + 1, k) - YC(j + 2, k);
+    k = M.ke + 1;
+    XC(j, k) = 2 * XC(j + 1, k) - XC(j + 2, k);
+    YC(j, k) = 2 * YC(j + 1, k) - YC(j + 2, k);
+    j = M.je + 1; k = M.ks - 1;
+    XC(j, k) = 2 * XC(j, k + 1) - XC(j, k + 2);
+    YC(j, k) = 2 * YC(j, k + 1) - YC(j, k + 2);
+    k = M.ke + 1;
+    XC(j, k) = 2 * XC(j, k - 1) - XC(j, k - 2);
+    YC(j, k) = 2 * YC(j, k - 1) - YC(j, k - 2);
+  }
+  __syncthreads();
+  for (int ib = 0; ib < M.nbc; ++ib) {
+    const DevBC& b = M.bc[ib];
+    int js = b.js, je = b.je, ks = b.ks, ke = b.ke, idir = b.idir;
+    int nj = je - js + 1, nk = ke - ks + 1;
+    if (b.type == 22) {
+      for (int w = t; w < nj * nk; w += nt) {
+        int j = js + w % nj, k = ks + w / nj;
+        if (idir == 1 || idir == -1) {
+          int jc = periodic_src(idir, js, je, ks, ke, jd, kd, j);
+          XC(j, k) = XC(jc, k); YC(j, k) = YC(jc, k);
+        } else {
+          int kc = periodic_src(idir, js, je, ks, ke, jd, kd, k);
+          XC(j, k) = XC(j, kc); YC(j, k) = YC(j, kc);
+        }
+      }
+    } else if (b.type == 51 && (idir == 2 || idir == -2)) {
+      int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+      for (int j = js + t; j <= je; j += nt) {
+        int jj = jd - j + 1;
+        double a = XC(jj, k1), bb = YC(jj, k1), c = XC(j, k1), d = YC(j, k1);
+        XC(j, k) = a; YC(j, k) = bb; XC(jj, k) = c; YC(jj, k) = d;
+      }
+    }
+    __syncthreads();
+  }
+  // ---- metfv
+  double* qj = M.q[M.nq - 1];
+  for (long long w = t; w < (long long)(jmax - 1) * (kmax - 1); w += nt) {
+    int j = 2 + (int)(w % (jmax - 1)), k = 2 + (int)(w / (jmax - 1));
+    long long i = cell(M, j, k);  // jc = j, kc = k with one halo
+    double dx1 = XV(j, k) - XV(j - 1, k - 1);
+    double dy1 = YV(j, k) - YV(j - 1, k - 1);
+    double dx2 = XV(j - 1, k) - XV(j, k - 1);
+    double dy2 = YV(j - 1, k) - YV(j, k - 1);
+    double a3 = 0.5 * (dx1 * dy2 - dx2 * dy1);
+    double J = 1. / a3;
+    qj[i] = J;
+    double xx = 0.5 * (YV(j, k) - YV(j, k - 1) + YV(j - 1, k) - YV(j - 1, k - 1));
+    double xy = -0.5 * (XV(j, k) - XV(j, k - 1) + XV(j - 1, k) - XV(j - 1, k - 1));
+    double yx = -0.5 * (YV(j, k) - YV(j - 1, k) + YV(j, k - 1) - YV(j - 1, k - 1));
+    double yy = 0.5 * (XV(j, k) - XV(j - 1, k) + XV(j, k - 1) - XV(j - 1, k - 1));
+    M.xx[i] = J * xx; M.xy[i] = J * xy; M.yx[i] = J * yx; M.yy[i] = J * yy;
+  }
+  __syncthreads();
+#define CPY(dst, src) { qj[dst] = qj[src]; M.xx[dst] = M.xx[src]; M.xy[dst] = M.xy[src]; \
+    M.yx[dst] = M.yx[src]; M.yy[dst] = M.yy[src]; }
+  for (int k = M.ks + t; k <= M.ke; k += nt) {
+    CPY(cell(M, M.js - 1, k), cell(M, M.js, k));
+    CPY(cell(M, M.je + 1, k), cell(M, M.je, k));
+  }
+  __syncthreads();
+  for (int j = M.js - 1 + t; j <= M.je + 1; j += nt) {
+    CPY(cell(M, j, M.ks - 1), cell(M, j, M.ks));
+    CPY(cell(M, j, M.ke + 1), cell(M, j, M.ke));
+  }
+  __syncthreads();
+  for (int ib = 0; ib < M.nbc; ++ib) {
+    const DevBC& b = M.bc[ib];
+    int js = b.js, je = b.je, ks = b.ks, ke = b.ke, idir = b.idir;
+    int nj = je - js + 1, nk = ke - ks + 1;
+    if (b.type == 22) {
+      for (int w = t; w < nj * nk; w += nt) {
+        int j = js + w % nj, k = ks + w / nj;
+        if (idir == 1 || idir == -1) {
+          int jc = periodic_src(idir, js, je, ks, ke, jd, kd, j);
+          CPY(cell(M, j, k), cell(M, jc, k));
+        } else {
+          int kc = periodic_src(idir, js, je, ks, ke, jd, kd, k);
+          CPY(cell(M, j, k), cell(M, j, kc));
+        }
+      }
+    } else if (b.type == 51 && (idir == 2 || idir == -2)) {
+      int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+      for (int j = js + t; j <= je; j += nt) {
+        int jj = jd - j + 1;
+        long long a = cell(M, j, k), bb = cell(M, jj, k1), cc = cell(M, jj, k), d = cell(M, j, k1);
+        double v0 = qj[bb], v1 = -M.xx[bb], v2 = -M.xy[bb], v3 = -M.yx[bb], v4 = -M.yy[bb];
+        double w0 = qj[d], w1 = -M.xx[d], w2 = -M.xy[d], w3 = -M.yx[d], w4 = -M.yy[d];
+        qj[a] = v0; M.xx[a] = v1; M.xy[a] = v2; M.yx[a] = v3; M.yy[a] = v4;
+        qj[cc] = w0; M.xx[cc] = w1; M.xy[cc] = w2; M.yx[cc] = w3; M.yy[cc] = w4;
+      }
+    }
+    __syncthreads();
+  }
+#undef CPY
+  // ---- find_timemetrics_TS (src/setup.f:727-786)
+  if (C.timespectral) {
+    double tspec = *M.tspec, rotf = C.rotf, am = C.angmax;
+    if (C.motiontype == 1) {
+      for (long long i = t; i < n2; i += nt) {
+        M.ug[i] = am * pi / 180. * rotf * (y[i] - C.yac) * sin(rotf * tspec);
+        M.vg[i] = -am * pi / 180. * rotf * (x[i] - C.xac) * sin(rotf * tspec);
+      }
+      for (long long i = t; i < nv2; i += nt) {
+        M.ugv[i] = am * pi / 180. * rotf * (yv[i] - C.yac) * sin(rotf * tspec);
+        M.vgv[i] = -am * pi / 180. * rotf * (xv[i] - C.xac) * sin(rotf * tspec);
+      }
+    } else if (C.motiontype == 2) {
+      for (long long i = t; i < n2; i += nt) { M.ug[i] = -rotf * (y[i] - C.yac); M.vg[i] = rotf * (x[i] - C.xac); }
+      for (long long i = t; i < nv2; i += nt) { M.ugv[i] = -rotf * (yv[i] - C.yac); M.vgv[i] = rotf * (xv[i] - C.xac); }
+    }
+  } else {
+    for (long long i = t; i < n2; i += nt) { M.ug[i] = 0.; M.vg[i] = 0.; }
+  }
+  // ---- qzero + qdivj
+  double ruinf = C.rinf * C.uinf, rvinf = C.rinf * C.vinf;
+  for (long long i = t; i < n2; i += nt) {
+    double J = qj[i];
+    M.q[0][i] = C.rinf / J;
+    M.q[1][i] = ruinf / J;
+    M.q[2][i] = rvinf / J;
+    M.q[3][i] = C.einf / J;
+    if (C.iturb == 1) M.q[4][i] = C.vnuinf;
+    M.turmu[i] = 0.;
+    M.iblank[i] = 1;
+    M.bt[i] = 1.;
+    M.tscale[i] = 0.;
+    for (int n = 0; n < M.nv; ++n) M.s[n][i] = 0.;
+  }
+#undef XV
+#undef YV
+#undef XC
+#undef YC
+}
+
+// ================================================================== forces
+// compute_forces/force2d (src/forces.f:2-398) for one instance: a single
+// block reduces the wall integrals in a fixed order.
+__global__ __launch_bounds__(256) void k_forces(const DevMesh* __restrict__ Ms, int mi, double* out) {
+  const DevMesh& M = Ms[mi];
+  const DevConst& C = M.c;
+  __shared__ double red[8][256];
+  const int t = threadIdx.x;
+  const int k = M.ks, k2 = k + 1, k3 = k2 + 1;
+  const int j1 = M.jtail1, j2 = M.jtail2;
+  const double* q1 = M.q[0];
+  const double* q2 = M.q[1];
+  const double* q3 = M.q[2];
+  const double* q4 = M.q[3];
+  const double* qj = M.q[M.nq - 1];
+  const double fmm = C.fmm, halfv = C.half;
+  double gm1 = C.gm1, gamma = C.gamma;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // cn, cc, cmle, cpinv, cnv, ccv, cmlev, cpv
+  double amu = C.rinf * 1. / C.rey;
+  double uinf2 = sq(fmm);
+  auto U = [&](int j, int kk) { long long i = cell(M, j, kk); return q2[i] / q1[i]; };
+  auto V = [&](int j, int kk) { long long i = cell(M, j, kk); return q3[i] / q1[i]; };
+  for (int j = j1 + t; j <= j2; j += 256) {
+    long long i = cell(M, j, k), im1 = cell(M, j, k - 1);
+    double rr = 1. / q1[i];
+    double rho = q1[i] * qj[i];
+    double u = q2[i] * rr, v = q3[i] * rr, e = q4[i] * qj[i];
+    double vsq = u * u + v * v;
+    double pp = gm1 * (e - rho * vsq / 2.);
+    double pp1 = pp / C.pinf;
+    double cp = 2. * (pp1 - 1.) / (gamma * sq(fmm));
+    double zz = cp;
+    rr = 1. / q1[im1];
+    rho = q1[im1] * qj[im1];
+    u = q2[im1] * rr; v = q3[im1] * rr; e = q4[im1] * qj[im1];
+    vsq = u * u + v * v;
+    pp = gm1 * (e - rho * vsq / 2.);
+    pp1 = pp / C.pinf;
+    cp = 2. * (pp1 - 1.) / (gamma * sq(fmm));
+    zz = 0.5 * (zz + cp);
+    int jv = j - 1, kv = k - 1;
+    double xa = M.xv[vert(M, jv, kv)], xb = M.xv[vert(M, jv + 1, kv)];
+    double ya = M.yv[vert(M, jv, kv)], yb = M.yv[vert(M, jv + 1, kv)];
+    acc[0] += -zz * (xb - xa);
+    acc[1] += zz * (yb - ya);
+    acc[2] += zz * (0.5 * (xb + xa) * (xb - xa) + 0.5 * (yb + ya) * (yb - ya));
+    double uu = q2[i] / q1[i], vv = q3[i] / q1[i];
+    double um = q2[im1] / q1[im1], vm = q3[im1] / q1[im1];
+    acc[3] += 0.5 * (uu + um) * zz * (yb - ya) - 0.5 * (vv + vm) * zz * (xb - xa);
+    if (!C.invisc) {
+      double uxi, vxi;
+      if (j == j1) { uxi = U(j + 1, k) - U(j, k); vxi = V(j + 1, k) - V(j, k); }
+      else if (j == j2) { uxi = U(j, k) - U(j - 1, k); vxi = V(j, k) - V(j - 1, k); }
+      else { uxi = .5 * (U(j + 1, k) - U(j - 1, k)); vxi = .5 * (V(j + 1, k) - V(j - 1, k)); }
+      double ueta = -1.5 * U(j, k) + 2. * U(j, k2) - .5 * U(j, k3);
+      double veta = -1.5 * V(j, k) + 2. * V(j, k2) - .5 * V(j, k3);
+      double xix = M.xx[i], xiy = M.xy[i], etax = M.yx[i], etay = M.yy[i];
+      double sfdiv = 2. / 3 * (uxi * xix + ueta * etax + vxi * xiy + veta * etay);
+      double sxx = amu * (2. * (uxi * xix + ueta * etax) - sfdiv);
+      double syy = amu * (2. * (vxi * xiy + veta * etay) - sfdiv);
+      double tauw = amu * ((uxi * xiy + ueta * etay) + (vxi * xix + veta * etax));
+      sxx = sxx / (.5 * C.rinf * uinf2);
+      syy = syy / (.5 * C.rinf * uinf2);
+      double zv = tauw / (.5 * C.rinf * uinf2);
+      acc[5] += zv * (xb - xa) - sxx * (yb - ya);
+      acc[4] += -zv * (yb - ya) + syy * (xb - xa);
+      acc[6] += zv * (0.5 * (xb + xa) * (yb - ya) + 0.5 * (yb + ya) * (xb - xa))
+                - 0.5 * (xb + xa) * (xb - xa) * syy - 0.5 * (yb + ya) * (yb - ya) * sxx;
+      acc[7] += 0.5 * (uu + um) * zv * (xb - xa) - 0.5 * (uu + um) * sxx * (yb - ya)
+                - 0.5 * (vv + vm) * zv * (yb - ya) + 0.5 * (vv + vm) * syy * (xb - xa);
+    }
+  }
+  for (int n = 0; n < 8; ++n) red[n][t] = acc[n];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+      for (int n = 0; n < 8; ++n) red[n][t] += red[n][t + off];
+    __syncthreads();
+  }
+  if (t == 0) {
+    double cn = red[0][0], cc = red[1][0], cmle = red[2][0], cpinv = red[3][0];
+    double cnv = red[4][0], ccv = red[5][0], cmlev = red[6][0], cpv = red[7][0];
+    cn = cn - halfv * cn; cmle = cmle - halfv * cmle; cc = cc + halfv * cc;
+    cnv = cnv - halfv * cnv; cmlev = cmlev - halfv * cmlev; ccv = ccv + halfv * ccv;
+    double cfx = cc + ccv, cfy = cn + cnv, cm0 = cmle + cmlev;
+    double cpower = (cpinv + cpv) / fmm;
+    double ca = C.ca, sa = C.sa;
+    out[0] = -cfx * sa + cfy * ca;
+    out[1] = cfx * ca + cfy * sa;
+    out[2] = cm0 - 0.0 * cfx + 0.25 * cfy;
+    out[3] = cpower;
+    out[4] = cfx;
+    out[5] = cfy;
+  }
+}
+
+// ================================================================ launchers
+static inline int nblk(long long n, int per) {
+  long long b = (n + per - 1) / per;
+  return (int)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_geometry, dim3(nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  hipLaunchKernelGGL(k_time_step, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st) {
+  hipLaunchKernelGGL(k_bc, dim3(nmi), dim3(256), 0, st, dM, zero_s);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, hipStream_t st) {
+  hipLaunchKernelGGL(k_residual, dim3(ntiles), dim3(RES_THREADS), 0, st, dM, tiles, do_norms);
+  return hipGetLastError();
+}
+
+hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  hipLaunchKernelGGL(k_tsrhs, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st) {
+  hipLaunchKernelGGL(k_rbc, dim3(nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+static int lu_threads(const DevMesh* hM, int nmi) {
+  int mx = 1;
+  for (int i = 0; i < nmi; ++i) {
+    int nk = hM[i].ke - hM[i].ks + 1;
+    mx = nk > mx ? nk : mx;
+  }
+  int nt = ((mx + 63) / 64) * 64;
+  return nt;
+}
+
+hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, hipStream_t st) {
+  (void)fp32;
+  int nt = lu_threads(hM, nmi);
+  if (nt > 1024) return hipErrorInvalidValue;
+  size_t lds = sizeof(double) * LU_NB_DBL * 2 * (size_t)nt;
+  hipLaunchKernelGGL(k_lu_fwd, dim3(nmi), dim3(nt), lds, st, dM);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lu_bwd, dim3(nmi), dim3(nt), lds, st, dM, fuse_update);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  hipLaunchKernelGGL(k_update, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  hipLaunchKernelGGL(k_monitor, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_begin(int* istep0, int* status, int ncase, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_begin, dim3((ncase + 255) / 256), dim3(256), 0, st, istep0, status, ncase);
+  return hipGetLastError();
+}
+
+hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, st, dM, mi, out);
+  return hipGetLastError();
+}
+
+}  // namespace ot2d
