@@ -47,6 +47,8 @@ struct MeshBuf {
   double *ug = nullptr, *vg = nullptr, *turmu = nullptr, *tscale = nullptr, *bt = nullptr;
   double *xv = nullptr, *yv = nullptr, *ugv = nullptr, *vgv = nullptr;
   int* iblank = nullptr;
+  double* lu = nullptr;
+  size_t lu_per = 0;  // doubles of LU workspace per instance
   bool has_grid = false;
 };
 
@@ -243,7 +245,7 @@ static void free_case(ot2d_case* c) {
     dfree(mb.q); dfree(mb.s); dfree(mb.x); dfree(mb.y); dfree(mb.xx); dfree(mb.xy);
     dfree(mb.yx); dfree(mb.yy); dfree(mb.ug); dfree(mb.vg); dfree(mb.turmu);
     dfree(mb.tscale); dfree(mb.bt); dfree(mb.xv); dfree(mb.yv); dfree(mb.ugv);
-    dfree(mb.vgv); dfree(mb.iblank);
+    dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu);
   }
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
   dfree(c->d_partials); dfree(c->d_counter); dfree(c->d_norms); dfree(c->d_tspec);
@@ -315,7 +317,16 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
     mb.d = md[im];
     if (md[im].jmax < 5 || md[im].kmax < 5) { free_case(c); return fail(OT2D_ERR_ARG, "mesh %d too small", im); }
     if (md[im].nbc < 0 || md[im].nbc > OT2D_MAXBC) { free_case(c); return fail(OT2D_ERR_ARG, "nbc"); }
-    if (md[im].kmax - 1 > 1024) { free_case(c); return fail(OT2D_ERR_UNSUPPORTED, "kmax > 1025"); }
+    if (md[im].kmax - 1 > 256) {
+      free_case(c);
+      return fail(OT2D_ERR_UNSUPPORTED, "kmax=%d: the LU-SGS sweep holds one row per lane, at most 256 rows",
+                  md[im].kmax);
+    }
+    if (lu_lds_need(md[im].kmax - 1, md[im].jmax - 1, p->lu_fp32) > 160 * 1024) {
+      free_case(c);
+      return fail(OT2D_ERR_UNSUPPORTED, "mesh %d (%d x %d) exceeds the LU-SGS sweep's LDS staging", im,
+                  md[im].jmax, md[im].kmax);
+    }
     mb.jmax = md[im].jmax; mb.kmax = md[im].kmax;
     mb.jd = mb.jmax + 1; mb.kd = mb.kmax + 1;
     mb.n2 = (size_t)mb.jd * mb.kd;
@@ -326,6 +337,8 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
     A(mb.turmu, mb.n2 * ns); A(mb.tscale, mb.n2 * ns); A(mb.bt, mb.n2 * ns);
     A(mb.xv, mb.nv2 * ns); A(mb.yv, mb.nv2 * ns); A(mb.ugv, mb.nv2 * ns); A(mb.vgv, mb.nv2 * ns);
     A(mb.iblank, mb.n2 * ns);
+    mb.lu_per = lu_workspace_bytes(mb.jd, mb.kd) / sizeof(double);
+    A(mb.lu, mb.lu_per * ns);
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
     ntile_total += tj * tk * ns;
   }
@@ -373,8 +386,10 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
       M.xv = mb.xv + ov; M.yv = mb.yv + ov; M.ugv = mb.ugv + ov; M.vgv = mb.vgv + ov;
       M.xv0 = mb.xv; M.yv0 = mb.yv;
       M.iblank = mb.iblank + o;
+      M.lu = mb.lu + mb.lu_per * nsp;
       M.status = c->d_status;
       M.istep0 = c->d_istep0;
+      M.stepc = c->d_stepc;
       M.counter = c->d_counter + mi;
       M.norms = c->d_norms + 6 * mi;
       M.tspec = c->d_tspec + nsp;
@@ -589,7 +604,7 @@ extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
 
 // one step (src/step.f:2-218) + monitor, enqueued without synchronisation
 static int32_t enqueue_step(ot2d_case* c) {
-  HIPCHK(launch_step_begin(c->d_istep0, c->d_status, 1, c->stream));
+  HIPCHK(launch_step_begin(c->d_istep0, c->d_status, c->d_stepc, 1, c->stream));
   HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream));
   if (int32_t rc = enqueue_residual(c, 1, 1)) return rc;
   HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));
@@ -607,6 +622,7 @@ extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
   if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int), c->stream));
   if (int32_t rc = enqueue_step(c)) return rc;
   int st = 0;
   if (int32_t rc = read_status(c, &st)) return rc;
@@ -644,30 +660,35 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
   if (!c || niter < 0) return fail(OT2D_ERR_ARG, "bad arguments");
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
-  if (int32_t rc = clear_status(c)) return rc;
-  if (int32_t rc = ensure_graph(c)) return rc;
   int per = nmi(c);
-  int st = 0, it = 0;
-  std::vector<double> nb(6 * per);
-  // replay in chunks; the residual history needs the norms of every step,
-  // so copy them out after each replay (device-to-host, async, pinned-free)
-  for (it = 0; it < niter; ++it) {
-    HIPCHK(hipGraphLaunch(c->graph, c->stream));
-    if (hist) {
-      HIPCHK(hipMemcpyAsync(nb.data(), c->d_norms, sizeof(double) * nb.size(), hipMemcpyDeviceToHost, c->stream));
-      if (int32_t rc = read_status(c, &st)) return rc;
-      for (int o = 0; o < per; ++o) hist[(size_t)it * per + o] = nb[6 * o];
-      if (st) { ++it; break; }
+  // device-side residual history: one rsum per step and instance
+  if (hist && niter > c->hist_cap) {
+    dfree(c->d_hist);
+    c->d_hist = nullptr;
+    hipError_t e = dalloc(&c->d_hist, (size_t)niter * per);
+    if (e != hipSuccess) return fail(OT2D_ERR_NOMEM, "history buffer: %s", hipGetErrorString(e));
+    c->hist_cap = niter;
+  }
+  double* want = hist ? c->d_hist : nullptr;
+  if (c->hM[0].hist != want) {
+    for (int i = 0; i < per; ++i) {
+      c->hM[i].hist = want;
+      c->hM[i].hist_cap = want ? c->hist_cap : 0;
+      c->hM[i].hist_stride = per;
+      c->hM[i].hist_slot = i;
     }
+    HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * per, hipMemcpyHostToDevice, c->stream));
   }
-  if (!hist) {
-    if (int32_t rc = read_status(c, &st)) return rc;
-  }
-  int i0 = 0;
-  HIPCHK(hipMemcpy(&i0, c->d_istep0, sizeof(int), hipMemcpyDeviceToHost));
-  if (!hist) it = i0 - c->istep0;
-  c->istep0 = i0;
-  if (done) *done = it;
+  if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int), c->stream));
+  if (int32_t rc = ensure_graph(c)) return rc;
+  for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  int st = 0, steps = 0;
+  HIPCHK(hipMemcpyAsync(&steps, c->d_stepc, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = read_status(c, &st)) return rc;
+  if (hist && steps > 0) HIPCHK(hipMemcpy(hist, c->d_hist, sizeof(double) * (size_t)steps * per, hipMemcpyDeviceToHost));
+  c->istep0 += steps;
+  if (done) *done = steps;
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;

@@ -48,13 +48,16 @@ struct DevMesh {
   double *xv, *yv, *ugv, *vgv;  // vertex arrays (jmax,kmax) of this instance
   const double *xv0, *yv0;      // instance-1 vertices (time-spectral rotation)
   int* iblank;
-  double* sdt;               // (jd,kd) scratch: ilu2d forward-sweep result
+  double* lu;                // LU-SGS hyperplane-major records (workspace)
   // per-case bookkeeping (shared by all instances of a case)
   int* status;               // 0 ok, 1 mstop, 2 norm
   int* istep0;               // device copy of istep0
   double* partials;          // norm partials (res_nblocks * 4 doubles)
   unsigned int* counter;     // last-block counter for the norm reduction
   double* norms;             // rsum, resrho, resmax, jmax, kmax, nmax
+  int* stepc;                // steps started in the current ot2d_iterate call
+  double* hist;              // residual history rsum[step][slot] (may be null)
+  int hist_cap, hist_stride, hist_slot;
   double* tspec;             // time instant of this instance
   const double* Ds;          // spectral derivative matrix (nspec x nspec)
   double* qall[OT2D_MAXQ];   // q planes of instance 0 (TS coupling base)
@@ -95,7 +98,9 @@ hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_upd
                      hipStream_t st);
 hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
-hipError_t launch_step_begin(int* istep0, int* status, int ncase, hipStream_t st);
+hipError_t launch_step_begin(int* istep0, int* status, int* stepc, int ncase, hipStream_t st);
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+size_t lu_workspace_bytes(int jd, int kd);
+size_t lu_lds_need(int nk, int ni, int fp32);
 }  // namespace ot2d

@@ -630,17 +630,33 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
   }
   __syncthreads();
   if (!last) return;
-  // last tile of this instance: reduce the partials in tile order
-  if (tid == 0) {
-    __threadfence();
-    double rsum = 0.0, resrho = 0.0, resmax = 0.0, kbest = 1e300;
-    volatile double* P = M.partials;
-    for (int b = 0; b < M.res_nblocks; ++b) {
-      rsum = rsum + P[4 * b + 0];
-      resrho = resrho + P[4 * b + 1];
-      double a = P[4 * b + 2], kb = P[4 * b + 3];
-      if (a > resmax || (a == resmax && kb < kbest)) { resmax = a; kbest = kb; }
+  // last tile of this instance: reduce the partials (fixed order: strided
+  // per-thread sums, then the same LDS tree as above)
+  __threadfence();
+  {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 1e300;
+    const volatile double* P = M.partials;
+    for (int b = tid; b < M.res_nblocks; b += RES_THREADS) {
+      a0 = a0 + P[4 * b + 0];
+      a1 = a1 + P[4 * b + 1];
+      double x = P[4 * b + 2], kb = P[4 * b + 3];
+      if (x > a2 || (x == a2 && kb < a3)) { a2 = x; a3 = kb; }
     }
+    red[0][tid] = a0; red[1][tid] = a1; red[2][tid] = a2; red[3][tid] = a3;
+  }
+  __syncthreads();
+  for (int off = RES_THREADS / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      red[0][tid] = red[0][tid] + red[0][tid + off];
+      red[1][tid] = red[1][tid] + red[1][tid + off];
+      double a = red[2][tid], b = red[2][tid + off];
+      double ka = red[3][tid], kb = red[3][tid + off];
+      if (b > a || (b == a && kb < ka)) { red[2][tid] = b; red[3][tid] = kb; }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double rsum = red[0][0], resrho = red[1][0], resmax = red[2][0], kbest = red[3][0];
     double volum = (double)(jd * kd);
     resrho = sqrt(resrho / volum);
     rsum = sqrt(rsum / volum);
@@ -654,6 +670,10 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
     }
     double* N = M.norms;
     N[0] = rsum; N[1] = resrho; N[2] = resmax; N[3] = jm; N[4] = km; N[5] = nm;
+    if (M.hist && M.stepc) {
+      int sc = *M.stepc;
+      if (sc >= 1 && sc <= M.hist_cap) M.hist[(long long)(sc - 1) * M.hist_stride + M.hist_slot] = rsum;
+    }
     *M.counter = 0u;
     if (rsum > 1000.0) atomicCAS(M.status, 0, 2);
   }
@@ -871,107 +891,302 @@ __device__ __forceinline__ void nb_k(const CellLoad& c, const LUCellF64& L, doub
   o.ri1 = c.yx; o.ri2 = c.yy; o.qq = L.qy; o.uu = L.uu; o.vv = L.vv; o.uv2 = L.uv2; o.ge = L.ge;
 }
 
-// Forward sweep of ilu2d.  One block per instance; thread t owns row
-// k = ks + t and visits cell (m-k, k) on hyperplane m = j + k.  The j-1
-// neighbour is the thread's own previous cell (kept in registers), the k-1
-// neighbour comes from thread t-1's previous cell through LDS.  The prologue
-// applies s *= tscale (src/step.f:141-148) to the halo ring and the
-// point-Jacobi update of the four halo sides (src/lhs.f:568-598).
-constexpr int LU_NB_DBL = 13;
+// ----------------------------------------------------------------------
+// LU-SGS in three kernels.
+//
+//  k_lu_prep    every cell in parallel: the quantities ilu2d derives per cell
+//               (src/lhs.f:524-564), s*tscale (src/step.f:141-148) and the
+//               point-Jacobi update of the halo sides (src/lhs.f:568-598),
+//               stored as one record per cell in hyperplane-major order:
+//               entry e = p*R + r with p = j+k-(js+ks-2) the hyperplane and
+//               r = k-ks+1 the row, so a sweep reads a hyperplane with unit
+//               stride.  Records are packed in 16-byte vectors.
+//  k_lu_sweep   one block per instance, one lane per row k, hyperplanes in
+//               order (src/lhs.f:616-714 forward, 719-800 backward).  A lane
+//               streams only its own cell's record (prefetched D
+//               hyperplanes ahead); the j neighbour is its own previous cell
+//               (registers), the k neighbour the adjacent lane's previous
+//               cell (LDS, one LDS-only barrier per hyperplane), and the
+//               halo row/column records are staged in LDS once.
+//  k_lu_finish  every cell in parallel: conservative restore of the in-place
+//               primitive conversion (src/lhs.f:805-811), the reference's s
+//               after ilu2d, and optionally the q update (src/step.f:205-211)
+//               and monitor (src/step.f:353-385).
+//
+// T is the arithmetic of the sweep: double reproduces the reference, float
+// is the mixed-precision defect-correction variant (records rounded from the
+// FP64 values computed in k_lu_prep).
+// ----------------------------------------------------------------------
 
-__global__ void k_lu_fwd(const DevMesh* __restrict__ Ms) {
-  extern __shared__ double lds[];
-  const DevMesh& M = Ms[blockIdx.x];
+// one cell's record as the sweeps use it (fields of one sweep direction)
+template <typename T>
+struct LURec {
+  T s[4], sv, di, uu, vv, uv2, ge, xx, xy, yx, yy, qx, qy, cj, pj, ck, pk;
+};
+
+// record packing: FP64 uses 12 double2 vectors, FP32 6 float4 vectors
+//  FP64  V0 (s0,s1)  V1 (s2,s3)  V2 (sv,di)  V3 (uu,vv)  V4 (uv2,ge)  V5 (xx,xy)
+//        V6 (yx,yy)  V7 (qx,qy)  V8 (cjf,pjf) V9 (ckf,pkf) V10 (cjb,pjb) V11 (ckb,pkb)
+//  FP32  W0 (s0..s3) W1 (sv,di,uu,vv) W2 (uv2,ge,xx,xy) W3 (yx,yy,qx,qy)
+//        W4 (cjf,pjf,ckf,pkf) W5 (cjb,pjb,ckb,pkb)
+template <typename T> struct LUPack;
+template <> struct LUPack<double> { static constexpr int NV = 12; };
+template <> struct LUPack<float> { static constexpr int NV = 6; };
+
+__host__ __device__ inline int lu_rows(const DevMesh& M) { return M.ke - M.ks + 3; }
+__host__ __device__ inline int lu_planes(const DevMesh& M) { return M.je + M.ke - M.js - M.ks + 5; }
+
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+struct D2 { double x, y; };
+
+// buffer descriptor built from wave-uniform values (keeps it in SGPRs, so
+// buffer accesses need no waterfall loop)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(void* p, long long bytes) {
+  unsigned long long v = (unsigned long long)p;
+  unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  void* up = (void*)(((unsigned long long)hi << 32) | lo);
+  int nb = __builtin_amdgcn_readfirstlane((int)bytes);
+  return __builtin_amdgcn_make_buffer_rsrc(up, 0, nb, 0x00020000);
+}
+struct F4 { float x, y, z, w; };
+
+__device__ __forceinline__ D2 ld2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ F4 ld4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(F4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ void st2(D2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void st4(F4 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, voff, soff, 0);
+}
+
+// one buffer descriptor per vector plane (wave-uniform, so loads carry no
+// offset arithmetic); vo = entry index * 16
+template <typename T>
+struct LUBufs {
+  __amdgpu_buffer_rsrc_t v[LUPack<T>::NV];
+};
+template <typename T>
+__device__ __forceinline__ LUBufs<T> lu_bufs(void* base, long long ne) {
+  LUBufs<T> b;
+#pragma unroll
+  for (int q = 0; q < LUPack<T>::NV; ++q) b.v[q] = uniform_rsrc((char*)base + (long long)q * ne * 16, ne * 16);
+  return b;
+}
+
+template <int DIR>
+__device__ __forceinline__ void rec_load(const LUBufs<double>& B, int vo, LURec<double>& o) {
+  D2 a = ld2(B.v[0], vo, 0), b = ld2(B.v[1], vo, 0), c = ld2(B.v[2], vo, 0), d = ld2(B.v[3], vo, 0);
+  D2 e = ld2(B.v[4], vo, 0), f = ld2(B.v[5], vo, 0), g = ld2(B.v[6], vo, 0), h = ld2(B.v[7], vo, 0);
+  D2 u = ld2(B.v[DIR > 0 ? 8 : 10], vo, 0), w = ld2(B.v[DIR > 0 ? 9 : 11], vo, 0);
+  o.s[0] = a.x; o.s[1] = a.y; o.s[2] = b.x; o.s[3] = b.y; o.sv = c.x; o.di = c.y; o.uu = d.x; o.vv = d.y;
+  o.uv2 = e.x; o.ge = e.y; o.xx = f.x; o.xy = f.y; o.yx = g.x; o.yy = g.y; o.qx = h.x; o.qy = h.y;
+  o.cj = u.x; o.pj = u.y; o.ck = w.x; o.pk = w.y;
+}
+template <int DIR>
+__device__ __forceinline__ void rec_load(const LUBufs<float>& B, int vo, LURec<float>& o) {
+  F4 a = ld4(B.v[0], vo, 0), b = ld4(B.v[1], vo, 0), c = ld4(B.v[2], vo, 0), d = ld4(B.v[3], vo, 0);
+  F4 u = ld4(B.v[DIR > 0 ? 4 : 5], vo, 0);
+  o.s[0] = a.x; o.s[1] = a.y; o.s[2] = a.z; o.s[3] = a.w;
+  o.sv = b.x; o.di = b.y; o.uu = b.z; o.vv = b.w;
+  o.uv2 = c.x; o.ge = c.y; o.xx = c.z; o.xy = c.w;
+  o.yx = d.x; o.yy = d.y; o.qx = d.z; o.qy = d.w;
+  o.cj = u.x; o.pj = u.y; o.ck = u.z; o.pk = u.w;
+}
+__device__ __forceinline__ void rec_store_s(const LUBufs<double>& B, int vo, const double* s) {
+  st2(D2{s[0], s[1]}, B.v[0], vo, 0);
+  st2(D2{s[2], s[3]}, B.v[1], vo, 0);
+}
+__device__ __forceinline__ void rec_store_s(const LUBufs<float>& B, int vo, const float* s) {
+  st4(F4{s[0], s[1], s[2], s[3]}, B.v[0], vo, 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const DevConst& C = M.c;
-  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke, jd = M.jd, kd = M.kd;
-  const int nk = ke - ks + 1;
+  const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
   const double eps2 = C.epse * 2.;
   const double epsv = 1. + eps2;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  // ---- prologue: halo ring
-  int nring = 2 * jd + 2 * (kd - 2);
-  for (int w = tid; w < nring; w += nt) {
-    int j, k;
-    if (w < jd) { j = w + 1; k = 1; }
-    else if (w < 2 * jd) { j = w - jd + 1; k = kd; }
-    else if (w < 2 * jd + kd - 2) { j = 1; k = w - 2 * jd + 2; }
-    else { j = jd; k = w - 2 * jd - (kd - 2) + 2; }
-    long long i = cell(M, j, k);
+  const int R = lu_rows(M);
+  const long long ne = (long long)lu_planes(M) * R;   // entries per vector plane
+  long long n2 = (long long)jd * kd;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
+    int j = 1 + (int)(i % jd), k = 1 + (int)(i / jd);
     double ts = M.tscale[i];
-    bool side = ((j == js - 1 || j == je + 1) && k >= ks && k <= ke) ||
-                ((k == ks - 1 || k == ke + 1) && j >= js && j <= je);
-    double d = 1.0;
-    if (side) {
-      CellLoad c = load_cell(M, i);
-      d = lu_of(C, c, epsv).d;
+    bool jin = (j >= js && j <= je), kin = (k >= ks && k <= ke);
+    if (!jin && !kin) {  // corner: only s*tscale
+      for (int n = 0; n < 4; ++n) M.s[n][i] = M.s[n][i] * ts;
+      continue;
     }
+    CellLoad c = load_cell(M, i);
+    LUCellF64 L = lu_of(C, c, epsv);
+    long long e = (long long)(j + k - (js + ks - 2)) * R + (k - ks + 1);
+    bool interior = jin && kin;
+    double st[4];
     for (int n = 0; n < 4; ++n) {
-      double s = M.s[n][i] * ts;
-      if (side) s = s * d;
-      M.s[n][i] = s;
+      st[n] = M.s[n][i] * ts;
+      if (!interior) {
+        st[n] = st[n] * L.d;
+        M.s[n][i] = st[n];
+      }
+    }
+    LUNb jf, jb, kf, kb;
+    nb_j(c, L, eps2, +1, jf);
+    nb_j(c, L, eps2, -1, jb);
+    nb_k(c, L, eps2, +1, kf);
+    nb_k(c, L, eps2, -1, kb);
+    double f[24] = {st[0], st[1], st[2], st[3], ts * 0.5, L.d, L.uu, L.vv, L.uv2, L.ge,
+                    c.xx, c.xy, c.yx, c.yy, L.qx, L.qy,
+                    jf.chk, jf.spec, kf.chk, kf.spec, jb.chk, jb.spec, kb.chk, kb.spec};
+    if (sizeof(T) == 8) {
+      D2* v = reinterpret_cast<D2*>(M.lu);
+      for (int q = 0; q < 12; ++q) v[q * ne + e] = D2{f[2 * q], f[2 * q + 1]};
+    } else {
+      F4* v = reinterpret_cast<F4*>(M.lu);
+      for (int q = 0; q < 6; ++q)
+        v[q * ne + e] = F4{(float)f[4 * q], (float)f[4 * q + 1], (float)f[4 * q + 2], (float)f[4 * q + 3]};
     }
   }
+}
+
+template <typename T>
+struct NbT {
+  T s[4];
+  T uu, vv, uv2, ge, ri1, ri2, qq, chk, spec;
+};
+
+template <typename T>
+__device__ __forceinline__ void apply_nb(T gm1, const NbT<T>& n, T* a) {
+  T a4 = n.chk * (n.ri1 * n.s[1] + n.ri2 * n.s[2] - n.qq * n.s[0]);
+  T a2 = n.chk * gm1 * (n.uv2 * n.s[0] - (n.uu * n.s[1] + n.vv * n.s[2]) + n.s[3]);
+  a[0] = a4 + n.spec * n.s[0];
+  a[1] = n.ri1 * a2 + n.uu * a4 + n.spec * n.s[1];
+  a[2] = n.ri2 * a2 + n.vv * a4 + n.spec * n.s[2];
+  a[3] = n.qq * a2 + n.ge * a4 + n.spec * n.s[3];
+}
+
+constexpr int XCH = 13;           // values passed between lanes (and per halo record)
+constexpr int XST = 14;           // LDS record stride (7 x 16 bytes for FP64)
+constexpr int LU_MAXROWS = 256;   // one lane per row, at most 4 waves
+
+// LDS layout (records of XST values, AoS): halo column [nk], halo row [ni],
+// exchange 2 x [nt]
+template <typename T>
+__host__ __device__ inline size_t lu_lds_bytes(int nk, int ni, int nt) {
+  return sizeof(T) * (size_t)XST * ((size_t)nk + (size_t)ni + 2 * (size_t)nt);
+}
+
+template <typename T>
+__device__ __forceinline__ void put13(T* x, const T* s, T uu, T vv, T uv2, T ge, T r1, T r2, T qq, T ck, T sp) {
+  x[0] = s[0]; x[1] = s[1]; x[2] = s[2]; x[3] = s[3];
+  x[4] = uu; x[5] = vv; x[6] = uv2; x[7] = ge;
+  x[8] = r1; x[9] = r2; x[10] = qq; x[11] = ck; x[12] = sp; x[13] = 0;
+}
+template <typename T>
+__device__ __forceinline__ void get13(const T* x, NbT<T>& n) {
+  n.s[0] = x[0]; n.s[1] = x[1]; n.s[2] = x[2]; n.s[3] = x[3];
+  n.uu = x[4]; n.vv = x[5]; n.uv2 = x[6]; n.ge = x[7];
+  n.ri1 = x[8]; n.ri2 = x[9]; n.qq = x[10]; n.chk = x[11]; n.spec = x[12];
+}
+
+template <typename T, int DIR, int D>
+__global__ __launch_bounds__(LU_MAXROWS) void k_lu_sweep(const DevMesh* __restrict__ Ms) {
+  extern __shared__ __attribute__((aligned(16))) double lds_raw[];
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int nk = ke - ks + 1, ni = je - js + 1;
+  const int R = lu_rows(M);
+  const int NV = LUPack<T>::NV;
+  const long long ne = (long long)lu_planes(M) * R;
+  const LUBufs<T> rs = lu_bufs<T>(M.lu, ne);
+  (void)NV;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  T* hcol = reinterpret_cast<T*>(lds_raw);
+  T* hrow = hcol + (size_t)XST * nk;
+  T* xb0 = hrow + (size_t)XST * ni;
+  T* xb1 = xb0 + (size_t)XST * nt;
+  const T gm1 = (T)M.c.gm1;
+  const int pbase = js + ks - 2;
+  // ---- stage the halo column (j neighbour of each row's first cell) and
+  //      the halo row (k neighbour of the boundary lane) in LDS
+  for (int w = tid; w < nk; w += nt) {
+    int k = ks + w, j = DIR > 0 ? js - 1 : je + 1;
+    LURec<T> o;
+    rec_load<DIR>(rs, ((j + k - pbase) * R + (k - ks + 1)) * 16, o);
+    put13(hcol + XST * w, o.s, o.uu, o.vv, o.uv2, o.ge, o.xx, o.xy, o.qx, o.cj, o.pj);
+  }
+  for (int w = tid; w < ni; w += nt) {
+    int j = js + w, k = DIR > 0 ? ks - 1 : ke + 1;
+    LURec<T> o;
+    rec_load<DIR>(rs, ((j + k - pbase) * R + (k - ks + 1)) * 16, o);
+    put13(hrow + XST * w, o.s, o.uu, o.vv, o.uv2, o.ge, o.yx, o.yy, o.qy, o.ck, o.pk);
+  }
   __syncthreads();
-  double* buf0 = lds;
-  double* buf1 = lds + (size_t)LU_NB_DBL * nt;
-  const int k = ks + tid;
+  const int k = ks + tid, r = tid + 1;
   const bool row = tid < nk;
-  LUNb prevA;  // own previous cell (j-1, k)
-  for (int m = js + ks; m <= je + ke; ++m) {
-    double* wr = ((m & 1) ? buf1 : buf0);
-    double* rd = ((m & 1) ? buf0 : buf1);
-    int j = m - k;
-    bool act = row && j >= js && j <= je;
-    if (act) {
-      long long i = cell(M, j, k);
-      CellLoad c = load_cell(M, i);
-      LUCellF64 L = lu_of(C, c, epsv);
-      // A neighbour (j-1, k)
-      if (j == js) {
-        long long ia = cell(M, js - 1, k);
-        CellLoad ca = load_cell(M, ia);
-        LUCellF64 La = lu_of(C, ca, epsv);
-        nb_j(ca, La, eps2, +1, prevA);
-        for (int n = 0; n < 4; ++n) prevA.s[n] = M.s[n][ia];
+  const int nplanes = je + ke - js - ks + 1;
+  const bool hb_lane = DIR > 0 ? tid == 0 : tid == nk - 1;
+  // every lane's first cell has the halo column as its j neighbour
+  NbT<T> A;
+  get13(hcol + XST * (row ? tid : 0), A);
+  LURec<T> P[D];
+  auto plane_of = [&](int it) { return DIR > 0 ? js + ks + it : je + ke - it; };
+  // record of this lane at hyperplane it (clamped into the grid for lanes
+  // outside it: they compute throw-away values and store nothing)
+  auto entry = [&](int it) {
+    int m = plane_of(it < nplanes ? it : nplanes - 1), j = m - k;
+    j = j < js ? js : (j > je ? je : j);
+    int kk = row ? k : ks;
+    return (j + kk - pbase) * R + (kk - ks + 1);
+  };
+#pragma unroll
+  for (int d = 0; d < D - 1; ++d) rec_load<DIR>(rs, entry(d) * 16, P[d]);
+  for (int it0 = 0; it0 < nplanes; it0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int it = it0 + d;
+      if (it >= nplanes) break;
+      rec_load<DIR>(rs, entry(it + D - 1) * 16, P[(d + D - 1) % D]);
+      const LURec<T>& o = P[d];
+      T* wr = (it & 1) ? xb1 : xb0;
+      const T* rd = (it & 1) ? xb0 : xb1;
+      const int m = plane_of(it), j = m - k;
+      const bool act = row && j >= js && j <= je;
+      // k neighbour: halo row for the boundary lane, else the adjacent lane
+      int jh = j - js;
+      jh = jh < 0 ? 0 : (jh >= ni ? ni - 1 : jh);
+      const T* bsrc = hb_lane ? hrow + XST * jh : rd + XST * (tid - DIR >= 0 ? tid - DIR : 0);
+      NbT<T> B;
+      get13(bsrc, B);
+      T a[4], b[4], sn[4];
+      apply_nb(gm1, A, a);
+      apply_nb(gm1, B, b);
+      for (int v = 0; v < 4; ++v) {
+        if (DIR > 0) sn[v] = (o.s[v] + o.sv * (a[v] + b[v])) * o.di;
+        else sn[v] = o.s[v] - o.sv * (a[v] + b[v]) * o.di;
       }
-      // B neighbour (j, k-1)
-      LUNb B;
-      if (k == ks) {
-        long long ib = cell(M, j, ks - 1);
-        CellLoad cb = load_cell(M, ib);
-        LUCellF64 Lb = lu_of(C, cb, epsv);
-        nb_k(cb, Lb, eps2, +1, B);
-        for (int n = 0; n < 4; ++n) B.s[n] = M.s[n][ib];
-      } else {
-        const double* p = rd + (size_t)LU_NB_DBL * (tid - 1);
-        for (int n = 0; n < 4; ++n) B.s[n] = p[n];
-        B.ri1 = p[4]; B.ri2 = p[5]; B.qq = p[6]; B.uu = p[7]; B.vv = p[8]; B.uv2 = p[9]; B.ge = p[10];
-        B.chk = p[11]; B.spec = p[12];
+      if (act) {
+        rec_store_s(rs, ((m - pbase) * R + r) * 16, sn);
+        for (int v = 0; v < 4; ++v) A.s[v] = sn[v];
+        A.uu = o.uu; A.vv = o.vv; A.uv2 = o.uv2; A.ge = o.ge;
+        A.ri1 = o.xx; A.ri2 = o.xy; A.qq = o.qx; A.chk = o.cj; A.spec = o.pj;
       }
-      double a[4], b[4];
-      lu_apply(C.gm1, prevA.chk, prevA.spec, prevA.ri1, prevA.ri2, prevA.qq, prevA.uu, prevA.vv, prevA.uv2,
-               prevA.ge, prevA.s, a);
-      lu_apply(C.gm1, B.chk, B.spec, B.ri1, B.ri2, B.qq, B.uu, B.vv, B.uv2, B.ge, B.s, b);
-      double sv = c.ts * 0.5;
-      double di = L.d;
-      double snew[4];
-      for (int n = 0; n < 4; ++n) {
-        double st = M.s[n][i] * c.ts;  // s*tscale (src/step.f:141-148)
-        snew[n] = (st + sv * (a[n] + b[n])) * di;
-        M.s[n][i] = snew[n];
-      }
-      // becomes the A neighbour of (j+1, k) and the B neighbour of (j, k+1)
-      nb_j(c, L, eps2, +1, prevA);
-      LUNb E;
-      nb_k(c, L, eps2, +1, E);
-      for (int n = 0; n < 4; ++n) prevA.s[n] = snew[n];
-      double* p = wr + (size_t)LU_NB_DBL * tid;
-      for (int n = 0; n < 4; ++n) p[n] = snew[n];
-      p[4] = E.ri1; p[5] = E.ri2; p[6] = E.qq; p[7] = E.uu; p[8] = E.vv; p[9] = E.uv2; p[10] = E.ge;
-      p[11] = E.chk; p[12] = E.spec;
+      put13(wr + XST * tid, sn, o.uu, o.vv, o.uv2, o.ge, o.yx, o.yy, o.qy, o.ck, o.pk);
+      lds_barrier();
     }
-    __syncthreads();
   }
 }
 
@@ -992,101 +1207,41 @@ __device__ __forceinline__ bool asq_bad(const DevConst& C, double q1, double q2,
   return asq <= 0.002 || asq + 1 == asq;
 }
 
-// Backward sweep of ilu2d (src/lhs.f:719-800) with the conservative
-// restore; with fuse_update the q update (src/step.f:205-211) and monitor
-// (src/step.f:353-385) are applied to each cell as soon as its correction
-// is final.
-__global__ void k_lu_bwd(const DevMesh* __restrict__ Ms, int fuse_update) {
-  extern __shared__ double lds[];
-  const DevMesh& M = Ms[blockIdx.x];
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ Ms, int fuse_update) {
+  const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
-  const DevConst& C = M.c;
-  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke, jd = M.jd, kd = M.kd;
-  const int nk = ke - ks + 1;
-  const double eps2 = C.epse * 2.;
-  const double epsv = 1. + eps2;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  double* buf0 = lds;
-  double* buf1 = lds + (size_t)LU_NB_DBL * nt;
-  const int k = ks + tid;
-  const bool row = tid < nk;
+  const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int R = lu_rows(M);
+  const long long ne = (long long)lu_planes(M) * R;
+  long long n2 = (long long)jd * kd;
   int bad = 0;
-  LUNb prevA;  // own previous cell (j+1, k)
-  for (int m = je + ke; m >= js + ks; --m) {
-    double* wr = ((m & 1) ? buf1 : buf0);
-    double* rd = ((m & 1) ? buf0 : buf1);
-    int j = m - k;
-    bool act = row && j >= js && j <= je;
-    if (act) {
-      long long i = cell(M, j, k);
-      CellLoad c = load_cell(M, i);
-      LUCellF64 L = lu_of(C, c, epsv);
-      if (j == je) {
-        long long ia = cell(M, je + 1, k);
-        CellLoad ca = load_cell(M, ia);
-        LUCellF64 La = lu_of(C, ca, epsv);
-        nb_j(ca, La, eps2, -1, prevA);
-        for (int n = 0; n < 4; ++n) prevA.s[n] = M.s[n][ia];
-      }
-      LUNb B;
-      if (k == ke) {
-        long long ib = cell(M, j, ke + 1);
-        CellLoad cb = load_cell(M, ib);
-        LUCellF64 Lb = lu_of(C, cb, epsv);
-        nb_k(cb, Lb, eps2, -1, B);
-        for (int n = 0; n < 4; ++n) B.s[n] = M.s[n][ib];
-      } else {
-        const double* p = rd + (size_t)LU_NB_DBL * (tid + 1);
-        for (int n = 0; n < 4; ++n) B.s[n] = p[n];
-        B.ri1 = p[4]; B.ri2 = p[5]; B.qq = p[6]; B.uu = p[7]; B.vv = p[8]; B.uv2 = p[9]; B.ge = p[10];
-        B.chk = p[11]; B.spec = p[12];
-      }
-      double a[4], b[4];
-      lu_apply(C.gm1, prevA.chk, prevA.spec, prevA.ri1, prevA.ri2, prevA.qq, prevA.uu, prevA.vv, prevA.uv2,
-               prevA.ge, prevA.s, a);
-      lu_apply(C.gm1, B.chk, B.spec, B.ri1, B.ri2, B.qq, B.uu, B.vv, B.uv2, B.ge, B.s, b);
-      double sv = c.ts * 0.5;
-      double di = L.d;
-      double snew[4];
-      for (int n = 0; n < 4; ++n) {
-        snew[n] = M.s[n][i] - sv * (a[n] + b[n]) * di;
-        M.s[n][i] = snew[n];
-      }
-      nb_j(c, L, eps2, -1, prevA);
-      LUNb E;
-      nb_k(c, L, eps2, -1, E);
-      for (int n = 0; n < 4; ++n) prevA.s[n] = snew[n];
-      double* p = wr + (size_t)LU_NB_DBL * tid;
-      for (int n = 0; n < 4; ++n) p[n] = snew[n];
-      p[4] = E.ri1; p[5] = E.ri2; p[6] = E.qq; p[7] = E.uu; p[8] = E.vv; p[9] = E.uv2; p[10] = E.ge;
-      p[11] = E.chk; p[12] = E.spec;
-      // conservative restore (+ update)
-      double q1 = c.q1, q2 = c.q2, q3 = c.q3, q4 = c.q4;
-      q_roundtrip(q1, q2, q3, q4, c.qj);
-      if (fuse_update) {
-        double ibp = (double)imax(M.iblank[i], 0);
-        q1 = q1 + snew[0] * ibp; q2 = q2 + snew[1] * ibp; q3 = q3 + snew[2] * ibp; q4 = q4 + snew[3] * ibp;
-        bad |= asq_bad(C, q1, q2, q3, q4);
-      }
-      M.q[0][i] = q1; M.q[1][i] = q2; M.q[2][i] = q3; M.q[3][i] = q4;
-    }
-    __syncthreads();
-  }
-  // halo ring: restore only (the sweep above has read every halo value it needs)
-  int nring = 2 * jd + 2 * (kd - 2);
-  for (int w = tid; w < nring; w += nt) {
-    int j, kk;
-    if (w < jd) { j = w + 1; kk = 1; }
-    else if (w < 2 * jd) { j = w - jd + 1; kk = kd; }
-    else if (w < 2 * jd + kd - 2) { j = 1; kk = w - 2 * jd + 2; }
-    else { j = jd; kk = w - 2 * jd - (kd - 2) + 2; }
-    long long i = cell(M, j, kk);
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
+    int j = 1 + (int)(i % jd), k = 1 + (int)(i / jd);
     double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
     q_roundtrip(q1, q2, q3, q4, M.q[M.nq - 1][i]);
+    if (j >= js && j <= je && k >= ks && k <= ke) {
+      long long e = (long long)(j + k - (js + ks - 2)) * R + (k - ks + 1);
+      double dq[4];
+      if (sizeof(T) == 8) {
+        const D2* v = reinterpret_cast<const D2*>(M.lu);
+        D2 a = v[e], b = v[ne + e];
+        dq[0] = a.x; dq[1] = a.y; dq[2] = b.x; dq[3] = b.y;
+      } else {
+        const F4* v = reinterpret_cast<const F4*>(M.lu);
+        F4 a = v[e];
+        dq[0] = (double)a.x; dq[1] = (double)a.y; dq[2] = (double)a.z; dq[3] = (double)a.w;
+      }
+      for (int n = 0; n < 4; ++n) M.s[n][i] = dq[n];
+      if (fuse_update) {
+        double ibp = (double)imax(M.iblank[i], 0);
+        q1 = q1 + dq[0] * ibp; q2 = q2 + dq[1] * ibp; q3 = q3 + dq[2] * ibp; q4 = q4 + dq[3] * ibp;
+      }
+    }
     M.q[0][i] = q1; M.q[1][i] = q2; M.q[2][i] = q3; M.q[3][i] = q4;
-    if (fuse_update) bad |= asq_bad(C, q1, q2, q3, q4);
+    if (fuse_update) bad |= asq_bad(M.c, q1, q2, q3, q4);
   }
-  if (fuse_update && bad) atomicCAS(M.status, 0, 1);
+  if (bad) atomicCAS(M.status, 0, 1);
 }
 
 // q update (src/step.f:205-211) when it cannot be fused
@@ -1114,9 +1269,12 @@ __global__ __launch_bounds__(256) void k_monitor(const DevMesh* __restrict__ Ms)
   if (bad) atomicCAS(M.status, 0, 1);
 }
 
-__global__ void k_step_begin(int* istep0, int* status, int ncase) {
+__global__ void k_step_begin(int* istep0, int* status, int* stepc, int ncase) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ncase && status[i] == 0) istep0[i] += 1;
+  if (i < ncase && status[i] == 0) {
+    istep0[i] += 1;
+    stepc[i] += 1;
+  }
 }
 
 // ================================================================ geometry
@@ -1471,20 +1629,52 @@ static int lu_threads(const DevMesh* hM, int nmi) {
     int nk = hM[i].ke - hM[i].ks + 1;
     mx = nk > mx ? nk : mx;
   }
-  int nt = ((mx + 63) / 64) * 64;
-  return nt;
+  return ((mx + 63) / 64) * 64;
+}
+
+template <typename T, int D>
+static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, hipStream_t st) {
+  int nt = lu_threads(hM, nmi);
+  if (nt > LU_MAXROWS) return hipErrorInvalidValue;
+  long long mx = 0;
+  int nkmax = 0, nimax = 0;
+  for (int i = 0; i < nmi; ++i) {
+    mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+    nkmax = hM[i].ke - hM[i].ks + 1 > nkmax ? hM[i].ke - hM[i].ks + 1 : nkmax;
+    nimax = hM[i].je - hM[i].js + 1 > nimax ? hM[i].je - hM[i].js + 1 : nimax;
+  }
+  size_t lds = lu_lds_bytes<T>(nkmax, nimax, nt);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep<T, 1, D>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep<T, -1, D>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_lu_prep<T>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_lu_sweep<T, 1, D>), dim3(nmi), dim3(nt), lds, st, dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL((k_lu_sweep<T, -1, D>), dim3(nmi), dim3(nt), lds, st, dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lu_finish<T>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, fuse_update);
+  return hipGetLastError();
 }
 
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, hipStream_t st) {
-  (void)fp32;
-  int nt = lu_threads(hM, nmi);
-  if (nt > 1024) return hipErrorInvalidValue;
-  size_t lds = sizeof(double) * LU_NB_DBL * 2 * (size_t)nt;
-  hipLaunchKernelGGL(k_lu_fwd, dim3(nmi), dim3(nt), lds, st, dM);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_lu_bwd, dim3(nmi), dim3(nt), lds, st, dM, fuse_update);
-  return hipGetLastError();
+  return fp32 ? launch_lu_t<float, 8>(dM, hM, nmi, fuse_update, st)
+              : launch_lu_t<double, 6>(dM, hM, nmi, fuse_update, st);
+}
+
+size_t lu_lds_need(int nk, int ni, int fp32) {
+  int nt = ((nk + 63) / 64) * 64;
+  return fp32 ? lu_lds_bytes<float>(nk, ni, nt) : lu_lds_bytes<double>(nk, ni, nt);
+}
+
+size_t lu_workspace_bytes(int jd, int kd) {
+  DevMesh M;
+  M.js = 2; M.je = jd - 1; M.ks = 2; M.ke = kd - 1;
+  return 16 * (size_t)LUPack<double>::NV * lu_planes(M) * lu_rows(M);
 }
 
 hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
@@ -1501,8 +1691,8 @@ hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
   return hipGetLastError();
 }
 
-hipError_t launch_step_begin(int* istep0, int* status, int ncase, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_begin, dim3((ncase + 255) / 256), dim3(256), 0, st, istep0, status, ncase);
+hipError_t launch_step_begin(int* istep0, int* status, int* stepc, int ncase, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_begin, dim3((ncase + 255) / 256), dim3(256), 0, st, istep0, status, stepc, ncase);
   return hipGetLastError();
 }
 
