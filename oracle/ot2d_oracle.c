@@ -1205,6 +1205,597 @@ static void timespectralrhs(or_case* c, or_mesh* M, int nsp) {
   }
 }
 
+/* ------------------------------------------------------- viscous (a5) */
+/* visrhs, src/visrhs.f:2-287 (nhalo = 1) */
+static void visrhs_mesh(or_case* c, or_mesh* M, int nsp) {
+  int jd = M->jd, kd = M->kd;
+  double gamma = c->p.gamma, pr = c->p.pr;
+  double gkpr = gamma / pr, prtr = pr / 0.9, dre = .5 / c->rey;
+  double c2b = 198.6 / c->p.tinf, c2bp = c2b + 1.;
+  double ggm1 = c->ggm1;
+  double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+  double* qj = QA(M, nsp, c->nq);
+  double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+  double* turmu = GA(M, turmu, nsp);
+  double *s2 = SA(M, nsp, 2), *s3 = SA(M, nsp, 3), *s4 = SA(M, nsp, 4);
+  int md = c->mdim + 2;
+  double *f2 = calloc(md, 8), *f3 = calloc(md, 8), *f4 = calloc(md, 8);
+  double *f2v = calloc(md, 8), *f3v = calloc(md, 8), *f4v = calloc(md, 8);
+  double *u = calloc(md, 8), *v = calloc(md, 8), *e = calloc(md, 8);
+  double *a1v = calloc(md, 8), *a2v = calloc(md, 8), *a4v = calloc(md, 8), *a5v = calloc(md, 8);
+  double *d1v = calloc(md, 8), *d2v = calloc(md, 8), *d4v = calloc(md, 8), *d5v = calloc(md, 8), *d7v = calloc(md, 8);
+#define I2(j, k) IX(M, j, k)
+  for (int j = M->jbeg; j <= M->jend; ++j) {
+    for (int k = 1; k <= kd; ++k) {
+      size_t i = I2(j, k), im = I2(j - 1, k), ip = I2(j + 1, k);
+      double ra = 1. / q1[i];
+      u[k] = q2[i] * ra;
+      v[k] = q3[i] * ra;
+      e[k] = q4[i] * ra - .5 * (sq(u[k]) + sq(v[k]));
+      double tt = ggm1 * e[k];
+      double vmue = c2bp * tt * sqrt(tt) / (c2b + tt);
+      double turm = turmu[i];
+      double vnu = vmue + turm;
+      double gkap = vmue + prtr * turm;
+      double rj = 1. / qj[i], rjm = 1. / qj[im], rjp = 1. / qj[ip];
+      double b4 = (sq(yx[i]) + sq(yy[i])) * rj;
+      a1v[k] = (b4 + sq(yx[i]) / 3. * rj) * vnu * dre;   /* b1v */
+      a2v[k] = (b4 + sq(yy[i]) / 3. * rj) * vnu * dre;   /* b2v */
+      a5v[k] = (yx[i] * yy[i] / 3. * rj) * vnu * dre;    /* b5v */
+      a4v[k] = b4 * gkpr * gkap * dre;                   /* b4v */
+      double t;
+      t = (4. / 3. * xx[im] * yx[im] + xy[im] * yy[im]) * rjm;
+      t = t + (4. / 3. * xx[ip] * yx[ip] + xy[ip] * yy[ip]) * rjp;
+      d1v[k] = vnu * dre * t;
+      t = (4. / 3. * xy[im] * yy[im] + xx[im] * yx[im]) * rjm;
+      t = t + (4. / 3. * xy[ip] * yy[ip] + xx[ip] * yx[ip]) * rjp;
+      d2v[k] = vnu * dre * t;
+      t = (xy[im] * yy[im] + xx[im] * yx[im]) * rjm;
+      t = t + (xy[ip] * yy[ip] + xx[ip] * yx[ip]) * rjp;
+      d4v[k] = t * gkpr * gkap * dre;
+      t = (-2. / 3. * xy[im] * yx[im] + xx[im] * yy[im]) * rjm;
+      t = t + (-2. / 3. * xy[ip] * yx[ip] + xx[ip] * yy[ip]) * rjp;
+      d5v[k] = vnu * dre * t;
+      t = (-2. / 3. * xx[im] * yy[im] + xy[im] * yx[im]) * rjm;
+      t = t + (-2. / 3. * xx[ip] * yy[ip] + xy[ip] * yx[ip]) * rjp;
+      d7v[k] = vnu * dre * t;
+    }
+    for (int k = 1; k <= kd - 1; ++k) {
+      int k1 = k + 1;
+      double b1 = a1v[k1] + a1v[k], b2 = a2v[k1] + a2v[k], b5 = a5v[k1] + a5v[k], b4 = a4v[k1] + a4v[k];
+      double d1 = d1v[k], d2 = d2v[k], d4 = d4v[k], d5 = d5v[k], d7 = d7v[k];
+      size_t ip = I2(j + 1, k), im = I2(j - 1, k);
+      double ujp1 = q2[ip] / q1[ip], ujm1 = q2[im] / q1[im];
+      double duj = 0.5 * (ujp1 - ujm1);
+      double vjp1 = q3[ip] / q1[ip], vjm1 = q3[im] / q1[im];
+      double dvj = 0.5 * (vjp1 - vjm1);
+      double ejp1 = q4[ip] / q1[ip] - .5 * (sq(ujp1) + sq(vjp1));
+      double ejm1 = q4[im] / q1[im] - .5 * (sq(ujm1) + sq(vjm1));
+      double dej = 0.5 * (ejp1 - ejm1);
+      double t4 = u[k1] * a1v[k1] + u[k] * a1v[k];
+      double t5 = v[k1] * a2v[k1] + v[k] * a2v[k];
+      double t6 = u[k1] * a5v[k1] + u[k] * a5v[k];
+      double t7 = v[k1] * a5v[k1] + v[k] * a5v[k];
+      double t8 = a4v[k1] + a4v[k];
+      double du = u[k1] - u[k], dv = v[k1] - v[k], dei = e[k1] - e[k];
+      f2[k] = b1 * du + b5 * dv;
+      f2v[k] = d1 * duj + d5 * dvj;
+      f3[k] = b5 * du + b2 * dv;
+      f3v[k] = d7 * duj + d2 * dvj;
+      f4[k] = (t4 + t7) * du + (t5 + t6) * dv + t8 * dei;
+      f4v[k] = 0.5 * d1 * 0.5 * (sq(ujp1) - sq(ujm1));
+      f4v[k] = f4v[k] + 0.5 * d2 * 0.5 * (sq(vjp1) - sq(vjm1));
+      f4v[k] = f4v[k] + d5 * 0.5 * (ujp1 + ujm1) * dvj;
+      f4v[k] = f4v[k] + d7 * 0.5 * (vjp1 + vjm1) * duj;
+      f4v[k] = f4v[k] + d4 * dej;
+    }
+    int ke = M->kend - 1;
+    int k = M->kend;
+    size_t i = I2(j, k);
+    s2[i] = s2[i] + (f2[k] - f2[k - 1]) + 0.5 * (f2v[k - 2] - 4. * f2v[k - 1] + 3 * f2v[k]);
+    s3[i] = s3[i] + (f3[k] - f3[k - 1]) + 0.5 * (f3v[k - 2] - 4. * f3v[k - 1] + 3 * f3v[k]);
+    s4[i] = s4[i] + (f4[k] - f4[k - 1]) + 0.5 * (f4v[k - 2] - 4. * f4v[k - 1] + 3. * f4v[k]);
+    for (k = M->kbeg; k <= ke; ++k) {
+      i = I2(j, k);
+      s2[i] = s2[i] + (f2[k] - f2[k - 1]) + 0.5 * (f2v[k + 1] - f2v[k - 1]);
+      s3[i] = s3[i] + (f3[k] - f3[k - 1]) + 0.5 * (f3v[k + 1] - f3v[k - 1]);
+      s4[i] = s4[i] + (f4[k] - f4[k - 1]) + 0.5 * (f4v[k + 1] - f4v[k - 1]);
+    }
+  }
+  for (int k = M->kbeg; k <= M->kend; ++k) {
+    for (int j = 1; j <= jd; ++j) {
+      size_t i = I2(j, k), km = I2(j, k - 1), kp = I2(j, k + 1);
+      double ra = 1. / q1[i];
+      u[j] = q2[i] * ra;
+      v[j] = q3[i] * ra;
+      e[j] = q4[i] * ra - .5 * (sq(u[j]) + sq(v[j]));
+      double tt = ggm1 * e[j];
+      double vmue = c2bp * tt * sqrt(tt) / (c2b + tt);
+      double turm = turmu[i];
+      double vnu = vmue + turm;
+      double gkap = vmue + prtr * turm;
+      double rj = 1. / qj[i], rkm = 1. / qj[km], rkp = 1. / qj[kp];
+      double a4 = (sq(xx[i]) + sq(xy[i])) * rj;
+      a1v[j] = (a4 + sq(xx[i]) / 3. * rj) * vnu * dre;
+      a2v[j] = (a4 + sq(xy[i]) / 3. * rj) * vnu * dre;
+      a5v[j] = (xx[i] * xy[i] / 3. * rj) * vnu * dre;
+      a4v[j] = a4 * gkpr * gkap * dre;
+      double t;
+      t = (4. / 3. * xx[km] * yx[km] + xy[km] * yy[km]) * rkm;
+      t = t + (4. / 3. * xx[kp] * yx[kp] + xy[kp] * yy[kp]) * rkp;
+      d1v[j] = vnu * dre * t;
+      t = (4. / 3. * xy[km] * yy[km] + xx[km] * yx[km]) * rkm;
+      t = t + (4. / 3. * xy[kp] * yy[kp] + xx[kp] * yx[kp]) * rkp;
+      d2v[j] = vnu * dre * t;
+      t = (xy[km] * yy[km] + xx[km] * yx[km]) * rkm;
+      t = t + (xy[kp] * yy[kp] + xx[kp] * yx[kp]) * rkp;
+      d4v[j] = t * gkpr * gkap * dre;
+      t = (-2. / 3. * xy[km] * yx[km] + xx[km] * yy[km]) * rkm;
+      t = t + (-2. / 3. * xy[kp] * yx[kp] + xx[kp] * yy[kp]) * rkp;
+      d5v[j] = vnu * dre * t;
+      t = (-2. / 3. * xx[km] * yy[km] + xy[km] * yx[km]) * rkm;
+      t = t + (-2. / 3. * xx[kp] * yy[kp] + xy[kp] * yx[kp]) * rkp;
+      d7v[j] = vnu * dre * t;
+    }
+    for (int j = 1; j <= jd - 1; ++j) {
+      int j1 = j + 1;
+      double a1 = a1v[j1] + a1v[j], a2 = a2v[j1] + a2v[j], a5 = a5v[j1] + a5v[j], a4 = a4v[j1] + a4v[j];
+      double d1 = d1v[j], d2 = d2v[j], d4 = d4v[j], d5 = d5v[j], d7 = d7v[j];
+      size_t kp = I2(j, k + 1), km = I2(j, k - 1);
+      double ukp1 = q2[kp] / q1[kp], ukm1 = q2[km] / q1[km];
+      double duk = 0.5 * (ukp1 - ukm1);
+      double vkp1 = q3[kp] / q1[kp], vkm1 = q3[km] / q1[km];
+      double dvk = 0.5 * (vkp1 - vkm1);
+      double ekp1 = q4[kp] / q1[kp] - .5 * (sq(ukp1) + sq(vkp1));
+      double ekm1 = q4[km] / q1[km] - .5 * (sq(ukm1) + sq(vkm1));
+      double dek = 0.5 * (ekp1 - ekm1);
+      double t4 = u[j1] * a1v[j1] + u[j] * a1v[j];
+      double t5 = v[j1] * a2v[j1] + v[j] * a2v[j];
+      double t6 = u[j1] * a5v[j1] + u[j] * a5v[j];
+      double t7 = v[j1] * a5v[j1] + v[j] * a5v[j];
+      double t8 = a4v[j1] + a4v[j];
+      double du = u[j1] - u[j], dv = v[j1] - v[j], dei = e[j1] - e[j];
+      f2[j] = a1 * du + a5 * dv;
+      f2v[j] = d1 * duk + d7 * dvk;
+      f3[j] = a5 * du + a2 * dv;
+      f3v[j] = d5 * duk + d2 * dvk;
+      f4[j] = (t4 + t7) * du + (t5 + t6) * dv + t8 * dei;
+      f4v[j] = 0.5 * d1 * 0.5 * (sq(ukp1) - sq(ukm1));
+      f4v[j] = f4v[j] + 0.5 * d2 * 0.5 * (sq(vkp1) - sq(vkm1));
+      f4v[j] = f4v[j] + d5 * 0.5 * (vkp1 + vkm1) * duk;
+      f4v[j] = f4v[j] + d7 * 0.5 * (ukp1 + ukm1) * dvk;
+      f4v[j] = f4v[j] + d4 * dek;
+    }
+    int je = M->jend - 1;
+    int j = M->jend;
+    size_t i = I2(j, k);
+    s2[i] = s2[i] + (f2[j] - f2[j - 1]) + 0.5 * (f2v[j - 2] - 4. * f2v[j - 1] + 3 * f2v[j]);
+    s3[i] = s3[i] + (f3[j] - f3[j - 1]) + 0.5 * (f3v[j - 2] - 4. * f3v[j - 1] + 3 * f3v[j]);
+    s4[i] = s4[i] + (f4[j] - f4[j - 1]) + 0.5 * (f4v[j - 2] - 4. * f4v[j - 1] + 3 * f4v[j]);
+    for (j = M->jbeg; j <= je; ++j) {
+      i = I2(j, k);
+      s2[i] = s2[i] + (f2[j] - f2[j - 1]) + 0.5 * (f2v[j + 1] - f2v[j - 1]);
+      s3[i] = s3[i] + (f3[j] - f3[j - 1]) + 0.5 * (f3v[j + 1] - f3v[j - 1]);
+      s4[i] = s4[i] + (f4[j] - f4[j - 1]) + 0.5 * (f4v[j + 1] - f4v[j - 1]);
+    }
+  }
+#undef I2
+  free(f2); free(f3); free(f4); free(f2v); free(f3v); free(f4v); free(u); free(v); free(e);
+  free(a1v); free(a2v); free(a4v); free(a5v); free(d1v); free(d2v); free(d4v); free(d5v); free(d7v);
+}
+
+/* ------------------------------------------------- SA turbulence (a12) */
+/* constants of src/sadata.h:2-4 */
+static const double SA_cb1 = 0.1355, SA_sigma = 0.6666667, SA_cb2 = 0.622, SA_akt = 0.41,
+                    SA_cw1 = 3.2390678, SA_cw2 = 0.3, SA_cw3 = 2., SA_cv1 = 7.1, SA_ct3 = 1.2,
+                    SA_ct4 = 0.5;
+
+/* integer powers as the reference build evaluates them: repeated
+ * multiplication left to right (x**6 = ((((x*x)*x)*x)*x)*x); real powers
+ * (x**(1./6.), x**5.) go to the C library pow */
+static double ipow3(double x) { return (x * x) * x; }
+static double ipow6(double x) { return ((((x * x) * x) * x) * x) * x; }
+static double ipow2(double x) { return x * x; }
+
+/* turbc, src/samodel2d.f:827-1303 */
+static void turbc_mesh(or_case* c, or_mesh* M, int nsp, const double* u, const double* v) {
+  double* q5 = QA(M, nsp, 5);
+  double *ug = GA(M, ug, nsp), *vg = GA(M, vg, nsp);
+  double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+  for (int ib = 0; ib < M->d.nbc; ++ib) {
+    const ot2d_bcseg* b = &M->d.bc[ib];
+    int js, je, ks, ke;
+    bc_range(M, b, &js, &je, &ks, &ke);
+    int idir = b->ibdir, iadd = idir >= 0 ? 1 : -1, iadir = abs(idir);
+    int t = b->ibtyp;
+    double foso = 1.0;
+    if (t == 3 || t == 4) {          /* turbc_extpt */
+      if (iadir == 1) {
+        for (int jc = 1; jc <= je - js + 1; ++jc) {
+          int j = (idir == 1) ? je - jc + 1 : js + jc - 1;
+          int j1 = j + iadd, j2 = j1 + iadd;
+          for (int k = ks; k <= ke; ++k)
+            q5[IX(M, j, k)] = (1. + foso) * q5[IX(M, j1, k)] - foso * q5[IX(M, j2, k)];
+        }
+      } else if (iadir == 2) {
+        for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+          int k = (idir == 2) ? ke - kc + 1 : ks + kc - 1;
+          int k1 = k + iadd, k2 = k1 + iadd;
+          for (int j = js; j <= je; ++j)
+            q5[IX(M, j, k)] = (1. + foso) * q5[IX(M, j, k1)] - foso * q5[IX(M, j, k2)];
+        }
+      }
+    } else if (t == 5) {             /* turbc_wall (one halo layer) */
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = j + iadd;
+        for (int k = ks; k <= ke; ++k) q5[IX(M, j, k)] = -q5[IX(M, j1, k)];
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = k + iadd;
+        for (int j = js; j <= je; ++j) q5[IX(M, j, k)] = -q5[IX(M, j, k1)];
+      }
+    } else if (t == 10 || t == 11) { /* turbc_sym */
+      if (iadir == 1) {
+        for (int jc = 1; jc <= je - js + 1; ++jc) {
+          int j, j1;
+          if (idir == 1) { j = je - jc + 1; j1 = je + jc; } else { j = js + jc - 1; j1 = js - jc; }
+          for (int k = ks; k <= ke; ++k) q5[IX(M, j, k)] = q5[IX(M, j1, k)];
+        }
+      } else if (iadir == 2) {
+        for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+          int k, k1;
+          if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+          for (int j = js; j <= je; ++j) q5[IX(M, j, k)] = q5[IX(M, j, k1)];
+        }
+      }
+    } else if (t == 22) {            /* turbc_periodic */
+      if (iadir == 1) {
+        for (int j = js; j <= je; ++j) {
+          int jc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, j);
+          for (int k = ks; k <= ke; ++k) q5[IX(M, j, k)] = q5[IX(M, jc, k)];
+        }
+      } else {
+        for (int k = ks; k <= ke; ++k) {
+          int kc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, k);
+          for (int j = js; j <= je; ++j) q5[IX(M, j, k)] = q5[IX(M, j, kc)];
+        }
+      }
+    } else if (t == 51 && iadir == 2) { /* turbc_wake */
+      for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+        int k, k1;
+        if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+        for (int j = js; j <= je; ++j) {
+          int jj = M->jd - j + 1;
+          q5[IX(M, j, k)] = q5[IX(M, jj, k1)];
+          q5[IX(M, jj, k)] = q5[IX(M, j, k1)];
+        }
+      }
+    } else if (t == 46) {            /* turbc_inf */
+      for (int k = ks; k <= ke; ++k)
+        for (int j = js; j <= je; ++j) q5[IX(M, j, k)] = c->p.vnuinf;
+    } else if (t == 47) {            /* turbc_out */
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = j + iadd;
+        for (int k = ks; k <= ke; ++k) {
+          size_t i = IX(M, j, k), i1 = IX(M, j1, k);
+          q5[i] = c->p.vnuinf;
+          double uu = (u[i] - ug[i] + u[i1] - ug[i1]) * xx[i];
+          uu = uu + (v[i] - vg[i] + v[i1] - vg[i1]) * xy[i];
+          uu = uu * iadd;
+          if (uu < 0.) q5[i] = q5[i1];
+        }
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = k + iadd;
+        for (int j = js; j <= je; ++j) {
+          size_t i = IX(M, j, k), i1 = IX(M, j, k1);
+          q5[i] = c->p.vnuinf;
+          double uu = (u[i] - ug[i] + u[i1] - ug[i1]) * yx[i];
+          uu = uu + (v[i] - vg[i] + v[i1] - vg[i1]) * yy[i];
+          uu = uu * iadd;
+          if (uu < 0.) q5[i] = q5[i1];
+        }
+      }
+    }
+  }
+}
+
+/* vmu_sa, src/samodel2d.f:2-197, itrans = 0, nturiter = nhuang = 1 */
+static void vmu_sa_mesh(or_case* c, or_mesh* M, int nsp) {
+  int jd = M->jd, kd = M->kd, js = M->jbeg, je = M->jend, ks = M->kbeg, ke = M->kend;
+  size_t n2 = M->n2;
+  double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+  double *q5 = QA(M, nsp, 5), *qj = QA(M, nsp, c->nq);
+  double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+  double *ug = GA(M, ug, nsp), *vg = GA(M, vg, nsp);
+  double *x = GA(M, x, nsp), *y = GA(M, y, nsp);
+  const double *xv = VA(M, xv, nsp), *yv = VA(M, yv, nsp);
+  double* turmu = GA(M, turmu, nsp);
+  double* tscale = GA(M, tscale, nsp);
+  const int32_t* ib = M->iblank + (size_t)nsp * n2;
+  double* s5 = SA(M, nsp, 5);
+  double *sjmat = calloc(n2, 8), *bjmat = calloc(n2, 8);
+  double *aj = calloc(n2, 8), *bj = calloc(n2, 8), *cj = calloc(n2, 8);
+  double *ak = calloc(n2, 8), *bk = calloc(n2, 8), *ck = calloc(n2, 8);
+  double *vmul = calloc(n2, 8), *vort = calloc(n2, 8), *sn = calloc(n2, 8);
+  double *u = calloc(n2, 8), *v = calloc(n2, 8), *s1 = calloc(n2, 8), *s2 = calloc(n2, 8);
+  double *up = calloc(n2, 8), *um = calloc(n2, 8), *vp = calloc(n2, 8), *vm = calloc(n2, 8);
+  double *chp = calloc(n2, 8), *dnuhp = calloc(n2, 8), *dmxh = calloc(n2, 8), *dmyh = calloc(n2, 8);
+  double relfac = 1.0, vnulim = 1.0e-20;
+  double h = c->h;
+  /* lamvis, src/samodel2d.f:200-238 */
+  {
+    double c2b = 198.6 / c->p.tinf, c2bp = c2b + 1.;
+    for (size_t i = 0; i < n2; ++i) {
+      double ra = 1. / q1[i];
+      double uvel = q2[i] * ra, vvel = q3[i] * ra;
+      double ei = q4[i] * ra - 0.5 * (sq(uvel) + sq(vvel));
+      double tt = c->ggm1 * ei;
+      vmul[i] = c2bp * tt * sqrt(tt) / (c2b + tt);
+    }
+  }
+  for (size_t i = 0; i < n2; ++i) { u[i] = q2[i] / q1[i]; v[i] = q3[i] / q1[i]; sn[i] = 1.e10; }
+  turbc_mesh(c, M, nsp, u, v);
+  if (M->bodyflag) {
+    /* vortic, src/samodel2d.f:296-333 */
+    for (int k = ks; k <= ke; ++k)
+      for (int j = js; j <= je; ++j) {
+        size_t i = IX(M, j, k);
+        double ux = (u[IX(M, j + 1, k)] - u[IX(M, j - 1, k)]) * 0.5;
+        double vx = (v[IX(M, j + 1, k)] - v[IX(M, j - 1, k)]) * 0.5;
+        double uy = (u[IX(M, j, k + 1)] - u[IX(M, j, k - 1)]) * 0.5;
+        double vy = (v[IX(M, j, k + 1)] - v[IX(M, j, k - 1)]) * 0.5;
+        double tx = xy[i] * ux - xx[i] * vx + yy[i] * uy - yx[i] * vy;
+        vort[i] = fabs(tx);
+      }
+    /* dist, src/samodel2d.f:243-293 */
+    for (int j = M->jtail1; j <= M->jtail2; ++j) sn[IX(M, j, 1)] = 0.0;
+    for (int j = M->jtail1; j <= M->jtail2; ++j)
+      for (int k = 2; k <= kd; ++k) {
+        int jv = j - 1;
+        size_t i = IX(M, j, k);
+        sn[i] = sqrt(sq(x[i] - 0.5 * (xv[VX(M, jv, 1)] + xv[VX(M, jv + 1, 1)])) +
+                     sq(y[i] - 0.5 * (yv[VX(M, jv, 1)] + yv[VX(M, jv + 1, 1)])));
+      }
+    int jv = M->jtail1 - 1;
+    for (int j = 1; j <= M->jtail1 - 1; ++j)
+      for (int k = 1; k <= kd; ++k) {
+        size_t i = IX(M, j, k);
+        sn[i] = sqrt(sq(x[i] - xv[VX(M, jv, 1)]) + sq(y[i] - yv[VX(M, jv, 1)]));
+      }
+    for (int j = M->jtail2 + 1; j <= jd; ++j)
+      for (int k = 1; k <= kd; ++k) {
+        size_t i = IX(M, j, k);
+        sn[i] = sqrt(sq(x[i] - xv[VX(M, jv, 1)]) + sq(y[i] - yv[VX(M, jv, 1)]));
+      }
+  }
+  /* rhslhs: sjmat = s5; convec, diffus, source (src/samodel2d.f:402-824) */
+  for (size_t i = 0; i < n2; ++i) sjmat[i] = s5[i];
+  /* convec, src/samodel2d.f:453-542 */
+  for (int k = ks - 1; k <= ke + 1; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k);
+      double uu = xx[i] * (u[i] - ug[i]) + xy[i] * (v[i] - vg[i]);
+      double vv = yx[i] * (u[i] - ug[i]) + yy[i] * (v[i] - vg[i]);
+      up[i] = 0.5 * (uu + fabs(uu)); um[i] = 0.5 * (uu - fabs(uu));
+      vp[i] = 0.5 * (vv + fabs(vv)); vm[i] = 0.5 * (vv - fabs(vv));
+    }
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      sjmat[i] = sjmat[i] - up[i] * (q5[i] - q5[IX(M, j - 1, k)]) - um[i] * (q5[IX(M, j + 1, k)] - q5[i]);
+      sjmat[i] = sjmat[i] - vp[i] * (q5[i] - q5[IX(M, j, k - 1)]) - vm[i] * (q5[IX(M, j, k + 1)] - q5[i]);
+    }
+  for (int k = ks - 1; k <= ke + 1; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      double fwd = up[IX(M, j + 1, k)] > 1.0e-12 ? 1.0 : 0.0;
+      double bck = um[IX(M, j - 1, k)] < -1.0e-12 ? 1.0 : 0.0;
+      aj[IX(M, j + 1, k)] = aj[IX(M, j + 1, k)] - fwd * (up[i] + um[i]);
+      bj[i] = bj[i] + up[i] - um[i];
+      cj[IX(M, j - 1, k)] = cj[IX(M, j - 1, k)] + bck * (up[i] + um[i]);
+    }
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k);
+      double fwd = vp[IX(M, j, k + 1)] > 1.0e-12 ? 1.0 : 0.0;
+      double bck = vm[IX(M, j, k - 1)] < -1.0e-12 ? 1.0 : 0.0;
+      ak[IX(M, j, k + 1)] = ak[IX(M, j, k + 1)] - fwd * (vp[i] + vm[i]);
+      bk[i] = bk[i] + vp[i] - vm[i];
+      ck[IX(M, j, k - 1)] = ck[IX(M, j, k - 1)] + bck * (vp[i] + vm[i]);
+    }
+  /* diffus, src/samodel2d.f:545-688 */
+  double rey = c->rey;
+  for (int k = ks - 1; k <= ke + 1; ++k)
+    for (int j = js - 1; j <= je; ++j) {
+      size_t i = IX(M, j, k), i1 = IX(M, j + 1, k);
+      dmxh[i] = 0.5 * (xx[i] + xx[i1]);
+      dmyh[i] = 0.5 * (xy[i] + xy[i1]);
+      double vnulh = 0.5 * (vmul[i] / q1[i] / qj[i] + vmul[i1] / q1[i1] / qj[i1]);
+      double vnuh = 0.5 * (q5[i] + q5[i1]);
+      chp[i] = (1. + SA_cb2) / SA_sigma / rey * (vnulh + vnuh);
+      dnuhp[i] = q5[i1] - q5[i];
+    }
+  for (int j = js; j <= je; ++j)
+    for (int k = ks - 1; k <= ke + 1; ++k) {
+      size_t i = IX(M, j, k), im1 = IX(M, j - 1, k);
+      double dxp = dmxh[i] * xx[i] + dmyh[i] * xy[i];
+      double dxm = dmxh[im1] * xx[i] + dmyh[im1] * xy[i];
+      double c2 = SA_cb2 / SA_sigma / rey;
+      c2 = c2 * (vmul[i] / q1[i] / qj[i] + q5[i]);
+      double dcp = dxp * (chp[i] - c2);
+      double dcm = dxm * (chp[im1] - c2);
+      double ax = 0.0, cx = 0.0;
+      if (k != ke + 1 && k != ks - 1) { ax = MAX(dcm, 0.0); cx = MAX(dcp, 0.0); }
+      sjmat[i] = sjmat[i] - ax * dnuhp[im1] + cx * dnuhp[i];
+      aj[i] = aj[i] - ax;
+      cj[i] = cj[i] - cx;
+      bj[i] = bj[i] + ax + cx;
+    }
+  for (int k = 1; k <= kd; ++k) { aj[IX(M, je + 1, k)] = 0.0; cj[IX(M, js - 1, k)] = 0.0; }
+  for (int k = ks - 1; k <= ke; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k), i1 = IX(M, j, k + 1);
+      dmxh[i] = 0.5 * (yx[i] + yx[i1]);
+      dmyh[i] = 0.5 * (yy[i] + yy[i1]);
+      double vnulh = 0.5 * (vmul[i] / q1[i] / qj[i] + vmul[i1] / q1[i1] / qj[i1]);
+      double vnuh = 0.5 * (q5[i] + q5[i1]);
+      chp[i] = (1. + SA_cb2) / SA_sigma / rey * (vnulh + vnuh);
+      dnuhp[i] = q5[i1] - q5[i];
+    }
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k), km1 = IX(M, j, k - 1);
+      double dyp = dmxh[i] * yx[i] + dmyh[i] * yy[i];
+      double dym = dmxh[km1] * yx[i] + dmyh[km1] * yy[i];
+      double c2 = SA_cb2 / SA_sigma / rey;
+      c2 = c2 * (vmul[i] / q1[i] / qj[i] + q5[i]);
+      double dcp = dyp * (chp[i] - c2);
+      double dcm = dym * (chp[km1] - c2);
+      double ay = 0.0, cy = 0.0;
+      if (j != js - 1 && j != je + 1) { ay = MAX(dcm, 0.0); cy = MAX(dcp, 0.0); }
+      sjmat[i] = sjmat[i] - ay * dnuhp[km1] + cy * dnuhp[i];
+      ak[i] = ak[i] - ay;
+      ck[i] = ck[i] - cy;
+      bk[i] = bk[i] + ay + cy;
+    }
+  for (int j = 1; j <= jd; ++j) { ak[IX(M, j, ke + 1)] = 0.0; ck[IX(M, j, ks - 1)] = 0.0; }
+  /* source, src/samodel2d.f:691-824 (isour = 0, fturf = 0) */
+  {
+    double d2min = 1.e-12, rmax = 10.0, stilim = 1.e-12, fturf = 0.0;
+    double cv1_3 = ipow3(SA_cv1), cw3_6 = ipow6(SA_cw3);
+    for (int k = ks; k <= ke; ++k)
+      for (int j = js; j <= je; ++j) {
+        size_t i = IX(M, j, k);
+        double nt = q5[i];
+        double vnul = vmul[i] / q1[i] / qj[i];
+        double chi = nt / vnul;
+        double d = sn[i];
+        chi = MAX(chi, 1.e-12);
+        double chi3 = ipow3(chi);
+        double fv1 = chi3 / (chi3 + cv1_3);
+        double fv2 = 1. - chi / (1. + chi * fv1);
+        double fv3 = 1.0;
+        double ft2 = fturf * SA_ct3 * exp(-1. * SA_ct4 * chi * chi);
+        double dchi = 1. / vnul;
+        double dfv1 = (3. * cv1_3) * ipow2(chi) * dchi * ipow2(1. / (chi3 + cv1_3));
+        double dfv2 = (fv2 - 1.) / chi / vnul + (1. - fv2) * (1 - fv2) * (dfv1 + fv1 / chi / vnul);
+        double dfv3 = 0.0;
+        double dft2 = (-2. * SA_ct4) * chi * dchi * ft2;
+        double d2 = MAX(ipow2(d), d2min);
+        double stilda = vort[i] * fv3 + vnul / (d2 * SA_akt * SA_akt * rey) * chi * fv2;
+        stilda = MAX(stilda, stilim);
+        double r = nt / (d2 * SA_akt * SA_akt * rey) / stilda;
+        r = MIN(r, rmax);
+        double g = r + SA_cw2 * (ipow6(r) - r);
+        double fw = (1. + cw3_6) / (ipow6(g) + cw3_6);
+        fw = g * pow(fw, 1. / 6.);
+        double dstild = vort[i] * dfv3 + vnul * (dchi * fv2 + chi * dfv2) / (d2 * SA_akt * SA_akt * rey);
+        double dr = vnul * (dchi - chi * dstild / stilda) / stilda / (d2 * SA_akt * SA_akt * rey);
+        double dg = dr * (1. + SA_cw2 * (6. * pow(r, 5.) - 1.));
+        double dfw = pow((1. + cw3_6) / (ipow6(g) + cw3_6), 1. / 6.);
+        dfw = dfw * dg * (1. - ipow6(g) / (ipow6(g) + cw3_6));
+        double pro = SA_cb1 * stilda * (1. - ft2);
+        double des = (SA_cw1 * fw - SA_cb1 / (SA_akt * SA_akt) * ft2) / d2 / rey;
+        double prod = SA_cb1 * stilda * (1. - ft2) * nt;
+        double dest = (SA_cw1 * fw - SA_cb1 / (SA_akt * SA_akt) * ft2) * nt * nt / d2 / rey;
+        double dpro = pro * dstild / stilda - SA_cb1 * stilda * dft2;
+        double ddes = (SA_cw1 * dfw - SA_cb1 / (SA_akt * SA_akt) * dft2) / d2 / rey * vnul * chi;
+        ddes = ddes + des;
+        ddes = ddes * nt;
+        dpro = dpro * nt;
+        sjmat[i] = sjmat[i] + prod - dest;
+        double tk1 = MAX(des * nt - pro, 0.0);
+        double tk2 = MAX(ddes - dpro, 0.0);
+        bk[i] = bk[i] + tk1 + tk2;
+      }
+  }
+  /* scale by the turbulence pseudo time step (src/samodel2d.f:101-131) */
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      double tscal = tscale[i];
+      if (c->p.timeac == 1) {
+        tscal = (double)MAX(ib[i], 0) * (1.0 + 0.002 * sqrt(qj[i])) / (1. + sqrt(qj[i]));
+        tscal = tscal * 1.0;
+        tscal = tscal / (1. + tscal / h);
+      }
+      sjmat[i] = sjmat[i] * tscal;
+      aj[i] = aj[i] * tscal; cj[i] = cj[i] * tscal; ak[i] = ak[i] * tscal; ck[i] = ck[i] * tscal;
+      bjmat[i] = 1. + (bj[i] + bk[i]) * tscal;
+      s1[i] = 0.; s2[i] = 0.;
+    }
+  /* lsolvej, src/samodel2d.f:1306-1344 */
+  for (int k = ks; k <= ke; ++k) {
+    size_t i = IX(M, js, k);
+    double bb = 1. / bjmat[i];
+    sjmat[i] = sjmat[i] * bb;
+    cj[i] = cj[i] * bb;
+  }
+  for (int j = js + 1; j <= je; ++j)
+    for (int k = ks; k <= ke; ++k) {
+      size_t i = IX(M, j, k), i1 = IX(M, j - 1, k);
+      double bb = 1. / (bjmat[i] - aj[i] * cj[i1]);
+      sjmat[i] = (sjmat[i] - aj[i] * sjmat[i1]) * bb;
+      cj[i] = cj[i] * bb;
+    }
+  for (int j = je - 1; j >= js; --j)
+    for (int k = ks; k <= ke; ++k) {
+      size_t i = IX(M, j, k);
+      sjmat[i] = sjmat[i] - cj[i] * sjmat[IX(M, j + 1, k)];
+    }
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      s1[i] = s1[i] + sjmat[i];
+      sjmat[i] = (s1[i] - s2[i]) * bjmat[i];
+    }
+  /* lsolvek, src/samodel2d.f:1347-1385 */
+  for (int j = js; j <= je; ++j) {
+    size_t i = IX(M, j, ks);
+    double bb = 1. / bjmat[i];
+    sjmat[i] = sjmat[i] * bb;
+    ck[i] = ck[i] * bb;
+  }
+  for (int k = ks + 1; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k), i1 = IX(M, j, k - 1);
+      double bb = 1. / (bjmat[i] - ak[i] * ck[i1]);
+      sjmat[i] = (sjmat[i] - ak[i] * sjmat[i1]) * bb;
+      ck[i] = ck[i] * bb;
+    }
+  for (int k = ke - 1; k >= ks; --k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      sjmat[i] = sjmat[i] - ck[i] * sjmat[IX(M, j, k + 1)];
+    }
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      s2[i] = s2[i] + sjmat[i];
+      sjmat[i] = s2[i];
+    }
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      sjmat[i] = relfac * sjmat[i];
+    }
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      q5[i] = MAX((q5[i] + sjmat[i] * (double)MAX(ib[i], 0)), vnulim);
+    }
+  turbc_mesh(c, M, nsp, u, v);
+  /* c_turm, src/samodel2d.f:372-399 */
+  for (size_t i = 0; i < n2; ++i) {
+    double vnul = vmul[i] / (q1[i] * qj[i]);
+    double chi = q5[i] / vnul;
+    double fv1 = ipow3(chi) / (ipow3(chi) + ipow3(SA_cv1));
+    turmu[i] = q1[i] * qj[i] * q5[i] * fv1;
+  }
+  free(sjmat); free(bjmat); free(aj); free(bj); free(cj); free(ak); free(bk); free(ck);
+  free(vmul); free(vort); free(sn); free(u); free(v); free(s1); free(s2);
+  free(up); free(um); free(vp); free(vm); free(chp); free(dnuhp); free(dmxh); free(dmyh);
+}
+
 /* compute_residual, src/step.f:221-300 */
 int32_t or_compute_residual(or_case* c) {
   for (int im = 0; im < c->nmesh; ++im) {
@@ -1213,10 +1804,19 @@ int32_t or_compute_residual(or_case* c) {
       bc_mesh(c, M, nsp);
       int rc = rhsup_mesh(c, M, nsp);
       if (rc) return rc;
-      if (!c->p.invisc) return OT2D_ERR_UNSUPPORTED;
+      if (!c->p.invisc) visrhs_mesh(c, M, nsp);
       if (c->timespectral) timespectralrhs(c, M, nsp);
     }
   }
+  return OT2D_OK;
+}
+
+/* solve_turbulence_equations, src/step.f:303-329 (SA only) */
+int32_t or_turbulence(or_case* c) {
+  if (c->lamin) return OT2D_OK;
+  if (c->iturb != 1 || c->p.itrans) return OT2D_ERR_UNSUPPORTED;
+  for (int im = 0; im < c->nmesh; ++im)
+    for (int nsp = 0; nsp < c->nspec; ++nsp) vmu_sa_mesh(c, &c->m[im], nsp);
   return OT2D_OK;
 }
 
@@ -1581,7 +2181,7 @@ int32_t or_step(or_case* c, double* norms, int32_t* argmax) {
       bc_mesh(c, M, nsp);
       int rc = rhsup_mesh(c, M, nsp);
       if (rc) return rc;
-      if (!c->p.invisc) return OT2D_ERR_UNSUPPORTED;
+      if (!c->p.invisc) visrhs_mesh(c, M, nsp);
       if (c->timespectral) timespectralrhs(c, M, nsp);
     }
     for (int nsp = 0; nsp < c->nspec; ++nsp) {
@@ -1600,7 +2200,10 @@ int32_t or_step(or_case* c, double* norms, int32_t* argmax) {
       if (c->p.ilhs != 1 || c->p.iprecon) return OT2D_ERR_UNSUPPORTED;
       lhs_mesh(c, M, nsp);
     }
-    if (!c->lamin) return OT2D_ERR_UNSUPPORTED;
+    if (!c->lamin) {
+      if (c->iturb != 1 || c->p.itrans) return OT2D_ERR_UNSUPPORTED;
+      for (int nsp = 0; nsp < c->nspec; ++nsp) vmu_sa_mesh(c, M, nsp);
+    }
     for (int nsp = 0; nsp < c->nspec; ++nsp) {
       const int32_t* ib = M->iblank + (size_t)nsp * M->n2;
       for (int n = 1; n <= 4; ++n) {

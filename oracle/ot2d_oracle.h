@@ -32,6 +32,7 @@ int32_t or_bc(or_case* c);
 int32_t or_compute_residual(or_case* c);
 int32_t or_rbc(or_case* c);
 int32_t or_lhs(or_case* c);
+int32_t or_turbulence(or_case* c);
 int32_t or_monitor(or_case* c, int32_t* mstop);
 int32_t or_step(or_case* c, double* norms, int32_t* argmax);
 int32_t or_iterate(or_case* c, int32_t niter, double* hist, int32_t* done);

@@ -101,6 +101,7 @@ program ref_harness
         if (.not. iprecon) Mp = 1.0
         dualtime = dtpseudo(itn)
         if (dump) call dump_r("qin", im, q(igq), jd*kd*nspec*nq)
+        if (dump) call dump_r("turmuin", im, turmu(ig), jd*kd*nspec)
         call time_step(q(igq), xx(ig), xy(ig), yx(ig), yy(ig), ug(ig), vg(ig), &
              tscale(ig), bt(ig), iblank(ig), jd, kd)
         if (dump) call dump_r("tscale", im, tscale(ig), jd*kd*nspec)
