@@ -2325,8 +2325,10 @@ int32_t or_forces(or_case* c, int32_t im, int32_t nsp, double out[6]) {
         uxi = .5 * (UQ(j + 1, k) - UQ(j - 1, k));
         vxi = .5 * (VQ(j + 1, k) - VQ(j - 1, k));
       }
-      double ueta = -1.5 * UQ(j, k) + 2. * UQ(j, k2) - .5 * UQ(j, k3);
-      double veta = -1.5 * VQ(j, k) + 2. * VQ(j, k2) - .5 * VQ(j, k3);
+      /* -1.5*q(j,k,2)/q(j,k,1): (-1.5*q2)/q1, src/forces.f:299-302 */
+      size_t i1 = IX(M, j, k), i2 = IX(M, j, k2), i3 = IX(M, j, k3);
+      double ueta = -1.5 * q2[i1] / q1[i1] + 2. * q2[i2] / q1[i2] - .5 * q2[i3] / q1[i3];
+      double veta = -1.5 * q3[i1] / q1[i1] + 2. * q3[i2] / q1[i2] - .5 * q3[i3] / q1[i3];
       size_t i = IX(M, j, k);
       double xix = xx[i], xiy = xy[i], etax = yx[i], etay = yy[i];
       double sfdiv = 2. / 3 * (uxi * xix + ueta * etax + vxi * xiy + veta * etay);

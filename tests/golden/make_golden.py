@@ -1,0 +1,73 @@
+"""Generate the golden fixtures under tests/golden/ from the reference itself.
+
+Runs oracle/_ref/ref_harness (the reference OVERTURNS-2D subroutines built
+from their own sources by oracle/Makefile) on small C-grid decks written by
+turns2d_urans_git_amd.grid and stores, per case, the grid, the deck, the
+per-stage arrays of one iteration, the residual history, the forces and the
+final state as a compressed .npz.  The fixtures are data only; this script is
+how they were made:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from turns2d_urans_git_amd import grid  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+EULER = dict(fsmach=0.5, alfa=1.25, invisc=True, lamin=True, irhsy=-3, ilim=1, ilhs=1,
+             iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1, timeac=1.0)
+SA = dict(fsmach=0.5, alfa=2.0, rey=1.0e6, invisc=False, lamin=False, iturb=1, vnuinf=3.0,
+          irhsy=-3, ilim=1, ilhs=1, iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1,
+          itnmax=1, timeac=1.0)
+
+CASES = {
+    # name: (jmax, kmax, ds_wall, camber m, p, t, deck, nsteps, dump iteration)
+    "euler_33x17": (33, 17, 1e-3, 0.0, 0.0, 0.12, EULER, 40, 20),
+    "sa_41x21": (41, 21, 1e-4, 0.02, 0.4, 0.12, SA, 40, 35),
+}
+
+STAGES = ["qin", "turmuin", "tscale", "qbc", "res", "rbc", "sdt", "dq", "qlu", "qturb", "qout", "turmu"]
+
+
+def run_case(name, jmax, kmax, ds, m, p, t, deck, nsteps, dumpat):
+    g = grid.naca4_cgrid(jmax, kmax, m=m, p=p, t=t, ds_wall=ds)
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_case(d, g, nsteps=nsteps, nrest=10**6, npnorm=1, **deck)
+        out = os.path.join(d, "out")
+        os.makedirs(out)
+        env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps), OT2D_DUMPAT=str(dumpat))
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([HARNESS], cwd=d, stdin=fin, env=env, check=True,
+                           stdout=subprocess.DEVNULL)
+        arrays = {}
+        for st in STAGES + ["qfinal", "turmufinal", "x", "y", "xx", "xy", "yx", "yy", "qinit"]:
+            fn = os.path.join(out, f"{st}_m1.bin")
+            if os.path.exists(fn):
+                arrays[st] = np.fromfile(fn)
+        norms = np.loadtxt(os.path.join(out, "norms.txt"), comments="#", ndmin=2)
+        forces = np.loadtxt(os.path.join(out, "forces.txt"), comments="#", ndmin=2)
+    deck_arr = {f"deck_{k}": np.asarray(float(v)) for k, v in deck.items()}
+    np.savez_compressed(
+        os.path.join(HERE, f"{name}.npz"),
+        xv=g.xv, yv=g.yv, jtail1=np.asarray(g.jtail1), nsteps=np.asarray(nsteps),
+        dumpat=np.asarray(dumpat), norms=norms, forces=forces, **arrays, **deck_arr)
+    print(name, "norm[-1]", norms[-1, 3], "cl", forces[-1, 1])
+
+
+if __name__ == "__main__":
+    if not os.path.exists(HARNESS):
+        sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
+    for name, spec in CASES.items():
+        run_case(name, *spec)

@@ -90,3 +90,52 @@ def assert_cellwise(a, b, rtol, what=""):
     bad = np.abs(a - b) > rtol * (np.abs(b) + scale)
     assert not bad.any(), (f"{what}: {bad.sum()} cells differ; max abs diff "
                           f"{np.abs(a - b).max():.3e}, scale {scale:.3e}")
+
+
+# ---------------------------------------------------------------- fixtures
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def load_golden(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    deck = {k[5:]: float(z[k]) for k in z.files if k.startswith("deck_")}
+    ints = {"invisc", "lamin", "irhsy", "ilim", "ilhs", "iconstant_cfl", "iunst", "ntac",
+            "itnmax", "iturb"}
+    deck = {k: (int(v) if k in ints else v) for k, v in deck.items()}
+    g = grid.CGrid(xv=np.asfortranarray(z["xv"]), yv=np.asfortranarray(z["yv"]),
+                   jtail1=int(z["jtail1"]), jmax=z["xv"].shape[0], kmax=z["xv"].shape[1])
+    return z, g, deck
+
+
+def golden_case(lib, prefix, name, errfn=None, **extra):
+    z, g, deck = load_golden(name)
+    deck = dict(deck, **extra)
+    return z, make_case(lib, prefix, g, deck, errfn=errfn)
+
+
+def replay_stages(case, z, ol=None):
+    """Replay the stage sequence of the fixture's dump iteration on ``case``
+    starting from the fixture's input state; returns {stage: array}."""
+    jd, kd, nq, nv, _, _ = case.dims[0]
+    sh_q, sh_s, sh_1 = (jd, kd, 1, nq), (jd, kd, 1, nv), (jd, kd, 1)
+    case.set_state(0, z["qin"].reshape(sh_q, order="F"))
+    if "turmuin" in z.files:
+        case.set_field(0, abi.F_TURMU, z["turmuin"].reshape(sh_1, order="F"))
+    case.istep0 = int(z["dumpat"])
+    out = {}
+    case.time_step()
+    out["tscale"] = case.get_field(0, abi.F_TSCALE)
+    case.set_field(0, abi.F_S, np.zeros(sh_s))
+    case.compute_residual()
+    out["qbc"] = case.get_state(0)
+    out["res"] = case.get_field(0, abi.F_S)
+    case.rbc()
+    out["rbc"] = case.get_field(0, abi.F_S)
+    case.lhs()
+    out["dq"] = case.get_field(0, abi.F_S)
+    out["qlu"] = case.get_state(0)
+    if ol is not None:
+        ol(case.h)
+        out["qturb"] = case.get_state(0)
+        out["turmu"] = case.get_field(0, abi.F_TURMU)
+    return out

@@ -1541,8 +1541,10 @@ __global__ __launch_bounds__(256) void k_forces(const DevMesh* __restrict__ Ms, 
       if (j == j1) { uxi = U(j + 1, k) - U(j, k); vxi = V(j + 1, k) - V(j, k); }
       else if (j == j2) { uxi = U(j, k) - U(j - 1, k); vxi = V(j, k) - V(j - 1, k); }
       else { uxi = .5 * (U(j + 1, k) - U(j - 1, k)); vxi = .5 * (V(j + 1, k) - V(j - 1, k)); }
-      double ueta = -1.5 * U(j, k) + 2. * U(j, k2) - .5 * U(j, k3);
-      double veta = -1.5 * V(j, k) + 2. * V(j, k2) - .5 * V(j, k3);
+      // -1.5*q(j,k,2)/q(j,k,1) is (-1.5*q2)/q1 (src/forces.f:299-302)
+      long long i1 = cell(M, j, k), i2 = cell(M, j, k2), i3 = cell(M, j, k3);
+      double ueta = -1.5 * q2[i1] / q1[i1] + 2. * q2[i2] / q1[i2] - .5 * q2[i3] / q1[i3];
+      double veta = -1.5 * q3[i1] / q1[i1] + 2. * q3[i2] / q1[i2] - .5 * q3[i3] / q1[i3];
       double xix = M.xx[i], xiy = M.xy[i], etax = M.yx[i], etay = M.yy[i];
       double sfdiv = 2. / 3 * (uxi * xix + ueta * etax + vxi * xiy + veta * etay);
       double sxx = amu * (2. * (uxi * xix + ueta * etax) - sfdiv);
