@@ -134,6 +134,7 @@ int32_t ot2d_bc(ot2d_case* c);                            /* src/bc.f:2         
 int32_t ot2d_compute_residual(ot2d_case* c);              /* src/step.f:221     */
 int32_t ot2d_rbc(ot2d_case* c);                           /* src/bc.f:88        */
 int32_t ot2d_lhs(ot2d_case* c);    /* s*=tscale, ilu2d   src/step.f:141-159, src/lhs.f:470 */
+int32_t ot2d_turbulence(ot2d_case* c);  /* solve_turbulence_equations, src/step.f:303 (SA) */
 int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop);       /* src/step.f:332     */
 
 /* One full step of every mesh (src/step.f:2-218) followed by monitor, as

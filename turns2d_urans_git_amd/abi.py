@@ -97,6 +97,7 @@ def bind(lib: C.CDLL, prefix: str):
         "compute_residual": (C.c_int32, [C.c_void_p]),
         "rbc": (C.c_int32, [C.c_void_p]),
         "lhs": (C.c_int32, [C.c_void_p]),
+        "turbulence": (C.c_int32, [C.c_void_p]),
         "monitor": (C.c_int32, [C.c_void_p, P(C.c_int32)]),
         "step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "iterate": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_int32)]),
@@ -231,6 +232,9 @@ class CaseAPI:
 
     def lhs(self):
         return self._check(self._fns["lhs"](self.h), "lhs")
+
+    def turbulence(self):
+        return self._check(self._fns["turbulence"](self.h), "turbulence")
 
     def monitor(self) -> int:
         m = C.c_int32()

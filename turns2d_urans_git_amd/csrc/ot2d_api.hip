@@ -48,6 +48,7 @@ struct MeshBuf {
   double *xv = nullptr, *yv = nullptr, *ugv = nullptr, *vgv = nullptr;
   int* iblank = nullptr;
   double* lu = nullptr;
+  double* sa = nullptr;  // 10 (jd,kd) planes per instance when SA is on
   size_t lu_per = 0;  // doubles of LU workspace per instance
   bool has_grid = false;
 };
@@ -207,7 +208,6 @@ static bool bc_in_range(const DevBC& b, int jd, int kd) {
         return in(b.js, k + 2 * iadd) && in(b.je, k + 2 * iadd) && b.ks == b.ke;
       }
       if (iadir == 1) {
-        if (b.type >= 3 && b.type <= 5) return true;  // not implemented for j walls
         int j = b.idir == 1 ? b.je : b.js;
         return in(j + 2 * iadd, b.ks) && in(j + 2 * iadd, b.ke) && b.js == b.je;
       }
@@ -245,7 +245,7 @@ static void free_case(ot2d_case* c) {
     dfree(mb.q); dfree(mb.s); dfree(mb.x); dfree(mb.y); dfree(mb.xx); dfree(mb.xy);
     dfree(mb.yx); dfree(mb.yy); dfree(mb.ug); dfree(mb.vg); dfree(mb.turmu);
     dfree(mb.tscale); dfree(mb.bt); dfree(mb.xv); dfree(mb.yv); dfree(mb.ugv);
-    dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu);
+    dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa);
   }
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
   dfree(c->d_partials); dfree(c->d_counter); dfree(c->d_norms); dfree(c->d_tspec);
@@ -277,7 +277,11 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
   ot2d_case* c = new ot2d_case();
   c->p = *p;
   derive_counts(p, &c->nq, &c->nv, &c->lamin, &c->iturb);
-  if (!p->invisc) { free_case(c); return fail(OT2D_ERR_UNSUPPORTED, "viscous flow not supported yet"); }
+  if (!c->lamin && (c->iturb != 1 || p->itrans != 0)) {
+    free_case(c);
+    return fail(OT2D_ERR_UNSUPPORTED, "turbulence model iturb=%d itrans=%d: only Spalart-Allmaras (iturb=1)",
+                p->iturb, p->itrans);
+  }
   c->nspec = p->timespectral ? p->nspec : 1;
   c->nmesh = nmesh;
   c->c = make_const(p, c->iturb, 4);
@@ -339,6 +343,7 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
     A(mb.iblank, mb.n2 * ns);
     mb.lu_per = lu_workspace_bytes(mb.jd, mb.kd) / sizeof(double);
     A(mb.lu, mb.lu_per * ns);
+    if (!c->lamin) A(mb.sa, 10 * mb.n2 * ns);
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
     ntile_total += tj * tk * ns;
   }
@@ -387,6 +392,15 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
       M.xv0 = mb.xv; M.yv0 = mb.yv;
       M.iblank = mb.iblank + o;
       M.lu = mb.lu + mb.lu_per * nsp;
+      M.bodyflag = 0;
+      for (int ib = 0; ib < md[im].nbc; ++ib)
+        if (md[im].bc[ib].ibtyp == 5) M.bodyflag = 1;  // src/init.f:59-66
+      if (mb.sa) {
+        double* b = mb.sa + 10 * mb.n2 * nsp;
+        M.sa_vmul = b; M.sa_u = b + mb.n2; M.sa_v = b + 2 * mb.n2; M.sa_sn = b + 3 * mb.n2;
+        M.sa_sj = b + 4 * mb.n2; M.sa_aj = b + 5 * mb.n2; M.sa_cj = b + 6 * mb.n2;
+        M.sa_ak = b + 7 * mb.n2; M.sa_ck = b + 8 * mb.n2; M.sa_bb = b + 9 * mb.n2;
+      }
       M.status = c->d_status;
       M.istep0 = c->d_istep0;
       M.stepc = c->d_stepc;
@@ -449,6 +463,8 @@ extern "C" int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, con
   // instance 0 first (the other instances rotate its vertices)
   HIPCHK(launch_init_geometry(c->dM + (size_t)im * c->nspec, 1, c->stream));
   if (c->nspec > 1) HIPCHK(launch_init_geometry(c->dM + (size_t)im * c->nspec + 1, c->nspec - 1, c->stream));
+  if (mb.sa) HIPCHK(launch_sa_dist(c->dM + (size_t)im * c->nspec, c->hM.data() + (size_t)im * c->nspec,
+                                    c->nspec, c->stream));
   mb.has_grid = true;
   c->istep0 = 0;
   HIPCHK(hipMemsetAsync(c->d_istep0, 0, sizeof(int), c->stream));
@@ -565,8 +581,12 @@ extern "C" int32_t ot2d_bc(ot2d_case* c) {
 
 static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms) {
   HIPCHK(launch_bc(c->dM, nmi(c), zero_halo_s, c->stream));
-  HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms, c->stream));
+  // the norms are taken on the complete residual (src/step.f:46-93)
+  const bool later = !c->p.invisc || c->p.timespectral;
+  HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms && !later, c->stream));
+  if (!c->p.invisc) HIPCHK(launch_visrhs(c->dM, c->hM.data(), nmi(c), c->stream));
   if (c->p.timespectral) HIPCHK(launch_tsrhs(c->dM, c->hM.data(), nmi(c), c->stream));
+  if (norms && later) HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
   return OT2D_OK;
 }
 
@@ -608,8 +628,24 @@ static int32_t enqueue_step(ot2d_case* c) {
   HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream));
   if (int32_t rc = enqueue_residual(c, 1, 1)) return rc;
   HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));
-  HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, c->stream));
+  if (c->lamin) {
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, c->stream));
+  } else {
+    // vmu_sa reads the state ilu2d leaves (before the q update), src/step.f:189-211
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, c->stream));
+    HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), c->stream));
+    HIPCHK(launch_update(c->dM, c->hM.data(), nmi(c), c->stream));
+    HIPCHK(launch_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
+  }
   return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_turbulence(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (c->lamin) return OT2D_OK;
+  if (int32_t rc = clear_status(c)) return rc;
+  HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), c->stream));
+  return sync(c);
 }
 
 static int32_t read_status(ot2d_case* c, int* st) {

@@ -48,6 +48,10 @@ struct DevMesh {
   double *xv, *yv, *ugv, *vgv;  // vertex arrays (jmax,kmax) of this instance
   const double *xv0, *yv0;      // instance-1 vertices (time-spectral rotation)
   int* iblank;
+  int bodyflag;              // mesh has a wall segment (type 5)
+  // SA workspace (jd,kd) planes: laminar viscosity, u, v, wall distance and
+  // the rows of the two tridiagonal systems (src/samodel2d.f:22-30)
+  double *sa_vmul, *sa_u, *sa_v, *sa_sn, *sa_sj, *sa_aj, *sa_cj, *sa_ak, *sa_ck, *sa_bb;
   double* lu;                // LU-SGS hyperplane-major records (workspace)
   // per-case bookkeeping (shared by all instances of a case)
   int* status;               // 0 ok, 1 mstop, 2 norm
@@ -92,6 +96,7 @@ hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipSt
 hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms,
                            hipStream_t st);
+hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st);
 hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st);
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32,
@@ -102,5 +107,8 @@ hipError_t launch_step_begin(int* istep0, int* status, int* stepc, int ncase, hi
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd);
+hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_lds_need(int nk, int ni, int fp32);
 }  // namespace ot2d

@@ -492,6 +492,9 @@ __device__ __forceinline__ void face_states(const double* sf0, int stv, int stc,
   }
 }
 
+__device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool valid, const double* s,
+                          double* redbuf);
+
 // Residual of one 32x8 tile: rhsup for nei=1 (src/eulerrhs.f:2-280), fluxes
 // of the tile's faces computed once into LDS, plus the residual-norm partials
 // of step (src/step.f:60-93).  Writes s(:,:,1:4) of the tile's interior cells.
@@ -584,30 +587,43 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
   // ---- flux balance
   int cx = tid % TJ, ry = tid / TJ;
   int j = j0 + cx, k = k0 + ry;
-  double rs = 0.0, rr1 = 0.0, rmax = 0.0, key = 0.0;
+  double sv[4] = {0.0, 0.0, 0.0, 0.0};
   bool valid = (j <= M.je && k <= M.ke);
   if (valid) {
     long long i = cell(M, j, k);
-    double ibp = (double)imax(M.iblank[i], 0);
-    double s[4];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      s[n] = 0.0 - (sF[n][ry][cx + 1] - sF[n][ry][cx]);
-      s[n] = s[n] - (sG[n][ry + 1][cx] - sG[n][ry][cx]);
-      M.s[n][i] = s[n];
+      sv[n] = 0.0 - (sF[n][ry][cx + 1] - sF[n][ry][cx]);
+      sv[n] = sv[n] - (sG[n][ry + 1][cx] - sG[n][ry][cx]);
+      M.s[n][i] = sv[n];
     }
-    if (do_norms) {
-      double m1 = s[0] * ibp, m2 = s[1] * ibp, m3 = s[2] * ibp, m4 = s[3] * ibp;
-      rs = m1 * m1 + m2 * m2 + m3 * m3 + m4 * m4;
-      rr1 = sq(s[0]);
-      // first maximum in (k, n, j) order (src/step.f:67-78)
-      for (int n = 0; n < 4; ++n) {
-        double a = fabs(s[n]) * ibp;
-        if (a > rmax) { rmax = a; key = (double)(((long long)(k - 1) * 4 + n) * jd + (j - 1)); }
-      }
+    // the reference clears all of s at step start (src/step.f:41)
+    for (int n = 4; n < M.nv; ++n) M.s[n][i] = 0.0;
+  }
+  if (do_norms) norm_tile(M, T, j, k, valid, sv, &red[0][0]);
+}
+
+// residual norms of step (src/step.f:53-115) for the cells of one tile:
+// block partial sums, then the last tile of the instance reduces the
+// partials in a fixed order and raises the "norm" status
+__device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool valid, const double* s,
+                          double* redbuf) {
+  double(*red)[RES_THREADS] = reinterpret_cast<double(*)[RES_THREADS]>(redbuf);
+  const int tid = threadIdx.x;
+  const int jd = M.jd, kd = M.kd;
+  double rs = 0.0, rr1 = 0.0, rmax = 0.0, key = 0.0;
+  if (valid) {
+    long long i = cell(M, j, k);
+    double ibp = (double)imax(M.iblank[i], 0);
+    double m1 = s[0] * ibp, m2 = s[1] * ibp, m3 = s[2] * ibp, m4 = s[3] * ibp;
+    rs = m1 * m1 + m2 * m2 + m3 * m3 + m4 * m4;
+    rr1 = sq(s[0]);
+    // first maximum in (k, n, j) order (src/step.f:67-78)
+    for (int n = 0; n < 4; ++n) {
+      double a = fabs(s[n]) * ibp;
+      if (a > rmax) { rmax = a; key = (double)(((long long)(k - 1) * 4 + n) * jd + (j - 1)); }
     }
   }
-  if (!do_norms) return;
   red[0][tid] = rs; red[1][tid] = rr1; red[2][tid] = rmax; red[3][tid] = valid && rmax > 0.0 ? key : 1e300;
   __syncthreads();
   for (int off = RES_THREADS / 2; off > 0; off >>= 1) {
@@ -677,6 +693,24 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
     *M.counter = 0u;
     if (rsum > 1000.0) atomicCAS(M.status, 0, 2);
   }
+}
+
+// residual norms over the final residual when viscous / time-spectral terms
+// were added after k_residual
+__global__ __launch_bounds__(RES_THREADS) void k_norms(const DevMesh* __restrict__ Ms,
+                                                       const ResTile* __restrict__ tiles) {
+  __shared__ double red[4][RES_THREADS];
+  const ResTile T = tiles[blockIdx.x];
+  const DevMesh& M = Ms[T.mesh];
+  const int tid = threadIdx.x;
+  int j = T.j0 + tid % RES_TJ, k = T.k0 + tid / RES_TJ;
+  bool valid = (j <= M.je && k <= M.ke);
+  double sv[4] = {0.0, 0.0, 0.0, 0.0};
+  if (valid) {
+    long long i = cell(M, j, k);
+    for (int n = 0; n < 4; ++n) sv[n] = M.s[n][i];
+  }
+  norm_tile(M, T, j, k, valid, sv, &red[0][0]);
 }
 
 // ===================================================== time-spectral coupling
@@ -1610,6 +1644,11 @@ hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st) {
 
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, hipStream_t st) {
   hipLaunchKernelGGL(k_residual, dim3(ntiles), dim3(RES_THREADS), 0, st, dM, tiles, do_norms);
+  return hipGetLastError();
+}
+
+hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL(k_norms, dim3(ntiles), dim3(RES_THREADS), 0, st, dM, tiles);
   return hipGetLastError();
 }
 

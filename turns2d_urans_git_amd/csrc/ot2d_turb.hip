@@ -1,0 +1,701 @@
+// gfx950 kernels of the viscous residual and the Spalart-Allmaras model.
+//
+//   visrhs (src/visrhs.f:2-287)        k_visrhs      one thread per interior cell,
+//                                                    gathers its thin-layer fluxes and
+//                                                    cross-derivative terms
+//   vmu_sa (src/samodel2d.f:2-197)     k_sa_prim     laminar viscosity, u, v (all cells)
+//                                      k_turbc       SA boundary conditions (1 block / instance)
+//                                      k_sa_rhs      convection, diffusion, source and the
+//                                                    two scalar tridiagonal systems, in
+//                                                    gather form (one thread per cell)
+//                                      k_sa_solve_j  tridiagonal solve along j (thread per k)
+//                                      k_sa_solve_k  tridiagonal solve along k (thread per j),
+//                                                    nu-tilde update and eddy viscosity
+//
+// Operand order follows the reference expressions (built with
+// -ffp-contract=off), so results agree with the reference to the rounding of
+// the device's sqrt/div/pow.
+#include "ot2d_dev.h"
+
+namespace ot2d {
+
+__device__ __forceinline__ double sqd(double a) { return a * a; }
+__device__ __forceinline__ int imx(int a, int b) { return a > b ? a : b; }
+
+// ====================================================================== visrhs
+struct VisPrim {
+  double u, v, e, vnu, gkap;
+};
+
+// per-cell quantities of the visrhs line loops (src/visrhs.f:47-57)
+__device__ __forceinline__ VisPrim vis_prim(const DevMesh& M, long long i, double c2b, double c2bp, double prtr) {
+  VisPrim p;
+  double ra = 1. / M.q[0][i];
+  p.u = M.q[1][i] * ra;
+  p.v = M.q[2][i] * ra;
+  p.e = M.q[3][i] * ra - .5 * (sqd(p.u) + sqd(p.v));
+  double tt = M.c.ggm1 * p.e;
+  double vmue = c2bp * tt * sqrt(tt) / (c2b + tt);
+  double turm = M.turmu[i];
+  p.vnu = vmue + turm;
+  p.gkap = vmue + prtr * turm;
+  return p;
+}
+
+struct VisCoef {
+  double c1, c2, c5, c4;          // thin-layer coefficients (b1v.. or a1v..)
+  double d1, d2, d4, d5, d7;      // cross-derivative coefficients
+};
+
+// eta direction (j = const line) coefficients of cell (j,k): thin layer from
+// the cell, cross terms from its j-1, j+1 neighbours (src/visrhs.f:59-92)
+__device__ __forceinline__ VisCoef eta_coef(const DevMesh& M, int j, int k, const VisPrim& p, double dre,
+                                            double gkpr) {
+  long long i = cell(M, j, k), im = cell(M, j - 1, k), ip = cell(M, j + 1, k);
+  const double *xx = M.xx, *xy = M.xy, *yx = M.yx, *yy = M.yy, *qj = M.q[M.nq - 1];
+  double rj = 1. / qj[i], rjm = 1. / qj[im], rjp = 1. / qj[ip];
+  VisCoef c;
+  double b4 = (sqd(yx[i]) + sqd(yy[i])) * rj;
+  c.c1 = (b4 + sqd(yx[i]) / 3. * rj) * p.vnu * dre;
+  c.c2 = (b4 + sqd(yy[i]) / 3. * rj) * p.vnu * dre;
+  c.c5 = (yx[i] * yy[i] / 3. * rj) * p.vnu * dre;
+  c.c4 = b4 * gkpr * p.gkap * dre;
+  double t;
+  t = (4. / 3. * xx[im] * yx[im] + xy[im] * yy[im]) * rjm;
+  t = t + (4. / 3. * xx[ip] * yx[ip] + xy[ip] * yy[ip]) * rjp;
+  c.d1 = p.vnu * dre * t;
+  t = (4. / 3. * xy[im] * yy[im] + xx[im] * yx[im]) * rjm;
+  t = t + (4. / 3. * xy[ip] * yy[ip] + xx[ip] * yx[ip]) * rjp;
+  c.d2 = p.vnu * dre * t;
+  t = (xy[im] * yy[im] + xx[im] * yx[im]) * rjm;
+  t = t + (xy[ip] * yy[ip] + xx[ip] * yx[ip]) * rjp;
+  c.d4 = t * gkpr * p.gkap * dre;
+  t = (-2. / 3. * xy[im] * yx[im] + xx[im] * yy[im]) * rjm;
+  t = t + (-2. / 3. * xy[ip] * yx[ip] + xx[ip] * yy[ip]) * rjp;
+  c.d5 = p.vnu * dre * t;
+  t = (-2. / 3. * xx[im] * yy[im] + xy[im] * yx[im]) * rjm;
+  t = t + (-2. / 3. * xx[ip] * yy[ip] + xy[ip] * yx[ip]) * rjp;
+  c.d7 = p.vnu * dre * t;
+  return c;
+}
+
+// xi direction (k = const line) coefficients (src/visrhs.f:184-217)
+__device__ __forceinline__ VisCoef xi_coef(const DevMesh& M, int j, int k, const VisPrim& p, double dre,
+                                           double gkpr) {
+  long long i = cell(M, j, k), km = cell(M, j, k - 1), kp = cell(M, j, k + 1);
+  const double *xx = M.xx, *xy = M.xy, *yx = M.yx, *yy = M.yy, *qj = M.q[M.nq - 1];
+  double rj = 1. / qj[i], rkm = 1. / qj[km], rkp = 1. / qj[kp];
+  VisCoef c;
+  double a4 = (sqd(xx[i]) + sqd(xy[i])) * rj;
+  c.c1 = (a4 + sqd(xx[i]) / 3. * rj) * p.vnu * dre;
+  c.c2 = (a4 + sqd(xy[i]) / 3. * rj) * p.vnu * dre;
+  c.c5 = (xx[i] * xy[i] / 3. * rj) * p.vnu * dre;
+  c.c4 = a4 * gkpr * p.gkap * dre;
+  double t;
+  t = (4. / 3. * xx[km] * yx[km] + xy[km] * yy[km]) * rkm;
+  t = t + (4. / 3. * xx[kp] * yx[kp] + xy[kp] * yy[kp]) * rkp;
+  c.d1 = p.vnu * dre * t;
+  t = (4. / 3. * xy[km] * yy[km] + xx[km] * yx[km]) * rkm;
+  t = t + (4. / 3. * xy[kp] * yy[kp] + xx[kp] * yx[kp]) * rkp;
+  c.d2 = p.vnu * dre * t;
+  t = (xy[km] * yy[km] + xx[km] * yx[km]) * rkm;
+  t = t + (xy[kp] * yy[kp] + xx[kp] * yx[kp]) * rkp;
+  c.d4 = t * gkpr * p.gkap * dre;
+  t = (-2. / 3. * xy[km] * yx[km] + xx[km] * yy[km]) * rkm;
+  t = t + (-2. / 3. * xy[kp] * yx[kp] + xx[kp] * yy[kp]) * rkp;
+  c.d5 = p.vnu * dre * t;
+  t = (-2. / 3. * xx[km] * yy[km] + xy[km] * yx[km]) * rkm;
+  t = t + (-2. / 3. * xx[kp] * yy[kp] + xy[kp] * yx[kp]) * rkp;
+  c.d7 = p.vnu * dre * t;
+  return c;
+}
+
+struct Tri { double f2, f3, f4; };
+
+// thin-layer flux across the face between cells a (low) and b (high) of a line
+__device__ __forceinline__ Tri vis_face(const VisPrim& pa, const VisCoef& ca, const VisPrim& pb, const VisCoef& cb) {
+  double b1 = cb.c1 + ca.c1, b2 = cb.c2 + ca.c2, b5 = cb.c5 + ca.c5;
+  double t4 = pb.u * cb.c1 + pa.u * ca.c1;
+  double t5 = pb.v * cb.c2 + pa.v * ca.c2;
+  double t6 = pb.u * cb.c5 + pa.u * ca.c5;
+  double t7 = pb.v * cb.c5 + pa.v * ca.c5;
+  double t8 = cb.c4 + ca.c4;
+  double du = pb.u - pa.u, dv = pb.v - pa.v, dei = pb.e - pa.e;
+  Tri f;
+  f.f2 = b1 * du + b5 * dv;
+  f.f3 = b5 * du + b2 * dv;
+  f.f4 = (t4 + t7) * du + (t5 + t6) * dv + t8 * dei;
+  return f;
+}
+
+// velocities and internal energy by division (q2/q1 ...), as the cross
+// derivative terms use them
+struct DivPrim { double u, v, e; };
+__device__ __forceinline__ DivPrim div_prim(const DevMesh& M, long long i) {
+  DivPrim d;
+  d.u = M.q[1][i] / M.q[0][i];
+  d.v = M.q[2][i] / M.q[0][i];
+  d.e = M.q[3][i] / M.q[0][i] - .5 * (sqd(d.u) + sqd(d.v));
+  return d;
+}
+
+// cross-derivative term of the eta line at cell c from its j neighbours
+// (src/visrhs.f:106-146)
+__device__ __forceinline__ Tri eta_cross(const VisCoef& c, const DivPrim& p, const DivPrim& m) {
+  double duj = 0.5 * (p.u - m.u), dvj = 0.5 * (p.v - m.v), dej = 0.5 * (p.e - m.e);
+  Tri f;
+  f.f2 = c.d1 * duj + c.d5 * dvj;
+  f.f3 = c.d7 * duj + c.d2 * dvj;
+  double t = 0.5 * c.d1 * 0.5 * (sqd(p.u) - sqd(m.u));
+  t = t + 0.5 * c.d2 * 0.5 * (sqd(p.v) - sqd(m.v));
+  t = t + c.d5 * 0.5 * (p.u + m.u) * dvj;
+  t = t + c.d7 * 0.5 * (p.v + m.v) * duj;
+  f.f4 = t + c.d4 * dej;
+  return f;
+}
+
+// cross-derivative term of the xi line (src/visrhs.f:227-268)
+__device__ __forceinline__ Tri xi_cross(const VisCoef& c, const DivPrim& p, const DivPrim& m) {
+  double duk = 0.5 * (p.u - m.u), dvk = 0.5 * (p.v - m.v), dek = 0.5 * (p.e - m.e);
+  Tri f;
+  f.f2 = c.d1 * duk + c.d7 * dvk;
+  f.f3 = c.d5 * duk + c.d2 * dvk;
+  double t = 0.5 * c.d1 * 0.5 * (sqd(p.u) - sqd(m.u));
+  t = t + 0.5 * c.d2 * 0.5 * (sqd(p.v) - sqd(m.v));
+  t = t + c.d5 * 0.5 * (p.v + m.v) * duk;
+  t = t + c.d7 * 0.5 * (p.u + m.u) * dvk;
+  f.f4 = t + c.d4 * dek;
+  return f;
+}
+
+__global__ __launch_bounds__(256) void k_visrhs(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const DevConst& C = M.c;
+  const double gkpr = C.gamma / C.pr, prtr = C.pr / 0.9, dre = .5 / C.rey;
+  const double c2b = 198.6 / C.tinf, c2bp = c2b + 1.;
+  const int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
+  long long ncell = (long long)nj * nk;
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
+    const int j = M.js + (int)(w % nj), k = M.ks + (int)(w / nj);
+    const long long i = cell(M, j, k);
+    double s2 = M.s[1][i], s3 = M.s[2][i], s4 = M.s[3][i];
+    // ---- eta part (j = const line): faces k-1|k, k|k+1, cross terms
+    {
+      VisPrim p0 = vis_prim(M, cell(M, j, k - 1), c2b, c2bp, prtr);
+      VisPrim p1 = vis_prim(M, i, c2b, c2bp, prtr);
+      VisPrim p2 = vis_prim(M, cell(M, j, k + 1), c2b, c2bp, prtr);
+      VisCoef c0 = eta_coef(M, j, k - 1, p0, dre, gkpr);
+      VisCoef c1 = eta_coef(M, j, k, p1, dre, gkpr);
+      VisCoef c2 = eta_coef(M, j, k + 1, p2, dre, gkpr);
+      Tri fm = vis_face(p0, c0, p1, c1);   // face k-1
+      Tri fp = vis_face(p1, c1, p2, c2);   // face k
+      Tri gm = eta_cross(c0, div_prim(M, cell(M, j + 1, k - 1)), div_prim(M, cell(M, j - 1, k - 1)));
+      Tri gc = eta_cross(c1, div_prim(M, cell(M, j + 1, k)), div_prim(M, cell(M, j - 1, k)));
+      if (k < M.ke) {
+        Tri gp = eta_cross(c2, div_prim(M, cell(M, j + 1, k + 1)), div_prim(M, cell(M, j - 1, k + 1)));
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
+      } else {  // one-sided closure at kend (src/visrhs.f:141-150)
+        VisPrim pq = vis_prim(M, cell(M, j, k - 2), c2b, c2bp, prtr);
+        VisCoef cq = eta_coef(M, j, k - 2, pq, dre, gkpr);
+        Tri gq = eta_cross(cq, div_prim(M, cell(M, j + 1, k - 2)), div_prim(M, cell(M, j - 1, k - 2)));
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3. * gc.f4);
+      }
+    }
+    // ---- xi part (k = const line)
+    {
+      VisPrim p0 = vis_prim(M, cell(M, j - 1, k), c2b, c2bp, prtr);
+      VisPrim p1 = vis_prim(M, i, c2b, c2bp, prtr);
+      VisPrim p2 = vis_prim(M, cell(M, j + 1, k), c2b, c2bp, prtr);
+      VisCoef c0 = xi_coef(M, j - 1, k, p0, dre, gkpr);
+      VisCoef c1 = xi_coef(M, j, k, p1, dre, gkpr);
+      VisCoef c2 = xi_coef(M, j + 1, k, p2, dre, gkpr);
+      Tri fm = vis_face(p0, c0, p1, c1);
+      Tri fp = vis_face(p1, c1, p2, c2);
+      Tri gm = xi_cross(c0, div_prim(M, cell(M, j - 1, k + 1)), div_prim(M, cell(M, j - 1, k - 1)));
+      Tri gc = xi_cross(c1, div_prim(M, cell(M, j, k + 1)), div_prim(M, cell(M, j, k - 1)));
+      if (j < M.je) {
+        Tri gp = xi_cross(c2, div_prim(M, cell(M, j + 1, k + 1)), div_prim(M, cell(M, j + 1, k - 1)));
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
+      } else {  // one-sided closure at jend (src/visrhs.f:262-271)
+        VisPrim pq = vis_prim(M, cell(M, j - 2, k), c2b, c2bp, prtr);
+        VisCoef cq = xi_coef(M, j - 2, k, pq, dre, gkpr);
+        Tri gq = xi_cross(cq, div_prim(M, cell(M, j - 2, k + 1)), div_prim(M, cell(M, j - 2, k - 1)));
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3 * gc.f4);
+      }
+    }
+    M.s[1][i] = s2;
+    M.s[2][i] = s3;
+    M.s[3][i] = s4;
+  }
+}
+
+// ========================================================================= SA
+// constants of src/sadata.h:2-4
+constexpr double SA_cb1 = 0.1355, SA_sigma = 0.6666667, SA_cb2 = 0.622, SA_akt = 0.41,
+                 SA_cw1 = 3.2390678, SA_cw2 = 0.3, SA_cw3 = 2., SA_cv1 = 7.1, SA_ct3 = 1.2, SA_ct4 = 0.5;
+
+// integer powers as the reference build evaluates them (left to right)
+__device__ __forceinline__ double ip3(double x) { return (x * x) * x; }
+__device__ __forceinline__ double ip6(double x) { return ((((x * x) * x) * x) * x) * x; }
+
+// laminar viscosity (lamvis, src/samodel2d.f:200-238) and the velocities
+// vmu_sa keeps (src/samodel2d.f:61-72), all cells
+__global__ __launch_bounds__(256) void k_sa_prim(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const double c2b = 198.6 / M.c.tinf, c2bp = c2b + 1.;
+  long long n2 = (long long)M.jd * M.kd;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
+    double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
+    double ra = 1. / q1;
+    double uvel = q2 * ra, vvel = q3 * ra;
+    double ei = q4 * ra - 0.5 * (sqd(uvel) + sqd(vvel));
+    double tt = M.c.ggm1 * ei;
+    M.sa_vmul[i] = c2bp * tt * sqrt(tt) / (c2b + tt);
+    M.sa_u[i] = q2 / q1;
+    M.sa_v[i] = q3 / q1;
+  }
+}
+
+__device__ __forceinline__ int sa_periodic_src(int idir, int js, int je, int ks, int ke, int jd, int kd, int jk) {
+  if (idir == 1) { int jj = je - js + 1; return jd - 2 * jj + (jk - js + 1); }
+  if (idir == -1) { int jj = je - js + 1; return 1 + 2 * jj - (je - jk + 1); }
+  if (idir == 2) { int kk = ke - ks + 1; return kd - 2 * kk + (jk - ks + 1); }
+  int kk = ke - ks + 1;
+  return 1 + 2 * kk - (ke - jk + 1);
+}
+
+// c_turm (src/samodel2d.f:372-399) for one cell
+__device__ __forceinline__ void c_turm_cell(const DevMesh& M, long long i) {
+  double rj = M.q[0][i] * M.q[M.nq - 1][i];
+  double vnul = M.sa_vmul[i] / rj;
+  double chi = M.q[4][i] / vnul;
+  double fv1 = ip3(chi) / (ip3(chi) + ip3(SA_cv1));
+  M.turmu[i] = M.q[0][i] * M.q[M.nq - 1][i] * M.q[4][i] * fv1;
+}
+
+// turbc (src/samodel2d.f:827-1303) for one instance; with eddy=1 the eddy
+// viscosity of the halo ring follows (c_turm after the final turbc)
+__global__ __launch_bounds__(256) void k_turbc(const DevMesh* __restrict__ Ms, int eddy) {
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
+  double* q5 = M.q[4];
+  const double* u = M.sa_u;
+  const double* v = M.sa_v;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int ib = 0; ib < M.nbc; ++ib) {
+    const DevBC& b = M.bc[ib];
+    int js = b.js, je = b.je, ks = b.ks, ke = b.ke, idir = b.idir;
+    int iadd = idir >= 0 ? 1 : -1, iadir = idir >= 0 ? idir : -idir;
+    int ty = b.type;
+    double foso = 1.0;
+    if (ty == 3 || ty == 4) {  // turbc_extpt, one halo layer
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = j + iadd, j2 = j1 + iadd;
+        for (int k = ks + t; k <= ke; k += nt)
+          q5[cell(M, j, k)] = (1. + foso) * q5[cell(M, j1, k)] - foso * q5[cell(M, j2, k)];
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = k + iadd, k2 = k1 + iadd;
+        for (int j = js + t; j <= je; j += nt)
+          q5[cell(M, j, k)] = (1. + foso) * q5[cell(M, j, k1)] - foso * q5[cell(M, j, k2)];
+      }
+    } else if (ty == 5) {  // turbc_wall
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = j + iadd;
+        for (int k = ks + t; k <= ke; k += nt) q5[cell(M, j, k)] = -q5[cell(M, j1, k)];
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = k + iadd;
+        for (int j = js + t; j <= je; j += nt) q5[cell(M, j, k)] = -q5[cell(M, j, k1)];
+      }
+    } else if (ty == 10 || ty == 11) {  // turbc_sym
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = (idir == 1) ? je + 1 : js - 1;
+        for (int k = ks + t; k <= ke; k += nt) q5[cell(M, j, k)] = q5[cell(M, j1, k)];
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+        for (int j = js + t; j <= je; j += nt) q5[cell(M, j, k)] = q5[cell(M, j, k1)];
+      }
+    } else if (ty == 22) {  // turbc_periodic
+      int nj = je - js + 1, nk = ke - ks + 1;
+      for (int w = t; w < nj * nk; w += nt) {
+        int j = js + w % nj, k = ks + w / nj;
+        long long src = (iadir == 1) ? cell(M, sa_periodic_src(idir, js, je, ks, ke, M.jd, M.kd, j), k)
+                                     : cell(M, j, sa_periodic_src(idir, js, je, ks, ke, M.jd, M.kd, k));
+        q5[cell(M, j, k)] = q5[src];
+      }
+    } else if (ty == 51 && iadir == 2) {  // turbc_wake
+      int k = (idir == 2) ? ke : ks, k1 = (idir == 2) ? ke + 1 : ks - 1;
+      for (int j = js + t; j <= je; j += nt) {
+        int jj = M.jd - j + 1;
+        double a = q5[cell(M, jj, k1)], c = q5[cell(M, j, k1)];
+        q5[cell(M, j, k)] = a;
+        q5[cell(M, jj, k)] = c;
+      }
+    } else if (ty == 46) {  // turbc_inf
+      int nj = je - js + 1, nk = ke - ks + 1;
+      for (int w = t; w < nj * nk; w += nt) q5[cell(M, js + w % nj, ks + w / nj)] = M.c.vnuinf;
+    } else if (ty == 47) {  // turbc_out
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = j + iadd;
+        for (int k = ks + t; k <= ke; k += nt) {
+          long long i = cell(M, j, k), i1 = cell(M, j1, k);
+          double ug = M.has_gv ? M.ug[i] : 0.0, vg = M.has_gv ? M.vg[i] : 0.0;
+          double ug1 = M.has_gv ? M.ug[i1] : 0.0, vg1 = M.has_gv ? M.vg[i1] : 0.0;
+          double val = M.c.vnuinf;
+          double uu = (u[i] - ug + u[i1] - ug1) * M.xx[i];
+          uu = uu + (v[i] - vg + v[i1] - vg1) * M.xy[i];
+          uu = uu * iadd;
+          if (uu < 0.) val = q5[i1];
+          q5[i] = val;
+        }
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = k + iadd;
+        for (int j = js + t; j <= je; j += nt) {
+          long long i = cell(M, j, k), i1 = cell(M, j, k1);
+          double ug = M.has_gv ? M.ug[i] : 0.0, vg = M.has_gv ? M.vg[i] : 0.0;
+          double ug1 = M.has_gv ? M.ug[i1] : 0.0, vg1 = M.has_gv ? M.vg[i1] : 0.0;
+          double val = M.c.vnuinf;
+          double uu = (u[i] - ug + u[i1] - ug1) * M.yx[i];
+          uu = uu + (v[i] - vg + v[i1] - vg1) * M.yy[i];
+          uu = uu * iadd;
+          if (uu < 0.) val = q5[i1];
+          q5[i] = val;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!eddy) return;
+  const int jd = M.jd, kd = M.kd;
+  int nring = 2 * jd + 2 * (kd - 2);
+  for (int w = t; w < nring; w += nt) {
+    int j, k;
+    if (w < jd) { j = w + 1; k = 1; }
+    else if (w < 2 * jd) { j = w - jd + 1; k = kd; }
+    else if (w < 2 * jd + kd - 2) { j = 1; k = w - 2 * jd + 2; }
+    else { j = jd; k = w - 2 * jd - (kd - 2) + 2; }
+    c_turm_cell(M, cell(M, j, k));
+  }
+}
+
+// convective upwind speeds of a cell (src/samodel2d.f:483-491)
+struct UpDown { double up, um, vp, vm; };
+__device__ __forceinline__ UpDown sa_updown(const DevMesh& M, long long i) {
+  double ug = M.has_gv ? M.ug[i] : 0.0, vg = M.has_gv ? M.vg[i] : 0.0;
+  double u = M.sa_u[i], v = M.sa_v[i];
+  double uu = M.xx[i] * (u - ug) + M.xy[i] * (v - vg);
+  double vv = M.yx[i] * (u - ug) + M.yy[i] * (v - vg);
+  UpDown o;
+  o.up = 0.5 * (uu + fabs(uu)); o.um = 0.5 * (uu - fabs(uu));
+  o.vp = 0.5 * (vv + fabs(vv)); o.vm = 0.5 * (vv - fabs(vv));
+  return o;
+}
+
+__device__ __forceinline__ double sa_vnul(const DevMesh& M, long long i) {
+  return M.sa_vmul[i] / M.q[0][i] / M.q[M.nq - 1][i];
+}
+
+// diffusion half-point coefficient chp and difference dnuhp of the face
+// between cells a and b (src/samodel2d.f:575-584, 622-631)
+__device__ __forceinline__ void sa_half(const DevMesh& M, long long a, long long b, double rey, double& chp,
+                                        double& dn) {
+  double vnulh = 0.5 * (sa_vnul(M, a) + sa_vnul(M, b));
+  double vnuh = 0.5 * (M.q[4][a] + M.q[4][b]);
+  chp = (1. + SA_cb2) / SA_sigma / rey * (vnulh + vnuh);
+  dn = M.q[4][b] - M.q[4][a];
+}
+
+// rhslhs (convec + diffus + source) and the pseudo-time scaling of vmu_sa
+// (src/samodel2d.f:101-131, 402-824) for every interior cell, in gather
+// form: each cell assembles its own row of the two tridiagonal systems.
+__global__ __launch_bounds__(256) void k_sa_rhs(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const DevConst& C = M.c;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int nj = je - js + 1, nk = ke - ks + 1;
+  const double rey = C.rey;
+  const double* q5 = M.q[4];
+  const double* xx = M.xx;
+  const double* xy = M.xy;
+  const double* yx = M.yx;
+  const double* yy = M.yy;
+  long long ncell = (long long)nj * nk;
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
+    const int j = js + (int)(w % nj), k = ks + (int)(w / nj);
+    const long long i = cell(M, j, k);
+    const long long iw = cell(M, j - 1, k), ie = cell(M, j + 1, k), is = cell(M, j, k - 1), in = cell(M, j, k + 1);
+    UpDown c = sa_updown(M, i);
+    UpDown cw = sa_updown(M, iw), ce = sa_updown(M, ie), cs = sa_updown(M, is), cn = sa_updown(M, in);
+    double nt = q5[i];
+    // ---- convec (src/samodel2d.f:493-538)
+    double sj = M.s[4][i];
+    sj = sj - c.up * (nt - q5[iw]) - c.um * (q5[ie] - nt);
+    sj = sj - c.vp * (nt - q5[is]) - c.vm * (q5[in] - nt);
+    double aj = 0.0, bj = 0.0, cj = 0.0, ak = 0.0, bk = 0.0, ck = 0.0;
+    if (j > js) aj = aj - (c.up > 1.0e-12 ? 1.0 : 0.0) * (cw.up + cw.um);
+    bj = bj + c.up - c.um;
+    if (j < je) cj = cj + (c.um < -1.0e-12 ? 1.0 : 0.0) * (ce.up + ce.um);
+    if (k > ks) ak = ak - (c.vp > 1.0e-12 ? 1.0 : 0.0) * (cs.vp + cs.vm);
+    bk = bk + c.vp - c.vm;
+    if (k < ke) ck = ck + (c.vm < -1.0e-12 ? 1.0 : 0.0) * (cn.vp + cn.vm);
+    // ---- diffus, j direction (src/samodel2d.f:570-607)
+    {
+      double chpm, dnm, chpp, dnp;
+      sa_half(M, iw, i, rey, chpm, dnm);
+      sa_half(M, i, ie, rey, chpp, dnp);
+      double dmxhm = 0.5 * (xx[iw] + xx[i]), dmyhm = 0.5 * (xy[iw] + xy[i]);
+      double dmxhp = 0.5 * (xx[i] + xx[ie]), dmyhp = 0.5 * (xy[i] + xy[ie]);
+      double dxp = dmxhp * xx[i] + dmyhp * xy[i];
+      double dxm = dmxhm * xx[i] + dmyhm * xy[i];
+      double c2 = SA_cb2 / SA_sigma / rey;
+      c2 = c2 * (sa_vnul(M, i) + nt);
+      double dcp = dxp * (chpp - c2);
+      double dcm = dxm * (chpm - c2);
+      double ax = fmax(dcm, 0.0), cx = fmax(dcp, 0.0);
+      sj = sj - ax * dnm + cx * dnp;
+      aj = aj - ax;
+      cj = cj - cx;
+      bj = bj + ax + cx;
+    }
+    // ---- diffus, k direction (src/samodel2d.f:617-654)
+    {
+      double chpm, dnm, chpp, dnp;
+      sa_half(M, is, i, rey, chpm, dnm);
+      sa_half(M, i, in, rey, chpp, dnp);
+      double dmxhm = 0.5 * (yx[is] + yx[i]), dmyhm = 0.5 * (yy[is] + yy[i]);
+      double dmxhp = 0.5 * (yx[i] + yx[in]), dmyhp = 0.5 * (yy[i] + yy[in]);
+      double dyp = dmxhp * yx[i] + dmyhp * yy[i];
+      double dym = dmxhm * yx[i] + dmyhm * yy[i];
+      double c2 = SA_cb2 / SA_sigma / rey;
+      c2 = c2 * (sa_vnul(M, i) + nt);
+      double dcp = dyp * (chpp - c2);
+      double dcm = dym * (chpm - c2);
+      double ay = fmax(dcm, 0.0), cy = fmax(dcp, 0.0);
+      sj = sj - ay * dnm + cy * dnp;
+      ak = ak - ay;
+      ck = ck - cy;
+      bk = bk + ay + cy;
+    }
+    // ---- source (src/samodel2d.f:729-811), isour = 0, fturf = 0
+    {
+      double vort = 0.0;
+      if (M.bodyflag) {
+        double ux = (M.sa_u[ie] - M.sa_u[iw]) * 0.5;
+        double vx = (M.sa_v[ie] - M.sa_v[iw]) * 0.5;
+        double uy = (M.sa_u[in] - M.sa_u[is]) * 0.5;
+        double vy = (M.sa_v[in] - M.sa_v[is]) * 0.5;
+        vort = fabs(xy[i] * ux - xx[i] * vx + yy[i] * uy - yx[i] * vy);
+      }
+      const double d2min = 1.e-12, rmax = 10.0, stilim = 1.e-12, fturf = 0.0;
+      const double cv1_3 = ip3(SA_cv1), cw3_6 = ip6(SA_cw3);
+      double vnul = sa_vnul(M, i);
+      double chi = nt / vnul;
+      double d = M.sa_sn[i];
+      chi = fmax(chi, 1.e-12);
+      double chi3 = ip3(chi);
+      double fv1 = chi3 / (chi3 + cv1_3);
+      double fv2 = 1. - chi / (1. + chi * fv1);
+      double fv3 = 1.0;
+      double ft2 = fturf * SA_ct3 * exp(-1. * SA_ct4 * chi * chi);
+      double dchi = 1. / vnul;
+      double dfv1 = (3. * cv1_3) * (chi * chi) * dchi * sqd(1. / (chi3 + cv1_3));
+      double dfv2 = (fv2 - 1.) / chi / vnul + (1. - fv2) * (1 - fv2) * (dfv1 + fv1 / chi / vnul);
+      double dfv3 = 0.0;
+      double dft2 = (-2. * SA_ct4) * chi * dchi * ft2;
+      double d2 = fmax(d * d, d2min);
+      double stilda = vort * fv3 + vnul / (d2 * SA_akt * SA_akt * rey) * chi * fv2;
+      stilda = fmax(stilda, stilim);
+      double r = nt / (d2 * SA_akt * SA_akt * rey) / stilda;
+      r = fmin(r, rmax);
+      double g = r + SA_cw2 * (ip6(r) - r);
+      double g6 = ip6(g);
+      double fw = (1. + cw3_6) / (g6 + cw3_6);
+      fw = g * pow(fw, 1. / 6.);
+      double dstild = vort * dfv3 + vnul * (dchi * fv2 + chi * dfv2) / (d2 * SA_akt * SA_akt * rey);
+      double dr = vnul * (dchi - chi * dstild / stilda) / stilda / (d2 * SA_akt * SA_akt * rey);
+      double dg = dr * (1. + SA_cw2 * (6. * pow(r, 5.) - 1.));
+      double dfw = pow((1. + cw3_6) / (g6 + cw3_6), 1. / 6.);
+      dfw = dfw * dg * (1. - g6 / (g6 + cw3_6));
+      double pro = SA_cb1 * stilda * (1. - ft2);
+      double des = (SA_cw1 * fw - SA_cb1 / (SA_akt * SA_akt) * ft2) / d2 / rey;
+      double prod = SA_cb1 * stilda * (1. - ft2) * nt;
+      double dest = (SA_cw1 * fw - SA_cb1 / (SA_akt * SA_akt) * ft2) * nt * nt / d2 / rey;
+      double dpro = pro * dstild / stilda - SA_cb1 * stilda * dft2;
+      double ddes = (SA_cw1 * dfw - SA_cb1 / (SA_akt * SA_akt) * dft2) / d2 / rey * vnul * chi;
+      ddes = ddes + des;
+      ddes = ddes * nt;
+      dpro = dpro * nt;
+      sj = sj + prod - dest;
+      double tk1 = fmax(des * nt - pro, 0.0);
+      double tk2 = fmax(ddes - dpro, 0.0);
+      bk = bk + tk1 + tk2;
+    }
+    // ---- scale by the turbulence pseudo time step
+    double tscal = M.tscale[i];
+    if (C.timeac == 1.) {
+      double qj = M.q[M.nq - 1][i];
+      tscal = (double)imx(M.iblank[i], 0) * (1.0 + 0.002 * sqrt(qj)) / (1. + sqrt(qj));
+      tscal = tscal * 1.0;
+      tscal = tscal / (1. + tscal / C.h);
+    }
+    M.sa_sj[i] = sj * tscal;
+    M.sa_aj[i] = aj * tscal;
+    M.sa_cj[i] = cj * tscal;
+    M.sa_ak[i] = ak * tscal;
+    M.sa_ck[i] = ck * tscal;
+    M.sa_bb[i] = 1. + (bj + bk) * tscal;
+  }
+}
+
+// lsolvej (src/samodel2d.f:1306-1344) and s1, sjmat*bjmat of vmu_sa
+// (src/samodel2d.f:148-152): one thread per k row
+__global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int k = ks + blockIdx.x * 64 + threadIdx.x;
+  if (k > ke) return;
+  double* s = M.sa_sj;
+  double* c = M.sa_cj;
+  const double* a = M.sa_aj;
+  const double* b = M.sa_bb;
+  long long i = cell(M, js, k);
+  double bb = 1. / b[i];
+  double sp = s[i] * bb, cp = c[i] * bb;
+  s[i] = sp; c[i] = cp;
+  for (int j = js + 1; j <= je; ++j) {
+    i = cell(M, j, k);
+    double ai = a[i];
+    bb = 1. / (b[i] - ai * cp);
+    sp = (s[i] - ai * sp) * bb;
+    cp = c[i] * bb;
+    s[i] = sp; c[i] = cp;
+  }
+  // back substitution with s1 = sjmat, sjmat = (s1 - s2)*bjmat, s2 = 0
+  i = cell(M, je, k);
+  double sn = sp;
+  s[i] = (0. + sn - 0.) * b[i];
+  for (int j = je - 1; j >= js; --j) {
+    i = cell(M, j, k);
+    sn = s[i] - c[i] * sn;
+    s[i] = (0. + sn - 0.) * b[i];
+  }
+}
+
+// lsolvek (src/samodel2d.f:1347-1385), s2 and the nu-tilde update
+// (src/samodel2d.f:157-188) with the interior eddy viscosity (c_turm)
+__global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int j = js + blockIdx.x * 64 + threadIdx.x;
+  if (j > je) return;
+  double* s = M.sa_sj;
+  double* c = M.sa_ck;
+  const double* a = M.sa_ak;
+  const double* b = M.sa_bb;
+  long long i = cell(M, j, ks);
+  double bb = 1. / b[i];
+  double sp = s[i] * bb, cp = c[i] * bb;
+  s[i] = sp; c[i] = cp;
+  for (int k = ks + 1; k <= ke; ++k) {
+    i = cell(M, j, k);
+    double ai = a[i];
+    bb = 1. / (b[i] - ai * cp);
+    sp = (s[i] - ai * sp) * bb;
+    cp = c[i] * bb;
+    s[i] = sp; c[i] = cp;
+  }
+  const double vnulim = 1.0e-20, relfac = 1.0;
+  double sn = sp;
+  for (int k = ke; k >= ks; --k) {
+    i = cell(M, j, k);
+    if (k < ke) sn = s[i] - c[i] * sn;
+    double dq = relfac * (0. + sn);
+    double q5 = M.q[4][i];
+    q5 = fmax((q5 + dq * (double)imx(M.iblank[i], 0)), vnulim);
+    M.q[4][i] = q5;
+    c_turm_cell(M, i);
+  }
+}
+
+// distance to the wall (dist, src/samodel2d.f:243-293); static grids: once
+__global__ __launch_bounds__(256) void k_sa_dist(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  const int jd = M.jd, kd = M.kd;
+  long long n2 = (long long)jd * kd;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
+    int j = 1 + (int)(i % jd), k = 1 + (int)(i / jd);
+    double sn = 1.e10;
+    if (M.bodyflag) {
+      if (j >= M.jtail1 && j <= M.jtail2) {
+        if (k == 1) sn = 0.0;
+        else {
+          int jv = j - 1;
+          sn = sqrt(sqd(M.x[i] - 0.5 * (M.xv[vert(M, jv, 1)] + M.xv[vert(M, jv + 1, 1)])) +
+                    sqd(M.y[i] - 0.5 * (M.yv[vert(M, jv, 1)] + M.yv[vert(M, jv + 1, 1)])));
+        }
+      } else {
+        int jv = M.jtail1 - 1;
+        sn = sqrt(sqd(M.x[i] - M.xv[vert(M, jv, 1)]) + sqd(M.y[i] - M.yv[vert(M, jv, 1)]));
+      }
+    }
+    M.sa_sn[i] = sn;
+  }
+}
+
+static inline int nblk2(long long n, int per) {
+  long long b = (n + per - 1) / per;
+  return (int)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+static long long max_cells(const DevMesh* hM, int nmi) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  return mx;
+}
+
+hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  hipLaunchKernelGGL(k_visrhs, dim3(nblk2(max_cells(hM, nmi), 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  hipLaunchKernelGGL(k_sa_dist, dim3(nblk2(max_cells(hM, nmi), 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+// vmu_sa for every instance (src/samodel2d.f:2-197, itrans = 0)
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = max_cells(hM, nmi);
+  int mj = 1, mk = 1;
+  for (int i = 0; i < nmi; ++i) {
+    mj = hM[i].je - hM[i].js + 1 > mj ? hM[i].je - hM[i].js + 1 : mj;
+    mk = hM[i].ke - hM[i].ks + 1 > mk ? hM[i].ke - hM[i].ks + 1 : mk;
+  }
+  hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(256), 0, st, dM, 0);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sa_rhs, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sa_solve_j, dim3((mk + 63) / 64, nmi), dim3(64), 0, st, dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sa_solve_k, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(256), 0, st, dM, 1);
+  return hipGetLastError();
+}
+
+}  // namespace ot2d
