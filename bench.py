@@ -1,0 +1,299 @@
+#!/usr/bin/env python3
+"""Throughput of the OVERTURNS-2D iteration on one MI355X.
+
+A step is one pseudo-time iteration of the reference's driver loop
+(src/Turns_overset.f:475-554): time_step, step (bc, rhsup, visrhs, norms,
+rbc, s*tscale, ilu2d, vmu_sa, update) and monitor, run by libot2d on a case
+resident in HBM.  The timed region replays the captured step graph K times
+through ot2d_iterate; ``value`` is interior cells x steps / wall seconds.
+
+Default workload: RANS with the Spalart-Allmaras model on a 513x129 C-grid
+around a NACA 0012 (first wall spacing 1e-6 chord, outer boundary at 20
+chords), M=0.7, alpha=2 deg, Re=9e6, nu_tilde_inf=3, MUSCL/Roe, LU-SGS at
+CFL 10, FP64 residual with FP32 LU-SGS operator, free-stream start.
+
+Beside the headline number the run reports
+  * roofline: per-kernel device times from HIP events on the case's stream
+    (ot2d_profile), the dominant kernel's algorithmic bytes / duration
+    against 8 TB/s;
+  * cpu_baseline: the reference program built from source
+    (oracle/_ref/overturns2d) timed on this host, one thread, on a bounded
+    number of iterations of the same deck (two run lengths differenced so
+    start-up and I/O cancel);
+  * converge: steps, seconds and solves/hour of a fresh case run until the
+    reference's L2 residual norm reaches 1e-10.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from turns2d_urans_git_amd import abi, grid  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+_SA = dict(fsmach=0.7, alfa=2.0, rey=9.0e6, invisc=False, lamin=False, iturb=1, vnuinf=3.0,
+           irhsy=-3, ilim=1, ilhs=1, iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1,
+           itnmax=1, timeac=1.0)
+_EULER = dict(fsmach=0.5, alfa=1.25, invisc=True, lamin=True, irhsy=-3, ilim=1, ilhs=1,
+              iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1, timeac=1.0)
+
+# The SA grid clusters surface points at the leading edge only and lets the
+# first spacing off the wake cut grow to 1/1000 of the streamwise spacing:
+# with cosine clustering at the trailing edge, or 1e-6 spacing all along the
+# wake, the reference's scheme diverges on this deck (checked on the oracle).
+WORKLOADS = {
+    "sa_513x129": dict(jmax=513, kmax=129, ds_wall=1e-6, deck=_SA, lu_fp32=1, model="sa",
+                       wake_aspect=1000.0, te_cluster=False),
+    "sa_513x129_fp64": dict(jmax=513, kmax=129, ds_wall=1e-6, deck=_SA, lu_fp32=0, model="sa",
+                            wake_aspect=1000.0, te_cluster=False),
+    "euler_257x65": dict(jmax=257, kmax=65, ds_wall=1e-4, deck=_EULER, lu_fp32=0, model="euler"),
+}
+
+# Algorithmic HBM bytes per interior cell and step of the minimal three-pass
+# schedule (A: bc + residual + norms; B: LU forward; C: LU backward + update
+# + monitor; S1-S3: SA right-hand side, j solve, k solve), each array counted
+# once per pass, FP64.  Each kernel stage is credited with the pass it
+# belongs to; a pass split over several kernels is reported as a pass too.
+PASS_BYTES = {
+    "euler": {"A": 124, "B": 136, "C": 140},
+    "sa": {"A": 132, "B": 144, "C": 148, "S1": 140, "S2": 48, "S3": 80},
+}
+STAGE_PASS = {
+    "bc": "A", "residual": "A", "visrhs": "A", "tsrhs": "A", "norms": "A", "rbc": "A",
+    "time_step": "A", "lu_prep": "B", "lu_forward": "B", "lu_backward": "C", "lu_finish": "C",
+    "update": "C", "monitor": "C", "sa_prim": "S1", "sa_bc": "S1", "sa_rhs": "S1",
+    "sa_solve_j": "S2", "sa_solve_k": "S3", "sa_eddy": "S3",
+}
+
+
+def _hip():
+    return C.CDLL("libamdhip64.so")
+
+
+def device_sync(hip):
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
+def make_grid(w):
+    return grid.naca4_cgrid(w["jmax"], w["kmax"], t=0.12, ds_wall=w["ds_wall"],
+                            wake_aspect=w.get("wake_aspect"), te_cluster=w.get("te_cluster", True))
+
+
+def make_gpu_case(w, g):
+    import turns2d_urans_git_amd as t2d
+    p = t2d.default_params(**w["deck"], lu_fp32=w["lu_fp32"])
+    md = abi.mesh_desc(g.jmax, g.kmax, grid.cgrid_bc(g.jtail1))
+    case = t2d.Case(p, [md])
+    case.set_grid(0, g.xv, g.yv)
+    return case
+
+
+def roofline(stage_ms, model, cells):
+    """Dominant kernel and per-pass achieved GB/s from per-stage device ms."""
+    pb = PASS_BYTES[model]
+    dom = max(stage_ms, key=stage_ms.get)
+    per_cell = pb[STAGE_PASS[dom]]
+    ach = per_cell * cells / (stage_ms[dom] * 1e-3) / 1e9
+    passes = {}
+    for st, ms in stage_ms.items():
+        ps = STAGE_PASS.get(st)
+        if ps is None:
+            continue
+        passes.setdefault(ps, 0.0)
+        passes[ps] += ms
+    pass_gbs = {ps: round(pb[ps] * cells / (ms * 1e-3) / 1e9, 1) for ps, ms in passes.items()}
+    return dict(
+        bound="hbm", kernel=dom, achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+        frac=round(ach / HBM_PEAK_GBS, 5), traffic=None,
+        bytes_per_cell=per_cell, cells_per_launch=cells, launch_ms=round(stage_ms[dom], 5),
+    ), pass_gbs
+
+
+def reference_seconds_per_step(w, g, n1, n2, timeout):
+    """Wall seconds per iteration of oracle/_ref/overturns2d on the deck
+    (runs of n1 and n2 iterations differenced), one thread."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "overturns2d")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    times = []
+    for n in (n1, n2):
+        with tempfile.TemporaryDirectory() as d:
+            grid.write_case(d, g, nsteps=n, nrest=10**7, npnorm=10**7, **w["deck"])
+            with open(os.path.join(d, "inputs")) as fin:
+                t0 = time.perf_counter()
+                subprocess.run([exe], cwd=d, stdin=fin, env=env, check=True, timeout=timeout,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                times.append(time.perf_counter() - t0)
+    return (times[1] - times[0]) / (n2 - n1), times
+
+
+def port_seconds_per_step(w, g, n):
+    """Wall seconds per iteration of the C restatement (oracle/liboracle.so)."""
+    from tests import helpers as H
+    o = H.oracle_case(g, dict(w["deck"]))
+    o.iterate(1)
+    t0 = time.perf_counter()
+    o.iterate(n)
+    return (time.perf_counter() - t0) / n
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="sa_513x129", choices=sorted(WORKLOADS))
+    ap.add_argument("--profile-steps", type=int, default=100,
+                    help="eager steps timed per kernel with HIP events (0: skip)")
+    ap.add_argument("--cpu-steps", type=str, default="20,420",
+                    help="reference run lengths n1,n2 for the CPU baseline (empty: skip)")
+    ap.add_argument("--converge", type=int, default=40000,
+                    help="run a fresh case up to N steps until rsum <= 1e-10; report solves/hour (0: skip)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this size with torch (counter scaling)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every case of "
+              "this workload fits one MI355X; see the sharding notes in the design document)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOADS[args.workload]
+    g = make_grid(w)
+    cells = (g.jmax - 1) * (g.kmax - 1)
+    hip = _hip()
+    case = make_gpu_case(w, g)
+
+    print(f"[bench] {args.workload}: {g.jmax}x{g.kmax}, warmup {args.warmup}", file=sys.stderr, flush=True)
+    rc, _, done = case.iterate(args.warmup)
+    if rc != 0 or done != args.warmup:
+        raise SystemExit(f"warmup stopped: rc={rc} after {done} steps")
+
+    device_sync(hip)
+    t0 = time.perf_counter()
+    rc, hist, done = case.iterate(args.steps)
+    device_sync(hip)
+    t1 = time.perf_counter()
+    if rc != 0 or done != args.steps:
+        raise SystemExit(f"timed run stopped: rc={rc} after {done} steps")
+    secs = t1 - t0
+    value = cells * args.steps / secs
+    print(f"[bench] timed {args.steps} steps: {1e3 * secs / args.steps:.4f} ms/step", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "cell-iter/s",
+        "value": value,
+        "unit": "cell-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32-lu" if w["lu_fp32"] else "f64",
+        "data": "synthetic (analytic NACA 4-digit C-grid, free-stream start)",
+        "config": {"workload": args.workload, "jmax": g.jmax, "kmax": g.kmax,
+                   "interior_cells": cells, "model": w["model"],
+                   "lu_sgs": "fp32" if w["lu_fp32"] else "fp64", "residual": "fp64",
+                   "wall_spacing": w["ds_wall"], "outer_radius": 20.0,
+                   "residual_l2": float(hist[-1, 0])},
+    }
+
+    if args.profile_steps > 0:
+        rc, stage_ms, pdone = case.profile(args.profile_steps)
+        if rc != 0:
+            raise SystemExit(f"profiled run stopped: rc={rc}")
+        rl, pass_gbs = roofline(stage_ms, w["model"], cells)
+        out["roofline"] = rl
+        out["kernels_ms"] = {k: round(v, 5) for k, v in sorted(stage_ms.items(), key=lambda kv: -kv[1])}
+        out["passes_gbs"] = pass_gbs
+        out["profiled_ms_per_step"] = round(sum(stage_ms.values()), 5)
+        print(f"[bench] profiled {pdone} steps: dominant {rl['kernel']} {rl['launch_ms']:.4f} ms",
+              file=sys.stderr, flush=True)
+
+    if args.converge > 0:
+        # a fresh case from free stream until the reference's L2 norm (rsum,
+        # src/step.f:60-93) reaches 1e-10; the free-stream start has a
+        # near-zero first residual, so the drop is also given from the peak
+        c2 = make_gpu_case(w, g)
+        steps, conv, peak, chunk = 0, None, 0.0, 1000
+        device_sync(hip)
+        tc0 = time.perf_counter()
+        while steps < args.converge:
+            rc, h, d = c2.iterate(min(chunk, args.converge - steps))
+            if d:
+                peak = max(peak, float(h[:, 0].max()))
+                hit = np.nonzero(h[:, 0] <= 1e-10)[0]
+                if hit.size:
+                    conv = steps + int(hit[0]) + 1
+            steps += d
+            if conv or rc != 0 or d == 0:
+                break
+        device_sync(hip)
+        tc = time.perf_counter() - tc0
+        f = c2.forces()
+        out["converge"] = {
+            "criterion": "rsum <= 1e-10", "steps": conv, "steps_run": steps, "seconds": round(tc, 3),
+            "peak_rsum": peak, "drop_from_peak": (1e-10 / peak if peak else None),
+            "solves_per_hr": (3600.0 / tc if conv else None),
+            "cl": f["cl"], "cd": f["cd"], "cm": f["cm"],
+        }
+        c2.close()
+        print(f"[bench] converge: {conv} steps ({steps} run, {tc:.1f} s)", file=sys.stderr, flush=True)
+
+    if args.cpu_steps:
+        n1, n2 = (int(x) for x in args.cpu_steps.split(","))
+        print(f"[bench] cpu baseline: reference program, {n1} and {n2} steps", file=sys.stderr, flush=True)
+        res = reference_seconds_per_step(w, g, n1, n2, timeout=600)
+        if res is not None:
+            sps, times = res
+            out["cpu_baseline"] = {
+                "value": cells / sps, "unit": "cell-iter/s", "cores": 1, "kind": "reference",
+                "sample": (f"oracle/_ref/overturns2d (reference sources, flang -O2 -fdefault-real-8), "
+                           f"same deck, runs of {n1} and {n2} steps differenced "
+                           f"({times[0]:.1f} s, {times[1]:.1f} s wall), OMP_NUM_THREADS=1"),
+            }
+        else:
+            n = 5
+            sps = port_seconds_per_step(w, g, n)
+            out["cpu_baseline"] = {
+                "value": cells / sps, "unit": "cell-iter/s", "cores": 1, "kind": "port",
+                "sample": f"oracle/liboracle.so (C restatement), same deck, {n} steps, 1 thread",
+            }
+        out["cpu_baseline"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+
+    if args.copy_probe_mib > 0:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = 2 * n * 4
+
+    case.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -155,6 +155,40 @@ int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32_t* done);
 int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]);
 
 /* Error text of the last failing call on this thread. */
+/* Kernel stages of one step, in launch order (ot2d_profile). */
+enum {
+  OT2D_ST_BEGIN = 0,     /* step counter / status reset                        */
+  OT2D_ST_TIME_STEP,     /* time_step                 src/time.f:2-107         */
+  OT2D_ST_BC,            /* bc                        src/bc.f:2-85            */
+  OT2D_ST_RESIDUAL,      /* rhsup (+ norms if inviscid) src/eulerrhs.f:2-280   */
+  OT2D_ST_VISRHS,        /* visrhs                    src/visrhs.f:2-287       */
+  OT2D_ST_TSRHS,         /* timespectralrhs           src/sources.f:2-27       */
+  OT2D_ST_NORMS,         /* residual norms            src/step.f:53-115        */
+  OT2D_ST_RBC,           /* rbc                       src/bc.f:88-177          */
+  OT2D_ST_LU_PREP,       /* s*=tscale, ilu2d set-up   src/lhs.f:470-612        */
+  OT2D_ST_LU_FORWARD,    /* ilu2d forward sweep       src/lhs.f:613-715        */
+  OT2D_ST_LU_BACKWARD,   /* ilu2d backward sweep      src/lhs.f:716-802        */
+  OT2D_ST_LU_FINISH,     /* ilu2d restore src/lhs.f:803-814 (+update, monitor if laminar) */
+  OT2D_ST_SA_PRIM,       /* lamvis, u, v              src/samodel2d.f:61-72, 200-238 */
+  OT2D_ST_SA_BC,         /* turbc                     src/samodel2d.f:827-1303 */
+  OT2D_ST_SA_RHS,        /* vortic, convec, diffus, source src/samodel2d.f:101-131, 402-824 */
+  OT2D_ST_SA_SOLVE_J,    /* lsolvej                   src/samodel2d.f:1306-1344 */
+  OT2D_ST_SA_SOLVE_K,    /* lsolvek + clip            src/samodel2d.f:157-188, 1347-1385 */
+  OT2D_ST_SA_EDDY,       /* turbc + c_turm            src/samodel2d.f:372-399  */
+  OT2D_ST_UPDATE,        /* q += s                    src/step.f:205-211       */
+  OT2D_ST_MONITOR,       /* monitor                   src/step.f:332-392       */
+  OT2D_NSTAGE
+};
+
+/* Device time of each kernel stage, measured with HIP events recorded on the
+ * case's stream between the kernels of nsteps eagerly launched steps (the
+ * same kernels ot2d_iterate replays; the case advances by those steps).
+ * stage_ms[OT2D_NSTAGE] receives the mean milliseconds per step of every
+ * stage (0 for stages this case does not run); *done = steps completed.
+ * Instrumentation for benchmarks; no counterpart in the reference. */
+int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, int32_t* done);
+const char* ot2d_stage_name(int32_t stage);
+
 const char* ot2d_last_error(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,74 @@
+"""CPU checks of the C-ABI: libot2d.so loads and exports every entry point
+include/ot2d.h declares, and the ctypes mirror has the header's layout.
+No compute call is made (there may be no GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from turns2d_urans_git_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ot2d.h")
+LIB = os.path.join(ROOT, "turns2d_urans_git_amd", "libot2d.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ot2d_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("ot2d_case_create", "ot2d_step", "ot2d_iterate", "ot2d_forces", "ot2d_time_step",
+              "ot2d_compute_residual", "ot2d_lhs", "ot2d_monitor", "ot2d_profile"):
+        assert n in names
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libot2d.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python mirror binds every entry point it knows under the same names
+    fns = abi.bind(lib, "ot2d_")
+    assert "profile" in fns and "iterate" in fns
+
+
+def _c_layout():
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "ot2d.h"
+int main(void) {
+  printf("%zu %zu %zu\n", sizeof(ot2d_params), sizeof(ot2d_mesh_desc), sizeof(ot2d_bcseg));
+  printf("%zu %zu %zu %zu\n", offsetof(ot2d_params, fsmach), offsetof(ot2d_params, tinf),
+         offsetof(ot2d_params, lu_fp32), offsetof(ot2d_mesh_desc, bc));
+  printf("%d %d\n", OT2D_NSTAGE, OT2D_F_COUNT);
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    return [int(x) for x in out]
+
+
+def test_ctypes_mirror_matches_header_layout():
+    (sp, sm, sb, o_fs, o_tinf, o_lu, o_bc, nstage, nfield) = _c_layout()
+    assert C.sizeof(abi.Params) == sp
+    assert C.sizeof(abi.MeshDesc) == sm
+    assert C.sizeof(abi.BCSeg) == sb
+    assert abi.Params.fsmach.offset == o_fs
+    assert abi.Params.tinf.offset == o_tinf
+    assert abi.Params.lu_fp32.offset == o_lu
+    assert abi.MeshDesc.bc.offset == o_bc
+    assert abi.NSTAGE == nstage
+    assert abi.F_VGV + 1 == nfield

@@ -122,3 +122,19 @@ def test_fp32_lusgs_defect_correction():
     assert rc_o == rc_d == 0
     np.testing.assert_allclose(h_d, h_o, rtol=1e-3)
     assert H.rel_err(d.get_state(0), o.get_state(0)) < 1e-5
+
+
+def test_history_after_buffer_growth():
+    """A longer ot2d_iterate after a short one records every step's norm
+    (the device history buffer is regrown between the calls)."""
+    g = H.cgrid(65, 25, ds_wall=1e-3)
+    o = H.oracle_case(g, H.EULER)
+    d = H.gpu_case(g, H.EULER)
+    _, h1o, _ = o.iterate(3)
+    _, h2o, _ = o.iterate(40)
+    rc1, h1, n1 = d.iterate(3)
+    rc2, h2, n2 = d.iterate(40)
+    assert rc1 == rc2 == 0 and n1 == 3 and n2 == 40
+    assert np.all(h2[:, 0] > 0.0)
+    np.testing.assert_allclose(h1, h1o, rtol=1e-10)
+    np.testing.assert_allclose(h2, h2o, rtol=1e-10)

@@ -103,13 +103,23 @@ def bind(lib: C.CDLL, prefix: str):
         "iterate": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_int32)]),
         "forces": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
     }
+    # instrumentation present in libot2d only
+    optional = {
+        "profile": (C.c_int32, [C.c_void_p, C.c_int32, _DP, P(C.c_int32)]),
+        "stage_name": (C.c_char_p, [C.c_int32]),
+    }
     fns = {}
-    for name, (res, args) in sig.items():
+    for name, (res, args) in list(sig.items()) + list(optional.items()):
+        if name in optional and not hasattr(lib, prefix + name):
+            continue
         f = getattr(lib, prefix + name)
         f.restype = res
         f.argtypes = args
         fns[name] = f
     return fns
+
+
+NSTAGE = 20  # OT2D_NSTAGE
 
 
 class CaseError(RuntimeError):
@@ -255,6 +265,15 @@ class CaseAPI:
         rc = self._check(self._fns["iterate"](self.h, int(niter), hist.ctypes.data,
                                               C.byref(done)), "iterate")
         return rc, hist.reshape(niter, n)[: done.value], done.value
+
+    def profile(self, nsteps: int):
+        """Advance ``nsteps`` eager steps timing every kernel stage with HIP
+        events on the case's stream; returns (rc, {stage: mean ms/step}, done)."""
+        ms = np.zeros(NSTAGE)
+        done = C.c_int32()
+        rc = self._check(self._fns["profile"](self.h, int(nsteps), ms, C.byref(done)), "profile")
+        names = [self._fns["stage_name"](i).decode() for i in range(NSTAGE)]
+        return rc, {n: float(v) for n, v in zip(names, ms) if v > 0.0}, done.value
 
     def forces(self, im: int = 0, nsp: int = 0):
         out = np.zeros(6)

@@ -326,6 +326,11 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
       return fail(OT2D_ERR_UNSUPPORTED, "kmax=%d: the LU-SGS sweep holds one row per lane, at most 256 rows",
                   md[im].kmax);
     }
+    if (!c->lamin && md[im].jmax - 1 > SA_LINE_MAX) {
+      free_case(c);
+      return fail(OT2D_ERR_UNSUPPORTED, "jmax=%d: the turbulence line solves hold at most %d cells per line",
+                  md[im].jmax, SA_LINE_MAX);
+    }
     if (lu_lds_need(md[im].kmax - 1, md[im].jmax - 1, p->lu_fp32) > 160 * 1024) {
       free_case(c);
       return fail(OT2D_ERR_UNSUPPORTED, "mesh %d (%d x %d) exceeds the LU-SGS sweep's LDS staging", im,
@@ -579,14 +584,25 @@ extern "C" int32_t ot2d_bc(ot2d_case* c) {
   return sync(c);
 }
 
-static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms) {
+static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks* mk = nullptr) {
   HIPCHK(launch_bc(c->dM, nmi(c), zero_halo_s, c->stream));
+  mark(mk, OT2D_ST_BC, c->stream);
   // the norms are taken on the complete residual (src/step.f:46-93)
   const bool later = !c->p.invisc || c->p.timespectral;
   HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms && !later, c->stream));
-  if (!c->p.invisc) HIPCHK(launch_visrhs(c->dM, c->hM.data(), nmi(c), c->stream));
-  if (c->p.timespectral) HIPCHK(launch_tsrhs(c->dM, c->hM.data(), nmi(c), c->stream));
-  if (norms && later) HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
+  mark(mk, OT2D_ST_RESIDUAL, c->stream);
+  if (!c->p.invisc) {
+    HIPCHK(launch_visrhs(c->dM, c->hM.data(), nmi(c), c->stream));
+    mark(mk, OT2D_ST_VISRHS, c->stream);
+  }
+  if (c->p.timespectral) {
+    HIPCHK(launch_tsrhs(c->dM, c->hM.data(), nmi(c), c->stream));
+    mark(mk, OT2D_ST_TSRHS, c->stream);
+  }
+  if (norms && later) {
+    HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
+    mark(mk, OT2D_ST_NORMS, c->stream);
+  }
   return OT2D_OK;
 }
 
@@ -607,7 +623,7 @@ extern "C" int32_t ot2d_rbc(ot2d_case* c) {
 extern "C" int32_t ot2d_lhs(ot2d_case* c) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
   if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, c->stream));
+  HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, nullptr, c->stream));
   return sync(c);
 }
 
@@ -623,19 +639,24 @@ extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
 }
 
 // one step (src/step.f:2-218) + monitor, enqueued without synchronisation
-static int32_t enqueue_step(ot2d_case* c) {
+static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
   HIPCHK(launch_step_begin(c->d_istep0, c->d_status, c->d_stepc, 1, c->stream));
+  mark(mk, OT2D_ST_BEGIN, c->stream);
   HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream));
-  if (int32_t rc = enqueue_residual(c, 1, 1)) return rc;
+  mark(mk, OT2D_ST_TIME_STEP, c->stream);
+  if (int32_t rc = enqueue_residual(c, 1, 1, mk)) return rc;
   HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));
+  mark(mk, OT2D_ST_RBC, c->stream);
   if (c->lamin) {
-    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, c->stream));
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, mk, c->stream));
   } else {
     // vmu_sa reads the state ilu2d leaves (before the q update), src/step.f:189-211
-    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, c->stream));
-    HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), c->stream));
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, mk, c->stream));
+    HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), mk, c->stream));
     HIPCHK(launch_update(c->dM, c->hM.data(), nmi(c), c->stream));
+    mark(mk, OT2D_ST_UPDATE, c->stream);
     HIPCHK(launch_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
+    mark(mk, OT2D_ST_MONITOR, c->stream);
   }
   return OT2D_OK;
 }
@@ -644,7 +665,7 @@ extern "C" int32_t ot2d_turbulence(ot2d_case* c) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
   if (c->lamin) return OT2D_OK;
   if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), c->stream));
+  HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), nullptr, c->stream));
   return sync(c);
 }
 
@@ -698,15 +719,17 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
   int per = nmi(c);
   // device-side residual history: one rsum per step and instance
+  bool grown = false;
   if (hist && niter > c->hist_cap) {
     dfree(c->d_hist);
     c->d_hist = nullptr;
     hipError_t e = dalloc(&c->d_hist, (size_t)niter * per);
     if (e != hipSuccess) return fail(OT2D_ERR_NOMEM, "history buffer: %s", hipGetErrorString(e));
     c->hist_cap = niter;
+    grown = true;  // the allocator may hand back the old address: refresh the capacity too
   }
   double* want = hist ? c->d_hist : nullptr;
-  if (c->hM[0].hist != want) {
+  if (grown || c->hM[0].hist != want) {
     for (int i = 0; i < per; ++i) {
       c->hM[i].hist = want;
       c->hM[i].hist_cap = want ? c->hist_cap : 0;
@@ -738,5 +761,76 @@ extern "C" int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out
   HIPCHK(hipMemcpyAsync(buf, c->d_forces, sizeof buf, hipMemcpyDeviceToHost, c->stream));
   if (int32_t rc = sync(c)) return rc;
   for (int i = 0; i < 6; ++i) out[i] = buf[i];
+  return OT2D_OK;
+}
+
+static const char* const kStageNames[OT2D_NSTAGE] = {
+    "begin",   "time_step", "bc",         "residual",    "visrhs",    "tsrhs",    "norms",
+    "rbc",     "lu_prep",   "lu_forward", "lu_backward", "lu_finish", "sa_prim",  "sa_bc",
+    "sa_rhs",  "sa_solve_j", "sa_solve_k", "sa_eddy",    "update",    "monitor"};
+
+extern "C" const char* ot2d_stage_name(int32_t stage) {
+  return (stage >= 0 && stage < OT2D_NSTAGE) ? kStageNames[stage] : "";
+}
+
+extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, int32_t* done) {
+  if (!c || nsteps < 0 || !stage_ms) return fail(OT2D_ERR_ARG, "bad arguments");
+  for (auto& mb : c->m)
+    if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  for (int i = 0; i < OT2D_NSTAGE; ++i) stage_ms[i] = 0.0;
+  // the history pointer baked into the device meshes stays as it is; the
+  // step counter restarts so that history slots are not overrun
+  Marks mk;
+  hipEvent_t ev0;
+  HIPCHK(hipEventCreate(&ev0));
+  int created = 0;
+  hipError_t e = hipSuccess;
+  for (; created < Marks::CAP && e == hipSuccess; ++created) e = hipEventCreate(&mk.ev[created]);
+  int32_t rc = OT2D_OK;
+  int steps = 0, st = 0;
+  if (e != hipSuccess) {
+    rc = fail(OT2D_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+    --created;
+  } else if ((rc = clear_status(c)) == OT2D_OK) {
+    e = hipMemsetAsync(c->d_stepc, 0, sizeof(int), c->stream);
+    if (e != hipSuccess) rc = fail(OT2D_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    for (int it = 0; it < nsteps && rc == OT2D_OK; ++it) {
+      mk.n = 0;
+      if ((e = hipEventRecord(ev0, c->stream)) != hipSuccess) {
+        rc = fail(OT2D_ERR_HIP, "hipEventRecord: %s", hipGetErrorString(e));
+        break;
+      }
+      if ((rc = enqueue_step(c, &mk)) != OT2D_OK) break;
+      if ((e = hipEventSynchronize(mk.ev[mk.n - 1])) != hipSuccess) {
+        rc = fail(OT2D_ERR_HIP, "hipEventSynchronize: %s", hipGetErrorString(e));
+        break;
+      }
+      hipEvent_t prev = ev0;
+      for (int i = 0; i < mk.n; ++i) {
+        float ms = 0.f;
+        if ((e = hipEventElapsedTime(&ms, prev, mk.ev[i])) != hipSuccess) break;
+        stage_ms[mk.stage[i]] += ms;
+        prev = mk.ev[i];
+      }
+      if (e != hipSuccess) {
+        rc = fail(OT2D_ERR_HIP, "hipEventElapsedTime: %s", hipGetErrorString(e));
+        break;
+      }
+    }
+    if (rc == OT2D_OK) {
+      e = hipMemcpyAsync(&steps, c->d_stepc, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+      if (e != hipSuccess) rc = fail(OT2D_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+      else rc = read_status(c, &st);
+    }
+  }
+  for (int i = 0; i < created; ++i) (void)hipEventDestroy(mk.ev[i]);
+  (void)hipEventDestroy(ev0);
+  if (rc != OT2D_OK) return rc;
+  c->istep0 += steps;
+  if (done) *done = steps;
+  if (steps > 0)
+    for (int i = 0; i < OT2D_NSTAGE; ++i) stage_ms[i] /= steps;
+  if (st == 2) return OT2D_NORM;
+  if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;
 }

@@ -10,7 +10,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/ot2d.h"
+
 #define OT2D_MAXQ 8
+// longest grid line (interior cells) of the turbulence-model line solves
+#define SA_LINE_MAX 1024
 #define OT2D_MAXBC_DEV 25
 
 namespace ot2d {
@@ -91,6 +95,21 @@ __host__ __device__ inline long long vert(const DevMesh& M, int j, int k) {
 
 // launch wrappers (ot2d_kernels.hip)
 namespace ot2d {
+// Optional per-kernel HIP events for profiling: mark(m, s, st) is recorded on
+// the launch stream right after the kernel of stage s (an OT2D_ST_* value).
+struct Marks {
+  static constexpr int CAP = 64;
+  hipEvent_t ev[CAP];
+  int stage[CAP];
+  int n = 0;
+};
+inline void mark(Marks* m, int s, hipStream_t st) {
+  if (m && m->n < Marks::CAP) {
+    (void)hipEventRecord(m->ev[m->n], st);
+    m->stage[m->n++] = s;
+  }
+}
+
 hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st);
 hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
@@ -99,7 +118,7 @@ hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, 
 hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st);
 hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st);
-hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32,
+hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
                      hipStream_t st);
 hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
@@ -108,7 +127,7 @@ hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st)
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd);
 hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
-hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_lds_need(int nk, int ni, int fp32);
 }  // namespace ot2d

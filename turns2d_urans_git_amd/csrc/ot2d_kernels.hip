@@ -1674,7 +1674,8 @@ static int lu_threads(const DevMesh* hM, int nmi) {
 }
 
 template <typename T, int D>
-static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, hipStream_t st) {
+static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, Marks* mk,
+                              hipStream_t st) {
   int nt = lu_threads(hM, nmi);
   if (nt > LU_MAXROWS) return hipErrorInvalidValue;
   long long mx = 0;
@@ -1694,17 +1695,23 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
   hipLaunchKernelGGL(k_lu_prep<T>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_PREP, st);
   hipLaunchKernelGGL((k_lu_sweep<T, 1, D>), dim3(nmi), dim3(nt), lds, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_FORWARD, st);
   hipLaunchKernelGGL((k_lu_sweep<T, -1, D>), dim3(nmi), dim3(nt), lds, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_BACKWARD, st);
   hipLaunchKernelGGL(k_lu_finish<T>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, fuse_update);
-  return hipGetLastError();
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_FINISH, st);
+  return hipSuccess;
 }
 
-hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, hipStream_t st) {
-  return fp32 ? launch_lu_t<float, 8>(dM, hM, nmi, fuse_update, st)
-              : launch_lu_t<double, 6>(dM, hM, nmi, fuse_update, st);
+hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
+                     hipStream_t st) {
+  return fp32 ? launch_lu_t<float, 8>(dM, hM, nmi, fuse_update, mk, st)
+              : launch_lu_t<double, 6>(dM, hM, nmi, fuse_update, mk, st);
 }
 
 size_t lu_lds_need(int nk, int ni, int fp32) {

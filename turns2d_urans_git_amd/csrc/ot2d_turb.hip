@@ -557,71 +557,104 @@ __global__ __launch_bounds__(256) void k_sa_rhs(const DevMesh* __restrict__ Ms) 
   }
 }
 
+// The two scalar tridiagonal solves of vmu_sa are sequential recurrences
+// along each grid line.  One wave owns one line: all lanes stage the line's
+// coefficients in LDS with one bulk load (coalesced along j, gathered along
+// k), lane 0 runs the reference's Thomas recurrence out of LDS (so the
+// dependent chain never waits on memory), and all lanes write the results.
+// The arithmetic is the reference's, operation for operation.
+
 // lsolvej (src/samodel2d.f:1306-1344) and s1, sjmat*bjmat of vmu_sa
-// (src/samodel2d.f:148-152): one thread per k row
+// (src/samodel2d.f:148-152) for the k row blockIdx.x
 __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ Ms) {
+  __shared__ double la[SA_LINE_MAX], lb[SA_LINE_MAX], lc[SA_LINE_MAX], ls[SA_LINE_MAX];
+  __shared__ double lcp[SA_LINE_MAX], lsp[SA_LINE_MAX];
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
-  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  const int k = ks + blockIdx.x * 64 + threadIdx.x;
-  if (k > ke) return;
-  double* s = M.sa_sj;
-  double* c = M.sa_cj;
-  const double* a = M.sa_aj;
-  const double* b = M.sa_bb;
-  long long i = cell(M, js, k);
-  double bb = 1. / b[i];
-  double sp = s[i] * bb, cp = c[i] * bb;
-  s[i] = sp; c[i] = cp;
-  for (int j = js + 1; j <= je; ++j) {
-    i = cell(M, j, k);
-    double ai = a[i];
-    bb = 1. / (b[i] - ai * cp);
-    sp = (s[i] - ai * sp) * bb;
-    cp = c[i] * bb;
-    s[i] = sp; c[i] = cp;
+  const int js = M.js, je = M.je;
+  const int k = M.ks + blockIdx.x;
+  if (k > M.ke) return;
+  const int n = je - js + 1;
+  const long long i0 = cell(M, js, k);
+  double* __restrict__ s = M.sa_sj;
+  const double* __restrict__ c = M.sa_cj;
+  const double* __restrict__ a = M.sa_aj;
+  const double* __restrict__ b = M.sa_bb;
+  for (int t = threadIdx.x; t < n; t += 64) {
+    la[t] = a[i0 + t]; lb[t] = b[i0 + t]; lc[t] = c[i0 + t]; ls[t] = s[i0 + t];
   }
-  // back substitution with s1 = sjmat, sjmat = (s1 - s2)*bjmat, s2 = 0
-  i = cell(M, je, k);
-  double sn = sp;
-  s[i] = (0. + sn - 0.) * b[i];
-  for (int j = je - 1; j >= js; --j) {
-    i = cell(M, j, k);
-    sn = s[i] - c[i] * sn;
-    s[i] = (0. + sn - 0.) * b[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bb = 1. / lb[0];
+    double sp = ls[0] * bb, cp = lc[0] * bb;
+    lsp[0] = sp; lcp[0] = cp;
+#pragma unroll 4
+    for (int t = 1; t < n; ++t) {
+      double ai = la[t];
+      bb = 1. / (lb[t] - ai * cp);
+      sp = (ls[t] - ai * sp) * bb;
+      cp = lc[t] * bb;
+      lsp[t] = sp; lcp[t] = cp;
+    }
+    double sn = sp;
+#pragma unroll 4
+    for (int t = n - 2; t >= 0; --t) {
+      sn = lsp[t] - lcp[t] * sn;
+      lsp[t] = sn;
+    }
   }
+  __syncthreads();
+  // sjmat = (s1 - s2) * bjmat with s2 = 0
+  for (int t = threadIdx.x; t < n; t += 64) s[i0 + t] = (0. + lsp[t] - 0.) * lb[t];
 }
 
-// lsolvek (src/samodel2d.f:1347-1385), s2 and the nu-tilde update
-// (src/samodel2d.f:157-188) with the interior eddy viscosity (c_turm)
+// lsolvek (src/samodel2d.f:1347-1385), s2 and the nu-tilde update with its
+// clip (src/samodel2d.f:157-188), then the interior eddy viscosity (c_turm)
+// for the j column blockIdx.x
 __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ Ms) {
+  __shared__ double la[SA_LINE_MAX], lb[SA_LINE_MAX], lc[SA_LINE_MAX], ls[SA_LINE_MAX];
+  __shared__ double lcp[SA_LINE_MAX], lsp[SA_LINE_MAX];
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
-  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  const int j = js + blockIdx.x * 64 + threadIdx.x;
-  if (j > je) return;
-  double* s = M.sa_sj;
-  double* c = M.sa_ck;
-  const double* a = M.sa_ak;
-  const double* b = M.sa_bb;
-  long long i = cell(M, j, ks);
-  double bb = 1. / b[i];
-  double sp = s[i] * bb, cp = c[i] * bb;
-  s[i] = sp; c[i] = cp;
-  for (int k = ks + 1; k <= ke; ++k) {
-    i = cell(M, j, k);
-    double ai = a[i];
-    bb = 1. / (b[i] - ai * cp);
-    sp = (s[i] - ai * sp) * bb;
-    cp = c[i] * bb;
-    s[i] = sp; c[i] = cp;
+  const int ks = M.ks, ke = M.ke;
+  const int j = M.js + blockIdx.x;
+  if (j > M.je) return;
+  const int n = ke - ks + 1;
+  const long long i0 = cell(M, j, ks);
+  const long long st = M.jd;
+  const double* __restrict__ s = M.sa_sj;
+  const double* __restrict__ c = M.sa_ck;
+  const double* __restrict__ a = M.sa_ak;
+  const double* __restrict__ b = M.sa_bb;
+  for (int t = threadIdx.x; t < n; t += 64) {
+    long long i = i0 + t * st;
+    la[t] = a[i]; lb[t] = b[i]; lc[t] = c[i]; ls[t] = s[i];
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bb = 1. / lb[0];
+    double sp = ls[0] * bb, cp = lc[0] * bb;
+    lsp[0] = sp; lcp[0] = cp;
+#pragma unroll 4
+    for (int t = 1; t < n; ++t) {
+      double ai = la[t];
+      bb = 1. / (lb[t] - ai * cp);
+      sp = (ls[t] - ai * sp) * bb;
+      cp = lc[t] * bb;
+      lsp[t] = sp; lcp[t] = cp;
+    }
+    double sn = sp;
+#pragma unroll 4
+    for (int t = n - 2; t >= 0; --t) {
+      sn = lsp[t] - lcp[t] * sn;
+      lsp[t] = sn;
+    }
+  }
+  __syncthreads();
   const double vnulim = 1.0e-20, relfac = 1.0;
-  double sn = sp;
-  for (int k = ke; k >= ks; --k) {
-    i = cell(M, j, k);
-    if (k < ke) sn = s[i] - c[i] * sn;
-    double dq = relfac * (0. + sn);
+  for (int t = threadIdx.x; t < n; t += 64) {
+    long long i = i0 + t * st;
+    double dq = relfac * (0. + lsp[t]);
     double q5 = M.q[4][i];
     q5 = fmax((q5 + dq * (double)imx(M.iblank[i], 0)), vnulim);
     M.q[4][i] = q5;
@@ -676,26 +709,33 @@ hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
 }
 
 // vmu_sa for every instance (src/samodel2d.f:2-197, itrans = 0)
-hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st) {
   long long mx = max_cells(hM, nmi);
-  int mj = 1, mk = 1;
+  int mj = 1, mk_ = 1;
   for (int i = 0; i < nmi; ++i) {
     mj = hM[i].je - hM[i].js + 1 > mj ? hM[i].je - hM[i].js + 1 : mj;
-    mk = hM[i].ke - hM[i].ks + 1 > mk ? hM[i].ke - hM[i].ks + 1 : mk;
+    mk_ = hM[i].ke - hM[i].ks + 1 > mk_ ? hM[i].ke - hM[i].ks + 1 : mk_;
   }
   hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  mark(mk, OT2D_ST_SA_PRIM, st);
   hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(256), 0, st, dM, 0);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_SA_BC, st);
   hipLaunchKernelGGL(k_sa_rhs, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sa_solve_j, dim3((mk + 63) / 64, nmi), dim3(64), 0, st, dM);
+  mark(mk, OT2D_ST_SA_RHS, st);
+  hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sa_solve_k, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
+  mark(mk, OT2D_ST_SA_SOLVE_J, st);
+  hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_SA_SOLVE_K, st);
   hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(256), 0, st, dM, 1);
-  return hipGetLastError();
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_SA_EDDY, st);
+  return hipSuccess;
 }
 
 }  // namespace ot2d

@@ -110,13 +110,23 @@ class CGrid:
 
 def naca4_cgrid(jmax: int = 257, kmax: int = 65, m: float = 0.0, p: float = 0.0,
                 t: float = 0.12, ds_wall: float = 1e-4, radius: float = 20.0,
-                jtail1: Optional[int] = None, wake_frac: float = 0.125) -> CGrid:
+                jtail1: Optional[int] = None, wake_frac: float = 0.125,
+                wake_aspect: Optional[float] = None, te_cluster: bool = True) -> CGrid:
     """Algebraic C-grid around a NACA 4-digit airfoil (see module docstring).
 
     ``jtail1`` defaults to about ``wake_frac * jmax`` wake points per side.
     Grid lines run straight from each inner point (airfoil or wake cut) to its
     outer-boundary point with geometric spacing from ``ds_wall``; straight
     lines between monotonically ordered inner and outer points never cross.
+    With ``wake_aspect`` the first spacing off the wake cut grows with the
+    streamwise spacing there, ``max(ds_wall, dx / wake_aspect)``, so cells at
+    the downstream boundary do not reach aspect ratios of 1e6 on wall-resolved
+    grids.  ``te_cluster=False`` clusters surface points at the leading edge
+    only (x = 1 - cos(pi s / 2)), which keeps the streamwise spacing at the
+    trailing edge near pi / (2 nside): the reference's implicit operator
+    carries the viscous term of the wall-normal direction only
+    (src/lhs.f:549-553), so tiny streamwise cells under a turbulent wake limit
+    its stable CFL.
     """
     if jtail1 is None:
         jtail1 = max(3, int(round(wake_frac * jmax)) + 1)
@@ -125,8 +135,13 @@ def naca4_cgrid(jmax: int = 257, kmax: int = 65, m: float = 0.0, p: float = 0.0,
         raise ValueError(f"jmax={jmax}, jtail1={jtail1} give {nsurf} surface points; need odd >= 5")
     nside = (nsurf - 1) // 2                 # intervals per side
     # cosine clustering at LE and TE
-    theta = np.linspace(0.0, np.pi, nside + 1)
-    xc = 0.5 * (1.0 + np.cos(theta))         # 1 -> 0  (TE -> LE)
+    if te_cluster:
+        theta = np.linspace(0.0, np.pi, nside + 1)
+        xc = 0.5 * (1.0 + np.cos(theta))     # 1 -> 0  (TE -> LE)
+    else:
+        sgrid = np.linspace(1.0, 0.0, nside + 1)
+        xc = 1.0 - np.cos(0.5 * np.pi * sgrid)
+        xc[0] = 1.0
     xu, yu, xl, yl = naca4_surface(m, p, t, xc)
     # lower surface TE->LE, then upper surface LE->TE
     sx = np.concatenate([xl, xu[::-1][1:]])
@@ -160,9 +175,16 @@ def naca4_cgrid(jmax: int = 257, kmax: int = 65, m: float = 0.0, p: float = 0.0,
 
     xv = np.empty((jmax, kmax), order="F")
     yv = np.empty((jmax, kmax), order="F")
+    ds = np.full(jmax, ds_wall)
+    if wake_aspect:
+        dxw = np.abs(np.diff(wx_lower))                 # lower wake, j = 1 .. jtail1
+        dxj = np.concatenate([dxw[:1], np.maximum(dxw[:-1], dxw[1:]) if dxw.size > 1 else dxw, [0.0]])
+        dsw = np.maximum(ds_wall, dxj / wake_aspect)
+        ds[:jtail1] = dsw
+        ds[jmax - jtail1:] = dsw[::-1]
     for j in range(jmax):
         L = float(np.hypot(x1[j] - x0[j], y1[j] - y0[j]))
-        d = geometric_distribution(ds_wall, L, kmax) / L
+        d = geometric_distribution(float(ds[j]), L, kmax) / L
         xv[j, :] = x0[j] + d * (x1[j] - x0[j])
         yv[j, :] = y0[j] + d * (y1[j] - y0[j])
     # wake cut: make mirrored vertices bit-identical
