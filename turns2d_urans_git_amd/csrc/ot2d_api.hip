@@ -60,6 +60,7 @@ struct ot2d_case {
   DevConst c;
   int nmesh = 0, nspec = 1, nq = 5, nv = 4, lamin = 1, iturb = -1;
   int device = 0;
+  int lu_rl = 1;             // rows per lane of the LU-SGS sweeps
   hipStream_t stream = nullptr;
   std::vector<MeshBuf> m;
   std::vector<DevMesh> hM;   // nmesh*nspec records (mesh-major)
@@ -315,6 +316,12 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
     (void)hipMemcpy(c->d_Ds, Ds.data(), sizeof(double) * ns * ns, hipMemcpyHostToDevice);
     (void)hipMemcpy(c->d_tspec, tspec.data(), sizeof(double) * ns, hipMemcpyHostToDevice);
   }
+  // rows per lane of the LU-SGS sweeps: one wavefront covers every row
+  c->lu_rl = 1;
+  for (int im = 0; im < nmesh; ++im) {
+    int rl = (md[im].kmax - 1 + 63) / 64;
+    c->lu_rl = rl > c->lu_rl ? rl : c->lu_rl;
+  }
   int ntile_total = 0;
   for (int im = 0; im < nmesh; ++im) {
     MeshBuf& mb = c->m[im];
@@ -331,7 +338,7 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
       return fail(OT2D_ERR_UNSUPPORTED, "jmax=%d: the turbulence line solves hold at most %d cells per line",
                   md[im].jmax, SA_LINE_MAX);
     }
-    if (lu_lds_need(md[im].kmax - 1, md[im].jmax - 1, p->lu_fp32) > 160 * 1024) {
+    if (lu_lds_need(md[im].jmax - 1, p->lu_fp32) > 160 * 1024) {
       free_case(c);
       return fail(OT2D_ERR_UNSUPPORTED, "mesh %d (%d x %d) exceeds the LU-SGS sweep's LDS staging", im,
                   md[im].jmax, md[im].kmax);
@@ -346,7 +353,7 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
     A(mb.turmu, mb.n2 * ns); A(mb.tscale, mb.n2 * ns); A(mb.bt, mb.n2 * ns);
     A(mb.xv, mb.nv2 * ns); A(mb.yv, mb.nv2 * ns); A(mb.ugv, mb.nv2 * ns); A(mb.vgv, mb.nv2 * ns);
     A(mb.iblank, mb.n2 * ns);
-    mb.lu_per = lu_workspace_bytes(mb.jd, mb.kd) / sizeof(double);
+    mb.lu_per = lu_workspace_bytes(mb.jd, mb.kd, c->lu_rl) / sizeof(double);
     A(mb.lu, mb.lu_per * ns);
     if (!c->lamin) A(mb.sa, 10 * mb.n2 * ns);
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
@@ -397,6 +404,8 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
       M.xv0 = mb.xv; M.yv0 = mb.yv;
       M.iblank = mb.iblank + o;
       M.lu = mb.lu + mb.lu_per * nsp;
+      M.lu_rl = c->lu_rl;
+      M.lu_fp32 = p->lu_fp32 ? 1 : 0;
       M.bodyflag = 0;
       for (int ib = 0; ib < md[im].nbc; ++ib)
         if (md[im].bc[ib].ibtyp == 5) M.bodyflag = 1;  // src/init.f:59-66
@@ -692,6 +701,7 @@ extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
     }
   }
   c->istep0 += 1;
+  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (spin limit)");
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;
@@ -748,6 +758,7 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
   if (hist && steps > 0) HIPCHK(hipMemcpy(hist, c->d_hist, sizeof(double) * (size_t)steps * per, hipMemcpyDeviceToHost));
   c->istep0 += steps;
   if (done) *done = steps;
+  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (spin limit)");
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;
@@ -830,6 +841,7 @@ extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, 
   if (done) *done = steps;
   if (steps > 0)
     for (int i = 0; i < OT2D_NSTAGE; ++i) stage_ms[i] /= steps;
+  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (spin limit)");
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;

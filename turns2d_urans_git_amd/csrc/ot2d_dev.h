@@ -56,7 +56,9 @@ struct DevMesh {
   // SA workspace (jd,kd) planes: laminar viscosity, u, v, wall distance and
   // the rows of the two tridiagonal systems (src/samodel2d.f:22-30)
   double *sa_vmul, *sa_u, *sa_v, *sa_sn, *sa_sj, *sa_aj, *sa_cj, *sa_ak, *sa_ck, *sa_bb;
-  double* lu;                // LU-SGS hyperplane-major records (workspace)
+  double* lu;                // LU-SGS sweep records (workspace)
+  int lu_rl;                 // rows per lane of the LU-SGS sweeps (1..4)
+  int lu_fp32;               // sweep arithmetic: 1 FP32, 0 FP64
   // per-case bookkeeping (shared by all instances of a case)
   int* status;               // 0 ok, 1 mstop, 2 norm
   int* istep0;               // device copy of istep0
@@ -125,9 +127,9 @@ hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
 hipError_t launch_step_begin(int* istep0, int* status, int* stepc, int ncase, hipStream_t st);
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
-size_t lu_workspace_bytes(int jd, int kd);
+size_t lu_workspace_bytes(int jd, int kd, int rl);
 hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
-size_t lu_lds_need(int nk, int ni, int fp32);
+size_t lu_lds_need(int ni, int fp32);
 }  // namespace ot2d

@@ -931,53 +931,80 @@ __device__ __forceinline__ void nb_k(const CellLoad& c, const LUCellF64& L, doub
 //  k_lu_prep    every cell in parallel: the quantities ilu2d derives per cell
 //               (src/lhs.f:524-564), s*tscale (src/step.f:141-148) and the
 //               point-Jacobi update of the halo sides (src/lhs.f:568-598),
-//               stored as one record per cell in hyperplane-major order:
-//               entry e = p*R + r with p = j+k-(js+ks-2) the hyperplane and
-//               r = k-ks+1 the row, so a sweep reads a hyperplane with unit
-//               stride.  Records are packed in 16-byte vectors.
-//  k_lu_sweep   one block per instance, one lane per row k, hyperplanes in
-//               order (src/lhs.f:616-714 forward, 719-800 backward).  A lane
-//               streams only its own cell's record (prefetched D
-//               hyperplanes ahead); the j neighbour is its own previous cell
-//               (registers), the k neighbour the adjacent lane's previous
-//               cell (LDS, one LDS-only barrier per hyperplane), and the
-//               halo row/column records are staged in LDS once.
+//               stored as 20-value records in the order the sweeps consume
+//               them (layout below).
+//  k_lu_sweep   one wavefront per instance and direction.  Lane l owns RL
+//               consecutive rows k = ks + l*RL + r and walks along j, one
+//               column per step, lane l one step behind lane l-1 (forward):
+//               cell (j,k) is done at step t = (j-js) + l.  Its j neighbour is
+//               the same lane's previous step (registers); its k neighbour is
+//               either the lane's own row below (same step) or lane l-1's top
+//               row of the previous step, passed by a DPP wavefront shift.
+//               No barrier and no LDS round trip sits on the dependent
+//               chain; records stream in D steps ahead.  The backward sweep
+//               mirrors this (t = (je-j) + (NL-1-l), DPP shift the other way).
+//               The result of every cell is the reference's (src/lhs.f:
+//               616-714 forward, 719-800 backward): a hyperplane order and
+//               this skewed order evaluate each cell from the same operands.
 //  k_lu_finish  every cell in parallel: conservative restore of the in-place
 //               primitive conversion (src/lhs.f:805-811), the reference's s
 //               after ilu2d, and optionally the q update (src/step.f:205-211)
 //               and monitor (src/step.f:353-385).
+//
+// Record (20 values): s0..s3, sv, di, uu, vv, uv2, ge, xx, xy, qx, cj, pj,
+// yx, yy, qy, ck, pk (cj/pj and ck/pk: switch and spectral term of the
+// j and k directions for the sweep that reads the record).  FP32 packs it in
+// 5 float4, FP64 in 10 double2; vector q of entry n sits at plane q.
+//
+// Workspace of one instance, in 16-byte vectors (NVEC = 10 planes reserved
+// so both precisions fit):
+//   F   forward records   entry n = (t*RL + r)*64 + l, t = (j-js) + l
+//   B   backward records  entry n = (t*RL + r)*64 + l, t = (je-j) + (NL-1-l);
+//       its s vectors receive the forward result, then the backward result
+//   HCF, HCB  halo column j = js-1 / je+1, entry r*64 + l
+//   HRF, HRB  halo row    k = ks-1 / ke+1, entry j-js
 //
 // T is the arithmetic of the sweep: double reproduces the reference, float
 // is the mixed-precision defect-correction variant (records rounded from the
 // FP64 values computed in k_lu_prep).
 // ----------------------------------------------------------------------
 
-// one cell's record as the sweeps use it (fields of one sweep direction)
-template <typename T>
-struct LURec {
-  T s[4], sv, di, uu, vv, uv2, ge, xx, xy, yx, yy, qx, qy, cj, pj, ck, pk;
+constexpr int LU_NVEC = 10;        // 16-byte planes reserved per array
+constexpr int LU_MAXRL = 4;        // rows per lane: at most 256 rows
+
+template <typename T> struct LUV;
+template <> struct LUV<float> { static constexpr int N = 5; };
+template <> struct LUV<double> { static constexpr int N = 10; };
+
+struct LUGeom {
+  int ni, nk, RL, NL, T;
+  long long nrec;                  // entries of F and of B
+  long long off_b, off_hcf, off_hcb, off_hrf, off_hrb, total;   // in vectors, plane 0 of each array
 };
 
-// record packing: FP64 uses 12 double2 vectors, FP32 6 float4 vectors
-//  FP64  V0 (s0,s1)  V1 (s2,s3)  V2 (sv,di)  V3 (uu,vv)  V4 (uv2,ge)  V5 (xx,xy)
-//        V6 (yx,yy)  V7 (qx,qy)  V8 (cjf,pjf) V9 (ckf,pkf) V10 (cjb,pjb) V11 (ckb,pkb)
-//  FP32  W0 (s0..s3) W1 (sv,di,uu,vv) W2 (uv2,ge,xx,xy) W3 (yx,yy,qx,qy)
-//        W4 (cjf,pjf,ckf,pkf) W5 (cjb,pjb,ckb,pkb)
-template <typename T> struct LUPack;
-template <> struct LUPack<double> { static constexpr int NV = 12; };
-template <> struct LUPack<float> { static constexpr int NV = 6; };
+__host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
+  LUGeom g;
+  g.ni = ni; g.nk = nk; g.RL = RL;
+  g.NL = (nk + RL - 1) / RL;
+  g.T = ni + g.NL - 1;
+  g.nrec = (long long)g.T * RL * 64;
+  long long hc = (long long)RL * 64;
+  g.off_b = LU_NVEC * g.nrec;
+  g.off_hcf = 2 * LU_NVEC * g.nrec;
+  g.off_hcb = g.off_hcf + LU_NVEC * hc;
+  g.off_hrf = g.off_hcb + LU_NVEC * hc;
+  g.off_hrb = g.off_hrf + (long long)LU_NVEC * ni;
+  g.total = g.off_hrb + (long long)LU_NVEC * ni;
+  return g;
+}
 
-__host__ __device__ inline int lu_rows(const DevMesh& M) { return M.ke - M.ks + 3; }
-__host__ __device__ inline int lu_planes(const DevMesh& M) { return M.je + M.ke - M.js - M.ks + 5; }
-
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+__device__ __forceinline__ LUGeom lu_geom(const DevMesh& M) {
+  return lu_geom(M.je - M.js + 1, M.ke - M.ks + 1, M.lu_rl);
 }
 
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 struct D2 { double x, y; };
+struct F4 { float x, y, z, w; };
 
 // buffer descriptor built from wave-uniform values (keeps it in SGPRs, so
 // buffer accesses need no waterfall loop)
@@ -989,63 +1016,71 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(void* p, long lon
   int nb = __builtin_amdgcn_readfirstlane((int)bytes);
   return __builtin_amdgcn_make_buffer_rsrc(up, 0, nb, 0x00020000);
 }
-struct F4 { float x, y, z, w; };
 
-__device__ __forceinline__ D2 ld2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
-}
-__device__ __forceinline__ F4 ld4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(F4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
-}
-__device__ __forceinline__ void st2(D2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, voff, soff, 0);
-}
-__device__ __forceinline__ void st4(F4 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, voff, soff, 0);
-}
-
-// one buffer descriptor per vector plane (wave-uniform, so loads carry no
-// offset arithmetic); vo = entry index * 16
-template <typename T>
-struct LUBufs {
-  __amdgpu_buffer_rsrc_t v[LUPack<T>::NV];
+// one array of the workspace: plane q holds vector q of every entry
+struct LUArr {
+  D2* base;
+  long long n;     // entries per plane
+  __device__ __forceinline__ D2* at(int q, long long e) const { return base + q * n + e; }
 };
+
+__device__ __forceinline__ LUArr lu_arr(const DevMesh& M, long long off, long long n) {
+  LUArr a;
+  a.base = reinterpret_cast<D2*>(M.lu) + off;
+  a.n = n;
+  return a;
+}
+
+// write one record (FP64 values, rounded for the FP32 layout)
 template <typename T>
-__device__ __forceinline__ LUBufs<T> lu_bufs(void* base, long long ne) {
-  LUBufs<T> b;
+__device__ __forceinline__ void rec_put(const LUArr& a, long long e, const double* f) {
+  if (sizeof(T) == 8) {
 #pragma unroll
-  for (int q = 0; q < LUPack<T>::NV; ++q) b.v[q] = uniform_rsrc((char*)base + (long long)q * ne * 16, ne * 16);
-  return b;
-}
-
-template <int DIR>
-__device__ __forceinline__ void rec_load(const LUBufs<double>& B, int vo, LURec<double>& o) {
-  D2 a = ld2(B.v[0], vo, 0), b = ld2(B.v[1], vo, 0), c = ld2(B.v[2], vo, 0), d = ld2(B.v[3], vo, 0);
-  D2 e = ld2(B.v[4], vo, 0), f = ld2(B.v[5], vo, 0), g = ld2(B.v[6], vo, 0), h = ld2(B.v[7], vo, 0);
-  D2 u = ld2(B.v[DIR > 0 ? 8 : 10], vo, 0), w = ld2(B.v[DIR > 0 ? 9 : 11], vo, 0);
-  o.s[0] = a.x; o.s[1] = a.y; o.s[2] = b.x; o.s[3] = b.y; o.sv = c.x; o.di = c.y; o.uu = d.x; o.vv = d.y;
-  o.uv2 = e.x; o.ge = e.y; o.xx = f.x; o.xy = f.y; o.yx = g.x; o.yy = g.y; o.qx = h.x; o.qy = h.y;
-  o.cj = u.x; o.pj = u.y; o.ck = w.x; o.pk = w.y;
-}
-template <int DIR>
-__device__ __forceinline__ void rec_load(const LUBufs<float>& B, int vo, LURec<float>& o) {
-  F4 a = ld4(B.v[0], vo, 0), b = ld4(B.v[1], vo, 0), c = ld4(B.v[2], vo, 0), d = ld4(B.v[3], vo, 0);
-  F4 u = ld4(B.v[DIR > 0 ? 4 : 5], vo, 0);
-  o.s[0] = a.x; o.s[1] = a.y; o.s[2] = a.z; o.s[3] = a.w;
-  o.sv = b.x; o.di = b.y; o.uu = b.z; o.vv = b.w;
-  o.uv2 = c.x; o.ge = c.y; o.xx = c.z; o.xy = c.w;
-  o.yx = d.x; o.yy = d.y; o.qx = d.z; o.qy = d.w;
-  o.cj = u.x; o.pj = u.y; o.ck = u.z; o.pk = u.w;
-}
-__device__ __forceinline__ void rec_store_s(const LUBufs<double>& B, int vo, const double* s) {
-  st2(D2{s[0], s[1]}, B.v[0], vo, 0);
-  st2(D2{s[2], s[3]}, B.v[1], vo, 0);
-}
-__device__ __forceinline__ void rec_store_s(const LUBufs<float>& B, int vo, const float* s) {
-  st4(F4{s[0], s[1], s[2], s[3]}, B.v[0], vo, 0);
+    for (int q = 0; q < 10; ++q) *a.at(q, e) = D2{f[2 * q], f[2 * q + 1]};
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      *reinterpret_cast<F4*>(a.at(q, e)) = F4{(float)f[4 * q], (float)f[4 * q + 1], (float)f[4 * q + 2],
+                                              (float)f[4 * q + 3]};
+  }
 }
 
 template <typename T>
+__device__ __forceinline__ void rec_get(const LUArr& a, long long e, T* f) {
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) { D2 v = *a.at(q, e); f[2 * q] = v.x; f[2 * q + 1] = v.y; }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      F4 v = *reinterpret_cast<const F4*>(a.at(q, e));
+      f[4 * q] = v.x; f[4 * q + 1] = v.y; f[4 * q + 2] = v.z; f[4 * q + 3] = v.w;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void rec_unpack(const u32x4_t* v, T* f) {
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) { D2 d = __builtin_bit_cast(D2, v[q]); f[2 * q] = d.x; f[2 * q + 1] = d.y; }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      F4 d = __builtin_bit_cast(F4, v[q]);
+      f[4 * q] = d.x; f[4 * q + 1] = d.y; f[4 * q + 2] = d.z; f[4 * q + 3] = d.w;
+    }
+  }
+}
+
+// entry of interior cell (j,k) in F (fwd) or B (bwd)
+__device__ __forceinline__ long long lu_entry(const DevMesh& M, const LUGeom& g, int j, int k, int dir) {
+  int kk = k - M.ks, l = kk / g.RL, r = kk - l * g.RL;
+  int t = dir > 0 ? (j - M.js) + l : (M.je - j) + (g.NL - 1 - l);
+  return ((long long)t * g.RL + r) * 64 + l;
+}
+
+// FP64 records (reference form, see above)
 __global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
@@ -1053,8 +1088,10 @@ __global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms)
   const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
   const double eps2 = C.epse * 2.;
   const double epsv = 1. + eps2;
-  const int R = lu_rows(M);
-  const long long ne = (long long)lu_planes(M) * R;   // entries per vector plane
+  const LUGeom g = lu_geom(M);
+  const LUArr F = lu_arr(M, 0, g.nrec), B = lu_arr(M, g.off_b, g.nrec);
+  const LUArr HCF = lu_arr(M, g.off_hcf, (long long)g.RL * 64), HCB = lu_arr(M, g.off_hcb, (long long)g.RL * 64);
+  const LUArr HRF = lu_arr(M, g.off_hrf, g.ni), HRB = lu_arr(M, g.off_hrb, g.ni);
   long long n2 = (long long)jd * kd;
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
     int j = 1 + (int)(i % jd), k = 1 + (int)(i / jd);
@@ -1066,7 +1103,6 @@ __global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms)
     }
     CellLoad c = load_cell(M, i);
     LUCellF64 L = lu_of(C, c, epsv);
-    long long e = (long long)(j + k - (js + ks - 2)) * R + (k - ks + 1);
     bool interior = jin && kin;
     double st[4];
     for (int n = 0; n < 4; ++n) {
@@ -1081,26 +1117,36 @@ __global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms)
     nb_j(c, L, eps2, -1, jb);
     nb_k(c, L, eps2, +1, kf);
     nb_k(c, L, eps2, -1, kb);
-    double f[24] = {st[0], st[1], st[2], st[3], ts * 0.5, L.d, L.uu, L.vv, L.uv2, L.ge,
-                    c.xx, c.xy, c.yx, c.yy, L.qx, L.qy,
-                    jf.chk, jf.spec, kf.chk, kf.spec, jb.chk, jb.spec, kb.chk, kb.spec};
-    if (sizeof(T) == 8) {
-      D2* v = reinterpret_cast<D2*>(M.lu);
-      for (int q = 0; q < 12; ++q) v[q * ne + e] = D2{f[2 * q], f[2 * q + 1]};
-    } else {
-      F4* v = reinterpret_cast<F4*>(M.lu);
-      for (int q = 0; q < 6; ++q)
-        v[q * ne + e] = F4{(float)f[4 * q], (float)f[4 * q + 1], (float)f[4 * q + 2], (float)f[4 * q + 3]};
+    double f[20] = {st[0], st[1], st[2], st[3], ts * 0.5, L.d, L.uu, L.vv, L.uv2, L.ge,
+                    c.xx, c.xy, L.qx, jf.chk, jf.spec, c.yx, c.yy, L.qy, kf.chk, kf.spec};
+    double b[20];
+    for (int n = 0; n < 20; ++n) b[n] = f[n];
+    b[13] = jb.chk; b[14] = jb.spec; b[18] = kb.chk; b[19] = kb.spec;
+    if (interior) {
+      rec_put<double>(F, lu_entry(M, g, j, k, +1), f);
+      rec_put<double>(B, lu_entry(M, g, j, k, -1), b);
+    } else if (kin) {  // halo column
+      int kk = k - ks, l = kk / g.RL, r = kk - l * g.RL;
+      long long e = (long long)r * 64 + l;
+      if (j == js - 1) rec_put<double>(HCF, e, f);
+      if (j == je + 1) rec_put<double>(HCB, e, b);
+    } else {           // halo row
+      long long e = j - js;
+      if (k == ks - 1) rec_put<double>(HRF, e, f);
+      if (k == ke + 1) rec_put<double>(HRB, e, b);
     }
   }
 }
 
+// what a cell needs of a neighbour: its new s and the static terms of the
+// direction it is a neighbour in
 template <typename T>
 struct NbT {
   T s[4];
   T uu, vv, uv2, ge, ri1, ri2, qq, chk, spec;
 };
 
+// A(+/-) or B(+/-) applied to the neighbour's s (src/lhs.f:633-651, 665-681)
 template <typename T>
 __device__ __forceinline__ void apply_nb(T gm1, const NbT<T>& n, T* a) {
   T a4 = n.chk * (n.ri1 * n.s[1] + n.ri2 * n.s[2] - n.qq * n.s[0]);
@@ -1111,117 +1157,420 @@ __device__ __forceinline__ void apply_nb(T gm1, const NbT<T>& n, T* a) {
   a[3] = n.qq * a2 + n.ge * a4 + n.spec * n.s[3];
 }
 
-constexpr int XCH = 13;           // values passed between lanes (and per halo record)
-constexpr int XST = 14;           // LDS record stride (7 x 16 bytes for FP64)
-constexpr int LU_MAXROWS = 256;   // one lane per row, at most 4 waves
-
-// LDS layout (records of XST values, AoS): halo column [nk], halo row [ni],
-// exchange 2 x [nt]
 template <typename T>
-__host__ __device__ inline size_t lu_lds_bytes(int nk, int ni, int nt) {
-  return sizeof(T) * (size_t)XST * ((size_t)nk + (size_t)ni + 2 * (size_t)nt);
-}
-
-template <typename T>
-__device__ __forceinline__ void put13(T* x, const T* s, T uu, T vv, T uv2, T ge, T r1, T r2, T qq, T ck, T sp) {
-  x[0] = s[0]; x[1] = s[1]; x[2] = s[2]; x[3] = s[3];
-  x[4] = uu; x[5] = vv; x[6] = uv2; x[7] = ge;
-  x[8] = r1; x[9] = r2; x[10] = qq; x[11] = ck; x[12] = sp; x[13] = 0;
+__device__ __forceinline__ void jrole(const T* s, const T* f, NbT<T>& n) {
+  n.s[0] = s[0]; n.s[1] = s[1]; n.s[2] = s[2]; n.s[3] = s[3];
+  n.uu = f[6]; n.vv = f[7]; n.uv2 = f[8]; n.ge = f[9];
+  n.ri1 = f[10]; n.ri2 = f[11]; n.qq = f[12]; n.chk = f[13]; n.spec = f[14];
 }
 template <typename T>
-__device__ __forceinline__ void get13(const T* x, NbT<T>& n) {
-  n.s[0] = x[0]; n.s[1] = x[1]; n.s[2] = x[2]; n.s[3] = x[3];
-  n.uu = x[4]; n.vv = x[5]; n.uv2 = x[6]; n.ge = x[7];
-  n.ri1 = x[8]; n.ri2 = x[9]; n.qq = x[10]; n.chk = x[11]; n.spec = x[12];
+__device__ __forceinline__ void krole(const T* s, const T* f, NbT<T>& n) {
+  n.s[0] = s[0]; n.s[1] = s[1]; n.s[2] = s[2]; n.s[3] = s[3];
+  n.uu = f[6]; n.vv = f[7]; n.uv2 = f[8]; n.ge = f[9];
+  n.ri1 = f[15]; n.ri2 = f[16]; n.qq = f[17]; n.chk = f[18]; n.spec = f[19];
 }
 
-template <typename T, int DIR, int D>
-__global__ __launch_bounds__(LU_MAXROWS) void k_lu_sweep(const DevMesh* __restrict__ Ms) {
+// wavefront shift of one value: DIR > 0 lane i receives lane i-1
+// (wave_shr:1), DIR < 0 lane i receives lane i+1 (wave_shl:1)
+template <int DIR>
+__device__ __forceinline__ int dpp_shift(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, DIR > 0 ? 0x138 : 0x130, 0xf, 0xf, false);
+}
+template <int DIR>
+__device__ __forceinline__ float dpp_shift(float v) {
+  return __builtin_bit_cast(float, dpp_shift<DIR>(__builtin_bit_cast(int, v)));
+}
+template <int DIR>
+__device__ __forceinline__ double dpp_shift(double v) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = dpp_shift<DIR>((int)(b & 0xffffffffll));
+  int hi = dpp_shift<DIR>((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <int DIR, typename T>
+__device__ __forceinline__ NbT<T> dpp_shift_nb(const NbT<T>& x) {
+  NbT<T> y;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) y.s[v] = dpp_shift<DIR>(x.s[v]);
+  y.uu = dpp_shift<DIR>(x.uu); y.vv = dpp_shift<DIR>(x.vv); y.uv2 = dpp_shift<DIR>(x.uv2);
+  y.ge = dpp_shift<DIR>(x.ge); y.ri1 = dpp_shift<DIR>(x.ri1); y.ri2 = dpp_shift<DIR>(x.ri2);
+  y.qq = dpp_shift<DIR>(x.qq); y.chk = dpp_shift<DIR>(x.chk); y.spec = dpp_shift<DIR>(x.spec);
+  return y;
+}
+
+constexpr int LU_HR = 13;   // values of a halo-row record kept in LDS
+
+template <typename T>
+__host__ __device__ inline size_t lu_lds_bytes(int ni) {
+  return sizeof(T) * (size_t)LU_HR * (size_t)ni;
+}
+
+template <typename T, int DIR, int RL, int D>
+__global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms) {
   extern __shared__ __attribute__((aligned(16))) double lds_raw[];
+  constexpr int NV = LUV<T>::N;
   const DevMesh& M = Ms[blockIdx.x];
   if (*M.status) return;
   const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  const int nk = ke - ks + 1, ni = je - js + 1;
-  const int R = lu_rows(M);
-  const int NV = LUPack<T>::NV;
-  const long long ne = (long long)lu_planes(M) * R;
-  const LUBufs<T> rs = lu_bufs<T>(M.lu, ne);
-  (void)NV;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  T* hcol = reinterpret_cast<T*>(lds_raw);
-  T* hrow = hcol + (size_t)XST * nk;
-  T* xb0 = hrow + (size_t)XST * ni;
-  T* xb1 = xb0 + (size_t)XST * nt;
+  const LUGeom g = lu_geom(M);
+  const int l = threadIdx.x;
   const T gm1 = (T)M.c.gm1;
-  const int pbase = js + ks - 2;
-  // ---- stage the halo column (j neighbour of each row's first cell) and
-  //      the halo row (k neighbour of the boundary lane) in LDS
-  for (int w = tid; w < nk; w += nt) {
-    int k = ks + w, j = DIR > 0 ? js - 1 : je + 1;
-    LURec<T> o;
-    rec_load<DIR>(rs, ((j + k - pbase) * R + (k - ks + 1)) * 16, o);
-    put13(hcol + XST * w, o.s, o.uu, o.vv, o.uv2, o.ge, o.xx, o.xy, o.qx, o.cj, o.pj);
-  }
-  for (int w = tid; w < ni; w += nt) {
-    int j = js + w, k = DIR > 0 ? ks - 1 : ke + 1;
-    LURec<T> o;
-    rec_load<DIR>(rs, ((j + k - pbase) * R + (k - ks + 1)) * 16, o);
-    put13(hrow + XST * w, o.s, o.uu, o.vv, o.uv2, o.ge, o.yx, o.yy, o.qy, o.ck, o.pk);
-  }
-  __syncthreads();
-  const int k = ks + tid, r = tid + 1;
-  const bool row = tid < nk;
-  const int nplanes = je + ke - js - ks + 1;
-  const bool hb_lane = DIR > 0 ? tid == 0 : tid == nk - 1;
-  // every lane's first cell has the halo column as its j neighbour
-  NbT<T> A;
-  get13(hcol + XST * (row ? tid : 0), A);
-  LURec<T> P[D];
-  auto plane_of = [&](int it) { return DIR > 0 ? js + ks + it : je + ke - it; };
-  // record of this lane at hyperplane it (clamped into the grid for lanes
-  // outside it: they compute throw-away values and store nothing)
-  auto entry = [&](int it) {
-    int m = plane_of(it < nplanes ? it : nplanes - 1), j = m - k;
-    j = j < js ? js : (j > je ? je : j);
-    int kk = row ? k : ks;
-    return (j + kk - pbase) * R + (kk - ks + 1);
-  };
+  const int NT = g.T;
+  // records this sweep reads, and where its results go (the s vectors of B:
+  // forward writes entry T-1-t, backward overwrites entry t)
+  const long long roff = DIR > 0 ? 0 : g.off_b;
+  __amdgpu_buffer_rsrc_t rec[NV];
 #pragma unroll
-  for (int d = 0; d < D - 1; ++d) rec_load<DIR>(rs, entry(d) * 16, P[d]);
-  for (int it0 = 0; it0 < nplanes; it0 += D) {
+  for (int q = 0; q < NV; ++q) rec[q] = uniform_rsrc(reinterpret_cast<D2*>(M.lu) + roff + q * g.nrec, g.nrec * 16);
+  __amdgpu_buffer_rsrc_t outq[2];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int it = it0 + d;
-      if (it >= nplanes) break;
-      rec_load<DIR>(rs, entry(it + D - 1) * 16, P[(d + D - 1) % D]);
-      const LURec<T>& o = P[d];
-      T* wr = (it & 1) ? xb1 : xb0;
-      const T* rd = (it & 1) ? xb0 : xb1;
-      const int m = plane_of(it), j = m - k;
-      const bool act = row && j >= js && j <= je;
-      // k neighbour: halo row for the boundary lane, else the adjacent lane
-      int jh = j - js;
-      jh = jh < 0 ? 0 : (jh >= ni ? ni - 1 : jh);
-      const T* bsrc = hb_lane ? hrow + XST * jh : rd + XST * (tid - DIR >= 0 ? tid - DIR : 0);
-      NbT<T> B;
-      get13(bsrc, B);
-      T a[4], b[4], sn[4];
-      apply_nb(gm1, A, a);
-      apply_nb(gm1, B, b);
-      for (int v = 0; v < 4; ++v) {
-        if (DIR > 0) sn[v] = (o.s[v] + o.sv * (a[v] + b[v])) * o.di;
-        else sn[v] = o.s[v] - o.sv * (a[v] + b[v]) * o.di;
-      }
-      if (act) {
-        rec_store_s(rs, ((m - pbase) * R + r) * 16, sn);
-        for (int v = 0; v < 4; ++v) A.s[v] = sn[v];
-        A.uu = o.uu; A.vv = o.vv; A.uv2 = o.uv2; A.ge = o.ge;
-        A.ri1 = o.xx; A.ri2 = o.xy; A.qq = o.qx; A.chk = o.cj; A.spec = o.pj;
-      }
-      put13(wr + XST * tid, sn, o.uu, o.vv, o.uv2, o.ge, o.yx, o.yy, o.qy, o.ck, o.pk);
-      lds_barrier();
+  for (int q = 0; q < 2; ++q) outq[q] = uniform_rsrc(reinterpret_cast<D2*>(M.lu) + g.off_b + q * g.nrec, g.nrec * 16);
+  // halo row (k neighbour of the boundary lane) staged in LDS
+  T* hr = reinterpret_cast<T*>(lds_raw);
+  {
+    const LUArr HR = lu_arr(M, DIR > 0 ? g.off_hrf : g.off_hrb, g.ni);
+    for (int w = l; w < g.ni; w += 64) {
+      T f[20];
+      rec_get<T>(HR, w, f);
+      NbT<T> n;
+      krole(f, f, n);
+      T* x = hr + LU_HR * w;
+      x[0] = n.s[0]; x[1] = n.s[1]; x[2] = n.s[2]; x[3] = n.s[3]; x[4] = n.uu; x[5] = n.vv; x[6] = n.uv2;
+      x[7] = n.ge; x[8] = n.ri1; x[9] = n.ri2; x[10] = n.qq; x[11] = n.chk; x[12] = n.spec;
     }
   }
+  __syncthreads();
+  // j neighbour of every row's first cell: the halo column
+  NbT<T> A[RL];
+  {
+    const LUArr HC = lu_arr(M, DIR > 0 ? g.off_hcf : g.off_hcb, (long long)g.RL * 64);
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      T f[20];
+      rec_get<T>(HC, (long long)r * 64 + l, f);
+      jrole(f, f, A[r]);
+    }
+  }
+  NbT<T> kout;
+  jrole(hr, hr, kout);   // any finite values: only read by lanes that are inactive
+  const bool blane = DIR > 0 ? (l == 0) : (l == g.NL - 1);
+  const int vo = l * 16;
+  u32x4_t P[D][RL][NV];
+  auto fetch = [&](int tp, u32x4_t (&dst)[RL][NV]) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r)
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+        dst[r][q] = __builtin_amdgcn_raw_buffer_load_b128(rec[q], vo, (tp * RL + r) * 1024, 0);
+  };
+#pragma unroll
+  for (int d = 0; d < D - 1; ++d)
+    if (d < NT) fetch(d, P[d]);
+  for (int t0 = 0; t0 < NT; t0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int t = t0 + d;
+      if (t >= NT) break;
+      if (t + D - 1 < NT) fetch(t + D - 1, P[(d + D - 1) % D]);
+      const int j = DIR > 0 ? js + t - l : je - t + (g.NL - 1 - l);
+      const bool lact = l < g.NL && j >= js && j <= je;
+      NbT<T> kin = dpp_shift_nb<DIR>(kout);
+      if (blane) {
+        int jh = j - js;
+        jh = jh < 0 ? 0 : (jh >= g.ni ? g.ni - 1 : jh);
+        const T* x = hr + LU_HR * jh;
+        kin.s[0] = x[0]; kin.s[1] = x[1]; kin.s[2] = x[2]; kin.s[3] = x[3];
+        kin.uu = x[4]; kin.vv = x[5]; kin.uv2 = x[6]; kin.ge = x[7]; kin.ri1 = x[8]; kin.ri2 = x[9];
+        kin.qq = x[10]; kin.chk = x[11]; kin.spec = x[12];
+      }
+      const int ostep = DIR > 0 ? NT - 1 - t : t;
+#pragma unroll
+      for (int rr = 0; rr < RL; ++rr) {
+        const int r = DIR > 0 ? rr : RL - 1 - rr;
+        const int k = ks + l * RL + r;
+        const bool v = lact && k <= ke;
+        T f[20];
+        rec_unpack<T>(P[d][r], f);
+        T a[4], b[4], sn[4];
+        apply_nb(gm1, A[r], a);
+        apply_nb(gm1, kin, b);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          if (DIR > 0) sn[w] = (f[w] + f[4] * (a[w] + b[w])) * f[5];
+          else sn[w] = f[w] - f[4] * (a[w] + b[w]) * f[5];
+        }
+        if (v) {
+          const int so = (ostep * RL + r) * 1024;
+          if constexpr (sizeof(T) == 8) {
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, D2{sn[0], sn[1]}), outq[0], vo, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, D2{sn[2], sn[3]}), outq[1], vo, so, 0);
+          } else {
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}),
+                                                   outq[0], vo, so, 0);
+          }
+          jrole(sn, f, A[r]);
+          krole(sn, f, kin);
+        }
+      }
+      kout = kin;
+    }
+  }
+}
+
+// ------------------------------------------------- FP32 LU-SGS (matrix form)
+// The mixed-precision sweeps evaluate the same update as ilu2d with the
+// operator assembled per cell in FP64 and rounded once:
+//   forward   s(j,k) = u + PA . s(j-1,k) + PB . s(j,k-1)
+//             u = st*di,  PA = sv*di*A+(j-1,k),  PB = sv*di*B+(j,k-1)
+//   backward  s(j,k) = s_fwd(j,k) + PA' . s(j+1,k) + PB' . s(j,k+1)
+//             PA' = -sv*di*A-(j+1,k),  PB' = -sv*di*B-(j,k+1)
+// (src/lhs.f:616-714 and 719-800 with a(j) = A . s, b(j) = B . s written
+// as 4x4 matrices).  Neighbours in the halo ring hold fixed point-Jacobi
+// values (src/lhs.f:568-598), so their terms are folded into u (forward)
+// and into cB (backward, added by the forward sweep to its output).
+//   F record (10 float4): u, cB, PA rows 0-3, PB rows 0-3
+//   B record ( 9 float4): s_fwd + cB (written by the forward sweep), PA'
+//                         rows 0-3, PB' rows 0-3
+// The halo-ring s values themselves are scaled in k_lu_finish (the prep
+// reads the unscaled neighbours).
+constexpr int LU32_NVF = 10, LU32_NVB = 9;
+
+// rows of the 4x4 matrix M with a = M . s for a neighbour's A(+/-) or
+// B(+/-) term (src/lhs.f:633-651)
+__device__ __forceinline__ void nb_matrix(const LUNb& n, double gm1, double m[16]) {
+  double v4[4] = {-n.chk * n.qq, n.chk * n.ri1, n.chk * n.ri2, 0.0};
+  double cg = n.chk * gm1;
+  double v2[4] = {cg * n.uv2, -cg * n.uu, -cg * n.vv, cg};
+  for (int i = 0; i < 4; ++i) {
+    m[0 * 4 + i] = v4[i];
+    m[1 * 4 + i] = n.ri1 * v2[i] + n.uu * v4[i];
+    m[2 * 4 + i] = n.ri2 * v2[i] + n.vv * v4[i];
+    m[3 * 4 + i] = n.qq * v2[i] + n.ge * v4[i];
+  }
+  for (int w = 0; w < 4; ++w) m[w * 4 + w] += n.spec;
+}
+
+// neighbour term of cell i: matrix scaled by f, or folded into c when the
+// neighbour is a halo-ring cell (its s after s*tscale and point Jacobi)
+__device__ __forceinline__ void lu32_term(const DevMesh& M, long long in, bool halo, int jdir, int dir, double f,
+                                          double eps2, double epsv, double* P, double* c) {
+  CellLoad cn = load_cell(M, in);
+  LUCellF64 Ln = lu_of(M.c, cn, epsv);
+  LUNb nb;
+  if (jdir) nb_j(cn, Ln, eps2, dir, nb);
+  else nb_k(cn, Ln, eps2, dir, nb);
+  double m[16];
+  nb_matrix(nb, M.c.gm1, m);
+  for (int q = 0; q < 16; ++q) P[q] = f * m[q];
+  if (halo) {
+    double sh[4];
+    for (int n = 0; n < 4; ++n) sh[n] = M.s[n][in] * cn.ts * Ln.d;
+    for (int w = 0; w < 4; ++w) c[w] += P[w * 4 + 0] * sh[0] + P[w * 4 + 1] * sh[1] + P[w * 4 + 2] * sh[2] + P[w * 4 + 3] * sh[3];
+    for (int q = 0; q < 16; ++q) P[q] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int jd = M.jd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const double eps2 = M.c.epse * 2.;
+  const double epsv = 1. + eps2;
+  const LUGeom g = lu_geom(M);
+  const LUArr F = lu_arr(M, 0, g.nrec), B = lu_arr(M, g.off_b, g.nrec);
+  const long long ncell = (long long)g.ni * g.nk;
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
+    const int j = js + (int)(w % g.ni), k = ks + (int)(w / g.ni);
+    const long long i = cell(M, j, k);
+    CellLoad c0 = load_cell(M, i);
+    LUCellF64 L0 = lu_of(M.c, c0, epsv);
+    const double di = L0.d, f = (c0.ts * 0.5) * di;
+    double u[4], cb[4] = {0.0, 0.0, 0.0, 0.0}, PA[16], PB[16], QA[16], QB[16];
+    for (int n = 0; n < 4; ++n) u[n] = (M.s[n][i] * c0.ts) * di;
+    lu32_term(M, i - 1, j == js, 1, +1, f, eps2, epsv, PA, u);
+    lu32_term(M, i - jd, k == ks, 0, +1, f, eps2, epsv, PB, u);
+    lu32_term(M, i + 1, j == je, 1, -1, -f, eps2, epsv, QA, cb);
+    lu32_term(M, i + jd, k == ke, 0, -1, -f, eps2, epsv, QB, cb);
+    const long long ef = lu_entry(M, g, j, k, +1), eb = lu_entry(M, g, j, k, -1);
+    auto put = [](const LUArr& a, long long e, int q, const double* v) {
+      *reinterpret_cast<F4*>(a.at(q, e)) = F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+    };
+    put(F, ef, 0, u);
+    put(F, ef, 1, cb);
+    for (int r = 0; r < 4; ++r) { put(F, ef, 2 + r, PA + 4 * r); put(F, ef, 6 + r, PB + 4 * r); }
+    for (int r = 0; r < 4; ++r) { put(B, eb, 1 + r, QA + 4 * r); put(B, eb, 5 + r, QB + 4 * r); }
+  }
+}
+
+// Bounded spin on an LDS word (workgroup-scope acquire).  Gives up after
+// LU32_SPIN polls, raising the shared abort flag, so every wave always
+// reaches the end of the kernel.
+constexpr int LU32_SPIN = 1 << 24;
+
+__device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_flag) {
+  for (int n = 0; n < LU32_SPIN; ++n) {
+    if (__hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return true;
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return false;
+}
+
+__device__ __forceinline__ float4 f4_of(u32x4_t v) { return __builtin_bit_cast(float4, v); }
+
+// One FP32 sweep of one instance.  Wave 0 computes (lane l owns RL rows,
+// skewed as in k_lu_sweep); waves 1..NP stream the records of step t into
+// LDS slot t % S and publish ready[slot] = t + 1; the computing wave
+// publishes consumed = t + 1 once slot t is no longer needed.  The only
+// barrier is the initialisation one.
+template <int DIR, int RL, int NV, int S, int NP>
+__global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __restrict__ Ms) {
+  extern __shared__ __attribute__((aligned(16))) double lds_raw[];
+  __shared__ int ready[S];
+  __shared__ int consumed, abort_flag, st0;
+  const DevMesh& M = Ms[blockIdx.x];
+  if (threadIdx.x == 0) {
+    st0 = *M.status;
+    consumed = 0;
+    abort_flag = 0;
+  }
+  if (threadIdx.x < S) ready[threadIdx.x] = 0;
+  __syncthreads();
+  if (st0) return;
+  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][RL][NV][64]
+  const LUGeom g = lu_geom(M);
+  const int NT = g.T;
+  // wave index made wave-uniform for the compiler (keeps buffer offsets in
+  // SGPRs: no waterfall loops around the loads)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const long long roff = DIR > 0 ? 0 : g.off_b;
+  if (wave > 0) {
+    // ---------------------------------------------------------- producers
+    __amdgpu_buffer_rsrc_t rec[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      rec[q] = uniform_rsrc(reinterpret_cast<D2*>(M.lu) + roff + q * g.nrec, g.nrec * 16);
+    auto fetch = [&](int tt, u32x4_t (&v)[RL][NV]) {
+#pragma unroll
+      for (int r = 0; r < RL; ++r)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[r][q] = __builtin_amdgcn_raw_buffer_load_b128(rec[q], l * 16, (tt * RL + r) * 1024, 0);
+    };
+    u32x4_t va[RL][NV], vb[RL][NV];
+    int tt = wave - 1;
+    if (tt < NT) fetch(tt, va);
+    for (; tt < NT; tt += 2 * NP) {
+      const int t2 = tt + NP;
+      if (t2 < NT) fetch(t2, vb);
+      if (!spin_until_ge(&consumed, tt - S, &abort_flag)) break;
+      {
+        u32x4_t* d = ring + (size_t)(tt % S) * RL * NV * 64;
+#pragma unroll
+        for (int r = 0; r < RL; ++r)
+#pragma unroll
+          for (int q = 0; q < NV; ++q) d[(r * NV + q) * 64 + l] = va[r][q];
+      }
+      __hip_atomic_store(&ready[tt % S], tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (t2 >= NT) break;
+      const int t3 = t2 + NP;
+      if (t3 < NT) fetch(t3, va);
+      if (!spin_until_ge(&consumed, t2 - S, &abort_flag)) break;
+      {
+        u32x4_t* d = ring + (size_t)(t2 % S) * RL * NV * 64;
+#pragma unroll
+        for (int r = 0; r < RL; ++r)
+#pragma unroll
+          for (int q = 0; q < NV; ++q) d[(r * NV + q) * 64 + l] = vb[r][q];
+      }
+      __hip_atomic_store(&ready[t2 % S], t2 + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return;
+  }
+  // ------------------------------------------------------------- compute
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  __amdgpu_buffer_rsrc_t out = uniform_rsrc(reinterpret_cast<D2*>(M.lu) + g.off_b, g.nrec * 16);
+  float sA[RL][4];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) sA[r][0] = sA[r][1] = sA[r][2] = sA[r][3] = 0.f;
+  // k neighbour leaving this lane.  Lanes >= NL never update it, so the
+  // wavefront shift hands zeros to the boundary lane in both directions
+  // (lane 0 forward reads the DPP fill value, lane NL-1 backward reads lane
+  // NL) and the folded halo terms need no special case.  The shift must run
+  // with every lane enabled: a source lane disabled in EXEC reads as the
+  // fill value, so no select or branch may guard it.
+  float kout[4] = {0.f, 0.f, 0.f, 0.f};
+  auto rd = [&](int t, u32x4_t (&v)[RL][NV]) {
+    const u32x4_t* s = ring + (size_t)(t % S) * RL * NV * 64;
+#pragma unroll
+    for (int r = 0; r < RL; ++r)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[r][q] = s[(r * NV + q) * 64 + l];
+  };
+  // one step with the data of step t in C; reads step t+1 into N first
+  auto step = [&](int t, const u32x4_t (&C)[RL][NV], u32x4_t (&N)[RL][NV]) {
+    if (t + 1 < NT) rd(t + 1, N);
+    const int f2 = t + 2 < NT ? __hip_atomic_load(&ready[(t + 2) % S], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                              : 0;
+    const int j = DIR > 0 ? js + t - l : je - t + (g.NL - 1 - l);
+    const bool lact = l < g.NL && j >= js && j <= je;
+    float kin[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) kin[w] = dpp_shift<DIR>(kout[w]);
+    const int ostep = DIR > 0 ? NT - 1 - t : t;
+#pragma unroll
+    for (int rr = 0; rr < RL; ++rr) {
+      const int r = DIR > 0 ? rr : RL - 1 - rr;
+      const int k = ks + l * RL + r;
+      const bool v = lact && k <= ke;
+      const int pa = DIR > 0 ? 2 : 1, pb = pa + 4;
+      float4 u = f4_of(C[r][0]);
+      float sn[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        float4 a = f4_of(C[r][pa + w]), b = f4_of(C[r][pb + w]);
+        float uw = w == 0 ? u.x : (w == 1 ? u.y : (w == 2 ? u.z : u.w));
+        float p = __builtin_fmaf(a.x, sA[r][0], uw);
+        p = __builtin_fmaf(a.y, sA[r][1], p);
+        p = __builtin_fmaf(a.z, sA[r][2], p);
+        p = __builtin_fmaf(a.w, sA[r][3], p);
+        p = __builtin_fmaf(b.x, kin[0], p);
+        p = __builtin_fmaf(b.y, kin[1], p);
+        p = __builtin_fmaf(b.z, kin[2], p);
+        sn[w] = __builtin_fmaf(b.w, kin[3], p);
+      }
+      if (v) {
+        float o[4] = {sn[0], sn[1], sn[2], sn[3]};
+        if (DIR > 0) {
+          float4 cb = f4_of(C[r][1]);
+          o[0] += cb.x; o[1] += cb.y; o[2] += cb.z; o[3] += cb.w;
+        }
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{o[0], o[1], o[2], o[3]}), out, l * 16,
+                                               (ostep * RL + r) * 1024, 0);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) { sA[r][w] = sn[w]; kin[w] = sn[w]; }
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) kout[w] = kin[w];
+    bool okk = true;
+    if (t + 2 < NT && f2 < t + 3) okk = spin_until_ge(&ready[(t + 2) % S], t + 3, &abort_flag);
+    // slot t+2 is published: keep the next step's reads of it after this point
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return okk;
+  };
+  u32x4_t C0[RL][NV], C1[RL][NV];
+  bool ok = spin_until_ge(&ready[0], 1, &abort_flag);
+  if (ok && NT > 1) ok = spin_until_ge(&ready[1 % S], 2, &abort_flag);
+  if (ok) rd(0, C0);
+  // two steps per trip, the two register sets alternating (no copies)
+  for (int t = 0; t < NT && ok; t += 2) {
+    ok = step(t, C0, C1);
+    if (t + 1 < NT && ok) ok = step(t + 1, C1, C0);
+  }
+  if (!ok || __hip_atomic_load(&abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+    if (l == 0) atomicCAS(M.status, 0, 3);
 }
 
 // round trip of ilu2d's in-place primitive conversion (src/lhs.f:524-531,
@@ -1246,24 +1595,36 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  const int R = lu_rows(M);
-  const long long ne = (long long)lu_planes(M) * R;
+  const LUGeom g = lu_geom(M);
+  const LUArr B = lu_arr(M, g.off_b, g.nrec);
   long long n2 = (long long)jd * kd;
   int bad = 0;
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
     int j = 1 + (int)(i % jd), k = 1 + (int)(i / jd);
+    const bool jin = j >= js && j <= je, kin = k >= ks && k <= ke;
+    if constexpr (sizeof(T) == 4) {
+      // halo ring: s*tscale (src/step.f:141-148), point Jacobi on the sides
+      // (src/lhs.f:568-598); the FP32 prep folded these values unscaled-read
+      if (!(jin && kin)) {
+        CellLoad c = load_cell(M, i);
+        double sc = c.ts;
+        double d = (jin || kin) ? lu_of(M.c, c, 1. + 2. * M.c.epse).d : 1.0;
+        for (int n = 0; n < 4; ++n) {
+          double v = M.s[n][i] * sc;
+          M.s[n][i] = (jin || kin) ? v * d : v;
+        }
+      }
+    }
     double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
     q_roundtrip(q1, q2, q3, q4, M.q[M.nq - 1][i]);
-    if (j >= js && j <= je && k >= ks && k <= ke) {
-      long long e = (long long)(j + k - (js + ks - 2)) * R + (k - ks + 1);
+    if (jin && kin) {
+      long long e = lu_entry(M, g, j, k, -1);
       double dq[4];
-      if (sizeof(T) == 8) {
-        const D2* v = reinterpret_cast<const D2*>(M.lu);
-        D2 a = v[e], b = v[ne + e];
+      if constexpr (sizeof(T) == 8) {
+        D2 a = *B.at(0, e), b = *B.at(1, e);
         dq[0] = a.x; dq[1] = a.y; dq[2] = b.x; dq[3] = b.y;
       } else {
-        const F4* v = reinterpret_cast<const F4*>(M.lu);
-        F4 a = v[e];
+        F4 a = *reinterpret_cast<const F4*>(B.at(0, e));
         dq[0] = (double)a.x; dq[1] = (double)a.y; dq[2] = (double)a.z; dq[3] = (double)a.w;
       }
       for (int n = 0; n < 4; ++n) M.s[n][i] = dq[n];
@@ -1664,42 +2025,32 @@ hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st) {
   return hipGetLastError();
 }
 
-static int lu_threads(const DevMesh* hM, int nmi) {
-  int mx = 1;
-  for (int i = 0; i < nmi; ++i) {
-    int nk = hM[i].ke - hM[i].ks + 1;
-    mx = nk > mx ? nk : mx;
-  }
-  return ((mx + 63) / 64) * 64;
-}
-
-template <typename T, int D>
+template <typename T, int RL, int D>
 static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, Marks* mk,
                               hipStream_t st) {
-  int nt = lu_threads(hM, nmi);
-  if (nt > LU_MAXROWS) return hipErrorInvalidValue;
   long long mx = 0;
-  int nkmax = 0, nimax = 0;
+  int nimax = 0;
   for (int i = 0; i < nmi; ++i) {
     mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
-    nkmax = hM[i].ke - hM[i].ks + 1 > nkmax ? hM[i].ke - hM[i].ks + 1 : nkmax;
     nimax = hM[i].je - hM[i].js + 1 > nimax ? hM[i].je - hM[i].js + 1 : nimax;
   }
-  size_t lds = lu_lds_bytes<T>(nkmax, nimax, nt);
+  size_t lds = lu_lds_bytes<T>(nimax);
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep<T, 1, D>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep<T, -1, D>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep<T, 1, RL, D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep<T, -1, RL, D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_lu_prep<T>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  hipLaunchKernelGGL(k_lu_prep, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_PREP, st);
-  hipLaunchKernelGGL((k_lu_sweep<T, 1, D>), dim3(nmi), dim3(nt), lds, st, dM);
+  hipLaunchKernelGGL((k_lu_sweep<T, 1, RL, D>), dim3(nmi), dim3(64), lds, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_FORWARD, st);
-  hipLaunchKernelGGL((k_lu_sweep<T, -1, D>), dim3(nmi), dim3(nt), lds, st, dM);
+  hipLaunchKernelGGL((k_lu_sweep<T, -1, RL, D>), dim3(nmi), dim3(64), lds, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_BACKWARD, st);
   hipLaunchKernelGGL(k_lu_finish<T>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, fuse_update);
@@ -1708,21 +2059,78 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
   return hipSuccess;
 }
 
+// LDS ring slots of the FP32 sweeps: as many slots of RL*NV KiB as fit in
+// 150 KiB
+constexpr int lu32_slots(int rl, int nv) { return (150 * 1024) / (rl * nv * 1024); }
+constexpr int LU32_NP = 7;
+
+template <int RL>
+static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, Marks* mk,
+                              hipStream_t st) {
+  constexpr int SF = lu32_slots(RL, LU32_NVF), SB = lu32_slots(RL, LU32_NVB);
+  static_assert(SF >= 3 && SB >= 3, "LDS ring too small");
+  long long mx = 0, nc = 0;
+  for (int i = 0; i < nmi; ++i) {
+    mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+    long long c = (long long)(hM[i].je - hM[i].js + 1) * (hM[i].ke - hM[i].ks + 1);
+    nc = c > nc ? c : nc;
+  }
+  const size_t ldsf = (size_t)SF * RL * LU32_NVF * 1024, ldsb = (size_t)SB * RL * LU32_NVB * 1024;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, RL, LU32_NVF, SF, LU32_NP>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsf);
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, RL, LU32_NVB, SB, LU32_NP>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_lu_prep32, dim3(nblk(nc, 256), nmi), dim3(256), 0, st, dM);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_PREP, st);
+  hipLaunchKernelGGL((k_lu_sweep32<1, RL, LU32_NVF, SF, LU32_NP>), dim3(nmi), dim3(64 * (1 + LU32_NP)), ldsf, st,
+                     dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_FORWARD, st);
+  hipLaunchKernelGGL((k_lu_sweep32<-1, RL, LU32_NVB, SB, LU32_NP>), dim3(nmi), dim3(64 * (1 + LU32_NP)), ldsb, st,
+                     dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_BACKWARD, st);
+  hipLaunchKernelGGL(k_lu_finish<float>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, fuse_update);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_FINISH, st);
+  return hipSuccess;
+}
+
+// rows per lane rl = ceil(rows / 64) of the case (all instances share it)
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
                      hipStream_t st) {
-  return fp32 ? launch_lu_t<float, 8>(dM, hM, nmi, fuse_update, mk, st)
-              : launch_lu_t<double, 6>(dM, hM, nmi, fuse_update, mk, st);
+  const int rl = hM[0].lu_rl;
+  if (fp32) {
+    switch (rl) {
+      case 1: return launch_lu32<1>(dM, hM, nmi, fuse_update, mk, st);
+      case 2: return launch_lu32<2>(dM, hM, nmi, fuse_update, mk, st);
+      case 3: return launch_lu32<3>(dM, hM, nmi, fuse_update, mk, st);
+      case 4: return launch_lu32<4>(dM, hM, nmi, fuse_update, mk, st);
+    }
+  } else {
+    // prefetch depth D chosen so the record ring fits the register file
+    switch (rl) {
+      case 1: return launch_lu_t<double, 1, 3>(dM, hM, nmi, fuse_update, mk, st);
+      case 2: return launch_lu_t<double, 2, 2>(dM, hM, nmi, fuse_update, mk, st);
+      case 3: return launch_lu_t<double, 3, 1>(dM, hM, nmi, fuse_update, mk, st);
+      case 4: return launch_lu_t<double, 4, 1>(dM, hM, nmi, fuse_update, mk, st);
+    }
+  }
+  return hipErrorInvalidValue;
 }
 
-size_t lu_lds_need(int nk, int ni, int fp32) {
-  int nt = ((nk + 63) / 64) * 64;
-  return fp32 ? lu_lds_bytes<float>(nk, ni, nt) : lu_lds_bytes<double>(nk, ni, nt);
+size_t lu_lds_need(int ni, int fp32) {
+  return fp32 ? lu_lds_bytes<float>(ni) : lu_lds_bytes<double>(ni);
 }
 
-size_t lu_workspace_bytes(int jd, int kd) {
-  DevMesh M;
-  M.js = 2; M.je = jd - 1; M.ks = 2; M.ke = kd - 1;
-  return 16 * (size_t)LUPack<double>::NV * lu_planes(M) * lu_rows(M);
+size_t lu_workspace_bytes(int jd, int kd, int rl) {
+  return 16 * (size_t)lu_geom(jd - 2, kd - 2, rl).total;
 }
 
 hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
