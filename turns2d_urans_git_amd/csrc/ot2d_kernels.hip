@@ -977,9 +977,10 @@ template <> struct LUV<float> { static constexpr int N = 5; };
 template <> struct LUV<double> { static constexpr int N = 10; };
 
 struct LUGeom {
-  int ni, nk, RL, NL, T;
-  long long nrec;                  // entries of F and of B
-  long long off_b, off_hcf, off_hcb, off_hrf, off_hrb, total;   // in vectors, plane 0 of each array
+  int ni, nk, RL, NL, T;           // FP64 sweep: NL lanes, T steps
+  int W, TW;                       // FP32 sweep: W workgroups of 64 rows, TW steps each
+  long long nrec;                  // entries of F and of B (capacity for both layouts)
+  long long off_b, off_hcf, off_hcb, off_hrf, off_hrb, off_mbf, off_mbb, total;  // in vectors
 };
 
 __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
@@ -987,14 +988,19 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   g.ni = ni; g.nk = nk; g.RL = RL;
   g.NL = (nk + RL - 1) / RL;
   g.T = ni + g.NL - 1;
-  g.nrec = (long long)g.T * RL * 64;
+  g.W = (nk + 63) / 64;
+  g.TW = ni + 63;
+  long long n64 = (long long)g.T * RL * 64, n32 = (long long)g.W * g.TW * 64;
+  g.nrec = n64 > n32 ? n64 : n32;
   long long hc = (long long)RL * 64;
   g.off_b = LU_NVEC * g.nrec;
   g.off_hcf = 2 * LU_NVEC * g.nrec;
   g.off_hcb = g.off_hcf + LU_NVEC * hc;
   g.off_hrf = g.off_hcb + LU_NVEC * hc;
   g.off_hrb = g.off_hrf + (long long)LU_NVEC * ni;
-  g.total = g.off_hrb + (long long)LU_NVEC * ni;
+  g.off_mbf = g.off_hrb + (long long)LU_NVEC * ni;       // FP32 mailboxes, W x ni float4 each
+  g.off_mbb = g.off_mbf + (long long)g.W * ni;
+  g.total = g.off_mbb + (long long)g.W * ni;
   return g;
 }
 
@@ -1081,6 +1087,15 @@ __device__ __forceinline__ long long lu_entry(const DevMesh& M, const LUGeom& g,
 }
 
 // FP64 records (reference form, see above)
+// entry of interior cell (j,k) in the FP32 layout: workgroup w = (k-ks)/64
+// owns rows ks+64w .. ks+64w+63, lane l = (k-ks)%64; forward step
+// t = (j-js) + l, backward step t = (je-j) + (63-l)
+__device__ __forceinline__ long long lu_entry32(const DevMesh& M, const LUGeom& g, int j, int k, int dir) {
+  int kk = k - M.ks, w = kk >> 6, l = kk & 63;
+  int t = dir > 0 ? (j - M.js) + l : (M.je - j) + (63 - l);
+  return ((long long)w * g.TW + t) * 64 + l;
+}
+
 __global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
@@ -1334,6 +1349,9 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // The halo-ring s values themselves are scaled in k_lu_finish (the prep
 // reads the unscaled neighbours).
 constexpr int LU32_NVF = 10, LU32_NVB = 9;
+// bit pattern of an empty mailbox word: a signalling NaN, which no
+// arithmetic result takes (they are quiet NaNs at worst)
+constexpr unsigned LU32_EMPTY = 0x7fa5a5a5u;
 
 // rows of the 4x4 matrix M with a = M . s for a neighbour's A(+/-) or
 // B(+/-) term (src/lhs.f:633-651)
@@ -1379,6 +1397,11 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
   const LUGeom g = lu_geom(M);
   const LUArr F = lu_arr(M, 0, g.nrec), B = lu_arr(M, g.off_b, g.nrec);
   const long long ncell = (long long)g.ni * g.nk;
+  // both mailboxes (2 * W * ni float4) start as the empty sentinel
+  const long long nmb = 2ll * g.W * g.ni;
+  u32x4_t* mb = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_mbf);
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < nmb; w += 256ll * gridDim.x)
+    mb[w] = u32x4_t{LU32_EMPTY, LU32_EMPTY, LU32_EMPTY, LU32_EMPTY};
   for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
     const int j = js + (int)(w % g.ni), k = ks + (int)(w / g.ni);
     const long long i = cell(M, j, k);
@@ -1391,7 +1414,7 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
     lu32_term(M, i - jd, k == ks, 0, +1, f, eps2, epsv, PB, u);
     lu32_term(M, i + 1, j == je, 1, -1, -f, eps2, epsv, QA, cb);
     lu32_term(M, i + jd, k == ke, 0, -1, -f, eps2, epsv, QB, cb);
-    const long long ef = lu_entry(M, g, j, k, +1), eb = lu_entry(M, g, j, k, -1);
+    const long long ef = lu_entry32(M, g, j, k, +1), eb = lu_entry32(M, g, j, k, -1);
     auto put = [](const LUArr& a, long long e, int q, const double* v) {
       *reinterpret_cast<F4*>(a.at(q, e)) = F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
     };
@@ -1402,16 +1425,49 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
   }
 }
 
-// Bounded spin on an LDS word (workgroup-scope acquire).  Gives up after
-// LU32_SPIN polls, raising the shared abort flag, so every wave always
-// reaches the end of the kernel.
-constexpr int LU32_SPIN = 1 << 24;
+// Bounded waits.  Every wait of the FP32 sweeps gives up after LU32_WAIT
+// ticks of the 100 MHz real-time counter (0.5 s; a normal wait is a few
+// microseconds), raising the workgroup's abort flag, so every wave of every
+// workgroup always reaches the end of the kernel; the sweep then reports
+// status 3.
+constexpr unsigned long long LU32_WAIT = 50000000ull;
+
+__device__ __forceinline__ unsigned long long rt_now() { return __builtin_amdgcn_s_memrealtime(); }
 
 __device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_flag) {
-  for (int n = 0; n < LU32_SPIN; ++n) {
+  const unsigned long long t0 = rt_now();
+  for (;;) {
     if (__hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return true;
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+    if (rt_now() - t0 > LU32_WAIT) break;
     __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return false;
+}
+
+// mailbox words are accessed through global (not generic) pointers: flat
+// instructions would also count against the LDS wait counter and stall the
+// LDS hand-offs behind device-scope stores
+typedef __attribute__((address_space(1))) unsigned gu32;
+
+// wait until all four words of a mailbox entry written by another
+// workgroup are filled (device-scope loads: the writer may sit on another
+// XCD)
+__device__ __forceinline__ bool wait_mail(gu32* p, u32x4_t& out, int* abort_flag) {
+  const unsigned long long t0 = rt_now();
+  for (;;) {
+    unsigned a = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned c = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned d = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a != LU32_EMPTY && b != LU32_EMPTY && c != LU32_EMPTY && d != LU32_EMPTY) {
+      out = u32x4_t{a, b, c, d};
+      return true;
+    }
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+    if (rt_now() - t0 > LU32_WAIT) break;
+    __builtin_amdgcn_s_sleep(2);
   }
   __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   return false;
@@ -1419,17 +1475,34 @@ __device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_fla
 
 __device__ __forceinline__ float4 f4_of(u32x4_t v) { return __builtin_bit_cast(float4, v); }
 
-// One FP32 sweep of one instance.  Wave 0 computes (lane l owns RL rows,
-// skewed as in k_lu_sweep); waves 1..NP stream the records of step t into
-// LDS slot t % S and publish ready[slot] = t + 1; the computing wave
-// publishes consumed = t + 1 once slot t is no longer needed.  The only
-// barrier is the initialisation one.
-template <int DIR, int RL, int NV, int S, int NP>
+// DPP wavefront shift whose edge lane (lane 0 for DIR > 0, lane 63 for
+// DIR < 0) receives `edge` instead
+template <int DIR>
+__device__ __forceinline__ float dpp_shift_in(float v, float edge) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v),
+                                                               DIR > 0 ? 0x138 : 0x130, 0xf, 0xf, false));
+}
+
+// One FP32 sweep.  The rows of an instance are cut into W blocks of 64;
+// block w is one workgroup (on its own CU), lane l owning row ks+64w+l and
+// walking along j one column per step, lane l one step behind lane l-1.
+// Inside the workgroup, wave 0 computes and waves 1..NP stream the records
+// of step t into LDS slot t % S (ready[slot] = t+1 once written; the
+// computing wave publishes consumed = t+1 when slot t may be reused).
+// Across workgroups the edge row passes through a mailbox per column in
+// global memory: block w's edge lane stores its result with device-scope
+// atomics, the producer waves of the neighbouring block wait for it and
+// place it in the slot, and the computing wave feeds it to its edge lane as
+// the fill value of the wavefront shift.  Forward, block w waits on block
+// w-1; backward, on block w+1; the dispatch order (blockIdx.x) puts the
+// block waited on first, so every wait is on a workgroup already running.
+template <int DIR, int NV, int S, int NP>
 __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __restrict__ Ms) {
   extern __shared__ __attribute__((aligned(16))) double lds_raw[];
   __shared__ int ready[S];
   __shared__ int consumed, abort_flag, st0;
-  const DevMesh& M = Ms[blockIdx.x];
+  const DevMesh& M = Ms[blockIdx.y];
+  const LUGeom g = lu_geom(M);
   if (threadIdx.x == 0) {
     st0 = *M.status;
     consumed = 0;
@@ -1437,122 +1510,136 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   }
   if (threadIdx.x < S) ready[threadIdx.x] = 0;
   __syncthreads();
-  if (st0) return;
-  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][RL][NV][64]
-  const LUGeom g = lu_geom(M);
-  const int NT = g.T;
-  // wave index made wave-uniform for the compiler (keeps buffer offsets in
-  // SGPRs: no waterfall loops around the loads)
+  const int wb = blockIdx.x;
+  if (st0 || wb >= g.W) return;
+  const int w = DIR > 0 ? wb : g.W - 1 - wb;   // row block of this workgroup
+  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][NV+1][64]; vector NV lane 0: mailbox value
+  const int NT = g.TW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
-  const long long roff = DIR > 0 ? 0 : g.off_b;
+  D2* lu = reinterpret_cast<D2*>(M.lu);
   if (wave > 0) {
     // ---------------------------------------------------------- producers
+    const long long roff = DIR > 0 ? 0 : g.off_b;
     __amdgpu_buffer_rsrc_t rec[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q)
-      rec[q] = uniform_rsrc(reinterpret_cast<D2*>(M.lu) + roff + q * g.nrec, g.nrec * 16);
-    auto fetch = [&](int tt, u32x4_t (&v)[RL][NV]) {
+    for (int q = 0; q < NV; ++q) rec[q] = uniform_rsrc(lu + roff + q * g.nrec, g.nrec * 16);
+    // mailbox written by the neighbour block (none at the grid edge)
+    const bool has_in = DIR > 0 ? w > 0 : w < g.W - 1;
+    gu32* mb_in = has_in ? (gu32*)(lu + (DIR > 0 ? g.off_mbf + (long long)(w - 1) * g.ni
+                                                  : g.off_mbb + (long long)(w + 1) * g.ni))
+                         : nullptr;
+    const int base = w * NT;
+    // the records of step tt and, issued with them, the neighbour's mailbox
+    // entry for the edge lane's column (forward lane 0 at j = js+tt,
+    // backward lane 63 at j = je-tt); written some 64 steps earlier, it is
+    // normally already there when checked
+    auto mail_ptr = [&](int tt) { return mb_in + 4ll * (DIR > 0 ? tt : g.ni - 1 - tt); };
+    auto fetch = [&](int tt, u32x4_t (&v)[NV + 1]) {
 #pragma unroll
-      for (int r = 0; r < RL; ++r)
-#pragma unroll
-        for (int q = 0; q < NV; ++q) v[r][q] = __builtin_amdgcn_raw_buffer_load_b128(rec[q], l * 16, (tt * RL + r) * 1024, 0);
+      for (int q = 0; q < NV; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(rec[q], l * 16, (base + tt) * 1024, 0);
+      v[NV] = u32x4_t{0u, 0u, 0u, 0u};
+      if (has_in && tt < g.ni) {
+        gu32* p = mail_ptr(tt);
+        v[NV] = u32x4_t{__hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                        __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                        __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                        __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+      }
     };
-    u32x4_t va[RL][NV], vb[RL][NV];
+    auto deliver = [&](int tt, u32x4_t (&v)[NV + 1]) -> bool {
+      if (has_in && tt < g.ni &&
+          (v[NV].x == LU32_EMPTY || v[NV].y == LU32_EMPTY || v[NV].z == LU32_EMPTY || v[NV].w == LU32_EMPTY))
+        if (!wait_mail(mail_ptr(tt), v[NV], &abort_flag)) return false;
+      if (!spin_until_ge(&consumed, tt - S, &abort_flag)) return false;
+      u32x4_t* d = ring + (size_t)(tt % S) * (NV + 1) * 64;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) d[q * 64 + l] = v[q];
+      if (l == 0) d[NV * 64] = v[NV];
+      __hip_atomic_store(&ready[tt % S], tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      return true;
+    };
+    u32x4_t va[NV + 1], vb[NV + 1];
     int tt = wave - 1;
     if (tt < NT) fetch(tt, va);
     for (; tt < NT; tt += 2 * NP) {
       const int t2 = tt + NP;
       if (t2 < NT) fetch(t2, vb);
-      if (!spin_until_ge(&consumed, tt - S, &abort_flag)) break;
-      {
-        u32x4_t* d = ring + (size_t)(tt % S) * RL * NV * 64;
-#pragma unroll
-        for (int r = 0; r < RL; ++r)
-#pragma unroll
-          for (int q = 0; q < NV; ++q) d[(r * NV + q) * 64 + l] = va[r][q];
-      }
-      __hip_atomic_store(&ready[tt % S], tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (!deliver(tt, va)) break;
       if (t2 >= NT) break;
       const int t3 = t2 + NP;
       if (t3 < NT) fetch(t3, va);
-      if (!spin_until_ge(&consumed, t2 - S, &abort_flag)) break;
-      {
-        u32x4_t* d = ring + (size_t)(t2 % S) * RL * NV * 64;
-#pragma unroll
-        for (int r = 0; r < RL; ++r)
-#pragma unroll
-          for (int q = 0; q < NV; ++q) d[(r * NV + q) * 64 + l] = vb[r][q];
-      }
-      __hip_atomic_store(&ready[t2 % S], t2 + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (!deliver(t2, vb)) break;
     }
     return;
   }
   // ------------------------------------------------------------- compute
-  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  __amdgpu_buffer_rsrc_t out = uniform_rsrc(reinterpret_cast<D2*>(M.lu) + g.off_b, g.nrec * 16);
-  float sA[RL][4];
-#pragma unroll
-  for (int r = 0; r < RL; ++r) sA[r][0] = sA[r][1] = sA[r][2] = sA[r][3] = 0.f;
-  // k neighbour leaving this lane.  Lanes >= NL never update it, so the
-  // wavefront shift hands zeros to the boundary lane in both directions
-  // (lane 0 forward reads the DPP fill value, lane NL-1 backward reads lane
-  // NL) and the folded halo terms need no special case.  The shift must run
-  // with every lane enabled: a source lane disabled in EXEC reads as the
-  // fill value, so no select or branch may guard it.
+  const int js = M.js, je = M.je;
+  const int rows = g.nk - 64 * w < 64 ? g.nk - 64 * w : 64;
+  __amdgpu_buffer_rsrc_t out = uniform_rsrc(lu + g.off_b, g.nrec * 16);
+  const bool has_out = DIR > 0 ? w < g.W - 1 : w > 0;
+  const bool edge_out = DIR > 0 ? l == 63 : l == 0;
+  gu32* mb_out = (gu32*)(lu + (DIR > 0 ? g.off_mbf : g.off_mbb) + (long long)w * g.ni);
+  const int base = w * NT;
+  float sA[4] = {0.f, 0.f, 0.f, 0.f};
+  // k neighbour leaving this lane.  Inactive lanes pass on what they
+  // receive, and the wavefront shift must run with every lane enabled (a
+  // disabled source lane reads as the fill value): no select or branch may
+  // guard it.
   float kout[4] = {0.f, 0.f, 0.f, 0.f};
-  auto rd = [&](int t, u32x4_t (&v)[RL][NV]) {
-    const u32x4_t* s = ring + (size_t)(t % S) * RL * NV * 64;
+  auto rd = [&](int t, u32x4_t (&v)[NV + 1]) {
+    const u32x4_t* s = ring + (size_t)(t % S) * (NV + 1) * 64;
 #pragma unroll
-    for (int r = 0; r < RL; ++r)
-#pragma unroll
-      for (int q = 0; q < NV; ++q) v[r][q] = s[(r * NV + q) * 64 + l];
+    for (int q = 0; q < NV; ++q) v[q] = s[q * 64 + l];
+    v[NV] = s[NV * 64];
   };
-  // one step with the data of step t in C; reads step t+1 into N first
-  auto step = [&](int t, const u32x4_t (&C)[RL][NV], u32x4_t (&N)[RL][NV]) {
+  auto step = [&](int t, const u32x4_t (&C)[NV + 1], u32x4_t (&N)[NV + 1]) {
     if (t + 1 < NT) rd(t + 1, N);
     const int f2 = t + 2 < NT ? __hip_atomic_load(&ready[(t + 2) % S], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
                               : 0;
-    const int j = DIR > 0 ? js + t - l : je - t + (g.NL - 1 - l);
-    const bool lact = l < g.NL && j >= js && j <= je;
+    const int j = DIR > 0 ? js + t - l : je - t + (63 - l);
+    const bool v = l < rows && j >= js && j <= je;
+    const float4 mail = f4_of(C[NV]);
     float kin[4];
+    kin[0] = dpp_shift_in<DIR>(kout[0], mail.x);
+    kin[1] = dpp_shift_in<DIR>(kout[1], mail.y);
+    kin[2] = dpp_shift_in<DIR>(kout[2], mail.z);
+    kin[3] = dpp_shift_in<DIR>(kout[3], mail.w);
+    const int pa = DIR > 0 ? 2 : 1, pb = pa + 4;
+    const float4 u = f4_of(C[0]);
+    float sn[4];
 #pragma unroll
-    for (int w = 0; w < 4; ++w) kin[w] = dpp_shift<DIR>(kout[w]);
-    const int ostep = DIR > 0 ? NT - 1 - t : t;
-#pragma unroll
-    for (int rr = 0; rr < RL; ++rr) {
-      const int r = DIR > 0 ? rr : RL - 1 - rr;
-      const int k = ks + l * RL + r;
-      const bool v = lact && k <= ke;
-      const int pa = DIR > 0 ? 2 : 1, pb = pa + 4;
-      float4 u = f4_of(C[r][0]);
-      float sn[4];
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        float4 a = f4_of(C[r][pa + w]), b = f4_of(C[r][pb + w]);
-        float uw = w == 0 ? u.x : (w == 1 ? u.y : (w == 2 ? u.z : u.w));
-        float p = __builtin_fmaf(a.x, sA[r][0], uw);
-        p = __builtin_fmaf(a.y, sA[r][1], p);
-        p = __builtin_fmaf(a.z, sA[r][2], p);
-        p = __builtin_fmaf(a.w, sA[r][3], p);
-        p = __builtin_fmaf(b.x, kin[0], p);
-        p = __builtin_fmaf(b.y, kin[1], p);
-        p = __builtin_fmaf(b.z, kin[2], p);
-        sn[w] = __builtin_fmaf(b.w, kin[3], p);
+    for (int q = 0; q < 4; ++q) {
+      float4 a = f4_of(C[pa + q]), b = f4_of(C[pb + q]);
+      float uq = q == 0 ? u.x : (q == 1 ? u.y : (q == 2 ? u.z : u.w));
+      float p = __builtin_fmaf(a.x, sA[0], uq);
+      p = __builtin_fmaf(a.y, sA[1], p);
+      p = __builtin_fmaf(a.z, sA[2], p);
+      p = __builtin_fmaf(a.w, sA[3], p);
+      p = __builtin_fmaf(b.x, kin[0], p);
+      p = __builtin_fmaf(b.y, kin[1], p);
+      p = __builtin_fmaf(b.z, kin[2], p);
+      sn[q] = __builtin_fmaf(b.w, kin[3], p);
+    }
+    if (v) {
+      float o[4] = {sn[0], sn[1], sn[2], sn[3]};
+      if (DIR > 0) {
+        float4 cb = f4_of(C[1]);
+        o[0] += cb.x; o[1] += cb.y; o[2] += cb.z; o[3] += cb.w;
       }
-      if (v) {
-        float o[4] = {sn[0], sn[1], sn[2], sn[3]};
-        if (DIR > 0) {
-          float4 cb = f4_of(C[r][1]);
-          o[0] += cb.x; o[1] += cb.y; o[2] += cb.z; o[3] += cb.w;
-        }
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{o[0], o[1], o[2], o[3]}), out, l * 16,
-                                               (ostep * RL + r) * 1024, 0);
+      const int ostep = DIR > 0 ? NT - 1 - t : t;
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{o[0], o[1], o[2], o[3]}), out, l * 16,
+                                             (base + ostep) * 1024, 0);
+      if (has_out && edge_out) {
+        gu32* p = mb_out + 4ll * (j - js);
 #pragma unroll
-        for (int w = 0; w < 4; ++w) { sA[r][w] = sn[w]; kin[w] = sn[w]; }
+        for (int q = 0; q < 4; ++q)
+          __hip_atomic_store(p + q, __builtin_bit_cast(unsigned, sn[q]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { sA[q] = sn[q]; kin[q] = sn[q]; }
     }
 #pragma unroll
-    for (int w = 0; w < 4; ++w) kout[w] = kin[w];
+    for (int q = 0; q < 4; ++q) kout[q] = kin[q];
     bool okk = true;
     if (t + 2 < NT && f2 < t + 3) okk = spin_until_ge(&ready[(t + 2) % S], t + 3, &abort_flag);
     // slot t+2 is published: keep the next step's reads of it after this point
@@ -1560,7 +1647,7 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
     __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     return okk;
   };
-  u32x4_t C0[RL][NV], C1[RL][NV];
+  u32x4_t C0[NV + 1], C1[NV + 1];
   bool ok = spin_until_ge(&ready[0], 1, &abort_flag);
   if (ok && NT > 1) ok = spin_until_ge(&ready[1 % S], 2, &abort_flag);
   if (ok) rd(0, C0);
@@ -1618,7 +1705,7 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
     double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
     q_roundtrip(q1, q2, q3, q4, M.q[M.nq - 1][i]);
     if (jin && kin) {
-      long long e = lu_entry(M, g, j, k, -1);
+      long long e = sizeof(T) == 8 ? lu_entry(M, g, j, k, -1) : lu_entry32(M, g, j, k, -1);
       double dq[4];
       if constexpr (sizeof(T) == 8) {
         D2 a = *B.at(0, e), b = *B.at(1, e);
@@ -2059,28 +2146,29 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
   return hipSuccess;
 }
 
-// LDS ring slots of the FP32 sweeps: as many slots of RL*NV KiB as fit in
+// LDS ring slots of the FP32 sweeps: as many slots of (NV+1) KiB as fit in
 // 150 KiB
-constexpr int lu32_slots(int rl, int nv) { return (150 * 1024) / (rl * nv * 1024); }
-constexpr int LU32_NP = 7;
+constexpr int lu32_slots(int nv) { return (150 * 1024) / ((nv + 1) * 1024); }
+constexpr int LU32_NP = 3;
 
-template <int RL>
 static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, Marks* mk,
                               hipStream_t st) {
-  constexpr int SF = lu32_slots(RL, LU32_NVF), SB = lu32_slots(RL, LU32_NVB);
-  static_assert(SF >= 3 && SB >= 3, "LDS ring too small");
+  constexpr int SF = lu32_slots(LU32_NVF), SB = lu32_slots(LU32_NVB);
   long long mx = 0, nc = 0;
+  int wmax = 1;
   for (int i = 0; i < nmi; ++i) {
     mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
-    long long c = (long long)(hM[i].je - hM[i].js + 1) * (hM[i].ke - hM[i].ks + 1);
+    int nk = hM[i].ke - hM[i].ks + 1;
+    long long c = (long long)(hM[i].je - hM[i].js + 1) * nk;
     nc = c > nc ? c : nc;
+    wmax = (nk + 63) / 64 > wmax ? (nk + 63) / 64 : wmax;
   }
-  const size_t ldsf = (size_t)SF * RL * LU32_NVF * 1024, ldsb = (size_t)SB * RL * LU32_NVB * 1024;
+  const size_t ldsf = (size_t)SF * (LU32_NVF + 1) * 1024, ldsb = (size_t)SB * (LU32_NVB + 1) * 1024;
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, RL, LU32_NVF, SF, LU32_NP>,
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, LU32_NVF, SF, LU32_NP>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsf);
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, RL, LU32_NVB, SB, LU32_NP>,
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, LU32_NVB, SB, LU32_NP>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
     attr_set = true;
   }
@@ -2088,11 +2176,10 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_PREP, st);
-  hipLaunchKernelGGL((k_lu_sweep32<1, RL, LU32_NVF, SF, LU32_NP>), dim3(nmi), dim3(64 * (1 + LU32_NP)), ldsf, st,
-                     dM);
+  hipLaunchKernelGGL((k_lu_sweep32<1, LU32_NVF, SF, LU32_NP>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), ldsf, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_FORWARD, st);
-  hipLaunchKernelGGL((k_lu_sweep32<-1, RL, LU32_NVB, SB, LU32_NP>), dim3(nmi), dim3(64 * (1 + LU32_NP)), ldsb, st,
+  hipLaunchKernelGGL((k_lu_sweep32<-1, LU32_NVB, SB, LU32_NP>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), ldsb, st,
                      dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_BACKWARD, st);
@@ -2107,12 +2194,7 @@ hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_upd
                      hipStream_t st) {
   const int rl = hM[0].lu_rl;
   if (fp32) {
-    switch (rl) {
-      case 1: return launch_lu32<1>(dM, hM, nmi, fuse_update, mk, st);
-      case 2: return launch_lu32<2>(dM, hM, nmi, fuse_update, mk, st);
-      case 3: return launch_lu32<3>(dM, hM, nmi, fuse_update, mk, st);
-      case 4: return launch_lu32<4>(dM, hM, nmi, fuse_update, mk, st);
-    }
+    return launch_lu32(dM, hM, nmi, fuse_update, mk, st);
   } else {
     // prefetch depth D chosen so the record ring fits the register file
     switch (rl) {
