@@ -66,8 +66,36 @@ def run_case(name, jmax, kmax, ds, m, p, t, deck, nsteps, dumpat):
     print(name, "norm[-1]", norms[-1, 3], "cl", forces[-1, 1])
 
 
+# Driver fixture: the reference program itself (oracle/_ref/overturns2d,
+# src/Turns_overset.f main) on a small SA deck; its fort.71 (norms), fort.11
+# (forces) and fort.8 (restart solution) are what ot2d_run must reproduce.
+PROGRAM = os.path.join(ROOT, "oracle", "_ref", "overturns2d")
+DRIVER_CASE = dict(jmax=41, kmax=21, m=0.02, p=0.4, t=0.12, ds_wall=1e-4, nsteps=12, nrest=5, npnorm=4)
+
+
+def driver_deck():
+    c = DRIVER_CASE
+    g = grid.naca4_cgrid(c["jmax"], c["kmax"], m=c["m"], p=c["p"], t=c["t"], ds_wall=c["ds_wall"])
+    return g, dict(SA, nsteps=c["nsteps"], nrest=c["nrest"], npnorm=c["npnorm"])
+
+
+def run_driver_case():
+    g, deck = driver_deck()
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_case(d, g, **deck)
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([PROGRAM], cwd=d, stdin=fin, check=True, stdout=subprocess.DEVNULL)
+        norms = np.loadtxt(os.path.join(d, "fort.71"), ndmin=2)
+        forces = np.loadtxt(os.path.join(d, "fort.11"), ndmin=2)
+        q8 = open(os.path.join(d, "fort.8"), "rb").read()
+    np.savez_compressed(os.path.join(HERE, "driver_sa_41x21.npz"), norms=norms, forces=forces,
+                        fort8=np.frombuffer(q8, dtype=np.uint8))
+    print("driver_sa_41x21 norms", norms[-1])
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
     for name, spec in CASES.items():
         run_case(name, *spec)
+    run_driver_case()

@@ -72,3 +72,19 @@ def test_ctypes_mirror_matches_header_layout():
     assert abi.MeshDesc.bc.offset == o_bc
     assert abi.NSTAGE == nstage
     assert abi.F_VGV + 1 == nfield
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libot2d.so not built")
+def test_fortran_driver_builds_and_links():
+    """The ISO_C_BINDING module and driver compile with flang and link
+    against libot2d.so (bind(C) names resolve)."""
+    import shutil
+    if not shutil.which("flang"):
+        pytest.skip("flang not available")
+    src = os.path.join(ROOT, "turns2d_urans_git_amd", "fortran")
+    with tempfile.TemporaryDirectory() as d:
+        for f in ("ot2d_capi.f90", "ot2d_run.f90"):
+            subprocess.run(["flang", "-O0", "-c", os.path.join(src, f), "-module-dir", d, "-o",
+                            os.path.join(d, f[:-4] + ".o")], check=True, cwd=d)
+        subprocess.run(["flang", "-o", os.path.join(d, "drv"), os.path.join(d, "ot2d_run.o"),
+                        os.path.join(d, "ot2d_capi.o"), "-L", os.path.dirname(LIB), "-lot2d"], check=True)
