@@ -167,7 +167,7 @@ def main():
     ap.add_argument("--converge", type=int, default=40000,
                     help="run a fresh case up to N steps until rsum <= 1e-10; report solves/hour (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="after the run, copy a buffer of this size with torch (counter scaling)")
+                    help="after the run, copy a device buffer of this size (counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -223,6 +223,14 @@ def main():
         if rc != 0:
             raise SystemExit(f"profiled run stopped: rc={rc}")
         rl, pass_gbs = roofline(stage_ms, w["model"], cells)
+        # HBM bytes per launch of the dominant kernel, from the committed
+        # rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this workload
+        tfile = os.path.join(ROOT, "profiles", f"{args.workload}_traffic.json")
+        if os.path.exists(tfile):
+            t = json.load(open(tfile)).get("kernels", {}).get(rl["kernel"])
+            if t:
+                rl["traffic"] = t["read_bytes"] + t["write_bytes"]
+                rl["traffic_source"] = os.path.relpath(tfile, ROOT)
         out["roofline"] = rl
         out["kernels_ms"] = {k: round(v, 5) for k, v in sorted(stage_ms.items(), key=lambda kv: -kv[1])}
         out["passes_gbs"] = pass_gbs
@@ -282,14 +290,20 @@ def main():
         out["cpu_baseline"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
 
     if args.copy_probe_mib > 0:
-        import torch
-        n = args.copy_probe_mib * (1 << 20) // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        out["copy_probe_bytes"] = 2 * n * 4
+        # a plain device-to-device copy of known size (the runtime's copy
+        # kernel): the reference reading for scaling FETCH_SIZE/WRITE_SIZE
+        nbytes = args.copy_probe_mib << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+            raise SystemExit("copy probe: hipMalloc failed")
+        hip.hipMemset(a, 1, C.c_size_t(nbytes))
+        device_sync(hip)
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):   # hipMemcpyDeviceToDevice
+            raise SystemExit("copy probe: hipMemcpy failed")
+        device_sync(hip)
+        hip.hipFree(a)
+        hip.hipFree(b)
+        out["copy_probe_bytes"] = nbytes
 
     case.close()
     print(json.dumps(out), flush=True)
