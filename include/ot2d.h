@@ -154,7 +154,6 @@ int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32_t* done);
  * cpower, cfx, cfy (src/forces.f:2-107, force2d src/forces.f:111-398). */
 int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]);
 
-/* Error text of the last failing call on this thread. */
 /* Kernel stages of one step, in launch order (ot2d_profile). */
 enum {
   OT2D_ST_BEGIN = 0,     /* step counter / status reset                        */
@@ -189,6 +188,7 @@ enum {
 int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, int32_t* done);
 const char* ot2d_stage_name(int32_t stage);
 
+/* Error text of the last failing call on this thread. */
 const char* ot2d_last_error(void);
 
 #ifdef __cplusplus
