@@ -66,6 +66,33 @@ def run_case(name, jmax, kmax, ds, m, p, t, deck, nsteps, dumpat):
     print(name, "norm[-1]", norms[-1, 3], "cl", forces[-1, 1])
 
 
+# Time-spectral fixtures (pitching airfoil, N instances coupled by the
+# spectral derivative): residual history of every instance and final state.
+TS_CASES = {
+    "ts3_euler_41x21": (41, 21, 1e-3, dict(EULER, fsmach=0.3, alfa=1.0, timespectral=True, nspec=3,
+                                           motiontype=1, rf=0.1, angmax=2.0), 12),
+    "ts4_sa_41x21": (41, 21, 2e-4, dict(SA, fsmach=0.3, alfa=1.0, rey=4.0e6, dt=5.0, timespectral=True,
+                                        nspec=4, motiontype=1, rf=0.2, angmax=3.0), 12),
+}
+
+
+def run_ts_case(name, jmax, kmax, ds, deck, nsteps):
+    g = grid.naca4_cgrid(jmax, kmax, ds_wall=ds)
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_case(d, g, nsteps=nsteps, nrest=10**6, npnorm=1, **deck)
+        out = os.path.join(d, "out")
+        os.makedirs(out)
+        env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps))
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([HARNESS], cwd=d, stdin=fin, env=env, check=True, stdout=subprocess.DEVNULL)
+        norms = np.loadtxt(os.path.join(out, "norms.txt"), comments="#", ndmin=2)
+        qfinal = np.fromfile(os.path.join(out, "qfinal_m1.bin"))
+    deck_arr = {f"deck_{k}": np.asarray(float(v)) for k, v in deck.items()}
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), xv=g.xv, yv=g.yv, jtail1=np.asarray(g.jtail1),
+                        nsteps=np.asarray(nsteps), norms=norms, qfinal=qfinal, **deck_arr)
+    print(name, "norm[-1]", norms[-1, 3])
+
+
 # Driver fixture: the reference program itself (oracle/_ref/overturns2d,
 # src/Turns_overset.f main) on a small SA deck; its fort.71 (norms), fort.11
 # (forces) and fort.8 (restart solution) are what ot2d_run must reproduce.
@@ -98,4 +125,6 @@ if __name__ == "__main__":
         sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
     for name, spec in CASES.items():
         run_case(name, *spec)
+    for name, spec in TS_CASES.items():
+        run_ts_case(name, *spec)
     run_driver_case()

@@ -100,7 +100,7 @@ def load_golden(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
     deck = {k[5:]: float(z[k]) for k in z.files if k.startswith("deck_")}
     ints = {"invisc", "lamin", "irhsy", "ilim", "ilhs", "iconstant_cfl", "iunst", "ntac",
-            "itnmax", "iturb"}
+            "itnmax", "iturb", "timespectral", "nspec", "motiontype"}
     deck = {k: (int(v) if k in ints else v) for k, v in deck.items()}
     g = grid.CGrid(xv=np.asfortranarray(z["xv"]), yv=np.asfortranarray(z["yv"]),
                    jtail1=int(z["jtail1"]), jmax=z["xv"].shape[0], kmax=z["xv"].shape[1])

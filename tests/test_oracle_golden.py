@@ -59,3 +59,18 @@ def test_trajectory_bitexact(name):
         np.testing.assert_array_equal([f["cl"], f["cd"], f["cm"], f["cpower"], f["cfx"]], forces[it, 1:6])
     np.testing.assert_array_equal(case.get_state(0).ravel(order="F"), z["qfinal"])
     np.testing.assert_array_equal(case.get_field(0, abi.F_TURMU).ravel(order="F"), z["turmufinal"])
+
+
+TS_CASES = ["ts3_euler_41x21", "ts4_sa_41x21"]
+
+
+@pytest.mark.parametrize("name", TS_CASES)
+def test_time_spectral_trajectory_bitexact(name):
+    """Time-spectral coupling (src/sources.f:2-27) with pitching instances
+    (src/setup.f:348-393, 727-786): every instance's norms and the final
+    state of all instances, bit for bit."""
+    z, case = H.golden_case(H.oracle_lib(), "or_", name)
+    n, ns = int(z["nsteps"]), int(z["deck_nspec"])
+    hist = np.array([case.step()[1].copy() for _ in range(n)]).reshape(n * ns, 3)
+    np.testing.assert_array_equal(hist, z["norms"][:, 3:6])
+    np.testing.assert_array_equal(case.get_state(0).ravel(order="F"), z["qfinal"])

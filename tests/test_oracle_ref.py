@@ -59,3 +59,35 @@ def test_oracle_matches_reference(name):
         hist.append(nrm[0].copy())
     np.testing.assert_array_equal(np.array(hist), norms_ref[:, 3:6])
     np.testing.assert_array_equal(case.get_state(0).ravel(order="F"), q_ref)
+
+
+# time-spectral pitching decks (src/sources.f:2-27, src/setup.f:348-393,
+# 727-786, 838-873): odd and even instance counts take the csc and cot forms
+# of the spectral derivative
+TS_DECKS = {
+    "ts3_euler": dict(H.EULER, fsmach=0.3, alfa=1.0, timespectral=1, nspec=3, motiontype=1, rf=0.1,
+                      angmax=2.0),
+    "ts4_sa": dict(fsmach=0.3, alfa=1.0, rey=4.0e6, invisc=False, lamin=False, iturb=1, vnuinf=3.0,
+                   irhsy=-3, ilim=1, ilhs=1, iconstant_cfl=1, dt=5.0, epse=0.01, iunst=0, ntac=1,
+                   itnmax=1, timeac=1.0, timespectral=1, nspec=4, motiontype=1, rf=0.2, angmax=3.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TS_DECKS))
+def test_oracle_matches_reference_time_spectral(name):
+    deck = TS_DECKS[name]
+    g = grid.naca4_cgrid(41, 21, ds_wall=1e-3 if deck.get("invisc", 1) else 2e-4)
+    nsteps, ns = 12, deck["nspec"]
+    ref_deck = {k: (bool(v) if k in ("invisc", "lamin", "timespectral") else v) for k, v in deck.items()}
+    q_ref, norms_ref = _ref_run(g, ref_deck, nsteps)
+    case = H.oracle_case(g, {k: int(v) if isinstance(v, bool) else v for k, v in deck.items()})
+    hist = []
+    for _ in range(nsteps):
+        rc, nrm, _ = case.step()
+        assert rc == 0
+        hist.append(nrm.copy())
+    hist = np.array(hist).reshape(nsteps * ns, 3)
+    assert norms_ref.shape[0] == nsteps * ns
+    np.testing.assert_array_equal(norms_ref[:, 2], np.tile(np.arange(1, ns + 1), nsteps))
+    np.testing.assert_array_equal(hist, norms_ref[:, 3:6])
+    np.testing.assert_array_equal(case.get_state(0).ravel(order="F"), q_ref)
