@@ -42,6 +42,7 @@ enum {
   OT2D_OK = 0,
   OT2D_MSTOP = 1,
   OT2D_NORM = 2,
+  OT2D_CONVERGED = 3,   /* batch: case retired (ot2d_batch_retire) */
   OT2D_ERR_ARG = -1,
   OT2D_ERR_HIP = -2,
   OT2D_ERR_UNSUPPORTED = -3,
@@ -153,6 +154,24 @@ int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32_t* done);
 /* Sectional forces of mesh im, instance nsp (0-based): out = cl, cd, cm,
  * cpower, cfx, cfy (src/forces.f:2-107, force2d src/forces.f:111-398). */
 int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]);
+
+/* Batches: ncase independent single-mesh cases advanced together by the
+ * same kernels (one launch per stage for all of them; a case of uneven size
+ * only occupies its own tiles, lines and sweep workgroups).  Each case has
+ * its own flow conditions, grid, step counter and status; the cases must
+ * share the model, the LU-SGS precision and the instance count.  Case i is
+ * mesh i for every per-mesh call (ot2d_set_grid, ot2d_get_state,
+ * ot2d_forces, ...).  ot2d_iterate advances every running case and writes
+ * rsum per step and case to hist; a case that stops (MSTOP, NORM) or is
+ * retired stays so until its grid or state is set again.  No counterpart
+ * in the reference, which runs one case per process. */
+int32_t ot2d_batch_create(int32_t ncase, const ot2d_params* params, const ot2d_mesh_desc* meshes,
+                          ot2d_case** out);
+/* Per case: OT2D_OK (running), OT2D_MSTOP, OT2D_NORM, OT2D_CONVERGED, or a
+ * negative error; istep0 of each case.  Either pointer may be NULL. */
+int32_t ot2d_batch_status(ot2d_case* c, int32_t* status, int32_t* istep0);
+/* Stop advancing case icase (status OT2D_CONVERGED). */
+int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase);
 
 /* Kernel stages of one step, in launch order (ot2d_profile). */
 enum {

@@ -1,0 +1,95 @@
+"""Batches (ot2d_batch_create): independent cases of uneven size and
+different flow conditions advanced by the same launches.  Every case's
+result must be the same in the batch as alone (bit for bit: the kernels
+compute each instance from its own data in a fixed order), including after
+another case of the batch stops or is retired."""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from turns2d_urans_git_amd import abi, grid
+
+pytestmark = pytest.mark.gpu
+
+SA = dict(fsmach=0.5, alfa=2.0, rey=1.0e6, invisc=0, lamin=0, iturb=1, vnuinf=3.0, irhsy=-3, ilim=1,
+          ilhs=1, iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1, timeac=1.0)
+CASES = [
+    # (jmax, kmax, camber m, p, t, ds_wall, deck overrides)
+    (65, 25, 0.0, 0.0, 0.12, 1e-4, dict(fsmach=0.5, alfa=2.0)),
+    (97, 33, 0.02, 0.4, 0.10, 1e-4, dict(fsmach=0.7, alfa=-1.5, rey=3.0e6)),
+    (81, 73, 0.04, 0.3, 0.15, 2e-4, dict(fsmach=0.3, alfa=5.0, rey=5.0e5)),
+]
+
+
+def _inputs(lu_fp32):
+    import turns2d_urans_git_amd as t2d
+    grids, params, meshes = [], [], []
+    for jm, km, m, p, t, ds, over in CASES:
+        g = grid.naca4_cgrid(jm, km, m=m, p=p, t=t, ds_wall=ds)
+        grids.append(g)
+        params.append(t2d.default_params(**dict(SA, **over), lu_fp32=lu_fp32))
+        meshes.append(abi.mesh_desc(jm, km, grid.cgrid_bc(g.jtail1)))
+    return grids, params, meshes
+
+
+@pytest.mark.parametrize("lu_fp32", [0, 1])
+def test_batch_equals_alone(lu_fp32):
+    import turns2d_urans_git_amd as t2d
+    grids, params, meshes = _inputs(lu_fp32)
+    b = t2d.Batch(params, meshes)
+    for i, g in enumerate(grids):
+        b.set_grid(i, g.xv, g.yv)
+    n = 25
+    rc, hist, done = b.iterate(n)
+    assert rc == 0 and done == n
+    st, ist = b.batch_status()
+    assert list(st) == [0, 0, 0] and list(ist) == [n, n, n]
+    for i, g in enumerate(grids):
+        solo = t2d.Case(params[i], [meshes[i]])
+        solo.set_grid(0, g.xv, g.yv)
+        rc_s, h_s, _ = solo.iterate(n)
+        assert rc_s == 0
+        np.testing.assert_array_equal(hist[:, i], h_s[:, 0])
+        np.testing.assert_array_equal(b.get_state(i), solo.get_state(0))
+        np.testing.assert_array_equal(b.get_field(i, abi.F_TURMU), solo.get_field(0, abi.F_TURMU))
+        fb, fs = b.forces(i), solo.forces(0)
+        assert fb == fs
+
+
+def test_batch_first_case_against_oracle():
+    import turns2d_urans_git_amd as t2d
+    grids, params, meshes = _inputs(0)
+    b = t2d.Batch(params, meshes)
+    for i, g in enumerate(grids):
+        b.set_grid(i, g.xv, g.yv)
+    rc, hist, _ = b.iterate(20)
+    jm, km, m, p, t, ds, over = CASES[1]
+    o = H.oracle_case(grids[1], dict(SA, **over))
+    _, h_o, _ = o.iterate(20)
+    np.testing.assert_allclose(hist[:, 1], h_o[:, 0], rtol=1e-10)
+    H.assert_cellwise(b.get_state(1), o.get_state(0), 1e-10, "q of case 1 in the batch")
+
+
+def test_retired_case_is_frozen_and_others_unaffected():
+    import turns2d_urans_git_amd as t2d
+    grids, params, meshes = _inputs(1)
+    b = t2d.Batch(params, meshes)
+    for i, g in enumerate(grids):
+        b.set_grid(i, g.xv, g.yv)
+    b.iterate(10)
+    q1 = b.get_state(1).copy()
+    b.batch_retire(1)
+    rc, hist, done = b.iterate(15)
+    assert rc == 0 and done == 15
+    st, ist = b.batch_status()
+    assert list(st) == [0, abi.CONVERGED, 0] and list(ist) == [25, 10, 25]
+    np.testing.assert_array_equal(b.get_state(1), q1)
+    assert np.all(hist[:, 1] == 0.0)          # no norms recorded for a retired case
+    solo = t2d.Case(params[2], [meshes[2]])
+    solo.set_grid(0, grids[2].xv, grids[2].yv)
+    solo.iterate(25)
+    np.testing.assert_array_equal(b.get_state(2), solo.get_state(0))
+    # setting the grid again restarts the retired case
+    b.set_grid(1, grids[1].xv, grids[1].yv)
+    st, ist = b.batch_status()
+    assert st[1] == 0 and ist[1] == 0

@@ -50,3 +50,11 @@ class Case(abi.CaseAPI):
 
     def __init__(self, params: abi.Params, meshes):
         super().__init__(load_library(), "ot2d_", params, meshes, errfn=last_error)
+
+
+class Batch(abi.CaseAPI):
+    """Independent cases advanced together (ot2d_batch_create): one Params
+    and one mesh description per case."""
+
+    def __init__(self, params_list, meshes):
+        super().__init__(load_library(), "ot2d_", params_list, meshes, errfn=last_error, batch=True)

@@ -14,7 +14,7 @@ import numpy as np
 
 MAXBC = 25
 
-OK, MSTOP, NORM = 0, 1, 2
+OK, MSTOP, NORM, CONVERGED = 0, 1, 2, 3
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4, -5
 
 (F_X, F_Y, F_XX, F_XY, F_YX, F_YY, F_UG, F_VG, F_TSCALE, F_BT, F_TURMU, F_S,
@@ -103,10 +103,13 @@ def bind(lib: C.CDLL, prefix: str):
         "iterate": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_int32)]),
         "forces": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
     }
-    # instrumentation present in libot2d only
+    # present in libot2d only (no counterpart in the reference)
     optional = {
         "profile": (C.c_int32, [C.c_void_p, C.c_int32, _DP, P(C.c_int32)]),
         "stage_name": (C.c_char_p, [C.c_int32]),
+        "batch_create": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, P(C.c_void_p)]),
+        "batch_status": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+        "batch_retire": (C.c_int32, [C.c_void_p, C.c_int32]),
     }
     fns = {}
     for name, (res, args) in list(sig.items()) + list(optional.items()):
@@ -131,25 +134,39 @@ class CaseError(RuntimeError):
 class CaseAPI:
     """One case (``nmesh`` meshes x ``nspec`` instances) behind a C-ABI library."""
 
-    def __init__(self, lib: C.CDLL, prefix: str, params: Params, meshes: Sequence[MeshDesc],
-                 errfn=None):
+    def __init__(self, lib: C.CDLL, prefix: str, params, meshes: Sequence[MeshDesc],
+                 errfn=None, batch: bool = False):
+        """``params`` is one Params (a case of ``len(meshes)`` meshes) or, with
+        ``batch=True``, one Params per mesh (independent cases, ot2d_batch_create)."""
         self._fns = bind(lib, prefix)
         self._errfn = errfn
-        self.params = params
         self.meshes = list(meshes)
         arr = (MeshDesc * len(meshes))(*meshes)
         self._arr = arr
         h = C.c_void_p()
-        self._check(self._fns["case_create"](C.byref(params), len(meshes), arr, C.byref(h)),
-                    "case_create")
+        if batch:
+            plist = list(params)
+            assert len(plist) == len(meshes)
+            parr = (Params * len(plist))(*plist)
+            self._parr = parr
+            self._check(self._fns["batch_create"](len(plist), C.cast(parr, C.c_void_p),
+                                                  C.cast(arr, C.c_void_p), C.byref(h)), "batch_create")
+            self.params = plist[0]
+            per_mesh = plist
+        else:
+            self._check(self._fns["case_create"](C.byref(params), len(meshes), arr, C.byref(h)),
+                        "case_create")
+            self.params = params
+            per_mesh = [params] * len(meshes)
         self.h = h
+        self.batch = batch
         self.dims = []
-        for md in meshes:
+        for md, pm in zip(meshes, per_mesh):
             jd, kd, nq, nv = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
-            self._fns["sizes"](C.byref(params), C.byref(md), C.byref(jd), C.byref(kd),
+            self._fns["sizes"](C.byref(pm), C.byref(md), C.byref(jd), C.byref(kd),
                                C.byref(nq), C.byref(nv))
             self.dims.append((jd.value, kd.value, nq.value, nv.value, md.jmax, md.kmax))
-        self.nspec = params.nspec if params.timespectral else 1
+        self.nspec = self.params.nspec if self.params.timespectral else 1
 
     # -------------------------------------------------------------- utils
     def _check(self, rc, where):
@@ -274,6 +291,17 @@ class CaseAPI:
         rc = self._check(self._fns["profile"](self.h, int(nsteps), ms, C.byref(done)), "profile")
         names = [self._fns["stage_name"](i).decode() for i in range(NSTAGE)]
         return rc, {n: float(v) for n, v in zip(names, ms) if v > 0.0}, done.value
+
+    def batch_status(self):
+        """(status per case, istep0 per case) of a batch."""
+        n = len(self.meshes)
+        st = np.zeros(n, dtype=np.int32)
+        ist = np.zeros(n, dtype=np.int32)
+        self._check(self._fns["batch_status"](self.h, st.ctypes.data, ist.ctypes.data), "batch_status")
+        return st, ist
+
+    def batch_retire(self, icase: int):
+        return self._check(self._fns["batch_retire"](self.h, int(icase)), "batch_retire")
 
     def forces(self, im: int = 0, nsp: int = 0):
         out = np.zeros(6)

@@ -61,6 +61,8 @@ struct ot2d_case {
   int nmesh = 0, nspec = 1, nq = 5, nv = 4, lamin = 1, iturb = -1;
   int device = 0;
   int lu_rl = 1;             // rows per lane of the LU-SGS sweeps
+  int ncase = 1;             // independent cases (batch: one per mesh; otherwise 1)
+  std::vector<int> mesh_case;  // case of each mesh
   hipStream_t stream = nullptr;
   std::vector<MeshBuf> m;
   std::vector<DevMesh> hM;   // nmesh*nspec records (mesh-major)
@@ -262,10 +264,7 @@ static hipError_t dalloc(T** p, size_t n) {
   return e;
 }
 
-extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const ot2d_mesh_desc* md,
-                                    ot2d_case** out) {
-  if (!p || !md || !out || nmesh < 1) return fail(OT2D_ERR_ARG, "bad arguments to ot2d_case_create");
-  *out = nullptr;
+static int32_t check_params(const ot2d_params* p) {
   if (p->nhalo != 1) return fail(OT2D_ERR_UNSUPPORTED, "nhalo=%d: only one halo ring is supported", p->nhalo);
   if (p->ilhs != 1) return fail(OT2D_ERR_UNSUPPORTED, "ilhs=%d: only LU-SGS (ilhs=1) is supported", p->ilhs);
   if (p->iprecon) return fail(OT2D_ERR_UNSUPPORTED, "low-Mach preconditioning is not supported");
@@ -275,7 +274,29 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
   if (p->itnmax != 1) return fail(OT2D_ERR_UNSUPPORTED, "itnmax=%d: Newton sub-iterations not supported", p->itnmax);
   if (p->ntac != 1) return fail(OT2D_ERR_UNSUPPORTED, "ntac=%d not supported", p->ntac);
   if (p->timespectral && (p->nspec < 1 || p->nspec > 64)) return fail(OT2D_ERR_ARG, "nspec=%d", p->nspec);
+  return OT2D_OK;
+}
+
+// cases of one batch must share what selects kernels and workspace layout
+static bool same_structure(const ot2d_params* a, const ot2d_params* b) {
+  return a->invisc == b->invisc && a->lamin == b->lamin && a->iturb == b->iturb && a->itrans == b->itrans &&
+         a->timespectral == b->timespectral && (a->timespectral ? a->nspec == b->nspec : true) &&
+         a->lu_fp32 == b->lu_fp32 && a->half == b->half;
+}
+
+// ncase parameter sets; mesh im belongs to case mesh_case[im]
+static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, const ot2d_mesh_desc* md,
+                           const int32_t* mesh_case, ot2d_case** out) {
+  *out = nullptr;
+  for (int i = 0; i < ncase; ++i) {
+    if (int32_t rc = check_params(ps + i)) return rc;
+    if (!same_structure(ps, ps + i))
+      return fail(OT2D_ERR_ARG, "case %d: model, precision and instance count must match case 0", i);
+  }
+  const ot2d_params* p = ps;
   ot2d_case* c = new ot2d_case();
+  c->ncase = ncase;
+  c->mesh_case.assign(mesh_case, mesh_case + nmesh);
   c->p = *p;
   derive_counts(p, &c->nq, &c->nv, &c->lamin, &c->iturb);
   if (!c->lamin && (c->iturb != 1 || p->itrans != 0)) {
@@ -296,12 +317,12 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
   // per-case bookkeeping
 #define A(ptr, n) do { if ((e = dalloc(&(ptr), (n))) != hipSuccess) { free_case(c); \
     return fail(OT2D_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e)); } } while (0)
-  A(c->d_status, 1); A(c->d_istep0, 1); A(c->d_stepc, 1);
-  A(c->d_counter, nmi); A(c->d_norms, 6 * nmi); A(c->d_tspec, ns); A(c->d_Ds, ns * ns);
+  A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase);
+  A(c->d_counter, nmi); A(c->d_norms, 6 * nmi); A(c->d_tspec, (size_t)ns * ncase); A(c->d_Ds, ns * ns);
   A(c->d_forces, 16);
   // spectral coefficients, src/setup.f:838-873
   {
-    std::vector<double> Ds(ns * ns, 0.0), tspec(ns, 0.0);
+    std::vector<double> Ds(ns * ns, 0.0), tspec((size_t)ns * ncase, 0.0);
     double pi = 4. * atan(1.);
     for (int i = 1; i <= ns; ++i)
       for (int j = 1; j <= ns; ++j) {
@@ -312,9 +333,11 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
         }
         Ds[(i - 1) + (size_t)(j - 1) * ns] = v;
       }
-    for (int n = 0; n < ns; ++n) tspec[n] = (p->rf != 0.0) ? 2 * pi * n / ns / fabs(p->rf) : 0.0;  // src/init.f:71-78
+    for (int ic = 0; ic < ncase; ++ic)  // src/init.f:71-78
+      for (int n = 0; n < ns; ++n)
+        tspec[(size_t)ic * ns + n] = (ps[ic].rf != 0.0) ? 2 * pi * n / ns / fabs(ps[ic].rf) : 0.0;
     (void)hipMemcpy(c->d_Ds, Ds.data(), sizeof(double) * ns * ns, hipMemcpyHostToDevice);
-    (void)hipMemcpy(c->d_tspec, tspec.data(), sizeof(double) * ns, hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->d_tspec, tspec.data(), sizeof(double) * ns * ncase, hipMemcpyHostToDevice);
   }
   // rows per lane of the LU-SGS sweeps: one wavefront covers every row
   c->lu_rl = 1;
@@ -415,15 +438,17 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
         M.sa_sj = b + 4 * mb.n2; M.sa_aj = b + 5 * mb.n2; M.sa_cj = b + 6 * mb.n2;
         M.sa_ak = b + 7 * mb.n2; M.sa_ck = b + 8 * mb.n2; M.sa_bb = b + 9 * mb.n2;
       }
-      M.status = c->d_status;
-      M.istep0 = c->d_istep0;
-      M.stepc = c->d_stepc;
+      const int ic = mesh_case[im];
+      const ot2d_params* pc = ps + ic;
+      M.status = c->d_status + ic;
+      M.istep0 = c->d_istep0 + ic;
+      M.stepc = c->d_stepc + ic;
       M.counter = c->d_counter + mi;
       M.norms = c->d_norms + 6 * mi;
-      M.tspec = c->d_tspec + nsp;
+      M.tspec = c->d_tspec + (size_t)ic * ns + nsp;
       M.Ds = c->d_Ds;
-      M.c = c->c;
-      M.c.rotf = (p->rf != 0.0) ? md[im].irot_dir * p->rf : 0.0;  // src/setup.f:96-101
+      M.c = make_const(pc, c->iturb, 4);
+      M.c.rotf = (pc->rf != 0.0) ? md[im].irot_dir * pc->rf : 0.0;  // src/setup.f:96-101
       M.c.xac = md[im].xcen; M.c.yac = md[im].ycen;
       int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
       M.res_block0 = pofs;
@@ -448,6 +473,21 @@ extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const o
   (void)hipMemcpy(c->dT, c->hT.data(), sizeof(ResTile) * c->hT.size(), hipMemcpyHostToDevice);
   *out = c;
   return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_case_create(const ot2d_params* p, int32_t nmesh, const ot2d_mesh_desc* md,
+                                    ot2d_case** out) {
+  if (!p || !md || !out || nmesh < 1) return fail(OT2D_ERR_ARG, "bad arguments to ot2d_case_create");
+  std::vector<int32_t> mc(nmesh, 0);
+  return create_impl(p, 1, nmesh, md, mc.data(), out);
+}
+
+extern "C" int32_t ot2d_batch_create(int32_t ncase, const ot2d_params* params, const ot2d_mesh_desc* meshes,
+                                     ot2d_case** out) {
+  if (!params || !meshes || !out || ncase < 1) return fail(OT2D_ERR_ARG, "bad arguments to ot2d_batch_create");
+  std::vector<int32_t> mc(ncase);
+  for (int i = 0; i < ncase; ++i) mc[i] = i;
+  return create_impl(params, ncase, ncase, meshes, mc.data(), out);
 }
 
 extern "C" int32_t ot2d_case_destroy(ot2d_case* c) {
@@ -480,9 +520,10 @@ extern "C" int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, con
   if (mb.sa) HIPCHK(launch_sa_dist(c->dM + (size_t)im * c->nspec, c->hM.data() + (size_t)im * c->nspec,
                                     c->nspec, c->stream));
   mb.has_grid = true;
-  c->istep0 = 0;
-  HIPCHK(hipMemsetAsync(c->d_istep0, 0, sizeof(int), c->stream));
-  HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  const int ic = c->mesh_case[im];
+  if (ic == 0) c->istep0 = 0;
+  HIPCHK(hipMemsetAsync(c->d_istep0 + ic, 0, sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_status + ic, 0, sizeof(int), c->stream));
   return sync(c);
 }
 
@@ -490,7 +531,7 @@ extern "C" int32_t ot2d_set_state(ot2d_case* c, int32_t im, const double* q) {
   if (int32_t rc = check_im(c, im)) return rc;
   MeshBuf& mb = c->m[im];
   HIPCHK(hipMemcpyAsync(mb.q, q, sizeof(double) * mb.n2 * c->nspec * c->nq, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_status + c->mesh_case[im], 0, sizeof(int), c->stream));
   return sync(c);
 }
 
@@ -574,8 +615,11 @@ extern "C" int32_t ot2d_get_istep0(ot2d_case* c, int32_t* istep0) {
 
 static int nmi(ot2d_case* c) { return c->nmesh * c->nspec; }
 
+// single-case calls start from a clean status; a batch keeps the status of
+// each case (stopped and retired cases stay so until their grid or state is
+// set again)
 static int32_t clear_status(ot2d_case* c) {
-  HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  if (c->ncase == 1) HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
   return OT2D_OK;
 }
 
@@ -649,7 +693,7 @@ extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
 
 // one step (src/step.f:2-218) + monitor, enqueued without synchronisation
 static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
-  HIPCHK(launch_step_begin(c->d_istep0, c->d_status, c->d_stepc, 1, c->stream));
+  HIPCHK(launch_step_begin(c->d_istep0, c->d_status, c->d_stepc, c->ncase, c->stream));
   mark(mk, OT2D_ST_BEGIN, c->stream);
   HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream));
   mark(mk, OT2D_ST_TIME_STEP, c->stream);
@@ -678,9 +722,30 @@ extern "C" int32_t ot2d_turbulence(ot2d_case* c) {
   return sync(c);
 }
 
+// status of a single case; for a batch, 3 if any case's sweep pipeline
+// stopped, else 0 (per-case states through ot2d_batch_status)
 static int32_t read_status(ot2d_case* c, int* st) {
-  HIPCHK(hipMemcpyAsync(st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  return sync(c);
+  std::vector<int> all(c->ncase);
+  HIPCHK(hipMemcpyAsync(all.data(), c->d_status, sizeof(int) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  if (c->ncase == 1) {
+    *st = all[0];
+  } else {
+    *st = 0;
+    for (int v : all)
+      if (v == 3) *st = 3;
+  }
+  return OT2D_OK;
+}
+
+// steps done by the furthest case in the current call
+static int32_t read_steps(ot2d_case* c, int* steps) {
+  std::vector<int> all(c->ncase);
+  HIPCHK(hipMemcpyAsync(all.data(), c->d_stepc, sizeof(int) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  *steps = 0;
+  for (int v : all) *steps = v > *steps ? v : *steps;
+  return OT2D_OK;
 }
 
 extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
@@ -688,7 +753,7 @@ extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
   if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
   if (int32_t rc = enqueue_step(c)) return rc;
   int st = 0;
   if (int32_t rc = read_status(c, &st)) return rc;
@@ -701,7 +766,8 @@ extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
     }
   }
   c->istep0 += 1;
-  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (spin limit)");
+  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (wait limit)");
+  if (st == 4) return OT2D_CONVERGED;
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;
@@ -749,16 +815,17 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
     HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * per, hipMemcpyHostToDevice, c->stream));
   }
   if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
   if (int32_t rc = ensure_graph(c)) return rc;
   for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
   int st = 0, steps = 0;
-  HIPCHK(hipMemcpyAsync(&steps, c->d_stepc, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = read_steps(c, &steps)) return rc;
   if (int32_t rc = read_status(c, &st)) return rc;
   if (hist && steps > 0) HIPCHK(hipMemcpy(hist, c->d_hist, sizeof(double) * (size_t)steps * per, hipMemcpyDeviceToHost));
   c->istep0 += steps;
   if (done) *done = steps;
-  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (spin limit)");
+  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (wait limit)");
+  if (st == 4) return OT2D_CONVERGED;
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;
@@ -803,7 +870,7 @@ extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, 
     rc = fail(OT2D_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
     --created;
   } else if ((rc = clear_status(c)) == OT2D_OK) {
-    e = hipMemsetAsync(c->d_stepc, 0, sizeof(int), c->stream);
+    e = hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream);
     if (e != hipSuccess) rc = fail(OT2D_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
     for (int it = 0; it < nsteps && rc == OT2D_OK; ++it) {
       mk.n = 0;
@@ -828,11 +895,7 @@ extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, 
         break;
       }
     }
-    if (rc == OT2D_OK) {
-      e = hipMemcpyAsync(&steps, c->d_stepc, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-      if (e != hipSuccess) rc = fail(OT2D_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
-      else rc = read_status(c, &st);
-    }
+    if (rc == OT2D_OK && (rc = read_steps(c, &steps)) == OT2D_OK) rc = read_status(c, &st);
   }
   for (int i = 0; i < created; ++i) (void)hipEventDestroy(mk.ev[i]);
   (void)hipEventDestroy(ev0);
@@ -841,8 +904,29 @@ extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, 
   if (done) *done = steps;
   if (steps > 0)
     for (int i = 0; i < OT2D_NSTAGE; ++i) stage_ms[i] /= steps;
-  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (spin limit)");
+  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (wait limit)");
+  if (st == 4) return OT2D_CONVERGED;
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_batch_status(ot2d_case* c, int32_t* status, int32_t* istep0) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  std::vector<int> st(c->ncase), is(c->ncase);
+  HIPCHK(hipMemcpyAsync(st.data(), c->d_status, sizeof(int) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(is.data(), c->d_istep0, sizeof(int) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  for (int i = 0; i < c->ncase; ++i) {
+    if (status) status[i] = st[i] == 4 ? OT2D_CONVERGED : (st[i] == 3 ? OT2D_ERR_HIP : st[i]);
+    if (istep0) istep0[i] = is[i];
+  }
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase) {
+  if (!c || icase < 0 || icase >= c->ncase) return fail(OT2D_ERR_ARG, "bad case index %d", icase);
+  static const int retired = 4;
+  HIPCHK(hipMemcpyAsync(c->d_status + icase, &retired, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  return sync(c);
 }
