@@ -80,6 +80,98 @@ STAGE_PASS = {
 }
 
 
+# The 64-case mix: random NACA 4-digit airfoils and flow conditions drawn
+# from one seed (m ~ U[0,0.06], p ~ U[0.2,0.6], t ~ U[0.08,0.18], jmax ~
+# U{257..769} odd, kmax ~ U{65..193}, wall spacing 1e-6, M ~ U[0.3,0.8],
+# alpha ~ U[-4,8] deg, Re log-uniform on [1e6, 2e7]), SA, FP64 residual,
+# FP32 LU-SGS.
+BATCH_SEED = 20260101
+
+
+def batch_cases(n=64, seed=BATCH_SEED):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        jmax = int(rng.integers(128, 385)) * 2 + 1          # odd in [257, 769]
+        kmax = int(rng.integers(65, 194))
+        c = dict(m=float(rng.uniform(0.0, 0.06)), p=float(rng.uniform(0.2, 0.6)), t=float(rng.uniform(0.08, 0.18)),
+                 jmax=jmax, kmax=kmax, fsmach=float(rng.uniform(0.3, 0.8)), alfa=float(rng.uniform(-4.0, 8.0)),
+                 rey=float(10.0 ** rng.uniform(6.0, np.log10(2.0e7))))
+        out.append(c)
+    return out
+
+
+def run_batch(args, hip):
+    import turns2d_urans_git_amd as t2d
+    cases = batch_cases(args.batch_size)
+    params, meshes, grids = [], [], []
+    for c in cases:
+        g = grid.naca4_cgrid(c["jmax"], c["kmax"], m=c["m"], p=c["p"], t=c["t"], ds_wall=1e-6,
+                             wake_aspect=1000.0, te_cluster=False)
+        grids.append(g)
+        deck = dict(_SA, fsmach=c["fsmach"], alfa=c["alfa"], rey=c["rey"])
+        params.append(t2d.default_params(**deck, lu_fp32=1))
+        meshes.append(abi.mesh_desc(g.jmax, g.kmax, grid.cgrid_bc(g.jtail1)))
+    cells = np.array([(g.jmax - 1) * (g.kmax - 1) for g in grids], dtype=np.float64)
+    print(f"[bench] batch of {len(cases)} cases, {cells.sum():.0f} interior cells", file=sys.stderr, flush=True)
+    b = t2d.Batch(params, meshes)
+    for i, g in enumerate(grids):
+        b.set_grid(i, g.xv, g.yv)
+    b.iterate(args.warmup)
+    n = len(cases)
+    conv_step = np.full(n, -1)
+    steps_run = np.full(n, args.warmup)
+    done_work = 0.0
+    chunk = args.batch_chunk
+    device_sync(hip)
+    t0 = time.perf_counter()
+    total = 0
+    while total < args.steps:
+        k = min(chunk, args.steps - total)
+        rc, hist, done = b.iterate(k)
+        st, ist = b.batch_status()
+        for i in range(n):
+            h = hist[:, i]
+            ran = int(np.count_nonzero(h))
+            done_work += cells[i] * ran
+            steps_run[i] += ran
+            if conv_step[i] < 0:
+                hit = np.nonzero((h > 0) & (h <= 1e-10))[0]
+                if hit.size:
+                    conv_step[i] = steps_run[i] - ran + int(hit[0]) + 1
+                    b.batch_retire(i)
+        total += k
+        st, ist = b.batch_status()
+        print(f"[bench] batch: {total} steps, {int((conv_step > 0).sum())} converged, "
+              f"{int(((st == 1) | (st == 2)).sum())} stopped, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        if not np.any(st == 0):
+            break
+    device_sync(hip)
+    secs = time.perf_counter() - t0
+    st, ist = b.batch_status()
+    nconv = int((conv_step > 0).sum())
+    out = {
+        "metric": "cell-iter/s",
+        "value": done_work / secs,
+        "unit": "cell-iter/s",
+        "n_gpus": 1, "steps": total, "warmup": args.warmup, "ms_per_step": 1e3 * secs / max(total, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32-lu",
+        "data": "synthetic (64 random NACA 4-digit C-grids and flow conditions, seed %d)" % BATCH_SEED,
+        "config": {"workload": "batch64_sa", "cases": n, "interior_cells": int(cells.sum()),
+                   "jmax_range": [int(min(g.jmax for g in grids)), int(max(g.jmax for g in grids))],
+                   "kmax_range": [int(min(g.kmax for g in grids)), int(max(g.kmax for g in grids))],
+                   "lu_sgs": "fp32", "residual": "fp64"},
+        "converge": {"criterion": "rsum <= 1e-10", "converged": nconv, "stopped": int(((st == 1) | (st == 2)).sum()),
+                     "running": int((st == 0).sum()), "seconds": round(secs, 2),
+                     "solves_per_hr": 3600.0 * nconv / secs,
+                     "steps_to_converge_median": (float(np.median(conv_step[conv_step > 0])) if nconv else None),
+                     "steps_to_converge_max": (int(conv_step.max()) if nconv else None)},
+    }
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 def _hip():
     return C.CDLL("libamdhip64.so")
 
@@ -159,7 +251,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="sa_513x129", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="sa_513x129", choices=sorted(WORKLOADS) + ["batch64_sa"])
+    ap.add_argument("--batch-size", type=int, default=64)
+    ap.add_argument("--batch-chunk", type=int, default=1000)
     ap.add_argument("--profile-steps", type=int, default=100,
                     help="eager steps timed per kernel with HIP events (0: skip)")
     ap.add_argument("--cpu-steps", type=str, default="20,420",
@@ -175,6 +269,10 @@ def main():
               "this workload fits one MI355X; see the sharding notes in the design document)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch64_sa":
+        run_batch(args, _hip())
+        return
 
     w = WORKLOADS[args.workload]
     g = make_grid(w)

@@ -93,3 +93,31 @@ def test_retired_case_is_frozen_and_others_unaffected():
     b.set_grid(1, grids[1].xv, grids[1].yv)
     st, ist = b.batch_status()
     assert st[1] == 0 and ist[1] == 0
+
+
+def test_many_line_batch_equals_alone():
+    """A batch with more than 4096 grid lines runs the SA line solves one lane
+    per line (tiled through LDS along j, row-contiguous along k); each case
+    must still match its solo run (one wave per line) bit for bit."""
+    import turns2d_urans_git_amd as t2d
+    rng = np.random.default_rng(7)
+    grids, params, meshes = [], [], []
+    for _ in range(24):
+        g = grid.naca4_cgrid(129, 49, m=float(rng.uniform(0, 0.05)), p=0.4, t=float(rng.uniform(0.09, 0.15)),
+                             ds_wall=1e-4)
+        grids.append(g)
+        params.append(t2d.default_params(**dict(SA, fsmach=float(rng.uniform(0.3, 0.7)),
+                                                alfa=float(rng.uniform(-2, 5))), lu_fp32=1))
+        meshes.append(abi.mesh_desc(129, 49, grid.cgrid_bc(g.jtail1)))
+    b = t2d.Batch(params, meshes)
+    for i, g in enumerate(grids):
+        b.set_grid(i, g.xv, g.yv)
+    rc, hist, done = b.iterate(15)
+    assert rc == 0 and done == 15
+    for i in (0, 11, 23):
+        solo = t2d.Case(params[i], [meshes[i]])
+        solo.set_grid(0, grids[i].xv, grids[i].yv)
+        _, h_s, _ = solo.iterate(15)
+        np.testing.assert_array_equal(hist[:, i], h_s[:, 0])
+        np.testing.assert_array_equal(b.get_state(i), solo.get_state(0))
+        np.testing.assert_array_equal(b.get_field(i, abi.F_TURMU), solo.get_field(0, abi.F_TURMU))

@@ -74,7 +74,6 @@ struct ot2d_case {
   int* d_istep0 = nullptr;
   int* d_stepc = nullptr;
   double* d_partials = nullptr;
-  unsigned int* d_counter = nullptr;
   double* d_norms = nullptr;
   double* d_tspec = nullptr;
   double* d_Ds = nullptr;
@@ -251,7 +250,7 @@ static void free_case(ot2d_case* c) {
     dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa);
   }
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
-  dfree(c->d_partials); dfree(c->d_counter); dfree(c->d_norms); dfree(c->d_tspec);
+  dfree(c->d_partials); dfree(c->d_norms); dfree(c->d_tspec);
   dfree(c->d_Ds); dfree(c->d_hist); dfree(c->d_forces);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -318,7 +317,7 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
 #define A(ptr, n) do { if ((e = dalloc(&(ptr), (n))) != hipSuccess) { free_case(c); \
     return fail(OT2D_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e)); } } while (0)
   A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase);
-  A(c->d_counter, nmi); A(c->d_norms, 6 * nmi); A(c->d_tspec, (size_t)ns * ncase); A(c->d_Ds, ns * ns);
+  A(c->d_norms, 6 * nmi); A(c->d_tspec, (size_t)ns * ncase); A(c->d_Ds, ns * ns);
   A(c->d_forces, 16);
   // spectral coefficients, src/setup.f:838-873
   {
@@ -443,7 +442,6 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
       M.status = c->d_status + ic;
       M.istep0 = c->d_istep0 + ic;
       M.stepc = c->d_stepc + ic;
-      M.counter = c->d_counter + mi;
       M.norms = c->d_norms + 6 * mi;
       M.tspec = c->d_tspec + (size_t)ic * ns + nsp;
       M.Ds = c->d_Ds;
@@ -652,8 +650,9 @@ static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks*
     HIPCHK(launch_tsrhs(c->dM, c->hM.data(), nmi(c), c->stream));
     mark(mk, OT2D_ST_TSRHS, c->stream);
   }
-  if (norms && later) {
-    HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
+  if (norms && later) HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
+  if (norms) {
+    HIPCHK(launch_norms_final(c->dM, nmi(c), c->stream));
     mark(mk, OT2D_ST_NORMS, c->stream);
   }
   return OT2D_OK;

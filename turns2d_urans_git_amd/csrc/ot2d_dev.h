@@ -62,8 +62,7 @@ struct DevMesh {
   // per-case bookkeeping (shared by all instances of a case)
   int* status;               // 0 ok, 1 mstop, 2 norm
   int* istep0;               // device copy of istep0
-  double* partials;          // norm partials (res_nblocks * 4 doubles)
-  unsigned int* counter;     // last-block counter for the norm reduction
+  double* partials;          // norm partials of this instance's tiles (res_nblocks * 4 doubles)
   double* norms;             // rsum, resrho, resmax, jmax, kmax, nmax
   int* stepc;                // steps started in the current ot2d_iterate call
   double* hist;              // residual history rsum[step][slot] (may be null)
@@ -118,6 +117,7 @@ hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms,
                            hipStream_t st);
 hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st);
+hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st);
 hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st);
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,

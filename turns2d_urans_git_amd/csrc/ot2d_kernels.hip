@@ -603,14 +603,14 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
   if (do_norms) norm_tile(M, T, j, k, valid, sv, &red[0][0]);
 }
 
-// residual norms of step (src/step.f:53-115) for the cells of one tile:
-// block partial sums, then the last tile of the instance reduces the
-// partials in a fixed order and raises the "norm" status
+// residual norms of step (src/step.f:53-115), first stage: the partial sums
+// (sum of squares, rho part, max |s| with its first (k,n,j) position) of the
+// cells of one tile, reduced in a fixed order and stored per tile
 __device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool valid, const double* s,
                           double* redbuf) {
   double(*red)[RES_THREADS] = reinterpret_cast<double(*)[RES_THREADS]>(redbuf);
   const int tid = threadIdx.x;
-  const int jd = M.jd, kd = M.kd;
+  const int jd = M.jd;
   double rs = 0.0, rr1 = 0.0, rmax = 0.0, key = 0.0;
   if (valid) {
     long long i = cell(M, j, k);
@@ -636,22 +636,24 @@ __device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool
     }
     __syncthreads();
   }
-  __shared__ bool last;
   if (tid == 0) {
     double* P = M.partials + 4ll * T.blk;
     P[0] = red[0][0]; P[1] = red[1][0]; P[2] = red[2][0]; P[3] = red[3][0];
-    __threadfence();
-    unsigned int done = atomicAdd(M.counter, 1u);
-    last = (done == (unsigned)M.res_nblocks - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  // last tile of this instance: reduce the partials (fixed order: strided
-  // per-thread sums, then the same LDS tree as above)
-  __threadfence();
+}
+
+// second stage, one block per instance: the tile partials in a fixed order
+// (strided per-thread sums, then an LDS tree), the norms, the history slot
+// and the "norm" status (src/step.f:104-110)
+__global__ __launch_bounds__(RES_THREADS) void k_norms_final(const DevMesh* __restrict__ Ms) {
+  __shared__ double red[4][RES_THREADS];
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
+  const int tid = threadIdx.x;
+  const int jd = M.jd, kd = M.kd;
   {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 1e300;
-    const volatile double* P = M.partials;
+    const double* P = M.partials;
     for (int b = tid; b < M.res_nblocks; b += RES_THREADS) {
       a0 = a0 + P[4 * b + 0];
       a1 = a1 + P[4 * b + 1];
@@ -690,7 +692,6 @@ __device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool
       int sc = *M.stepc;
       if (sc >= 1 && sc <= M.hist_cap) M.hist[(long long)(sc - 1) * M.hist_stride + M.hist_slot] = rsum;
     }
-    *M.counter = 0u;
     if (rsum > 1000.0) atomicCAS(M.status, 0, 2);
   }
 }
@@ -979,8 +980,11 @@ template <> struct LUV<double> { static constexpr int N = 10; };
 struct LUGeom {
   int ni, nk, RL, NL, T;           // FP64 sweep: NL lanes, T steps
   int W, TW;                       // FP32 sweep: W workgroups of 64 rows, TW steps each
-  long long nrec;                  // entries of F and of B (capacity for both layouts)
-  long long off_b, off_hcf, off_hcb, off_hrf, off_hrb, off_mbf, off_mbb, total;  // in vectors
+  long long nrec;                  // FP64: entries of F and of B
+  long long off_b, off_hcf, off_hcb, off_hrf, off_hrb;   // FP64 arrays (16-byte vectors)
+  long long ncell;                 // FP32: interior cells
+  long long off_f32, off_b32, off_h, off_mbf, off_mbb;   // FP32 arrays
+  long long total;                 // vectors the workspace must hold (either layout)
 };
 
 __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
@@ -990,17 +994,24 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   g.T = ni + g.NL - 1;
   g.W = (nk + 63) / 64;
   g.TW = ni + 63;
-  long long n64 = (long long)g.T * RL * 64, n32 = (long long)g.W * g.TW * 64;
-  g.nrec = n64 > n32 ? n64 : n32;
+  g.nrec = (long long)g.T * RL * 64;
   long long hc = (long long)RL * 64;
   g.off_b = LU_NVEC * g.nrec;
   g.off_hcf = 2 * LU_NVEC * g.nrec;
   g.off_hcb = g.off_hcf + LU_NVEC * hc;
   g.off_hrf = g.off_hcb + LU_NVEC * hc;
   g.off_hrb = g.off_hrf + (long long)LU_NVEC * ni;
-  g.off_mbf = g.off_hrb + (long long)LU_NVEC * ni;       // FP32 mailboxes, W x ni float4 each
+  long long t64 = g.off_hrb + (long long)LU_NVEC * ni;
+  // FP32: per-cell records (AoS, interior cells j fastest), the step-layout
+  // hand-off between the sweeps, the inter-workgroup mailboxes
+  g.ncell = (long long)ni * nk;
+  g.off_f32 = 0;
+  g.off_b32 = g.off_f32 + 10 * g.ncell;
+  g.off_h = g.off_b32 + 8 * g.ncell;
+  g.off_mbf = g.off_h + (long long)g.W * g.TW * 64;
   g.off_mbb = g.off_mbf + (long long)g.W * ni;
-  g.total = g.off_mbb + (long long)g.W * ni;
+  long long t32 = g.off_mbb + (long long)g.W * ni;
+  g.total = t64 > t32 ? t64 : t32;
   return g;
 }
 
@@ -1087,9 +1098,9 @@ __device__ __forceinline__ long long lu_entry(const DevMesh& M, const LUGeom& g,
 }
 
 // FP64 records (reference form, see above)
-// entry of interior cell (j,k) in the FP32 layout: workgroup w = (k-ks)/64
-// owns rows ks+64w .. ks+64w+63, lane l = (k-ks)%64; forward step
-// t = (j-js) + l, backward step t = (je-j) + (63-l)
+// entry of interior cell (j,k) in the FP32 hand-off layout: workgroup
+// w = (k-ks)/64 owns rows ks+64w .. ks+64w+63, lane l = (k-ks)%64; forward
+// step t = (j-js) + l, backward step t = (je-j) + (63-l)
 __device__ __forceinline__ long long lu_entry32(const DevMesh& M, const LUGeom& g, int j, int k, int dir) {
   int kk = k - M.ks, w = kk >> 6, l = kk & 63;
   int t = dir > 0 ? (j - M.js) + l : (M.je - j) + (63 - l);
@@ -1334,7 +1345,7 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 
 // ------------------------------------------------- FP32 LU-SGS (matrix form)
 // The mixed-precision sweeps evaluate the same update as ilu2d with the
-// operator assembled per cell in FP64 and rounded once:
+// operator assembled per cell in FP64 and rounded once (k_lu_prep32):
 //   forward   s(j,k) = u + PA . s(j-1,k) + PB . s(j,k-1)
 //             u = st*di,  PA = sv*di*A+(j-1,k),  PB = sv*di*B+(j,k-1)
 //   backward  s(j,k) = s_fwd(j,k) + PA' . s(j+1,k) + PB' . s(j,k+1)
@@ -1343,9 +1354,10 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // as 4x4 matrices).  Neighbours in the halo ring hold fixed point-Jacobi
 // values (src/lhs.f:568-598), so their terms are folded into u (forward)
 // and into cB (backward, added by the forward sweep to its output).
-//   F record (10 float4): u, cB, PA rows 0-3, PB rows 0-3
-//   B record ( 9 float4): s_fwd + cB (written by the forward sweep), PA'
-//                         rows 0-3, PB' rows 0-3
+//   F record (10 float4 per interior cell): u, cB, PA rows 0-3, PB rows 0-3
+//   B record ( 8 float4 per interior cell): PA' rows 0-3, PB' rows 0-3
+//   H (one float4 per cell in sweep-step order): s_fwd + cB written by the
+//     forward sweep, overwritten with the result by the backward sweep
 // The halo-ring s values themselves are scaled in k_lu_finish (the prep
 // reads the unscaled neighbours).
 constexpr int LU32_NVF = 10, LU32_NVB = 9;
@@ -1395,15 +1407,17 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
   const double eps2 = M.c.epse * 2.;
   const double epsv = 1. + eps2;
   const LUGeom g = lu_geom(M);
-  const LUArr F = lu_arr(M, 0, g.nrec), B = lu_arr(M, g.off_b, g.nrec);
-  const long long ncell = (long long)g.ni * g.nk;
+  F4* recf = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_f32);
+  F4* recb = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_b32);
   // both mailboxes (2 * W * ni float4) start as the empty sentinel
   const long long nmb = 2ll * g.W * g.ni;
   u32x4_t* mb = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_mbf);
   for (long long w = blockIdx.x * 256ll + threadIdx.x; w < nmb; w += 256ll * gridDim.x)
     mb[w] = u32x4_t{LU32_EMPTY, LU32_EMPTY, LU32_EMPTY, LU32_EMPTY};
-  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
-    const int j = js + (int)(w % g.ni), k = ks + (int)(w / g.ni);
+  // one thread per interior cell, j fastest: coalesced reads and, the
+  // records being per cell, coalesced writes
+  for (long long ic = blockIdx.x * 256ll + threadIdx.x; ic < g.ncell; ic += 256ll * gridDim.x) {
+    const int j = js + (int)(ic % g.ni), k = ks + (int)(ic / g.ni);
     const long long i = cell(M, j, k);
     CellLoad c0 = load_cell(M, i);
     LUCellF64 L0 = lu_of(M.c, c0, epsv);
@@ -1414,14 +1428,13 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
     lu32_term(M, i - jd, k == ks, 0, +1, f, eps2, epsv, PB, u);
     lu32_term(M, i + 1, j == je, 1, -1, -f, eps2, epsv, QA, cb);
     lu32_term(M, i + jd, k == ke, 0, -1, -f, eps2, epsv, QB, cb);
-    const long long ef = lu_entry32(M, g, j, k, +1), eb = lu_entry32(M, g, j, k, -1);
-    auto put = [](const LUArr& a, long long e, int q, const double* v) {
-      *reinterpret_cast<F4*>(a.at(q, e)) = F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
-    };
-    put(F, ef, 0, u);
-    put(F, ef, 1, cb);
-    for (int r = 0; r < 4; ++r) { put(F, ef, 2 + r, PA + 4 * r); put(F, ef, 6 + r, PB + 4 * r); }
-    for (int r = 0; r < 4; ++r) { put(B, eb, 1 + r, QA + 4 * r); put(B, eb, 5 + r, QB + 4 * r); }
+    auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
+    F4* rf = recf + 10 * ic;
+    F4* rb = recb + 8 * ic;
+    rf[0] = f4(u);
+    rf[1] = f4(cb);
+    for (int r = 0; r < 4; ++r) { rf[2 + r] = f4(PA + 4 * r); rf[6 + r] = f4(PB + 4 * r); }
+    for (int r = 0; r < 4; ++r) { rb[r] = f4(QA + 4 * r); rb[4 + r] = f4(QB + 4 * r); }
   }
 }
 
@@ -1519,10 +1532,13 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   D2* lu = reinterpret_cast<D2*>(M.lu);
   if (wave > 0) {
     // ---------------------------------------------------------- producers
-    const long long roff = DIR > 0 ? 0 : g.off_b;
-    __amdgpu_buffer_rsrc_t rec[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) rec[q] = uniform_rsrc(lu + roff + q * g.nrec, g.nrec * 16);
+    // per-cell records: lane l's cell at step tt is (js+tt-l, ks+64w+l)
+    // forward, (je-tt+63-l, ks+64w+l) backward (clamped into the grid for
+    // lanes outside it, whose values are never used)
+    constexpr int NR = DIR > 0 ? 10 : 8;
+    const __amdgpu_buffer_rsrc_t rec = uniform_rsrc(lu + (DIR > 0 ? g.off_f32 : g.off_b32), g.ncell * NR * 16);
+    const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
+    const int krow = (64 * w + l) < g.nk ? 64 * w + l : g.nk - 1;
     // mailbox written by the neighbour block (none at the grid edge)
     const bool has_in = DIR > 0 ? w > 0 : w < g.W - 1;
     gu32* mb_in = has_in ? (gu32*)(lu + (DIR > 0 ? g.off_mbf + (long long)(w - 1) * g.ni
@@ -1535,8 +1551,17 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
     // normally already there when checked
     auto mail_ptr = [&](int tt) { return mb_in + 4ll * (DIR > 0 ? tt : g.ni - 1 - tt); };
     auto fetch = [&](int tt, u32x4_t (&v)[NV + 1]) {
+      int jc = DIR > 0 ? tt - l : (g.ni - 1) - tt + (63 - l);
+      jc = jc < 0 ? 0 : (jc >= g.ni ? g.ni - 1 : jc);
+      const int vo = (int)(((long long)krow * g.ni + jc) * NR * 16);
+      if constexpr (DIR > 0) {
 #pragma unroll
-      for (int q = 0; q < NV; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(rec[q], l * 16, (base + tt) * 1024, 0);
+        for (int q = 0; q < NR; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(rec, vo + q * 16, 0, 0);
+      } else {
+        v[0] = __builtin_amdgcn_raw_buffer_load_b128(hnd, l * 16, (base + tt) * 1024, 0);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) v[1 + q] = __builtin_amdgcn_raw_buffer_load_b128(rec, vo + q * 16, 0, 0);
+      }
       v[NV] = u32x4_t{0u, 0u, 0u, 0u};
       if (has_in && tt < g.ni) {
         gu32* p = mail_ptr(tt);
@@ -1575,7 +1600,7 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   // ------------------------------------------------------------- compute
   const int js = M.js, je = M.je;
   const int rows = g.nk - 64 * w < 64 ? g.nk - 64 * w : 64;
-  __amdgpu_buffer_rsrc_t out = uniform_rsrc(lu + g.off_b, g.nrec * 16);
+  __amdgpu_buffer_rsrc_t out = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
   const bool has_out = DIR > 0 ? w < g.W - 1 : w > 0;
   const bool edge_out = DIR > 0 ? l == 63 : l == 0;
   gu32* mb_out = (gu32*)(lu + (DIR > 0 ? g.off_mbf : g.off_mbb) + (long long)w * g.ni);
@@ -1683,7 +1708,7 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
   if (*M.status) return;
   const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
   const LUGeom g = lu_geom(M);
-  const LUArr B = lu_arr(M, g.off_b, g.nrec);
+  const LUArr B = sizeof(T) == 8 ? lu_arr(M, g.off_b, g.nrec) : lu_arr(M, g.off_h, (long long)g.W * g.TW * 64);
   long long n2 = (long long)jd * kd;
   int bad = 0;
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
@@ -2100,6 +2125,11 @@ hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hip
   return hipGetLastError();
 }
 
+hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st) {
+  hipLaunchKernelGGL(k_norms_final, dim3(nmi), dim3(RES_THREADS), 0, st, dM);
+  return hipGetLastError();
+}
+
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
   long long mx = 0;
   for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
@@ -2158,9 +2188,8 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
   int wmax = 1;
   for (int i = 0; i < nmi; ++i) {
     mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
-    int nk = hM[i].ke - hM[i].ks + 1;
-    long long c = (long long)(hM[i].je - hM[i].js + 1) * nk;
-    nc = c > nc ? c : nc;
+    int nk = hM[i].ke - hM[i].ks + 1, ni = hM[i].je - hM[i].js + 1;
+    nc = (long long)ni * nk > nc ? (long long)ni * nk : nc;
     wmax = (nk + 63) / 64 > wmax ? (nk + 63) / 64 : wmax;
   }
   const size_t ldsf = (size_t)SF * (LU32_NVF + 1) * 1024, ldsb = (size_t)SB * (LU32_NVB + 1) * 1024;

@@ -662,6 +662,211 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
   }
 }
 
+// Line solves for many lines at once: one lane per line, 64 lines per
+// block, the same reference arithmetic.
+//
+// k_sa_lines_j: lines along j (one per k row).  The block's 64 rows are
+// streamed through LDS in tiles of SA_TJ columns: the tile is loaded with
+// row-contiguous (coalesced) reads while the previous tile is solved, each
+// lane runs its row's recurrence across the tile from LDS, and the results
+// go back through LDS with row-contiguous writes.  The forward pass leaves
+// c' and s' in place (as lsolvej does), the backward pass writes
+// s = (s1 - s2) * bjmat.
+constexpr int SA_TJ = 16;
+// total lines (both directions, all instances) above which the solves run one lane per line
+constexpr long long SA_LANE_LINES = 4096;
+
+__global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ Ms) {
+  __shared__ double ta[64][SA_TJ + 1], tb[64][SA_TJ + 1], tc[64][SA_TJ + 1], ts[64][SA_TJ + 1];
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int k0 = ks + 64 * blockIdx.x;
+  if (k0 > ke) return;
+  const int nrow = ke - k0 + 1 < 64 ? ke - k0 + 1 : 64;
+  const int n = je - js + 1;
+  const int ntile = (n + SA_TJ - 1) / SA_TJ;
+  const int l = threadIdx.x;
+  double* __restrict__ sS = M.sa_sj;
+  double* __restrict__ sC = M.sa_cj;
+  const double* __restrict__ sA = M.sa_aj;
+  const double* __restrict__ sB = M.sa_bb;
+  // element w of a tile: row w / SA_TJ, column w % SA_TJ (16 lanes read one
+  // row's 16 consecutive cells)
+  constexpr int PER = 64 * SA_TJ / 64;
+  auto gidx = [&](int t, int w, int& r, int& cc, bool& ok) {
+    r = w / SA_TJ; cc = w % SA_TJ;
+    int j = js + t * SA_TJ + cc;
+    ok = r < nrow && j <= je;
+    return cell(M, j < je ? j : je, k0 + (r < nrow ? r : 0));
+  };
+  double ra[PER], rb[PER], rc[PER], rs[PER];
+  auto load = [&](int t) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, cc; bool ok;
+      long long i = gidx(t, l + 64 * p, r, cc, ok);
+      ra[p] = ok ? sA[i] : 0.0; rb[p] = ok ? sB[i] : 1.0; rc[p] = ok ? sC[i] : 0.0; rs[p] = ok ? sS[i] : 0.0;
+    }
+  };
+  // ---- forward elimination
+  double cp = 0.0, sp = 0.0;
+  load(0);
+  for (int t = 0; t < ntile; ++t) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int w = l + 64 * p, r = w / SA_TJ, cc = w % SA_TJ;
+      ta[r][cc] = ra[p]; tb[r][cc] = rb[p]; tc[r][cc] = rc[p]; ts[r][cc] = rs[p];
+    }
+    if (t + 1 < ntile) load(t + 1);
+    __syncthreads();
+    if (l < nrow) {
+      const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
+      for (int cc = 0; cc < cn; ++cc) {
+        if (c0 + cc == 0) {
+          double bb = 1. / tb[l][0];
+          sp = ts[l][0] * bb; cp = tc[l][0] * bb;
+        } else {
+          double ai = ta[l][cc];
+          double bb = 1. / (tb[l][cc] - ai * cp);
+          sp = (ts[l][cc] - ai * sp) * bb;
+          cp = tc[l][cc] * bb;
+        }
+        ts[l][cc] = sp; tc[l][cc] = cp;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, cc; bool ok;
+      long long i = gidx(t, l + 64 * p, r, cc, ok);
+      if (ok) { sS[i] = ts[r][cc]; sC[i] = tc[r][cc]; }
+    }
+  }
+  // ---- back substitution, tiles in reverse; s = (0 + sn - 0) * bjmat
+  double sn = 0.0;
+  auto load2 = [&](int t) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, cc; bool ok;
+      long long i = gidx(t, l + 64 * p, r, cc, ok);
+      rb[p] = ok ? sB[i] : 1.0; rc[p] = ok ? sC[i] : 0.0; rs[p] = ok ? sS[i] : 0.0;
+    }
+  };
+  load2(ntile - 1);
+  for (int t = ntile - 1; t >= 0; --t) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int w = l + 64 * p, r = w / SA_TJ, cc = w % SA_TJ;
+      tb[r][cc] = rb[p]; tc[r][cc] = rc[p]; ts[r][cc] = rs[p];
+    }
+    if (t > 0) load2(t - 1);
+    __syncthreads();
+    if (l < nrow) {
+      const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
+      for (int cc = cn - 1; cc >= 0; --cc) {
+        if (c0 + cc == n - 1) sn = ts[l][cc];
+        else sn = ts[l][cc] - tc[l][cc] * sn;
+        ts[l][cc] = (0. + sn - 0.) * tb[l][cc];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int r, cc; bool ok;
+      long long i = gidx(t, l + 64 * p, r, cc, ok);
+      if (ok) sS[i] = ts[r][cc];
+    }
+  }
+}
+
+// k_sa_lines_k: lines along k (one per j column), lanes on consecutive
+// columns, so every step of the recurrence reads one contiguous row
+// segment; the coefficients of the next SA_PK rows are loaded ahead.  The
+// backward pass finishes each cell: nu-tilde update with its clip
+// (src/samodel2d.f:157-188) and the interior eddy viscosity (c_turm).
+constexpr int SA_PK = 8;
+
+__global__ __launch_bounds__(64) void k_sa_lines_k(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int j = js + 64 * blockIdx.x + threadIdx.x;
+  if (js + 64 * (int)blockIdx.x > je) return;
+  const bool lane_ok = j <= je;
+  const int jj = lane_ok ? j : je;
+  const int n = ke - ks + 1;
+  const long long st = M.jd;
+  const long long i0 = cell(M, jj, ks);
+  double* __restrict__ sS = M.sa_sj;
+  double* __restrict__ sC = M.sa_ck;
+  const double* __restrict__ sA = M.sa_ak;
+  const double* __restrict__ sB = M.sa_bb;
+  double pa[SA_PK], pb[SA_PK], pc[SA_PK], ps[SA_PK];
+#pragma unroll
+  for (int d = 0; d < SA_PK; ++d)
+    if (d < n) { long long i = i0 + d * st; pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i]; }
+  double cp = 0.0, sp = 0.0;
+  for (int t0 = 0; t0 < n; t0 += SA_PK) {
+#pragma unroll
+    for (int d = 0; d < SA_PK; ++d) {
+      const int t = t0 + d;
+      if (t < n) {
+        double a = pa[d], b = pb[d], c = pc[d], s = ps[d];
+        if (t + SA_PK < n) {
+          long long i = i0 + (t + SA_PK) * st;
+          pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i];
+        }
+        if (t == 0) {
+          double bb = 1. / b;
+          sp = s * bb; cp = c * bb;
+        } else {
+          double bb = 1. / (b - a * cp);
+          sp = (s - a * sp) * bb;
+          cp = c * bb;
+        }
+        if (lane_ok) { long long i = i0 + t * st; sS[i] = sp; sC[i] = cp; }
+      }
+    }
+  }
+  // back substitution, rows in reverse (the values this lane just wrote),
+  // next rows loaded ahead; dq = s2 goes back into sjmat
+  double qs[SA_PK], qc[SA_PK];
+#pragma unroll
+  for (int d = 0; d < SA_PK; ++d)
+    if (d < n) { long long i = i0 + (n - 1 - d) * st; qs[d] = sS[i]; qc[d] = sC[i]; }
+  double sn = sp;
+  for (int u0 = 0; u0 < n; u0 += SA_PK) {
+#pragma unroll
+    for (int d = 0; d < SA_PK; ++d) {
+      const int u = u0 + d;
+      if (u < n) {
+        const int t = n - 1 - u;
+        double s = qs[d], c = qc[d];
+        if (u + SA_PK < n) {
+          long long i = i0 + (t - SA_PK) * st;
+          qs[d] = sS[i]; qc[d] = sC[i];
+        }
+        if (u > 0) sn = s - c * sn;
+        if (lane_ok) sS[i0 + t * st] = sn;
+      }
+    }
+  }
+  // nu-tilde update with its clip and the eddy viscosity, cell by cell
+  if (!lane_ok) return;
+  const double vnulim = 1.0e-20, relfac = 1.0;
+  for (int t = 0; t < n; ++t) {
+    long long i = i0 + t * st;
+    double dq = relfac * (0. + sS[i]);
+    double q5 = M.q[4][i];
+    q5 = fmax((q5 + dq * (double)imx(M.iblank[i], 0)), vnulim);
+    M.q[4][i] = q5;
+    c_turm_cell(M, i);
+  }
+}
+
 // distance to the wall (dist, src/samodel2d.f:243-293); static grids: once
 __global__ __launch_bounds__(256) void k_sa_dist(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
@@ -726,10 +931,17 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   hipLaunchKernelGGL(k_sa_rhs, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_RHS, st);
-  hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
+  // few lines (one case): one wave per line keeps each recurrence's chain
+  // short in latency; many lines (batches): one lane per line
+  long long lines = 0;
+  for (int i = 0; i < nmi; ++i) lines += (hM[i].ke - hM[i].ks + 1) + (hM[i].je - hM[i].js + 1);
+  const bool per_lane = lines > SA_LANE_LINES;
+  if (per_lane) hipLaunchKernelGGL(k_sa_lines_j, dim3((mk_ + 63) / 64, nmi), dim3(64), 0, st, dM);
+  else hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_J, st);
-  hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
+  if (per_lane) hipLaunchKernelGGL(k_sa_lines_k, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
+  else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
   hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(256), 0, st, dM, 1);
