@@ -1008,6 +1008,88 @@ static void muscld_new(double* f, double* ql, double* qr, int mdim, int is, int 
   }
 }
 
+/* ------------------------------------------------------------ WENO5 (a3) */
+/* minmod of src/eulerrhs.f:1036 (ammd) */
+static double ammd(double fl, double fr) {
+  return 0.5 * (copysign(1.0, fl) + copysign(1.0, fr)) * MIN(fabs(fl), fabs(fr));
+}
+
+/* weno5, src/eulerrhs.f:1192-1228: Jiang-Shu weights, epsw = 1e-6 */
+static double weno5(double a, double b, double c, double d, double e) {
+  double b1 = 13. / 12., b2 = 1. / 6., epsw = 1.e-6;
+  double djm1 = a - 2. * b + c, ejm1 = a - 4. * b + 3. * c;
+  double dj = b - 2. * c + d, ej = b - d;
+  double djp1 = c - 2. * d + e, ejp1 = 3. * c - 4. * d + e;
+  double dis0 = b1 * djm1 * djm1 + 0.25 * ejm1 * ejm1 + epsw;
+  double dis1 = b1 * dj * dj + 0.25 * ej * ej + epsw;
+  double dis2 = b1 * djp1 * djp1 + 0.25 * ejp1 * ejp1 + epsw;
+  double q30 = 2. * a - 7. * b + 11. * c;
+  double q31 = -b + 5. * c + 2. * d;
+  double q32 = 2. * c + 5. * d - e;
+  double d01 = dis0 / dis1, d02 = dis0 / dis2;
+  double a1ba0 = 6. * d01, a2ba0 = 3. * d02;
+  double w0 = 1. / (1. + a1ba0 + a2ba0);
+  double w1 = a1ba0 * w0;
+  double w2 = 1. - w0 - w1;
+  return b2 * (w0 * q30 + w1 * q31 + w2 * q32);
+}
+
+/* SONIC-A slope at i from limited curvatures (src/eulerrhs.f:1125-1140) */
+static double sonic_slope(const double* f1, const double* f2m, int i) {
+  double at = f1[i] - 0.5 * f2m[i];
+  double ati1 = f1[i - 1] + 0.5 * f2m[i - 1];
+  double s1 = ammd(f1[i], f1[i - 1]);
+  double t1 = ammd(at, ati1);
+  return copysign(1.0, t1) * MIN(0.5 * fabs(at + ati1), MAX(2. * fabs(s1), fabs(t1)));
+}
+
+/* weno, src/eulerrhs.f:1011-1189, as rhsup calls it (ibmin = ibmax = 1:
+   the fmin/fmax boundary branches are never taken).  Lines of cells
+   is..ie; ql(ie) and qr(is) are not used by the caller. */
+static void weno_line(const double* f, double* ql, double* qr, int mdim, int is, int ie, double qt,
+                      const int* iba, double* wk) {
+  if (qt == 0.0) {
+    for (int n = 0; n < 4; ++n)
+      for (int i = is; i <= ie; ++i) { ql[n * mdim + i] = f[n * mdim + i]; qr[n * mdim + i] = f[n * mdim + i]; }
+    return;
+  }
+  double* f1 = wk;
+  double* f2 = wk + mdim;
+  double* f2m = wk + 2 * mdim;
+  for (int n = 0; n < 4; ++n) {
+    const double* f0 = f + n * mdim;
+    double* l = ql + n * mdim;
+    double* r = qr + n * mdim;
+    for (int i = is; i <= ie - 1; ++i) f1[i] = f0[i + 1] - f0[i];
+    for (int i = is + 1; i <= ie - 1; ++i) f2[i] = f1[i] - f1[i - 1];
+    f2[is] = 2. * f2[is + 1] - f2[is + 2];
+    f2[ie] = 2. * f2[ie - 1] - f2[ie - 2];
+    for (int i = is; i <= ie - 1; ++i) f2m[i] = ammd(f2[i], f2[i + 1]);
+    /* first order-like ends (src/eulerrhs.f:1089-1091, 1104-1106) */
+    double at = f1[is] - 0.5 * f2m[is];
+    l[is] = f0[is] + 0.5 * ammd(at, 2. * f1[is]);
+    double at1 = f1[ie - 1] + 0.5 * f2m[ie - 1];
+    r[ie] = f0[ie] - 0.5 * ammd(at1, 2. * f1[ie - 1]);
+    /* SONIC-A at is+1 and ie-1 (src/eulerrhs.f:1121-1141) */
+    for (int i = is + 1; i <= ie - 1; i += ie - is - 2) {
+      double sl = sonic_slope(f1, f2m, i);
+      l[i] = f0[i] + 0.5 * sl;
+      r[i] = f0[i] - 0.5 * sl;
+    }
+    /* WENO5 inside, first order next to blanked cells (1145-1153) */
+    for (int i = is + 2; i <= ie - 2; ++i) {
+      l[i] = weno5(f0[i - 2], f0[i - 1], f0[i], f0[i + 1], f0[i + 2]);
+      r[i] = weno5(f0[i + 2], f0[i + 1], f0[i], f0[i - 1], f0[i - 2]);
+      double blankv = (double)(iba[i] * iba[i - 1] * iba[i + 1] * iba[i - 2] * iba[i + 2]);
+      l[i] = l[i] * blankv + (1. - blankv) * f0[i];
+      r[i] = r[i] * blankv + (1. - blankv) * f0[i];
+    }
+    /* SONIC-A again for ql(is+2) and qr(ie-1) (1157-1187) */
+    l[is + 2] = f0[is + 2] + 0.5 * sonic_slope(f1, f2m, is + 2);
+    r[ie - 1] = f0[ie - 1] - 0.5 * sonic_slope(f1, f2m, ie - 1);
+  }
+}
+
 /* ---------------------------------------------------------- Roe (a4) */
 /* roeflx, src/eulerrhs.f:283-446 */
 static void roeflx(or_case* c, double* f, const double* ql, const double* qr, int mdim,
@@ -1102,7 +1184,7 @@ static int rhsup_mesh(or_case* c, or_mesh* M, int nsp) {
   double* f = (double*)calloc((size_t)mdim * 4, sizeof(double));
   double* ql = (double*)calloc((size_t)mdim * 4, sizeof(double));
   double* qr = (double*)calloc((size_t)mdim * 4, sizeof(double));
-  double* f2 = (double*)calloc((size_t)mdim, sizeof(double));
+  double* f2 = (double*)calloc((size_t)mdim * 3, sizeof(double));
   double* xa = (double*)calloc((size_t)mdim, sizeof(double));
   double* ya = (double*)calloc((size_t)mdim, sizeof(double));
   double* tj = (double*)calloc((size_t)mdim, sizeof(double));
@@ -1131,6 +1213,7 @@ static int rhsup_mesh(or_case* c, or_mesh* M, int nsp) {
       iba[j] = abs(ib[i]);
     }
     if (ilima == 1) muscld_new(f, ql, qr, mdim, 1, M->jd, M->jd - 1, th, qt, epsj, iba, f2);
+    if (c->p.ilim == 4) weno_line(f, ql, qr, mdim, 1, M->jd, qt, iba, f2);
     for (int j = M->jbeg - 1; j <= M->jend; ++j) {
       int jv = j;
       xa[j] = (yv[VX(M, jv, kv)] - yv[VX(M, jv, kv - 1)]);
@@ -1164,6 +1247,7 @@ static int rhsup_mesh(or_case* c, or_mesh* M, int nsp) {
       iba[k] = abs(ib[i]);
     }
     if (ilima == 1) muscld_new(f, ql, qr, mdim, 1, M->kd, M->kd - 1, th, qt, epsk, iba, f2);
+    if (c->p.ilim == 4) weno_line(f, ql, qr, mdim, 1, M->kd, qt, iba, f2);
     for (int k = M->kbeg - 1; k <= M->kend; ++k) {
       int kv = k;
       xa[k] = -(yv[VX(M, jv, kv)] - yv[VX(M, jv - 1, kv)]);

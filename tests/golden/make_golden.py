@@ -36,6 +36,8 @@ CASES = {
     # name: (jmax, kmax, ds_wall, camber m, p, t, deck, nsteps, dump iteration)
     "euler_33x17": (33, 17, 1e-3, 0.0, 0.0, 0.12, EULER, 40, 20),
     "sa_41x21": (41, 21, 1e-4, 0.02, 0.4, 0.12, SA, 40, 35),
+    # WENO5 reconstruction (ilim=4, src/eulerrhs.f:1011-1230)
+    "weno_sa_41x21": (41, 21, 1e-4, 0.02, 0.4, 0.12, dict(SA, ilim=4), 40, 35),
 }
 
 STAGES = ["qin", "turmuin", "tscale", "qbc", "res", "rbc", "sdt", "dq", "qlu", "qturb", "qout", "turmu"]
@@ -123,8 +125,12 @@ def run_driver_case():
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
+    only = set(sys.argv[1:])   # fixture names to (re)write; none: all
     for name, spec in CASES.items():
-        run_case(name, *spec)
+        if not only or name in only:
+            run_case(name, *spec)
     for name, spec in TS_CASES.items():
-        run_ts_case(name, *spec)
-    run_driver_case()
+        if not only or name in only:
+            run_ts_case(name, *spec)
+    if not only or "driver_sa_41x21" in only:
+        run_driver_case()

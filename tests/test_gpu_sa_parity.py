@@ -20,7 +20,7 @@ def _gpu_golden(name, **extra):
     return H.golden_case(t2d.load_library(), "ot2d_", name, errfn=t2d.last_error, **extra)
 
 
-@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21"])
+@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21", "weno_sa_41x21"])
 def test_stages_vs_reference(name):
     z, case = _gpu_golden(name)
     lamin = name.startswith("euler")
@@ -29,7 +29,7 @@ def test_stages_vs_reference(name):
         H.assert_cellwise(a.ravel(order="F"), z[st], 1e-12, st)
 
 
-@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21"])
+@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21", "weno_sa_41x21"])
 def test_trajectory_vs_reference(name):
     z, case = _gpu_golden(name)
     n = int(z["nsteps"])
@@ -71,3 +71,35 @@ def test_sa_fp32_lusgs():
     assert rc_o == rc_d == 0
     np.testing.assert_allclose(h_d, h_o, rtol=1e-3)
     assert H.rel_err(d.get_state(0), o.get_state(0)) < 1e-5
+
+
+WENO_EULER = dict(H.EULER, ilim=4, fsmach=0.6)
+
+
+@pytest.mark.parametrize("deck", ["euler", "sa", "sa_fp32", "first_order"])
+def test_weno5_larger_grid_vs_oracle(deck):
+    """WENO5 reconstruction (ilim=4) on a grid with several residual tiles
+    per line: the tile halo of three cells and the line-end formulas."""
+    d_deck = {"euler": WENO_EULER, "sa": dict(SA_DECK, ilim=4), "sa_fp32": dict(SA_DECK, ilim=4),
+              "first_order": dict(WENO_EULER, irhsy=-1)}[deck]
+    g = H.cgrid(129, 49, ds_wall=1e-3 if d_deck.get("invisc") else 1e-5)
+    o = H.oracle_case(g, d_deck)
+    d = H.gpu_case(g, dict(d_deck, lu_fp32=1) if deck == "sa_fp32" else d_deck)
+    rc_o, h_o, n_o = o.iterate(25)
+    rc_d, h_d, n_d = d.iterate(25)
+    assert rc_o == rc_d == 0 and n_o == n_d == 25
+    if deck == "sa_fp32":
+        np.testing.assert_allclose(h_d, h_o, rtol=1e-3)
+        assert H.rel_err(d.get_state(0), o.get_state(0)) < 1e-5
+        return
+    np.testing.assert_allclose(h_d, h_o, rtol=1e-10)
+    H.assert_cellwise(d.get_state(0), o.get_state(0), 1e-10, "q")
+
+
+def test_weno5_rejects_ilim_minus4():
+    """ilim=-4 selects no reconstruction in the reference (src/eulerrhs.f:85-97)."""
+    import turns2d_urans_git_amd as t2d
+    g = H.cgrid(33, 17)
+    with pytest.raises(RuntimeError, match="ilim"):
+        H.gpu_case(g, dict(H.EULER, ilim=-4))
+    assert t2d.last_error()

@@ -22,6 +22,12 @@ DECKS = {
     "cambered_neg_alpha": dict(H.EULER, alfa=-3.0, fsmach=0.7),
     "muscl_2nd_order": dict(H.EULER, irhsy=-2),
     "const_dt": dict(H.EULER, iconstant_cfl=0, dt=0.05),
+    # WENO5 reconstruction (ilim=4, src/eulerrhs.f:1011-1230), inviscid and SA
+    "weno5_euler": dict(H.EULER, ilim=4, fsmach=0.6),
+    "weno5_sa": dict(fsmach=0.5, alfa=2.0, rey=3.0e6, invisc=False, lamin=False, iturb=1, vnuinf=3.0,
+                     irhsy=-3, ilim=4, ilhs=1, iconstant_cfl=1, dt=5.0, epse=0.01, iunst=0, ntac=1,
+                     itnmax=1, timeac=1.0),
+    "weno5_first_order": dict(H.EULER, ilim=4, irhsy=-1),
     "sa_m08": dict(fsmach=0.8, alfa=1.0, rey=5.0e6, invisc=False, lamin=False, iturb=1, vnuinf=0.1,
                    irhsy=-3, ilim=1, ilhs=1, iconstant_cfl=1, dt=5.0, epse=0.02, iunst=0, ntac=1,
                    itnmax=1, timeac=1.0),

@@ -61,6 +61,7 @@ struct ot2d_case {
   int nmesh = 0, nspec = 1, nq = 5, nv = 4, lamin = 1, iturb = -1;
   int device = 0;
   int lu_rl = 1;             // rows per lane of the LU-SGS sweeps
+  int weno = 0;              // some case reconstructs with WENO5 (wider residual tiles)
   int ncase = 1;             // independent cases (batch: one per mesh; otherwise 1)
   std::vector<int> mesh_case;  // case of each mesh
   hipStream_t stream = nullptr;
@@ -267,7 +268,10 @@ static int32_t check_params(const ot2d_params* p) {
   if (p->nhalo != 1) return fail(OT2D_ERR_UNSUPPORTED, "nhalo=%d: only one halo ring is supported", p->nhalo);
   if (p->ilhs != 1) return fail(OT2D_ERR_UNSUPPORTED, "ilhs=%d: only LU-SGS (ilhs=1) is supported", p->ilhs);
   if (p->iprecon) return fail(OT2D_ERR_UNSUPPORTED, "low-Mach preconditioning is not supported");
-  if (!(abs(p->ilim) == 1)) return fail(OT2D_ERR_UNSUPPORTED, "ilim=%d not supported (1: Koren MUSCL)", p->ilim);
+  // rhsup reconstructs with muscld_new for |ilim| = 1 and weno for ilim = 4
+  // (src/eulerrhs.f:85-97); ilim = -4 reaches neither
+  if (!(abs(p->ilim) == 1 || p->ilim == 4))
+    return fail(OT2D_ERR_UNSUPPORTED, "ilim=%d not supported (1: Koren MUSCL, 4: WENO5)", p->ilim);
   if (!p->timespectral && p->iunst != 0)
     return fail(OT2D_ERR_UNSUPPORTED, "iunst=%d: time-accurate motion is not supported", p->iunst);
   if (p->itnmax != 1) return fail(OT2D_ERR_UNSUPPORTED, "itnmax=%d: Newton sub-iterations not supported", p->itnmax);
@@ -349,6 +353,11 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     MeshBuf& mb = c->m[im];
     mb.d = md[im];
     if (md[im].jmax < 5 || md[im].kmax < 5) { free_case(c); return fail(OT2D_ERR_ARG, "mesh %d too small", im); }
+    if (ps[mesh_case[im]].ilim == 4 && (md[im].jmax < 7 || md[im].kmax < 7)) {
+      free_case(c);
+      return fail(OT2D_ERR_ARG, "mesh %d: WENO5 needs jmax, kmax >= 7", im);
+    }
+    if (ps[mesh_case[im]].ilim == 4) c->weno = 1;
     if (md[im].nbc < 0 || md[im].nbc > OT2D_MAXBC) { free_case(c); return fail(OT2D_ERR_ARG, "nbc"); }
     if (md[im].kmax - 1 > 256) {
       free_case(c);
@@ -640,7 +649,7 @@ static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks*
   mark(mk, OT2D_ST_BC, c->stream);
   // the norms are taken on the complete residual (src/step.f:46-93)
   const bool later = !c->p.invisc || c->p.timespectral;
-  HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms && !later, c->stream));
+  HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms && !later, c->weno, c->stream));
   mark(mk, OT2D_ST_RESIDUAL, c->stream);
   if (!c->p.invisc) {
     HIPCHK(launch_visrhs(c->dM, c->hM.data(), nmi(c), c->stream));

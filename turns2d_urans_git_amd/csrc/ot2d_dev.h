@@ -114,7 +114,7 @@ inline void mark(Marks* m, int s, hipStream_t st) {
 hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st);
 hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
-hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms,
+hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, int weno,
                            hipStream_t st);
 hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st);
 hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st);

@@ -492,15 +492,133 @@ __device__ __forceinline__ void face_states(const double* sf0, int stv, int stc,
   }
 }
 
+// ---- WENO5 (src/eulerrhs.f:1011-1230) as rhsup calls it: ibmin = ibmax = 1,
+// so the fmin/fmax branches are dead.  For a line of n cells the reference
+// leaves ql(1) and qr(n) from its end formulas, ql and qr of cells 2 and n-1
+// and ql of cell 3 from SONIC-A slopes, and WENO5 (blended to first order
+// next to blanked cells) elsewhere; the functions below evaluate each case
+// from the cells it reads, with the reference's operation order.
+__device__ __forceinline__ double ammd(double fl, double fr) {
+  double a = fabs(fl), b = fabs(fr);
+  return 0.5 * (copysign(1.0, fl) + copysign(1.0, fr)) * (a < b ? a : b);
+}
+
+__device__ __forceinline__ double weno5(double a, double b, double c, double d, double e) {
+  const double b1 = 13. / 12., b2 = 1. / 6., epsw = 1.e-6;
+  double djm1 = a - 2. * b + c, ejm1 = a - 4. * b + 3. * c;
+  double dj = b - 2. * c + d, ej = b - d;
+  double djp1 = c - 2. * d + e, ejp1 = 3. * c - 4. * d + e;
+  double dis0 = b1 * djm1 * djm1 + 0.25 * ejm1 * ejm1 + epsw;
+  double dis1 = b1 * dj * dj + 0.25 * ej * ej + epsw;
+  double dis2 = b1 * djp1 * djp1 + 0.25 * ejp1 * ejp1 + epsw;
+  double q30 = 2. * a - 7. * b + 11. * c;
+  double q31 = -b + 5. * c + 2. * d;
+  double q32 = 2. * c + 5. * d - e;
+  double a1ba0 = 6. * (dis0 / dis1), a2ba0 = 3. * (dis0 / dis2);
+  double w0 = 1. / (1. + a1ba0 + a2ba0);
+  double w1 = a1ba0 * w0;
+  double w2 = 1. - w0 - w1;
+  return b2 * (w0 * q30 + w1 * q31 + w2 * q32);
+}
+
+// SONIC-A slope from f1(i), f1(i-1) and the limited curvatures f2m(i), f2m(i-1)
+__device__ __forceinline__ double sonic(double f1i, double f1m, double f2mi, double f2mm) {
+  double at = f1i - 0.5 * f2mi, ati1 = f1m + 0.5 * f2mm;
+  double s1 = ammd(f1i, f1m), t1 = ammd(at, ati1);
+  double a = 0.5 * fabs(at + ati1), b = 2. * fabs(s1), c = fabs(t1);
+  double mx = b > c ? b : c;
+  return copysign(1.0, t1) * (a < mx ? a : mx);
+}
+
+// slope at cell 2 from f0(1..4) (f2(1) extrapolated)
+__device__ __forceinline__ double sonic_lo(double x1, double x2, double x3, double x4, double& f1_1, double& f2m_1) {
+  double f11 = x2 - x1, f12 = x3 - x2, f13 = x4 - x3;
+  double f22 = f12 - f11, f23 = f13 - f12;
+  double f21 = 2. * f22 - f23;
+  f1_1 = f11;
+  f2m_1 = ammd(f21, f22);
+  return sonic(f12, f11, ammd(f22, f23), f2m_1);
+}
+
+// slope at cell n-1 from f0(n-3..n) (f2(n) extrapolated)
+__device__ __forceinline__ double sonic_hi(double y0, double y1, double y2, double y3, double& f1_n1, double& f2m_n1) {
+  double fa = y1 - y0, fb = y2 - y1, fc = y3 - y2;    // f1(n-3), f1(n-2), f1(n-1)
+  double f2a = fb - fa, f2b = fc - fb;                // f2(n-2), f2(n-1)
+  double f2n = 2. * f2b - f2a;
+  f1_n1 = fc;
+  f2m_n1 = ammd(f2b, f2n);
+  return sonic(fc, fb, f2m_n1, ammd(f2a, f2b));
+}
+
+// slope at cell i from f0(i-2..i+2) (no extrapolated curvature)
+__device__ __forceinline__ double sonic_mid(double z0, double z1, double z2, double z3, double z4) {
+  double fa = z1 - z0, fb = z2 - z1, fc = z3 - z2, fd = z4 - z3;   // f1(i-2..i+1)
+  double f2a = fb - fa, f2b = fc - fb, f2c = fd - fc;             // f2(i-1..i+1)
+  return sonic(fc, fb, ammd(f2b, f2c), ammd(f2a, f2b));
+}
+
+// left state of cell i and right state of cell i+1 (the two sides of face
+// i) of a line of n cells, from x[0..5] = f0(i-2..i+3), ib[0..5] = |iblank|
+__device__ __forceinline__ void weno_face(const double* x, const int* ib, int i, int n, double& ql, double& qr) {
+  double f1, f2m;
+  if (i == 1) {
+    sonic_lo(x[2], x[3], x[4], x[5], f1, f2m);
+    double at = f1 - 0.5 * f2m;
+    ql = x[2] + 0.5 * ammd(at, 2. * f1);
+  } else if (i == 2) {
+    ql = x[2] + 0.5 * sonic_lo(x[1], x[2], x[3], x[4], f1, f2m);
+  } else if (i == 3 || i == n - 1) {
+    ql = x[2] + 0.5 * (i == 3 ? sonic_mid(x[0], x[1], x[2], x[3], x[4]) : sonic_hi(x[0], x[1], x[2], x[3], f1, f2m));
+  } else {
+    double w = weno5(x[0], x[1], x[2], x[3], x[4]);
+    double bv = (double)(ib[2] * ib[1] * ib[3] * ib[0] * ib[4]);
+    ql = w * bv + (1. - bv) * x[2];
+  }
+  const int m = i + 1;
+  if (m == n) {
+    sonic_hi(x[0], x[1], x[2], x[3], f1, f2m);
+    double at1 = f1 + 0.5 * f2m;
+    qr = x[3] - 0.5 * ammd(at1, 2. * f1);
+  } else if (m == 2) {
+    qr = x[3] - 0.5 * sonic_lo(x[2], x[3], x[4], x[5], f1, f2m);
+  } else if (m == n - 1) {
+    qr = x[3] - 0.5 * sonic_hi(x[1], x[2], x[3], x[4], f1, f2m);
+  } else {
+    double w = weno5(x[5], x[4], x[3], x[2], x[1]);
+    double bv = (double)(ib[3] * ib[2] * ib[4] * ib[1] * ib[5]);
+    qr = w * bv + (1. - bv) * x[3];
+  }
+}
+
+// face_states for ilim = 4: c0 = LDS column of line cell i; the tile holds
+// three halo cells on each side
+__device__ __forceinline__ void face_states_weno(const double* sf0, int stv, int stc, const int* sib, int stci,
+                                                 int c0, int i, int n, double qt, double* l, double* r) {
+  int ib[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) ib[d] = sib[(c0 - 2 + d) * stci];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double* f = sf0 + v * stv;
+    if (qt == 0.0) { l[v] = f[c0 * stc]; r[v] = f[(c0 + 1) * stc]; continue; }
+    double x[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) x[d] = f[(c0 - 2 + d) * stc];
+    weno_face(x, ib, i, n, l[v], r[v]);
+  }
+}
+
 __device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool valid, const double* s,
                           double* redbuf);
 
 // Residual of one 32x8 tile: rhsup for nei=1 (src/eulerrhs.f:2-280), fluxes
 // of the tile's faces computed once into LDS, plus the residual-norm partials
 // of step (src/step.f:60-93).  Writes s(:,:,1:4) of the tile's interior cells.
+// H: halo cells the tile loads on each side (2 for MUSCL, 3 for WENO5)
+template <int H>
 __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restrict__ Ms,
                                                           const ResTile* __restrict__ tiles, int do_norms) {
-  constexpr int TJ = RES_TJ, TK = RES_TK, LJ = TJ + 4, LK = TK + 4;
+  constexpr int TJ = RES_TJ, TK = RES_TK, LJ = TJ + 2 * H, LK = TK + 2 * H;
   __shared__ double sf[4][LK][LJ];   // primitives rho, u, v, p
   __shared__ int sib[LK][LJ];
   __shared__ double sxv[TK + 1][TJ + 1], syv[TK + 1][TJ + 1];
@@ -514,11 +632,11 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
   const int jd = M.jd, kd = M.kd;
   const int j0 = T.j0, k0 = T.k0;
   QV v = qview(M);
-  // ---- load primitives of the tile + 2-cell halo (corners unused)
+  // ---- load primitives of the tile + H-cell halo (corners unused)
   for (int w = tid; w < LK * LJ; w += RES_THREADS) {
     int r = w / LJ, cc = w % LJ;
-    int j = j0 - 2 + cc, k = k0 - 2 + r;
-    bool row_in = (r >= 2 && r < 2 + TK), col_in = (cc >= 2 && cc < 2 + TJ);
+    int j = j0 - H + cc, k = k0 - H + r;
+    bool row_in = (r >= H && r < H + TK), col_in = (cc >= H && cc < H + TJ);
     if (!row_in && !col_in) continue;
     int jj = j < 1 ? 1 : (j > jd ? jd : j);
     int kk = k < 1 ? 1 : (k > kd ? kd : k);
@@ -550,8 +668,11 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
     int fi = j0 - 1 + cc, k = k0 + r;
     if (fi > M.je || k > M.ke) continue;
     double l[4], rr[4], f[4];
-    // LDS column of cell fi: fi - (j0 - 2)
-    face_states(&sf[0][r + 2][0], LK * LJ, 1, &sib[r + 2][0], 1, fi - (j0 - 2), fi, jd, th, qt, M.epsj, l, rr);
+    // LDS column of cell fi: fi - (j0 - H)
+    if (H == 3 && C.ilim == 4)
+      face_states_weno(&sf[0][r + H][0], LK * LJ, 1, &sib[r + H][0], 1, fi - (j0 - H), fi, jd, qt, l, rr);
+    else
+      face_states(&sf[0][r + H][0], LK * LJ, 1, &sib[r + H][0], 1, fi - (j0 - H), fi, jd, th, qt, M.epsj, l, rr);
     // face vector at vertex column jv = fi, rows kv-1..kv with kv = k
     double xa = syv[r + 1][cc] - syv[r][cc];
     double ya = -(sxv[r + 1][cc] - sxv[r][cc]);
@@ -570,7 +691,10 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
     int fk = k0 - 1 + r, j = j0 + cc;
     if (fk > M.ke || j > M.je) continue;
     double l[4], rr[4], f[4];
-    face_states(&sf[0][0][cc + 2], LK * LJ, LJ, &sib[0][cc + 2], LJ, fk - (k0 - 2), fk, kd, th, qt, M.epsk, l, rr);
+    if (H == 3 && C.ilim == 4)
+      face_states_weno(&sf[0][0][cc + H], LK * LJ, LJ, &sib[0][cc + H], LJ, fk - (k0 - H), fk, kd, qt, l, rr);
+    else
+      face_states(&sf[0][0][cc + H], LK * LJ, LJ, &sib[0][cc + H], LJ, fk - (k0 - H), fk, kd, th, qt, M.epsk, l, rr);
     // vertex row kv = fk, columns jv-1..jv with jv = j
     double xa = -(syv[r][cc + 1] - syv[r][cc]);
     double ya = (sxv[r][cc + 1] - sxv[r][cc]);
@@ -2115,8 +2239,12 @@ hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, hipStream_t st) {
-  hipLaunchKernelGGL(k_residual, dim3(ntiles), dim3(RES_THREADS), 0, st, dM, tiles, do_norms);
+hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, int weno,
+                           hipStream_t st) {
+  if (weno)
+    hipLaunchKernelGGL(k_residual<3>, dim3(ntiles), dim3(RES_THREADS), 0, st, dM, tiles, do_norms);
+  else
+    hipLaunchKernelGGL(k_residual<2>, dim3(ntiles), dim3(RES_THREADS), 0, st, dM, tiles, do_norms);
   return hipGetLastError();
 }
 
