@@ -1880,6 +1880,458 @@ static void vmu_sa_mesh(or_case* c, or_mesh* M, int nsp) {
   free(up); free(um); free(vp); free(vm); free(chp); free(dnuhp); free(dmxh); free(dmyh);
 }
 
+/* ------------------------------------------------------- k-omega SST (a13) */
+/* constants, src/komega.h:3-8 */
+static const double KW_sigmak1 = 0.85, KW_sigmaw1 = 0.5, KW_beta1 = 0.075, KW_sigmak2 = 1.0,
+                    KW_sigmaw2 = 0.856, KW_beta2 = 0.0828, KW_betastar = 0.09;
+static const double KW_gamma1 = 5. / 9, KW_gamma2 = 0.44;
+
+/* komegabc, src/komegasst.f:906-1434 (one halo layer).  rho, u, v, vmul
+   are the arrays komegasst filled before the call. */
+static int komegabc_mesh(or_case* c, or_mesh* M, int nsp, const double* rho, const double* u, const double* v,
+                         const double* vmul) {
+  double *q5 = QA(M, nsp, 5), *q6 = QA(M, nsp, 6);
+  double *ug = GA(M, ug, nsp), *vg = GA(M, vg, nsp);
+  double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+  double *x = GA(M, x, nsp), *y = GA(M, y, nsp);
+  const double *xv = VA(M, xv, nsp), *yv = VA(M, yv, nsp);
+  double rei = 1. / c->rey;
+  for (int ib = 0; ib < M->d.nbc; ++ib) {
+    const ot2d_bcseg* b = &M->d.bc[ib];
+    int js, je, ks, ke;
+    bc_range(M, b, &js, &je, &ks, &ke);
+    int idir = b->ibdir, iadd = idir >= 0 ? 1 : -1, iadir = abs(idir);
+    int t = b->ibtyp;
+    double foso = 1.0;
+    if (t == 3 || t == 4) {          /* komegabc_extpt, 1079-1129 */
+      if (iadir == 1) {
+        for (int jc = 1; jc <= je - js + 1; ++jc) {
+          int j = (idir == 1) ? je - jc + 1 : js + jc - 1;
+          int j1 = j + iadd, j2 = j1 + iadd;
+          for (int k = ks; k <= ke; ++k) {
+            q5[IX(M, j, k)] = (1. + foso) * q5[IX(M, j1, k)] - foso * q5[IX(M, j2, k)];
+            q6[IX(M, j, k)] = (1. + foso) * q6[IX(M, j1, k)] - foso * q6[IX(M, j2, k)];
+          }
+        }
+      } else if (iadir == 2) {
+        for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+          int k = (idir == 2) ? ke - kc + 1 : ks + kc - 1;
+          int k1 = k + iadd, k2 = k1 + iadd;
+          for (int j = js; j <= je; ++j) {
+            q5[IX(M, j, k)] = (1. + foso) * q5[IX(M, j, k1)] - foso * q5[IX(M, j, k2)];
+            q6[IX(M, j, k)] = (1. + foso) * q6[IX(M, j, k1)] - foso * q6[IX(M, j, k2)];
+          }
+        }
+      }
+    } else if (t == 5) {             /* komegabc_wall, 977-1076 (one halo layer) */
+      if (iadir != 2) return OT2D_ERR_UNSUPPORTED;
+      double tt = ((double)c->istep0 - 1.) / 30.;
+      if (tt > 1. || c->p.iread > 0) tt = 1.;
+      double scal = (10. - 15. * tt + 6. * tt * tt) * ((tt * tt) * tt);
+      int k = (idir == 2) ? ke : ks, k1 = k + iadd, kv = k1 - 1;
+      for (int j = js; j <= je; ++j) {
+        int jv = j - 1;
+        size_t i = IX(M, j, k), i1 = IX(M, j, k1);
+        double dely2 = sq(x[i1] - 0.5 * (xv[VX(M, jv, kv)] + xv[VX(M, jv + 1, kv)])) +
+                       sq(y[i1] - 0.5 * (yv[VX(M, jv, kv)] + yv[VX(M, jv + 1, kv)]));
+        double vmulwall = 0.5 * (vmul[i1] + vmul[i1]);
+        double omegawall = 60. * vmulwall * rei * rei / (KW_beta1 * dely2);
+        double qwall5 = -q5[i1];
+        double qwall6 = 2 * omegawall - 6. * vmul[i1] * rei * rei / (KW_beta1 * dely2);
+        q5[i] = scal * qwall5 + (1. - scal) * q5[i];
+        q6[i] = scal * qwall6 + (1. - scal) * q6[i];
+      }
+    } else if (t == 10 || t == 11) { /* komegabc_sym, 1132-1180 */
+      if (iadir == 1) {
+        for (int jc = 1; jc <= je - js + 1; ++jc) {
+          int j, j1;
+          if (idir == 1) { j = je - jc + 1; j1 = je + jc; } else { j = js + jc - 1; j1 = js - jc; }
+          for (int k = ks; k <= ke; ++k) { q5[IX(M, j, k)] = q5[IX(M, j1, k)]; q6[IX(M, j, k)] = q6[IX(M, j1, k)]; }
+        }
+      } else if (iadir == 2) {
+        for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+          int k, k1;
+          if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+          for (int j = js; j <= je; ++j) { q5[IX(M, j, k)] = q5[IX(M, j, k1)]; q6[IX(M, j, k)] = q6[IX(M, j, k1)]; }
+        }
+      }
+    } else if (t == 22) {            /* komegabc_periodic, 1183-1251 */
+      if (iadir == 1) {
+        for (int j = js; j <= je; ++j) {
+          int jc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, j);
+          for (int k = ks; k <= ke; ++k) { q5[IX(M, j, k)] = q5[IX(M, jc, k)]; q6[IX(M, j, k)] = q6[IX(M, jc, k)]; }
+        }
+      } else {
+        for (int k = ks; k <= ke; ++k) {
+          int kc = periodic_src(idir, js, je, ks, ke, M->jd, M->kd, k);
+          for (int j = js; j <= je; ++j) { q5[IX(M, j, k)] = q5[IX(M, j, kc)]; q6[IX(M, j, k)] = q6[IX(M, j, kc)]; }
+        }
+      }
+    } else if (t == 51 && iadir == 2) { /* komegabc_wake, 1254-1295 */
+      for (int kc = 1; kc <= ke - ks + 1; ++kc) {
+        int k, k1;
+        if (idir == 2) { k = ke - kc + 1; k1 = ke + kc; } else { k = ks + kc - 1; k1 = ks - kc; }
+        for (int j = js; j <= je; ++j) {
+          int jj = M->jd - j + 1;
+          q5[IX(M, j, k)] = q5[IX(M, jj, k1)];
+          q6[IX(M, j, k)] = q6[IX(M, jj, k1)];
+          q5[IX(M, jj, k)] = q5[IX(M, j, k1)];
+          q6[IX(M, jj, k)] = q6[IX(M, j, k1)];
+        }
+      }
+    } else if (t == 46) {            /* komegabc_inf, 1298-1332 */
+      for (int k = ks; k <= ke; ++k)
+        for (int j = js; j <= je; ++j) {
+          size_t i = IX(M, j, k);
+          q5[i] = c->tkeinf * rho[i];
+          q6[i] = c->tomegainf * rho[i];
+        }
+    } else if (t == 47) {            /* komegabc_out, 1335-1432 */
+      if (iadir == 1) {
+        int j = (idir == 1) ? je : js, j1 = j + iadd;
+        for (int k = ks; k <= ke; ++k) {
+          size_t i = IX(M, j, k), i1 = IX(M, j1, k);
+          q5[i] = c->tkeinf * rho[i];
+          q6[i] = c->tomegainf * rho[i];
+          double uu = (u[i] - ug[i] + u[i1] - ug[i1]) * xx[i];
+          uu = uu + (v[i] - vg[i] + v[i1] - vg[i1]) * xy[i];
+          uu = uu * iadd;
+          if (uu < 0.) { q5[i] = q5[i1]; q6[i] = q6[i1]; }
+        }
+      } else if (iadir == 2) {
+        int k = (idir == 2) ? ke : ks, k1 = k + iadd;
+        for (int j = js; j <= je; ++j) {
+          size_t i = IX(M, j, k), i1 = IX(M, j, k1);
+          q5[i] = c->tkeinf * rho[i];
+          q6[i] = c->tomegainf * rho[i];
+          double uu = (u[i] - ug[i] + u[i1] - ug[i1]) * yx[i];
+          uu = uu + (v[i] - vg[i] + v[i1] - vg[i1]) * yy[i];
+          uu = uu * iadd;
+          if (uu < 0.) { q5[i] = q5[i1]; q6[i] = q6[i1]; }
+        }
+      }
+    }
+  }
+  return OT2D_OK;
+}
+
+/* tridiagonal solves of one equation along j (lsolvej2, 1435-1475) or
+   along k (lsolvek2, 1478-1518): stride st between line cells, n cells */
+static void kw_thomas(double* a, const double* b, double* cc, double* s, size_t i0, size_t st, int n) {
+  double bb = 1. / b[i0];
+  s[i0] = s[i0] * bb;
+  cc[i0] = cc[i0] * bb;
+  for (int t = 1; t < n; ++t) {
+    size_t i = i0 + t * st, im = i - st;
+    bb = 1. / (b[i] - a[i] * cc[im]);
+    s[i] = (s[i] - a[i] * s[im]) * bb;
+    cc[i] = cc[i] * bb;
+  }
+  for (int t = n - 2; t >= 0; --t) {
+    size_t i = i0 + t * st;
+    s[i] = s[i] - cc[i] * s[i + st];
+  }
+}
+
+/* komegasst, src/komegasst.f:2-187 (itrans = 0).  The reference reads F1,
+   F2 and strain on the halo ring without writing them there (blend and
+   kwvortic fill the interior only; findturm and komegadiffus read every
+   cell): allocations here are zero-filled, and the reference run the same
+   way (zero-filled malloc) reproduces these values bit for bit. */
+static int komegasst_mesh(or_case* c, or_mesh* M, int nsp) {
+  int jd = M->jd, kd = M->kd, js = M->jbeg, je = M->jend, ks = M->kbeg, ke = M->kend;
+  size_t n2 = M->n2;
+  double *q1 = QA(M, nsp, 1), *q2 = QA(M, nsp, 2), *q3 = QA(M, nsp, 3), *q4 = QA(M, nsp, 4);
+  double *q5 = QA(M, nsp, 5), *q6 = QA(M, nsp, 6), *qj = QA(M, nsp, c->nq);
+  double *xx = GA(M, xx, nsp), *xy = GA(M, xy, nsp), *yx = GA(M, yx, nsp), *yy = GA(M, yy, nsp);
+  double *ug = GA(M, ug, nsp), *vg = GA(M, vg, nsp);
+  double *x = GA(M, x, nsp), *y = GA(M, y, nsp);
+  const double *xv = VA(M, xv, nsp), *yv = VA(M, yv, nsp);
+  double* turmu = GA(M, turmu, nsp);
+  double* tscale = GA(M, tscale, nsp);
+  const int32_t* ib = M->iblank + (size_t)nsp * n2;
+  double *s5 = SA(M, nsp, 5), *s6 = SA(M, nsp, 6);
+  double *rho = calloc(n2, 8), *u = calloc(n2, 8), *v = calloc(n2, 8), *vmul = calloc(n2, 8), *sn = calloc(n2, 8);
+  double *tauxx = calloc(n2, 8), *tauxy = calloc(n2, 8), *tauyy = calloc(n2, 8);
+  double *ux = calloc(n2, 8), *uy = calloc(n2, 8), *vx = calloc(n2, 8), *vy = calloc(n2, 8);
+  double *cdkw = calloc(n2, 8), *F1 = calloc(n2, 8), *F2 = calloc(n2, 8), *strain = calloc(n2, 8);
+  double *aj[2], *bj[2], *cj[2], *ak[2], *ck[2], *sj[2];
+  for (int n = 0; n < 2; ++n) {
+    aj[n] = calloc(n2, 8); bj[n] = calloc(n2, 8); cj[n] = calloc(n2, 8);
+    ak[n] = calloc(n2, 8); ck[n] = calloc(n2, 8); sj[n] = calloc(n2, 8);
+  }
+  double *chp = calloc(n2, 8), *dnuhp = calloc(n2, 8), *dmxh = calloc(n2, 8), *dmyh = calloc(n2, 8);
+  double *up = calloc(n2, 8), *um = calloc(n2, 8);
+  double* qk[2] = {q5, q6};
+  double* sk[2] = {s5, s6};
+  double rei = 1. / c->rey;
+  int rc = OT2D_OK;
+  /* lamvis, src/samodel2d.f:200-238 */
+  {
+    double c2b = 198.6 / c->p.tinf, c2bp = c2b + 1.;
+    for (size_t i = 0; i < n2; ++i) {
+      double ra = 1. / q1[i];
+      double uvel = q2[i] * ra, vvel = q3[i] * ra;
+      double ei = q4[i] * ra - 0.5 * (sq(uvel) + sq(vvel));
+      double tt = c->ggm1 * ei;
+      vmul[i] = c2bp * tt * sqrt(tt) / (c2b + tt);
+    }
+  }
+  for (size_t i = 0; i < n2; ++i) {
+    rho[i] = q1[i] * qj[i];
+    u[i] = q2[i] / q1[i];
+    v[i] = q3[i] / q1[i];
+    sn[i] = 1.e10;
+  }
+  if ((rc = komegabc_mesh(c, M, nsp, rho, u, v, vmul))) goto done;
+  /* kwdist, 218-267 */
+  if (M->bodyflag) {
+    for (int j = M->jtail1; j <= M->jtail2; ++j) sn[IX(M, j, 1)] = 0.0;
+    for (int j = M->jtail1; j <= M->jtail2; ++j)
+      for (int k = 2; k <= kd; ++k) {
+        int jv = j - 1;
+        size_t i = IX(M, j, k);
+        sn[i] = sqrt(sq(x[i] - 0.5 * (xv[VX(M, jv, 1)] + xv[VX(M, jv + 1, 1)])) +
+                     sq(y[i] - 0.5 * (yv[VX(M, jv, 1)] + yv[VX(M, jv + 1, 1)])));
+      }
+    int jv = M->jtail1 - 1;
+    for (int j = 1; j <= M->jtail1 - 1; ++j)
+      for (int k = 1; k <= kd; ++k) {
+        size_t i = IX(M, j, k);
+        sn[i] = sqrt(sq(x[i] - xv[VX(M, jv, 1)]) + sq(y[i] - yv[VX(M, jv, 1)]));
+      }
+    for (int j = M->jtail2 + 1; j <= jd; ++j)
+      for (int k = 1; k <= kd; ++k) {
+        size_t i = IX(M, j, k);
+        sn[i] = sqrt(sq(x[i] - xv[VX(M, jv, 1)]) + sq(y[i] - yv[VX(M, jv, 1)]));
+      }
+  }
+  /* kwvortic (270-318, strain only), turbstress (321-371), crossdiff
+     (374-414) and blend (417-461) over the interior */
+  for (int j = js; j <= je; ++j)
+    for (int k = ks; k <= ke; ++k) {
+      size_t i = IX(M, j, k);
+      size_t ie = IX(M, j + 1, k), iw = IX(M, j - 1, k), in = IX(M, j, k + 1), is = IX(M, j, k - 1);
+      double usi = 0.5 * (u[ie] - u[iw]), vsi = 0.5 * (v[ie] - v[iw]);
+      double ueta = 0.5 * (u[in] - u[is]), veta = 0.5 * (v[in] - v[is]);
+      double lux = xx[i] * usi + yx[i] * ueta, luy = xy[i] * usi + yy[i] * ueta;
+      double lvx = xx[i] * vsi + yx[i] * veta, lvy = xy[i] * vsi + yy[i] * veta;
+      double sxx = lux, sxy = 0.5 * (luy + lvx), syy = lvy;
+      strain[i] = sqrt(2. * (sxx * sxx + 2. * sxy * sxy + syy * syy));
+      ux[i] = lux; uy[i] = luy; vx[i] = lvx; vy[i] = lvy;
+      double sfdiv = 2. / 3 * (ux[i] + vy[i]);
+      tauxx[i] = turmu[i] * rei * (2.0 * ux[i] - sfdiv) - 2. / 3 * q5[i];
+      tauyy[i] = turmu[i] * rei * (2.0 * vy[i] - sfdiv) - 2. / 3 * q5[i];
+      tauxy[i] = turmu[i] * rei * (uy[i] + vx[i]);
+    }
+  for (int j = js; j <= je; ++j)
+    for (int k = ks; k <= ke; ++k) {
+      size_t i = IX(M, j, k);
+      size_t ie = IX(M, j + 1, k), iw = IX(M, j - 1, k), in = IX(M, j, k + 1), is = IX(M, j, k - 1);
+      double ksi = 0.5 * (q5[ie] / rho[ie] - q5[iw] / rho[iw]);
+      double keta = 0.5 * (q5[in] / rho[in] - q5[is] / rho[is]);
+      double omegasi = 0.5 * (q6[ie] / rho[ie] - q6[iw] / rho[iw]);
+      double omegaeta = 0.5 * (q6[in] / rho[in] - q6[is] / rho[is]);
+      double kx = ksi * xx[i] + keta * yx[i], ky = ksi * xy[i] + keta * yy[i];
+      double omegax = omegasi * xx[i] + omegaeta * yx[i], omegay = omegasi * xy[i] + omegaeta * yy[i];
+      cdkw[i] = kx * omegax + ky * omegay;
+      cdkw[i] = cdkw[i] * 2 * KW_sigmaw2 * rho[i] * rho[i] / q6[i];
+    }
+  for (int j = js; j <= je; ++j)
+    for (int k = ks; k <= ke; ++k) {
+      size_t i = IX(M, j, k);
+      double a1 = rei * sqrt(q5[i] * rho[i]) / (0.09 * q6[i] * sn[i]);
+      double a2 = 500 * rei * rei * vmul[i] / (q6[i] * sq(sn[i]));
+      double cd = MAX(cdkw[i], 1e-10);
+      double a3 = 4 * q5[i] * KW_sigmaw2 / (cd * sq(sn[i]));
+      double arg1 = MIN(MAX(a1, a2), a3);
+      double arg2 = MAX(2 * a1, a2);
+      F1[i] = tanh(((arg1 * arg1) * arg1) * arg1);
+      F2[i] = tanh(arg2 * arg2);
+    }
+  /* findturm, 190-215 */
+  for (size_t i = 0; i < n2; ++i) {
+    double a1 = 0.31;
+    turmu[i] = a1 * rho[i] * q5[i] / MAX(a1 * q6[i], F2[i] * rho[i] * strain[i] * rei);
+    turmu[i] = MAX(turmu[i], 1e-20);
+  }
+  /* komegarhslhs, 464-590 */
+  for (size_t i = 0; i < n2; ++i)
+    for (int n = 0; n < 2; ++n) { aj[n][i] = bj[n][i] = cj[n][i] = ak[n][i] = ck[n][i] = 0.0; sj[n][i] = sk[n][i]; }
+  for (int k = ks; k <= ke; ++k) {
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k);
+      double uu = xx[i] * (u[i] - ug[i]) + xy[i] * (v[i] - vg[i]);
+      up[i] = 0.5 * (uu + fabs(uu));
+      um[i] = 0.5 * (uu - fabs(uu));
+    }
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k), iw = IX(M, j - 1, k), ie = IX(M, j + 1, k);
+      for (int n = 0; n < 2; ++n)
+        sj[n][i] = sj[n][i] - up[i] * (qk[n][i] - qk[n][iw]) - um[i] * (qk[n][ie] - qk[n][i]);
+      for (int n = 0; n < 2; ++n) {
+        aj[n][i] = aj[n][i] - up[i];
+        cj[n][i] = cj[n][i] + um[i];
+        bj[n][i] = bj[n][i] + up[i] - um[i];
+      }
+    }
+  }
+  for (int j = js; j <= je; ++j) {
+    for (int k = ks - 1; k <= ke + 1; ++k) {
+      size_t i = IX(M, j, k);
+      double vv = yx[i] * (u[i] - ug[i]) + yy[i] * (v[i] - vg[i]);
+      up[i] = 0.5 * (vv + fabs(vv));
+      um[i] = 0.5 * (vv - fabs(vv));
+    }
+    for (int k = ks; k <= ke; ++k) {
+      size_t i = IX(M, j, k), is = IX(M, j, k - 1), in = IX(M, j, k + 1);
+      for (int n = 0; n < 2; ++n)
+        sj[n][i] = sj[n][i] - up[i] * (qk[n][i] - qk[n][is]) - um[i] * (qk[n][in] - qk[n][i]);
+      for (int n = 0; n < 2; ++n) {
+        ak[n][i] = ak[n][i] - up[i];
+        ck[n][i] = ck[n][i] + um[i];
+        bj[n][i] = bj[n][i] + up[i] - um[i];
+      }
+    }
+  }
+  /* komegasource, 847-903 (itrans = 0) */
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      double prod = tauxx[i] * ux[i] + tauxy[i] * (uy[i] + vx[i]) + tauyy[i] * vy[i];
+      double kdes = KW_betastar * c->rey * q6[i] * q5[i] / rho[i];
+      double ksust = KW_betastar * c->rey * c->tkeinf * c->tomegainf * rho[i];
+      double prodlim = 10. * kdes;
+      prod = MIN(prod, prodlim);
+      double kprod = prod;
+      double beta = F1[i] * KW_beta1 + (1 - F1[i]) * KW_beta2;
+      double omegades = beta * c->rey * q6[i] * q6[i] / rho[i];
+      double omegasust = beta * c->rey * (c->tomegainf * c->tomegainf) * rho[i];
+      double omegacrossdiff = (1 - F1[i]) * cdkw[i] * rei;
+      sj[0][i] = sj[0][i] + kprod - kdes + ksust;
+      double gammap = F1[i] * KW_gamma1 + (1 - F1[i]) * KW_gamma2;
+      sj[1][i] = sj[1][i] + gammap * rho[i] / turmu[i] * prod - omegades + omegacrossdiff + omegasust;
+      bj[0][i] = bj[0][i] + kdes / q5[i];
+      bj[1][i] = bj[1][i] + (2 * omegades + fabs(omegacrossdiff)) / q6[i];
+    }
+  /* komegadiffus, 593-844 */
+  for (int j = js - 1; j <= je; ++j)
+    for (int k = ks - 1; k <= ke + 1; ++k) {
+      size_t i = IX(M, j, k), i1 = IX(M, j + 1, k);
+      dmxh[i] = 0.5 * (xx[i] + xx[i1]);
+      dmyh[i] = 0.5 * (xy[i] + xy[i1]);
+    }
+  for (int n = 0; n < 2; ++n) {
+    double s1 = n == 0 ? KW_sigmak1 : KW_sigmaw1, s2 = n == 0 ? KW_sigmak2 : KW_sigmaw2;
+    for (int j = js - 1; j <= je; ++j)
+      for (int k = ks - 1; k <= ke + 1; ++k) {
+        size_t i = IX(M, j, k), i1 = IX(M, j + 1, k);
+        double vnulh = 0.5 * (vmul[i] + vmul[i1]);
+        double vnuh = 0.5 * (turmu[i] + turmu[i1]);
+        double sig = F1[i] * s1 + (1. - F1[i]) * s2;
+        chp[i] = rei * (vnulh + sig * vnuh);
+        dnuhp[i] = qk[n][i1] / rho[i1] - qk[n][i] / rho[i];
+      }
+    for (int j = js; j <= je; ++j)
+      for (int k = ks - 1; k <= ke + 1; ++k) {
+        size_t i = IX(M, j, k), iw = IX(M, j - 1, k), ie = IX(M, j + 1, k);
+        double dxp = dmxh[i] * xx[i] + dmyh[i] * xy[i];
+        double dxm = dmxh[iw] * xx[i] + dmyh[iw] * xy[i];
+        double dcp = dxp * chp[i], dcm = dxm * chp[iw];
+        double ax = 0.0, cx = 0.0;
+        if (k != ke + 1 && k != ks - 1) { ax = MAX(dcm, 0.0); cx = MAX(dcp, 0.0); }
+        sj[n][i] = sj[n][i] - ax * dnuhp[iw] + cx * dnuhp[i];
+        aj[n][i] = aj[n][i] - ax / rho[iw];
+        cj[n][i] = cj[n][i] - cx / rho[ie];
+        bj[n][i] = bj[n][i] + (ax + cx) / rho[i];
+      }
+    for (int k = 1; k <= kd; ++k) { aj[n][IX(M, je + 1, k)] = 0.0; cj[n][IX(M, js - 1, k)] = 0.0; }
+  }
+  for (int k = ks - 1; k <= ke; ++k)
+    for (int j = js - 1; j <= je + 1; ++j) {
+      size_t i = IX(M, j, k), i1 = IX(M, j, k + 1);
+      dmxh[i] = 0.5 * (yx[i] + yx[i1]);
+      dmyh[i] = 0.5 * (yy[i] + yy[i1]);
+    }
+  for (int n = 0; n < 2; ++n) {
+    double s1 = n == 0 ? KW_sigmak1 : KW_sigmaw1, s2 = n == 0 ? KW_sigmak2 : KW_sigmaw2;
+    for (int k = ks - 1; k <= ke; ++k)
+      for (int j = js - 1; j <= je + 1; ++j) {
+        size_t i = IX(M, j, k), i1 = IX(M, j, k + 1);
+        double vnulh = 0.5 * (vmul[i] + vmul[i1]);
+        double vnuh = 0.5 * (turmu[i] + turmu[i1]);
+        double sig = F1[i] * s1 + (1. - F1[i]) * s2;
+        chp[i] = rei * (vnulh + sig * vnuh);
+        dnuhp[i] = qk[n][i1] / rho[i1] - qk[n][i] / rho[i];
+      }
+    for (int k = ks; k <= ke; ++k)
+      for (int j = js - 1; j <= je + 1; ++j) {
+        size_t i = IX(M, j, k), is = IX(M, j, k - 1), in = IX(M, j, k + 1);
+        double dyp = dmxh[i] * yx[i] + dmyh[i] * yy[i];
+        double dym = dmxh[is] * yx[i] + dmyh[is] * yy[i];
+        double dcp = dyp * chp[i], dcm = dym * chp[is];
+        double ay = 0.0, cy = 0.0;
+        if (j != js - 1 && j != je + 1) { ay = MAX(dcm, 0.0); cy = MAX(dcp, 0.0); }
+        sj[n][i] = sj[n][i] - ay * dnuhp[is] + cy * dnuhp[i];
+        ak[n][i] = ak[n][i] - ay / rho[is];
+        ck[n][i] = ck[n][i] - cy / rho[in];
+        bj[n][i] = bj[n][i] + (ay + cy) / rho[i];
+      }
+    for (int j = 1; j <= kd && j <= jd; ++j) { ak[n][IX(M, j, ke + 1)] = 0.0; ck[n][IX(M, j, ks - 1)] = 0.0; }
+  }
+  /* pseudo-time scaling, 104-131 (oat = 1) */
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      double tscal = tscale[i];
+      if (c->p.timeac == 1.) {
+        double dtpseudo_turb = 10.0, oat = 1.0;
+        tscal = (double)MAX(ib[i], 0) * (1.0 + 0.002 * sqrt(qj[i])) / (1. + sqrt(qj[i]));
+        tscal = tscal * dtpseudo_turb;
+        tscal = tscal / (1. + tscal / c->h / oat);
+      }
+      for (int n = 0; n < 2; ++n) {
+        aj[n][i] = aj[n][i] * tscal;
+        cj[n][i] = cj[n][i] * tscal;
+        ak[n][i] = ak[n][i] * tscal;
+        ck[n][i] = ck[n][i] * tscal;
+        bj[n][i] = 1. + bj[n][i] * tscal;
+        sj[n][i] = sj[n][i] * tscal;
+      }
+    }
+  for (int n = 0; n < 2; ++n)
+    for (int k = ks; k <= ke; ++k) kw_thomas(aj[n], bj[n], cj[n], sj[n], IX(M, js, k), 1, je - js + 1);
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j)
+      for (int n = 0; n < 2; ++n) sj[n][IX(M, j, k)] = sj[n][IX(M, j, k)] * bj[n][IX(M, j, k)];
+  for (int n = 0; n < 2; ++n)
+    for (int j = js; j <= je; ++j) kw_thomas(ak[n], bj[n], ck[n], sj[n], IX(M, j, ks), (size_t)jd, ke - ks + 1);
+  /* update with the clip (relfac = 1), 146-172 */
+  for (int k = ks; k <= ke; ++k)
+    for (int j = js; j <= je; ++j) {
+      size_t i = IX(M, j, k);
+      double ibp = (double)MAX(ib[i], 0);
+      s5[i] = 1. * sj[0][i];
+      s6[i] = 1. * sj[1][i];
+      q5[i] = q5[i] + s5[i] * ibp;
+      q6[i] = q6[i] + s6[i] * ibp;
+      q5[i] = MAX(q5[i], 1e-20 * rho[i]);
+      q6[i] = MAX(q6[i], 1e-20 * rho[i]);
+    }
+  if ((rc = komegabc_mesh(c, M, nsp, rho, u, v, vmul))) goto done;
+  for (size_t i = 0; i < n2; ++i) {
+    double a1 = 0.31;
+    turmu[i] = a1 * rho[i] * q5[i] / MAX(a1 * q6[i], F2[i] * rho[i] * strain[i] * rei);
+    turmu[i] = MAX(turmu[i], 1e-20);
+  }
+done:
+  free(rho); free(u); free(v); free(vmul); free(sn); free(tauxx); free(tauxy); free(tauyy);
+  free(ux); free(uy); free(vx); free(vy); free(cdkw); free(F1); free(F2); free(strain);
+  for (int n = 0; n < 2; ++n) { free(aj[n]); free(bj[n]); free(cj[n]); free(ak[n]); free(ck[n]); free(sj[n]); }
+  free(chp); free(dnuhp); free(dmxh); free(dmyh); free(up); free(um);
+  return rc;
+}
+
 /* compute_residual, src/step.f:221-300 */
 int32_t or_compute_residual(or_case* c) {
   for (int im = 0; im < c->nmesh; ++im) {
@@ -1895,12 +2347,18 @@ int32_t or_compute_residual(or_case* c) {
   return OT2D_OK;
 }
 
-/* solve_turbulence_equations, src/step.f:303-329 (SA only) */
+/* solve_turbulence_equations, src/step.f:303-329 (SA and SST, no transition) */
 int32_t or_turbulence(or_case* c) {
   if (c->lamin) return OT2D_OK;
-  if (c->iturb != 1 || c->p.itrans) return OT2D_ERR_UNSUPPORTED;
+  if ((c->iturb != 1 && c->iturb != 2) || c->p.itrans) return OT2D_ERR_UNSUPPORTED;
   for (int im = 0; im < c->nmesh; ++im)
-    for (int nsp = 0; nsp < c->nspec; ++nsp) vmu_sa_mesh(c, &c->m[im], nsp);
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      if (c->iturb == 1) vmu_sa_mesh(c, &c->m[im], nsp);
+      else {
+        int rc = komegasst_mesh(c, &c->m[im], nsp);
+        if (rc) return rc;
+      }
+    }
   return OT2D_OK;
 }
 
@@ -2285,8 +2743,14 @@ int32_t or_step(or_case* c, double* norms, int32_t* argmax) {
       lhs_mesh(c, M, nsp);
     }
     if (!c->lamin) {
-      if (c->iturb != 1 || c->p.itrans) return OT2D_ERR_UNSUPPORTED;
-      for (int nsp = 0; nsp < c->nspec; ++nsp) vmu_sa_mesh(c, M, nsp);
+      if ((c->iturb != 1 && c->iturb != 2) || c->p.itrans) return OT2D_ERR_UNSUPPORTED;
+      for (int nsp = 0; nsp < c->nspec; ++nsp) {
+        if (c->iturb == 1) vmu_sa_mesh(c, M, nsp);
+        else {
+          int rc = komegasst_mesh(c, M, nsp);
+          if (rc) return rc;
+        }
+      }
     }
     for (int nsp = 0; nsp < c->nspec; ++nsp) {
       const int32_t* ib = M->iblank + (size_t)nsp * M->n2;

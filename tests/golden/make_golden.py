@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 from turns2d_urans_git_amd import grid  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+# komegasst reads F1, F2 and strain on the halo ring without writing them
+# there (src/komegasst.f:190-215, 593-844); the reference is run with
+# zero-filled allocations so those reads are defined (zero)
+ZERO_MALLOC = {"GLIBC_TUNABLES": "glibc.malloc.perturb=255"}
 
 EULER = dict(fsmach=0.5, alfa=1.25, invisc=True, lamin=True, irhsy=-3, ilim=1, ilhs=1,
              iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1, timeac=1.0)
@@ -38,6 +42,8 @@ CASES = {
     "sa_41x21": (41, 21, 1e-4, 0.02, 0.4, 0.12, SA, 40, 35),
     # WENO5 reconstruction (ilim=4, src/eulerrhs.f:1011-1230)
     "weno_sa_41x21": (41, 21, 1e-4, 0.02, 0.4, 0.12, dict(SA, ilim=4), 40, 35),
+    # k-omega SST (iturb=2, src/komegasst.f)
+    "sst_41x21": (41, 21, 1e-4, 0.02, 0.4, 0.12, dict(SA, iturb=2), 40, 35),
 }
 
 STAGES = ["qin", "turmuin", "tscale", "qbc", "res", "rbc", "sdt", "dq", "qlu", "qturb", "qout", "turmu"]
@@ -49,7 +55,7 @@ def run_case(name, jmax, kmax, ds, m, p, t, deck, nsteps, dumpat):
         grid.write_case(d, g, nsteps=nsteps, nrest=10**6, npnorm=1, **deck)
         out = os.path.join(d, "out")
         os.makedirs(out)
-        env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps), OT2D_DUMPAT=str(dumpat))
+        env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps), OT2D_DUMPAT=str(dumpat), **ZERO_MALLOC)
         with open(os.path.join(d, "inputs")) as fin:
             subprocess.run([HARNESS], cwd=d, stdin=fin, env=env, check=True,
                            stdout=subprocess.DEVNULL)

@@ -20,7 +20,7 @@ def _gpu_golden(name, **extra):
     return H.golden_case(t2d.load_library(), "ot2d_", name, errfn=t2d.last_error, **extra)
 
 
-@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21", "weno_sa_41x21"])
+@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21", "weno_sa_41x21", "sst_41x21"])
 def test_stages_vs_reference(name):
     z, case = _gpu_golden(name)
     lamin = name.startswith("euler")
@@ -29,7 +29,7 @@ def test_stages_vs_reference(name):
         H.assert_cellwise(a.ravel(order="F"), z[st], 1e-12, st)
 
 
-@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21", "weno_sa_41x21"])
+@pytest.mark.parametrize("name", ["euler_33x17", "sa_41x21", "weno_sa_41x21", "sst_41x21"])
 def test_trajectory_vs_reference(name):
     z, case = _gpu_golden(name)
     n = int(z["nsteps"])
@@ -103,3 +103,28 @@ def test_weno5_rejects_ilim_minus4():
     with pytest.raises(RuntimeError, match="ilim"):
         H.gpu_case(g, dict(H.EULER, ilim=-4))
     assert t2d.last_error()
+
+
+SST_DECK = dict(SA_DECK, iturb=2, fsmach=0.3, alfa=6.0, rey=3.0e6, dt=5.0)
+
+
+@pytest.mark.parametrize("variant", ["muscl", "weno5", "const_dt", "fp32_lu"])
+def test_sst_larger_grid_vs_oracle(variant):
+    """k-omega SST (iturb=2) on a grid with many lines: the wall, wake and
+    far-field turbulence boundary conditions, the blending functions and
+    both line solves, against the oracle."""
+    deck = {"muscl": SST_DECK, "weno5": dict(SST_DECK, ilim=4), "fp32_lu": SST_DECK,
+            "const_dt": dict(SST_DECK, iconstant_cfl=0, dt=0.02, timeac=0.0)}[variant]
+    g = H.cgrid(129, 49, ds_wall=1e-5, m=0.02, p=0.4)
+    o = H.oracle_case(g, deck)
+    d = H.gpu_case(g, dict(deck, lu_fp32=1) if variant == "fp32_lu" else deck)
+    rc_o, h_o, n_o = o.iterate(30)
+    rc_d, h_d, n_d = d.iterate(30)
+    assert rc_o == rc_d == 0 and n_o == n_d == 30
+    if variant == "fp32_lu":
+        np.testing.assert_allclose(h_d, h_o, rtol=1e-3)
+        assert H.rel_err(d.get_state(0), o.get_state(0)) < 1e-5
+        return
+    np.testing.assert_allclose(h_d, h_o, rtol=1e-10)
+    H.assert_cellwise(d.get_state(0), o.get_state(0), 1e-10, "q")
+    H.assert_cellwise(d.get_field(0, abi.F_TURMU), o.get_field(0, abi.F_TURMU), 1e-10, "turmu")

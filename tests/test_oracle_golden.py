@@ -14,7 +14,7 @@ import pytest
 from tests import helpers as H
 from turns2d_urans_git_amd import abi
 
-CASES = ["euler_33x17", "sa_41x21", "weno_sa_41x21"]
+CASES = ["euler_33x17", "sa_41x21", "weno_sa_41x21", "sst_41x21"]
 
 
 def _turb(case):

@@ -28,11 +28,26 @@ DECKS = {
                      irhsy=-3, ilim=4, ilhs=1, iconstant_cfl=1, dt=5.0, epse=0.01, iunst=0, ntac=1,
                      itnmax=1, timeac=1.0),
     "weno5_first_order": dict(H.EULER, ilim=4, irhsy=-1),
+    # k-omega SST (iturb=2), also with WENO5 as the overset configuration runs it
+    "sst_m05": dict(fsmach=0.5, alfa=2.0, rey=1.0e6, invisc=False, lamin=False, iturb=2, irhsy=-3, ilim=1,
+                    ilhs=1, iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1, timeac=1.0),
+    "sst_weno_m03": dict(fsmach=0.3, alfa=6.0, rey=3.0e6, invisc=False, lamin=False, iturb=2, irhsy=-3,
+                         ilim=4, ilhs=1, iconstant_cfl=1, dt=5.0, epse=0.01, iunst=0, ntac=1, itnmax=1,
+                         timeac=1.0),
+    "sst_const_dt": dict(fsmach=0.6, alfa=1.0, rey=2.0e6, invisc=False, lamin=False, iturb=2, irhsy=-3,
+                         ilim=1, ilhs=1, iconstant_cfl=0, dt=0.02, epse=0.01, iunst=0, ntac=1, itnmax=1,
+                         timeac=0.0),
     "sa_m08": dict(fsmach=0.8, alfa=1.0, rey=5.0e6, invisc=False, lamin=False, iturb=1, vnuinf=0.1,
                    irhsy=-3, ilim=1, ilhs=1, iconstant_cfl=1, dt=5.0, epse=0.02, iunst=0, ntac=1,
                    itnmax=1, timeac=1.0),
 }
 GRIDS = {"cambered_neg_alpha": dict(m=0.04, p=0.4, t=0.10)}
+
+
+# komegasst reads F1, F2 and strain on the halo ring without writing them
+# there (src/komegasst.f:190-215, 593-844): with zero-filled allocations the
+# reference is deterministic and the oracle (calloc) reproduces it
+ZERO_MALLOC = {"GLIBC_TUNABLES": "glibc.malloc.perturb=255"}
 
 
 def _ref_run(g, deck, nsteps):
@@ -41,7 +56,7 @@ def _ref_run(g, deck, nsteps):
                         **{k: (bool(v) if k in ("invisc", "lamin") else v) for k, v in deck.items()})
         out = os.path.join(d, "out")
         os.makedirs(out)
-        env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps))
+        env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps), **ZERO_MALLOC)
         with open(os.path.join(d, "inputs")) as fin:
             subprocess.run([H.REF_HARNESS], cwd=d, stdin=fin, env=env, check=True,
                            stdout=subprocess.DEVNULL, timeout=120)

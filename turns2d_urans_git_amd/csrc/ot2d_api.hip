@@ -176,6 +176,9 @@ static DevConst make_const(const ot2d_params* p, int iturb, int nmv) {
   c.iturb = iturb;
   c.nmv = nmv;
   c.motiontype = p->motiontype;
+  // src/setup.f:1232-1234 (itrans = 0), after rey = rey/fsmach
+  c.tkeinf = 1e-6 * (p->fsmach * p->fsmach);
+  c.tomegainf = 5 * p->fsmach / c.rey;
   return c;
 }
 
@@ -302,9 +305,10 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
   c->mesh_case.assign(mesh_case, mesh_case + nmesh);
   c->p = *p;
   derive_counts(p, &c->nq, &c->nv, &c->lamin, &c->iturb);
-  if (!c->lamin && (c->iturb != 1 || p->itrans != 0)) {
+  if (!c->lamin && ((c->iturb != 1 && c->iturb != 2) || p->itrans != 0)) {
     free_case(c);
-    return fail(OT2D_ERR_UNSUPPORTED, "turbulence model iturb=%d itrans=%d: only Spalart-Allmaras (iturb=1)",
+    return fail(OT2D_ERR_UNSUPPORTED,
+                "turbulence model iturb=%d itrans=%d: Spalart-Allmaras (1) and k-omega SST (2) without transition",
                 p->iturb, p->itrans);
   }
   c->nspec = p->timespectral ? p->nspec : 1;
@@ -358,16 +362,27 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
       return fail(OT2D_ERR_ARG, "mesh %d: WENO5 needs jmax, kmax >= 7", im);
     }
     if (ps[mesh_case[im]].ilim == 4) c->weno = 1;
+    if (c->iturb == 2 && !c->lamin)
+      for (int ib = 0; ib < md[im].nbc; ++ib)
+        if (md[im].bc[ib].ibtyp == 5) {
+          // komegabc_wall reads the wall vertices jv = j-1 .. j (src/komegasst.f:1036-1040)
+          DevBC b = fix_bc(md[im].bc[ib], md[im].jmax, md[im].kmax, md[im].jmax + 1, md[im].kmax + 1);
+          if (abs(b.idir) != 2 || b.js < 2 || b.je > md[im].jmax) {
+            free_case(c);
+            return fail(OT2D_ERR_UNSUPPORTED,
+                        "mesh %d: SST wall segment %d must lie along k (ibdir=+-2) on interior columns", im, ib + 1);
+          }
+        }
     if (md[im].nbc < 0 || md[im].nbc > OT2D_MAXBC) { free_case(c); return fail(OT2D_ERR_ARG, "nbc"); }
     if (md[im].kmax - 1 > 256) {
       free_case(c);
       return fail(OT2D_ERR_UNSUPPORTED, "kmax=%d: the LU-SGS sweep holds one row per lane, at most 256 rows",
                   md[im].kmax);
     }
-    if (!c->lamin && md[im].jmax - 1 > SA_LINE_MAX) {
+    if (!c->lamin && (md[im].jmax - 1 > SA_LINE_MAX || md[im].kmax - 1 > SA_LINE_MAX)) {
       free_case(c);
-      return fail(OT2D_ERR_UNSUPPORTED, "jmax=%d: the turbulence line solves hold at most %d cells per line",
-                  md[im].jmax, SA_LINE_MAX);
+      return fail(OT2D_ERR_UNSUPPORTED, "%d x %d: the turbulence line solves hold at most %d cells per line",
+                  md[im].jmax, md[im].kmax, SA_LINE_MAX);
     }
     if (lu_lds_need(md[im].jmax - 1, p->lu_fp32) > 160 * 1024) {
       free_case(c);
@@ -386,7 +401,7 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     A(mb.iblank, mb.n2 * ns);
     mb.lu_per = lu_workspace_bytes(mb.jd, mb.kd, c->lu_rl) / sizeof(double);
     A(mb.lu, mb.lu_per * ns);
-    if (!c->lamin) A(mb.sa, 10 * mb.n2 * ns);
+    if (!c->lamin) A(mb.sa, (c->iturb == 2 ? 20 : 10) * mb.n2 * ns);
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
     ntile_total += tj * tk * ns;
   }
@@ -441,10 +456,20 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
       for (int ib = 0; ib < md[im].nbc; ++ib)
         if (md[im].bc[ib].ibtyp == 5) M.bodyflag = 1;  // src/init.f:59-66
       if (mb.sa) {
-        double* b = mb.sa + 10 * mb.n2 * nsp;
+        const size_t np = c->iturb == 2 ? 20 : 10;
+        double* b = mb.sa + np * mb.n2 * nsp;
         M.sa_vmul = b; M.sa_u = b + mb.n2; M.sa_v = b + 2 * mb.n2; M.sa_sn = b + 3 * mb.n2;
-        M.sa_sj = b + 4 * mb.n2; M.sa_aj = b + 5 * mb.n2; M.sa_cj = b + 6 * mb.n2;
-        M.sa_ak = b + 7 * mb.n2; M.sa_ck = b + 8 * mb.n2; M.sa_bb = b + 9 * mb.n2;
+        if (c->iturb == 1) {
+          M.sa_sj = b + 4 * mb.n2; M.sa_aj = b + 5 * mb.n2; M.sa_cj = b + 6 * mb.n2;
+          M.sa_ak = b + 7 * mb.n2; M.sa_ck = b + 8 * mb.n2; M.sa_bb = b + 9 * mb.n2;
+        } else {
+          M.kw_f1 = b + 4 * mb.n2; M.kw_x = b + 5 * mb.n2; M.kw_prod = b + 6 * mb.n2; M.kw_cd = b + 7 * mb.n2;
+          for (int n = 0; n < 2; ++n) {
+            double* e = b + (8 + 6 * n) * mb.n2;
+            M.kw_s[n] = e; M.kw_a[n] = e + mb.n2; M.kw_b[n] = e + 2 * mb.n2;
+            M.kw_c[n] = e + 3 * mb.n2; M.kw_ak[n] = e + 4 * mb.n2; M.kw_ck[n] = e + 5 * mb.n2;
+          }
+        }
       }
       const int ic = mesh_case[im];
       const ot2d_params* pc = ps + ic;
@@ -713,7 +738,8 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
   } else {
     // vmu_sa reads the state ilu2d leaves (before the q update), src/step.f:189-211
     HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, mk, c->stream));
-    HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), mk, c->stream));
+    if (c->iturb == 2) HIPCHK(launch_sst(c->dM, c->hM.data(), nmi(c), mk, c->stream));
+    else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), mk, c->stream));
     HIPCHK(launch_update(c->dM, c->hM.data(), nmi(c), c->stream));
     mark(mk, OT2D_ST_UPDATE, c->stream);
     HIPCHK(launch_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
@@ -726,7 +752,8 @@ extern "C" int32_t ot2d_turbulence(ot2d_case* c) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
   if (c->lamin) return OT2D_OK;
   if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), nullptr, c->stream));
+  if (c->iturb == 2) HIPCHK(launch_sst(c->dM, c->hM.data(), nmi(c), nullptr, c->stream));
+  else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), nullptr, c->stream));
   return sync(c);
 }
 

@@ -31,6 +31,7 @@ struct DevConst {
   double gamma, gm1, ggm1, einf, pinf, rinf, uinf, vinf, rey, rmue, h;
   double th, qt, epse, Mp, dualtime, timeac, jac_factor, pr, tinf;
   double vnuinf, rotf;
+  double tkeinf, tomegainf;  // SST free stream (src/setup.f:1232-1234)
   double angmax, xac, yac;   // time-spectral pitching (src/setup.f:348-393)
   double fmm, half, ca, sa;  // force normalisation (src/forces.f:27-31, 166-172)
   int invisc, iread, ilim, iconstant_cfl, idual, iprecon, timespectral, ntac;
@@ -56,6 +57,11 @@ struct DevMesh {
   // SA workspace (jd,kd) planes: laminar viscosity, u, v, wall distance and
   // the rows of the two tridiagonal systems (src/samodel2d.f:22-30)
   double *sa_vmul, *sa_u, *sa_v, *sa_sn, *sa_sj, *sa_aj, *sa_cj, *sa_ak, *sa_ck, *sa_bb;
+  // SST workspace (src/komegasst.f:20-41): F1, F2*rho*strain/Re, production,
+  // cross diffusion, and per equation (k, omega) the rows of the j and k
+  // tridiagonal systems sharing one diagonal
+  double *kw_f1, *kw_x, *kw_prod, *kw_cd;
+  double *kw_s[2], *kw_a[2], *kw_b[2], *kw_c[2], *kw_ak[2], *kw_ck[2];
   double* lu;                // LU-SGS sweep records (workspace)
   int lu_rl;                 // rows per lane of the LU-SGS sweeps (1..4)
   int lu_fp32;               // sweep arithmetic: 1 FP32, 0 FP64
@@ -131,5 +137,7 @@ size_t lu_workspace_bytes(int jd, int kd, int rl);
 hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_sst(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);
 size_t lu_lds_need(int ni, int fp32);
 }  // namespace ot2d

@@ -2106,6 +2106,10 @@ __global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict
     M.q[2][i] = rvinf / J;
     M.q[3][i] = C.einf / J;
     if (C.iturb == 1) M.q[4][i] = C.vnuinf;
+    if (C.iturb == 2) {  // src/init.f:246-254
+      M.q[4][i] = C.rinf * C.tkeinf;
+      M.q[5][i] = C.rinf * C.tomegainf;
+    }
     M.turmu[i] = 0.;
     M.iblank[i] = 1;
     M.bt[i] = 1.;

@@ -913,6 +913,11 @@ hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
   return hipGetLastError();
 }
 
+hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(max_cells(hM, nmi), 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
 // vmu_sa for every instance (src/samodel2d.f:2-197, itrans = 0)
 hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st) {
   long long mx = max_cells(hM, nmi);
