@@ -195,6 +195,7 @@ enum {
   OT2D_ST_SA_EDDY,       /* turbc + c_turm            src/samodel2d.f:372-399  */
   OT2D_ST_UPDATE,        /* q += s                    src/step.f:205-211       */
   OT2D_ST_MONITOR,       /* monitor                   src/step.f:332-392       */
+  OT2D_ST_INTERP,        /* do_interpolations         src/connectivity.f:2-189 */
   OT2D_NSTAGE
 };
 
@@ -206,6 +207,62 @@ enum {
  * Instrumentation for benchmarks; no counterpart in the reference. */
 int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, int32_t* done);
 const char* ot2d_stage_name(int32_t stage);
+
+/* ------------------------------------------------------------ overset meshes
+ * Connectivity of overlapping meshes by implicit hole cutting (connect2d,
+ * ihc/src/ihc.f90:252-448, called from src/Turns_overset.f:294-353) and the
+ * per-iteration donor interpolation (do_interpolations,
+ * src/connectivity.f:2-189).  The hole cutting runs once per static overset
+ * configuration on the host; the interpolation runs on the device inside
+ * every step of a case whose connectivity is set. */
+
+/* One ihcbc.<n> file (ihc/src/ihc.f90:147-208).  Types: 1 not a receiver,
+ * 2 blanked, 3 forced receiver, 4 never blanked, 5 wall points (indices count
+ * vertices), 6 = 1 and 4, 7 = 3 and 4.  Other ranges count cells without the
+ * halo; negative values count from the end. */
+typedef struct ot2d_ihcbc {
+  int32_t nbc;
+  int32_t ibtyp[OT2D_MAXBC], ibdir[OT2D_MAXBC];
+  int32_t jbcs[OT2D_MAXBC], jbce[OT2D_MAXBC], kbcs[OT2D_MAXBC], kbce[OT2D_MAXBC];
+} ot2d_ihcbc;
+
+/* One mesh as connect2d reads it (src/Turns_overset.f:302-322): vertices
+ * (jmax x kmax), cell centres and cell areas (1/J) of all (jmax+1) x (kmax+1)
+ * cells including the halo ring, j fastest. */
+typedef struct ot2d_ihc_mesh {
+  int32_t jmax, kmax;
+  const double *xv, *yv;
+  const double *xc, *yc, *vol;
+  ot2d_ihcbc bc;
+} ot2d_ihc_mesh;
+
+typedef struct ot2d_ihc_lists ot2d_ihc_lists;
+
+/* connect2d over nmesh meshes on the host (the reference's module state is
+ * rebuilt on every call).  *out owns the result; free with ot2d_ihc_free. */
+int32_t ot2d_ihc_connect(const ot2d_ihc_mesh* meshes, int32_t nmesh, ot2d_ihc_lists** out);
+/* Counts of mesh im (0-based): fringe cells, donor cells and the 1-based
+ * range of its donors in the concatenated donor list (iisptr, iieptr). */
+int32_t ot2d_ihc_counts(const ot2d_ihc_lists* l, int32_t im, int32_t* nfringe, int32_t* ndonor, int32_t* iisptr,
+                        int32_t* iieptr);
+/* Lists of mesh im in the reference's layout (src/Turns_overset.f:223-226):
+ * iblank (jmax+1)*(kmax+1) after update_halo_iblanks (src/connectivity.f:192-234);
+ * imesh (nfringe x 2) and idonor (ndonor x 2) 1-based solver cell indices,
+ * column-major; ibc (nfringe) 1-based rows of the concatenated donor list;
+ * frac (ndonor x 2).  Any pointer may be NULL. */
+int32_t ot2d_ihc_get(const ot2d_ihc_lists* l, int32_t im, int32_t* iblank, int32_t* imesh, int32_t* ibc,
+                     int32_t* idonor, double* frac);
+int32_t ot2d_ihc_free(ot2d_ihc_lists* l);
+
+/* Installs connectivity for time instance nsp (0-based) of every mesh of the
+ * case: the iblank fields and the interpolation lists of later steps. */
+int32_t ot2d_set_connectivity(ot2d_case* c, int32_t nsp, const ot2d_ihc_lists* l);
+/* Gathers every instance's grid from the device, runs ot2d_ihc_connect with
+ * ihcbc[nmesh] and installs the result (src/Turns_overset.f:294-353).
+ * *nfringe_total (may be NULL) receives the fringe cells of all meshes. */
+int32_t ot2d_connect(ot2d_case* c, const ot2d_ihcbc* ihcbc, int32_t* nfringe_total);
+/* do_interpolations once, outside a step. */
+int32_t ot2d_interpolate(ot2d_case* c);
 
 /* Error text of the last failing call on this thread. */
 const char* ot2d_last_error(void);

@@ -31,6 +31,10 @@ typedef struct or_mesh {
   double *q, *s, *x, *y, *xv, *yv, *xx, *xy, *yx, *yy, *ug, *vg, *ugv, *vgv;
   double *turmu, *tscale, *bt;
   int32_t* iblank;
+  /* overset lists per time instance (reference layout, 1-based indices) */
+  int32_t *nfringe, *ndonor;
+  int32_t **imesh, **ibc, **idonor;
+  double** frac;
 } or_mesh;
 
 struct or_case {
@@ -43,6 +47,7 @@ struct or_case {
   double* Ds;
   double* tspec;
   or_mesh* m;
+  int overset;   /* connectivity installed: interpolate after every step */
 };
 
 /* ---------------------------------------------------------------- access */
@@ -196,6 +201,12 @@ int32_t or_case_destroy(or_case* c) {
     free(M->q); free(M->s); free(M->x); free(M->y); free(M->xx); free(M->xy); free(M->yx);
     free(M->yy); free(M->ug); free(M->vg); free(M->turmu); free(M->tscale); free(M->bt);
     free(M->xv); free(M->yv); free(M->ugv); free(M->vgv); free(M->iblank);
+    if (M->nfringe) {
+      for (int nsp = 0; nsp < c->nspec; ++nsp) {
+        free(M->imesh[nsp]); free(M->ibc[nsp]); free(M->idonor[nsp]); free(M->frac[nsp]);
+      }
+      free(M->nfringe); free(M->ndonor); free(M->imesh); free(M->ibc); free(M->idonor); free(M->frac);
+    }
   }
   free(c->m); free(c->Ds); free(c->tspec); free(c);
   return OT2D_OK;
@@ -2332,6 +2343,109 @@ done:
   return rc;
 }
 
+/* ------------------------------------------------ overset interpolation (a19) */
+/* connectivity of mesh im, instance nsp, in the layout connect2d returns
+   (src/Turns_overset.f:223-226): iblank (jd*kd, after update_halo_iblanks),
+   imesh(nfringe,2), ibc(nfringe), idonor(ndonor,2), frac(ndonor,2) */
+int32_t or_set_mesh_connectivity(or_case* c, int32_t nsp, int32_t im, const int32_t* iblank, int32_t nfringe,
+                                 const int32_t* imesh, const int32_t* ibc, int32_t ndonor, const int32_t* idonor,
+                                 const double* frac) {
+  if (!c || im < 0 || im >= c->nmesh || nsp < 0 || nsp >= c->nspec || nfringe < 0 || ndonor < 0)
+    return OT2D_ERR_ARG;
+  or_mesh* M = &c->m[im];
+  if (!M->nfringe) {
+    M->nfringe = calloc(c->nspec, sizeof(int32_t));
+    M->ndonor = calloc(c->nspec, sizeof(int32_t));
+    M->imesh = calloc(c->nspec, sizeof(int32_t*));
+    M->ibc = calloc(c->nspec, sizeof(int32_t*));
+    M->idonor = calloc(c->nspec, sizeof(int32_t*));
+    M->frac = calloc(c->nspec, sizeof(double*));
+  }
+  free(M->imesh[nsp]); free(M->ibc[nsp]); free(M->idonor[nsp]); free(M->frac[nsp]);
+  M->nfringe[nsp] = nfringe;
+  M->ndonor[nsp] = ndonor;
+  M->imesh[nsp] = malloc(sizeof(int32_t) * (2 * (size_t)nfringe + 1));
+  M->ibc[nsp] = malloc(sizeof(int32_t) * ((size_t)nfringe + 1));
+  M->idonor[nsp] = malloc(sizeof(int32_t) * (2 * (size_t)ndonor + 1));
+  M->frac[nsp] = malloc(sizeof(double) * (2 * (size_t)ndonor + 1));
+  memcpy(M->imesh[nsp], imesh, sizeof(int32_t) * 2 * (size_t)nfringe);
+  memcpy(M->ibc[nsp], ibc, sizeof(int32_t) * (size_t)nfringe);
+  memcpy(M->idonor[nsp], idonor, sizeof(int32_t) * 2 * (size_t)ndonor);
+  memcpy(M->frac[nsp], frac, sizeof(double) * 2 * (size_t)ndonor);
+  memcpy(M->iblank + (size_t)nsp * M->n2, iblank, sizeof(int32_t) * M->n2);
+  c->overset = 1;
+  return OT2D_OK;
+}
+
+/* do_interpolations, src/connectivity.f:2-189 */
+static void do_interpolations(or_case* c) {
+  double onefourth = 1. / 4;
+  for (int nsp = 0; nsp < c->nspec; ++nsp) {
+    int bcdim = 0;
+    for (int im = 0; im < c->nmesh; ++im) bcdim += c->m[im].nfringe ? c->m[im].ndonor[nsp] : 0;
+    double* qbc = calloc((size_t)(bcdim > 0 ? bcdim : 1) * c->nv, sizeof(double));
+    int iisptr = 1;
+    for (int im = 0; im < c->nmesh; ++im) {
+      or_mesh* M = &c->m[im];
+      if (!M->nfringe) continue;
+      const int nd = M->ndonor[nsp];
+      const double* qj = QA(M, nsp, c->nq);
+      for (int id = 1; id <= nd; ++id) {
+        int ii = M->idonor[nsp][id - 1], jj = M->idonor[nsp][nd + id - 1];
+        int iim = ii - 1, jjm = jj - 1, iip = ii + 1, jjp = jj + 1;
+        double f1 = M->frac[nsp][id - 1], f2 = M->frac[nsp][nd + id - 1];
+        double djm1 = 1. + f1, dkm1 = 1. + f2, dj0 = f1, dk0 = f2, djp1 = 1. - f1, dkp1 = 1. - f2;
+        double w1 = (dj0 * djp1 * dk0 * dkp1) * onefourth;
+        double w2 = -2 * (djm1 * djp1 * dk0 * dkp1) * onefourth;
+        double w3 = -(djm1 * dj0 * dk0 * dkp1) * onefourth;
+        double w4 = -2 * (dj0 * djp1 * dkm1 * dkp1) * onefourth;
+        double w5 = 4 * (djm1 * djp1 * dkm1 * dkp1) * onefourth;
+        double w6 = 2 * (djm1 * dj0 * dkm1 * dkp1) * onefourth;
+        double w7 = -(dj0 * djp1 * dkm1 * dk0) * onefourth;
+        double w8 = 2 * (djm1 * djp1 * dkm1 * dk0) * onefourth;
+        double w9 = (djm1 * dj0 * dkm1 * dk0) * onefourth;
+        size_t a = IX(M, iim, jjm), b = IX(M, ii, jjm), cc = IX(M, iip, jjm);
+        size_t d = IX(M, iim, jj), e = IX(M, ii, jj), f = IX(M, iip, jj);
+        size_t g = IX(M, iim, jjp), h = IX(M, ii, jjp), k = IX(M, iip, jjp);
+        double* row = qbc + (size_t)(iisptr - 1 + id - 1) * c->nv;
+        for (int n = 1; n <= c->nmv; ++n) {
+          const double* q = QA(M, nsp, n);
+          row[n - 1] = w1 * q[a] * qj[a] + w2 * q[b] * qj[b] + w3 * q[cc] * qj[cc] + w4 * q[d] * qj[d] +
+                       w5 * q[e] * qj[e] + w6 * q[f] * qj[f] + w7 * q[g] * qj[g] + w8 * q[h] * qj[h] +
+                       w9 * q[k] * qj[k];
+        }
+        for (int n = c->nmv + 1; n <= c->nv; ++n) {
+          const double* q = QA(M, nsp, n);
+          row[n - 1] = w1 * q[a] + w2 * q[b] + w3 * q[cc] + w4 * q[d] + w5 * q[e] + w6 * q[f] + w7 * q[g] +
+                       w8 * q[h] + w9 * q[k];
+        }
+      }
+      iisptr += nd;
+    }
+    for (int im = 0; im < c->nmesh; ++im) {
+      or_mesh* M = &c->m[im];
+      if (!M->nfringe) continue;
+      const int nf = M->nfringe[nsp];
+      double* qj = QA(M, nsp, c->nq);
+      for (int id = 1; id <= nf; ++id) {
+        int j = M->ibc[nsp][id - 1];
+        int ii = M->imesh[nsp][id - 1], jj = M->imesh[nsp][nf + id - 1];
+        size_t i = IX(M, ii, jj);
+        const double* row = qbc + (size_t)(j - 1) * c->nv;
+        for (int n = 1; n <= c->nmv; ++n) QA(M, nsp, n)[i] = row[n - 1] / qj[i];
+        for (int n = c->nmv + 1; n <= c->nv; ++n) QA(M, nsp, n)[i] = row[n - 1];
+      }
+    }
+    free(qbc);
+  }
+}
+
+int32_t or_interpolate(or_case* c) {
+  if (!c) return OT2D_ERR_ARG;
+  if (c->overset) do_interpolations(c);
+  return OT2D_OK;
+}
+
 /* compute_residual, src/step.f:221-300 */
 int32_t or_compute_residual(or_case* c) {
   for (int im = 0; im < c->nmesh; ++im) {
@@ -2781,6 +2895,8 @@ int32_t or_step(or_case* c, double* norms, int32_t* argmax) {
     }
     if (ms) { status = OT2D_MSTOP; break; }
   }
+  /* after every mesh has stepped (src/Turns_overset.f:500-503) */
+  if (status == OT2D_OK && c->overset) do_interpolations(c);
   return status;
 }
 

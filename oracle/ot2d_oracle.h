@@ -37,6 +37,10 @@ int32_t or_monitor(or_case* c, int32_t* mstop);
 int32_t or_step(or_case* c, double* norms, int32_t* argmax);
 int32_t or_iterate(or_case* c, int32_t niter, double* hist, int32_t* done);
 int32_t or_forces(or_case* c, int32_t im, int32_t nsp, double out[6]);
+int32_t or_set_mesh_connectivity(or_case* c, int32_t nsp, int32_t im, const int32_t* iblank, int32_t nfringe,
+                                 const int32_t* imesh, const int32_t* ibc, int32_t ndonor, const int32_t* idonor,
+                                 const double* frac);
+int32_t or_interpolate(or_case* c);
 int32_t or_sizes(const ot2d_params* p, const ot2d_mesh_desc* m, int32_t* jd, int32_t* kd,
                  int32_t* nq, int32_t* nv);
 int32_t or_default_params(ot2d_params* p);

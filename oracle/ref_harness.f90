@@ -21,13 +21,18 @@
 !   OT2D_ISTEP0  istep0 to start from       (default 0)
 !   OT2D_OUT     output directory           (default .)
 !   OT2D_DUMPAT  in run mode also dump stage arrays at this iteration (0: none)
+! With several meshes (num_grids > 1) the connectivity is found as the main
+! program does (connect2d, update_halo_iblanks) and do_interpolations runs
+! after every iteration.
 ! Outputs (raw little-endian float64 / int32, Fortran order):
 !   norms.txt          it rsum resrho resmax jmax kmax nmax   (full precision)
 !   forces.txt         it cl cd cm cpower (every npnorm iterations, mesh 1)
 !   <stage>_m<im>.bin  per-stage arrays for mesh im
 !   grid_m<im>.bin     x,y,xx,xy,yx,yy,q(:,:,nq) after initialisation
+!   conn_s<nsp>_m<im>.txt, imesh/ibc/idonor/frac_s<nsp>_m<im>.bin  overset lists
 program ref_harness
   use params_global
+  use ihc
   implicit none
   real, allocatable :: s(:), q(:), qtn(:), qtnm1(:), qnewt(:), ss(:)
   real, allocatable :: x(:), y(:), xv(:), yv(:), tscale(:), bt(:)
@@ -42,6 +47,12 @@ program ref_harness
   character(len=40) :: bcfile, fprefix
   logical :: dump
   real :: cfx, cfy, cmt, clt, cdt, cpw
+  ! overset connectivity, as the main program holds it (src/Turns_overset.f:209-237)
+  real, allocatable :: xgl(:,:,:), ygl(:,:,:), volg(:,:,:), xglv(:,:,:), yglv(:,:,:), frac(:,:,:,:)
+  integer, allocatable :: ibgl(:,:,:), imesh(:,:,:,:), idonor(:,:,:,:), ibc(:,:,:)
+  integer, allocatable :: nfringe(:,:), ndonor(:,:), iisptr(:,:), iieptr(:,:), jgmxv(:), kgmxv(:)
+  character(len=40), allocatable :: ihcbcfile(:)
+  integer :: Nj, Nk, idsize, nsp, j, k
 
   nmesh = 6
   allocate(jgmx(nmesh), kgmx(nmesh))
@@ -74,6 +85,7 @@ program ref_harness
   call getenv_str("OT2D_NSTEPS", "-1", buf); read(buf, *) nit
   call getenv_str("OT2D_DUMPAT", "0", buf); read(buf, *) dumpat
   if (nit < 0) nit = nsteps
+  if (num_grids > 1) call connectivity()
 
   if (len_trim(qin) > 0) then
     open(unit=91, file=trim(qin), access="stream", form="unformatted", status="old", iostat=ios)
@@ -125,6 +137,9 @@ program ref_harness
           goto 19
         end if
       end do
+      ! src/Turns_overset.f:500-503
+      if (nmesh > 1) call do_interpolations(q, jgmx, kgmx, ibc, imesh, idonor, frac, &
+           nfringe, ndonor, iisptr, iieptr, idsize, iqdim, nmesh)
     end do
     if (mod(it, npnorm) == 0) then
       im = 1
@@ -146,6 +161,89 @@ program ref_harness
   end do
 
 contains
+
+  ! connectivity set-up of the main program (src/Turns_overset.f:209-237,
+  ! 245-256, 296-353), then the lists of every mesh are written out:
+  ! conn_m<im>.txt holds nfringe ndonor iisptr iieptr, the .bin files
+  ! imesh(nfringe,2), ibc(nfringe), idonor(ndonor,2), frac(ndonor,2)
+  subroutine connectivity()
+    integer :: jj
+    character(len=40) :: istr
+    Nj = jgmx(1); Nk = kgmx(1); idsize = jgmx(1)*kgmx(1)
+    do im = 2, nmesh
+      if (Nj <= jgmx(im)) Nj = jgmx(im)
+      if (Nk <= kgmx(im)) Nk = kgmx(im)
+      if (idsize <= jgmx(im)*kgmx(im)) idsize = jgmx(im)*kgmx(im)
+    end do
+    allocate(xgl(Nj,Nk,nmesh), ygl(Nj,Nk,nmesh), volg(Nj,Nk,nmesh), ibgl(Nj,Nk,nmesh))
+    allocate(imesh(idsize,2,nmesh,nspec), idonor(idsize,2,nmesh,nspec))
+    allocate(frac(idsize,2,nmesh,nspec), ibc(idsize,nmesh,nspec))
+    allocate(nfringe(nmesh,nspec), ndonor(nmesh,nspec), iisptr(nmesh,nspec), iieptr(nmesh,nspec))
+    allocate(ihcbcfile(nmesh), jgmxv(nmesh), kgmxv(nmesh))
+    Nj = jgmx(1)-nadd; Nk = kgmx(1)-nadd
+    do im = 2, nmesh
+      if (Nj <= jgmx(im)-nadd) Nj = jgmx(im)-nadd
+      if (Nk <= kgmx(im)-nadd) Nk = kgmx(im)-nadd
+    end do
+    allocate(xglv(Nj,Nk,nmesh), yglv(Nj,Nk,nmesh))
+    do im = 1, nmesh
+      call set_pointers_globals(im, ipointer, ig, igq, igs, igv, igb, jd, kd, jgmx, kgmx, nmesh)
+      jgmxv(im) = jmax
+      kgmxv(im) = kmax
+      write(istr, *) im-1
+      istr = adjustl(istr)
+      ihcbcfile(im) = 'ihcbc.'//trim(istr)
+    end do
+    do nsp = 1, nspec
+      do im = 1, nmesh
+        call set_pointers_globals(im, ipointer, ig, igq, igs, igv, igb, jd, kd, jgmx, kgmx, nmesh)
+        do k = 1, kd
+          do j = 1, jd
+            xgl(j,k,im) = x(ig-1 + jd*kd*(nsp-1) + jd*(k-1) + j)
+            ygl(j,k,im) = y(ig-1 + jd*kd*(nsp-1) + jd*(k-1) + j)
+            volg(j,k,im) = 1./q(igq - 1 + jd*kd*nspec*(nq-1) + jd*kd*(nsp-1) + jd*(k-1) + j)
+          end do
+        end do
+        do k = 1, kmax
+          do j = 1, jmax
+            xglv(j,k,im) = xv(igv-1 + jmax*kmax*(nsp-1) + jmax*(k-1) + j)
+            yglv(j,k,im) = yv(igv-1 + jmax*kmax*(nsp-1) + jmax*(k-1) + j)
+          end do
+        end do
+      end do
+      call connect2d(xglv, yglv, xgl, ygl, volg, ibgl, &
+           idonor(:,:,:,nsp), frac(:,:,:,nsp), imesh(:,:,:,nsp), &
+           ibc(:,:,nsp), iisptr(:,nsp), iieptr(:,nsp), ndonor(:,nsp), &
+           nfringe(:,nsp), jgmxv, kgmxv, nhalo, nmesh, ihcbcfile)
+      do im = 1, nmesh
+        call set_pointers_globals(im, ipointer, ig, igq, igs, igv, igb, jd, kd, jgmx, kgmx, nmesh)
+        do k = 1, kd
+          do j = 1, jd
+            iblank(ig-1 + jd*kd*(nsp-1) + jd*(k-1) + j) = ibgl(j,k,im)
+          end do
+        end do
+      end do
+    end do
+    do im = 1, nmesh
+      call set_pointers_globals(im, ipointer, ig, igq, igs, igv, igb, jd, kd, jgmx, kgmx, nmesh)
+      call update_halo_iblanks(iblank(ig), jd, kd)
+    end do
+    do im = 1, nmesh
+      do nsp = 1, nspec
+        write(istr, '(a,i0,a,i0,a)') "_s", nsp, "_m", im, ".txt"
+        open(unit=93, file=trim(outdir)//"/conn"//trim(istr), status="replace")
+        write(93, '(4i10)') nfringe(im,nsp), ndonor(im,nsp), iisptr(im,nsp), iieptr(im,nsp)
+        close(93)
+        jj = nfringe(im,nsp)
+        write(istr, '(a,i0)') "_s", nsp
+        call dump_i("imesh"//trim(istr), im, reshape(imesh(1:jj,1:2,im,nsp), (/2*jj/)), 2*jj)
+        call dump_i("ibc"//trim(istr), im, ibc(1:jj,im,nsp), jj)
+        jj = ndonor(im,nsp)
+        call dump_i("idonor"//trim(istr), im, reshape(idonor(1:jj,1:2,im,nsp), (/2*jj/)), 2*jj)
+        call dump_r("frac"//trim(istr), im, reshape(frac(1:jj,1:2,im,nsp), (/2*jj/)), 2*jj)
+      end do
+    end do
+  end subroutine
 
   subroutine getenv_str(name, default, val)
     character(len=*), intent(in) :: name, default

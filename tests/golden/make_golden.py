@@ -11,6 +11,7 @@ how they were made:
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -128,6 +129,53 @@ def run_driver_case():
     print("driver_sa_41x21 norms", norms[-1])
 
 
+# Overset fixtures (num_grids = 2): connectivity by the reference's connect2d
+# and, for the small pair, a trajectory with do_interpolations after every
+# iteration.  name: (near-body jmax, kmax, background jmax, kmax, deck, nsteps)
+SST_WENO = dict(SA, iturb=2, ilim=4, fsmach=0.3, alfa=6.0, rey=3.0e6, dt=5.0)
+OVERSET_CASES = {
+    "overset_97x25_61x61": (97, 25, 61, 61, SST_WENO, 10),
+    "ihc_385x97_401x401": (385, 97, 401, 401, SST_WENO, 0),
+}
+
+
+def run_overset_case(name, nj, nk, bj, bk, deck, nsteps):
+    meshes = grid.airfoil_overset(near=(nj, nk), back=(bj, bk), ds_wall=1e-4)
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_overset_case(d, meshes, nsteps=max(nsteps, 1), nrest=10**6, npnorm=1, **deck)
+        out = os.path.join(d, "out")
+        os.makedirs(out)
+        env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps), **ZERO_MALLOC)
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([HARNESS], cwd=d, stdin=fin, env=env, check=True, stdout=subprocess.DEVNULL)
+        arrays = {}
+        for im in (1, 2):
+            m = meshes[im - 1]
+            if nsteps:
+                arrays[f"xv{im}"], arrays[f"yv{im}"] = m.xv, m.yv
+            else:
+                # large pair: the grid is regenerated by grid.airfoil_overset,
+                # pinned by the digest of its vertices
+                h = hashlib.sha256(np.ascontiguousarray(m.xv).tobytes() + np.ascontiguousarray(m.yv).tobytes())
+                arrays[f"grid_sha{im}"] = np.asarray(h.hexdigest())
+            arrays[f"bc{im}"] = np.array([[b.ibtyp, b.ibdir, b.jbcs, b.jbce, b.kbcs, b.kbce] for b in m.bc])
+            arrays[f"ihc{im}"] = np.array([[b.ibtyp, b.ibdir, b.jbcs, b.jbce, b.kbcs, b.kbce] for b in m.ihc],
+                                          dtype=np.int64).reshape(-1, 6)
+            arrays[f"counts{im}"] = np.loadtxt(os.path.join(out, f"conn_s1_m{im}.txt"), dtype=np.int64)
+            arrays[f"iblank{im}"] = np.fromfile(os.path.join(out, f"iblank_m{im}.bin"), dtype=np.int32)
+            for key, dt in (("imesh", np.int32), ("ibc", np.int32), ("idonor", np.int32), ("frac", np.float64)):
+                arrays[f"{key}{im}"] = np.fromfile(os.path.join(out, f"{key}_s1_m{im}.bin"), dtype=dt)
+            if nsteps:
+                arrays[f"qfinal{im}"] = np.fromfile(os.path.join(out, f"qfinal_m{im}.bin"))
+                arrays[f"turmufinal{im}"] = np.fromfile(os.path.join(out, f"turmufinal_m{im}.bin"))
+        if nsteps:
+            arrays["norms"] = np.loadtxt(os.path.join(out, "norms.txt"), comments="#", ndmin=2)
+    deck_arr = {f"deck_{k}": np.asarray(float(v)) for k, v in deck.items()}
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), nsteps=np.asarray(nsteps), jtail1=np.asarray(meshes[0].jtail1),
+                        **arrays, **deck_arr)
+    print(name, "fringes", arrays["counts1"][0], arrays["counts2"][0])
+
+
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
@@ -140,3 +188,6 @@ if __name__ == "__main__":
             run_ts_case(name, *spec)
     if not only or "driver_sa_41x21" in only:
         run_driver_case()
+    for name, spec in OVERSET_CASES.items():
+        if not only or name in only:
+            run_overset_case(name, *spec)

@@ -60,6 +60,27 @@ class Params(C.Structure):
         return {n: getattr(self, n) for n in _PARAM_INTS + _PARAM_DBLS + ["lu_fp32"]}
 
 
+class IHCBC(C.Structure):
+    """One ``ihcbc.<n>`` file (``ot2d_ihcbc``)."""
+    _fields_ = [("nbc", C.c_int32)] + [(n, C.c_int32 * MAXBC) for n in
+                                        ("ibtyp", "ibdir", "jbcs", "jbce", "kbcs", "kbce")]
+
+
+def ihcbc(segs: Sequence) -> IHCBC:
+    b = IHCBC()
+    b.nbc = len(segs)
+    for i, s in enumerate(segs):
+        for n in ("ibtyp", "ibdir", "jbcs", "jbce", "kbcs", "kbce"):
+            getattr(b, n)[i] = getattr(s, n)
+    return b
+
+
+class IHCMesh(C.Structure):
+    _fields_ = [("jmax", C.c_int32), ("kmax", C.c_int32),
+                ("xv", C.c_void_p), ("yv", C.c_void_p), ("xc", C.c_void_p), ("yc", C.c_void_p),
+                ("vol", C.c_void_p), ("bc", IHCBC)]
+
+
 def mesh_desc(jmax: int, kmax: int, segs: Sequence, irot_dir: int = 1,
               xcen: float = 0.0, ycen: float = 0.0) -> MeshDesc:
     md = MeshDesc()
@@ -110,6 +131,18 @@ def bind(lib: C.CDLL, prefix: str):
         "batch_create": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, P(C.c_void_p)]),
         "batch_status": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "batch_retire": (C.c_int32, [C.c_void_p, C.c_int32]),
+        "connect": (C.c_int32, [C.c_void_p, C.c_void_p, P(C.c_int32)]),
+        "interpolate": (C.c_int32, [C.c_void_p]),
+        "set_connectivity": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
+        "ihc_connect": (C.c_int32, [C.c_void_p, C.c_int32, P(C.c_void_p)]),
+        "ihc_counts": (C.c_int32, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                   P(C.c_int32)]),
+        "ihc_get": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p]),
+        "ihc_free": (C.c_int32, [C.c_void_p]),
+        # oracle only: connectivity of one mesh as plain arrays
+        "set_mesh_connectivity": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _IP, C.c_int32, _IP, _IP,
+                                              C.c_int32, _IP, _DP]),
     }
     fns = {}
     for name, (res, args) in list(sig.items()) + list(optional.items()):
@@ -122,7 +155,7 @@ def bind(lib: C.CDLL, prefix: str):
     return fns
 
 
-NSTAGE = 20  # OT2D_NSTAGE
+NSTAGE = 21  # OT2D_NSTAGE
 
 
 class CaseError(RuntimeError):
@@ -185,6 +218,32 @@ class CaseAPI:
             self.close()
         except Exception:
             pass
+
+    # ---------------------------------------------------------- overset
+    def connect(self, ihc_rows: Sequence[Sequence]) -> int:
+        """Hole cutting and donor search over the case's meshes (one list of
+        ihcbc rows per mesh); returns the fringe cells of all meshes."""
+        arr = (IHCBC * len(ihc_rows))(*[ihcbc(r) for r in ihc_rows])
+        n = C.c_int32()
+        self._check(self._fns["connect"](self.h, arr, C.byref(n)), "connect")
+        return n.value
+
+    def set_connectivity(self, nsp: int, lists: "IHCLists"):
+        self._check(self._fns["set_connectivity"](self.h, nsp, lists.h), "set_connectivity")
+
+    def set_mesh_connectivity(self, nsp: int, im: int, iblank, imesh, ibc, idonor, frac):
+        """Oracle form: one mesh's iblank and lists in the reference layout."""
+        iblank = np.ascontiguousarray(np.asarray(iblank).ravel(order="F"), dtype=np.int32)
+        imesh = np.ascontiguousarray(imesh, dtype=np.int32)
+        ibc = np.ascontiguousarray(ibc, dtype=np.int32)
+        idonor = np.ascontiguousarray(idonor, dtype=np.int32)
+        frac = np.ascontiguousarray(frac, dtype=np.float64)
+        self._check(self._fns["set_mesh_connectivity"](self.h, nsp, im, iblank, len(ibc), imesh, ibc,
+                                                       len(idonor) // 2, idonor, frac),
+                    "set_mesh_connectivity")
+
+    def interpolate(self):
+        self._check(self._fns["interpolate"](self.h), "interpolate")
 
     # ------------------------------------------------------------ state
     def set_grid(self, im: int, xv: np.ndarray, yv: np.ndarray):
@@ -317,3 +376,68 @@ def default_params(lib: C.CDLL, prefix: str, **overrides) -> Params:
     f(C.byref(p))
     p.update(**overrides)
     return p
+
+
+# ------------------------------------------------------------------ overset
+class IHCLists:
+    """Result of ``ot2d_ihc_connect`` (owned; freed with the object)."""
+
+    def __init__(self, lib, prefix, handle, dims, errfn=None):
+        self._fns = bind(lib, prefix)
+        self.h = handle
+        self.dims = dims
+        self._err = errfn
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._fns["ihc_free"](self.h)
+            self.h = None
+
+    def counts(self, im):
+        a = [C.c_int32() for _ in range(4)]
+        rc = self._fns["ihc_counts"](self.h, im, *[C.byref(x) for x in a])
+        if rc:
+            raise CaseError(rc, "ihc_counts", self._err() if self._err else "")
+        return tuple(x.value for x in a)
+
+    def get(self, im):
+        """iblank (jmax+1, kmax+1) and the lists of mesh im, as the
+        reference lays them out."""
+        jmax, kmax = self.dims[im]
+        nf, nd, _, _ = self.counts(im)
+        ib = np.zeros((jmax + 1) * (kmax + 1), dtype=np.int32)
+        imesh = np.zeros(2 * nf, dtype=np.int32)
+        ibc = np.zeros(nf, dtype=np.int32)
+        idonor = np.zeros(2 * nd, dtype=np.int32)
+        frac = np.zeros(2 * nd, dtype=np.float64)
+        rc = self._fns["ihc_get"](self.h, im, ib.ctypes.data, imesh.ctypes.data, ibc.ctypes.data,
+                                  idonor.ctypes.data, frac.ctypes.data)
+        if rc:
+            raise CaseError(rc, "ihc_get", self._err() if self._err else "")
+        return dict(iblank=ib, imesh=imesh, ibc=ibc, idonor=idonor, frac=frac)
+
+
+def ihc_connect(lib, prefix, meshes, errfn=None) -> IHCLists:
+    """Hole cutting and donor search over ``meshes``: dicts with ``xv, yv``
+    (jmax, kmax), ``xc, yc, vol`` ((jmax+1, kmax+1), halo included) and
+    ``ihc`` (rows of the mesh's ihcbc file)."""
+    fns = bind(lib, prefix)
+    arr = (IHCMesh * len(meshes))()
+    keep = []
+    dims = []
+    for i, m in enumerate(meshes):
+        xv, yv = (np.asfortranarray(m[k], dtype=np.float64) for k in ("xv", "yv"))
+        xc, yc, vol = (np.asfortranarray(m[k], dtype=np.float64) for k in ("xc", "yc", "vol"))
+        jmax, kmax = xv.shape
+        assert xc.shape == (jmax + 1, kmax + 1), (xc.shape, jmax, kmax)
+        keep += [xv, yv, xc, yc, vol]
+        arr[i].jmax, arr[i].kmax = jmax, kmax
+        arr[i].xv, arr[i].yv = xv.ctypes.data, yv.ctypes.data
+        arr[i].xc, arr[i].yc, arr[i].vol = xc.ctypes.data, yc.ctypes.data, vol.ctypes.data
+        arr[i].bc = ihcbc(m.get("ihc", []))
+        dims.append((jmax, kmax))
+    h = C.c_void_p()
+    rc = fns["ihc_connect"](arr, len(meshes), C.byref(h))
+    if rc:
+        raise CaseError(rc, "ihc_connect", errfn() if errfn else "")
+    return IHCLists(lib, prefix, h, dims, errfn)

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libot2d.so")
-SOURCES = ["ot2d_kernels.hip", "ot2d_turb.hip", "ot2d_sst.hip", "ot2d_api.hip"]
+SOURCES = ["ot2d_kernels.hip", "ot2d_turb.hip", "ot2d_sst.hip", "ot2d_overset.hip", "ot2d_ihc.cpp",
+           "ot2d_api.hip"]
 ARCH = os.environ.get("OT2D_ARCH", "gfx950")
 
 # -ffp-contract=off keeps the FP64 path operand-for-operand with the

@@ -11,6 +11,7 @@
 
 #include "../../include/ot2d.h"
 #include "ot2d_dev.h"
+#include "ot2d_ihc.h"
 
 using namespace ot2d;
 
@@ -85,6 +86,22 @@ struct ot2d_case {
   // captured step
   hipGraphExec_t graph = nullptr;
   int graph_valid = 0;
+  // overset interpolation lists per time instance (host) and all instances
+  // concatenated on the device
+  struct HDonor { int mi, jj, kk, pad; double f1, f2; };
+  struct HFringe { int mi, jj, kk, src; };
+  std::vector<std::vector<HDonor>> don;
+  std::vector<std::vector<HFringe>> fr;
+  void* d_don = nullptr;
+  void* d_fr = nullptr;
+  double* d_qbc = nullptr;
+  int ndon = 0, nfr = 0;
+};
+
+// result of ot2d_ihc_connect
+struct ot2d_ihc_lists {
+  ot2d::ihc::Result r;
+  std::vector<int> jmax, kmax;
 };
 
 // ------------------------------------------------------------- constants
@@ -245,6 +262,9 @@ static void dfree(void* p) {
 }
 
 static void free_case(ot2d_case* c) {
+  dfree(c->d_don);
+  dfree(c->d_fr);
+  dfree(c->d_qbc);
   if (!c) return;
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (auto& mb : c->m) {
@@ -374,9 +394,9 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
           }
         }
     if (md[im].nbc < 0 || md[im].nbc > OT2D_MAXBC) { free_case(c); return fail(OT2D_ERR_ARG, "nbc"); }
-    if (md[im].kmax - 1 > 256) {
+    if (md[im].kmax - 1 > 512) {
       free_case(c);
-      return fail(OT2D_ERR_UNSUPPORTED, "kmax=%d: the LU-SGS sweep holds one row per lane, at most 256 rows",
+      return fail(OT2D_ERR_UNSUPPORTED, "kmax=%d: the FP64 LU-SGS sweep holds at most 8 rows per lane, 512 rows",
                   md[im].kmax);
     }
     if (!c->lamin && (md[im].jmax - 1 > SA_LINE_MAX || md[im].kmax - 1 > SA_LINE_MAX)) {
@@ -745,6 +765,11 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
     HIPCHK(launch_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
     mark(mk, OT2D_ST_MONITOR, c->stream);
   }
+  // overset meshes exchange fringe values (src/Turns_overset.f:500-503)
+  if (c->ndon > 0 || c->nfr > 0) {
+    HIPCHK(launch_interp(c->dM, c->d_don, c->ndon, c->d_fr, c->nfr, c->d_qbc, c->nv, c->stream));
+    mark(mk, OT2D_ST_INTERP, c->stream);
+  }
   return OT2D_OK;
 }
 
@@ -880,7 +905,7 @@ extern "C" int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out
 static const char* const kStageNames[OT2D_NSTAGE] = {
     "begin",   "time_step", "bc",         "residual",    "visrhs",    "tsrhs",    "norms",
     "rbc",     "lu_prep",   "lu_forward", "lu_backward", "lu_finish", "sa_prim",  "sa_bc",
-    "sa_rhs",  "sa_solve_j", "sa_solve_k", "sa_eddy",    "update",    "monitor"};
+    "sa_rhs",  "sa_solve_j", "sa_solve_k", "sa_eddy",    "update",    "monitor",  "interp"};
 
 extern "C" const char* ot2d_stage_name(int32_t stage) {
   return (stage >= 0 && stage < OT2D_NSTAGE) ? kStageNames[stage] : "";
@@ -963,5 +988,210 @@ extern "C" int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase) {
   if (!c || icase < 0 || icase >= c->ncase) return fail(OT2D_ERR_ARG, "bad case index %d", icase);
   static const int retired = 4;
   HIPCHK(hipMemcpyAsync(c->d_status + icase, &retired, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+// ============================================================ overset meshes
+static void halo_iblank(std::vector<int>& ib, int jd, int kd) {
+  // update_halo_iblanks (src/connectivity.f:192-234), one halo layer
+  for (int k = 1; k <= kd; ++k) {
+    ib[(size_t)(k - 1) * jd] = ib[1 + (size_t)(k - 1) * jd];
+    ib[(jd - 1) + (size_t)(k - 1) * jd] = ib[(jd - 2) + (size_t)(k - 1) * jd];
+  }
+  for (int j = 1; j <= jd; ++j) {
+    ib[j - 1] = ib[(j - 1) + (size_t)jd];
+    ib[(j - 1) + (size_t)(kd - 1) * jd] = ib[(j - 1) + (size_t)(kd - 2) * jd];
+  }
+}
+
+extern "C" int32_t ot2d_ihc_connect(const ot2d_ihc_mesh* meshes, int32_t nmesh, ot2d_ihc_lists** out) {
+  if (!meshes || !out || nmesh < 1) return fail(OT2D_ERR_ARG, "ot2d_ihc_connect: null argument or no mesh");
+  *out = nullptr;
+  std::vector<ihc::MeshIn> in(nmesh);
+  for (int n = 0; n < nmesh; ++n) {
+    const ot2d_ihc_mesh& m = meshes[n];
+    if (m.jmax < 3 || m.kmax < 3 || !m.xv || !m.yv || !m.xc || !m.yc || !m.vol)
+      return fail(OT2D_ERR_ARG, "ot2d_ihc_connect: mesh %d incomplete", n);
+    if (m.bc.nbc < 0 || m.bc.nbc > OT2D_MAXBC) return fail(OT2D_ERR_ARG, "mesh %d: nbc=%d", n, m.bc.nbc);
+    ihc::MeshIn& d = in[n];
+    d.jmax = m.jmax; d.kmax = m.kmax;
+    d.xv = m.xv; d.yv = m.yv; d.xc = m.xc; d.yc = m.yc; d.vol = m.vol;
+    d.nbc = m.bc.nbc;
+    for (int b = 0; b < m.bc.nbc; ++b) {
+      d.ibtyp[b] = m.bc.ibtyp[b];
+      d.jbcs[b] = m.bc.jbcs[b]; d.jbce[b] = m.bc.jbce[b];
+      d.kbcs[b] = m.bc.kbcs[b]; d.kbce[b] = m.bc.kbce[b];
+      // every range must index cells 0..jmax, 0..kmax once resolved
+      int lim_j = m.bc.ibtyp[b] == 5 ? m.jmax : m.jmax - 1, lim_k = m.bc.ibtyp[b] == 5 ? m.kmax : m.kmax - 1;
+      int js = d.jbcs[b] < 0 ? lim_j + d.jbcs[b] + 1 : d.jbcs[b];
+      int je = d.jbce[b] < 0 ? lim_j + d.jbce[b] + 1 : d.jbce[b];
+      int ks = d.kbcs[b] < 0 ? lim_k + d.kbcs[b] + 1 : d.kbcs[b];
+      int ke = d.kbce[b] < 0 ? lim_k + d.kbce[b] + 1 : d.kbce[b];
+      if (js < 0 || ks < 0 || (je >= js && je > m.jmax) || (ke >= ks && ke > m.kmax))
+        return fail(OT2D_ERR_ARG, "mesh %d ihcbc row %d indexes outside the mesh", n, b + 1);
+    }
+  }
+  ot2d_ihc_lists* l = new ot2d_ihc_lists();
+  if (ihc::connect(in.data(), nmesh, 1, &l->r) != 0) {
+    delete l;
+    return fail(OT2D_ERR_ARG, "ot2d_ihc_connect: mesh too small");
+  }
+  for (int n = 0; n < nmesh; ++n) {
+    l->jmax.push_back(meshes[n].jmax);
+    l->kmax.push_back(meshes[n].kmax);
+    halo_iblank(l->r.mesh[n].iblank, meshes[n].jmax + 1, meshes[n].kmax + 1);
+  }
+  *out = l;
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_ihc_counts(const ot2d_ihc_lists* l, int32_t im, int32_t* nfringe, int32_t* ndonor,
+                                   int32_t* iisptr, int32_t* iieptr) {
+  if (!l || im < 0 || im >= (int)l->r.mesh.size()) return fail(OT2D_ERR_ARG, "ot2d_ihc_counts: bad lists or mesh");
+  const ihc::MeshOut& m = l->r.mesh[im];
+  if (nfringe) *nfringe = (int32_t)m.ibc.size();
+  if (ndonor) *ndonor = (int32_t)(m.idonor.size() / 2);
+  if (iisptr) *iisptr = m.iisptr;
+  if (iieptr) *iieptr = m.iieptr;
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_ihc_get(const ot2d_ihc_lists* l, int32_t im, int32_t* iblank, int32_t* imesh, int32_t* ibc,
+                                int32_t* idonor, double* frac) {
+  if (!l || im < 0 || im >= (int)l->r.mesh.size()) return fail(OT2D_ERR_ARG, "ot2d_ihc_get: bad lists or mesh");
+  const ihc::MeshOut& m = l->r.mesh[im];
+  const size_t nf = m.ibc.size(), nd = m.idonor.size() / 2;
+  if (iblank) memcpy(iblank, m.iblank.data(), m.iblank.size() * sizeof(int32_t));
+  for (size_t i = 0; i < nf; ++i) {
+    if (imesh) { imesh[i] = m.imesh[2 * i]; imesh[nf + i] = m.imesh[2 * i + 1]; }
+    if (ibc) ibc[i] = m.ibc[i];
+  }
+  for (size_t i = 0; i < nd; ++i) {
+    if (idonor) { idonor[i] = m.idonor[2 * i]; idonor[nd + i] = m.idonor[2 * i + 1]; }
+    if (frac) { frac[i] = m.frac[2 * i]; frac[nd + i] = m.frac[2 * i + 1]; }
+  }
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_ihc_free(ot2d_ihc_lists* l) {
+  delete l;
+  return OT2D_OK;
+}
+
+// uploads every instance's lists as one donor and one fringe array
+static int32_t upload_interp(ot2d_case* c) {
+  std::vector<ot2d_case::HDonor> D;
+  std::vector<ot2d_case::HFringe> F;
+  for (size_t s = 0; s < c->don.size(); ++s) {
+    int base = (int)D.size();
+    D.insert(D.end(), c->don[s].begin(), c->don[s].end());
+    for (ot2d_case::HFringe f : c->fr[s]) {
+      f.src += base;
+      F.push_back(f);
+    }
+  }
+  if (sizeof(ot2d_case::HDonor) != interp_donor_bytes() || sizeof(ot2d_case::HFringe) != interp_fringe_bytes())
+    return fail(OT2D_ERR_STATE, "interpolation record layouts differ");
+  dfree(c->d_don); dfree(c->d_fr); dfree(c->d_qbc);
+  c->d_don = c->d_fr = nullptr; c->d_qbc = nullptr;
+  c->ndon = c->nfr = 0;
+  if (!D.empty()) {
+    HIPCHK(hipMalloc(&c->d_don, D.size() * sizeof(D[0])));
+    HIPCHK(hipMemcpy(c->d_don, D.data(), D.size() * sizeof(D[0]), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void**)&c->d_qbc, D.size() * (size_t)c->nv * sizeof(double)));
+  }
+  if (!F.empty()) {
+    HIPCHK(hipMalloc(&c->d_fr, F.size() * sizeof(F[0])));
+    HIPCHK(hipMemcpy(c->d_fr, F.data(), F.size() * sizeof(F[0]), hipMemcpyHostToDevice));
+  }
+  c->ndon = (int)D.size();
+  c->nfr = (int)F.size();
+  c->graph_valid = 0;   // the captured step gains (or changes) its interpolation
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_set_connectivity(ot2d_case* c, int32_t nsp, const ot2d_ihc_lists* l) {
+  if (!c || !l) return fail(OT2D_ERR_ARG, "ot2d_set_connectivity: null argument");
+  if (c->ncase != 1) return fail(OT2D_ERR_UNSUPPORTED, "overset connectivity in a batch of cases");
+  if (nsp < 0 || nsp >= c->nspec) return fail(OT2D_ERR_ARG, "nsp=%d", nsp);
+  if ((int)l->r.mesh.size() != c->nmesh) return fail(OT2D_ERR_ARG, "lists for %d meshes, case has %d",
+                                                     (int)l->r.mesh.size(), c->nmesh);
+  for (int im = 0; im < c->nmesh; ++im)
+    if (l->jmax[im] != c->m[im].jmax || l->kmax[im] != c->m[im].kmax)
+      return fail(OT2D_ERR_ARG, "mesh %d dimensions differ from the lists'", im);
+  // validate every index the kernels will use
+  std::vector<int> dbase(c->nmesh + 1, 0);
+  for (int im = 0; im < c->nmesh; ++im) dbase[im + 1] = dbase[im] + (int)(l->r.mesh[im].idonor.size() / 2);
+  std::vector<ot2d_case::HDonor> D;
+  std::vector<ot2d_case::HFringe> F;
+  for (int im = 0; im < c->nmesh; ++im) {
+    const ihc::MeshOut& m = l->r.mesh[im];
+    const MeshBuf& mb = c->m[im];
+    const int mi = im * c->nspec + nsp;
+    for (size_t i = 0; i < m.idonor.size() / 2; ++i) {
+      ot2d_case::HDonor d{mi, m.idonor[2 * i], m.idonor[2 * i + 1], 0, m.frac[2 * i], m.frac[2 * i + 1]};
+      if (d.jj < 2 || d.jj > mb.jd - 1 || d.kk < 2 || d.kk > mb.kd - 1)
+        return fail(OT2D_ERR_ARG, "mesh %d donor %d: cell (%d,%d) has no 3x3 stencil", im, (int)i + 1, d.jj, d.kk);
+      D.push_back(d);
+    }
+    for (size_t i = 0; i < m.ibc.size(); ++i) {
+      ot2d_case::HFringe f{mi, m.imesh[2 * i], m.imesh[2 * i + 1], m.ibc[i] - 1};
+      if (f.jj < 1 || f.jj > mb.jd || f.kk < 1 || f.kk > mb.kd || f.src < 0 || f.src >= dbase[c->nmesh])
+        return fail(OT2D_ERR_ARG, "mesh %d fringe %d out of range", im, (int)i + 1);
+      F.push_back(f);
+    }
+  }
+  if ((int)c->don.size() != c->nspec) { c->don.assign(c->nspec, {}); c->fr.assign(c->nspec, {}); }
+  c->don[nsp] = D;
+  c->fr[nsp] = F;
+  for (int im = 0; im < c->nmesh; ++im) {
+    const DevMesh& M = c->hM[(size_t)im * c->nspec + nsp];
+    HIPCHK(hipMemcpy(M.iblank, l->r.mesh[im].iblank.data(), sizeof(int32_t) * c->m[im].n2, hipMemcpyHostToDevice));
+  }
+  return upload_interp(c);
+}
+
+extern "C" int32_t ot2d_connect(ot2d_case* c, const ot2d_ihcbc* ihcbc, int32_t* nfringe_total) {
+  if (!c || !ihcbc) return fail(OT2D_ERR_ARG, "ot2d_connect: null argument");
+  if (c->nmesh < 2) return fail(OT2D_ERR_ARG, "ot2d_connect: a single mesh has no overset partner");
+  for (int im = 0; im < c->nmesh; ++im)
+    if (!c->m[im].has_grid) return fail(OT2D_ERR_STATE, "mesh %d has no grid", im);
+  int32_t total = 0;
+  for (int nsp = 0; nsp < c->nspec; ++nsp) {
+    std::vector<std::vector<double>> xv(c->nmesh), yv(c->nmesh), xc(c->nmesh), yc(c->nmesh), vol(c->nmesh);
+    std::vector<ot2d_ihc_mesh> in(c->nmesh);
+    for (int im = 0; im < c->nmesh; ++im) {
+      const MeshBuf& mb = c->m[im];
+      const DevMesh& M = c->hM[(size_t)im * c->nspec + nsp];
+      xv[im].resize(mb.nv2); yv[im].resize(mb.nv2);
+      xc[im].resize(mb.n2); yc[im].resize(mb.n2); vol[im].resize(mb.n2);
+      HIPCHK(hipMemcpy(xv[im].data(), M.xv, mb.nv2 * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(yv[im].data(), M.yv, mb.nv2 * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(xc[im].data(), M.x, mb.n2 * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(yc[im].data(), M.y, mb.n2 * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(vol[im].data(), M.q[c->nq - 1], mb.n2 * sizeof(double), hipMemcpyDeviceToHost));
+      for (double& v : vol[im]) v = 1. / v;   // volg = 1/q(nq), src/Turns_overset.f:314
+      ot2d_ihc_mesh& m = in[im];
+      m.jmax = mb.jmax; m.kmax = mb.kmax;
+      m.xv = xv[im].data(); m.yv = yv[im].data();
+      m.xc = xc[im].data(); m.yc = yc[im].data(); m.vol = vol[im].data();
+      m.bc = ihcbc[im];
+    }
+    ot2d_ihc_lists* l = nullptr;
+    if (int32_t rc = ot2d_ihc_connect(in.data(), c->nmesh, &l)) return rc;
+    int32_t rc = ot2d_set_connectivity(c, nsp, l);
+    for (const ihc::MeshOut& m : l->r.mesh) total += (int32_t)m.ibc.size();
+    ot2d_ihc_free(l);
+    if (rc) return rc;
+  }
+  if (nfringe_total) *nfringe_total = total;
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_interpolate(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (int32_t rc = clear_status(c)) return rc;
+  if (c->ndon > 0 || c->nfr > 0)
+    HIPCHK(launch_interp(c->dM, c->d_don, c->ndon, c->d_fr, c->nfr, c->d_qbc, c->nv, c->stream));
   return sync(c);
 }

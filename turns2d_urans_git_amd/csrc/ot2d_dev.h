@@ -139,5 +139,11 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sst(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);
+// overset interpolation (ot2d_overset.hip): don/fr are device arrays of
+// {int mi, jj, kk, pad; double f1, f2} and {int mi, jj, kk, src}
+hipError_t launch_interp(const DevMesh* dM, const void* don, int ndon, const void* fr, int nfr, double* qbc, int nv,
+                         hipStream_t st);
+size_t interp_donor_bytes();
+size_t interp_fringe_bytes();
 size_t lu_lds_need(int ni, int fp32);
 }  // namespace ot2d

@@ -1095,7 +1095,7 @@ __device__ __forceinline__ void nb_k(const CellLoad& c, const LUCellF64& L, doub
 // ----------------------------------------------------------------------
 
 constexpr int LU_NVEC = 10;        // 16-byte planes reserved per array
-constexpr int LU_MAXRL = 4;        // rows per lane: at most 256 rows
+constexpr int LU_MAXRL = 8;        // rows per lane: at most 512 rows
 
 template <typename T> struct LUV;
 template <> struct LUV<float> { static constexpr int N = 5; };
@@ -2363,6 +2363,12 @@ hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_upd
       case 2: return launch_lu_t<double, 2, 2>(dM, hM, nmi, fuse_update, mk, st);
       case 3: return launch_lu_t<double, 3, 1>(dM, hM, nmi, fuse_update, mk, st);
       case 4: return launch_lu_t<double, 4, 1>(dM, hM, nmi, fuse_update, mk, st);
+      // tall meshes (Cartesian backgrounds of overset cases); the FP32 sweep
+      // has no such bound
+      case 5: return launch_lu_t<double, 5, 1>(dM, hM, nmi, fuse_update, mk, st);
+      case 6: return launch_lu_t<double, 6, 1>(dM, hM, nmi, fuse_update, mk, st);
+      case 7: return launch_lu_t<double, 7, 1>(dM, hM, nmi, fuse_update, mk, st);
+      case 8: return launch_lu_t<double, 8, 1>(dM, hM, nmi, fuse_update, mk, st);
     }
   }
   return hipErrorInvalidValue;

@@ -350,3 +350,88 @@ def write_case(dirpath: str, grid: CGrid, segs: Optional[List[BCSeg]] = None, **
     base = dict(jmax=grid.jmax, kmax=grid.kmax, jtail1=grid.jtail1)
     base.update(nml)
     write_inputs(os.path.join(dirpath, "inputs"), **base)
+
+
+# --------------------------------------------------------------------------
+# overset decks
+# --------------------------------------------------------------------------
+@dataclasses.dataclass
+class IHCSeg:
+    """One row of an ``ihcbc.<n>`` file (``ihc/src/ihc.f90:147-208``).
+
+    ``ibtyp``: 1 not a receiver, 2 blanked, 3 forced receiver, 4 never
+    blanked, 5 wall points (vertex-count indexing), 6 = 1 and 4, 7 = 3 and 4.
+    Ranges are cell indices of the mesh without halo (negative counts from
+    the end)."""
+    ibtyp: int
+    ibdir: int
+    jbcs: int
+    jbce: int
+    kbcs: int
+    kbce: int
+
+
+def write_ihcbc(path: str, segs: Sequence[IHCSeg]):
+    """Write one ``&ihcbcinp`` namelist file."""
+    with open(path, "w") as f:
+        f.write(" &ihcbcinp\n")
+        f.write(f"  nbc = {len(segs)},\n")
+        if segs:
+            for name in ("ibtyp", "ibdir", "jbcs", "jbce", "kbcs", "kbce"):
+                vals = ",".join(str(getattr(sg, name)) for sg in segs)
+                f.write(f"  {name} = {vals},\n")
+        f.write(" /\n")
+
+
+def box_bc() -> List[BCSeg]:
+    """Characteristic far field (type 47) on the four sides of a Cartesian
+    background mesh."""
+    return [
+        BCSeg(47, 2, 1, -1, 1, 1),
+        BCSeg(47, -2, 1, -1, -1, -1),
+        BCSeg(47, 1, 1, 1, 1, -1),
+        BCSeg(47, -1, -1, -1, 1, -1),
+    ]
+
+
+def cgrid_ihc(jtail1: int, jmax: int, fringe_layers: int = 2) -> List[IHCSeg]:
+    """IHC rows for a near-body C-grid: the airfoil wall points (cells on
+    ``k = 1`` between the trailing-edge vertices) and forced receivers on the
+    outermost ``fringe_layers`` cell layers."""
+    return [
+        IHCSeg(5, 2, jtail1, jmax - jtail1, 1, 1),
+        IHCSeg(3, -2, 1, -1, -fringe_layers, -1),
+    ]
+
+
+@dataclasses.dataclass
+class OversetMesh:
+    xv: np.ndarray
+    yv: np.ndarray
+    bc: List[BCSeg]
+    ihc: List[IHCSeg]
+    jtail1: int = 1
+
+
+def airfoil_overset(near=(385, 97), back=(401, 401), m=0.02, p=0.4, t=0.12, ds_wall=1e-5,
+                    radius=1.5, half_width=10.0, fringe_layers=2, **kw) -> List[OversetMesh]:
+    """Near-body C-grid around a NACA 4-digit airfoil (outer radius
+    ``radius`` chords) inside a uniform Cartesian background over
+    ``[-half_width, half_width]^2``."""
+    g = naca4_cgrid(near[0], near[1], m=m, p=p, t=t, ds_wall=ds_wall, radius=radius, **kw)
+    xb, yb = cartesian_grid(back[0], back[1], -half_width, half_width, -half_width, half_width)
+    return [OversetMesh(g.xv, g.yv, cgrid_bc(g.jtail1), cgrid_ihc(g.jtail1, g.jmax, fringe_layers), g.jtail1),
+            OversetMesh(xb, yb, box_bc(), [], 1)]
+
+
+def write_overset_case(dirpath: str, meshes: Sequence[OversetMesh], **nml):
+    """Write ``fort.1``, ``bc.inp``, ``ihcbc.<n>`` and ``inputs`` for an
+    overset case (``num_grids`` meshes)."""
+    os.makedirs(dirpath, exist_ok=True)
+    write_fort1(os.path.join(dirpath, "fort.1"), [(mm.xv, mm.yv) for mm in meshes])
+    write_bcinp(os.path.join(dirpath, "bc.inp"), [mm.bc for mm in meshes])
+    for i, mm in enumerate(meshes):
+        write_ihcbc(os.path.join(dirpath, f"ihcbc.{i}"), mm.ihc)
+    base = dict(num_grids=len(meshes), jtail1=meshes[0].jtail1)
+    base.update(nml)
+    write_inputs(os.path.join(dirpath, "inputs"), **base)
