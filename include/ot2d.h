@@ -173,6 +173,27 @@ int32_t ot2d_batch_status(ot2d_case* c, int32_t* status, int32_t* istep0);
 /* Stop advancing case icase (status OT2D_CONVERGED). */
 int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase);
 
+/* --------------------------------------------------------------- serving
+ * Primitives of a request loop that keeps many cases resident and admits and
+ * retires them between iterations.  No counterpart in the reference (its
+ * driver runs one case to nsteps); the convergence test is the reference's
+ * residual norm rsum (src/step.f:60-93). */
+
+/* Device bytes ot2d_case_create (ncase = 1) or ot2d_batch_create
+ * (ncase = nmesh) would hold for these meshes, for admission by footprint. */
+int32_t ot2d_footprint(int32_t ncase, const ot2d_params* params, int32_t nmesh, const ot2d_mesh_desc* meshes,
+                       int64_t* bytes);
+/* Device bytes the case holds now. */
+int32_t ot2d_case_bytes(ot2d_case* c, int64_t* bytes);
+/* Case icase (-1: all) stops, with status OT2D_CONVERGED, at the first step
+ * whose rsum is at or below rsum_tol on every instance, before that step's
+ * update; rsum_tol <= 0 switches the test off. */
+int32_t ot2d_set_convergence(ot2d_case* c, int32_t icase, double rsum_tol);
+/* Enqueues niter steps on the case's stream and returns without waiting
+ * (no residual history); ot2d_sync or ot2d_batch_status waits. */
+int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter);
+int32_t ot2d_sync(ot2d_case* c);
+
 /* Kernel stages of one step, in launch order (ot2d_profile). */
 enum {
   OT2D_ST_BEGIN = 0,     /* step counter / status reset                        */

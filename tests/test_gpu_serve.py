@@ -1,0 +1,102 @@
+"""The request loop on the GPU: a result is the same alone and in a mix of
+concurrently resident cases (bit for bit), admission keeps the resident
+cases within the memory budget, and a converged case stops on the device."""
+import numpy as np
+import pytest
+
+from turns2d_urans_git_amd import abi, grid, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    import turns2d_urans_git_amd as t2d
+    return t2d.load_library(), t2d.last_error
+
+
+def _polar(rid, arrival, m=0.02, alfa=2.0, jmax=41, kmax=21, fp32=1):
+    g = grid.naca4_cgrid(jmax, kmax, m=m, p=0.4, ds_wall=2e-4)
+    deck = dict(fsmach=0.5, alfa=alfa, rey=1e6, invisc=0, lamin=0, iturb=1, vnuinf=3.0, irhsy=-3, ilim=1,
+                ilhs=1, iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1, timeac=1.0,
+                lu_fp32=fp32)
+    return serve.Request(rid, arrival, deck, g, "polar")
+
+
+def _pitch(rid, arrival, n=3):
+    g = grid.naca4_cgrid(41, 21, ds_wall=2e-4)
+    deck = dict(fsmach=0.3, alfa=-1.0, angmax=2.0, rf=0.05, rey=4e6, invisc=0, lamin=0, iturb=1, vnuinf=3.0,
+                irhsy=-3, ilim=1, ilhs=1, iconstant_cfl=1, dt=5.0, epse=0.01, iunst=0, ntac=1, itnmax=1,
+                timeac=1.0, timespectral=1, nspec=n, motiontype=1, lu_fp32=1)
+    return serve.Request(rid, arrival, deck, g, "pitch")
+
+
+TOL, K, MAXS = 1e-7, 50, 3000
+
+
+def _serve(reqs, **kw):
+    lib, err = _lib()
+    srv = serve.Server(lib, steps_per_tick=K, tol=TOL, max_steps=MAXS, keep_state=True, errfn=err, **kw)
+    return srv, srv.run(reqs)
+
+
+@pytest.mark.parametrize("which", ["polar", "pitch"])
+def test_same_alone_and_in_the_mix(which):
+    target = _polar(0, 0.0, m=0.03, alfa=3.0) if which == "polar" else _pitch(0, 0.0, n=3)
+    _, alone = _serve([target])
+    mix = [_polar(1, 0.0, alfa=1.0), _polar(2, 0.0, jmax=49, alfa=-2.0), _pitch(3, 0.0, n=5),
+           _polar(4, 0.0, fp32=0)]
+    t2 = serve.Request(9, 0.05, target.deck, target.grid, target.kind)    # arrives while the mix runs
+    _, both = _serve(mix + [t2, _polar(5, 0.1, alfa=5.0)])
+    a, b = alone[0], [r for r in both if r.rid == 9][0]
+    print(which, "alone:", a.status, a.steps, "mix:", b.status, b.steps, [(r.rid, r.status, r.steps) for r in both])
+    assert a.status == b.status and a.steps == b.steps
+    assert a.status in (abi.CONVERGED, -100)
+    np.testing.assert_array_equal(a.state, b.state)
+    np.testing.assert_array_equal(a.forces, b.forces)
+
+
+def test_admission_respects_the_budget():
+    lib, err = _lib()
+    reqs = [_polar(i, 0.0, alfa=float(i % 4)) for i in range(6)]
+    srv0 = serve.Server(lib, errfn=err)
+    one = srv0.footprint(reqs[:1])
+    srv, res = _serve(reqs, budget=int(2.5 * one))
+    assert srv.peak_bytes <= int(2.5 * one)
+    assert all(r.status in (abi.CONVERGED, -100) for r in res)
+    adm = sorted(r.admitted for r in res)
+    fin = sorted(r.finished for r in res)
+    # at most two resident at a time: the third admission waits for a finish
+    assert adm[2] >= fin[0]
+
+
+def test_converged_case_stops_on_the_device():
+    """With a tolerance the residual history reaches at step s, the case
+    stops at exactly step s (status CONVERGED, before that step's update),
+    and further iterations leave it untouched."""
+    lib, err = _lib()
+    r = _polar(0, 0.0)
+    p = abi.Params()
+    lib.ot2d_default_params.argtypes = [abi.C.POINTER(abi.Params)]
+    lib.ot2d_default_params(abi.C.byref(p))
+    p.update(**r.deck)
+    md = abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1))
+    ref = abi.CaseAPI(lib, "ot2d_", [p], [md], errfn=err, batch=True)
+    ref.set_grid(0, r.grid.xv, r.grid.yv)
+    rc, hist, done = ref.iterate(600)
+    assert done == 600
+    tol = float(hist[300:, 0].min()) * (1 + 1e-12)
+    s_first = int(np.argmax(hist[:, 0] <= tol)) + 1          # 1-based step
+    case = abi.CaseAPI(lib, "ot2d_", [p], [md], errfn=err, batch=True)
+    case.set_grid(0, r.grid.xv, r.grid.yv)
+    case.set_convergence(tol)
+    for _ in range(20):
+        case.iterate_async(K)
+        st, ist = case.batch_status()
+        if st[0] == abi.CONVERGED:
+            break
+    assert st[0] == abi.CONVERGED and int(ist[0]) == s_first
+    q1 = case.get_state(0)
+    case.iterate_async(200)
+    st2, ist2 = case.batch_status()
+    assert st2[0] == abi.CONVERGED and int(ist2[0]) == s_first
+    np.testing.assert_array_equal(case.get_state(0), q1)

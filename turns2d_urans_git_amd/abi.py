@@ -140,6 +140,11 @@ def bind(lib: C.CDLL, prefix: str):
         "ihc_get": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
         "ihc_free": (C.c_int32, [C.c_void_p]),
+        "footprint": (C.c_int32, [C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, P(C.c_int64)]),
+        "case_bytes": (C.c_int32, [C.c_void_p, P(C.c_int64)]),
+        "set_convergence": (C.c_int32, [C.c_void_p, C.c_int32, C.c_double]),
+        "iterate_async": (C.c_int32, [C.c_void_p, C.c_int32]),
+        "sync": (C.c_int32, [C.c_void_p]),
         # oracle only: connectivity of one mesh as plain arrays
         "set_mesh_connectivity": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _IP, C.c_int32, _IP, _IP,
                                               C.c_int32, _IP, _DP]),
@@ -241,6 +246,21 @@ class CaseAPI:
         self._check(self._fns["set_mesh_connectivity"](self.h, nsp, im, iblank, len(ibc), imesh, ibc,
                                                        len(idonor) // 2, idonor, frac),
                     "set_mesh_connectivity")
+
+    # ---------------------------------------------------------- serving
+    def set_convergence(self, tol: float, icase: int = -1):
+        self._check(self._fns["set_convergence"](self.h, icase, float(tol)), "set_convergence")
+
+    def iterate_async(self, niter: int):
+        self._check(self._fns["iterate_async"](self.h, int(niter)), "iterate_async")
+
+    def sync(self):
+        self._check(self._fns["sync"](self.h), "sync")
+
+    def bytes(self) -> int:
+        b = C.c_int64()
+        self._check(self._fns["case_bytes"](self.h, C.byref(b)), "case_bytes")
+        return b.value
 
     def interpolate(self):
         self._check(self._fns["interpolate"](self.h), "interpolate")
@@ -441,3 +461,17 @@ def ihc_connect(lib, prefix, meshes, errfn=None) -> IHCLists:
     if rc:
         raise CaseError(rc, "ihc_connect", errfn() if errfn else "")
     return IHCLists(lib, prefix, h, dims, errfn)
+
+
+def footprint(lib, prefix, params_list, meshes) -> int:
+    """Device bytes of a batch of these cases (or one case when a single
+    Params is given for several meshes)."""
+    fns = bind(lib, prefix)
+    plist = list(params_list) if isinstance(params_list, (list, tuple)) else [params_list]
+    parr = (Params * len(plist))(*plist)
+    marr = (MeshDesc * len(meshes))(*meshes)
+    b = C.c_int64()
+    rc = fns["footprint"](len(plist), C.cast(parr, C.c_void_p), len(meshes), C.cast(marr, C.c_void_p), C.byref(b))
+    if rc:
+        raise CaseError(rc, "footprint")
+    return b.value

@@ -64,6 +64,7 @@ struct ot2d_case {
   int lu_rl = 1;             // rows per lane of the LU-SGS sweeps
   int weno = 0;              // some case reconstructs with WENO5 (wider residual tiles)
   int ncase = 1;             // independent cases (batch: one per mesh; otherwise 1)
+  int batch = 0;             // made by ot2d_batch_create: per-case states persist across calls
   std::vector<int> mesh_case;  // case of each mesh
   hipStream_t stream = nullptr;
   std::vector<MeshBuf> m;
@@ -75,6 +76,10 @@ struct ot2d_case {
   int* d_status = nullptr;
   int* d_istep0 = nullptr;
   int* d_stepc = nullptr;
+  int* d_cinst = nullptr;      // first, one-past-last instance record of every case
+  double* d_ctol = nullptr;    // convergence tolerance on rsum per case (0: off)
+  int conv_on = 0;             // some case has a tolerance: the step checks it
+  size_t bytes = 0;            // device memory held by the case
   double* d_partials = nullptr;
   double* d_norms = nullptr;
   double* d_tspec = nullptr;
@@ -274,6 +279,7 @@ static void free_case(ot2d_case* c) {
     dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa);
   }
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
+  dfree(c->d_cinst); dfree(c->d_ctol);
   dfree(c->d_partials); dfree(c->d_norms); dfree(c->d_tspec);
   dfree(c->d_Ds); dfree(c->d_hist); dfree(c->d_forces);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -343,8 +349,10 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
   c->hM.resize(nmi);
   // per-case bookkeeping
 #define A(ptr, n) do { if ((e = dalloc(&(ptr), (n))) != hipSuccess) { free_case(c); \
-    return fail(OT2D_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e)); } } while (0)
+    return fail(OT2D_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e)); } \
+    c->bytes += (size_t)(n) * sizeof(*(ptr)); } while (0)
   A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase);
+  A(c->d_cinst, 2 * (size_t)ncase); A(c->d_ctol, ncase);
   A(c->d_norms, 6 * nmi); A(c->d_tspec, (size_t)ns * ncase); A(c->d_Ds, ns * ns);
   A(c->d_forces, 16);
   // spectral coefficients, src/setup.f:838-873
@@ -523,7 +531,80 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
   }
   (void)hipMemcpy(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice);
   (void)hipMemcpy(c->dT, c->hT.data(), sizeof(ResTile) * c->hT.size(), hipMemcpyHostToDevice);
+  c->bytes += sizeof(DevMesh) * c->hM.size() + sizeof(ResTile) * c->hT.size();
+  // instance range of every case (meshes of one case are consecutive)
+  {
+    std::vector<int> inst(2 * (size_t)ncase, 0);
+    for (int ic = 0; ic < ncase; ++ic) {
+      int lo = -1, hi = -1;
+      for (int im = 0; im < nmesh; ++im)
+        if (mesh_case[im] == ic) {
+          if (lo < 0) lo = im * c->nspec;
+          hi = (im + 1) * c->nspec;
+        }
+      inst[2 * ic] = lo < 0 ? 0 : lo;
+      inst[2 * ic + 1] = hi < 0 ? 0 : hi;
+    }
+    (void)hipMemcpy(c->d_cinst, inst.data(), sizeof(int) * inst.size(), hipMemcpyHostToDevice);
+  }
   *out = c;
+  return OT2D_OK;
+}
+
+// device bytes ot2d_case_create / ot2d_batch_create would hold for these
+// meshes (same sums as create_impl, without allocating)
+static size_t footprint(const ot2d_params* ps, int32_t ncase, int32_t nmesh, const ot2d_mesh_desc* md,
+                        const int32_t* mesh_case) {
+  int nq, nv, lamin, iturb;
+  derive_counts(ps, &nq, &nv, &lamin, &iturb);
+  const size_t ns = ps->timespectral ? (size_t)ps->nspec : 1;
+  int rl = 1;
+  for (int im = 0; im < nmesh; ++im) rl = (md[im].kmax - 1 + 63) / 64 > rl ? (md[im].kmax - 1 + 63) / 64 : rl;
+  size_t b = (size_t)ncase * (3 * sizeof(int) + 2 * sizeof(int) + sizeof(double)) +
+             (6 * (size_t)nmesh * ns + ns * ncase + ns * ns + 16) * sizeof(double);
+  for (int im = 0; im < nmesh; ++im) {
+    const size_t jd = md[im].jmax + 1, kd = md[im].kmax + 1, n2 = jd * kd, nv2 = (size_t)md[im].jmax * md[im].kmax;
+    size_t d = n2 * (nq + nv) + 11 * n2 + 4 * nv2;
+    if (!lamin) d += (iturb == 2 ? 20 : 10) * n2;
+    const size_t tiles = ((jd - 2 + RES_TJ - 1) / RES_TJ) * ((kd - 2 + RES_TK - 1) / RES_TK);
+    b += ns * (d * sizeof(double) + n2 * sizeof(int) + lu_workspace_bytes((int)jd, (int)kd, rl) +
+               tiles * (4 * sizeof(double) + sizeof(ResTile)) + sizeof(DevMesh));
+  }
+  (void)mesh_case;
+  return b;
+}
+
+extern "C" int32_t ot2d_footprint(int32_t ncase, const ot2d_params* params, int32_t nmesh,
+                                  const ot2d_mesh_desc* meshes, int64_t* bytes) {
+  if (!params || !meshes || !bytes || ncase < 1 || nmesh < 1) return fail(OT2D_ERR_ARG, "bad arguments");
+  if (ncase != 1 && ncase != nmesh) return fail(OT2D_ERR_ARG, "ncase must be 1 or nmesh");
+  std::vector<int32_t> mc(nmesh, 0);
+  for (int i = 0; i < nmesh && ncase > 1; ++i) mc[i] = i;
+  *bytes = (int64_t)footprint(params, ncase, nmesh, meshes, mc.data());
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_case_bytes(ot2d_case* c, int64_t* bytes) {
+  if (!c || !bytes) return fail(OT2D_ERR_ARG, "null argument");
+  *bytes = (int64_t)(c->bytes + (size_t)c->hist_cap * c->hM.size() * sizeof(double) +
+                     (size_t)c->ndon * (sizeof(ot2d_case::HDonor) + c->nv * sizeof(double)) +
+                     (size_t)c->nfr * sizeof(ot2d_case::HFringe));
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_set_convergence(ot2d_case* c, int32_t icase, double rsum_tol) {
+  if (!c || icase < -1 || icase >= c->ncase) return fail(OT2D_ERR_ARG, "bad case index");
+  std::vector<double> tol(c->ncase, 0.0);
+  HIPCHK(hipMemcpy(tol.data(), c->d_ctol, sizeof(double) * c->ncase, hipMemcpyDeviceToHost));
+  for (int i = 0; i < c->ncase; ++i)
+    if (icase < 0 || i == icase) tol[i] = rsum_tol;
+  HIPCHK(hipMemcpy(c->d_ctol, tol.data(), sizeof(double) * c->ncase, hipMemcpyHostToDevice));
+  int on = 0;
+  for (double t : tol) on |= t > 0.;
+  if (on != c->conv_on) {
+    c->conv_on = on;
+    c->graph_valid = 0;   // the captured step gains (or loses) the check
+  }
   return OT2D_OK;
 }
 
@@ -539,7 +620,9 @@ extern "C" int32_t ot2d_batch_create(int32_t ncase, const ot2d_params* params, c
   if (!params || !meshes || !out || ncase < 1) return fail(OT2D_ERR_ARG, "bad arguments to ot2d_batch_create");
   std::vector<int32_t> mc(ncase);
   for (int i = 0; i < ncase; ++i) mc[i] = i;
-  return create_impl(params, ncase, ncase, meshes, mc.data(), out);
+  int32_t rc = create_impl(params, ncase, ncase, meshes, mc.data(), out);
+  if (rc == OT2D_OK) (*out)->batch = 1;
+  return rc;
 }
 
 extern "C" int32_t ot2d_case_destroy(ot2d_case* c) {
@@ -671,7 +754,7 @@ static int nmi(ot2d_case* c) { return c->nmesh * c->nspec; }
 // each case (stopped and retired cases stay so until their grid or state is
 // set again)
 static int32_t clear_status(ot2d_case* c) {
-  if (c->ncase == 1) HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  if (!c->batch) HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
   return OT2D_OK;
 }
 
@@ -707,6 +790,7 @@ static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks*
   if (norms && later) HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
   if (norms) {
     HIPCHK(launch_norms_final(c->dM, nmi(c), c->stream));
+    if (c->conv_on) HIPCHK(launch_conv_check(c->dM, c->d_cinst, c->d_ctol, c->d_status, c->ncase, c->stream));
     mark(mk, OT2D_ST_NORMS, c->stream);
   }
   return OT2D_OK;
@@ -788,7 +872,7 @@ static int32_t read_status(ot2d_case* c, int* st) {
   std::vector<int> all(c->ncase);
   HIPCHK(hipMemcpyAsync(all.data(), c->d_status, sizeof(int) * c->ncase, hipMemcpyDeviceToHost, c->stream));
   if (int32_t rc = sync(c)) return rc;
-  if (c->ncase == 1) {
+  if (!c->batch) {
     *st = all[0];
   } else {
     *st = 0;
@@ -889,6 +973,27 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
   if (st == 2) return OT2D_NORM;
   if (st == 1) return OT2D_MSTOP;
   return OT2D_OK;
+}
+
+// niter steps enqueued on the case's stream without waiting (no history);
+// ot2d_batch_status / ot2d_sync wait for them
+extern "C" int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter) {
+  if (!c || niter < 0) return fail(OT2D_ERR_ARG, "bad arguments");
+  for (auto& mb : c->m)
+    if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  if (c->hM[0].hist != nullptr) {
+    for (auto& M : c->hM) { M.hist = nullptr; M.hist_cap = 0; }
+    HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  if (int32_t rc = clear_status(c)) return rc;
+  if (int32_t rc = ensure_graph(c)) return rc;
+  for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_sync(ot2d_case* c) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  return sync(c);
 }
 
 extern "C" int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]) {
@@ -1112,7 +1217,7 @@ static int32_t upload_interp(ot2d_case* c) {
 
 extern "C" int32_t ot2d_set_connectivity(ot2d_case* c, int32_t nsp, const ot2d_ihc_lists* l) {
   if (!c || !l) return fail(OT2D_ERR_ARG, "ot2d_set_connectivity: null argument");
-  if (c->ncase != 1) return fail(OT2D_ERR_UNSUPPORTED, "overset connectivity in a batch of cases");
+  if (c->batch) return fail(OT2D_ERR_UNSUPPORTED, "overset connectivity in a batch of cases");
   if (nsp < 0 || nsp >= c->nspec) return fail(OT2D_ERR_ARG, "nsp=%d", nsp);
   if ((int)l->r.mesh.size() != c->nmesh) return fail(OT2D_ERR_ARG, "lists for %d meshes, case has %d",
                                                      (int)l->r.mesh.size(), c->nmesh);

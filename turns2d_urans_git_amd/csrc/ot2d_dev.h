@@ -124,6 +124,8 @@ hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, 
                            hipStream_t st);
 hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st);
 hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st);
+hipError_t launch_conv_check(const DevMesh* dM, const int* inst, const double* tol, int* status, int ncase,
+                             hipStream_t st);
 hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st);
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,

@@ -2257,6 +2257,27 @@ hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hip
   return hipGetLastError();
 }
 
+// convergence of every case: status 4 once the rsum of all its instances is
+// at or below its tolerance (tol <= 0: never).  inst[2c], inst[2c+1]: first
+// and one-past-last instance record of case c.
+__global__ __launch_bounds__(256) void k_conv_check(const DevMesh* __restrict__ Ms, const int* __restrict__ inst,
+                                                    const double* __restrict__ tol, int* status, int ncase) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncase || status[c] != 0 || !(tol[c] > 0.)) return;
+  double mx = 0.;
+  for (int i = inst[2 * c]; i < inst[2 * c + 1]; ++i) {
+    double r = Ms[i].norms[0];
+    mx = r > mx ? r : mx;
+  }
+  if (mx <= tol[c]) status[c] = 4;
+}
+
+hipError_t launch_conv_check(const DevMesh* dM, const int* inst, const double* tol, int* status, int ncase,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_conv_check, dim3((ncase + 255) / 256), dim3(256), 0, st, dM, inst, tol, status, ncase);
+  return hipGetLastError();
+}
+
 hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st) {
   hipLaunchKernelGGL(k_norms_final, dim3(nmi), dim3(RES_THREADS), 0, st, dM);
   return hipGetLastError();

@@ -1,0 +1,300 @@
+"""Continuous-batching request loop over libot2d.
+
+Requests (steady polar points and time-spectral pitching cases) arrive over
+time.  Between iterations the loop
+
+* admits waiting requests, in arrival order, while the device memory they
+  need (``ot2d_footprint``) fits the budget: the requests admitted at one
+  moment that share a structure (model, instance count) become one batch
+  (``ot2d_batch_create``), a *cohort*, with its own HIP stream;
+* advances every resident cohort by ``steps_per_tick`` iterations with
+  ``ot2d_iterate_async``, so the cohorts' kernels overlap on the device;
+* reads every cohort's per-case states (``ot2d_batch_status``): a case whose
+  residual has reached the tolerance stopped on the device (status
+  ``CONVERGED``, set by the step itself before its update), and its
+  time-to-converge is recorded; a cohort whose cases have all finished is
+  destroyed and its memory returns to the budget.
+
+State never leaves the device while a case runs.  Cases never interact: a
+case gives the same result alone and in any mix (the kernels of one case
+read only its own records).
+
+The reference has no request loop; its driver runs one case for ``nsteps``
+(src/Turns_overset.f:378-560).  The convergence measure is its residual
+norm rsum (src/step.f:60-93).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import json
+import math
+import time
+from collections import deque
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import abi, grid
+
+
+# ------------------------------------------------------------------ requests
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float                  # seconds after the stream starts
+    deck: Dict[str, float]          # ot2d_params fields
+    grid: grid.CGrid
+    kind: str = "polar"             # "polar" (steady) or "pitch" (time-spectral)
+
+    @property
+    def nspec(self) -> int:
+        return int(self.deck.get("nspec", 1)) if self.deck.get("timespectral") else 1
+
+    @property
+    def cells(self) -> int:
+        return (self.grid.jmax - 1) * (self.grid.kmax - 1) * self.nspec
+
+    def key(self):
+        d = self.deck
+        return (int(d.get("timespectral", 0)), self.nspec, int(d.get("iturb", 1)), int(d.get("lamin", 0)),
+                int(d.get("invisc", 0)), int(d.get("lu_fp32", 0)))
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    kind: str
+    arrival: float
+    admitted: float = math.nan
+    finished: float = math.nan
+    steps: int = 0
+    status: int = 0                 # abi.CONVERGED, or the failure code
+    cells: int = 0
+    forces: Optional[np.ndarray] = None   # (nspec, 6): cl cd cm cpower cfx cfy
+    state: Optional[np.ndarray] = None    # final q when kept
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+# ------------------------------------------------------------- device memory
+def device_memory():
+    """(free, total) bytes of the current HIP device."""
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    rc = hip.hipMemGetInfo(C.byref(free), C.byref(total))
+    if rc != 0:
+        raise RuntimeError(f"hipMemGetInfo failed ({rc})")
+    return free.value, total.value
+
+
+# -------------------------------------------------------------------- server
+class _Cohort:
+    def __init__(self, lib, reqs: List[Request], tol: float, errfn):
+        self.reqs = reqs
+        params = []
+        for r in reqs:
+            p = abi.Params()
+            lib.ot2d_default_params(C.byref(p))
+            p.update(**r.deck)
+            params.append(p)
+        mds = [abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1)) for r in reqs]
+        self.case = abi.CaseAPI(lib, "ot2d_", params, mds, errfn=errfn, batch=True)
+        for i, r in enumerate(reqs):
+            self.case.set_grid(i, r.grid.xv, r.grid.yv)
+        self.case.set_convergence(tol)
+        self.open = set(range(len(reqs)))
+        self.bytes = self.case.bytes()
+
+    def close(self):
+        self.case.close()
+
+
+class Server:
+    """Continuous-batching loop; see the module docstring.
+
+    ``budget``: device bytes the resident cohorts may hold (default: 80 % of
+    the free memory at construction).  ``max_steps``: a case still running
+    after that many iterations is retired as not converged."""
+
+    def __init__(self, lib, budget: Optional[int] = None, steps_per_tick: int = 50, tol: float = 1e-10,
+                 max_cohort: int = 32, max_steps: int = 100000, keep_state: bool = False, errfn=None):
+        self.lib = lib
+        abi.bind(lib, "ot2d_")
+        lib.ot2d_default_params.argtypes = [C.POINTER(abi.Params)]
+        self.budget = int(budget if budget is not None else 0.8 * device_memory()[0])
+        self.k = int(steps_per_tick)
+        self.tol = float(tol)
+        self.max_cohort = int(max_cohort)
+        self.max_steps = int(max_steps)
+        self.keep_state = keep_state
+        self.errfn = errfn
+        self.peak_bytes = 0
+        self.ticks = 0
+
+    def footprint(self, reqs: Sequence[Request]) -> int:
+        params = []
+        for r in reqs:
+            p = abi.Params()
+            self.lib.ot2d_default_params(C.byref(p))
+            p.update(**r.deck)
+            params.append(p)
+        mds = [abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1)) for r in reqs]
+        return abi.footprint(self.lib, "ot2d_", params, mds)
+
+    def run(self, requests: Sequence[Request], time_limit: Optional[float] = None,
+            clock=time.perf_counter) -> List[Result]:
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting: deque = deque()
+        cohorts: List[_Cohort] = []
+        results: Dict[int, Result] = {r.rid: Result(r.rid, r.kind, r.arrival, cells=r.cells) for r in requests}
+        held = 0
+        t0 = clock()
+        while pending or waiting or cohorts:
+            now = clock() - t0
+            if time_limit is not None and now > time_limit:
+                break
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            # ---- admission: arrival order, while the footprint fits
+            admitted: Dict[tuple, List[Request]] = {}
+            while waiting:
+                r = waiting[0]
+                group = admitted.get(r.key(), [])
+                if len(group) >= self.max_cohort:
+                    break
+                need = self.footprint(group + [r]) - (self.footprint(group) if group else 0)
+                if held + need > self.budget:
+                    break
+                held += need
+                admitted.setdefault(r.key(), []).append(waiting.popleft())
+            for group in admitted.values():
+                c = _Cohort(self.lib, group, self.tol, self.errfn)
+                held += c.bytes - self.footprint(group)   # the case's own count
+                tnow = clock() - t0
+                for r in group:
+                    results[r.rid].admitted = tnow
+                cohorts.append(c)
+            self.peak_bytes = max(self.peak_bytes, held)
+            if not cohorts:
+                if pending:
+                    time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
+                continue
+            # ---- one tick: every cohort advances, overlapping on the device
+            for c in cohorts:
+                c.case.iterate_async(self.k)
+            self.ticks += 1
+            done_cohorts = []
+            for c in cohorts:
+                st, ist = c.case.batch_status()
+                tnow = clock() - t0
+                for i in list(c.open):
+                    res = results[c.reqs[i].rid]
+                    res.steps = int(ist[i])
+                    if st[i] != abi.OK or ist[i] >= self.max_steps:
+                        if st[i] == abi.OK:
+                            c.case.batch_retire(i)
+                            res.status = -100          # not converged within max_steps
+                        else:
+                            res.status = int(st[i])
+                        res.finished = tnow
+                        n = c.reqs[i].nspec
+                        res.forces = np.array([c.case.forces(i, s)[k] for s in range(n)
+                                               for k in ("cl", "cd", "cm", "cpower", "cfx", "cfy")]).reshape(n, 6)
+                        if self.keep_state:
+                            res.state = c.case.get_state(i)
+                        c.open.discard(i)
+                if not c.open:
+                    done_cohorts.append(c)
+            for c in done_cohorts:
+                held -= c.bytes
+                c.close()
+                cohorts.remove(c)
+        for c in cohorts:
+            c.close()
+        return [results[r.rid] for r in requests]
+
+
+# ---------------------------------------------------------------- streams
+def polar_request(rid, arrival, rng, scale=1.0, lu_fp32=True) -> Request:
+    """A steady polar point of the 64-case mix's distributions: NACA m~U[0,.06],
+    p~U[.2,.6], t~U[.08,.18]; jmax~U{257..769}, kmax~U{65..193} (scaled);
+    M~U[.3,.8], alpha~U[-4,8], Re log-U[1e6,2e7]; SA."""
+    m, p, t = rng.uniform(0, 0.06), rng.uniform(0.2, 0.6), rng.uniform(0.08, 0.18)
+    jmax = int(rng.integers(257, 770) * scale) // 2 * 2 + 1
+    kmax = int(rng.integers(65, 194) * scale)
+    fsmach, alfa = rng.uniform(0.3, 0.8), rng.uniform(-4, 8)
+    rey = float(np.exp(rng.uniform(np.log(1e6), np.log(2e7))))
+    g = grid.naca4_cgrid(jmax, kmax, m=m, p=p, t=t, ds_wall=1e-6, wake_aspect=1000.0, te_cluster=False)
+    deck = dict(fsmach=fsmach, alfa=alfa, rey=rey, invisc=0, lamin=0, iturb=1, vnuinf=3.0, irhsy=-3, ilim=1,
+                ilhs=1, iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1, timeac=1.0,
+                lu_fp32=int(lu_fp32))
+    return Request(rid, arrival, deck, g, "polar")
+
+
+def pitch_request(rid, arrival, rng, scale=1.0, lu_fp32=True) -> Request:
+    """A time-spectral pitching case: alpha(t) = alpha0 + alpha1 sin(omega t)
+    with alpha0~U[0,5], alpha1~U[1,5] deg, reduced frequency k~U[.02,.1],
+    N~{3,5,7} instances, 385x97 C-grid (scaled), M 0.3, Re 4e6, SA.  The
+    reference's motion is alfa + angmax (1 - cos(rotf t)) (src/setup.f:367),
+    so alfa = alpha0 - alpha1, angmax = alpha1 (a phase shift), rf = 2 k M."""
+    a0, a1, k = rng.uniform(0, 5), rng.uniform(1, 5), rng.uniform(0.02, 0.1)
+    n = int(rng.choice([3, 5, 7]))
+    jmax = int(385 * scale) // 2 * 2 + 1
+    kmax = int(97 * scale)
+    g = grid.naca4_cgrid(jmax, kmax, ds_wall=1e-6, wake_aspect=1000.0, te_cluster=False)
+    deck = dict(fsmach=0.3, alfa=a0 - a1, angmax=a1, rf=2 * k * 0.3, rey=4e6, invisc=0, lamin=0, iturb=1,
+                vnuinf=3.0, irhsy=-3, ilim=1, ilhs=1, iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1,
+                itnmax=1, timeac=1.0, timespectral=1, nspec=n, motiontype=1, lu_fp32=int(lu_fp32))
+    return Request(rid, arrival, deck, g, "pitch")
+
+
+def poisson_stream(lam, duration, seed=20261202, pitch_frac=0.3, scale=1.0, lu_fp32=True) -> List[Request]:
+    """Poisson arrivals at ``lam`` requests/s for ``duration`` s."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / lam)
+        if t > duration:
+            break
+        mk = pitch_request if rng.random() < pitch_frac else polar_request
+        out.append(mk(len(out), t, rng, scale, lu_fp32))
+    return out
+
+
+def summarize(results: Sequence[Result], wall: float) -> dict:
+    ok = [r for r in results if r.status == abi.CONVERGED]
+    lat = np.array([r.latency for r in ok]) if ok else np.array([math.nan])
+    cell_iters = float(sum(r.cells * r.steps for r in results))
+    return dict(requests=len(results), converged=len(ok), wall_s=wall,
+                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                solves_per_hr=len(ok) / wall * 3600.0 if wall > 0 else 0.0,
+                cell_iter_per_s=cell_iters / wall if wall > 0 else 0.0)
+
+
+def main(argv=None):
+    import argparse
+    import turns2d_urans_git_amd as t2d
+    ap = argparse.ArgumentParser(description="serve a Poisson stream of polar and pitching requests")
+    ap.add_argument("--rate", type=float, default=1.0)
+    ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--scale", type=float, default=1.0, help="grid size factor")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--steps-per-tick", type=int, default=100)
+    ap.add_argument("--max-steps", type=int, default=40000)
+    ap.add_argument("--time-limit", type=float, default=None)
+    a = ap.parse_args(argv)
+    reqs = poisson_stream(a.rate, a.duration, scale=a.scale)
+    srv = Server(t2d.load_library(), steps_per_tick=a.steps_per_tick, tol=a.tol, max_steps=a.max_steps,
+                 errfn=t2d.last_error)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, time_limit=a.time_limit)
+    out = summarize(res, time.perf_counter() - t0)
+    out.update(rate=a.rate, duration=a.duration, scale=a.scale, tol=a.tol, peak_bytes=srv.peak_bytes)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
