@@ -61,6 +61,14 @@ WORKLOADS = {
     "sa_513x129_fp64": dict(jmax=513, kmax=129, ds_wall=1e-6, deck=_SA, lu_fp32=0, model="sa",
                             wake_aspect=1000.0, te_cluster=False),
     "euler_257x65": dict(jmax=257, kmax=65, ds_wall=1e-4, deck=_EULER, lu_fp32=0, model="euler"),
+    # overset k-omega SST with WENO5: near-body 385x97 C-grid around a NACA
+    # 2412-like airfoil (wall spacing 1e-6, outer edge 1.5 chords) in a
+    # uniform 401x401 Cartesian background over [-10,10]^2, IHC hole cut and
+    # 3x3 donor interpolation after every step, M 0.3, alpha 6, Re 3e6
+    "overset_sst_weno": dict(near=(385, 97), back=(401, 401), ds_wall=1e-6, lu_fp32=0, model="sst",
+                             deck=dict(_SA, iturb=2, ilim=4, fsmach=0.3, alfa=6.0, rey=3.0e6, dt=5.0)),
+    "overset_sst_weno_fp32": dict(near=(385, 97), back=(401, 401), ds_wall=1e-6, lu_fp32=1, model="sst",
+                                  deck=dict(_SA, iturb=2, ilim=4, fsmach=0.3, alfa=6.0, rey=3.0e6, dt=5.0)),
 }
 
 # Algorithmic HBM bytes per interior cell and step of the minimal three-pass
@@ -71,12 +79,14 @@ WORKLOADS = {
 PASS_BYTES = {
     "euler": {"A": 124, "B": 136, "C": 140},
     "sa": {"A": 132, "B": 144, "C": 148, "S1": 140, "S2": 48, "S3": 80},
+    # SST: two transport equations (about 840 B per cell-iteration in all)
+    "sst": {"A": 132, "B": 144, "C": 148, "S1": 232, "S2": 96, "S3": 88},
 }
 STAGE_PASS = {
     "bc": "A", "residual": "A", "visrhs": "A", "tsrhs": "A", "norms": "A", "rbc": "A",
     "time_step": "A", "lu_prep": "B", "lu_forward": "B", "lu_backward": "C", "lu_finish": "C",
     "update": "C", "monitor": "C", "sa_prim": "S1", "sa_bc": "S1", "sa_rhs": "S1",
-    "sa_solve_j": "S2", "sa_solve_k": "S3", "sa_eddy": "S3",
+    "sa_solve_j": "S2", "sa_solve_k": "S3", "sa_eddy": "S3", "interp": "A",
 }
 
 
@@ -187,6 +197,21 @@ def make_grid(w):
                             wake_aspect=w.get("wake_aspect"), te_cluster=w.get("te_cluster", True))
 
 
+def make_overset_case(w):
+    """Two-mesh case with its connectivity found on the device; returns
+    (case, interior cells of both meshes, fringe cells)."""
+    import turns2d_urans_git_amd as t2d
+    ms = grid.airfoil_overset(near=w["near"], back=w["back"], ds_wall=w["ds_wall"])
+    p = t2d.default_params(**w["deck"], lu_fp32=w["lu_fp32"])
+    mds = [abi.mesh_desc(m.xv.shape[0], m.xv.shape[1], m.bc) for m in ms]
+    case = t2d.Case(p, mds)
+    for im, m in enumerate(ms):
+        case.set_grid(im, m.xv, m.yv)
+    nfr = case.connect([m.ihc for m in ms])
+    cells = sum((m.xv.shape[0] - 1) * (m.xv.shape[1] - 1) for m in ms)
+    return case, cells, nfr, ms
+
+
 def make_gpu_case(w, g):
     import turns2d_urans_git_amd as t2d
     p = t2d.default_params(**w["deck"], lu_fp32=w["lu_fp32"])
@@ -236,6 +261,55 @@ def reference_seconds_per_step(w, g, n1, n2, timeout):
     return (times[1] - times[0]) / (n2 - n1), times
 
 
+def reference_overset_seconds_per_step(w, ms, n1, n2, timeout):
+    """As reference_seconds_per_step, on the overset deck (connect2d once,
+    do_interpolations every iteration)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "overturns2d")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    times = []
+    for n in (n1, n2):
+        with tempfile.TemporaryDirectory() as d:
+            grid.write_overset_case(d, ms, nsteps=n, nrest=10**7, npnorm=10**7, **w["deck"])
+            with open(os.path.join(d, "inputs")) as fin:
+                t0 = time.perf_counter()
+                subprocess.run([exe], cwd=d, stdin=fin, env=env, check=True, timeout=timeout,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                times.append(time.perf_counter() - t0)
+    return (times[1] - times[0]) / (n2 - n1), times
+
+
+def run_serve(args):
+    """The request loop on a Poisson stream; one JSON line of latency and
+    throughput figures."""
+    import turns2d_urans_git_amd as t2d
+    from turns2d_urans_git_amd import serve
+    reqs = serve.poisson_stream(args.rate, args.duration, scale=args.serve_scale)
+    srv = serve.Server(t2d.load_library(), steps_per_tick=args.serve_tick, tol=1e-10,
+                       max_steps=args.serve_max_steps, errfn=t2d.last_error)
+    print(f"[bench] serve: {len(reqs)} requests over {args.duration} s at {args.rate}/s, grids x{args.serve_scale}",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, time_limit=args.serve_time_limit)
+    wall = time.perf_counter() - t0
+    s = serve.summarize(res, wall)
+    out = {
+        "metric": "solves/hr", "value": s["solves_per_hr"], "unit": "solves/hr", "n_gpus": 1,
+        "steps": srv.ticks * args.serve_tick, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32-lu",
+        "data": ("synthetic Poisson stream: 70 percent polar points of the 64-case distributions, 30 percent "
+                 f"time-spectral pitching cases (N in 3,5,7), grids scaled by {args.serve_scale:g}"),
+        "config": {"workload": "serve", "rate_per_s": args.rate, "duration_s": args.duration,
+                   "grid_scale": args.serve_scale, "steps_per_tick": args.serve_tick,
+                   "max_steps": args.serve_max_steps},
+        "serve": dict(s, peak_bytes=srv.peak_bytes, budget_bytes=srv.budget,
+                      not_converged=sum(1 for r in res if r.status == -100),
+                      failed=sum(1 for r in res if r.status not in (abi.CONVERGED, -100) and r.finished == r.finished)),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def port_seconds_per_step(w, g, n):
     """Wall seconds per iteration of the C restatement (oracle/liboracle.so)."""
     from tests import helpers as H
@@ -251,7 +325,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="sa_513x129", choices=sorted(WORKLOADS) + ["batch64_sa"])
+    ap.add_argument("--workload", default="sa_513x129", choices=sorted(WORKLOADS) + ["batch64_sa", "serve"])
+    ap.add_argument("--rate", type=float, default=1.0, help="serve: arrivals per second")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals")
+    ap.add_argument("--serve-scale", type=float, default=0.5, help="serve: grid size factor")
+    ap.add_argument("--serve-tick", type=int, default=200, help="serve: steps between admissions")
+    ap.add_argument("--serve-max-steps", type=int, default=30000)
+    ap.add_argument("--serve-time-limit", type=float, default=None)
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-chunk", type=int, default=1000)
     ap.add_argument("--profile-steps", type=int, default=100,
@@ -273,14 +353,23 @@ def main():
     if args.workload == "batch64_sa":
         run_batch(args, _hip())
         return
+    if args.workload == "serve":
+        run_serve(args)
+        return
 
     w = WORKLOADS[args.workload]
-    g = make_grid(w)
-    cells = (g.jmax - 1) * (g.kmax - 1)
     hip = _hip()
-    case = make_gpu_case(w, g)
-
-    print(f"[bench] {args.workload}: {g.jmax}x{g.kmax}, warmup {args.warmup}", file=sys.stderr, flush=True)
+    overset = "near" in w
+    if overset:
+        case, cells, nfr, ms = make_overset_case(w)
+        g = None
+        print(f"[bench] {args.workload}: {w['near']} + {w['back']}, {nfr} fringe cells, warmup {args.warmup}",
+              file=sys.stderr, flush=True)
+    else:
+        g = make_grid(w)
+        cells = (g.jmax - 1) * (g.kmax - 1)
+        case = make_gpu_case(w, g)
+        print(f"[bench] {args.workload}: {g.jmax}x{g.kmax}, warmup {args.warmup}", file=sys.stderr, flush=True)
     rc, _, done = case.iterate(args.warmup)
     if rc != 0 or done != args.warmup:
         raise SystemExit(f"warmup stopped: rc={rc} after {done} steps")
@@ -309,11 +398,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+f32-lu" if w["lu_fp32"] else "f64",
         "data": "synthetic (analytic NACA 4-digit C-grid, free-stream start)",
-        "config": {"workload": args.workload, "jmax": g.jmax, "kmax": g.kmax,
-                   "interior_cells": cells, "model": w["model"],
-                   "lu_sgs": "fp32" if w["lu_fp32"] else "fp64", "residual": "fp64",
-                   "wall_spacing": w["ds_wall"], "outer_radius": 20.0,
-                   "residual_l2": float(hist[-1, 0])},
+        "config": ({"workload": args.workload, "jmax": g.jmax, "kmax": g.kmax,
+                    "interior_cells": cells, "model": w["model"],
+                    "lu_sgs": "fp32" if w["lu_fp32"] else "fp64", "residual": "fp64",
+                    "wall_spacing": w["ds_wall"], "outer_radius": 20.0,
+                    "residual_l2": float(hist[-1, 0])} if not overset else
+                   {"workload": args.workload, "near_body": list(w["near"]), "background": list(w["back"]),
+                    "interior_cells": cells, "fringe_cells": nfr, "model": "sst+weno5",
+                    "lu_sgs": "fp32" if w["lu_fp32"] else "fp64", "residual": "fp64",
+                    "residual_l2": [float(v) for v in hist[-1]]}),
     }
 
     if args.profile_steps > 0:
@@ -336,7 +429,7 @@ def main():
         print(f"[bench] profiled {pdone} steps: dominant {rl['kernel']} {rl['launch_ms']:.4f} ms",
               file=sys.stderr, flush=True)
 
-    if args.converge > 0:
+    if args.converge > 0 and not overset:
         # a fresh case from free stream until the reference's L2 norm (rsum,
         # src/step.f:60-93) reaches 1e-10; the free-stream start has a
         # near-zero first residual, so the drop is also given from the peak
@@ -369,7 +462,8 @@ def main():
     if args.cpu_steps:
         n1, n2 = (int(x) for x in args.cpu_steps.split(","))
         print(f"[bench] cpu baseline: reference program, {n1} and {n2} steps", file=sys.stderr, flush=True)
-        res = reference_seconds_per_step(w, g, n1, n2, timeout=600)
+        res = (reference_overset_seconds_per_step(w, ms, n1, n2, timeout=600) if overset
+               else reference_seconds_per_step(w, g, n1, n2, timeout=600))
         if res is not None:
             sps, times = res
             out["cpu_baseline"] = {
@@ -378,14 +472,15 @@ def main():
                            f"same deck, runs of {n1} and {n2} steps differenced "
                            f"({times[0]:.1f} s, {times[1]:.1f} s wall), OMP_NUM_THREADS=1"),
             }
-        else:
+        elif not overset:
             n = 5
             sps = port_seconds_per_step(w, g, n)
             out["cpu_baseline"] = {
                 "value": cells / sps, "unit": "cell-iter/s", "cores": 1, "kind": "port",
                 "sample": f"oracle/liboracle.so (C restatement), same deck, {n} steps, 1 thread",
             }
-        out["cpu_baseline"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        if "cpu_baseline" in out:
+            out["cpu_baseline"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
 
     if args.copy_probe_mib > 0:
         # a plain device-to-device copy of known size (the runtime's copy

@@ -100,3 +100,42 @@ def test_converged_case_stops_on_the_device():
     st2, ist2 = case.batch_status()
     assert st2[0] == abi.CONVERGED and int(ist2[0]) == s_first
     np.testing.assert_array_equal(case.get_state(0), q1)
+
+
+def test_concurrent_cohorts_do_not_change_results():
+    """Cases advanced on separate streams at the same time give the results
+    they give alone, bit for bit (FP32 and FP64 LU-SGS, steady and
+    time-spectral cohorts side by side)."""
+    import turns2d_urans_git_amd as t2d
+    lib, err = _lib()
+
+    def batch(reqs):
+        ps, mds = [], []
+        for r in reqs:
+            ps.append(t2d.default_params(**r.deck))
+            mds.append(abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1)))
+        b = t2d.Batch(ps, mds)
+        for i, r in enumerate(reqs):
+            b.set_grid(i, r.grid.xv, r.grid.yv)
+        return b
+
+    target = _polar(0, 0.0, m=0.03, alfa=3.0)
+    alone = batch([target])
+    alone.iterate(600)
+    ref = alone.get_state(0)
+    alone.close()
+    for trial in range(3):
+        x = batch([target])
+        others = [batch([_polar(1, 0.0, alfa=5.0), _polar(2, 0.0, jmax=49, alfa=2.0)]),
+                  batch([_polar(3, 0.0, fp32=0)]), batch([_pitch(4, 0.0, n=3)])]
+        for _ in range(12):
+            x.iterate_async(50)
+            for o in others:
+                o.iterate_async(50)
+            x.batch_status()
+            for o in others:
+                o.batch_status()
+        np.testing.assert_array_equal(x.get_state(0), ref, err_msg=f"trial {trial}")
+        x.close()
+        for o in others:
+            o.close()
