@@ -37,6 +37,12 @@ module ot2d_capi
     integer(c_int32_t) :: reserved(7)
   end type ot2d_params
 
+  ! one ihcbc.<n> file (ihc/src/ihc.f90:147-208)
+  type, bind(C) :: ot2d_ihcbc
+    integer(c_int32_t) :: nbc
+    integer(c_int32_t) :: ibtyp(25), ibdir(25), jbcs(25), jbce(25), kbcs(25), kbce(25)
+  end type ot2d_ihcbc
+
   interface
     integer(c_int32_t) function ot2d_default_params(p) bind(C, name='ot2d_default_params')
       import :: c_int32_t, ot2d_params
@@ -133,6 +139,33 @@ module ot2d_capi
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s
     end function c_strlen
+    ! overset meshes: connect2d on the host, do_interpolations in every step
+    integer(c_int32_t) function ot2d_connect(case, ihcbc, nfringe) bind(C, name='ot2d_connect')
+      import :: c_int32_t, c_ptr, ot2d_ihcbc
+      type(c_ptr), value :: case
+      type(ot2d_ihcbc), intent(in) :: ihcbc(*)
+      integer(c_int32_t), intent(out) :: nfringe
+    end function ot2d_connect
+    integer(c_int32_t) function ot2d_interpolate(case) bind(C, name='ot2d_interpolate')
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: case
+    end function ot2d_interpolate
+    ! serving primitives
+    integer(c_int32_t) function ot2d_set_convergence(case, icase, tol) bind(C, name='ot2d_set_convergence')
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: case
+      integer(c_int32_t), value :: icase
+      real(c_double), value :: tol
+    end function ot2d_set_convergence
+    integer(c_int32_t) function ot2d_iterate_async(case, niter) bind(C, name='ot2d_iterate_async')
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: case
+      integer(c_int32_t), value :: niter
+    end function ot2d_iterate_async
+    integer(c_int32_t) function ot2d_sync(case) bind(C, name='ot2d_sync')
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: case
+    end function ot2d_sync
   end interface
 
   integer(c_int32_t), parameter :: OT2D_F_X = 0, OT2D_F_Y = 1
