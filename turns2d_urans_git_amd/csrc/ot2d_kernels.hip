@@ -1107,7 +1107,7 @@ struct LUGeom {
   long long nrec;                  // FP64: entries of F and of B
   long long off_b, off_hcf, off_hcb, off_hrf, off_hrb;   // FP64 arrays (16-byte vectors)
   long long ncell;                 // FP32: interior cells
-  long long off_f32, off_b32, off_h, off_mbf, off_mbb;   // FP32 arrays
+  long long off_f32, off_b32, off_cb, off_h, off_mbf, off_mbb;   // FP32 arrays
   long long total;                 // vectors the workspace must hold (either layout)
 };
 
@@ -1126,12 +1126,14 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   g.off_hrf = g.off_hcb + LU_NVEC * hc;
   g.off_hrb = g.off_hrf + (long long)LU_NVEC * ni;
   long long t64 = g.off_hrb + (long long)LU_NVEC * ni;
-  // FP32: per-cell records (AoS, interior cells j fastest), the step-layout
-  // hand-off between the sweeps, the inter-workgroup mailboxes
+  // FP32: per-cell records (AoS, interior cells j fastest), cB and the
+  // hand-off between the sweeps in step layout, the inter-workgroup
+  // mailboxes
   g.ncell = (long long)ni * nk;
   g.off_f32 = 0;
-  g.off_b32 = g.off_f32 + 10 * g.ncell;
-  g.off_h = g.off_b32 + 8 * g.ncell;
+  g.off_b32 = g.off_f32 + 9 * g.ncell;
+  g.off_cb = g.off_b32 + 8 * g.ncell;
+  g.off_h = g.off_cb + (long long)g.W * g.TW * 64;
   g.off_mbf = g.off_h + (long long)g.W * g.TW * 64;
   g.off_mbb = g.off_mbf + (long long)g.W * ni;
   long long t32 = g.off_mbb + (long long)g.W * ni;
@@ -1478,13 +1480,18 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // as 4x4 matrices).  Neighbours in the halo ring hold fixed point-Jacobi
 // values (src/lhs.f:568-598), so their terms are folded into u (forward)
 // and into cB (backward, added by the forward sweep to its output).
-//   F record (10 float4 per interior cell): u, cB, PA rows 0-3, PB rows 0-3
-//   B record ( 8 float4 per interior cell): PA' rows 0-3, PB' rows 0-3
-//   H (one float4 per cell in sweep-step order): s_fwd + cB written by the
-//     forward sweep, overwritten with the result by the backward sweep
+//   F record (9 float4 per interior cell): u, PA rows 0-3, PB rows 0-3
+//   B record (8 float4 per interior cell): PA' rows 0-3, PB' rows 0-3
+//   cB (one float4 per cell in sweep-step order, as H)
+//   H (one float4 per cell in sweep-step order): s_fwd written by the
+//     forward sweep (the backward producers add cB to it), overwritten with
+//     the result by the backward sweep
 // The halo-ring s values themselves are scaled in k_lu_finish (the prep
 // reads the unscaled neighbours).
-constexpr int LU32_NVF = 10, LU32_NVB = 9;
+// Vectors per sweep step in an LDS ring slot: forward u, PA rows, PB rows;
+// backward s_fwd + cB, PA' rows, PB' rows (one more vector carries the
+// mailbox value).
+constexpr int LU32_NV = 9;
 // bit pattern of an empty mailbox word: a signalling NaN, which no
 // arithmetic result takes (they are quiet NaNs at worst)
 constexpr unsigned LU32_EMPTY = 0x7fa5a5a5u;
@@ -1533,6 +1540,7 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
   const LUGeom g = lu_geom(M);
   F4* recf = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_f32);
   F4* recb = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_b32);
+  F4* cbs = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_cb);
   // both mailboxes (2 * W * ni float4) start as the empty sentinel
   const long long nmb = 2ll * g.W * g.ni;
   u32x4_t* mb = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_mbf);
@@ -1553,12 +1561,14 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
     lu32_term(M, i + 1, j == je, 1, -1, -f, eps2, epsv, QA, cb);
     lu32_term(M, i + jd, k == ke, 0, -1, -f, eps2, epsv, QB, cb);
     auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
-    F4* rf = recf + 10 * ic;
+    F4* rf = recf + 9 * ic;
     F4* rb = recb + 8 * ic;
     rf[0] = f4(u);
-    rf[1] = f4(cb);
-    for (int r = 0; r < 4; ++r) { rf[2 + r] = f4(PA + 4 * r); rf[6 + r] = f4(PB + 4 * r); }
+    for (int r = 0; r < 4; ++r) { rf[1 + r] = f4(PA + 4 * r); rf[5 + r] = f4(PB + 4 * r); }
     for (int r = 0; r < 4; ++r) { rb[r] = f4(QA + 4 * r); rb[4 + r] = f4(QB + 4 * r); }
+    // cB at the cell's forward step: read by the backward producers one
+    // step-row (1 KiB) at a time
+    cbs[lu_entry32(M, g, j, k, +1)] = f4(cb);
   }
 }
 
@@ -1571,13 +1581,15 @@ constexpr unsigned long long LU32_WAIT = 50000000ull;
 
 __device__ __forceinline__ unsigned long long rt_now() { return __builtin_amdgcn_s_memrealtime(); }
 
+template <int SLEEP>
 __device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_flag) {
+  if (__hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return true;
   const unsigned long long t0 = rt_now();
   for (;;) {
     if (__hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) return true;
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
     if (rt_now() - t0 > LU32_WAIT) break;
-    __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_s_sleep(SLEEP);
   }
   __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   return false;
@@ -1624,8 +1636,20 @@ __device__ __forceinline__ float dpp_shift_in(float v, float edge) {
 // block w is one workgroup (on its own CU), lane l owning row ks+64w+l and
 // walking along j one column per step, lane l one step behind lane l-1.
 // Inside the workgroup, wave 0 computes and waves 1..NP stream the records
-// of step t into LDS slot t % S (ready[slot] = t+1 once written; the
-// computing wave publishes consumed = t+1 when slot t may be reused).
+// of step t into LDS slot t % S; each producer wave keeps the loads of PD
+// of its steps in flight.  Hand-off protocol (LDS, workgroup scope):
+//   tag of slot t % S = t + 1   stored (release) by the producer after the
+//                               records of step t
+//   consumed = c                stored by the computing wave once the reads
+//                               of steps < c have executed, so a producer
+//                               may fill slot t % S when consumed >= t-S+1
+// The computing wave issues the tag read of step t+1 and then its record
+// reads while it computes step t, and checks the tag when it starts step
+// t+1.  This relies on the LDS executing one wave's operations in issue
+// order: the reads that follow a fresh tag see the records stored before
+// it, and the reads of a slot execute before the consumed store that
+// frees it.  Compiler barriers (no instructions) keep that program order
+// without the waits a fence would add.
 // Across workgroups the edge row passes through a mailbox per column in
 // global memory: block w's edge lane stores its result with device-scope
 // atomics, the producer waves of the neighbouring block wait for it and
@@ -1633,11 +1657,13 @@ __device__ __forceinline__ float dpp_shift_in(float v, float edge) {
 // the fill value of the wavefront shift.  Forward, block w waits on block
 // w-1; backward, on block w+1; the dispatch order (blockIdx.x) puts the
 // block waited on first, so every wait is on a workgroup already running.
-template <int DIR, int NV, int S, int NP>
+template <int DIR, int S, int NP, int PD>
 __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __restrict__ Ms) {
+  constexpr int NV = LU32_NV;
   extern __shared__ __attribute__((aligned(16))) double lds_raw[];
-  __shared__ int ready[S];
   __shared__ int consumed, abort_flag, st0;
+  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][NV+1][64]; vector NV: lane 0 mailbox value, lane 1 tag
+  auto tag_of = [&](int slot) { return reinterpret_cast<int*>(ring + ((size_t)slot * (NV + 1) + NV) * 64 + 1); };
   const DevMesh& M = Ms[blockIdx.y];
   const LUGeom g = lu_geom(M);
   if (threadIdx.x == 0) {
@@ -1645,120 +1671,161 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
     consumed = 0;
     abort_flag = 0;
   }
-  if (threadIdx.x < S) ready[threadIdx.x] = 0;
+  if (threadIdx.x < S) *tag_of(threadIdx.x) = 0;
   __syncthreads();
   const int wb = blockIdx.x;
   if (st0 || wb >= g.W) return;
   const int w = DIR > 0 ? wb : g.W - 1 - wb;   // row block of this workgroup
-  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][NV+1][64]; vector NV lane 0: mailbox value
   const int NT = g.TW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   D2* lu = reinterpret_cast<D2*>(M.lu);
+  const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
+  const int base = w * NT;
   if (wave > 0) {
     // ---------------------------------------------------------- producers
     // per-cell records: lane l's cell at step tt is (js+tt-l, ks+64w+l)
-    // forward, (je-tt+63-l, ks+64w+l) backward (clamped into the grid for
-    // lanes outside it, whose values are never used)
-    constexpr int NR = DIR > 0 ? 10 : 8;
-    const __amdgpu_buffer_rsrc_t rec = uniform_rsrc(lu + (DIR > 0 ? g.off_f32 : g.off_b32), g.ncell * NR * 16);
-    const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
-    const int krow = (64 * w + l) < g.nk ? 64 * w + l : g.nk - 1;
-    // mailbox written by the neighbour block (none at the grid edge)
+    // forward, (je-tt+63-l, ks+64w+l) backward.  For a cell outside the
+    // grid the offset points past the end of the records, where buffer
+    // loads return zeros: such lanes compute exact zeros from finite
+    // inputs.  Steps past the end are clamped to the last one; the loop
+    // body has no conditional loads.
+    const __amdgpu_buffer_rsrc_t recf = uniform_rsrc(lu + g.off_f32, g.ncell * 9 * 16);
+    const __amdgpu_buffer_rsrc_t recb = uniform_rsrc(lu + g.off_b32, g.ncell * 8 * 16);
+    const __amdgpu_buffer_rsrc_t cbr = uniform_rsrc(lu + g.off_cb, (long long)g.W * g.TW * 64 * 16);
+    const int krow = 64 * w + l;
+    const bool rowin = krow < g.nk;
+    // mailbox written by the neighbour block; at the grid edge (no
+    // neighbour) this block's own, read and ignored
     const bool has_in = DIR > 0 ? w > 0 : w < g.W - 1;
-    gu32* mb_in = has_in ? (gu32*)(lu + (DIR > 0 ? g.off_mbf + (long long)(w - 1) * g.ni
-                                                  : g.off_mbb + (long long)(w + 1) * g.ni))
-                         : nullptr;
-    const int base = w * NT;
-    // the records of step tt and, issued with them, the neighbour's mailbox
-    // entry for the edge lane's column (forward lane 0 at j = js+tt,
-    // backward lane 63 at j = je-tt); written some 64 steps earlier, it is
-    // normally already there when checked
-    auto mail_ptr = [&](int tt) { return mb_in + 4ll * (DIR > 0 ? tt : g.ni - 1 - tt); };
-    auto fetch = [&](int tt, u32x4_t (&v)[NV + 1]) {
-      int jc = DIR > 0 ? tt - l : (g.ni - 1) - tt + (63 - l);
-      jc = jc < 0 ? 0 : (jc >= g.ni ? g.ni - 1 : jc);
-      const int vo = (int)(((long long)krow * g.ni + jc) * NR * 16);
-      if constexpr (DIR > 0) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(rec, vo + q * 16, 0, 0);
-      } else {
-        v[0] = __builtin_amdgcn_raw_buffer_load_b128(hnd, l * 16, (base + tt) * 1024, 0);
-#pragma unroll
-        for (int q = 0; q < NR; ++q) v[1 + q] = __builtin_amdgcn_raw_buffer_load_b128(rec, vo + q * 16, 0, 0);
-      }
-      v[NV] = u32x4_t{0u, 0u, 0u, 0u};
-      if (has_in && tt < g.ni) {
-        gu32* p = mail_ptr(tt);
-        v[NV] = u32x4_t{__hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                        __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                        __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                        __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
-      }
+    gu32* mb_in = (gu32*)(lu + (DIR > 0 ? g.off_mbf : g.off_mbb) + (long long)(has_in ? w - DIR : w) * g.ni);
+    // the neighbour's mailbox entry for the edge lane's column at step tt
+    // (forward lane 0 at j = js+tt, backward lane 63 at j = je-tt)
+    auto mail_ptr = [&](int tt) {
+      int c = DIR > 0 ? tt : g.ni - 1 - tt;
+      c = c < 0 ? 0 : (c >= g.ni ? g.ni - 1 : c);
+      return mb_in + 4ll * c;
     };
-    auto deliver = [&](int tt, u32x4_t (&v)[NV + 1]) -> bool {
-      if (has_in && tt < g.ni &&
-          (v[NV].x == LU32_EMPTY || v[NV].y == LU32_EMPTY || v[NV].z == LU32_EMPTY || v[NV].w == LU32_EMPTY))
-        if (!wait_mail(mail_ptr(tt), v[NV], &abort_flag)) return false;
-      if (!spin_until_ge(&consumed, tt - S, &abort_flag)) return false;
+    constexpr int NB = NV + 1 + (DIR < 0 ? 1 : 0);   // backward: + cB of the forward record
+    auto fetch = [&](int tt, u32x4_t (&v)[NB]) {
+      const int tc = tt < NT ? tt : NT - 1;
+      const int jr = DIR > 0 ? tc - l : (g.ni - 1) - tc + (63 - l);
+      const bool cin = rowin && (unsigned)jr < (unsigned)g.ni;
+      const long long ci = cin ? (long long)krow * g.ni + jr : g.ncell;
+      if constexpr (DIR > 0) {
+        const int vo = (int)(ci * 144);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(recf, vo + 16 * q, 0, 0);
+      } else {
+        const int vo = (int)(ci * 128);
+        v[0] = __builtin_amdgcn_raw_buffer_load_b128(hnd, l * 16, (base + tc) * 1024, 0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[1 + q] = __builtin_amdgcn_raw_buffer_load_b128(recb, vo + 16 * q, 0, 0);
+        // cB of the cell, kept at its forward step NT-1-tc; zero outside
+        // the grid (those entries are never written)
+        const u32x4_t cb = __builtin_amdgcn_raw_buffer_load_b128(cbr, l * 16, (base + NT - 1 - tc) * 1024, 0);
+        v[NV + 1] = cin ? cb : u32x4_t{0u, 0u, 0u, 0u};
+      }
+      gu32* p = mail_ptr(tc);
+      v[NV] = u32x4_t{__hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+    };
+    auto deliver = [&](int tt, u32x4_t (&v)[NB]) -> bool {
+      if (tt >= NT) return true;
+      if (has_in && tt < g.ni) {
+        // written some 64 steps earlier, the entry is normally already there
+        if (v[NV].x == LU32_EMPTY || v[NV].y == LU32_EMPTY || v[NV].z == LU32_EMPTY || v[NV].w == LU32_EMPTY)
+          if (!wait_mail(mail_ptr(tt), v[NV], &abort_flag)) return false;
+      } else {
+        v[NV] = u32x4_t{0u, 0u, 0u, 0u};
+      }
+      if constexpr (DIR < 0) {
+        const float4 h = f4_of(v[0]), cb = f4_of(v[NV + 1]);
+        v[0] = __builtin_bit_cast(u32x4_t, F4{h.x + cb.x, h.y + cb.y, h.z + cb.z, h.w + cb.w});
+      }
+      if (!spin_until_ge<4>(&consumed, tt - S + 1, &abort_flag)) return false;
       u32x4_t* d = ring + (size_t)(tt % S) * (NV + 1) * 64;
 #pragma unroll
       for (int q = 0; q < NV; ++q) d[q * 64 + l] = v[q];
       if (l == 0) d[NV * 64] = v[NV];
-      __hip_atomic_store(&ready[tt % S], tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (l == 0) __hip_atomic_store(tag_of(tt % S), tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       return true;
     };
-    u32x4_t va[NV + 1], vb[NV + 1];
-    int tt = wave - 1;
-    if (tt < NT) fetch(tt, va);
-    for (; tt < NT; tt += 2 * NP) {
-      const int t2 = tt + NP;
-      if (t2 < NT) fetch(t2, vb);
-      if (!deliver(tt, va)) break;
-      if (t2 >= NT) break;
-      const int t3 = t2 + NP;
-      if (t3 < NT) fetch(t3, va);
-      if (!deliver(t2, vb)) break;
+    // producer wave p handles steps p-1, p-1+NP, ...; buffer k % PD holds the
+    // k-th of them, fetched PD-1 deliveries ahead
+    u32x4_t vb[PD][NB];
+    const int t0 = wave - 1;
+#pragma unroll
+    for (int k = 0; k < PD - 1; ++k) fetch(t0 + k * NP, vb[k]);
+    bool ok = true;
+    for (int tb = t0; tb < NT && ok; tb += PD * NP) {
+#pragma unroll
+      for (int k = 0; k < PD; ++k) {
+        const int tt = tb + k * NP;
+        fetch(tt + (PD - 1) * NP, vb[(k + PD - 1) % PD]);
+        if (ok) ok = deliver(tt, vb[k]);
+      }
     }
     return;
   }
   // ------------------------------------------------------------- compute
-  const int js = M.js, je = M.je;
-  const int rows = g.nk - 64 * w < 64 ? g.nk - 64 * w : 64;
-  __amdgpu_buffer_rsrc_t out = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
+  // Lanes whose cell lies outside the grid compute exact zeros (their
+  // records read as zeros), and a lane inside the grid takes its
+  // neighbours' values only from cells inside the grid or through zero
+  // matrices (the halo terms are folded into u and cB): no lane needs a
+  // select.
+  __builtin_amdgcn_s_setprio(3);
   const bool has_out = DIR > 0 ? w < g.W - 1 : w > 0;
-  const bool edge_out = DIR > 0 ? l == 63 : l == 0;
+  const int edge = DIR > 0 ? 63 : 0;
+  // the edge lane's cell is inside the grid at steps 63 .. ni+62 when the
+  // block's last row exists (every block with a neighbour after it is full)
   gu32* mb_out = (gu32*)(lu + (DIR > 0 ? g.off_mbf : g.off_mbb) + (long long)w * g.ni);
-  const int base = w * NT;
   float sA[4] = {0.f, 0.f, 0.f, 0.f};
-  // k neighbour leaving this lane.  Inactive lanes pass on what they
-  // receive, and the wavefront shift must run with every lane enabled (a
-  // disabled source lane reads as the fill value): no select or branch may
-  // guard it.
+  // k neighbour leaving this lane; the wavefront shift runs with every lane
+  // enabled (a disabled source lane reads as the fill value)
   float kout[4] = {0.f, 0.f, 0.f, 0.f};
-  auto rd = [&](int t, u32x4_t (&v)[NV + 1]) {
-    const u32x4_t* s = ring + (size_t)(t % S) * (NV + 1) * 64;
+  // tag, then records, of the step in `slot`, in this order
+  auto rd = [&](int slot, int& tag, u32x4_t (&v)[NV + 1]) {
+    tag = __hip_atomic_load(tag_of(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    asm volatile("" ::: "memory");
+    const u32x4_t* s = ring + (size_t)slot * (NV + 1) * 64;
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = s[q * 64 + l];
     v[NV] = s[NV * 64];
   };
-  auto step = [&](int t, const u32x4_t (&C)[NV + 1], u32x4_t (&N)[NV + 1]) {
-    if (t + 1 < NT) rd(t + 1, N);
-    const int f2 = t + 2 < NT ? __hip_atomic_load(&ready[(t + 2) % S], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                              : 0;
-    const int j = DIR > 0 ? js + t - l : je - t + (63 - l);
-    const bool v = l < rows && j >= js && j <= je;
+  bool ok = true;
+  int tagN = 0;   // tag read with the records of the next step
+  // step t from slot t % S (= slot, a constant once the loop below is
+  // unrolled); reads step t+1 into N
+  auto step = [&](int slot, int t, u32x4_t (&C)[NV + 1], u32x4_t (&N)[NV + 1]) {
+    const int s1 = slot + 1 < S ? slot + 1 : 0;
+    // compared where it is used (as a vector value: the compiler would
+    // otherwise move it to a scalar register, waiting for it, as soon as
+    // it is loaded)
+    int tg = tagN;
+    asm volatile("" : "+v"(tg));
+    if (tg < t + 1) {
+      // the records of step t were read before the producer stored them:
+      // wait for the tag and read them again
+      ok = ok && spin_until_ge<1>(tag_of(slot), t + 1, &abort_flag);
+      rd(slot, tg, C);
+    }
+    // the reads of steps <= t execute before this store
+    asm volatile("" ::: "memory");
+    __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    rd(s1, tagN, N);   // past the last step: a harmless read of a stale slot
     const float4 mail = f4_of(C[NV]);
     float kin[4];
     kin[0] = dpp_shift_in<DIR>(kout[0], mail.x);
     kin[1] = dpp_shift_in<DIR>(kout[1], mail.y);
     kin[2] = dpp_shift_in<DIR>(kout[2], mail.z);
     kin[3] = dpp_shift_in<DIR>(kout[3], mail.w);
-    const int pa = DIR > 0 ? 2 : 1, pb = pa + 4;
     const float4 u = f4_of(C[0]);
     float sn[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      float4 a = f4_of(C[pa + q]), b = f4_of(C[pb + q]);
+      float4 a = f4_of(C[1 + q]), b = f4_of(C[5 + q]);
       float uq = q == 0 ? u.x : (q == 1 ? u.y : (q == 2 ? u.z : u.w));
       float p = __builtin_fmaf(a.x, sA[0], uq);
       p = __builtin_fmaf(a.y, sA[1], p);
@@ -1769,41 +1836,36 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
       p = __builtin_fmaf(b.z, kin[2], p);
       sn[q] = __builtin_fmaf(b.w, kin[3], p);
     }
-    if (v) {
-      float o[4] = {sn[0], sn[1], sn[2], sn[3]};
-      if (DIR > 0) {
-        float4 cb = f4_of(C[1]);
-        o[0] += cb.x; o[1] += cb.y; o[2] += cb.z; o[3] += cb.w;
-      }
-      const int ostep = DIR > 0 ? NT - 1 - t : t;
-      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{o[0], o[1], o[2], o[3]}), out, l * 16,
-                                             (base + ostep) * 1024, 0);
-      if (has_out && edge_out) {
-        gu32* p = mb_out + 4ll * (j - js);
+    // H entry of (step, lane): every lane stores (the entries of cells
+    // outside the grid are never read as results)
+    const int ostep = DIR > 0 ? NT - 1 - t : t;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}), hnd, l * 16,
+                                           (base + ostep) * 1024, 0);
+    if (has_out && t >= 63 && t < g.ni + 63 && l == edge) {
+      const int jm = DIR > 0 ? t - 63 : g.ni - 1 - (t - 63);   // column index j - js of the edge lane
+      gu32* p = mb_out + 4ll * jm;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          __hip_atomic_store(p + q, __builtin_bit_cast(unsigned, sn[q]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) { sA[q] = sn[q]; kin[q] = sn[q]; }
+      for (int q = 0; q < 4; ++q)
+        __hip_atomic_store(p + q, __builtin_bit_cast(unsigned, sn[q]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) kout[q] = kin[q];
-    bool okk = true;
-    if (t + 2 < NT && f2 < t + 3) okk = spin_until_ge(&ready[(t + 2) % S], t + 3, &abort_flag);
-    // slot t+2 is published: keep the next step's reads of it after this point
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-    return okk;
+    for (int q = 0; q < 4; ++q) {
+      sA[q] = sn[q];
+      kout[q] = sn[q];
+    }
   };
   u32x4_t C0[NV + 1], C1[NV + 1];
-  bool ok = spin_until_ge(&ready[0], 1, &abort_flag);
-  if (ok && NT > 1) ok = spin_until_ge(&ready[1 % S], 2, &abort_flag);
-  if (ok) rd(0, C0);
-  // two steps per trip, the two register sets alternating (no copies)
-  for (int t = 0; t < NT && ok; t += 2) {
-    ok = step(t, C0, C1);
-    if (t + 1 < NT && ok) ok = step(t + 1, C1, C0);
+  ok = spin_until_ge<1>(tag_of(0), 1, &abort_flag);
+  rd(0, tagN, C0);
+  // S steps per trip (S even), the two register sets alternating, so that
+  // every slot index and LDS offset is a constant
+  static_assert(S % 2 == 0, "the ring holds an even number of steps");
+  for (int t0 = 0; t0 < NT && ok; t0 += S) {
+#pragma unroll
+    for (int i = 0; i < S; i += 2) {
+      if (t0 + i < NT) step(i, t0 + i, C0, C1);
+      if (t0 + i + 1 < NT) step(i + 1, t0 + i + 1, C1, C0);
+    }
   }
   if (!ok || __hip_atomic_load(&abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
     if (l == 0) atomicCAS(M.status, 0, 3);
@@ -2330,13 +2392,13 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
 }
 
 // LDS ring slots of the FP32 sweeps: as many slots of (NV+1) KiB as fit in
-// 150 KiB
-constexpr int lu32_slots(int nv) { return (150 * 1024) / ((nv + 1) * 1024); }
-constexpr int LU32_NP = 3;
+// 150 KiB, rounded down to even; producer waves and the steps each keeps in
+// flight
+constexpr int LU32_S = ((150 * 1024) / ((LU32_NV + 1) * 1024)) & ~1;
+constexpr int LU32_NP = 3, LU32_PD = 5;
 
 static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, Marks* mk,
                               hipStream_t st) {
-  constexpr int SF = lu32_slots(LU32_NVF), SB = lu32_slots(LU32_NVB);
   long long mx = 0, nc = 0;
   int wmax = 1;
   for (int i = 0; i < nmi; ++i) {
@@ -2345,23 +2407,24 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
     nc = (long long)ni * nk > nc ? (long long)ni * nk : nc;
     wmax = (nk + 63) / 64 > wmax ? (nk + 63) / 64 : wmax;
   }
-  const size_t ldsf = (size_t)SF * (LU32_NVF + 1) * 1024, ldsb = (size_t)SB * (LU32_NVB + 1) * 1024;
+  const size_t lds = (size_t)LU32_S * (LU32_NV + 1) * 1024;
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, LU32_NVF, SF, LU32_NP>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsf);
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, LU32_NVB, SB, LU32_NP>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb);
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, LU32_S, LU32_NP, LU32_PD>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, LU32_S, LU32_NP, LU32_PD>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   hipLaunchKernelGGL(k_lu_prep32, dim3(nblk(nc, 256), nmi), dim3(256), 0, st, dM);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_PREP, st);
-  hipLaunchKernelGGL((k_lu_sweep32<1, LU32_NVF, SF, LU32_NP>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), ldsf, st, dM);
+  hipLaunchKernelGGL((k_lu_sweep32<1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), lds, st,
+                     dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_FORWARD, st);
-  hipLaunchKernelGGL((k_lu_sweep32<-1, LU32_NVB, SB, LU32_NP>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), ldsb, st,
+  hipLaunchKernelGGL((k_lu_sweep32<-1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), lds, st,
                      dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_BACKWARD, st);
