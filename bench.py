@@ -111,6 +111,35 @@ def batch_cases(n=64, seed=BATCH_SEED):
     return out
 
 
+def converge_batch(t2d, hip, params, meshes, grids, steps, chunk, label):
+    """Advance one batch until every case has stopped (rsum <= 1e-10 on the
+    device) or `steps` steps; returns (status, steps per case, seconds,
+    steps run, forces per case)."""
+    b = t2d.Batch(params, meshes)
+    for i, g in enumerate(grids):
+        b.set_grid(i, g.xv, g.yv)
+    b.set_convergence(1e-10)
+    device_sync(hip)
+    t0 = time.perf_counter()
+    total = 0
+    while total < steps:
+        k = min(chunk, steps - total)
+        b.iterate_async(k)
+        st, ist = b.batch_status()
+        total += k
+        print(f"[bench] {label}: {total} steps, {int((st == abi.CONVERGED).sum())} converged, "
+              f"{int(((st == 1) | (st == 2)).sum())} stopped, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        if not np.any(st == 0):
+            break
+    device_sync(hip)
+    secs = time.perf_counter() - t0
+    st, ist = b.batch_status()
+    forces = [b.forces(i) for i in range(len(grids))]
+    b.close()
+    return st, ist, secs, total, forces
+
+
 def run_batch(args, hip):
     """The 64-case mix as one batch with FP32 LU-SGS: every case stops on the
     device at its first step with rsum <= 1e-10 (ot2d_set_convergence).
@@ -129,32 +158,11 @@ def run_batch(args, hip):
         meshes.append(abi.mesh_desc(g.jmax, g.kmax, grid.cgrid_bc(g.jtail1)))
     cells = np.array([(g.jmax - 1) * (g.kmax - 1) for g in grids], dtype=np.float64)
     print(f"[bench] batch of {len(cases)} cases, {cells.sum():.0f} interior cells", file=sys.stderr, flush=True)
-    b = t2d.Batch(params, meshes)
-    for i, g in enumerate(grids):
-        b.set_grid(i, g.xv, g.yv)
-    b.set_convergence(1e-10)
     n = len(cases)
-    chunk = args.batch_chunk
-    device_sync(hip)
-    t0 = time.perf_counter()
-    total = 0
-    while total < args.steps:
-        k = min(chunk, args.steps - total)
-        b.iterate_async(k)
-        st, ist = b.batch_status()
-        total += k
-        print(f"[bench] batch: {total} steps, {int((st == abi.CONVERGED).sum())} converged, "
-              f"{int(((st == 1) | (st == 2)).sum())} stopped, {time.perf_counter() - t0:.1f} s",
-              file=sys.stderr, flush=True)
-        if not np.any(st == 0):
-            break
-    device_sync(hip)
-    secs = time.perf_counter() - t0
-    st, ist = b.batch_status()
+    st, ist, secs, total, forces = converge_batch(t2d, hip, params, meshes, grids, args.steps, args.batch_chunk, "batch")
     conv = st == abi.CONVERGED
     nconv = int(conv.sum())
     done_work = float((cells * ist).sum())
-    forces = [b.forces(i) for i in range(n)]
     out = {
         "metric": "cell-iter/s",
         "value": done_work / secs,
@@ -173,7 +181,48 @@ def run_batch(args, hip):
                      "steps_to_converge_max": (int(ist[conv].max()) if nconv else None),
                      "cl": [round(f["cl"], 7) for f in forces], "cd": [round(f["cd"], 7) for f in forces]},
     }
-    b.close()
+    if args.fp64_batch:
+        # the same cases converged with FP64 LU-SGS (one batch): Cl and Cd of
+        # the mixed-precision solves against the all-FP64 ones
+        p64 = [t2d.default_params(**dict(_SA, fsmach=c["fsmach"], alfa=c["alfa"], rey=c["rey"]), lu_fp32=0)
+               for c in cases]
+        st64, ist64, secs64, _, f64 = converge_batch(t2d, hip, p64, meshes, grids, args.steps, args.batch_chunk,
+                                                     "fp64 batch")
+        both = conv & (st64 == abi.CONVERGED)
+        dcl = [abs(forces[i]["cl"] - f64[i]["cl"]) for i in range(n) if both[i]]
+        dcd = [abs(forces[i]["cd"] - f64[i]["cd"]) for i in range(n) if both[i]]
+        out["fp64_batch"] = {
+            "converged": int((st64 == abi.CONVERGED).sum()), "seconds": round(secs64, 2),
+            "solves_per_hr": 3600.0 * int((st64 == abi.CONVERGED).sum()) / secs64,
+            "compared": int(both.sum()),
+            "max_abs_dcl": max(dcl) if dcl else None, "max_abs_dcd": max(dcd) if dcd else None,
+            "steps_to_converge_median": (float(np.median(ist64[st64 == abi.CONVERGED]))
+                                         if np.any(st64 == abi.CONVERGED) else None),
+        }
+    if args.profile_steps > 0:
+        # eager steps of a fresh batch, every kernel stage timed with HIP
+        # events; per-pass GB/s over the batch's interior cells
+        pb = t2d.Batch(params, meshes)
+        for i, g in enumerate(grids):
+            pb.set_grid(i, g.xv, g.yv)
+        rc, stage_ms, pdone = pb.profile(args.profile_steps)
+        pb.close()
+        if rc != 0:
+            raise SystemExit(f"profiled batch stopped: rc={rc}")
+        rl, pass_gbs = roofline(stage_ms, "sa", int(cells.sum()))
+        tfile = os.path.join(ROOT, "profiles", "batch64_sa_traffic.json")
+        if os.path.exists(tfile):
+            t = json.load(open(tfile)).get("kernels", {}).get(rl["kernel"])
+            if t:
+                rl["traffic"] = t["read_bytes"] + t["write_bytes"]
+                rl["traffic_source"] = os.path.relpath(tfile, ROOT)
+        out["roofline"] = rl
+        out["kernels_ms"] = {k: round(v, 5) for k, v in sorted(stage_ms.items(), key=lambda kv: -kv[1])}
+        out["passes_gbs"] = pass_gbs
+        out["profiled_ms_per_step"] = round(sum(stage_ms.values()), 5)
+    if args.copy_probe_mib > 0:
+        copy_probe(hip, args.copy_probe_mib)
+        out["copy_probe_bytes"] = args.copy_probe_mib << 20
     if args.plain_steps > 0:
         # the plain FP64 path: every case alone, FP64 LU-SGS
         ms = np.zeros(n)
@@ -237,6 +286,22 @@ def make_gpu_case(w, g):
     case = t2d.Case(p, [md])
     case.set_grid(0, g.xv, g.yv)
     return case
+
+
+def copy_probe(hip, mib):
+    """A plain device-to-device copy of known size (the runtime's copy
+    kernel): the reference reading for scaling FETCH_SIZE/WRITE_SIZE."""
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise SystemExit("copy probe: hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    device_sync(hip)
+    if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):   # hipMemcpyDeviceToDevice
+        raise SystemExit("copy probe: hipMemcpy failed")
+    device_sync(hip)
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def roofline(stage_ms, model, cells):
@@ -352,6 +417,8 @@ def main():
     ap.add_argument("--serve-time-limit", type=float, default=None)
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-chunk", type=int, default=1000)
+    ap.add_argument("--fp64-batch", action="store_true",
+                    help="batch64_sa: also converge the cases with FP64 LU-SGS and compare Cl, Cd")
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="batch64_sa: also time every case alone with FP64 LU-SGS for this many steps")
     ap.add_argument("--profile-steps", type=int, default=100,
@@ -503,20 +570,8 @@ def main():
             out["cpu_baseline"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
 
     if args.copy_probe_mib > 0:
-        # a plain device-to-device copy of known size (the runtime's copy
-        # kernel): the reference reading for scaling FETCH_SIZE/WRITE_SIZE
-        nbytes = args.copy_probe_mib << 20
-        a, b = C.c_void_p(), C.c_void_p()
-        if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
-            raise SystemExit("copy probe: hipMalloc failed")
-        hip.hipMemset(a, 1, C.c_size_t(nbytes))
-        device_sync(hip)
-        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):   # hipMemcpyDeviceToDevice
-            raise SystemExit("copy probe: hipMemcpy failed")
-        device_sync(hip)
-        hip.hipFree(a)
-        hip.hipFree(b)
-        out["copy_probe_bytes"] = nbytes
+        copy_probe(hip, args.copy_probe_mib)
+        out["copy_probe_bytes"] = args.copy_probe_mib << 20
 
     case.close()
     print(json.dumps(out), flush=True)

@@ -87,9 +87,37 @@ struct ResTile {
   int blk;    // tile index within the instance (norm partial slot)
 };
 
+// Stage one grid line of four coefficient arrays into LDS (element t at
+// p[i0 + t*st]) with a 64-lane workgroup.  Each lane issues its loads for
+// eight elements of all four arrays before its first LDS write, so a line
+// of up to 512 elements arrives in one memory round trip rather than one
+// per element and array.
+__device__ __forceinline__ void stage_line4(const double* a, const double* b, const double* c, const double* s,
+                                            long long i0, long long st, int n, double* la, double* lb, double* lc,
+                                            double* ls) {
+  for (int t0 = 0; t0 < n; t0 += 512) {
+    double ra[8], rb[8], rc[8], rs[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + (int)threadIdx.x + 64 * u;
+      const long long i = i0 + (long long)(t < n ? t : n - 1) * st;
+      ra[u] = a[i]; rb[u] = b[i]; rc[u] = c[i]; rs[u] = s[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + (int)threadIdx.x + 64 * u;
+      if (t < n) { la[t] = ra[u]; lb[t] = rb[u]; lc[t] = rc[u]; ls[t] = rs[u]; }
+    }
+  }
+}
+
 constexpr int RES_TJ = 32;
 constexpr int RES_TK = 8;
 constexpr int RES_THREADS = RES_TJ * RES_TK;
+
+// boundary-condition kernels run one workgroup per instance, its segments
+// one after another
+constexpr int BC_THREADS = 256;
 
 __host__ __device__ inline long long cell(const DevMesh& M, int j, int k) {
   return (long long)(j - 1) + (long long)(k - 1) * M.jd;

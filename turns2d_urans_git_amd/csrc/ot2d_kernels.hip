@@ -368,7 +368,7 @@ __device__ void zero_halo_s(const DevMesh& M) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_bc(const DevMesh* __restrict__ Ms, int zero_s) {
+__global__ __launch_bounds__(BC_THREADS) void k_bc(const DevMesh* __restrict__ Ms, int zero_s) {
   const DevMesh& M = Ms[blockIdx.x];
   if (*M.status) return;
   int istep0 = *M.istep0;
@@ -935,7 +935,7 @@ __device__ void rbc_segment(const DevMesh& M, const DevBC& b) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_rbc(const DevMesh* __restrict__ Ms) {
+__global__ __launch_bounds__(BC_THREADS) void k_rbc(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.x];
   if (*M.status) return;
   for (int ib = 0; ib < M.nbc; ++ib) {
@@ -2301,7 +2301,7 @@ hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipSt
 }
 
 hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st) {
-  hipLaunchKernelGGL(k_bc, dim3(nmi), dim3(256), 0, st, dM, zero_s);
+  hipLaunchKernelGGL(k_bc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, zero_s);
   return hipGetLastError();
 }
 
@@ -2353,7 +2353,7 @@ hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream
 }
 
 hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st) {
-  hipLaunchKernelGGL(k_rbc, dim3(nmi), dim3(256), 0, st, dM);
+  hipLaunchKernelGGL(k_rbc, dim3(nmi), dim3(BC_THREADS), 0, st, dM);
   return hipGetLastError();
 }
 

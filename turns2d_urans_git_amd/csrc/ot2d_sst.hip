@@ -54,7 +54,7 @@ __device__ __forceinline__ double findturm(double rho, double q5, double q6, dou
 }  // namespace
 
 // komegabc for one instance, segments in the reference's order
-__global__ __launch_bounds__(256) void k_sst_bc(const DevMesh* __restrict__ Ms, int eddy) {
+__global__ __launch_bounds__(BC_THREADS) void k_sst_bc(const DevMesh* __restrict__ Ms, int eddy) {
   const DevMesh& M = Ms[blockIdx.x];
   if (*M.status) return;
   double* q5 = M.q[4];
@@ -423,9 +423,7 @@ __global__ __launch_bounds__(64) void k_sst_solve_j(const DevMesh* __restrict__ 
   const double* __restrict__ a = M.kw_a[eq];
   const double* __restrict__ b = M.kw_b[eq];
   const double* __restrict__ c = M.kw_c[eq];
-  for (int t = threadIdx.x; t < n; t += 64) {
-    la[t] = a[i0 + t]; lb[t] = b[i0 + t]; lc[t] = c[i0 + t]; ls[t] = s[i0 + t];
-  }
+  stage_line4(a, b, c, s, i0, 1, n, la, lb, lc, ls);
   __syncthreads();
   kw_line_solve(la, lb, lc, ls, lcp, n);
   __syncthreads();
@@ -447,10 +445,7 @@ __global__ __launch_bounds__(64) void k_sst_solve_k(const DevMesh* __restrict__ 
   const double* __restrict__ a = M.kw_ak[eq];
   const double* __restrict__ b = M.kw_b[eq];
   const double* __restrict__ c = M.kw_ck[eq];
-  for (int t = threadIdx.x; t < n; t += 64) {
-    long long i = i0 + t * st;
-    la[t] = a[i]; lb[t] = b[i]; lc[t] = c[i]; ls[t] = s[i];
-  }
+  stage_line4(a, b, c, s, i0, st, n, la, lb, lc, ls);
   __syncthreads();
   kw_line_solve(la, lb, lc, ls, lcp, n);
   __syncthreads();
@@ -496,7 +491,7 @@ hipError_t launch_sst(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, 
   hipError_t e;
   if ((e = launch_sa_prim(dM, hM, nmi, st)) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_PRIM, st);
-  hipLaunchKernelGGL(k_sst_bc, dim3(nmi), dim3(256), 0, st, dM, 0);
+  hipLaunchKernelGGL(k_sst_bc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, 0);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_BC, st);
   hipLaunchKernelGGL(k_sst_blend, dim3(nblk3(mx, 256), nmi), dim3(256), 0, st, dM);
@@ -510,7 +505,7 @@ hipError_t launch_sst(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, 
   hipLaunchKernelGGL(k_sst_solve_k, dim3(2 * mj, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
-  hipLaunchKernelGGL(k_sst_bc, dim3(nmi), dim3(256), 0, st, dM, 1);
+  hipLaunchKernelGGL(k_sst_bc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, 1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_sst_turm, dim3(nblk3(mx, 256), nmi), dim3(256), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -285,7 +285,7 @@ __device__ __forceinline__ void c_turm_cell(const DevMesh& M, long long i) {
 
 // turbc (src/samodel2d.f:827-1303) for one instance; with eddy=1 the eddy
 // viscosity of the halo ring follows (c_turm after the final turbc)
-__global__ __launch_bounds__(256) void k_turbc(const DevMesh* __restrict__ Ms, int eddy) {
+__global__ __launch_bounds__(BC_THREADS) void k_turbc(const DevMesh* __restrict__ Ms, int eddy) {
   const DevMesh& M = Ms[blockIdx.x];
   if (*M.status) return;
   double* q5 = M.q[4];
@@ -559,8 +559,8 @@ __global__ __launch_bounds__(256) void k_sa_rhs(const DevMesh* __restrict__ Ms) 
 
 // The two scalar tridiagonal solves of vmu_sa are sequential recurrences
 // along each grid line.  One wave owns one line: all lanes stage the line's
-// coefficients in LDS with one bulk load (coalesced along j, gathered along
-// k), lane 0 runs the reference's Thomas recurrence out of LDS (so the
+// coefficients in LDS (stage_line4: every load in flight at once, coalesced
+// along j, gathered along k), lane 0 runs the reference's Thomas recurrence out of LDS (so the
 // dependent chain never waits on memory), and all lanes write the results.
 // The arithmetic is the reference's, operation for operation.
 
@@ -580,9 +580,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ M
   const double* __restrict__ c = M.sa_cj;
   const double* __restrict__ a = M.sa_aj;
   const double* __restrict__ b = M.sa_bb;
-  for (int t = threadIdx.x; t < n; t += 64) {
-    la[t] = a[i0 + t]; lb[t] = b[i0 + t]; lc[t] = c[i0 + t]; ls[t] = s[i0 + t];
-  }
+  stage_line4(a, b, c, s, i0, 1, n, la, lb, lc, ls);
   __syncthreads();
   if (threadIdx.x == 0) {
     double bb = 1. / lb[0];
@@ -626,10 +624,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
   const double* __restrict__ c = M.sa_ck;
   const double* __restrict__ a = M.sa_ak;
   const double* __restrict__ b = M.sa_bb;
-  for (int t = threadIdx.x; t < n; t += 64) {
-    long long i = i0 + t * st;
-    la[t] = a[i]; lb[t] = b[i]; lc[t] = c[i]; ls[t] = s[i];
-  }
+  stage_line4(a, b, c, s, i0, st, n, la, lb, lc, ls);
   __syncthreads();
   if (threadIdx.x == 0) {
     double bb = 1. / lb[0];
@@ -930,7 +925,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_PRIM, st);
-  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(256), 0, st, dM, 0);
+  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, 0);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_BC, st);
   hipLaunchKernelGGL(k_sa_rhs, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
@@ -949,7 +944,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
-  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(256), 0, st, dM, 1);
+  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, 1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_EDDY, st);
   return hipSuccess;
