@@ -1103,11 +1103,12 @@ template <> struct LUV<double> { static constexpr int N = 10; };
 
 struct LUGeom {
   int ni, nk, RL, NL, T;           // FP64 sweep: NL lanes, T steps
-  int W, TW;                       // FP32 sweep: W workgroups of 64 rows, TW steps each
+  int W, TW, TA;                   // FP32 sweep: W workgroups of 64 rows, TW steps each,
+                                   // TA >= TW step rows of the record arrays (a multiple of 8)
   long long nrec;                  // FP64: entries of F and of B
   long long off_b, off_hcf, off_hcb, off_hrf, off_hrb;   // FP64 arrays (16-byte vectors)
   long long ncell;                 // FP32: interior cells
-  long long off_f32, off_b32, off_cb, off_h, off_mbf, off_mbb;   // FP32 arrays
+  long long off_f32, off_b32, off_cb, off_h, off_h2;   // FP32 arrays
   long long total;                 // vectors the workspace must hold (either layout)
 };
 
@@ -1118,6 +1119,7 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   g.T = ni + g.NL - 1;
   g.W = (nk + 63) / 64;
   g.TW = ni + 63;
+  g.TA = (g.TW + 7) & ~7;
   g.nrec = (long long)g.T * RL * 64;
   long long hc = (long long)RL * 64;
   g.off_b = LU_NVEC * g.nrec;
@@ -1126,17 +1128,16 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   g.off_hrf = g.off_hcb + LU_NVEC * hc;
   g.off_hrb = g.off_hrf + (long long)LU_NVEC * ni;
   long long t64 = g.off_hrb + (long long)LU_NVEC * ni;
-  // FP32: per-cell records (AoS, interior cells j fastest), cB and the
-  // hand-off between the sweeps in step layout, the inter-workgroup
-  // mailboxes
+  // FP32: the records, cB, the forward results and the backward results,
+  // all in step layout (a 1 KiB row of 64 lanes per vector, block and
+  // step); the results of a block's edge lane are the inter-workgroup mail
   g.ncell = (long long)ni * nk;
   g.off_f32 = 0;
-  g.off_b32 = g.off_f32 + 9 * g.ncell;
-  g.off_cb = g.off_b32 + 8 * g.ncell;
+  g.off_b32 = g.off_f32 + 9ll * g.W * g.TA * 64;
+  g.off_cb = g.off_b32 + 8ll * g.W * g.TA * 64;
   g.off_h = g.off_cb + (long long)g.W * g.TW * 64;
-  g.off_mbf = g.off_h + (long long)g.W * g.TW * 64;
-  g.off_mbb = g.off_mbf + (long long)g.W * ni;
-  long long t32 = g.off_mbb + (long long)g.W * ni;
+  g.off_h2 = g.off_h + (long long)g.W * g.TW * 64;
+  long long t32 = g.off_h2 + (long long)g.W * g.TW * 64;
   g.total = t64 > t32 ? t64 : t32;
   return g;
 }
@@ -1480,12 +1481,16 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // as 4x4 matrices).  Neighbours in the halo ring hold fixed point-Jacobi
 // values (src/lhs.f:568-598), so their terms are folded into u (forward)
 // and into cB (backward, added by the forward sweep to its output).
-//   F record (9 float4 per interior cell): u, PA rows 0-3, PB rows 0-3
-//   B record (8 float4 per interior cell): PA' rows 0-3, PB' rows 0-3
-//   cB (one float4 per cell in sweep-step order, as H)
+//   F record (9 float4 per cell): u, PA rows 0-3, PB rows 0-3
+//   B record (8 float4 per cell): PA' rows 0-3, PB' rows 0-3
+//   cB (one float4 per cell)
+// all stored by forward step: F vector q of lane l at step t of block w is
+// float4 ((w*TA + t)*9 + q)*64 + l, so a producer reads a step's vector as
+// one coalesced 1 KiB row; B is kept at the cell's forward step too (the
+// backward sweep reaches it at step NT-1-t).
 //   H (one float4 per cell in sweep-step order): s_fwd written by the
-//     forward sweep (the backward producers add cB to it), overwritten with
-//     the result by the backward sweep
+//     forward sweep (the backward producers add cB to it)
+//   H2 (same order): the result, written by the backward sweep
 // The halo-ring s values themselves are scaled in k_lu_finish (the prep
 // reads the unscaled neighbours).
 // Vectors per sweep step in an LDS ring slot: forward u, PA rows, PB rows;
@@ -1495,6 +1500,9 @@ constexpr int LU32_NV = 9;
 // bit pattern of an empty mailbox word: a signalling NaN, which no
 // arithmetic result takes (they are quiet NaNs at worst)
 constexpr unsigned LU32_EMPTY = 0x7fa5a5a5u;
+// cache-policy operand of a buffer store that writes through to the device
+// coherence point (the sc1 bit), as an agent-scope atomic store does
+constexpr int LU32_SC1 = 16;
 
 // rows of the 4x4 matrix M with a = M . s for a neighbour's A(+/-) or
 // B(+/-) term (src/lhs.f:633-651)
@@ -1541,15 +1549,42 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
   F4* recf = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_f32);
   F4* recb = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_b32);
   F4* cbs = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_cb);
-  // both mailboxes (2 * W * ni float4) start as the empty sentinel
+  // the mail entries start as the empty sentinel: forward, lane 63 of H
+  // rows 0 .. ni-1; backward, lane 0 of H2 rows 63 .. ni+62 (every block)
   const long long nmb = 2ll * g.W * g.ni;
-  u32x4_t* mb = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_mbf);
-  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < nmb; w += 256ll * gridDim.x)
-    mb[w] = u32x4_t{LU32_EMPTY, LU32_EMPTY, LU32_EMPTY, LU32_EMPTY};
-  // one thread per interior cell, j fastest: coalesced reads and, the
-  // records being per cell, coalesced writes
-  for (long long ic = blockIdx.x * 256ll + threadIdx.x; ic < g.ncell; ic += 256ll * gridDim.x) {
-    const int j = js + (int)(ic % g.ni), k = ks + (int)(ic / g.ni);
+  u32x4_t* h = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_h);
+  u32x4_t* h2 = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_h2);
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < nmb; w += 256ll * gridDim.x) {
+    const long long e = w % ((long long)g.W * g.ni), blk = e / g.ni, r = e - blk * g.ni;
+    const u32x4_t empty = u32x4_t{LU32_EMPTY, LU32_EMPTY, LU32_EMPTY, LU32_EMPTY};
+    if (w < (long long)g.W * g.ni) h[(blk * g.TW + r) * 64 + 63] = empty;
+    else h2[(blk * g.TW + r + 63) * 64] = empty;
+  }
+  // one thread per (block, step, lane) entry.  A wave covers 8 steps x 8
+  // rows: 8 consecutive lanes are 8 consecutive rows at one step, so each
+  // vector of a step row is stored as whole 128-byte lines, and the 8
+  // steps of one row are 8 consecutive cells along j, so the reads come in
+  // row segments.  Entries outside the grid store zero records.
+  const long long nent = (long long)g.W * g.TA * 64;
+  const int tiles_t = g.TA >> 3;
+  auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
+  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < nent; e += 256ll * gridDim.x) {
+    const int lane = (int)(e & 63);
+    const long long tile = e >> 6;
+    const int lb = (int)(tile & 7);
+    const long long rest = tile >> 3;
+    const int tb = (int)(rest % tiles_t), w = (int)(rest / tiles_t);
+    const int l = lb * 8 + (lane & 7), t = tb * 8 + (lane >> 3);
+    const int kk = 64 * w + l, jr = t - l;
+    F4* rf = recf + ((long long)(w * g.TA + t) * 9) * 64 + l;
+    F4* rb = recb + ((long long)(w * g.TA + t) * 8) * 64 + l;
+    if (kk >= g.nk || jr < 0 || jr >= g.ni) {
+      const F4 z = F4{0.f, 0.f, 0.f, 0.f};
+      for (int q = 0; q < 9; ++q) rf[q * 64] = z;
+      for (int q = 0; q < 8; ++q) rb[q * 64] = z;
+      continue;
+    }
+    const int j = js + jr, k = ks + kk;
     const long long i = cell(M, j, k);
     CellLoad c0 = load_cell(M, i);
     LUCellF64 L0 = lu_of(M.c, c0, epsv);
@@ -1560,15 +1595,10 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
     lu32_term(M, i - jd, k == ks, 0, +1, f, eps2, epsv, PB, u);
     lu32_term(M, i + 1, j == je, 1, -1, -f, eps2, epsv, QA, cb);
     lu32_term(M, i + jd, k == ke, 0, -1, -f, eps2, epsv, QB, cb);
-    auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
-    F4* rf = recf + 9 * ic;
-    F4* rb = recb + 8 * ic;
     rf[0] = f4(u);
-    for (int r = 0; r < 4; ++r) { rf[1 + r] = f4(PA + 4 * r); rf[5 + r] = f4(PB + 4 * r); }
-    for (int r = 0; r < 4; ++r) { rb[r] = f4(QA + 4 * r); rb[4 + r] = f4(QB + 4 * r); }
-    // cB at the cell's forward step: read by the backward producers one
-    // step-row (1 KiB) at a time
-    cbs[lu_entry32(M, g, j, k, +1)] = f4(cb);
+    for (int r = 0; r < 4; ++r) { rf[(1 + r) * 64] = f4(PA + 4 * r); rf[(5 + r) * 64] = f4(PB + 4 * r); }
+    for (int r = 0; r < 4; ++r) { rb[r * 64] = f4(QA + 4 * r); rb[(4 + r) * 64] = f4(QB + 4 * r); }
+    cbs[((long long)w * g.TW + t) * 64 + l] = f4(cb);
   }
 }
 
@@ -1637,33 +1667,40 @@ __device__ __forceinline__ float dpp_shift_in(float v, float edge) {
 // walking along j one column per step, lane l one step behind lane l-1.
 // Inside the workgroup, wave 0 computes and waves 1..NP stream the records
 // of step t into LDS slot t % S; each producer wave keeps the loads of PD
-// of its steps in flight.  Hand-off protocol (LDS, workgroup scope):
+// of its steps in flight.  Vector NV of a slot carries, in the lane that
+// takes the wavefront shift's fill value (forward 0, backward 63), the
+// neighbour block's edge value, and in one other lane the slot's tag.
+// Hand-off protocol (LDS, workgroup scope):
 //   tag of slot t % S = t + 1   stored (release) by the producer after the
 //                               records of step t
 //   consumed = c                stored by the computing wave once the reads
 //                               of steps < c have executed, so a producer
 //                               may fill slot t % S when consumed >= t-S+1
-// The computing wave issues the tag read of step t+1 and then its record
-// reads while it computes step t, and checks the tag when it starts step
-// t+1.  This relies on the LDS executing one wave's operations in issue
-// order: the reads that follow a fresh tag see the records stored before
-// it, and the reads of a slot execute before the consumed store that
-// frees it.  Compiler barriers (no instructions) keep that program order
-// without the waits a fence would add.
-// Across workgroups the edge row passes through a mailbox per column in
-// global memory: block w's edge lane stores its result with device-scope
-// atomics, the producer waves of the neighbouring block wait for it and
-// place it in the slot, and the computing wave feeds it to its edge lane as
-// the fill value of the wavefront shift.  Forward, block w waits on block
-// w-1; backward, on block w+1; the dispatch order (blockIdx.x) puts the
-// block waited on first, so every wait is on a workgroup already running.
+// The computing wave reads step t+1's slot while it computes step t and
+// checks the tag when it starts step t+1.  This relies on the LDS
+// executing one wave's operations in issue order: reads issued after the
+// tag's read see the records stored before the tag (the tag is read with
+// them, so a stale tag means the slot is read again), and the reads of a
+// slot execute before the consumed store that frees it.  Compiler barriers
+// (no instructions) keep that program order without the waits a fence
+// would add.
+// Across workgroups: every lane's result of every step goes to the step
+// ordered array (forward H, backward H2) with device-scope stores; the
+// producers of the neighbouring block read the edge lane's entries there,
+// waiting while they still hold the empty sentinel the prep stored.
+// Forward, block w waits on block w-1; backward, on block w+1; the dispatch
+// order (blockIdx.x) puts the block waited on first, so every wait is on a
+// workgroup already running.
 template <int DIR, int S, int NP, int PD>
 __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __restrict__ Ms) {
   constexpr int NV = LU32_NV;
+  constexpr int MAIL_LANE = DIR > 0 ? 0 : 63, TAG_LANE = DIR > 0 ? 1 : 0;
   extern __shared__ __attribute__((aligned(16))) double lds_raw[];
   __shared__ int consumed, abort_flag, st0;
-  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][NV+1][64]; vector NV: lane 0 mailbox value, lane 1 tag
-  auto tag_of = [&](int slot) { return reinterpret_cast<int*>(ring + ((size_t)slot * (NV + 1) + NV) * 64 + 1); };
+  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][NV+1][64]
+  auto tag_of = [&](int slot) {
+    return reinterpret_cast<int*>(ring + ((size_t)slot * (NV + 1) + NV) * 64 + TAG_LANE);
+  };
   const DevMesh& M = Ms[blockIdx.y];
   const LUGeom g = lu_geom(M);
   if (threadIdx.x == 0) {
@@ -1679,47 +1716,51 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   const int NT = g.TW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   D2* lu = reinterpret_cast<D2*>(M.lu);
+  // step-ordered result arrays: forward H (row NT-1-t for step t), backward
+  // H2 (row t); the backward producers read H
   const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
+  const __amdgpu_buffer_rsrc_t hout = DIR > 0 ? hnd : uniform_rsrc(lu + g.off_h2, (long long)g.W * g.TW * 64 * 16);
   const int base = w * NT;
   if (wave > 0) {
     // ---------------------------------------------------------- producers
-    // per-cell records: lane l's cell at step tt is (js+tt-l, ks+64w+l)
-    // forward, (je-tt+63-l, ks+64w+l) backward.  For a cell outside the
-    // grid the offset points past the end of the records, where buffer
-    // loads return zeros: such lanes compute exact zeros from finite
+    // records in step layout: one coalesced 1 KiB row per vector and step
+    // (lane l's cell at step tt is (js+tt-l, ks+64w+l) forward, the cell of
+    // forward step NT-1-tt backward).  Entries of cells outside the grid
+    // hold zero records: such lanes compute exact zeros from finite
     // inputs.  Steps past the end are clamped to the last one; the loop
     // body has no conditional loads.
-    const __amdgpu_buffer_rsrc_t recf = uniform_rsrc(lu + g.off_f32, g.ncell * 9 * 16);
-    const __amdgpu_buffer_rsrc_t recb = uniform_rsrc(lu + g.off_b32, g.ncell * 8 * 16);
+    const __amdgpu_buffer_rsrc_t recf = uniform_rsrc(lu + g.off_f32, 9ll * g.W * g.TA * 64 * 16);
+    const __amdgpu_buffer_rsrc_t recb = uniform_rsrc(lu + g.off_b32, 8ll * g.W * g.TA * 64 * 16);
     const __amdgpu_buffer_rsrc_t cbr = uniform_rsrc(lu + g.off_cb, (long long)g.W * g.TW * 64 * 16);
     const int krow = 64 * w + l;
     const bool rowin = krow < g.nk;
-    // mailbox written by the neighbour block; at the grid edge (no
-    // neighbour) this block's own, read and ignored
+    // the neighbour block's edge value for this block's fill lane at step
+    // tt: forward, lane 63 of block w-1 at column j = js+tt (its step tt+63,
+    // H row ni-1-tt); backward, lane 0 of block w+1 at column j = je-tt
+    // (its step tt+63, H2 row tt+63).  At the grid edge (no neighbour) the
+    // pointer is a valid entry that is read and ignored.
     const bool has_in = DIR > 0 ? w > 0 : w < g.W - 1;
-    gu32* mb_in = (gu32*)(lu + (DIR > 0 ? g.off_mbf : g.off_mbb) + (long long)(has_in ? w - DIR : w) * g.ni);
-    // the neighbour's mailbox entry for the edge lane's column at step tt
-    // (forward lane 0 at j = js+tt, backward lane 63 at j = je-tt)
+    const int wn = has_in ? w - DIR : w;
+    gu32* mail_base = (gu32*)(lu + (DIR > 0 ? g.off_h : g.off_h2));
     auto mail_ptr = [&](int tt) {
-      int c = DIR > 0 ? tt : g.ni - 1 - tt;
-      c = c < 0 ? 0 : (c >= g.ni ? g.ni - 1 : c);
-      return mb_in + 4ll * c;
+      const int c = tt < 0 ? 0 : (tt >= g.ni ? g.ni - 1 : tt);
+      const long long row = (long long)wn * NT + (DIR > 0 ? g.ni - 1 - c : c + 63);
+      return mail_base + (row * 64 + (DIR > 0 ? 63 : 0)) * 4;
     };
-    constexpr int NB = NV + 1 + (DIR < 0 ? 1 : 0);   // backward: + cB of the forward record
+    constexpr int NB = NV + 1 + (DIR < 0 ? 1 : 0);   // backward: + cB
     auto fetch = [&](int tt, u32x4_t (&v)[NB]) {
       const int tc = tt < NT ? tt : NT - 1;
       const int jr = DIR > 0 ? tc - l : (g.ni - 1) - tc + (63 - l);
       const bool cin = rowin && (unsigned)jr < (unsigned)g.ni;
-      const long long ci = cin ? (long long)krow * g.ni + jr : g.ncell;
       if constexpr (DIR > 0) {
-        const int vo = (int)(ci * 144);
+        const int so = (w * g.TA + tc) * 9 * 1024;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(recf, vo + 16 * q, 0, 0);
+        for (int q = 0; q < 9; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(recf, l * 16, so + q * 1024, 0);
       } else {
-        const int vo = (int)(ci * 128);
+        const int so = (w * g.TA + NT - 1 - tc) * 8 * 1024;
         v[0] = __builtin_amdgcn_raw_buffer_load_b128(hnd, l * 16, (base + tc) * 1024, 0);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[1 + q] = __builtin_amdgcn_raw_buffer_load_b128(recb, vo + 16 * q, 0, 0);
+        for (int q = 0; q < 8; ++q) v[1 + q] = __builtin_amdgcn_raw_buffer_load_b128(recb, l * 16, so + q * 1024, 0);
         // cB of the cell, kept at its forward step NT-1-tc; zero outside
         // the grid (those entries are never written)
         const u32x4_t cb = __builtin_amdgcn_raw_buffer_load_b128(cbr, l * 16, (base + NT - 1 - tc) * 1024, 0);
@@ -1748,7 +1789,7 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
       u32x4_t* d = ring + (size_t)(tt % S) * (NV + 1) * 64;
 #pragma unroll
       for (int q = 0; q < NV; ++q) d[q * 64 + l] = v[q];
-      if (l == 0) d[NV * 64] = v[NV];
+      if (l == 0) d[NV * 64 + MAIL_LANE] = v[NV];
       if (l == 0) __hip_atomic_store(tag_of(tt % S), tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       return true;
     };
@@ -1776,45 +1817,34 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   // matrices (the halo terms are folded into u and cB): no lane needs a
   // select.
   __builtin_amdgcn_s_setprio(3);
-  const bool has_out = DIR > 0 ? w < g.W - 1 : w > 0;
-  const int edge = DIR > 0 ? 63 : 0;
-  // the edge lane's cell is inside the grid at steps 63 .. ni+62 when the
-  // block's last row exists (every block with a neighbour after it is full)
-  gu32* mb_out = (gu32*)(lu + (DIR > 0 ? g.off_mbf : g.off_mbb) + (long long)w * g.ni);
   float sA[4] = {0.f, 0.f, 0.f, 0.f};
   // k neighbour leaving this lane; the wavefront shift runs with every lane
   // enabled (a disabled source lane reads as the fill value)
   float kout[4] = {0.f, 0.f, 0.f, 0.f};
-  // tag, then records, of the step in `slot`, in this order
-  auto rd = [&](int slot, int& tag, u32x4_t (&v)[NV + 1]) {
-    tag = __hip_atomic_load(tag_of(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    asm volatile("" ::: "memory");
+  // records of the step in `slot`; v[NV] holds the fill value in lane
+  // MAIL_LANE and the tag in lane TAG_LANE
+  auto rd = [&](int slot, u32x4_t (&v)[NV + 1]) {
     const u32x4_t* s = ring + (size_t)slot * (NV + 1) * 64;
+    v[NV] = s[NV * 64 + l];
+    asm volatile("" ::: "memory");   // the tag's read before the records'
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = s[q * 64 + l];
-    v[NV] = s[NV * 64];
   };
   bool ok = true;
-  int tagN = 0;   // tag read with the records of the next step
   // step t from slot t % S (= slot, a constant once the loop below is
   // unrolled); reads step t+1 into N
   auto step = [&](int slot, int t, u32x4_t (&C)[NV + 1], u32x4_t (&N)[NV + 1]) {
     const int s1 = slot + 1 < S ? slot + 1 : 0;
-    // compared where it is used (as a vector value: the compiler would
-    // otherwise move it to a scalar register, waiting for it, as soon as
-    // it is loaded)
-    int tg = tagN;
-    asm volatile("" : "+v"(tg));
-    if (tg < t + 1) {
+    if (__builtin_amdgcn_readlane((int)C[NV].x, TAG_LANE) < t + 1) {
       // the records of step t were read before the producer stored them:
       // wait for the tag and read them again
       ok = ok && spin_until_ge<1>(tag_of(slot), t + 1, &abort_flag);
-      rd(slot, tg, C);
+      rd(slot, C);
     }
     // the reads of steps <= t execute before this store
     asm volatile("" ::: "memory");
     __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    rd(s1, tagN, N);   // past the last step: a harmless read of a stale slot
+    rd(s1, N);   // past the last step: a harmless read of a stale slot
     const float4 mail = f4_of(C[NV]);
     float kin[4];
     kin[0] = dpp_shift_in<DIR>(kout[0], mail.x);
@@ -1836,18 +1866,12 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
       p = __builtin_fmaf(b.z, kin[2], p);
       sn[q] = __builtin_fmaf(b.w, kin[3], p);
     }
-    // H entry of (step, lane): every lane stores (the entries of cells
-    // outside the grid are never read as results)
-    const int ostep = DIR > 0 ? NT - 1 - t : t;
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}), hnd, l * 16,
-                                           (base + ostep) * 1024, 0);
-    if (has_out && t >= 63 && t < g.ni + 63 && l == edge) {
-      const int jm = DIR > 0 ? t - 63 : g.ni - 1 - (t - 63);   // column index j - js of the edge lane
-      gu32* p = mb_out + 4ll * jm;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        __hip_atomic_store(p + q, __builtin_bit_cast(unsigned, sn[q]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    // the result of (step, lane), device-scope (sc1): the edge lane's entry
+    // is the neighbouring block's mail; entries of cells outside the grid
+    // are never read as results
+    const int orow = DIR > 0 ? NT - 1 - t : t;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}), hout, l * 16,
+                                           (base + orow) * 1024, LU32_SC1);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       sA[q] = sn[q];
@@ -1856,11 +1880,20 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   };
   u32x4_t C0[NV + 1], C1[NV + 1];
   ok = spin_until_ge<1>(tag_of(0), 1, &abort_flag);
-  rd(0, tagN, C0);
+  rd(0, C0);
   // S steps per trip (S even), the two register sets alternating, so that
-  // every slot index and LDS offset is a constant
+  // every slot index and LDS offset is a constant; the last, partial trip
+  // checks each step against NT
   static_assert(S % 2 == 0, "the ring holds an even number of steps");
-  for (int t0 = 0; t0 < NT && ok; t0 += S) {
+  int t0 = 0;
+  for (; t0 + S <= NT && ok; t0 += S) {
+#pragma unroll
+    for (int i = 0; i < S; i += 2) {
+      step(i, t0 + i, C0, C1);
+      step(i + 1, t0 + i + 1, C1, C0);
+    }
+  }
+  if (ok) {
 #pragma unroll
     for (int i = 0; i < S; i += 2) {
       if (t0 + i < NT) step(i, t0 + i, C0, C1);
@@ -1894,7 +1927,7 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
   if (*M.status) return;
   const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
   const LUGeom g = lu_geom(M);
-  const LUArr B = sizeof(T) == 8 ? lu_arr(M, g.off_b, g.nrec) : lu_arr(M, g.off_h, (long long)g.W * g.TW * 64);
+  const LUArr B = sizeof(T) == 8 ? lu_arr(M, g.off_b, g.nrec) : lu_arr(M, g.off_h2, (long long)g.W * g.TW * 64);
   long long n2 = (long long)jd * kd;
   int bad = 0;
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
@@ -2404,8 +2437,10 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
   for (int i = 0; i < nmi; ++i) {
     mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
     int nk = hM[i].ke - hM[i].ks + 1, ni = hM[i].je - hM[i].js + 1;
-    nc = (long long)ni * nk > nc ? (long long)ni * nk : nc;
-    wmax = (nk + 63) / 64 > wmax ? (nk + 63) / 64 : wmax;
+    const LUGeom gg = lu_geom(ni, nk, hM[i].lu_rl);
+    const long long nent = (long long)gg.W * gg.TA * 64;   // prep entries
+    nc = nent > nc ? nent : nc;
+    wmax = gg.W > wmax ? gg.W : wmax;
   }
   const size_t lds = (size_t)LU32_S * (LU32_NV + 1) * 1024;
   static bool attr_set = false;
