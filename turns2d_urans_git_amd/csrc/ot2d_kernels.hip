@@ -1564,16 +1564,18 @@ __global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ M
   // rows: 8 consecutive lanes are 8 consecutive rows at one step, so each
   // vector of a step row is stored as whole 128-byte lines, and the 8
   // steps of one row are 8 consecutive cells along j, so the reads come in
-  // row segments.  Entries outside the grid store zero records.
+  // row segments; consecutive waves take the next 8 steps of the same 8
+  // rows, so a workgroup reads 39-cell row segments.  Entries outside the
+  // grid store zero records.
   const long long nent = (long long)g.W * g.TA * 64;
   const int tiles_t = g.TA >> 3;
   auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
   for (long long e = blockIdx.x * 256ll + threadIdx.x; e < nent; e += 256ll * gridDim.x) {
     const int lane = (int)(e & 63);
     const long long tile = e >> 6;
-    const int lb = (int)(tile & 7);
-    const long long rest = tile >> 3;
-    const int tb = (int)(rest % tiles_t), w = (int)(rest / tiles_t);
+    const int tb = (int)(tile % tiles_t);
+    const long long rest = tile / tiles_t;
+    const int lb = (int)(rest & 7), w = (int)(rest >> 3);
     const int l = lb * 8 + (lane & 7), t = tb * 8 + (lane >> 3);
     const int kk = 64 * w + l, jr = t - l;
     F4* rf = recf + ((long long)(w * g.TA + t) * 9) * 64 + l;
