@@ -87,6 +87,13 @@ struct ResTile {
   int blk;    // tile index within the instance (norm partial slot)
 };
 
+// Pointers into device memory as the global address space: loads through
+// them are global (not flat) instructions, whose completions the compiler
+// counts exactly, so loads issued ahead of their use stay in flight.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
+__device__ __forceinline__ const gdouble* gptr(const double* p) { return (const gdouble*)p; }
+
 // Stage one grid line of four coefficient arrays into LDS (element t at
 // p[i0 + t*st]) with a 64-lane workgroup.  Each lane issues its loads for
 // eight elements of all four arrays before its first LDS write, so a line

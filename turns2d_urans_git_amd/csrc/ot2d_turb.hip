@@ -274,13 +274,17 @@ __device__ __forceinline__ int sa_periodic_src(int idir, int js, int je, int ks,
   return 1 + 2 * kk - (ke - jk + 1);
 }
 
-// c_turm (src/samodel2d.f:372-399) for one cell
-__device__ __forceinline__ void c_turm_cell(const DevMesh& M, long long i) {
-  double rj = M.q[0][i] * M.q[M.nq - 1][i];
-  double vnul = M.sa_vmul[i] / rj;
-  double chi = M.q[4][i] / vnul;
+// c_turm (src/samodel2d.f:372-399) for one cell, from its q(1), q(nq),
+// laminar viscosity and q(5)
+__device__ __forceinline__ double c_turm_val(double q1, double qn, double vmul, double q5) {
+  double rj = q1 * qn;
+  double vnul = vmul / rj;
+  double chi = q5 / vnul;
   double fv1 = ip3(chi) / (ip3(chi) + ip3(SA_cv1));
-  M.turmu[i] = M.q[0][i] * M.q[M.nq - 1][i] * M.q[4][i] * fv1;
+  return q1 * qn * q5 * fv1;
+}
+__device__ __forceinline__ void c_turm_cell(const DevMesh& M, long long i) {
+  M.turmu[i] = c_turm_val(M.q[0][i], M.q[M.nq - 1][i], M.sa_vmul[i], M.q[4][i]);
 }
 
 // turbc (src/samodel2d.f:827-1303) for one instance; with eddy=1 the eddy
@@ -659,7 +663,6 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
 
 // Line solves for many lines at once: one lane per line, 64 lines per
 // block, the same reference arithmetic.
-//
 // k_sa_lines_j: lines along j (one per k row).  The block's 64 rows are
 // streamed through LDS in tiles of SA_TJ columns: the tile is loaded with
 // row-contiguous (coalesced) reads while the previous tile is solved, each
@@ -682,10 +685,10 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
   const int n = je - js + 1;
   const int ntile = (n + SA_TJ - 1) / SA_TJ;
   const int l = threadIdx.x;
-  double* __restrict__ sS = M.sa_sj;
-  double* __restrict__ sC = M.sa_cj;
-  const double* __restrict__ sA = M.sa_aj;
-  const double* __restrict__ sB = M.sa_bb;
+  gdouble* __restrict__ sS = gptr(M.sa_sj);
+  gdouble* __restrict__ sC = gptr(M.sa_cj);
+  const gdouble* __restrict__ sA = gptr(M.sa_aj);
+  const gdouble* __restrict__ sB = gptr(M.sa_bb);
   // element w of a tile: row w / SA_TJ, column w % SA_TJ (16 lanes read one
   // row's 16 consecutive cells)
   constexpr int PER = 64 * SA_TJ / 64;
@@ -701,7 +704,10 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
     for (int p = 0; p < PER; ++p) {
       int r, cc; bool ok;
       long long i = gidx(t, l + 64 * p, r, cc, ok);
-      ra[p] = ok ? sA[i] : 0.0; rb[p] = ok ? sB[i] : 1.0; rc[p] = ok ? sC[i] : 0.0; rs[p] = ok ? sS[i] : 0.0;
+      // the load itself is unconditional (the index is clamped into the
+      // grid), so no loaded value merges across a branch
+      const double va = sA[i], vb = sB[i], vc = sC[i], vs = sS[i];
+      ra[p] = ok ? va : 0.0; rb[p] = ok ? vb : 1.0; rc[p] = ok ? vc : 0.0; rs[p] = ok ? vs : 0.0;
     }
   };
   // ---- forward elimination
@@ -717,18 +723,22 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
     if (t + 1 < ntile) load(t + 1);
     __syncthreads();
     if (l < nrow) {
+      // unrolled over the tile so the LDS reads run ahead of the recurrence
       const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
-      for (int cc = 0; cc < cn; ++cc) {
-        if (c0 + cc == 0) {
-          double bb = 1. / tb[l][0];
-          sp = ts[l][0] * bb; cp = tc[l][0] * bb;
-        } else {
-          double ai = ta[l][cc];
-          double bb = 1. / (tb[l][cc] - ai * cp);
-          sp = (ts[l][cc] - ai * sp) * bb;
-          cp = tc[l][cc] * bb;
+#pragma unroll
+      for (int cc = 0; cc < SA_TJ; ++cc) {
+        if (cc < cn) {
+          if (c0 + cc == 0) {
+            double bb = 1. / tb[l][0];
+            sp = ts[l][0] * bb; cp = tc[l][0] * bb;
+          } else {
+            double ai = ta[l][cc];
+            double bb = 1. / (tb[l][cc] - ai * cp);
+            sp = (ts[l][cc] - ai * sp) * bb;
+            cp = tc[l][cc] * bb;
+          }
+          ts[l][cc] = sp; tc[l][cc] = cp;
         }
-        ts[l][cc] = sp; tc[l][cc] = cp;
       }
     }
     __syncthreads();
@@ -746,7 +756,8 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
     for (int p = 0; p < PER; ++p) {
       int r, cc; bool ok;
       long long i = gidx(t, l + 64 * p, r, cc, ok);
-      rb[p] = ok ? sB[i] : 1.0; rc[p] = ok ? sC[i] : 0.0; rs[p] = ok ? sS[i] : 0.0;
+      const double vb = sB[i], vc = sC[i], vs = sS[i];
+      rb[p] = ok ? vb : 1.0; rc[p] = ok ? vc : 0.0; rs[p] = ok ? vs : 0.0;
     }
   };
   load2(ntile - 1);
@@ -761,10 +772,13 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
     __syncthreads();
     if (l < nrow) {
       const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
-      for (int cc = cn - 1; cc >= 0; --cc) {
-        if (c0 + cc == n - 1) sn = ts[l][cc];
-        else sn = ts[l][cc] - tc[l][cc] * sn;
-        ts[l][cc] = (0. + sn - 0.) * tb[l][cc];
+#pragma unroll
+      for (int cc = SA_TJ - 1; cc >= 0; --cc) {
+        if (cc < cn) {
+          if (c0 + cc == n - 1) sn = ts[l][cc];
+          else sn = ts[l][cc] - tc[l][cc] * sn;
+          ts[l][cc] = (0. + sn - 0.) * tb[l][cc];
+        }
       }
     }
     __syncthreads();
@@ -777,88 +791,141 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
   }
 }
 
-// k_sa_lines_k: lines along k (one per j column), lanes on consecutive
-// columns, so every step of the recurrence reads one contiguous row
-// segment; the coefficients of the next SA_PK rows are loaded ahead.  The
-// backward pass finishes each cell: nu-tilde update with its clip
-// (src/samodel2d.f:157-188) and the interior eddy viscosity (c_turm).
-constexpr int SA_PK = 8;
+// k_sa_lines<JL>: one lane per line, each lane walking its line with the
+// coefficients of the next SA_PK cells loaded ahead into registers.
+//   JL = 0: lines along k (one per j column), lanes on consecutive columns,
+//           so every step reads one contiguous row segment.  The backward
+//           pass is followed by the nu-tilde update with its clip
+//           (src/samodel2d.f:157-188) and the interior eddy viscosity.
+//   JL = 1: lines along j (one per k row); the lanes' rows are jd apart,
+//           but each lane's next cells lie in lines it has already read.
+//           The forward pass leaves c' and s' in place (as lsolvej does),
+//           the backward pass writes s = (s1 - s2) * bjmat.
+constexpr int SA_PK = 16;
 
-__global__ __launch_bounds__(64) void k_sa_lines_k(const DevMesh* __restrict__ Ms) {
+template <int JL>
+__global__ __launch_bounds__(64) void k_sa_lines(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  const int j = js + 64 * blockIdx.x + threadIdx.x;
-  if (js + 64 * (int)blockIdx.x > je) return;
-  const bool lane_ok = j <= je;
-  const int jj = lane_ok ? j : je;
-  const int n = ke - ks + 1;
-  const long long st = M.jd;
-  const long long i0 = cell(M, jj, ks);
-  double* __restrict__ sS = M.sa_sj;
-  double* __restrict__ sC = M.sa_ck;
-  const double* __restrict__ sA = M.sa_ak;
-  const double* __restrict__ sB = M.sa_bb;
+  const int l0 = JL ? ks : js, l1 = JL ? ke : je;   // line index range
+  if (l0 + 64 * (int)blockIdx.x > l1) return;
+  const int line = l0 + 64 * blockIdx.x + threadIdx.x;
+  const bool lane_ok = line <= l1;
+  const int ll = lane_ok ? line : l1;
+  const int n = JL ? je - js + 1 : ke - ks + 1;
+  const long long st = JL ? 1 : M.jd;
+  const long long i0 = JL ? cell(M, js, ll) : cell(M, ll, ks);
+  // global (not generic) pointers: the compiler then counts each load's
+  // completion exactly and the loads ahead stay in flight
+  gdouble* __restrict__ sS = gptr(M.sa_sj);
+  gdouble* __restrict__ sC = gptr(JL ? M.sa_cj : M.sa_ck);
+  const gdouble* __restrict__ sA = gptr(JL ? M.sa_aj : M.sa_ak);
+  const gdouble* __restrict__ sB = gptr(M.sa_bb);
+  // Loads ahead are unconditional (indices clamped to the line) and the
+  // unrolled loops run whole chunks, the remainder separately, so no loaded
+  // value merges across a branch: the compiler then waits for each value
+  // exactly where it is used.  Slot d of a prefetch array holds the cell
+  // whose index is congruent to d modulo SA_PK.
+  auto at = [&](int t) { return i0 + (long long)(t < n ? t : n - 1) * st; };
   double pa[SA_PK], pb[SA_PK], pc[SA_PK], ps[SA_PK];
 #pragma unroll
-  for (int d = 0; d < SA_PK; ++d)
-    if (d < n) { long long i = i0 + d * st; pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i]; }
-  double cp = 0.0, sp = 0.0;
-  for (int t0 = 0; t0 < n; t0 += SA_PK) {
-#pragma unroll
-    for (int d = 0; d < SA_PK; ++d) {
-      const int t = t0 + d;
-      if (t < n) {
-        double a = pa[d], b = pb[d], c = pc[d], s = ps[d];
-        if (t + SA_PK < n) {
-          long long i = i0 + (t + SA_PK) * st;
-          pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i];
-        }
-        if (t == 0) {
-          double bb = 1. / b;
-          sp = s * bb; cp = c * bb;
-        } else {
-          double bb = 1. / (b - a * cp);
-          sp = (s - a * sp) * bb;
-          cp = c * bb;
-        }
-        if (lane_ok) { long long i = i0 + t * st; sS[i] = sp; sC[i] = cp; }
-      }
-    }
+  for (int d = 0; d < SA_PK; ++d) {
+    const long long i = at(d);
+    pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i];
   }
-  // back substitution, rows in reverse (the values this lane just wrote),
-  // next rows loaded ahead; dq = s2 goes back into sjmat
-  double qs[SA_PK], qc[SA_PK];
+  double cp, sp;
+  {
+    // first cell (its a is not used)
+    const double b = pb[0], c = pc[0], s = ps[0];
+    const long long i = at(SA_PK);
+    pa[0] = sA[i]; pb[0] = sB[i]; pc[0] = sC[i]; ps[0] = sS[i];
+    double bb = 1. / b;
+    sp = s * bb; cp = c * bb;
+    if (lane_ok) { sS[i0] = sp; sC[i0] = cp; }
+  }
+  auto fwd = [&](int t, int d) {
+    const double a = pa[d], b = pb[d], c = pc[d], s = ps[d];
+    const long long i = at(t + SA_PK);
+    pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i];
+    double bb = 1. / (b - a * cp);
+    sp = (s - a * sp) * bb;
+    cp = c * bb;
+    if (lane_ok) { const long long o = i0 + (long long)t * st; sS[o] = sp; sC[o] = cp; }
+  };
+  int tb = 1;
+  for (; tb + SA_PK <= n; tb += SA_PK) {
 #pragma unroll
-  for (int d = 0; d < SA_PK; ++d)
-    if (d < n) { long long i = i0 + (n - 1 - d) * st; qs[d] = sS[i]; qc[d] = sC[i]; }
+    for (int e = 0; e < SA_PK; ++e) fwd(tb + e, (1 + e) % SA_PK);
+  }
+#pragma unroll
+  for (int e = 0; e < SA_PK; ++e)
+    if (tb + e < n) fwd(tb + e, (1 + e) % SA_PK);
+  // back substitution, cells in reverse (the values this lane just wrote),
+  // the next cells loaded ahead; slot d holds u = n-1-t congruent to d
+  auto atb = [&](int u) { return i0 + (long long)(u < n ? n - 1 - u : 0) * st; };
+  double qs[SA_PK], qc[SA_PK], qb[SA_PK];
+#pragma unroll
+  for (int d = 0; d < SA_PK; ++d) {
+    const long long i = atb(d);
+    qs[d] = sS[i]; qc[d] = sC[i];
+    if (JL) qb[d] = sB[i];
+  }
   double sn = sp;
-  for (int u0 = 0; u0 < n; u0 += SA_PK) {
+  {
+    // last cell: sn is its s'
+    const double b = JL ? qb[0] : 0.0;
+    const long long i = atb(SA_PK);
+    qs[0] = sS[i]; qc[0] = sC[i];
+    if (JL) qb[0] = sB[i];
+    // JL: sjmat = (s1 - s2) * bjmat with s2 = 0; k lines: dq = s2 goes
+    // back into sjmat
+    if (lane_ok) sS[i0 + (long long)(n - 1) * st] = JL ? (0. + sn - 0.) * b : sn;
+  }
+  auto bwd = [&](int u, int d) {
+    const double s = qs[d], c = qc[d], b = JL ? qb[d] : 0.0;
+    const long long i = atb(u + SA_PK);
+    qs[d] = sS[i]; qc[d] = sC[i];
+    if (JL) qb[d] = sB[i];
+    sn = s - c * sn;
+    if (lane_ok) sS[i0 + (long long)(n - 1 - u) * st] = JL ? (0. + sn - 0.) * b : sn;
+  };
+  int ub = 1;
+  for (; ub + SA_PK <= n; ub += SA_PK) {
 #pragma unroll
-    for (int d = 0; d < SA_PK; ++d) {
-      const int u = u0 + d;
-      if (u < n) {
-        const int t = n - 1 - u;
-        double s = qs[d], c = qc[d];
-        if (u + SA_PK < n) {
-          long long i = i0 + (t - SA_PK) * st;
-          qs[d] = sS[i]; qc[d] = sC[i];
-        }
-        if (u > 0) sn = s - c * sn;
-        if (lane_ok) sS[i0 + t * st] = sn;
+    for (int e = 0; e < SA_PK; ++e) bwd(ub + e, (1 + e) % SA_PK);
+  }
+#pragma unroll
+  for (int e = 0; e < SA_PK; ++e)
+    if (ub + e < n) bwd(ub + e, (1 + e) % SA_PK);
+  if (JL || !lane_ok) return;
+  // nu-tilde update with its clip and the eddy viscosity, eight cells at a
+  // time with every load of a group issued before its stores
+  const double vnulim = 1.0e-20, relfac = 1.0;
+  gdouble* __restrict__ q5p = gptr(M.q[4]);
+  const gdouble* __restrict__ q1p = gptr(M.q[0]);
+  const gdouble* __restrict__ qnp = gptr(M.q[M.nq - 1]);
+  const gdouble* __restrict__ vmp = gptr(M.sa_vmul);
+  const __attribute__((address_space(1))) int* __restrict__ ibp =
+      (const __attribute__((address_space(1))) int*)M.iblank;
+  gdouble* __restrict__ tmu = gptr(M.turmu);
+  for (int t0 = 0; t0 < n; t0 += 8) {
+    double dq[8], q5[8], q1[8], qn[8], vm[8];
+    int ib[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const long long i = i0 + (long long)(t0 + d < n ? t0 + d : n - 1) * st;
+      dq[d] = sS[i]; q5[d] = q5p[i]; q1[d] = q1p[i]; qn[d] = qnp[i]; vm[d] = vmp[i]; ib[d] = ibp[i];
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      if (t0 + d < n) {
+        const long long i = i0 + (long long)(t0 + d) * st;
+        const double v = fmax((q5[d] + relfac * (0. + dq[d]) * (double)imx(ib[d], 0)), vnulim);
+        q5p[i] = v;
+        tmu[i] = c_turm_val(q1[d], qn[d], vm[d], v);
       }
     }
-  }
-  // nu-tilde update with its clip and the eddy viscosity, cell by cell
-  if (!lane_ok) return;
-  const double vnulim = 1.0e-20, relfac = 1.0;
-  for (int t = 0; t < n; ++t) {
-    long long i = i0 + t * st;
-    double dq = relfac * (0. + sS[i]);
-    double q5 = M.q[4][i];
-    q5 = fmax((q5 + dq * (double)imx(M.iblank[i], 0)), vnulim);
-    M.q[4][i] = q5;
-    c_turm_cell(M, i);
   }
 }
 
@@ -940,7 +1007,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   else hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_J, st);
-  if (per_lane) hipLaunchKernelGGL(k_sa_lines_k, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
+  if (per_lane) hipLaunchKernelGGL(k_sa_lines<0>, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
   else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
