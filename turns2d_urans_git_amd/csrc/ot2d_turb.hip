@@ -723,22 +723,31 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
     if (t + 1 < ntile) load(t + 1);
     __syncthreads();
     if (l < nrow) {
-      // unrolled over the tile so the LDS reads run ahead of the recurrence
+      // a full tile is one straight-line unrolled block, so its LDS reads
+      // run ahead of the recurrence; the line's first cell and a partial
+      // last tile take separate paths
       const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
+      auto elem = [&](int cc) {
+        double ai = ta[l][cc];
+        double bb = 1. / (tb[l][cc] - ai * cp);
+        sp = (ts[l][cc] - ai * sp) * bb;
+        cp = tc[l][cc] * bb;
+        ts[l][cc] = sp; tc[l][cc] = cp;
+      };
+      if (c0 == 0) {
+        double bb = 1. / tb[l][0];
+        sp = ts[l][0] * bb; cp = tc[l][0] * bb;
+        ts[l][0] = sp; tc[l][0] = cp;
 #pragma unroll
-      for (int cc = 0; cc < SA_TJ; ++cc) {
-        if (cc < cn) {
-          if (c0 + cc == 0) {
-            double bb = 1. / tb[l][0];
-            sp = ts[l][0] * bb; cp = tc[l][0] * bb;
-          } else {
-            double ai = ta[l][cc];
-            double bb = 1. / (tb[l][cc] - ai * cp);
-            sp = (ts[l][cc] - ai * sp) * bb;
-            cp = tc[l][cc] * bb;
-          }
-          ts[l][cc] = sp; tc[l][cc] = cp;
-        }
+        for (int cc = 1; cc < SA_TJ; ++cc)
+          if (cc < cn) elem(cc);
+      } else if (cn == SA_TJ) {
+#pragma unroll
+        for (int cc = 0; cc < SA_TJ; ++cc) elem(cc);
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < SA_TJ; ++cc)
+          if (cc < cn) elem(cc);
       }
     }
     __syncthreads();
@@ -772,13 +781,20 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
     __syncthreads();
     if (l < nrow) {
       const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
+      auto elem = [&](int cc) {
+        sn = ts[l][cc] - tc[l][cc] * sn;
+        ts[l][cc] = (0. + sn - 0.) * tb[l][cc];
+      };
+      if (t == ntile - 1) {
+        // the line's last cell starts the recurrence
+        sn = ts[l][cn - 1];
+        ts[l][cn - 1] = (0. + sn - 0.) * tb[l][cn - 1];
 #pragma unroll
-      for (int cc = SA_TJ - 1; cc >= 0; --cc) {
-        if (cc < cn) {
-          if (c0 + cc == n - 1) sn = ts[l][cc];
-          else sn = ts[l][cc] - tc[l][cc] * sn;
-          ts[l][cc] = (0. + sn - 0.) * tb[l][cc];
-        }
+        for (int cc = SA_TJ - 2; cc >= 0; --cc)
+          if (cc < cn - 1) elem(cc);
+      } else {
+#pragma unroll
+        for (int cc = SA_TJ - 1; cc >= 0; --cc) elem(cc);
       }
     }
     __syncthreads();
