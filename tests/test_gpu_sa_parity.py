@@ -128,3 +128,37 @@ def test_sst_larger_grid_vs_oracle(variant):
     np.testing.assert_allclose(h_d, h_o, rtol=1e-10)
     H.assert_cellwise(d.get_state(0), o.get_state(0), 1e-10, "q")
     H.assert_cellwise(d.get_field(0, abi.F_TURMU), o.get_field(0, abi.F_TURMU), 1e-10, "turmu")
+
+
+def _cartesian_sa(lib_case):
+    """A 66 x 26 Cartesian mesh (65 x 25 interior cells) with far field on
+    all sides and a perturbed, sheared state with eddy viscosity."""
+    from turns2d_urans_git_amd import grid
+    xv, yv = grid.cartesian_grid(66, 26, -1.0, 1.0, -0.4, 0.4)
+    g = grid.CGrid(xv=np.asfortranarray(xv), yv=np.asfortranarray(yv), jtail1=1, jmax=66, kmax=26)
+    case = lib_case(g, SA_DECK, segs=grid.box_bc())
+    q = case.get_state(0)
+    rng = np.random.default_rng(7)
+    jd, kd = q.shape[:2]
+    J, K = np.meshgrid(np.arange(jd), np.arange(kd), indexing="ij")
+    for n in range(4):
+        f = 1 + 0.03 * np.sin(0.37 * J + 0.5 * n) * np.cos(0.53 * K - 0.2 * n) + 0.003 * rng.standard_normal((jd, kd))
+        q[:, :, 0, n] *= f
+    q[:, :, 0, 1] += 0.02 * q[:, :, 0, 0]
+    case.set_state(0, q)
+    case.set_field(0, abi.F_TURMU, 5.0 + 3.0 * rng.random((jd, kd, 1)))
+    return case
+
+
+def test_viscous_residual_ragged_tiles_vs_oracle():
+    """Viscous residual where the interior (65 x 25 cells) leaves one column
+    and one row in the last tiles, so both one-sided closures (src/visrhs.f:
+    141-150, 262-271) fall on a tile's first row or column."""
+    o, d = _cartesian_sa(H.oracle_case), _cartesian_sa(H.gpu_case)
+    for c in (o, d):
+        c.time_step()
+        c.set_field(0, abi.F_S, np.zeros(c.get_field(0, abi.F_S).shape))
+        c.compute_residual()
+    ro, rd = o.get_field(0, abi.F_S), d.get_field(0, abi.F_S)
+    assert np.abs(ro).max() > 1e-4
+    H.assert_cellwise(rd, ro, 1e-12, "residual")

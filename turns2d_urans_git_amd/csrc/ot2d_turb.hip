@@ -1,8 +1,9 @@
 // gfx950 kernels of the viscous residual and the Spalart-Allmaras model.
 //
-//   visrhs (src/visrhs.f:2-287)        k_visrhs      one thread per interior cell,
-//                                                    gathers its thin-layer fluxes and
-//                                                    cross-derivative terms
+//   visrhs (src/visrhs.f:2-287)        k_visrhs      one workgroup per 32x8 tile of cells:
+//                                                    per-cell primitives, coefficients and
+//                                                    cross terms once into LDS, then each
+//                                                    cell's thin-layer fluxes from there
 //   vmu_sa (src/samodel2d.f:2-197)     k_sa_prim     laminar viscosity, u, v (all cells)
 //                                      k_turbc       SA boundary conditions (1 block / instance)
 //                                      k_sa_rhs      convection, diffusion, source and the
@@ -28,18 +29,21 @@ struct VisPrim {
 };
 
 // per-cell quantities of the visrhs line loops (src/visrhs.f:47-57)
-__device__ __forceinline__ VisPrim vis_prim(const DevMesh& M, long long i, double c2b, double c2bp, double prtr) {
+__device__ __forceinline__ VisPrim vis_prim_of(double q1, double q2, double q3, double q4, double turm, double ggm1,
+                                               double c2b, double c2bp, double prtr) {
   VisPrim p;
-  double ra = 1. / M.q[0][i];
-  p.u = M.q[1][i] * ra;
-  p.v = M.q[2][i] * ra;
-  p.e = M.q[3][i] * ra - .5 * (sqd(p.u) + sqd(p.v));
-  double tt = M.c.ggm1 * p.e;
+  double ra = 1. / q1;
+  p.u = q2 * ra;
+  p.v = q3 * ra;
+  p.e = q4 * ra - .5 * (sqd(p.u) + sqd(p.v));
+  double tt = ggm1 * p.e;
   double vmue = c2bp * tt * sqrt(tt) / (c2b + tt);
-  double turm = M.turmu[i];
   p.vnu = vmue + turm;
   p.gkap = vmue + prtr * turm;
   return p;
+}
+__device__ __forceinline__ VisPrim vis_prim(const DevMesh& M, long long i, double c2b, double c2bp, double prtr) {
+  return vis_prim_of(M.q[0][i], M.q[1][i], M.q[2][i], M.q[3][i], M.turmu[i], M.c.ggm1, c2b, c2bp, prtr);
 }
 
 struct VisCoef {
@@ -47,67 +51,65 @@ struct VisCoef {
   double d1, d2, d4, d5, d7;      // cross-derivative coefficients
 };
 
-// eta direction (j = const line) coefficients of cell (j,k): thin layer from
-// the cell, cross terms from its j-1, j+1 neighbours (src/visrhs.f:59-92)
-__device__ __forceinline__ VisCoef eta_coef(const DevMesh& M, int j, int k, const VisPrim& p, double dre,
-                                            double gkpr) {
-  long long i = cell(M, j, k), im = cell(M, j - 1, k), ip = cell(M, j + 1, k);
-  const double *xx = M.xx, *xy = M.xy, *yx = M.yx, *yy = M.yy, *qj = M.q[M.nq - 1];
-  double rj = 1. / qj[i], rjm = 1. / qj[im], rjp = 1. / qj[ip];
-  VisCoef c;
-  double b4 = (sqd(yx[i]) + sqd(yy[i])) * rj;
-  c.c1 = (b4 + sqd(yx[i]) / 3. * rj) * p.vnu * dre;
-  c.c2 = (b4 + sqd(yy[i]) / 3. * rj) * p.vnu * dre;
-  c.c5 = (yx[i] * yy[i] / 3. * rj) * p.vnu * dre;
-  c.c4 = b4 * gkpr * p.gkap * dre;
-  double t;
-  t = (4. / 3. * xx[im] * yx[im] + xy[im] * yy[im]) * rjm;
-  t = t + (4. / 3. * xx[ip] * yx[ip] + xy[ip] * yy[ip]) * rjp;
-  c.d1 = p.vnu * dre * t;
-  t = (4. / 3. * xy[im] * yy[im] + xx[im] * yx[im]) * rjm;
-  t = t + (4. / 3. * xy[ip] * yy[ip] + xx[ip] * yx[ip]) * rjp;
-  c.d2 = p.vnu * dre * t;
-  t = (xy[im] * yy[im] + xx[im] * yx[im]) * rjm;
-  t = t + (xy[ip] * yy[ip] + xx[ip] * yx[ip]) * rjp;
-  c.d4 = t * gkpr * p.gkap * dre;
-  t = (-2. / 3. * xy[im] * yx[im] + xx[im] * yy[im]) * rjm;
-  t = t + (-2. / 3. * xy[ip] * yx[ip] + xx[ip] * yy[ip]) * rjp;
-  c.d5 = p.vnu * dre * t;
-  t = (-2. / 3. * xx[im] * yy[im] + xy[im] * yx[im]) * rjm;
-  t = t + (-2. / 3. * xx[ip] * yy[ip] + xy[ip] * yx[ip]) * rjp;
-  c.d7 = p.vnu * dre * t;
-  return c;
+// metrics of a cell and the reciprocal of its Jacobian q(.,.,nq)
+struct VisMet {
+  double xx, xy, yx, yy, r;
+};
+__device__ __forceinline__ VisMet vis_met(const DevMesh& M, long long i) {
+  VisMet m;
+  m.xx = M.xx[i];
+  m.xy = M.xy[i];
+  m.yx = M.yx[i];
+  m.yy = M.yy[i];
+  m.r = 1. / M.q[M.nq - 1][i];
+  return m;
 }
 
-// xi direction (k = const line) coefficients (src/visrhs.f:184-217)
+// coefficients of one line direction at a cell: the thin-layer terms from
+// the cell's metric pair (ma, mb) = (yx, yy) along eta, (xx, xy) along xi,
+// and the cross terms from its two neighbours m, p across the line
+// (src/visrhs.f:59-92 eta, 184-217 xi; both use the same expressions)
+__device__ __forceinline__ VisCoef vis_coef(double ma, double mb, double rj, const VisMet& m, const VisMet& p,
+                                            const VisPrim& c, double dre, double gkpr) {
+  VisCoef o;
+  double b4 = (sqd(ma) + sqd(mb)) * rj;
+  o.c1 = (b4 + sqd(ma) / 3. * rj) * c.vnu * dre;
+  o.c2 = (b4 + sqd(mb) / 3. * rj) * c.vnu * dre;
+  o.c5 = (ma * mb / 3. * rj) * c.vnu * dre;
+  o.c4 = b4 * gkpr * c.gkap * dre;
+  double t;
+  t = (4. / 3. * m.xx * m.yx + m.xy * m.yy) * m.r;
+  t = t + (4. / 3. * p.xx * p.yx + p.xy * p.yy) * p.r;
+  o.d1 = c.vnu * dre * t;
+  t = (4. / 3. * m.xy * m.yy + m.xx * m.yx) * m.r;
+  t = t + (4. / 3. * p.xy * p.yy + p.xx * p.yx) * p.r;
+  o.d2 = c.vnu * dre * t;
+  t = (m.xy * m.yy + m.xx * m.yx) * m.r;
+  t = t + (p.xy * p.yy + p.xx * p.yx) * p.r;
+  o.d4 = t * gkpr * c.gkap * dre;
+  t = (-2. / 3. * m.xy * m.yx + m.xx * m.yy) * m.r;
+  t = t + (-2. / 3. * p.xy * p.yx + p.xx * p.yy) * p.r;
+  o.d5 = c.vnu * dre * t;
+  t = (-2. / 3. * m.xx * m.yy + m.xy * m.yx) * m.r;
+  t = t + (-2. / 3. * p.xx * p.yy + p.xy * p.yx) * p.r;
+  o.d7 = c.vnu * dre * t;
+  return o;
+}
+
+// eta direction (j = const line) coefficients of cell (j,k) from global memory
+__device__ __forceinline__ VisCoef eta_coef(const DevMesh& M, int j, int k, const VisPrim& p, double dre,
+                                            double gkpr) {
+  long long i = cell(M, j, k);
+  return vis_coef(M.yx[i], M.yy[i], 1. / M.q[M.nq - 1][i], vis_met(M, cell(M, j - 1, k)),
+                  vis_met(M, cell(M, j + 1, k)), p, dre, gkpr);
+}
+
+// xi direction (k = const line) coefficients of cell (j,k) from global memory
 __device__ __forceinline__ VisCoef xi_coef(const DevMesh& M, int j, int k, const VisPrim& p, double dre,
                                            double gkpr) {
-  long long i = cell(M, j, k), km = cell(M, j, k - 1), kp = cell(M, j, k + 1);
-  const double *xx = M.xx, *xy = M.xy, *yx = M.yx, *yy = M.yy, *qj = M.q[M.nq - 1];
-  double rj = 1. / qj[i], rkm = 1. / qj[km], rkp = 1. / qj[kp];
-  VisCoef c;
-  double a4 = (sqd(xx[i]) + sqd(xy[i])) * rj;
-  c.c1 = (a4 + sqd(xx[i]) / 3. * rj) * p.vnu * dre;
-  c.c2 = (a4 + sqd(xy[i]) / 3. * rj) * p.vnu * dre;
-  c.c5 = (xx[i] * xy[i] / 3. * rj) * p.vnu * dre;
-  c.c4 = a4 * gkpr * p.gkap * dre;
-  double t;
-  t = (4. / 3. * xx[km] * yx[km] + xy[km] * yy[km]) * rkm;
-  t = t + (4. / 3. * xx[kp] * yx[kp] + xy[kp] * yy[kp]) * rkp;
-  c.d1 = p.vnu * dre * t;
-  t = (4. / 3. * xy[km] * yy[km] + xx[km] * yx[km]) * rkm;
-  t = t + (4. / 3. * xy[kp] * yy[kp] + xx[kp] * yx[kp]) * rkp;
-  c.d2 = p.vnu * dre * t;
-  t = (xy[km] * yy[km] + xx[km] * yx[km]) * rkm;
-  t = t + (xy[kp] * yy[kp] + xx[kp] * yx[kp]) * rkp;
-  c.d4 = t * gkpr * p.gkap * dre;
-  t = (-2. / 3. * xy[km] * yx[km] + xx[km] * yy[km]) * rkm;
-  t = t + (-2. / 3. * xy[kp] * yx[kp] + xx[kp] * yy[kp]) * rkp;
-  c.d5 = p.vnu * dre * t;
-  t = (-2. / 3. * xx[km] * yy[km] + xy[km] * yx[km]) * rkm;
-  t = t + (-2. / 3. * xx[kp] * yy[kp] + xy[kp] * yx[kp]) * rkp;
-  c.d7 = p.vnu * dre * t;
-  return c;
+  long long i = cell(M, j, k);
+  return vis_coef(M.xx[i], M.xy[i], 1. / M.q[M.nq - 1][i], vis_met(M, cell(M, j, k - 1)),
+                  vis_met(M, cell(M, j, k + 1)), p, dre, gkpr);
 }
 
 struct Tri { double f2, f3, f4; };
@@ -131,12 +133,15 @@ __device__ __forceinline__ Tri vis_face(const VisPrim& pa, const VisCoef& ca, co
 // velocities and internal energy by division (q2/q1 ...), as the cross
 // derivative terms use them
 struct DivPrim { double u, v, e; };
-__device__ __forceinline__ DivPrim div_prim(const DevMesh& M, long long i) {
+__device__ __forceinline__ DivPrim div_prim_of(double q1, double q2, double q3, double q4) {
   DivPrim d;
-  d.u = M.q[1][i] / M.q[0][i];
-  d.v = M.q[2][i] / M.q[0][i];
-  d.e = M.q[3][i] / M.q[0][i] - .5 * (sqd(d.u) + sqd(d.v));
+  d.u = q2 / q1;
+  d.v = q3 / q1;
+  d.e = q4 / q1 - .5 * (sqd(d.u) + sqd(d.v));
   return d;
+}
+__device__ __forceinline__ DivPrim div_prim(const DevMesh& M, long long i) {
+  return div_prim_of(M.q[0][i], M.q[1][i], M.q[2][i], M.q[3][i]);
 }
 
 // cross-derivative term of the eta line at cell c from its j neighbours
@@ -168,74 +173,173 @@ __device__ __forceinline__ Tri xi_cross(const VisCoef& c, const DivPrim& p, cons
   return f;
 }
 
-__global__ __launch_bounds__(256) void k_visrhs(const DevMesh* __restrict__ Ms) {
+// A workgroup owns VIS_TJ x VIS_TK interior cells.  Every per-cell quantity
+// of the reference's line loops -- primitives, quotient primitives, metrics,
+// the coefficients and cross terms of both directions -- is formed once for
+// the cells of the tile and of its one-cell ring and kept in LDS; each cell
+// then assembles its two face fluxes per direction from there.  The
+// arithmetic of every quantity is the reference's, so the result does not
+// depend on the tiling.
+constexpr int VIS_TJ = 32, VIS_TK = 8, VIS_THREADS = VIS_TJ * VIS_TK;
+constexpr int VIS_RJ = VIS_TJ + 2, VIS_RK = VIS_TK + 2;
+constexpr int VIS_NR = VIS_RJ * VIS_RK;   // tile and ring, ring row 0 = k0-1, column 0 = j0-1
+constexpr int VIS_NE = VIS_TJ * VIS_RK;   // eta cells: tile columns, rows k0-1 .. k0+TK
+constexpr int VIS_NX = VIS_RJ * VIS_TK;   // xi cells: columns j0-1 .. j0+TJ, tile rows
+
+struct VisLds {
+  double u[VIS_NR], v[VIS_NR], e[VIS_NR], vnu[VIS_NR], gkap[VIS_NR];
+  double du[VIS_NR], dv[VIS_NR], de[VIS_NR];
+  double xx[VIS_NR], xy[VIS_NR], yx[VIS_NR], yy[VIS_NR], r[VIS_NR];
+  double ec[4][VIS_NE], eg[3][VIS_NE];     // eta: c1 c2 c5 c4 and the cross term
+  double xc[4][VIS_NX], xg[3][VIS_NX];     // xi
+};
+constexpr size_t VIS_LDS = sizeof(VisLds);
+
+__device__ __forceinline__ VisPrim lds_prim(const VisLds& L, int c) {
+  VisPrim p;
+  p.u = L.u[c]; p.v = L.v[c]; p.e = L.e[c]; p.vnu = L.vnu[c]; p.gkap = L.gkap[c];
+  return p;
+}
+__device__ __forceinline__ DivPrim lds_div(const VisLds& L, int c) {
+  DivPrim d;
+  d.u = L.du[c]; d.v = L.dv[c]; d.e = L.de[c];
+  return d;
+}
+__device__ __forceinline__ VisMet lds_met(const VisLds& L, int c) {
+  VisMet m;
+  m.xx = L.xx[c]; m.xy = L.xy[c]; m.yx = L.yx[c]; m.yy = L.yy[c]; m.r = L.r[c];
+  return m;
+}
+template <int N>
+__device__ __forceinline__ VisCoef lds_coef(const double (&cf)[4][N], int c) {
+  VisCoef o{};
+  o.c1 = cf[0][c]; o.c2 = cf[1][c]; o.c5 = cf[2][c]; o.c4 = cf[3][c];
+  return o;
+}
+template <int N>
+__device__ __forceinline__ Tri lds_tri(const double (&g)[3][N], int c) {
+  Tri f;
+  f.f2 = g[0][c]; f.f3 = g[1][c]; f.f4 = g[2][c];
+  return f;
+}
+
+__global__ __launch_bounds__(VIS_THREADS) void k_visrhs(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
-  if (*M.status) return;
+  const int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
+  const int tiles_j = (nj + VIS_TJ - 1) / VIS_TJ;
+  if ((int)blockIdx.x >= tiles_j * ((nk + VIS_TK - 1) / VIS_TK) || *M.status) return;
+  extern __shared__ __attribute__((aligned(16))) double vis_raw[];
+  VisLds& L = *reinterpret_cast<VisLds*>(vis_raw);
   const DevConst& C = M.c;
   const double gkpr = C.gamma / C.pr, prtr = C.pr / 0.9, dre = .5 / C.rey;
   const double c2b = 198.6 / C.tinf, c2bp = c2b + 1.;
-  const int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
-  long long ncell = (long long)nj * nk;
-  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
-    const int j = M.js + (int)(w % nj), k = M.ks + (int)(w / nj);
+  const int j0 = M.js + ((int)blockIdx.x % tiles_j) * VIS_TJ, k0 = M.ks + ((int)blockIdx.x / tiles_j) * VIS_TK;
+  const int t = threadIdx.x;
+  const double* qj = M.q[M.nq - 1];
+
+  // tile and ring: primitives (src/visrhs.f:47-57), quotient primitives
+  // (src/visrhs.f:108-119), metrics and 1/q(nq)
+  for (int n = t; n < VIS_NR; n += VIS_THREADS) {
+    const int j = j0 - 1 + n % VIS_RJ, k = k0 - 1 + n / VIS_RJ;
+    if (j > M.je + 1 || k > M.ke + 1) continue;
     const long long i = cell(M, j, k);
-    double s2 = M.s[1][i], s3 = M.s[2][i], s4 = M.s[3][i];
-    // ---- eta part (j = const line): faces k-1|k, k|k+1, cross terms
-    {
-      VisPrim p0 = vis_prim(M, cell(M, j, k - 1), c2b, c2bp, prtr);
-      VisPrim p1 = vis_prim(M, i, c2b, c2bp, prtr);
-      VisPrim p2 = vis_prim(M, cell(M, j, k + 1), c2b, c2bp, prtr);
-      VisCoef c0 = eta_coef(M, j, k - 1, p0, dre, gkpr);
-      VisCoef c1 = eta_coef(M, j, k, p1, dre, gkpr);
-      VisCoef c2 = eta_coef(M, j, k + 1, p2, dre, gkpr);
-      Tri fm = vis_face(p0, c0, p1, c1);   // face k-1
-      Tri fp = vis_face(p1, c1, p2, c2);   // face k
-      Tri gm = eta_cross(c0, div_prim(M, cell(M, j + 1, k - 1)), div_prim(M, cell(M, j - 1, k - 1)));
-      Tri gc = eta_cross(c1, div_prim(M, cell(M, j + 1, k)), div_prim(M, cell(M, j - 1, k)));
-      if (k < M.ke) {
-        Tri gp = eta_cross(c2, div_prim(M, cell(M, j + 1, k + 1)), div_prim(M, cell(M, j - 1, k + 1)));
-        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
-        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
-        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
-      } else {  // one-sided closure at kend (src/visrhs.f:141-150)
-        VisPrim pq = vis_prim(M, cell(M, j, k - 2), c2b, c2bp, prtr);
-        VisCoef cq = eta_coef(M, j, k - 2, pq, dre, gkpr);
-        Tri gq = eta_cross(cq, div_prim(M, cell(M, j + 1, k - 2)), div_prim(M, cell(M, j - 1, k - 2)));
-        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
-        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
-        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3. * gc.f4);
-      }
-    }
-    // ---- xi part (k = const line)
-    {
-      VisPrim p0 = vis_prim(M, cell(M, j - 1, k), c2b, c2bp, prtr);
-      VisPrim p1 = vis_prim(M, i, c2b, c2bp, prtr);
-      VisPrim p2 = vis_prim(M, cell(M, j + 1, k), c2b, c2bp, prtr);
-      VisCoef c0 = xi_coef(M, j - 1, k, p0, dre, gkpr);
-      VisCoef c1 = xi_coef(M, j, k, p1, dre, gkpr);
-      VisCoef c2 = xi_coef(M, j + 1, k, p2, dre, gkpr);
-      Tri fm = vis_face(p0, c0, p1, c1);
-      Tri fp = vis_face(p1, c1, p2, c2);
-      Tri gm = xi_cross(c0, div_prim(M, cell(M, j - 1, k + 1)), div_prim(M, cell(M, j - 1, k - 1)));
-      Tri gc = xi_cross(c1, div_prim(M, cell(M, j, k + 1)), div_prim(M, cell(M, j, k - 1)));
-      if (j < M.je) {
-        Tri gp = xi_cross(c2, div_prim(M, cell(M, j + 1, k + 1)), div_prim(M, cell(M, j + 1, k - 1)));
-        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
-        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
-        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
-      } else {  // one-sided closure at jend (src/visrhs.f:262-271)
-        VisPrim pq = vis_prim(M, cell(M, j - 2, k), c2b, c2bp, prtr);
-        VisCoef cq = xi_coef(M, j - 2, k, pq, dre, gkpr);
-        Tri gq = xi_cross(cq, div_prim(M, cell(M, j - 2, k + 1)), div_prim(M, cell(M, j - 2, k - 1)));
-        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
-        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
-        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3 * gc.f4);
-      }
-    }
-    M.s[1][i] = s2;
-    M.s[2][i] = s3;
-    M.s[3][i] = s4;
+    const double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i], turm = M.turmu[i];
+    const double xx = M.xx[i], xy = M.xy[i], yx = M.yx[i], yy = M.yy[i], qn = qj[i];
+    const VisPrim p = vis_prim_of(q1, q2, q3, q4, turm, C.ggm1, c2b, c2bp, prtr);
+    const DivPrim d = div_prim_of(q1, q2, q3, q4);
+    L.u[n] = p.u; L.v[n] = p.v; L.e[n] = p.e; L.vnu[n] = p.vnu; L.gkap[n] = p.gkap;
+    L.du[n] = d.u; L.dv[n] = d.v; L.de[n] = d.e;
+    L.xx[n] = xx; L.xy[n] = xy; L.yx[n] = yx; L.yy[n] = yy; L.r[n] = 1. / qn;
   }
+  __syncthreads();
+
+  // coefficients and cross terms: eta along the tile columns for rows
+  // k0-1 .. k0+TK, xi along the tile rows for columns j0-1 .. j0+TJ
+  for (int n = t; n < VIS_NE + VIS_NX; n += VIS_THREADS) {
+    if (n < VIS_NE) {
+      const int a = n % VIS_TJ, b = n / VIS_TJ;
+      if (j0 + a > M.je || k0 - 1 + b > M.ke + 1) continue;
+      const int c = b * VIS_RJ + a + 1;
+      const VisCoef o = vis_coef(L.yx[c], L.yy[c], L.r[c], lds_met(L, c - 1), lds_met(L, c + 1), lds_prim(L, c), dre,
+                                 gkpr);
+      const Tri g = eta_cross(o, lds_div(L, c + 1), lds_div(L, c - 1));
+      L.ec[0][n] = o.c1; L.ec[1][n] = o.c2; L.ec[2][n] = o.c5; L.ec[3][n] = o.c4;
+      L.eg[0][n] = g.f2; L.eg[1][n] = g.f3; L.eg[2][n] = g.f4;
+    } else {
+      const int m = n - VIS_NE, a = m % VIS_RJ, b = m / VIS_RJ;
+      if (j0 - 1 + a > M.je + 1 || k0 + b > M.ke) continue;
+      const int c = (b + 1) * VIS_RJ + a;
+      const VisCoef o = vis_coef(L.xx[c], L.xy[c], L.r[c], lds_met(L, c - VIS_RJ), lds_met(L, c + VIS_RJ),
+                                 lds_prim(L, c), dre, gkpr);
+      const Tri g = xi_cross(o, lds_div(L, c + VIS_RJ), lds_div(L, c - VIS_RJ));
+      L.xc[0][m] = o.c1; L.xc[1][m] = o.c2; L.xc[2][m] = o.c5; L.xc[3][m] = o.c4;
+      L.xg[0][m] = g.f2; L.xg[1][m] = g.f3; L.xg[2][m] = g.f4;
+    }
+  }
+  __syncthreads();
+
+  const int a = t % VIS_TJ, b = t / VIS_TJ;
+  const int j = j0 + a, k = k0 + b;
+  if (j > M.je || k > M.ke) return;
+  const long long i = cell(M, j, k);
+  double s2 = M.s[1][i], s3 = M.s[2][i], s4 = M.s[3][i];
+  const int c = (b + 1) * VIS_RJ + a + 1;
+  const VisPrim p1 = lds_prim(L, c);
+  // ---- eta part (j = const line): faces k-1|k, k|k+1, cross terms
+  {
+    const int e = (b + 1) * VIS_TJ + a;
+    const VisCoef c0 = lds_coef(L.ec, e - VIS_TJ), c1 = lds_coef(L.ec, e), c2 = lds_coef(L.ec, e + VIS_TJ);
+    const Tri fm = vis_face(lds_prim(L, c - VIS_RJ), c0, p1, c1);   // face k-1
+    const Tri fp = vis_face(p1, c1, lds_prim(L, c + VIS_RJ), c2);   // face k
+    const Tri gm = lds_tri(L.eg, e - VIS_TJ), gc = lds_tri(L.eg, e);
+    if (k < M.ke) {
+      const Tri gp = lds_tri(L.eg, e + VIS_TJ);
+      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
+      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
+      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
+    } else {  // one-sided closure at kend (src/visrhs.f:141-150)
+      Tri gq;
+      if (b >= 1) {
+        gq = lds_tri(L.eg, e - 2 * VIS_TJ);
+      } else {  // row k-2 lies below the tile's ring
+        const VisPrim pq = vis_prim(M, cell(M, j, k - 2), c2b, c2bp, prtr);
+        const VisCoef cq = eta_coef(M, j, k - 2, pq, dre, gkpr);
+        gq = eta_cross(cq, div_prim(M, cell(M, j + 1, k - 2)), div_prim(M, cell(M, j - 1, k - 2)));
+      }
+      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
+      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
+      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3. * gc.f4);
+    }
+  }
+  // ---- xi part (k = const line)
+  {
+    const int x = b * VIS_RJ + a + 1;
+    const VisCoef c0 = lds_coef(L.xc, x - 1), c1 = lds_coef(L.xc, x), c2 = lds_coef(L.xc, x + 1);
+    const Tri fm = vis_face(lds_prim(L, c - 1), c0, p1, c1);
+    const Tri fp = vis_face(p1, c1, lds_prim(L, c + 1), c2);
+    const Tri gm = lds_tri(L.xg, x - 1), gc = lds_tri(L.xg, x);
+    if (j < M.je) {
+      const Tri gp = lds_tri(L.xg, x + 1);
+      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
+      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
+      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
+    } else {  // one-sided closure at jend (src/visrhs.f:262-271)
+      Tri gq;
+      if (a >= 1) {
+        gq = lds_tri(L.xg, x - 2);
+      } else {  // column j-2 lies left of the tile's ring
+        const VisPrim pq = vis_prim(M, cell(M, j - 2, k), c2b, c2bp, prtr);
+        const VisCoef cq = xi_coef(M, j - 2, k, pq, dre, gkpr);
+        gq = xi_cross(cq, div_prim(M, cell(M, j - 2, k + 1)), div_prim(M, cell(M, j - 2, k - 1)));
+      }
+      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
+      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
+      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3 * gc.f4);
+    }
+  }
+  M.s[1][i] = s2;
+  M.s[2][i] = s3;
+  M.s[3][i] = s4;
 }
 
 // ========================================================================= SA
@@ -982,7 +1086,19 @@ static long long max_cells(const DevMesh* hM, int nmi) {
 }
 
 hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
-  hipLaunchKernelGGL(k_visrhs, dim3(nblk2(max_cells(hM, nmi), 256), nmi), dim3(256), 0, st, dM);
+  int tiles = 1;
+  for (int i = 0; i < nmi; ++i) {
+    const int nj = hM[i].je - hM[i].js + 1, nk = hM[i].ke - hM[i].ks + 1;
+    const int n = ((nj + VIS_TJ - 1) / VIS_TJ) * ((nk + VIS_TK - 1) / VIS_TK);
+    tiles = n > tiles ? n : tiles;
+  }
+  static bool lds_set = false;
+  if (!lds_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_visrhs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)VIS_LDS);
+    if (e != hipSuccess) return e;
+    lds_set = true;
+  }
+  hipLaunchKernelGGL(k_visrhs, dim3(tiles, nmi), dim3(VIS_THREADS), VIS_LDS, st, dM);
   return hipGetLastError();
 }
 
