@@ -1519,89 +1519,120 @@ __device__ __forceinline__ void nb_matrix(const LUNb& n, double gm1, double m[16
   for (int w = 0; w < 4; ++w) m[w * 4 + w] += n.spec;
 }
 
-// neighbour term of cell i: matrix scaled by f, or folded into c when the
-// neighbour is a halo-ring cell (its s after s*tscale and point Jacobi)
-__device__ __forceinline__ void lu32_term(const DevMesh& M, long long in, bool halo, int jdir, int dir, double f,
-                                          double eps2, double epsv, double* P, double* c) {
-  CellLoad cn = load_cell(M, in);
-  LUCellF64 Ln = lu_of(M.c, cn, epsv);
+// Prep tiles.  A workgroup of LP_WAVES waves covers 8 rows of a 64-row
+// block and LP_WAVES*8 consecutive sweep steps; row l at step t is cell
+// jr = t - l, so its entries lie in a parallelogram of cells whose bounding
+// box, with the one-cell ring the neighbour terms reach, is LP_RJ x LP_RK
+// cells.  The per-cell quantities of ilu2d (src/lhs.f:524-564) are formed
+// once per cell of that box in LDS; each entry then assembles its operator
+// from the cell and its four neighbours there.
+constexpr int LP_WAVES = 4, LP_STEPS = 8 * LP_WAVES;
+constexpr int LP_RJ = LP_STEPS + 7 + 2, LP_RK = 8 + 2, LP_N = LP_RJ * LP_RK;
+
+struct LUPrepLds {
+  double uu[LP_N], vv[LP_N], uv2[LP_N], ge[LP_N], qx[LP_N], qy[LP_N], cx[LP_N], cy[LP_N], vn[LP_N], d[LP_N],
+      qqx[LP_N], qqy[LP_N];
+  double xx[LP_N], xy[LP_N], yx[LP_N], yy[LP_N], ts[LP_N];
+  double sh[4][LP_N];   // s * tscale * d (src/step.f:141-148, src/lhs.f:568-598)
+};
+constexpr size_t LP_LDS = sizeof(LUPrepLds);
+
+// neighbour term of an entry from the cell at LDS index n: matrix scaled by
+// f, or folded into c when the neighbour is a halo-ring cell (its s after
+// s*tscale and point Jacobi)
+__device__ __forceinline__ void lu32_term(const LUPrepLds& S, int n, bool halo, int jdir, int dir, double f,
+                                          double eps2, double gm1, double* P, double* c) {
+  CellLoad cn;
+  cn.xx = S.xx[n]; cn.xy = S.xy[n]; cn.yx = S.yx[n]; cn.yy = S.yy[n];
+  LUCellF64 Ln;
+  Ln.uu = S.uu[n]; Ln.vv = S.vv[n]; Ln.uv2 = S.uv2[n]; Ln.ge = S.ge[n]; Ln.qx = S.qx[n]; Ln.qy = S.qy[n];
+  Ln.cx = S.cx[n]; Ln.cy = S.cy[n]; Ln.vn = S.vn[n]; Ln.qqx_s = S.qqx[n]; Ln.qqy_s = S.qqy[n];
   LUNb nb;
   if (jdir) nb_j(cn, Ln, eps2, dir, nb);
   else nb_k(cn, Ln, eps2, dir, nb);
   double m[16];
-  nb_matrix(nb, M.c.gm1, m);
+  nb_matrix(nb, gm1, m);
   for (int q = 0; q < 16; ++q) P[q] = f * m[q];
   if (halo) {
     double sh[4];
-    for (int n = 0; n < 4; ++n) sh[n] = M.s[n][in] * cn.ts * Ln.d;
+    for (int w = 0; w < 4; ++w) sh[w] = S.sh[w][n];
     for (int w = 0; w < 4; ++w) c[w] += P[w * 4 + 0] * sh[0] + P[w * 4 + 1] * sh[1] + P[w * 4 + 2] * sh[2] + P[w * 4 + 3] * sh[3];
     for (int q = 0; q < 16; ++q) P[q] = 0.0;
   }
 }
 
-__global__ __launch_bounds__(256) void k_lu_prep32(const DevMesh* __restrict__ Ms) {
+__global__ __launch_bounds__(64 * LP_WAVES) void k_lu_prep32(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
-  const int jd = M.jd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
+  const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
   const double eps2 = M.c.epse * 2.;
   const double epsv = 1. + eps2;
   const LUGeom g = lu_geom(M);
   F4* recf = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_f32);
   F4* recb = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_b32);
   F4* cbs = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_cb);
+  const int nthr = 64 * LP_WAVES;
   // the mail entries start as the empty sentinel: forward, lane 63 of H
   // rows 0 .. ni-1; backward, lane 0 of H2 rows 63 .. ni+62 (every block)
   const long long nmb = 2ll * g.W * g.ni;
   u32x4_t* h = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_h);
   u32x4_t* h2 = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_h2);
-  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < nmb; w += 256ll * gridDim.x) {
+  for (long long w = blockIdx.x * (long long)nthr + threadIdx.x; w < nmb; w += (long long)nthr * gridDim.x) {
     const long long e = w % ((long long)g.W * g.ni), blk = e / g.ni, r = e - blk * g.ni;
     const u32x4_t empty = u32x4_t{LU32_EMPTY, LU32_EMPTY, LU32_EMPTY, LU32_EMPTY};
     if (w < (long long)g.W * g.ni) h[(blk * g.TW + r) * 64 + 63] = empty;
     else h2[(blk * g.TW + r + 63) * 64] = empty;
   }
-  // one thread per (block, step, lane) entry.  A wave covers 8 steps x 8
-  // rows: 8 consecutive lanes are 8 consecutive rows at one step, so each
-  // vector of a step row is stored as whole 128-byte lines, and the 8
-  // steps of one row are 8 consecutive cells along j, so the reads come in
-  // row segments; consecutive waves take the next 8 steps of the same 8
-  // rows, so a workgroup reads 39-cell row segments.  Entries outside the
-  // grid store zero records.
-  const long long nent = (long long)g.W * g.TA * 64;
-  const int tiles_t = g.TA >> 3;
-  auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
-  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < nent; e += 256ll * gridDim.x) {
-    const int lane = (int)(e & 63);
-    const long long tile = e >> 6;
-    const int tb = (int)(tile % tiles_t);
-    const long long rest = tile / tiles_t;
-    const int lb = (int)(rest & 7), w = (int)(rest >> 3);
-    const int l = lb * 8 + (lane & 7), t = tb * 8 + (lane >> 3);
-    const int kk = 64 * w + l, jr = t - l;
-    F4* rf = recf + ((long long)(w * g.TA + t) * 9) * 64 + l;
-    F4* rb = recb + ((long long)(w * g.TA + t) * 8) * 64 + l;
-    if (kk >= g.nk || jr < 0 || jr >= g.ni) {
-      const F4 z = F4{0.f, 0.f, 0.f, 0.f};
-      for (int q = 0; q < 9; ++q) rf[q * 64] = z;
-      for (int q = 0; q < 8; ++q) rb[q * 64] = z;
-      continue;
-    }
-    const int j = js + jr, k = ks + kk;
-    const long long i = cell(M, j, k);
-    CellLoad c0 = load_cell(M, i);
-    LUCellF64 L0 = lu_of(M.c, c0, epsv);
-    const double di = L0.d, f = (c0.ts * 0.5) * di;
-    double u[4], cb[4] = {0.0, 0.0, 0.0, 0.0}, PA[16], PB[16], QA[16], QB[16];
-    for (int n = 0; n < 4; ++n) u[n] = (M.s[n][i] * c0.ts) * di;
-    lu32_term(M, i - 1, j == js, 1, +1, f, eps2, epsv, PA, u);
-    lu32_term(M, i - jd, k == ks, 0, +1, f, eps2, epsv, PB, u);
-    lu32_term(M, i + 1, j == je, 1, -1, -f, eps2, epsv, QA, cb);
-    lu32_term(M, i + jd, k == ke, 0, -1, -f, eps2, epsv, QB, cb);
-    rf[0] = f4(u);
-    for (int r = 0; r < 4; ++r) { rf[(1 + r) * 64] = f4(PA + 4 * r); rf[(5 + r) * 64] = f4(PB + 4 * r); }
-    for (int r = 0; r < 4; ++r) { rb[r * 64] = f4(QA + 4 * r); rb[(4 + r) * 64] = f4(QB + 4 * r); }
-    cbs[((long long)w * g.TW + t) * 64 + l] = f4(cb);
+  // tile of this workgroup: block w, rows 8*lb .. 8*lb+7, steps T0 .. T0+LP_STEPS-1
+  const int tiles_t = (g.TA + LP_STEPS - 1) / LP_STEPS;
+  if ((int)blockIdx.x >= g.W * 8 * tiles_t) return;
+  const int tb = (int)blockIdx.x % tiles_t, rest = (int)blockIdx.x / tiles_t;
+  const int lb = rest & 7, w = rest >> 3;
+  const int T0 = tb * LP_STEPS, L0 = lb * 8;
+  // box origin: cell (jr, kk) = (T0 - L0 - 8, 64w + L0 - 1) is LDS cell 0
+  const int jr0 = T0 - L0 - 8, kk0 = 64 * w + L0 - 1;
+  extern __shared__ __attribute__((aligned(16))) double lp_raw[];
+  LUPrepLds& S = *reinterpret_cast<LUPrepLds*>(lp_raw);
+  for (int n = threadIdx.x; n < LP_N; n += nthr) {
+    const int jr = jr0 + n % LP_RJ, kk = kk0 + n / LP_RJ;
+    if (jr < -1 || jr > g.ni || kk < -1 || kk > g.nk) continue;
+    const long long i = cell(M, js + jr, ks + kk);
+    const CellLoad c = load_cell(M, i);
+    double sv[4];
+    for (int q = 0; q < 4; ++q) sv[q] = M.s[q][i];
+    const LUCellF64 L = lu_of(M.c, c, epsv);
+    S.uu[n] = L.uu; S.vv[n] = L.vv; S.uv2[n] = L.uv2; S.ge[n] = L.ge; S.qx[n] = L.qx; S.qy[n] = L.qy;
+    S.cx[n] = L.cx; S.cy[n] = L.cy; S.vn[n] = L.vn; S.d[n] = L.d; S.qqx[n] = L.qqx_s; S.qqy[n] = L.qqy_s;
+    S.xx[n] = c.xx; S.xy[n] = c.xy; S.yx[n] = c.yx; S.yy[n] = c.yy; S.ts[n] = c.ts;
+    for (int q = 0; q < 4; ++q) S.sh[q][n] = sv[q] * c.ts * L.d;
   }
+  __syncthreads();
+  // one thread per (step, row) entry: 8 consecutive lanes are 8 consecutive
+  // rows at one step, so each vector of a step row is stored as whole
+  // 128-byte lines.  Entries outside the grid (rows past nk, steps before a
+  // row's first cell or after its last) keep zero records: the workspace is
+  // zero-filled when the case is created and nothing else writes there.
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l = L0 + (lane & 7), t = T0 + 8 * wave + (lane >> 3);
+  const int kk = 64 * w + l, jr = t - l;
+  if (t >= g.TA || kk >= g.nk || jr < 0 || jr >= g.ni) return;
+  F4* rf = recf + ((long long)(w * g.TA + t) * 9) * 64 + l;
+  F4* rb = recb + ((long long)(w * g.TA + t) * 8) * 64 + l;
+  auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
+  const int j = js + jr, k = ks + kk;
+  const int n0 = (kk - kk0) * LP_RJ + (jr - jr0);
+  const double di = S.d[n0], f = (S.ts[n0] * 0.5) * di;
+  const double gm1 = M.c.gm1;
+  double u[4], cb[4] = {0.0, 0.0, 0.0, 0.0}, PA[16], PB[16], QA[16], QB[16];
+  for (int q = 0; q < 4; ++q) u[q] = S.sh[q][n0];
+  lu32_term(S, n0 - 1, j == js, 1, +1, f, eps2, gm1, PA, u);
+  lu32_term(S, n0 - LP_RJ, k == ks, 0, +1, f, eps2, gm1, PB, u);
+  lu32_term(S, n0 + 1, j == je, 1, -1, -f, eps2, gm1, QA, cb);
+  lu32_term(S, n0 + LP_RJ, k == ke, 0, -1, -f, eps2, gm1, QB, cb);
+  rf[0] = f4(u);
+  for (int r = 0; r < 4; ++r) { rf[(1 + r) * 64] = f4(PA + 4 * r); rf[(5 + r) * 64] = f4(PB + 4 * r); }
+  for (int r = 0; r < 4; ++r) { rb[r * 64] = f4(QA + 4 * r); rb[(4 + r) * 64] = f4(QB + 4 * r); }
+  cbs[((long long)w * g.TW + t) * 64 + l] = f4(cb);
 }
 
 // Bounded waits.  Every wait of the FP32 sweeps gives up after LU32_WAIT
@@ -2440,8 +2471,8 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
     mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
     int nk = hM[i].ke - hM[i].ks + 1, ni = hM[i].je - hM[i].js + 1;
     const LUGeom gg = lu_geom(ni, nk, hM[i].lu_rl);
-    const long long nent = (long long)gg.W * gg.TA * 64;   // prep entries
-    nc = nent > nc ? nent : nc;
+    const long long nprep = (long long)gg.W * 8 * ((gg.TA + LP_STEPS - 1) / LP_STEPS);   // prep tiles
+    nc = nprep > nc ? nprep : nc;
     wmax = gg.W > wmax ? gg.W : wmax;
   }
   const size_t lds = (size_t)LU32_S * (LU32_NV + 1) * 1024;
@@ -2451,9 +2482,10 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, LU32_S, LU32_NP, LU32_PD>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)k_lu_prep32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LP_LDS);
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_lu_prep32, dim3(nblk(nc, 256), nmi), dim3(256), 0, st, dM);
+  hipLaunchKernelGGL(k_lu_prep32, dim3((unsigned)nc, nmi), dim3(64 * LP_WAVES), LP_LDS, st, dM);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_PREP, st);
