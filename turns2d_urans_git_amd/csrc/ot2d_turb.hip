@@ -765,21 +765,25 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
   }
 }
 
-// Line solves for many lines at once: one lane per line, 64 lines per
-// block, the same reference arithmetic.
-// k_sa_lines_j: lines along j (one per k row).  The block's 64 rows are
-// streamed through LDS in tiles of SA_TJ columns: the tile is loaded with
-// row-contiguous (coalesced) reads while the previous tile is solved, each
-// lane runs its row's recurrence across the tile from LDS, and the results
-// go back through LDS with row-contiguous writes.  The forward pass leaves
-// c' and s' in place (as lsolvej does), the backward pass writes
-// s = (s1 - s2) * bjmat.
-constexpr int SA_TJ = 16;
+// Line solves for many lines at once: one lane per line, the same
+// reference arithmetic.
+// k_sa_lines_j: lines along j (one per k row), 64 rows per workgroup.  Wave
+// 0 runs the 64 recurrences; waves 1..SAJ_MW move the coefficient tiles
+// (SA_TJ consecutive cells of each of the 64 rows, read and written as
+// row-contiguous segments) between global memory and two LDS slots.  While
+// wave 0 works on tile x in one slot, the movers write tile x-1's results
+// back from the other slot and refill it with tile x+1, whose loads they
+// issued SAJ_D tiles earlier, so the recurrence does not wait on memory.
+// The forward pass leaves c' and s' in place (as lsolvej does), the
+// backward pass writes s = (s1 - s2) * bjmat.
+constexpr int SA_TJ = 16, SAJ_MW = 4, SAJ_D = 4;
+constexpr int SAJ_THREADS = 64 * (1 + SAJ_MW);
+constexpr int SAJ_PER = 64 * SA_TJ / (64 * SAJ_MW);   // elements per mover lane and array
 // total lines (both directions, all instances) above which the solves run one lane per line
 constexpr long long SA_LANE_LINES = 4096;
 
-__global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ Ms) {
-  __shared__ double ta[64][SA_TJ + 1], tb[64][SA_TJ + 1], tc[64][SA_TJ + 1], ts[64][SA_TJ + 1];
+__global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __restrict__ Ms) {
+  __shared__ double ta[2][64][SA_TJ + 1], tb[2][64][SA_TJ + 1], tc[2][64][SA_TJ + 1], ts[2][64][SA_TJ + 1];
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
@@ -788,127 +792,189 @@ __global__ __launch_bounds__(64) void k_sa_lines_j(const DevMesh* __restrict__ M
   const int nrow = ke - k0 + 1 < 64 ? ke - k0 + 1 : 64;
   const int n = je - js + 1;
   const int ntile = (n + SA_TJ - 1) / SA_TJ;
-  const int l = threadIdx.x;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, ml = (int)threadIdx.x - 64;
   gdouble* __restrict__ sS = gptr(M.sa_sj);
   gdouble* __restrict__ sC = gptr(M.sa_cj);
   const gdouble* __restrict__ sA = gptr(M.sa_aj);
   const gdouble* __restrict__ sB = gptr(M.sa_bb);
-  // element w of a tile: row w / SA_TJ, column w % SA_TJ (16 lanes read one
-  // row's 16 consecutive cells)
-  constexpr int PER = 64 * SA_TJ / 64;
-  auto gidx = [&](int t, int w, int& r, int& cc, bool& ok) {
-    r = w / SA_TJ; cc = w % SA_TJ;
-    int j = js + t * SA_TJ + cc;
+  // element p of mover lane ml in a tile: e = ml + 256 p, row e / SA_TJ,
+  // column e % SA_TJ; the global index is clamped into the grid, so every
+  // load is unconditional and no loaded value merges across a branch
+  auto gidx = [&](int t, int p, int& r, int& cc, bool& ok) {
+    const int e = ml + 64 * SAJ_MW * p;
+    r = e / SA_TJ; cc = e % SA_TJ;
+    const int j = js + t * SA_TJ + cc;
     ok = r < nrow && j <= je;
     return cell(M, j < je ? j : je, k0 + (r < nrow ? r : 0));
   };
-  double ra[PER], rb[PER], rc[PER], rs[PER];
-  auto load = [&](int t) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      int r, cc; bool ok;
-      long long i = gidx(t, l + 64 * p, r, cc, ok);
-      // the load itself is unconditional (the index is clamped into the
-      // grid), so no loaded value merges across a branch
-      const double va = sA[i], vb = sB[i], vc = sC[i], vs = sS[i];
-      ra[p] = ok ? va : 0.0; rb[p] = ok ? vb : 1.0; rc[p] = ok ? vc : 0.0; rs[p] = ok ? vs : 0.0;
-    }
-  };
+  double ra[SAJ_D][SAJ_PER], rb[SAJ_D][SAJ_PER], rc[SAJ_D][SAJ_PER], rs[SAJ_D][SAJ_PER];
+
   // ---- forward elimination
-  double cp = 0.0, sp = 0.0;
-  load(0);
-  for (int t = 0; t < ntile; ++t) {
-    __syncthreads();
+  auto fetch_f = [&](int t, double (&a)[SAJ_PER], double (&b)[SAJ_PER], double (&c)[SAJ_PER], double (&s)[SAJ_PER]) {
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      int w = l + 64 * p, r = w / SA_TJ, cc = w % SA_TJ;
-      ta[r][cc] = ra[p]; tb[r][cc] = rb[p]; tc[r][cc] = rc[p]; ts[r][cc] = rs[p];
-    }
-    if (t + 1 < ntile) load(t + 1);
-    __syncthreads();
-    if (l < nrow) {
-      // a full tile is one straight-line unrolled block, so its LDS reads
-      // run ahead of the recurrence; the line's first cell and a partial
-      // last tile take separate paths
-      const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
-      auto elem = [&](int cc) {
-        double ai = ta[l][cc];
-        double bb = 1. / (tb[l][cc] - ai * cp);
-        sp = (ts[l][cc] - ai * sp) * bb;
-        cp = tc[l][cc] * bb;
-        ts[l][cc] = sp; tc[l][cc] = cp;
-      };
-      if (c0 == 0) {
-        double bb = 1. / tb[l][0];
-        sp = ts[l][0] * bb; cp = tc[l][0] * bb;
-        ts[l][0] = sp; tc[l][0] = cp;
-#pragma unroll
-        for (int cc = 1; cc < SA_TJ; ++cc)
-          if (cc < cn) elem(cc);
-      } else if (cn == SA_TJ) {
-#pragma unroll
-        for (int cc = 0; cc < SA_TJ; ++cc) elem(cc);
-      } else {
-#pragma unroll
-        for (int cc = 0; cc < SA_TJ; ++cc)
-          if (cc < cn) elem(cc);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
+    for (int p = 0; p < SAJ_PER; ++p) {
       int r, cc; bool ok;
-      long long i = gidx(t, l + 64 * p, r, cc, ok);
-      if (ok) { sS[i] = ts[r][cc]; sC[i] = tc[r][cc]; }
-    }
-  }
-  // ---- back substitution, tiles in reverse; s = (0 + sn - 0) * bjmat
-  double sn = 0.0;
-  auto load2 = [&](int t) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      int r, cc; bool ok;
-      long long i = gidx(t, l + 64 * p, r, cc, ok);
-      const double vb = sB[i], vc = sC[i], vs = sS[i];
-      rb[p] = ok ? vb : 1.0; rc[p] = ok ? vc : 0.0; rs[p] = ok ? vs : 0.0;
+      const long long i = gidx(t, p, r, cc, ok);
+      const double va = sA[i], vb = sB[i], vc = sC[i], vs = sS[i];
+      a[p] = ok ? va : 0.0; b[p] = ok ? vb : 1.0; c[p] = ok ? vc : 0.0; s[p] = ok ? vs : 0.0;
     }
   };
-  load2(ntile - 1);
-  for (int t = ntile - 1; t >= 0; --t) {
-    __syncthreads();
+  auto put_f = [&](int slot, const double (&a)[SAJ_PER], const double (&b)[SAJ_PER], const double (&c)[SAJ_PER],
+                   const double (&s)[SAJ_PER]) {
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      int w = l + 64 * p, r = w / SA_TJ, cc = w % SA_TJ;
-      tb[r][cc] = rb[p]; tc[r][cc] = rc[p]; ts[r][cc] = rs[p];
+    for (int p = 0; p < SAJ_PER; ++p) {
+      const int e = ml + 64 * SAJ_MW * p, r = e / SA_TJ, cc = e % SA_TJ;
+      ta[slot][r][cc] = a[p]; tb[slot][r][cc] = b[p]; tc[slot][r][cc] = c[p]; ts[slot][r][cc] = s[p];
     }
-    if (t > 0) load2(t - 1);
-    __syncthreads();
-    if (l < nrow) {
-      const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
-      auto elem = [&](int cc) {
-        sn = ts[l][cc] - tc[l][cc] * sn;
-        ts[l][cc] = (0. + sn - 0.) * tb[l][cc];
-      };
-      if (t == ntile - 1) {
-        // the line's last cell starts the recurrence
-        sn = ts[l][cn - 1];
-        ts[l][cn - 1] = (0. + sn - 0.) * tb[l][cn - 1];
+  };
+  auto back_f = [&](int t, int slot) {   // c' and s' of tile t to global memory
 #pragma unroll
-        for (int cc = SA_TJ - 2; cc >= 0; --cc)
-          if (cc < cn - 1) elem(cc);
-      } else {
-#pragma unroll
-        for (int cc = SA_TJ - 1; cc >= 0; --cc) elem(cc);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
+    for (int p = 0; p < SAJ_PER; ++p) {
       int r, cc; bool ok;
-      long long i = gidx(t, l + 64 * p, r, cc, ok);
-      if (ok) sS[i] = ts[r][cc];
+      const long long i = gidx(t, p, r, cc, ok);
+      if (ok) { sS[i] = ts[slot][r][cc]; sC[i] = tc[slot][r][cc]; }
+    }
+  };
+  double cp = 0.0, sp = 0.0;
+  auto solve_f = [&](int t, int slot) {
+    if (l >= nrow) return;
+    // a full tile is one straight-line unrolled block, so its LDS reads run
+    // ahead of the recurrence; the line's first cell and a partial last
+    // tile take separate paths
+    const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
+    double (*A)[SA_TJ + 1] = ta[slot];
+    double (*B)[SA_TJ + 1] = tb[slot];
+    double (*Cc)[SA_TJ + 1] = tc[slot];
+    double (*S)[SA_TJ + 1] = ts[slot];
+    auto elem = [&](int cc) {
+      double ai = A[l][cc];
+      double bb = 1. / (B[l][cc] - ai * cp);
+      sp = (S[l][cc] - ai * sp) * bb;
+      cp = Cc[l][cc] * bb;
+      S[l][cc] = sp; Cc[l][cc] = cp;
+    };
+    if (c0 == 0) {
+      double bb = 1. / B[l][0];
+      sp = S[l][0] * bb; cp = Cc[l][0] * bb;
+      S[l][0] = sp; Cc[l][0] = cp;
+#pragma unroll
+      for (int cc = 1; cc < SA_TJ; ++cc)
+        if (cc < cn) elem(cc);
+    } else if (cn == SA_TJ) {
+#pragma unroll
+      for (int cc = 0; cc < SA_TJ; ++cc) elem(cc);
+    } else {
+#pragma unroll
+      for (int cc = 0; cc < SA_TJ; ++cc)
+        if (cc < cn) elem(cc);
+    }
+  };
+  if (wave > 0) {
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d)
+      if (d < ntile) fetch_f(d, ra[d], rb[d], rc[d], rs[d]);
+    put_f(0, ra[0], rb[0], rc[0], rs[0]);
+    if (SAJ_D < ntile) fetch_f(SAJ_D, ra[0], rb[0], rc[0], rs[0]);
+  }
+  __syncthreads();
+  for (int t0 = 0; t0 < ntile; t0 += SAJ_D) {
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d) {
+      const int t = t0 + d;
+      if (t >= ntile) break;
+      const int d1 = (d + 1) % SAJ_D;
+      if (wave == 0) {
+        solve_f(t, t & 1);
+      } else {
+        if (t >= 1) back_f(t - 1, (t - 1) & 1);
+        if (t + 1 < ntile) {
+          put_f((t + 1) & 1, ra[d1], rb[d1], rc[d1], rs[d1]);
+          if (t + 1 + SAJ_D < ntile) fetch_f(t + 1 + SAJ_D, ra[d1], rb[d1], rc[d1], rs[d1]);
+        }
+      }
+      __syncthreads();
     }
   }
+  if (wave > 0) back_f(ntile - 1, (ntile - 1) & 1);
+
+  // ---- back substitution, tiles in reverse (iteration x takes tile
+  // ntile-1-x); s = (0 + sn - 0) * bjmat.  Each mover lane reloads the
+  // elements it wrote itself, so program order makes them visible.
+  auto fetch_b = [&](int t, double (&b)[SAJ_PER], double (&c)[SAJ_PER], double (&s)[SAJ_PER]) {
+#pragma unroll
+    for (int p = 0; p < SAJ_PER; ++p) {
+      int r, cc; bool ok;
+      const long long i = gidx(t, p, r, cc, ok);
+      const double vb = sB[i], vc = sC[i], vs = sS[i];
+      b[p] = ok ? vb : 1.0; c[p] = ok ? vc : 0.0; s[p] = ok ? vs : 0.0;
+    }
+  };
+  auto put_b = [&](int slot, const double (&b)[SAJ_PER], const double (&c)[SAJ_PER], const double (&s)[SAJ_PER]) {
+#pragma unroll
+    for (int p = 0; p < SAJ_PER; ++p) {
+      const int e = ml + 64 * SAJ_MW * p, r = e / SA_TJ, cc = e % SA_TJ;
+      tb[slot][r][cc] = b[p]; tc[slot][r][cc] = c[p]; ts[slot][r][cc] = s[p];
+    }
+  };
+  auto back_b = [&](int t, int slot) {
+#pragma unroll
+    for (int p = 0; p < SAJ_PER; ++p) {
+      int r, cc; bool ok;
+      const long long i = gidx(t, p, r, cc, ok);
+      if (ok) sS[i] = ts[slot][r][cc];
+    }
+  };
+  double sn = 0.0;
+  auto solve_b = [&](int t, int slot) {
+    if (l >= nrow) return;
+    const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
+    double (*B)[SA_TJ + 1] = tb[slot];
+    double (*Cc)[SA_TJ + 1] = tc[slot];
+    double (*S)[SA_TJ + 1] = ts[slot];
+    auto elem = [&](int cc) {
+      sn = S[l][cc] - Cc[l][cc] * sn;
+      S[l][cc] = (0. + sn - 0.) * B[l][cc];
+    };
+    if (t == ntile - 1) {
+      // the line's last cell starts the recurrence
+      sn = S[l][cn - 1];
+      S[l][cn - 1] = (0. + sn - 0.) * B[l][cn - 1];
+#pragma unroll
+      for (int cc = SA_TJ - 2; cc >= 0; --cc)
+        if (cc < cn - 1) elem(cc);
+    } else {
+#pragma unroll
+      for (int cc = SA_TJ - 1; cc >= 0; --cc) elem(cc);
+    }
+  };
+  __syncthreads();
+  if (wave > 0) {
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d)
+      if (d < ntile) fetch_b(ntile - 1 - d, rb[d], rc[d], rs[d]);
+    put_b(0, rb[0], rc[0], rs[0]);
+    if (SAJ_D < ntile) fetch_b(ntile - 1 - SAJ_D, rb[0], rc[0], rs[0]);
+  }
+  __syncthreads();
+  for (int x0 = 0; x0 < ntile; x0 += SAJ_D) {
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d) {
+      const int x = x0 + d;
+      if (x >= ntile) break;
+      const int d1 = (d + 1) % SAJ_D;
+      if (wave == 0) {
+        solve_b(ntile - 1 - x, x & 1);
+      } else {
+        if (x >= 1) back_b(ntile - x, (x - 1) & 1);
+        if (x + 1 < ntile) {
+          put_b((x + 1) & 1, rb[d1], rc[d1], rs[d1]);
+          if (x + 1 + SAJ_D < ntile) fetch_b(ntile - 2 - x - SAJ_D, rb[d1], rc[d1], rs[d1]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (wave > 0) back_b(0, (ntile - 1) & 1);
 }
 
 // k_sa_lines<JL>: one lane per line, each lane walking its line with the
@@ -1135,7 +1201,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   long long lines = 0;
   for (int i = 0; i < nmi; ++i) lines += (hM[i].ke - hM[i].ks + 1) + (hM[i].je - hM[i].js + 1);
   const bool per_lane = lines > SA_LANE_LINES;
-  if (per_lane) hipLaunchKernelGGL(k_sa_lines_j, dim3((mk_ + 63) / 64, nmi), dim3(64), 0, st, dM);
+  if (per_lane) hipLaunchKernelGGL(k_sa_lines_j, dim3((mk_ + 63) / 64, nmi), dim3(SAJ_THREADS), 0, st, dM);
   else hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_J, st);
