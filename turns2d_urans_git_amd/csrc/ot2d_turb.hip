@@ -776,7 +776,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
 // issued SAJ_D tiles earlier, so the recurrence does not wait on memory.
 // The forward pass leaves c' and s' in place (as lsolvej does), the
 // backward pass writes s = (s1 - s2) * bjmat.
-constexpr int SA_TJ = 16, SAJ_MW = 4, SAJ_D = 4;
+constexpr int SA_TJ = 32, SAJ_MW = 4, SAJ_D = 2;
 constexpr int SAJ_THREADS = 64 * (1 + SAJ_MW);
 constexpr int SAJ_PER = 64 * SA_TJ / (64 * SAJ_MW);   // elements per mover lane and array
 // total lines (both directions, all instances) above which the solves run one lane per line
@@ -815,10 +815,12 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     for (int p = 0; p < SAJ_PER; ++p) {
       int r, cc; bool ok;
       const long long i = gidx(t, p, r, cc, ok);
-      const double va = sA[i], vb = sB[i], vc = sC[i], vs = sS[i];
-      a[p] = ok ? va : 0.0; b[p] = ok ? vb : 1.0; c[p] = ok ? vc : 0.0; s[p] = ok ? vs : 0.0;
+      (void)ok;
+      a[p] = sA[i]; b[p] = sB[i]; c[p] = sC[i]; s[p] = sS[i];
     }
   };
+  // elements outside the grid (rows past the last, columns past je in the
+  // last tile) hold copies of clamped cells: the recurrences never read them
   auto put_f = [&](int slot, const double (&a)[SAJ_PER], const double (&b)[SAJ_PER], const double (&c)[SAJ_PER],
                    const double (&s)[SAJ_PER]) {
 #pragma unroll
@@ -869,33 +871,48 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
         if (cc < cn) elem(cc);
     }
   };
-  if (wave > 0) {
-#pragma unroll
-    for (int d = 0; d < SAJ_D; ++d)
-      if (d < ntile) fetch_f(d, ra[d], rb[d], rc[d], rs[d]);
-    put_f(0, ra[0], rb[0], rc[0], rs[0]);
-    if (SAJ_D < ntile) fetch_f(SAJ_D, ra[0], rb[0], rc[0], rs[0]);
-  }
-  __syncthreads();
-  for (int t0 = 0; t0 < ntile; t0 += SAJ_D) {
-#pragma unroll
-    for (int d = 0; d < SAJ_D; ++d) {
-      const int t = t0 + d;
-      if (t >= ntile) break;
-      const int d1 = (d + 1) % SAJ_D;
-      if (wave == 0) {
-        solve_f(t, t & 1);
-      } else {
-        if (t >= 1) back_f(t - 1, (t - 1) & 1);
-        if (t + 1 < ntile) {
-          put_f((t + 1) & 1, ra[d1], rb[d1], rc[d1], rs[d1]);
-          if (t + 1 + SAJ_D < ntile) fetch_f(t + 1 + SAJ_D, ra[d1], rb[d1], rc[d1], rs[d1]);
-        }
-      }
+  // Tile x sits in register set x % SAJ_D.  Every fetch is unconditional
+  // (tile indices past the line are clamped to its last tile) and the
+  // movers' main loop runs whole groups of SAJ_D tiles in code of their own,
+  // so no loaded value merges across a branch and each load is waited for
+  // only where it is stored to LDS.  Solver and movers pass the same
+  // sequence of barriers: one after each pass's prologue, one per tile, and
+  // one between the passes.
+  auto clampt = [&](int t) { return t < ntile ? t : ntile - 1; };
+  auto step_f = [&](int t, double (&a)[SAJ_PER], double (&b)[SAJ_PER], double (&c)[SAJ_PER],
+                    double (&s)[SAJ_PER]) {
+    if (t >= 1) back_f(t - 1, (t - 1) & 1);
+    if (t + 1 < ntile) put_f((t + 1) & 1, a, b, c, s);
+    fetch_f(clampt(t + 1 + SAJ_D), a, b, c, s);
+    __syncthreads();
+  };
+  if (wave == 0) {
+    __syncthreads();
+    for (int t = 0; t < ntile; ++t) {
+      solve_f(t, t & 1);
       __syncthreads();
     }
+  } else {
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d) fetch_f(clampt(d), ra[d], rb[d], rc[d], rs[d]);
+    put_f(0, ra[0], rb[0], rc[0], rs[0]);
+    fetch_f(clampt(SAJ_D), ra[0], rb[0], rc[0], rs[0]);
+    __syncthreads();
+    int t0 = 0;
+    for (; t0 + SAJ_D <= ntile; t0 += SAJ_D) {
+#pragma unroll
+      for (int d = 0; d < SAJ_D; ++d) {
+        const int d1 = (d + 1) % SAJ_D;
+        step_f(t0 + d, ra[d1], rb[d1], rc[d1], rs[d1]);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d) {
+      const int d1 = (d + 1) % SAJ_D;
+      if (t0 + d < ntile) step_f(t0 + d, ra[d1], rb[d1], rc[d1], rs[d1]);
+    }
+    back_f(ntile - 1, (ntile - 1) & 1);
   }
-  if (wave > 0) back_f(ntile - 1, (ntile - 1) & 1);
 
   // ---- back substitution, tiles in reverse (iteration x takes tile
   // ntile-1-x); s = (0 + sn - 0) * bjmat.  Each mover lane reloads the
@@ -905,8 +922,8 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     for (int p = 0; p < SAJ_PER; ++p) {
       int r, cc; bool ok;
       const long long i = gidx(t, p, r, cc, ok);
-      const double vb = sB[i], vc = sC[i], vs = sS[i];
-      b[p] = ok ? vb : 1.0; c[p] = ok ? vc : 0.0; s[p] = ok ? vs : 0.0;
+      (void)ok;
+      b[p] = sB[i]; c[p] = sC[i]; s[p] = sS[i];
     }
   };
   auto put_b = [&](int slot, const double (&b)[SAJ_PER], const double (&c)[SAJ_PER], const double (&s)[SAJ_PER]) {
@@ -948,33 +965,42 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     }
   };
   __syncthreads();
-  if (wave > 0) {
-#pragma unroll
-    for (int d = 0; d < SAJ_D; ++d)
-      if (d < ntile) fetch_b(ntile - 1 - d, rb[d], rc[d], rs[d]);
-    put_b(0, rb[0], rc[0], rs[0]);
-    if (SAJ_D < ntile) fetch_b(ntile - 1 - SAJ_D, rb[0], rc[0], rs[0]);
-  }
-  __syncthreads();
-  for (int x0 = 0; x0 < ntile; x0 += SAJ_D) {
-#pragma unroll
-    for (int d = 0; d < SAJ_D; ++d) {
-      const int x = x0 + d;
-      if (x >= ntile) break;
-      const int d1 = (d + 1) % SAJ_D;
-      if (wave == 0) {
-        solve_b(ntile - 1 - x, x & 1);
-      } else {
-        if (x >= 1) back_b(ntile - x, (x - 1) & 1);
-        if (x + 1 < ntile) {
-          put_b((x + 1) & 1, rb[d1], rc[d1], rs[d1]);
-          if (x + 1 + SAJ_D < ntile) fetch_b(ntile - 2 - x - SAJ_D, rb[d1], rc[d1], rs[d1]);
-        }
-      }
+  // iteration x takes tile ntile-1-x; register set x % SAJ_D, tiles past
+  // the line's start clamped to tile 0
+  auto rtile = [&](int x) { return x < ntile ? ntile - 1 - x : 0; };
+  auto step_b = [&](int x, double (&b)[SAJ_PER], double (&c)[SAJ_PER], double (&s)[SAJ_PER]) {
+    if (x >= 1) back_b(ntile - x, (x - 1) & 1);
+    if (x + 1 < ntile) put_b((x + 1) & 1, b, c, s);
+    fetch_b(rtile(x + 1 + SAJ_D), b, c, s);
+    __syncthreads();
+  };
+  if (wave == 0) {
+    __syncthreads();
+    for (int x = 0; x < ntile; ++x) {
+      solve_b(ntile - 1 - x, x & 1);
       __syncthreads();
     }
+  } else {
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d) fetch_b(rtile(d), rb[d], rc[d], rs[d]);
+    put_b(0, rb[0], rc[0], rs[0]);
+    fetch_b(rtile(SAJ_D), rb[0], rc[0], rs[0]);
+    __syncthreads();
+    int x0 = 0;
+    for (; x0 + SAJ_D <= ntile; x0 += SAJ_D) {
+#pragma unroll
+      for (int d = 0; d < SAJ_D; ++d) {
+        const int d1 = (d + 1) % SAJ_D;
+        step_b(x0 + d, rb[d1], rc[d1], rs[d1]);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < SAJ_D; ++d) {
+      const int d1 = (d + 1) % SAJ_D;
+      if (x0 + d < ntile) step_b(x0 + d, rb[d1], rc[d1], rs[d1]);
+    }
+    back_b(0, (ntile - 1) & 1);
   }
-  if (wave > 0) back_b(0, (ntile - 1) & 1);
 }
 
 // k_sa_lines<JL>: one lane per line, each lane walking its line with the
