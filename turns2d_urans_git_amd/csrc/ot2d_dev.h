@@ -118,6 +118,86 @@ __device__ __forceinline__ void stage_line4(const double* a, const double* b, co
   }
 }
 
+// Thomas recurrence of one tridiagonal line held in LDS, run by one thread
+// with the reference's arithmetic (lsolvej / lsolvek and their two-equation
+// forms):
+//   forward   bb = 1/(b - a c'),  s' = (s - a s') bb,  c' = c bb
+//   backward  s' = s' - c' s'_next
+// s' and c' go to os and oc (os may be ls itself).  The recurrence runs in
+// chunks of THOMAS_CH cells whose LDS reads are issued a chunk ahead into
+// registers, so the dependent chain does not wait on LDS.  Chunk reads past
+// the line's end are clamped to its last cell and never used.
+constexpr int THOMAS_CH = 8;
+__device__ __forceinline__ void thomas_lds(const double* la, const double* lb, const double* lc, const double* ls,
+                                           double* os, double* oc, int n) {
+  constexpr int CH = THOMAS_CH;
+  double bb = 1. / lb[0];
+  double sp = ls[0] * bb, cp = lc[0] * bb;
+  os[0] = sp; oc[0] = cp;
+  double pa[CH], pb[CH], pc[CH], ps[CH], qa[CH], qb[CH], qc[CH], qs[CH];
+  auto ld = [&](double (&xa)[CH], double (&xb)[CH], double (&xc)[CH], double (&xs)[CH], int t) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int i = t + u < n ? t + u : n - 1;
+      xa[u] = la[i]; xb[u] = lb[i]; xc[u] = lc[i]; xs[u] = ls[i];
+    }
+  };
+  auto run = [&](const double (&xa)[CH], const double (&xb)[CH], const double (&xc)[CH], const double (&xs)[CH],
+                 int t, int cnt) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (u < cnt) {
+        const double ai = xa[u];
+        bb = 1. / (xb[u] - ai * cp);
+        sp = (xs[u] - ai * sp) * bb;
+        cp = xc[u] * bb;
+        os[t + u] = sp; oc[t + u] = cp;
+      }
+    }
+  };
+  int t = 1;
+  ld(pa, pb, pc, ps, t);
+  for (; t + 2 * CH <= n; t += 2 * CH) {
+    ld(qa, qb, qc, qs, t + CH);
+    run(pa, pb, pc, ps, t, CH);
+    ld(pa, pb, pc, ps, t + 2 * CH);
+    run(qa, qb, qc, qs, t + CH, CH);
+  }
+  ld(qa, qb, qc, qs, t + CH);
+  run(pa, pb, pc, ps, t, n - t);
+  run(qa, qb, qc, qs, t + CH, n - t - CH);
+  // back substitution over cells n-2 .. 0, chunks of cells t-CH+1 .. t
+  double sn = sp;
+  double ys[CH], yc[CH], zs[CH], zc[CH];
+  auto ldb = [&](double (&xs)[CH], double (&xc)[CH], int t) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int i = t - u >= 0 ? t - u : 0;
+      xs[u] = os[i]; xc[u] = oc[i];
+    }
+  };
+  auto runb = [&](const double (&xs)[CH], const double (&xc)[CH], int t, int cnt) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (u < cnt) {
+        sn = xs[u] - xc[u] * sn;
+        os[t - u] = sn;
+      }
+    }
+  };
+  int b = n - 2;
+  ldb(ys, yc, b);
+  for (; b - 2 * CH + 1 >= 0; b -= 2 * CH) {
+    ldb(zs, zc, b - CH);
+    runb(ys, yc, b, CH);
+    ldb(ys, yc, b - 2 * CH);
+    runb(zs, zc, b - CH, CH);
+  }
+  ldb(zs, zc, b - CH);
+  runb(ys, yc, b, b + 1);
+  runb(zs, zc, b - CH, b - CH + 1);
+}
+
 constexpr int RES_TJ = 32;
 constexpr int RES_TK = 8;
 constexpr int RES_THREADS = RES_TJ * RES_TK;

@@ -391,21 +391,7 @@ __global__ __launch_bounds__(256) void k_sst_rhs(const DevMesh* __restrict__ Ms)
 __device__ __forceinline__ void kw_line_solve(const double* la, const double* lb, const double* lc, double* ls,
                                               double* lcp, int n) {
   if (threadIdx.x != 0) return;
-  double bb = 1. / lb[0];
-  double sp = ls[0] * bb, cp = lc[0] * bb;
-  ls[0] = sp; lcp[0] = cp;
-  for (int t = 1; t < n; ++t) {
-    double ai = la[t];
-    bb = 1. / (lb[t] - ai * cp);
-    sp = (ls[t] - ai * sp) * bb;
-    cp = lc[t] * bb;
-    ls[t] = sp; lcp[t] = cp;
-  }
-  double sn = sp;
-  for (int t = n - 2; t >= 0; --t) {
-    sn = ls[t] - lcp[t] * sn;
-    ls[t] = sn;
-  }
+  thomas_lds(la, lb, lc, ls, ls, lcp, n);
 }
 
 // lsolvej2 for k row blockIdx.x / 2, equation blockIdx.x % 2, then

@@ -668,8 +668,9 @@ __global__ __launch_bounds__(256) void k_sa_rhs(const DevMesh* __restrict__ Ms) 
 // The two scalar tridiagonal solves of vmu_sa are sequential recurrences
 // along each grid line.  One wave owns one line: all lanes stage the line's
 // coefficients in LDS (stage_line4: every load in flight at once, coalesced
-// along j, gathered along k), lane 0 runs the reference's Thomas recurrence out of LDS (so the
-// dependent chain never waits on memory), and all lanes write the results.
+// along j, gathered along k), lane 0 runs the reference's Thomas recurrence
+// out of LDS (thomas_lds: its reads run a chunk ahead of the dependent
+// chain), and all lanes write the results.
 // The arithmetic is the reference's, operation for operation.
 
 // lsolvej (src/samodel2d.f:1306-1344) and s1, sjmat*bjmat of vmu_sa
@@ -690,25 +691,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ M
   const double* __restrict__ b = M.sa_bb;
   stage_line4(a, b, c, s, i0, 1, n, la, lb, lc, ls);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double bb = 1. / lb[0];
-    double sp = ls[0] * bb, cp = lc[0] * bb;
-    lsp[0] = sp; lcp[0] = cp;
-#pragma unroll 4
-    for (int t = 1; t < n; ++t) {
-      double ai = la[t];
-      bb = 1. / (lb[t] - ai * cp);
-      sp = (ls[t] - ai * sp) * bb;
-      cp = lc[t] * bb;
-      lsp[t] = sp; lcp[t] = cp;
-    }
-    double sn = sp;
-#pragma unroll 4
-    for (int t = n - 2; t >= 0; --t) {
-      sn = lsp[t] - lcp[t] * sn;
-      lsp[t] = sn;
-    }
-  }
+  if (threadIdx.x == 0) thomas_lds(la, lb, lc, ls, lsp, lcp, n);
   __syncthreads();
   // sjmat = (s1 - s2) * bjmat with s2 = 0
   for (int t = threadIdx.x; t < n; t += 64) s[i0 + t] = (0. + lsp[t] - 0.) * lb[t];
@@ -734,25 +717,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
   const double* __restrict__ b = M.sa_bb;
   stage_line4(a, b, c, s, i0, st, n, la, lb, lc, ls);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double bb = 1. / lb[0];
-    double sp = ls[0] * bb, cp = lc[0] * bb;
-    lsp[0] = sp; lcp[0] = cp;
-#pragma unroll 4
-    for (int t = 1; t < n; ++t) {
-      double ai = la[t];
-      bb = 1. / (lb[t] - ai * cp);
-      sp = (ls[t] - ai * sp) * bb;
-      cp = lc[t] * bb;
-      lsp[t] = sp; lcp[t] = cp;
-    }
-    double sn = sp;
-#pragma unroll 4
-    for (int t = n - 2; t >= 0; --t) {
-      sn = lsp[t] - lcp[t] * sn;
-      lsp[t] = sn;
-    }
-  }
+  if (threadIdx.x == 0) thomas_lds(la, lb, lc, ls, lsp, lcp, n);
   __syncthreads();
   const double vnulim = 1.0e-20, relfac = 1.0;
   for (int t = threadIdx.x; t < n; t += 64) {
@@ -863,8 +828,30 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
       for (int cc = 1; cc < SA_TJ; ++cc)
         if (cc < cn) elem(cc);
     } else if (cn == SA_TJ) {
+      // chunks of THOMAS_CH cells, each chunk's LDS reads issued before the
+      // previous chunk's recurrence
+      constexpr int CH = THOMAS_CH;
+      double xa[2][CH], xb[2][CH], xc[2][CH], xs[2][CH];
+      auto ld = [&](int q, int c0) {
 #pragma unroll
-      for (int cc = 0; cc < SA_TJ; ++cc) elem(cc);
+        for (int u = 0; u < CH; ++u) {
+          xa[q][u] = A[l][c0 + u]; xb[q][u] = B[l][c0 + u]; xc[q][u] = Cc[l][c0 + u]; xs[q][u] = S[l][c0 + u];
+        }
+      };
+      ld(0, 0);
+#pragma unroll
+      for (int k = 0; k < SA_TJ / CH; ++k) {
+        if (k + 1 < SA_TJ / CH) ld((k + 1) & 1, (k + 1) * CH);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          const int cc = k * CH + u;
+          const double ai = xa[k & 1][u];
+          double bb = 1. / (xb[k & 1][u] - ai * cp);
+          sp = (xs[k & 1][u] - ai * sp) * bb;
+          cp = xc[k & 1][u] * bb;
+          S[l][cc] = sp; Cc[l][cc] = cp;
+        }
+      }
     } else {
 #pragma unroll
       for (int cc = 0; cc < SA_TJ; ++cc)
@@ -960,8 +947,26 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
       for (int cc = SA_TJ - 2; cc >= 0; --cc)
         if (cc < cn - 1) elem(cc);
     } else {
+      // chunks of THOMAS_CH cells from the tile's end, reads a chunk ahead
+      constexpr int CH = THOMAS_CH;
+      double xb[2][CH], xc[2][CH], xs[2][CH];
+      auto ld = [&](int q, int c1) {
 #pragma unroll
-      for (int cc = SA_TJ - 1; cc >= 0; --cc) elem(cc);
+        for (int u = 0; u < CH; ++u) {
+          xb[q][u] = B[l][c1 - u]; xc[q][u] = Cc[l][c1 - u]; xs[q][u] = S[l][c1 - u];
+        }
+      };
+      ld(0, SA_TJ - 1);
+#pragma unroll
+      for (int k = 0; k < SA_TJ / CH; ++k) {
+        if (k + 1 < SA_TJ / CH) ld((k + 1) & 1, SA_TJ - 1 - (k + 1) * CH);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          const int cc = SA_TJ - 1 - k * CH - u;
+          sn = xs[k & 1][u] - xc[k & 1][u] * sn;
+          S[l][cc] = (0. + sn - 0.) * xb[k & 1][u];
+        }
+      }
     }
   };
   __syncthreads();
