@@ -98,23 +98,29 @@ def test_retired_case_is_frozen_and_others_unaffected():
 def test_many_line_batch_equals_alone():
     """A batch with more than 4096 grid lines runs the SA line solves one lane
     per line (tiled through LDS along j, row-contiguous along k); each case
-    must still match its solo run (one wave per line) bit for bit."""
+    must still match its solo run (one wave per line) bit for bit.  The grid
+    sizes leave partial last tiles along j (j lines of 134, 100, 76 cells),
+    a line shorter than one tile (24 cells), and partial row blocks (69 and
+    65 rows: a second block of 5 and of 1 row)."""
     import turns2d_urans_git_amd as t2d
     rng = np.random.default_rng(7)
+    sizes = [(129, 49), (135, 70), (101, 33), (77, 66), (25, 17)]
     grids, params, meshes = [], [], []
-    for _ in range(24):
-        g = grid.naca4_cgrid(129, 49, m=float(rng.uniform(0, 0.05)), p=0.4, t=float(rng.uniform(0.09, 0.15)),
+    for n in range(32):
+        jm, km = sizes[n % len(sizes)]
+        g = grid.naca4_cgrid(jm, km, m=float(rng.uniform(0, 0.05)), p=0.4, t=float(rng.uniform(0.09, 0.15)),
                              ds_wall=1e-4)
         grids.append(g)
         params.append(t2d.default_params(**dict(SA, fsmach=float(rng.uniform(0.3, 0.7)),
                                                 alfa=float(rng.uniform(-2, 5))), lu_fp32=1))
-        meshes.append(abi.mesh_desc(129, 49, grid.cgrid_bc(g.jtail1)))
+        meshes.append(abi.mesh_desc(jm, km, grid.cgrid_bc(g.jtail1)))
+    assert sum((jm - 1) + (km - 1) for jm, km in (sizes[n % len(sizes)] for n in range(32))) > 4096
     b = t2d.Batch(params, meshes)
     for i, g in enumerate(grids):
         b.set_grid(i, g.xv, g.yv)
     rc, hist, done = b.iterate(15)
     assert rc == 0 and done == 15
-    for i in (0, 11, 23):
+    for i in (0, 1, 2, 3, 4, 31):
         solo = t2d.Case(params[i], [meshes[i]])
         solo.set_grid(0, grids[i].xv, grids[i].yv)
         _, h_s, _ = solo.iterate(15)
