@@ -111,38 +111,70 @@ def batch_cases(n=64, seed=BATCH_SEED):
     return out
 
 
-def converge_batch(t2d, hip, params, meshes, grids, steps, chunk, label):
-    """Advance one batch until every case has stopped (rsum <= 1e-10 on the
+def batch_groups(cells, ngroups):
+    """Cases dealt into ngroups groups of similar size (largest first,
+    round robin), each group to run as its own batch on its own stream."""
+    order = sorted(range(len(cells)), key=lambda i: -cells[i])
+    return [sorted(order[g::ngroups]) for g in range(ngroups) if order[g::ngroups]]
+
+
+def converge_batch(t2d, hip, params, meshes, grids, steps, chunk, label, groups=None):
+    """Advance the cases until every one has stopped (rsum <= 1e-10 on the
     device) or `steps` steps; returns (status, steps per case, seconds,
-    steps run, forces per case)."""
-    b = t2d.Batch(params, meshes)
-    for i, g in enumerate(grids):
-        b.set_grid(i, g.xv, g.yv)
-    b.set_convergence(1e-10)
+    steps run, forces per case).  With `groups`, each group of cases is one
+    batch on its own stream, all enqueued before any is waited for, so the
+    groups' kernels overlap on the device (a latency-bound sweep of one group
+    beside bandwidth-bound kernels of another)."""
+    groups = groups or [list(range(len(params)))]
+    bs = []
+    for grp in groups:
+        b = t2d.Batch([params[i] for i in grp], [meshes[i] for i in grp])
+        for n, i in enumerate(grp):
+            b.set_grid(n, grids[i].xv, grids[i].yv)
+        b.set_convergence(1e-10)
+        bs.append(b)
+    n = len(params)
+    st = np.zeros(n, dtype=np.int32)
+    ist = np.zeros(n, dtype=np.int32)
+    running = [True] * len(bs)
     device_sync(hip)
     t0 = time.perf_counter()
     total = 0
-    while total < steps:
+    while total < steps and any(running):
         k = min(chunk, steps - total)
-        b.iterate_async(k)
-        st, ist = b.batch_status()
+        # the groups' steps are enqueued in rounds of a few steps each, so
+        # that no stream's queue fills up while the others wait their turn
+        for r0 in range(0, k, 10):
+            for g, b in enumerate(bs):
+                if running[g]:
+                    b.iterate_async(min(10, k - r0))
+        for g, b in enumerate(bs):
+            if running[g]:
+                sg, ig = b.batch_status()
+                st[groups[g]] = sg
+                ist[groups[g]] = ig
+                running[g] = bool(np.any(sg == 0))
         total += k
         print(f"[bench] {label}: {total} steps, {int((st == abi.CONVERGED).sum())} converged, "
               f"{int(((st == 1) | (st == 2)).sum())} stopped, {time.perf_counter() - t0:.1f} s",
               file=sys.stderr, flush=True)
-        if not np.any(st == 0):
-            break
     device_sync(hip)
     secs = time.perf_counter() - t0
-    st, ist = b.batch_status()
-    forces = [b.forces(i) for i in range(len(grids))]
-    b.close()
+    forces = [None] * n
+    for g, b in enumerate(bs):
+        sg, ig = b.batch_status()
+        st[groups[g]] = sg
+        ist[groups[g]] = ig
+        for m, i in enumerate(groups[g]):
+            forces[i] = b.forces(m)
+        b.close()
     return st, ist, secs, total, forces
 
 
 def run_batch(args, hip):
-    """The 64-case mix as one batch with FP32 LU-SGS: every case stops on the
-    device at its first step with rsum <= 1e-10 (ot2d_set_convergence).
+    """The 64-case mix with FP32 LU-SGS, as --batch-groups batches of similar
+    size on their own streams: every case stops on the device at its first
+    step with rsum <= 1e-10 (ot2d_set_convergence).
     With --plain-steps, each case also runs alone with FP64 LU-SGS for that
     many steps, and the plain path's time for the same solves is estimated
     from its per-case step time and the batch's steps to convergence."""
@@ -159,7 +191,9 @@ def run_batch(args, hip):
     cells = np.array([(g.jmax - 1) * (g.kmax - 1) for g in grids], dtype=np.float64)
     print(f"[bench] batch of {len(cases)} cases, {cells.sum():.0f} interior cells", file=sys.stderr, flush=True)
     n = len(cases)
-    st, ist, secs, total, forces = converge_batch(t2d, hip, params, meshes, grids, args.steps, args.batch_chunk, "batch")
+    groups = batch_groups(cells, args.batch_groups) if args.batch_groups > 1 else None
+    st, ist, secs, total, forces = converge_batch(t2d, hip, params, meshes, grids, args.steps, args.batch_chunk, "batch",
+                                                  groups)
     conv = st == abi.CONVERGED
     nconv = int(conv.sum())
     done_work = float((cells * ist).sum())
@@ -173,7 +207,7 @@ def run_batch(args, hip):
         "config": {"workload": "batch64_sa", "cases": n, "interior_cells": int(cells.sum()),
                    "jmax_range": [int(min(g.jmax for g in grids)), int(max(g.jmax for g in grids))],
                    "kmax_range": [int(min(g.kmax for g in grids)), int(max(g.kmax for g in grids))],
-                   "lu_sgs": "fp32", "residual": "fp64"},
+                   "lu_sgs": "fp32", "residual": "fp64", "batch_groups": max(1, args.batch_groups)},
         "converge": {"criterion": "rsum <= 1e-10", "converged": nconv, "stopped": int(((st == 1) | (st == 2)).sum()),
                      "running": int((st == 0).sum()), "seconds": round(secs, 2),
                      "solves_per_hr": 3600.0 * nconv / secs,
@@ -187,7 +221,7 @@ def run_batch(args, hip):
         p64 = [t2d.default_params(**dict(_SA, fsmach=c["fsmach"], alfa=c["alfa"], rey=c["rey"]), lu_fp32=0)
                for c in cases]
         st64, ist64, secs64, _, f64 = converge_batch(t2d, hip, p64, meshes, grids, args.steps, args.batch_chunk,
-                                                     "fp64 batch")
+                                                     "fp64 batch", groups)
         both = conv & (st64 == abi.CONVERGED)
         dcl = [abs(forces[i]["cl"] - f64[i]["cl"]) for i in range(n) if both[i]]
         dcd = [abs(forces[i]["cd"] - f64[i]["cd"]) for i in range(n) if both[i]]
@@ -417,6 +451,8 @@ def main():
     ap.add_argument("--serve-time-limit", type=float, default=None)
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-chunk", type=int, default=1000)
+    ap.add_argument("--batch-groups", type=int, default=3,
+                    help="batch64_sa: run the cases as this many batches on their own streams")
     ap.add_argument("--fp64-batch", action="store_true",
                     help="batch64_sa: also converge the cases with FP64 LU-SGS and compare Cl, Cd")
     ap.add_argument("--plain-steps", type=int, default=0,

@@ -1011,9 +1011,8 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
 // k_sa_lines<JL>: one lane per line, each lane walking its line with the
 // coefficients of the next SA_PK cells loaded ahead into registers.
 //   JL = 0: lines along k (one per j column), lanes on consecutive columns,
-//           so every step reads one contiguous row segment.  The backward
-//           pass is followed by the nu-tilde update with its clip
-//           (src/samodel2d.f:157-188) and the interior eddy viscosity.
+//           so every step reads one contiguous row segment; the nu-tilde
+//           update follows in k_sa_update.
 //   JL = 1: lines along j (one per k row); the lanes' rows are jd apart,
 //           but each lane's next cells lie in lines it has already read.
 //           The forward pass leaves c' and s' in place (as lsolvej does),
@@ -1115,34 +1114,25 @@ __global__ __launch_bounds__(64) void k_sa_lines(const DevMesh* __restrict__ Ms)
 #pragma unroll
   for (int e = 0; e < SA_PK; ++e)
     if (ub + e < n) bwd(ub + e, (1 + e) % SA_PK);
-  if (JL || !lane_ok) return;
-  // nu-tilde update with its clip and the eddy viscosity, eight cells at a
-  // time with every load of a group issued before its stores
+}
+
+// nu-tilde update with its clip (src/samodel2d.f:157-188) and the interior
+// eddy viscosity after the lane-per-line k solve: a pointwise pass over the
+// interior cells, so its loads stream at full bandwidth instead of waiting
+// on each line's recurrence
+__global__ __launch_bounds__(256) void k_sa_update(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
+  const long long ncell = (long long)nj * nk;
   const double vnulim = 1.0e-20, relfac = 1.0;
-  gdouble* __restrict__ q5p = gptr(M.q[4]);
-  const gdouble* __restrict__ q1p = gptr(M.q[0]);
-  const gdouble* __restrict__ qnp = gptr(M.q[M.nq - 1]);
-  const gdouble* __restrict__ vmp = gptr(M.sa_vmul);
-  const __attribute__((address_space(1))) int* __restrict__ ibp =
-      (const __attribute__((address_space(1))) int*)M.iblank;
-  gdouble* __restrict__ tmu = gptr(M.turmu);
-  for (int t0 = 0; t0 < n; t0 += 8) {
-    double dq[8], q5[8], q1[8], qn[8], vm[8];
-    int ib[8];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      const long long i = i0 + (long long)(t0 + d < n ? t0 + d : n - 1) * st;
-      dq[d] = sS[i]; q5[d] = q5p[i]; q1[d] = q1p[i]; qn[d] = qnp[i]; vm[d] = vmp[i]; ib[d] = ibp[i];
-    }
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      if (t0 + d < n) {
-        const long long i = i0 + (long long)(t0 + d) * st;
-        const double v = fmax((q5[d] + relfac * (0. + dq[d]) * (double)imx(ib[d], 0)), vnulim);
-        q5p[i] = v;
-        tmu[i] = c_turm_val(q1[d], qn[d], vm[d], v);
-      }
-    }
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
+    const int j = M.js + (int)(w % nj), k = M.ks + (int)(w / nj);
+    const long long i = cell(M, j, k);
+    const double dq = M.sa_sj[i];
+    const double v = fmax((M.q[4][i] + relfac * (0. + dq) * (double)imx(M.iblank[i], 0)), vnulim);
+    M.q[4][i] = v;
+    M.turmu[i] = c_turm_val(M.q[0][i], M.q[M.nq - 1][i], M.sa_vmul[i], v);
   }
 }
 
@@ -1236,8 +1226,13 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   else hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_J, st);
-  if (per_lane) hipLaunchKernelGGL(k_sa_lines<0>, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
-  else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
+  if (per_lane) {
+    hipLaunchKernelGGL(k_sa_lines<0>, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sa_update, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
+  } else {
+    hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
+  }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
   hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, 1);
