@@ -830,9 +830,8 @@ extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
 
 // one step (src/step.f:2-218) + monitor, enqueued without synchronisation
 static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
-  HIPCHK(launch_step_begin(c->d_istep0, c->d_status, c->d_stepc, c->ncase, c->stream));
-  mark(mk, OT2D_ST_BEGIN, c->stream);
-  HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream));
+  // the step opens (istep0, step counters) inside the time-step kernel
+  HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream, c->d_istep0, c->d_status, c->d_stepc, c->ncase));
   mark(mk, OT2D_ST_TIME_STEP, c->stream);
   if (int32_t rc = enqueue_residual(c, 1, 1, mk)) return rc;
   HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));

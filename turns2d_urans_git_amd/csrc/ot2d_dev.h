@@ -233,7 +233,10 @@ inline void mark(Marks* m, int s, hipStream_t st) {
 }
 
 hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st);
-hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+// with ncase > 0 also opens the step of every running case (istep0 and the
+// step counter)
+hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int* istep0 = nullptr,
+                            const int* status = nullptr, int* stepc = nullptr, int ncase = 0);
 hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, int weno,
                            hipStream_t st);
@@ -247,7 +250,6 @@ hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_upd
                      hipStream_t st);
 hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
-hipError_t launch_step_begin(int* istep0, int* status, int* stepc, int ncase, hipStream_t st);
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd, int rl);

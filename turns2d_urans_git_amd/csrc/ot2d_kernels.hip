@@ -73,7 +73,17 @@ __device__ inline void time_step_cell(const DevMesh& M, long long i) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_time_step(const DevMesh* __restrict__ Ms) {
+// With ncase > 0 the first workgroup also opens the step of every running
+// case (istep0 and the step counter; src/Turns_overset.f:364): the
+// boundary conditions launched next are the first readers of istep0.
+__global__ __launch_bounds__(256) void k_time_step(const DevMesh* __restrict__ Ms, int* istep0, const int* status,
+                                                   int* stepc, int ncase) {
+  if (blockIdx.x == 0 && blockIdx.y == 0)
+    for (int c = threadIdx.x; c < ncase; c += 256)
+      if (status[c] == 0) {
+        istep0[c] += 1;
+        stepc[c] += 1;
+      }
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   long long n2 = (long long)M.jd * M.kd;
@@ -2028,14 +2038,6 @@ __global__ __launch_bounds__(256) void k_monitor(const DevMesh* __restrict__ Ms)
   if (bad) atomicCAS(M.status, 0, 1);
 }
 
-__global__ void k_step_begin(int* istep0, int* status, int* stepc, int ncase) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ncase && status[i] == 0) {
-    istep0[i] += 1;
-    stepc[i] += 1;
-  }
-}
-
 // ================================================================ geometry
 // initia for one instance (src/init.f:2-177): find_coord_TS (setup.f:348),
 // find_cellcenter (setup.f:111-345), metfv (setup.f:440-724),
@@ -2359,10 +2361,11 @@ hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int* istep0,
+                            const int* status, int* stepc, int ncase) {
   long long mx = 0;
   for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
-  hipLaunchKernelGGL(k_time_step, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  hipLaunchKernelGGL(k_time_step, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, istep0, status, stepc, ncase);
   return hipGetLastError();
 }
 
@@ -2546,11 +2549,6 @@ hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
   long long mx = 0;
   for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
   hipLaunchKernelGGL(k_monitor, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
-  return hipGetLastError();
-}
-
-hipError_t launch_step_begin(int* istep0, int* status, int* stepc, int ncase, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_begin, dim3((ncase + 255) / 256), dim3(256), 0, st, istep0, status, stepc, ncase);
   return hipGetLastError();
 }
 
