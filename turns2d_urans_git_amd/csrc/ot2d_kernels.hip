@@ -1510,9 +1510,14 @@ constexpr int LU32_NV = 9;
 // bit pattern of an empty mailbox word: a signalling NaN, which no
 // arithmetic result takes (they are quiet NaNs at worst)
 constexpr unsigned LU32_EMPTY = 0x7fa5a5a5u;
-// cache-policy operand of a buffer store that writes through to the device
-// coherence point (the sc1 bit), as an agent-scope atomic store does
-constexpr int LU32_SC1 = 16;
+// cache-policy operand (sc0 and sc1 bits: system scope) of the result
+// stores whose edge lanes are the inter-workgroup mail.  The mail crosses
+// XCDs within one kernel; with device-scope (sc1 only) stores and loads,
+// sweeps overlapping other streams' kernels read wrong mail now and then
+// (the 64-case mix split over three streams: 7 of 14 runs of 3000 steps
+// ended with a corrupted case), with system-scope stores and loads none did
+// (0 of 10).
+constexpr int LU32_SYS = 17;
 
 // rows of the 4x4 matrix M with a = M . s for a neighbour's A(+/-) or
 // B(+/-) term (src/lhs.f:633-651)
@@ -1670,19 +1675,19 @@ __device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_fla
 
 // mailbox words are accessed through global (not generic) pointers: flat
 // instructions would also count against the LDS wait counter and stall the
-// LDS hand-offs behind device-scope stores
+// LDS hand-offs behind the result stores
 typedef __attribute__((address_space(1))) unsigned gu32;
 
 // wait until all four words of a mailbox entry written by another
-// workgroup are filled (device-scope loads: the writer may sit on another
+// workgroup are filled (system-scope loads: the writer may sit on another
 // XCD)
 __device__ __forceinline__ bool wait_mail(gu32* p, u32x4_t& out, int* abort_flag) {
   const unsigned long long t0 = rt_now();
   for (;;) {
-    unsigned a = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned c = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned d = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned a = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    unsigned b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    unsigned c = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    unsigned d = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     if (a != LU32_EMPTY && b != LU32_EMPTY && c != LU32_EMPTY && d != LU32_EMPTY) {
       out = u32x4_t{a, b, c, d};
       return true;
@@ -1728,7 +1733,7 @@ __device__ __forceinline__ float dpp_shift_in(float v, float edge) {
 // (no instructions) keep that program order without the waits a fence
 // would add.
 // Across workgroups: every lane's result of every step goes to the step
-// ordered array (forward H, backward H2) with device-scope stores; the
+// ordered array (forward H, backward H2) with system-scope stores; the
 // producers of the neighbouring block read the edge lane's entries there,
 // waiting while they still hold the empty sentinel the prep stored.
 // Forward, block w waits on block w-1; backward, on block w+1; the dispatch
@@ -1810,10 +1815,10 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
         v[NV + 1] = cin ? cb : u32x4_t{0u, 0u, 0u, 0u};
       }
       gu32* p = mail_ptr(tc);
-      v[NV] = u32x4_t{__hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                      __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                      __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                      __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+      v[NV] = u32x4_t{__hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                      __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                      __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                      __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)};
     };
     auto deliver = [&](int tt, u32x4_t (&v)[NB]) -> bool {
       if (tt >= NT) return true;
@@ -1909,12 +1914,12 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
       p = __builtin_fmaf(b.z, kin[2], p);
       sn[q] = __builtin_fmaf(b.w, kin[3], p);
     }
-    // the result of (step, lane), device-scope (sc1): the edge lane's entry
-    // is the neighbouring block's mail; entries of cells outside the grid
-    // are never read as results
+    // the result of (step, lane), system scope: the edge lane's entry is
+    // the neighbouring block's mail; entries of cells outside the grid are
+    // never read as results
     const int orow = DIR > 0 ? NT - 1 - t : t;
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}), hout, l * 16,
-                                           (base + orow) * 1024, LU32_SC1);
+                                           (base + orow) * 1024, LU32_SYS);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       sA[q] = sn[q];

@@ -8,9 +8,7 @@ time.  Between iterations the loop
   moment that share a structure (model, instance count) become one batch
   (``ot2d_batch_create``), a *cohort*, with its own HIP stream;
 * advances every resident cohort by ``steps_per_tick`` iterations with
-  ``ot2d_iterate_async``, so the cohorts' kernels overlap on the device
-  (cohorts with FP32 LU-SGS run one after another while the cause of their
-  nondeterministic results under overlap is open);
+  ``ot2d_iterate_async``, so the cohorts' kernels overlap on the device;
 * reads every cohort's per-case states (``ot2d_batch_status``): a case whose
   residual has reached the tolerance stopped on the device (status
   ``CONVERGED``, set by the step itself before its update), and its
@@ -96,7 +94,6 @@ def device_memory():
 class _Cohort:
     def __init__(self, lib, reqs: List[Request], tol: float, errfn):
         self.reqs = reqs
-        self.fp32 = any(int(r.deck.get("lu_fp32", 0)) for r in reqs)
         params = []
         for r in reqs:
             p = abi.Params()
@@ -185,15 +182,9 @@ class Server:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
-            # ---- one tick: every cohort advances, overlapping on the device.
-            # Cohorts with FP32 LU-SGS run one after another: their sweeps
-            # running concurrently with other cohorts have given
-            # nondeterministic results, which FP64 LU-SGS cohorts have not.
-            serial = len(cohorts) > 1 and any(c.fp32 for c in cohorts)
+            # ---- one tick: every cohort advances, overlapping on the device
             for c in cohorts:
                 c.case.iterate_async(self.k)
-                if serial:
-                    c.case.sync()
             self.ticks += 1
             done_cohorts = []
             for c in cohorts:
