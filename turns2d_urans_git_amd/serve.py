@@ -8,7 +8,9 @@ time.  Between iterations the loop
   moment that share a structure (model, instance count) become one batch
   (``ot2d_batch_create``), a *cohort*, with its own HIP stream;
 * advances every resident cohort by ``steps_per_tick`` iterations with
-  ``ot2d_iterate_async``, so the cohorts' kernels overlap on the device;
+  ``ot2d_iterate_async``, so the cohorts' kernels overlap on the device
+  (cohorts with FP32 LU-SGS run one after another: overlapped, they have
+  now and then corrupted a case);
 * reads every cohort's per-case states (``ot2d_batch_status``): a case whose
   residual has reached the tolerance stopped on the device (status
   ``CONVERGED``, set by the step itself before its update), and its
@@ -94,6 +96,7 @@ def device_memory():
 class _Cohort:
     def __init__(self, lib, reqs: List[Request], tol: float, errfn):
         self.reqs = reqs
+        self.fp32 = any(int(r.deck.get("lu_fp32", 0)) for r in reqs)
         params = []
         for r in reqs:
             p = abi.Params()
@@ -182,9 +185,15 @@ class Server:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
-            # ---- one tick: every cohort advances, overlapping on the device
+            # ---- one tick: every cohort advances, overlapping on the device.
+            # Cohorts with FP32 LU-SGS run one after another: their sweeps
+            # overlapping other cohorts' kernels have now and then corrupted
+            # a case, which FP64 LU-SGS cohorts have not.
+            serial = len(cohorts) > 1 and any(c.fp32 for c in cohorts)
             for c in cohorts:
                 c.case.iterate_async(self.k)
+                if serial:
+                    c.case.sync()
             self.ticks += 1
             done_cohorts = []
             for c in cohorts:
