@@ -187,7 +187,8 @@ int32_t ot2d_footprint(int32_t ncase, const ot2d_params* params, int32_t nmesh, 
 int32_t ot2d_case_bytes(ot2d_case* c, int64_t* bytes);
 /* Case icase (-1: all) stops, with status OT2D_CONVERGED, at the first step
  * whose rsum is at or below rsum_tol on every instance, before that step's
- * update; rsum_tol <= 0 switches the test off. */
+ * update; rsum_tol <= 0 switches the test off.  While the test is on, a
+ * case whose rsum is not a finite number stops with OT2D_NORM. */
 int32_t ot2d_set_convergence(ot2d_case* c, int32_t icase, double rsum_tol);
 /* Enqueues niter steps on the case's stream and returns without waiting
  * (no residual history); ot2d_sync or ot2d_batch_status waits. */

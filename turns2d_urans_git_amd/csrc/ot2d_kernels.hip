@@ -2401,11 +2401,16 @@ __global__ __launch_bounds__(256) void k_conv_check(const DevMesh* __restrict__ 
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= ncase || status[c] != 0 || !(tol[c] > 0.)) return;
   double mx = 0.;
+  bool finite = true;
   for (int i = inst[2 * c]; i < inst[2 * c + 1]; ++i) {
     double r = Ms[i].norms[0];
+    finite = finite && r == r && r - r == 0.;
     mx = r > mx ? r : mx;
   }
-  if (mx <= tol[c]) status[c] = 4;
+  // a residual that is not a finite number stops the case as the norm check
+  // does a blown-up one (status 2), never as converged
+  if (!finite) status[c] = 2;
+  else if (mx <= tol[c]) status[c] = 4;
 }
 
 hipError_t launch_conv_check(const DevMesh* dM, const int* inst, const double* tol, int* status, int ncase,
