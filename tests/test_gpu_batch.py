@@ -127,3 +127,22 @@ def test_many_line_batch_equals_alone():
         np.testing.assert_array_equal(hist[:, i], h_s[:, 0])
         np.testing.assert_array_equal(b.get_state(i), solo.get_state(0))
         np.testing.assert_array_equal(b.get_field(i, abi.F_TURMU), solo.get_field(0, abi.F_TURMU))
+
+
+def test_nonfinite_residual_stops_with_norm():
+    """With the convergence test on, a case whose residual is not a finite
+    number stops with NORM, not CONVERGED; the other cases run on."""
+    import turns2d_urans_git_amd as t2d
+    grids, params, meshes = _inputs(0)
+    b = t2d.Batch(params, meshes)
+    for i, g in enumerate(grids):
+        b.set_grid(i, g.xv, g.yv)
+    b.set_convergence(1e-10)
+    q = b.get_state(1)
+    q[5:9, 5:9, :, :4] = np.nan
+    b.set_state(1, q)
+    b.iterate_async(3)
+    st, ist = b.batch_status()
+    assert st[1] == abi.NORM, st
+    assert st[0] == abi.OK and st[2] == abi.OK, st
+    assert ist[0] == 3 and ist[2] == 3
