@@ -452,8 +452,7 @@ def main():
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-chunk", type=int, default=1000)
     ap.add_argument("--batch-groups", type=int, default=1,
-                    help="batch64_sa: run the cases as this many batches on their own streams (FP32 LU-SGS "
-                         "batches overlapping on several streams have now and then corrupted a case)")
+                    help="batch64_sa: run the cases as this many batches on their own streams, overlapping")
     ap.add_argument("--fp64-batch", action="store_true",
                     help="batch64_sa: also converge the cases with FP64 LU-SGS and compare Cl, Cd")
     ap.add_argument("--plain-steps", type=int, default=0,

@@ -105,7 +105,11 @@ def test_converged_case_stops_on_the_device():
 def test_concurrent_cohorts_do_not_change_results():
     """Cases advanced on separate streams at the same time give the results
     they give alone, bit for bit (FP32 and FP64 LU-SGS, steady and
-    time-spectral cohorts side by side)."""
+    time-spectral cohorts side by side).  These grids keep every FP32 sweep
+    inside one 64-row workgroup; overlapped FP32 sweeps over several
+    workgroups (the inter-workgroup mail) are covered by
+    test_gpu_lu32_multiblock.py and by test_multiblock_fp32_cohorts_overlap
+    below."""
     import turns2d_urans_git_amd as t2d
     lib, err = _lib()
 
@@ -139,3 +143,19 @@ def test_concurrent_cohorts_do_not_change_results():
         x.close()
         for o in others:
             o.close()
+
+
+def test_multiblock_fp32_cohorts_overlap():
+    """Served cohorts whose FP32 sweeps span two and three 64-row workgroups
+    (kmax 97 and 161) beside other cohorts, all overlapping on their streams:
+    every request's result equals its result served alone, bit for bit."""
+    target = _polar(0, 0.0, m=0.03, alfa=3.0, jmax=97, kmax=97)
+    _, alone = _serve([target])
+    mix = [_polar(1, 0.0, alfa=1.0, jmax=65, kmax=161), _polar(2, 0.0, jmax=81, kmax=129, alfa=-2.0),
+           _pitch(3, 0.0, n=3), _polar(4, 0.0, fp32=0, kmax=97)]
+    t2 = serve.Request(9, 0.05, target.deck, target.grid, target.kind)
+    _, both = _serve(mix + [t2])
+    a, b = alone[0], [r for r in both if r.rid == 9][0]
+    assert a.status == b.status and a.steps == b.steps
+    np.testing.assert_array_equal(a.state, b.state)
+    np.testing.assert_array_equal(a.forces, b.forces)

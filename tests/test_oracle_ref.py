@@ -3,7 +3,10 @@
 Skipped where the reference has not been built (it is built from
 /root/reference by `make -C oracle ref`).  Decks beyond the committed
 fixtures: laminar Navier-Stokes, a cambered airfoil at negative incidence,
-second-order MUSCL, constant-dt pseudo time.
+second-order MUSCL, constant-dt pseudo time, WENO5, SST, time spectral; the
+boundary types 10, 11, 22 and 46 in every direction (Euler, SA, SST); and
+both stops of the reference (monitor's negative speed of sound, the norm
+out of bounds) at the same iteration with the same state.
 """
 import os
 import subprocess
@@ -50,18 +53,29 @@ GRIDS = {"cambered_neg_alpha": dict(m=0.04, p=0.4, t=0.10)}
 ZERO_MALLOC = {"GLIBC_TUNABLES": "glibc.malloc.perturb=255"}
 
 
-def _ref_run(g, deck, nsteps):
+def _ref_run(g, deck, nsteps, segs=None, qin=None, notes=None):
+    """Run the reference harness on one mesh; with qin, start from that state
+    (q(jd,kd,1,nq) in the reference's layout; the harness reads the whole
+    internal array, one word longer).  Returns the final state and the norm
+    history; `notes` collects the harness's comment lines (stops)."""
     with tempfile.TemporaryDirectory() as d:
-        grid.write_case(d, g, nsteps=nsteps, nrest=10**6, npnorm=1,
+        grid.write_case(d, g, segs=segs, nsteps=nsteps, nrest=10**6, npnorm=1,
                         **{k: (bool(v) if k in ("invisc", "lamin") else v) for k, v in deck.items()})
         out = os.path.join(d, "out")
         os.makedirs(out)
         env = dict(os.environ, OT2D_OUT=out, OT2D_NSTEPS=str(nsteps), **ZERO_MALLOC)
+        if qin is not None:
+            np.append(np.asarray(qin, dtype="<f8").ravel(order="F"), 0.0).tofile(os.path.join(d, "qin.bin"))
+            env["OT2D_QIN"] = os.path.join(d, "qin.bin")
         with open(os.path.join(d, "inputs")) as fin:
             subprocess.run([H.REF_HARNESS], cwd=d, stdin=fin, env=env, check=True,
                            stdout=subprocess.DEVNULL, timeout=120)
-        q = np.fromfile(os.path.join(out, "qfinal_m1.bin"))
+        qf = os.path.join(out, "qfinal_m1.bin")
+        q = np.fromfile(qf) if os.path.exists(qf) else None   # absent after stop 'norm'
         norms = np.loadtxt(os.path.join(out, "norms.txt"), comments="#", ndmin=2)
+        if notes is not None:
+            with open(os.path.join(out, "norms.txt")) as f:
+                notes.extend(x.strip() for x in f if x.startswith("#"))
     return q, norms
 
 
@@ -112,3 +126,71 @@ def test_oracle_matches_reference_time_spectral(name):
     np.testing.assert_array_equal(norms_ref[:, 2], np.tile(np.arange(1, ns + 1), nsteps))
     np.testing.assert_array_equal(hist, norms_ref[:, 3:6])
     np.testing.assert_array_equal(case.get_state(0).ravel(order="F"), q_ref)
+
+
+# ------------------------------------------------- boundary types and stops
+def _box_oracle(deck, segs):
+    from tests import bc_decks as B
+    g = B.box()
+    return g, H.oracle_case(g, deck, segs=segs)
+
+
+@pytest.mark.parametrize("name", sorted(__import__("tests.bc_decks", fromlist=["x"]).BC_DECKS))
+def test_oracle_matches_reference_bc_types(name):
+    """Extrapolation, symmetry, periodic and free-stream boundaries (types
+    10, 11, 22, 46) in every direction, Euler, SA and SST, 15 steps from a
+    perturbed state: norms and final state bit for bit."""
+    from tests import bc_decks as B
+    deck, segs = B.BC_DECKS[name]
+    g, case = _box_oracle(deck, segs)
+    q0 = B.perturbed(case.get_state(0))
+    case.set_state(0, q0)
+    nsteps = 15
+    q_ref, norms_ref = _ref_run(g, deck, nsteps, segs=segs, qin=q0)
+    hist = []
+    for _ in range(nsteps):
+        rc, nrm, _ = case.step()
+        assert rc == 0
+        hist.append(nrm[0].copy())
+    np.testing.assert_array_equal(np.array(hist), norms_ref[:, 3:6])
+    np.testing.assert_array_equal(case.get_state(0).ravel(order="F"), q_ref)
+
+
+@pytest.mark.parametrize("name", sorted(__import__("tests.bc_decks", fromlist=["x"]).MSTOP_CASES))
+def test_oracle_matches_reference_monitor_stop(name):
+    """monitor (src/step.f:332-392) finds a^2 <= 0.002 and the driver loop
+    stops (src/Turns_overset.f:495-497): same iteration, same state."""
+    from tests import bc_decks as B
+    asq, amp, at = B.MSTOP_CASES[name]
+    g, case = _box_oracle(B.EULER, B.CLOSED)
+    q0 = B.quiescent(case.get_state(0), asq, amp)
+    case.set_state(0, q0)
+    notes = []
+    q_ref, norms_ref = _ref_run(g, B.EULER, 30, segs=B.CLOSED, qin=q0, notes=notes)
+    assert any("mstop at iteration" in x and int(x.split()[-1]) == at for x in notes), notes
+    hist, rc = [], 0
+    for _ in range(30):
+        rc, nrm, _ = case.step()
+        hist.append(nrm[0].copy())
+        if rc:
+            break
+    assert rc == abi.MSTOP and len(hist) == at
+    np.testing.assert_array_equal(np.array(hist), norms_ref[:, 3:6])
+    np.testing.assert_array_equal(case.get_state(0).ravel(order="F"), q_ref)
+
+
+def test_oracle_matches_reference_norm_stop():
+    """rsum > 1000 stops the reference with stop 'norm' (src/step.f:105-110)
+    before the update: the oracle returns the norm status at the same
+    iteration with the same norms."""
+    from tests import bc_decks as B
+    g, case = _box_oracle(B.EULER, B.EXTP_SYM_INF)
+    q0 = B.blown_up(case.get_state(0))
+    case.set_state(0, q0)
+    notes = []
+    _, norms_ref = _ref_run(g, B.EULER, 5, segs=B.EXTP_SYM_INF, qin=q0, notes=notes)
+    assert "# norm out of bounds" in notes and norms_ref.shape[0] == 1
+    assert norms_ref[0, 3] > 1000.0
+    rc, nrm, _ = case.step()
+    assert rc == abi.NORM
+    np.testing.assert_array_equal(nrm[0], norms_ref[0, 3:6])

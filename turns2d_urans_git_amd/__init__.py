@@ -3,7 +3,7 @@
 The product is ``libot2d.so`` (built in-tree by ``build.py`` from ``csrc/``)
 whose entry points are declared in ``include/ot2d.h``.  This package holds
 the host-side pieces around it: the ctypes mirror of the ABI (``abi``), the
-grid/deck tools (``grid``), the deck reader (``deck``) and ``Case``, a thin
+grid and deck writer (``grid``), the request loop (``serve``) and ``Case``, a thin
 Python driver equivalent to the Fortran one in ``fortran/``.
 
 There is no CPU fallback: if the HIP library is missing or no GPU is

@@ -267,10 +267,10 @@ static void dfree(void* p) {
 }
 
 static void free_case(ot2d_case* c) {
+  if (!c) return;
   dfree(c->d_don);
   dfree(c->d_fr);
   dfree(c->d_qbc);
-  if (!c) return;
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (auto& mb : c->m) {
     dfree(mb.q); dfree(mb.s); dfree(mb.x); dfree(mb.y); dfree(mb.xx); dfree(mb.xy);
@@ -371,8 +371,11 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     for (int ic = 0; ic < ncase; ++ic)  // src/init.f:71-78
       for (int n = 0; n < ns; ++n)
         tspec[(size_t)ic * ns + n] = (ps[ic].rf != 0.0) ? 2 * pi * n / ns / fabs(ps[ic].rf) : 0.0;
-    (void)hipMemcpy(c->d_Ds, Ds.data(), sizeof(double) * ns * ns, hipMemcpyHostToDevice);
-    (void)hipMemcpy(c->d_tspec, tspec.data(), sizeof(double) * ns * ncase, hipMemcpyHostToDevice);
+    if ((e = hipMemcpy(c->d_Ds, Ds.data(), sizeof(double) * ns * ns, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->d_tspec, tspec.data(), sizeof(double) * ns * ncase, hipMemcpyHostToDevice)) != hipSuccess) {
+      free_case(c);
+      return fail(OT2D_ERR_HIP, "spectral coefficients: %s", hipGetErrorString(e));
+    }
   }
   // rows per lane of the LU-SGS sweeps: one wavefront covers every row
   c->lu_rl = 1;
@@ -529,8 +532,11 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     free_case(c);
     return fail(OT2D_ERR_NOMEM, "device allocation failed");
   }
-  (void)hipMemcpy(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice);
-  (void)hipMemcpy(c->dT, c->hT.data(), sizeof(ResTile) * c->hT.size(), hipMemcpyHostToDevice);
+  if ((e = hipMemcpy(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(c->dT, c->hT.data(), sizeof(ResTile) * c->hT.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+    free_case(c);
+    return fail(OT2D_ERR_HIP, "mesh records: %s", hipGetErrorString(e));
+  }
   c->bytes += sizeof(DevMesh) * c->hM.size() + sizeof(ResTile) * c->hT.size();
   // instance range of every case (meshes of one case are consecutive)
   {
@@ -545,7 +551,10 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
       inst[2 * ic] = lo < 0 ? 0 : lo;
       inst[2 * ic + 1] = hi < 0 ? 0 : hi;
     }
-    (void)hipMemcpy(c->d_cinst, inst.data(), sizeof(int) * inst.size(), hipMemcpyHostToDevice);
+    if ((e = hipMemcpy(c->d_cinst, inst.data(), sizeof(int) * inst.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+      free_case(c);
+      return fail(OT2D_ERR_HIP, "case ranges: %s", hipGetErrorString(e));
+    }
   }
   *out = c;
   return OT2D_OK;
@@ -594,11 +603,15 @@ extern "C" int32_t ot2d_case_bytes(ot2d_case* c, int64_t* bytes) {
 
 extern "C" int32_t ot2d_set_convergence(ot2d_case* c, int32_t icase, double rsum_tol) {
   if (!c || icase < -1 || icase >= c->ncase) return fail(OT2D_ERR_ARG, "bad case index");
+  // on the case's stream, after every step already queued there (a
+  // convergence check queued before this call reads the old tolerances)
   std::vector<double> tol(c->ncase, 0.0);
-  HIPCHK(hipMemcpy(tol.data(), c->d_ctol, sizeof(double) * c->ncase, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(tol.data(), c->d_ctol, sizeof(double) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->ncase; ++i)
     if (icase < 0 || i == icase) tol[i] = rsum_tol;
-  HIPCHK(hipMemcpy(c->d_ctol, tol.data(), sizeof(double) * c->ncase, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(c->d_ctol, tol.data(), sizeof(double) * c->ncase, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   int on = 0;
   for (double t : tol) on |= t > 0.;
   if (on != c->conv_on) {
@@ -923,8 +936,15 @@ static int32_t ensure_graph(ot2d_case* c) {
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int32_t rc = enqueue_step(c);
   hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-  if (rc) return rc;
+  if (rc) {
+    if (e2 == hipSuccess) (void)hipGraphDestroy(g);
+    return rc;
+  }
   if (e2 != hipSuccess) return fail(OT2D_ERR_HIP, "capture: %s", hipGetErrorString(e2));
+  if (c->graph) {   // a re-capture (the step changed): the old executable goes
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
   hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e3 != hipSuccess) return fail(OT2D_ERR_HIP, "instantiate: %s", hipGetErrorString(e3));

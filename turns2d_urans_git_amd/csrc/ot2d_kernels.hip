@@ -1119,6 +1119,8 @@ struct LUGeom {
   long long off_b, off_hcf, off_hcb, off_hrf, off_hrb;   // FP64 arrays (16-byte vectors)
   long long ncell;                 // FP32: interior cells
   long long off_f32, off_b32, off_cb, off_h, off_h2;   // FP32 arrays
+  long long off_mf, off_mb;        // FP32 inter-workgroup mail, forward / backward (2 vectors per entry)
+  long long off_tk;                // FP32 workgroup tickets (forward, backward): one vector
   long long total;                 // vectors the workspace must hold (either layout)
 };
 
@@ -1140,14 +1142,18 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   long long t64 = g.off_hrb + (long long)LU_NVEC * ni;
   // FP32: the records, cB, the forward results and the backward results,
   // all in step layout (a 1 KiB row of 64 lanes per vector, block and
-  // step); the results of a block's edge lane are the inter-workgroup mail
+  // step); then the mail of each block's edge lane (one 32-byte entry per
+  // block and step, two 16-byte vectors) and the two ticket counters
   g.ncell = (long long)ni * nk;
   g.off_f32 = 0;
   g.off_b32 = g.off_f32 + 9ll * g.W * g.TA * 64;
   g.off_cb = g.off_b32 + 8ll * g.W * g.TA * 64;
   g.off_h = g.off_cb + (long long)g.W * g.TW * 64;
   g.off_h2 = g.off_h + (long long)g.W * g.TW * 64;
-  long long t32 = g.off_h2 + (long long)g.W * g.TW * 64;
+  g.off_mf = g.off_h2 + (long long)g.W * g.TW * 64;
+  g.off_mb = g.off_mf + 2ll * g.W * g.TW;
+  g.off_tk = g.off_mb + 2ll * g.W * g.TW;
+  long long t32 = g.off_tk + 1;
   g.total = t64 > t32 ? t64 : t32;
   return g;
 }
@@ -1507,16 +1513,8 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // backward s_fwd + cB, PA' rows, PB' rows (one more vector carries the
 // mailbox value).
 constexpr int LU32_NV = 9;
-// bit pattern of an empty mailbox word: a signalling NaN, which no
-// arithmetic result takes (they are quiet NaNs at worst)
-constexpr unsigned LU32_EMPTY = 0x7fa5a5a5u;
-// cache-policy operand (sc0 and sc1 bits: system scope) of the result
-// stores whose edge lanes are the inter-workgroup mail.  The mail crosses
-// XCDs within one kernel; with device-scope (sc1 only) stores and loads,
-// sweeps overlapping other streams' kernels read wrong mail now and then
-// (the 64-case mix split over three streams: 7 of 14 runs of 3000 steps
-// ended with a corrupted case), with system-scope stores and loads none did
-// (0 of 10).
+// cache-policy operand (sc0 and sc1 bits: system scope) of the mail stores
+// and loads, which cross XCDs within one kernel
 constexpr int LU32_SYS = 17;
 
 // rows of the 4x4 matrix M with a = M . s for a neighbour's A(+/-) or
@@ -1587,17 +1585,6 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_lu_prep32(const DevMesh* __re
   F4* recb = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_b32);
   F4* cbs = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_cb);
   const int nthr = 64 * LP_WAVES;
-  // the mail entries start as the empty sentinel: forward, lane 63 of H
-  // rows 0 .. ni-1; backward, lane 0 of H2 rows 63 .. ni+62 (every block)
-  const long long nmb = 2ll * g.W * g.ni;
-  u32x4_t* h = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_h);
-  u32x4_t* h2 = reinterpret_cast<u32x4_t*>(reinterpret_cast<D2*>(M.lu) + g.off_h2);
-  for (long long w = blockIdx.x * (long long)nthr + threadIdx.x; w < nmb; w += (long long)nthr * gridDim.x) {
-    const long long e = w % ((long long)g.W * g.ni), blk = e / g.ni, r = e - blk * g.ni;
-    const u32x4_t empty = u32x4_t{LU32_EMPTY, LU32_EMPTY, LU32_EMPTY, LU32_EMPTY};
-    if (w < (long long)g.W * g.ni) h[(blk * g.TW + r) * 64 + 63] = empty;
-    else h2[(blk * g.TW + r + 63) * 64] = empty;
-  }
   // tile of this workgroup: block w, rows 8*lb .. 8*lb+7, steps T0 .. T0+LP_STEPS-1
   const int tiles_t = (g.TA + LP_STEPS - 1) / LP_STEPS;
   if ((int)blockIdx.x >= g.W * 8 * tiles_t) return;
@@ -1673,25 +1660,44 @@ __device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_fla
   return false;
 }
 
-// mailbox words are accessed through global (not generic) pointers: flat
-// instructions would also count against the LDS wait counter and stall the
-// LDS hand-offs behind the result stores
-typedef __attribute__((address_space(1))) unsigned gu32;
+// Inter-workgroup mail.  The edge lane of block w stores its result of
+// sweep step t as entry w*TW + t of its direction's mail array: four 8-byte
+// words, word q holding component q's bits in the low half and the sweep's
+// tag in the high half, each written by one 8-byte atomic store.  A reader
+// accepts an entry only when all four words, each read by one 8-byte atomic
+// load, carry the running sweep's tag.  An entry left by an earlier sweep
+// (an older tag) or one still being written (an older tag in some word) is
+// therefore never taken, and nothing else writes the mail: it needs no
+// reset between sweeps.  Accesses go through global (not generic) pointers:
+// flat instructions would also count against the LDS wait counter and stall
+// the LDS hand-offs behind them.  System scope: writer and reader may sit on
+// different XCDs.
+typedef __attribute__((address_space(1))) unsigned long long gu64;
 
-// wait until all four words of a mailbox entry written by another
-// workgroup are filled (system-scope loads: the writer may sit on another
-// XCD)
-__device__ __forceinline__ bool wait_mail(gu32* p, u32x4_t& out, int* abort_flag) {
+struct Mail {
+  unsigned long long w[4];
+};
+
+__device__ __forceinline__ void mail_load(const gu64* p, Mail& m) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) m.w[q] = __hip_atomic_load(p + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ bool mail_ok(const Mail& m, unsigned tag) {
+  return (unsigned)(m.w[0] >> 32) == tag && (unsigned)(m.w[1] >> 32) == tag && (unsigned)(m.w[2] >> 32) == tag &&
+         (unsigned)(m.w[3] >> 32) == tag;
+}
+
+__device__ __forceinline__ u32x4_t mail_value(const Mail& m) {
+  return u32x4_t{(unsigned)m.w[0], (unsigned)m.w[1], (unsigned)m.w[2], (unsigned)m.w[3]};
+}
+
+// wait until the entry at p carries the running sweep's tag
+__device__ __forceinline__ bool wait_mail(const gu64* p, unsigned tag, Mail& m, int* abort_flag) {
   const unsigned long long t0 = rt_now();
   for (;;) {
-    unsigned a = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    unsigned b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    unsigned c = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    unsigned d = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    if (a != LU32_EMPTY && b != LU32_EMPTY && c != LU32_EMPTY && d != LU32_EMPTY) {
-      out = u32x4_t{a, b, c, d};
-      return true;
-    }
+    mail_load(p, m);
+    if (mail_ok(m, tag)) return true;
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
     if (rt_now() - t0 > LU32_WAIT) break;
     __builtin_amdgcn_s_sleep(2);
@@ -1732,42 +1738,54 @@ __device__ __forceinline__ float dpp_shift_in(float v, float edge) {
 // slot execute before the consumed store that frees it.  Compiler barriers
 // (no instructions) keep that program order without the waits a fence
 // would add.
-// Across workgroups: every lane's result of every step goes to the step
-// ordered array (forward H, backward H2) with system-scope stores; the
-// producers of the neighbouring block read the edge lane's entries there,
-// waiting while they still hold the empty sentinel the prep stored.
-// Forward, block w waits on block w-1; backward, on block w+1; the dispatch
-// order (blockIdx.x) puts the block waited on first, so every wait is on a
-// workgroup already running.
+// Across workgroups: forward, block w waits on block w-1's mail; backward,
+// on block w+1's.  A workgroup takes its block from a ticket counter of the
+// instance (one atomic add at its start, never reset): ticket n is block
+// n mod W of the instance's sweep number n / W, in the order the workgroups
+// actually started, so the block a workgroup waits on belongs to a
+// workgroup that is already running, whatever order the XCDs dispatch in.
+// Every launch takes exactly W tickets per instance (workgroups x < W take
+// one before anything else), and the sweep number + 1 is the mail tag.
 template <int DIR, int S, int NP, int PD>
 __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __restrict__ Ms) {
   constexpr int NV = LU32_NV;
   constexpr int MAIL_LANE = DIR > 0 ? 0 : 63, TAG_LANE = DIR > 0 ? 1 : 0;
   extern __shared__ __attribute__((aligned(16))) double lds_raw[];
-  __shared__ int consumed, abort_flag, st0;
+  __shared__ int consumed, abort_flag, st0, wb_s;
+  __shared__ unsigned tag_s;
   u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][NV+1][64]
   auto tag_of = [&](int slot) {
     return reinterpret_cast<int*>(ring + ((size_t)slot * (NV + 1) + NV) * 64 + TAG_LANE);
   };
   const DevMesh& M = Ms[blockIdx.y];
   const LUGeom g = lu_geom(M);
+  D2* lu = reinterpret_cast<D2*>(M.lu);
   if (threadIdx.x == 0) {
-    st0 = *M.status;
     consumed = 0;
     abort_flag = 0;
+    wb_s = 0;
+    tag_s = 0;
+    if ((int)blockIdx.x < g.W) {
+      unsigned long long* tk = reinterpret_cast<unsigned long long*>(lu + g.off_tk) + (DIR > 0 ? 0 : 1);
+      const unsigned long long n = __hip_atomic_fetch_add(tk, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      wb_s = (int)(n % (unsigned long long)g.W);
+      tag_s = (unsigned)(n / (unsigned long long)g.W) + 1u;
+    }
+    st0 = *M.status;
   }
   if (threadIdx.x < S) *tag_of(threadIdx.x) = 0;
   __syncthreads();
-  const int wb = blockIdx.x;
-  if (st0 || wb >= g.W) return;
+  if (st0 || (int)blockIdx.x >= g.W) return;
+  const int wb = wb_s;
+  const unsigned tag = tag_s;
   const int w = DIR > 0 ? wb : g.W - 1 - wb;   // row block of this workgroup
   const int NT = g.TW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
-  D2* lu = reinterpret_cast<D2*>(M.lu);
   // step-ordered result arrays: forward H (row NT-1-t for step t), backward
   // H2 (row t); the backward producers read H
   const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
   const __amdgpu_buffer_rsrc_t hout = DIR > 0 ? hnd : uniform_rsrc(lu + g.off_h2, (long long)g.W * g.TW * 64 * 16);
+  gu64* const mail = (gu64*)(lu + (DIR > 0 ? g.off_mf : g.off_mb));
   const int base = w * NT;
   if (wave > 0) {
     // ---------------------------------------------------------- producers
@@ -1783,20 +1801,18 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
     const int krow = 64 * w + l;
     const bool rowin = krow < g.nk;
     // the neighbour block's edge value for this block's fill lane at step
-    // tt: forward, lane 63 of block w-1 at column j = js+tt (its step tt+63,
-    // H row ni-1-tt); backward, lane 0 of block w+1 at column j = je-tt
-    // (its step tt+63, H2 row tt+63).  At the grid edge (no neighbour) the
-    // pointer is a valid entry that is read and ignored.
+    // tt: forward, lane 63 of block w-1 at column j = js+tt (its step tt+63);
+    // backward, lane 0 of block w+1 at column j = je-tt (its step tt+63).
+    // At the grid edge (no neighbour) the entry read is a valid one that is
+    // ignored.
     const bool has_in = DIR > 0 ? w > 0 : w < g.W - 1;
     const int wn = has_in ? w - DIR : w;
-    gu32* mail_base = (gu32*)(lu + (DIR > 0 ? g.off_h : g.off_h2));
-    auto mail_ptr = [&](int tt) {
+    auto mail_ptr = [&](int tt) -> const gu64* {
       const int c = tt < 0 ? 0 : (tt >= g.ni ? g.ni - 1 : tt);
-      const long long row = (long long)wn * NT + (DIR > 0 ? g.ni - 1 - c : c + 63);
-      return mail_base + (row * 64 + (DIR > 0 ? 63 : 0)) * 4;
+      return mail + ((long long)wn * NT + c + 63) * 4;
     };
-    constexpr int NB = NV + 1 + (DIR < 0 ? 1 : 0);   // backward: + cB
-    auto fetch = [&](int tt, u32x4_t (&v)[NB]) {
+    constexpr int NB = NV + (DIR < 0 ? 1 : 0);   // backward: + cB
+    auto fetch = [&](int tt, u32x4_t (&v)[NB], Mail& m) {
       const int tc = tt < NT ? tt : NT - 1;
       const int jr = DIR > 0 ? tc - l : (g.ni - 1) - tc + (63 - l);
       const bool cin = rowin && (unsigned)jr < (unsigned)g.ni;
@@ -1812,48 +1828,45 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
         // cB of the cell, kept at its forward step NT-1-tc; zero outside
         // the grid (those entries are never written)
         const u32x4_t cb = __builtin_amdgcn_raw_buffer_load_b128(cbr, l * 16, (base + NT - 1 - tc) * 1024, 0);
-        v[NV + 1] = cin ? cb : u32x4_t{0u, 0u, 0u, 0u};
+        v[NV] = cin ? cb : u32x4_t{0u, 0u, 0u, 0u};
       }
-      gu32* p = mail_ptr(tc);
-      v[NV] = u32x4_t{__hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-                      __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-                      __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-                      __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)};
+      mail_load(mail_ptr(tc), m);
     };
-    auto deliver = [&](int tt, u32x4_t (&v)[NB]) -> bool {
+    auto deliver = [&](int tt, u32x4_t (&v)[NB], Mail& m) -> bool {
       if (tt >= NT) return true;
+      u32x4_t fill = u32x4_t{0u, 0u, 0u, 0u};
       if (has_in && tt < g.ni) {
         // written some 64 steps earlier, the entry is normally already there
-        if (v[NV].x == LU32_EMPTY || v[NV].y == LU32_EMPTY || v[NV].z == LU32_EMPTY || v[NV].w == LU32_EMPTY)
-          if (!wait_mail(mail_ptr(tt), v[NV], &abort_flag)) return false;
-      } else {
-        v[NV] = u32x4_t{0u, 0u, 0u, 0u};
+        if (!mail_ok(m, tag))
+          if (!wait_mail(mail_ptr(tt), tag, m, &abort_flag)) return false;
+        fill = mail_value(m);
       }
       if constexpr (DIR < 0) {
-        const float4 h = f4_of(v[0]), cb = f4_of(v[NV + 1]);
+        const float4 h = f4_of(v[0]), cb = f4_of(v[NV]);
         v[0] = __builtin_bit_cast(u32x4_t, F4{h.x + cb.x, h.y + cb.y, h.z + cb.z, h.w + cb.w});
       }
       if (!spin_until_ge<4>(&consumed, tt - S + 1, &abort_flag)) return false;
       u32x4_t* d = ring + (size_t)(tt % S) * (NV + 1) * 64;
 #pragma unroll
       for (int q = 0; q < NV; ++q) d[q * 64 + l] = v[q];
-      if (l == 0) d[NV * 64 + MAIL_LANE] = v[NV];
+      if (l == 0) d[NV * 64 + MAIL_LANE] = fill;
       if (l == 0) __hip_atomic_store(tag_of(tt % S), tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       return true;
     };
     // producer wave p handles steps p-1, p-1+NP, ...; buffer k % PD holds the
     // k-th of them, fetched PD-1 deliveries ahead
     u32x4_t vb[PD][NB];
+    Mail mb[PD];
     const int t0 = wave - 1;
 #pragma unroll
-    for (int k = 0; k < PD - 1; ++k) fetch(t0 + k * NP, vb[k]);
+    for (int k = 0; k < PD - 1; ++k) fetch(t0 + k * NP, vb[k], mb[k]);
     bool ok = true;
     for (int tb = t0; tb < NT && ok; tb += PD * NP) {
 #pragma unroll
       for (int k = 0; k < PD; ++k) {
         const int tt = tb + k * NP;
-        fetch(tt + (PD - 1) * NP, vb[(k + PD - 1) % PD]);
-        if (ok) ok = deliver(tt, vb[k]);
+        fetch(tt + (PD - 1) * NP, vb[(k + PD - 1) % PD], mb[(k + PD - 1) % PD]);
+        if (ok) ok = deliver(tt, vb[k], mb[k]);
       }
     }
     return;
@@ -1865,6 +1878,12 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   // matrices (the halo terms are folded into u and cB): no lane needs a
   // select.
   __builtin_amdgcn_s_setprio(3);
+  // the edge lane's results are the neighbouring block's mail (entries
+  // before step 63 are never read)
+  const bool has_out = DIR > 0 ? w < g.W - 1 : w > 0;
+  const bool mail_lane = has_out && l == (DIR > 0 ? 63 : 0);
+  gu64* const mail_out = mail + (long long)w * NT * 4;
+  const unsigned long long tg = (unsigned long long)tag << 32;
   float sA[4] = {0.f, 0.f, 0.f, 0.f};
   // k neighbour leaving this lane; the wavefront shift runs with every lane
   // enabled (a disabled source lane reads as the fill value)
@@ -1893,12 +1912,12 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
     asm volatile("" ::: "memory");
     __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     rd(s1, N);   // past the last step: a harmless read of a stale slot
-    const float4 mail = f4_of(C[NV]);
+    const float4 fill = f4_of(C[NV]);
     float kin[4];
-    kin[0] = dpp_shift_in<DIR>(kout[0], mail.x);
-    kin[1] = dpp_shift_in<DIR>(kout[1], mail.y);
-    kin[2] = dpp_shift_in<DIR>(kout[2], mail.z);
-    kin[3] = dpp_shift_in<DIR>(kout[3], mail.w);
+    kin[0] = dpp_shift_in<DIR>(kout[0], fill.x);
+    kin[1] = dpp_shift_in<DIR>(kout[1], fill.y);
+    kin[2] = dpp_shift_in<DIR>(kout[2], fill.z);
+    kin[3] = dpp_shift_in<DIR>(kout[3], fill.w);
     const float4 u = f4_of(C[0]);
     float sn[4];
 #pragma unroll
@@ -1914,12 +1933,18 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
       p = __builtin_fmaf(b.z, kin[2], p);
       sn[q] = __builtin_fmaf(b.w, kin[3], p);
     }
-    // the result of (step, lane), system scope: the edge lane's entry is
-    // the neighbouring block's mail; entries of cells outside the grid are
-    // never read as results
+    // the result of (step, lane); entries of cells outside the grid are
+    // never read as results.  Later kernels read it.
     const int orow = DIR > 0 ? NT - 1 - t : t;
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}), hout, l * 16,
-                                           (base + orow) * 1024, LU32_SYS);
+                                           (base + orow) * 1024, 0);
+    if (mail_lane) {
+      gu64* p = mail_out + (long long)t * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        __hip_atomic_store(p + q, tg | (unsigned long long)__builtin_bit_cast(unsigned, sn[q]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       sA[q] = sn[q];

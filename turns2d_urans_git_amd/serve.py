@@ -96,7 +96,6 @@ def device_memory():
 class _Cohort:
     def __init__(self, lib, reqs: List[Request], tol: float, errfn):
         self.reqs = reqs
-        self.fp32 = any(int(r.deck.get("lu_fp32", 0)) for r in reqs)
         params = []
         for r in reqs:
             p = abi.Params()
@@ -185,15 +184,10 @@ class Server:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
-            # ---- one tick: every cohort advances, overlapping on the device.
-            # Cohorts with FP32 LU-SGS run one after another: their sweeps
-            # overlapping other cohorts' kernels have now and then corrupted
-            # a case, which FP64 LU-SGS cohorts have not.
-            serial = len(cohorts) > 1 and any(c.fp32 for c in cohorts)
+            # ---- one tick: every cohort advances, their kernels overlapping
+            # on the device (each cohort has its own stream)
             for c in cohorts:
                 c.case.iterate_async(self.k)
-                if serial:
-                    c.case.sync()
             self.ticks += 1
             done_cohorts = []
             for c in cohorts:
