@@ -191,7 +191,10 @@ int32_t ot2d_case_bytes(ot2d_case* c, int64_t* bytes);
  * case whose rsum is not a finite number stops with OT2D_NORM. */
 int32_t ot2d_set_convergence(ot2d_case* c, int32_t icase, double rsum_tol);
 /* Enqueues niter steps on the case's stream and returns without waiting
- * (no residual history); ot2d_sync or ot2d_batch_status waits. */
+ * (no residual history); ot2d_sync or ot2d_batch_status waits.  Cases on
+ * different streams overlap on the device, except that the steps of cases
+ * with lu_fp32 = 1 are ordered across such cases on one device (each
+ * waits for the FP32 work enqueued before it by any case). */
 int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter);
 int32_t ot2d_sync(ot2d_case* c);
 

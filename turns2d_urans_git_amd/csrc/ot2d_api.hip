@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -108,6 +109,57 @@ struct ot2d_ihc_lists {
   ot2d::ihc::Result r;
   std::vector<int> jmax, kmax;
 };
+
+// ------------------------------------------------ order of FP32 LU-SGS work
+// Cases whose LU-SGS sweeps run in FP32 (lu_fp32) never run at the same time
+// as one another on a device: every call that enqueues such a case's sweeps
+// first makes the case's stream wait for the FP32 work last enqueued by any
+// case on that device, and records the end of its own.  FP32 batches
+// overlapping one another on several streams have diverged from their
+// one-stream results now and then; the cause is not shown (it is not the
+// sweeps' inter-workgroup mail, whose entries carry the sweep number), so
+// the library keeps such work apart.  FP64 cases still overlap freely.
+namespace {
+constexpr int kMaxDevices = 64;
+struct Fp32Chain {
+  std::mutex mu;
+  hipEvent_t ev[kMaxDevices] = {};
+};
+Fp32Chain g_fp32;
+
+// held while one FP32 case enqueues its work
+class Fp32Order {
+ public:
+  explicit Fp32Order(const ot2d_case* c);
+  ~Fp32Order();
+  hipError_t error() const { return e_; }
+
+ private:
+  const ot2d_case* c_ = nullptr;
+  hipError_t e_ = hipSuccess;
+};
+}  // namespace
+
+Fp32Order::Fp32Order(const ot2d_case* c) {
+  if (!c->p.lu_fp32) return;
+  c_ = c;
+  g_fp32.mu.lock();
+  e_ = hipStreamWaitEvent(c->stream, g_fp32.ev[c->device], 0);
+}
+
+Fp32Order::~Fp32Order() {
+  if (!c_) return;
+  (void)hipEventRecord(g_fp32.ev[c_->device], c_->stream);
+  g_fp32.mu.unlock();
+}
+
+// the device's ordering event, created with the first FP32 case on it
+static int32_t fp32_chain_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(OT2D_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_fp32.mu);
+  if (!g_fp32.ev[dev]) HIPCHK(hipEventCreateWithFlags(&g_fp32.ev[dev], hipEventDisableTiming));
+  return OT2D_OK;
+}
 
 // ------------------------------------------------------------- constants
 extern "C" int32_t ot2d_default_params(ot2d_params* p) {
@@ -343,6 +395,8 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
   (void)hipGetDevice(&c->device);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { free_case(c); return fail(OT2D_ERR_HIP, "stream: %s", hipGetErrorString(e)); }
+  if (p->lu_fp32)
+    if (int32_t rc = fp32_chain_init(c->device)) { free_case(c); return rc; }
   int ns = c->nspec;
   c->m.resize(nmesh);
   int nmi = nmesh * ns;
@@ -825,8 +879,12 @@ extern "C" int32_t ot2d_rbc(ot2d_case* c) {
 
 extern "C" int32_t ot2d_lhs(ot2d_case* c) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
-  if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, nullptr, c->stream));
+  {
+    Fp32Order order(c);
+    HIPCHK(order.error());
+    if (int32_t rc = clear_status(c)) return rc;
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, nullptr, c->stream));
+  }
   return sync(c);
 }
 
@@ -908,9 +966,13 @@ extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
-  if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
-  if (int32_t rc = enqueue_step(c)) return rc;
+  {
+    Fp32Order order(c);
+    HIPCHK(order.error());
+    if (int32_t rc = clear_status(c)) return rc;
+    HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
+    if (int32_t rc = enqueue_step(c)) return rc;
+  }
   int st = 0;
   if (int32_t rc = read_status(c, &st)) return rc;
   std::vector<double> nb(6 * nmi(c));
@@ -977,10 +1039,14 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
     }
     HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * per, hipMemcpyHostToDevice, c->stream));
   }
-  if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
-  if (int32_t rc = ensure_graph(c)) return rc;
-  for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  {
+    Fp32Order order(c);
+    HIPCHK(order.error());
+    if (int32_t rc = clear_status(c)) return rc;
+    HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
+    if (int32_t rc = ensure_graph(c)) return rc;
+    for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+  }
   int st = 0, steps = 0;
   if (int32_t rc = read_steps(c, &steps)) return rc;
   if (int32_t rc = read_status(c, &st)) return rc;
@@ -1004,6 +1070,8 @@ extern "C" int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter) {
     for (auto& M : c->hM) { M.hist = nullptr; M.hist_cap = 0; }
     HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice, c->stream));
   }
+  Fp32Order order(c);
+  HIPCHK(order.error());
   if (int32_t rc = clear_status(c)) return rc;
   if (int32_t rc = ensure_graph(c)) return rc;
   for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
@@ -1054,6 +1122,8 @@ extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, 
     rc = fail(OT2D_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
     --created;
   } else if ((rc = clear_status(c)) == OT2D_OK) {
+    Fp32Order order(c);
+    if (order.error() != hipSuccess) rc = fail(OT2D_ERR_HIP, "hipStreamWaitEvent: %s", hipGetErrorString(order.error()));
     e = hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream);
     if (e != hipSuccess) rc = fail(OT2D_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
     for (int it = 0; it < nsteps && rc == OT2D_OK; ++it) {

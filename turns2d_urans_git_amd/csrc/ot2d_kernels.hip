@@ -1513,8 +1513,8 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // backward s_fwd + cB, PA' rows, PB' rows (one more vector carries the
 // mailbox value).
 constexpr int LU32_NV = 9;
-// cache-policy operand (sc0 and sc1 bits: system scope) of the mail stores
-// and loads, which cross XCDs within one kernel
+// cache-policy operand (sc0 and sc1 bits: system scope) of the result
+// stores of the sweeps
 constexpr int LU32_SYS = 17;
 
 // rows of the 4x4 matrix M with a = M . s for a neighbour's A(+/-) or
@@ -1933,11 +1933,13 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
       p = __builtin_fmaf(b.z, kin[2], p);
       sn[q] = __builtin_fmaf(b.w, kin[3], p);
     }
-    // the result of (step, lane); entries of cells outside the grid are
-    // never read as results.  Later kernels read it.
+    // the result of (step, lane), read by the next kernels; entries of
+    // cells outside the grid are never read as results.  Written through to
+    // memory (system scope): with plain stores, FP32 batches overlapping
+    // other FP32 batches on other streams diverged more often.
     const int orow = DIR > 0 ? NT - 1 - t : t;
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}), hout, l * 16,
-                                           (base + orow) * 1024, 0);
+                                           (base + orow) * 1024, LU32_SYS);
     if (mail_lane) {
       gu64* p = mail_out + (long long)t * 4;
 #pragma unroll
