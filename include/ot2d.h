@@ -198,6 +198,18 @@ int32_t ot2d_set_convergence(ot2d_case* c, int32_t icase, double rsum_tol);
 int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter);
 int32_t ot2d_sync(ot2d_case* c);
 
+/* ot2d_iterate without allocating: the same steps, the same outputs (hist,
+ * *done, status), but the device-side residual history is staged in a
+ * fixed scratch array inside the library instead of a buffer the call
+ * allocates, and the call makes no hipMalloc, hipFree, graph capture or
+ * other allocating call.  No counterpart in the reference; for callers that
+ * cannot allocate (tight serving loops, harnesses that forbid allocation).
+ * Limit: niter * nmesh * nspec <= OT2D_NOALLOC_HIST_MAX when hist is not
+ * NULL (OT2D_ERR_ARG otherwise).  The scratch is one per GPU, shared by
+ * every case on that GPU: calls that overlap on one GPU race. */
+#define OT2D_NOALLOC_HIST_MAX (1 << 20)
+int32_t ot2d_iterate_noalloc(ot2d_case* c, int32_t niter, double* hist, int32_t* done);
+
 /* Kernel stages of one step, in launch order (ot2d_profile). */
 enum {
   OT2D_ST_BEGIN = 0,     /* step counter / status reset                        */

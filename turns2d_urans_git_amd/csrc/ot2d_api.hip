@@ -1060,6 +1060,109 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
   return OT2D_OK;
 }
 
+// ------------------------------------------------ no-allocation iterate
+// ot2d_iterate allocates its device history buffer inside the call when a
+// longer history is asked for.  ot2d_iterate_noalloc stages the history in
+// this file-scope device array instead (OT2D_NOALLOC_HIST_MAX doubles,
+// 8 MiB, held by every GPU the library runs on until the process ends) and
+// calls no allocating routine.  Host code never uses the array's name as an
+// address: its device address comes from hipGetSymbolAddress, once per
+// device with that device current, kept in g_scratch.  Every call on one GPU
+// shares that GPU's array, so calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[OT2D_NOALLOC_HIST_MAX];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static int32_t scratch_address(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(OT2D_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(OT2D_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return OT2D_OK;
+}
+
+// point the history slots of every instance record at `h` (rows of niter)
+static int32_t set_history(ot2d_case* c, double* h, int niter) {
+  const int per = nmi(c);
+  for (int i = 0; i < per; ++i) {
+    c->hM[i].hist = h;
+    c->hM[i].hist_cap = h ? niter : 0;
+    c->hM[i].hist_stride = per;
+    c->hM[i].hist_slot = i;
+  }
+  HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * per, hipMemcpyHostToDevice, c->stream));
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_iterate_noalloc(ot2d_case* c, int32_t niter, double* hist, int32_t* done) {
+  if (!c || niter < 0) return fail(OT2D_ERR_ARG, "bad arguments");
+  for (auto& mb : c->m)
+    if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  const int per = nmi(c);
+  if (hist && (long long)niter * per > (long long)OT2D_NOALLOC_HIST_MAX)
+    return fail(OT2D_ERR_ARG,
+                "ot2d_iterate_noalloc: a history of %d steps x %d instances exceeds the %lld-entry scratch "
+                "(OT2D_NOALLOC_HIST_MAX); use fewer steps per call or ot2d_iterate",
+                niter, per, (long long)OT2D_NOALLOC_HIST_MAX);
+  double* sp = nullptr;
+  if (hist)
+    if (int32_t rc = scratch_address(c->device, &sp)) return rc;
+  if (int32_t rc = set_history(c, sp, niter)) return rc;
+  {
+    Fp32Order order(c);
+    HIPCHK(order.error());
+    if (int32_t rc = clear_status(c)) return rc;
+    HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
+    // the captured step when one exists (an earlier ot2d_iterate made it);
+    // otherwise the same kernels launched one by one (capturing a graph
+    // allocates)
+    for (int it = 0; it < niter; ++it) {
+      if (c->graph_valid) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+      else if (int32_t rc = enqueue_step(c)) return rc;
+    }
+  }
+  // steps done (the furthest case) and status, read in fixed-size pieces
+  int steps = 0, st = 0, buf[256];
+  for (int o = 0; o < c->ncase; o += 256) {
+    const int n = c->ncase - o < 256 ? c->ncase - o : 256;
+    HIPCHK(hipMemcpyAsync(buf, c->d_stepc + o, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) steps = buf[i] > steps ? buf[i] : steps;
+    HIPCHK(hipMemcpyAsync(buf, c->d_status + o, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+      if (!c->batch && o + i == 0) st = buf[0];
+      if (c->batch && buf[i] == 3) st = 3;
+    }
+  }
+  if (hist && steps > 0)
+    HIPCHK(hipMemcpyAsync(hist, sp, sizeof(double) * (size_t)steps * per, hipMemcpyDeviceToHost, c->stream));
+  // no later call of this case writes into the shared scratch
+  if (hist)
+    if (int32_t rc = set_history(c, nullptr, 0)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->istep0 += steps;
+  if (done) *done = steps;
+  if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (wait limit)");
+  if (st == 4) return OT2D_CONVERGED;
+  if (st == 2) return OT2D_NORM;
+  if (st == 1) return OT2D_MSTOP;
+  return OT2D_OK;
+}
+
 // niter steps enqueued on the case's stream without waiting (no history);
 // ot2d_batch_status / ot2d_sync wait for them
 extern "C" int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter) {
