@@ -106,10 +106,9 @@ def test_concurrent_cohorts_do_not_change_results():
     """Cases advanced on separate streams at the same time give the results
     they give alone, bit for bit (FP32 and FP64 LU-SGS, steady and
     time-spectral cohorts side by side).  These grids keep every FP32 sweep
-    inside one 64-row workgroup; overlapped FP32 sweeps over several
-    workgroups (the inter-workgroup mail) are covered by
-    test_gpu_lu32_multiblock.py and by test_multiblock_fp32_cohorts_overlap
-    below."""
+    inside one workgroup; FP32 sweeps over several workgroups (the
+    inter-workgroup mail) are covered by test_gpu_lu32_multiblock.py and by
+    test_multiblock_fp32_cohorts_overlap below."""
     import turns2d_urans_git_amd as t2d
     lib, err = _lib()
 
@@ -146,9 +145,9 @@ def test_concurrent_cohorts_do_not_change_results():
 
 
 def test_multiblock_fp32_cohorts_overlap():
-    """Served cohorts whose FP32 sweeps span two and three 64-row workgroups
-    (kmax 97 and 161) beside other cohorts, all overlapping on their streams:
-    every request's result equals its result served alone, bit for bit."""
+    """Served cohorts whose FP32 sweeps span several workgroups (kmax 97 to
+    161) beside other cohorts on their own streams: every request's result
+    equals its result served alone, bit for bit."""
     target = _polar(0, 0.0, m=0.03, alfa=3.0, jmax=97, kmax=97)
     _, alone = _serve([target])
     mix = [_polar(1, 0.0, alfa=1.0, jmax=65, kmax=161), _polar(2, 0.0, jmax=81, kmax=129, alfa=-2.0),

@@ -1105,6 +1105,7 @@ __device__ __forceinline__ void nb_k(const CellLoad& c, const LUCellF64& L, doub
 // ----------------------------------------------------------------------
 
 constexpr int LU_NVEC = 10;        // 16-byte planes reserved per array
+constexpr int LU32_R = 32;         // rows of one FP32 sweep workgroup (two lanes per row)
 constexpr int LU_MAXRL = 8;        // rows per lane: at most 512 rows
 
 template <typename T> struct LUV;
@@ -1113,7 +1114,7 @@ template <> struct LUV<double> { static constexpr int N = 10; };
 
 struct LUGeom {
   int ni, nk, RL, NL, T;           // FP64 sweep: NL lanes, T steps
-  int W, TW, TA;                   // FP32 sweep: W workgroups of 64 rows, TW steps each,
+  int W, TW, TA;                   // FP32 sweep: W workgroups of LU32_R rows, TW steps each,
                                    // TA >= TW step rows of the record arrays (a multiple of 8)
   long long nrec;                  // FP64: entries of F and of B
   long long off_b, off_hcf, off_hcb, off_hrf, off_hrb;   // FP64 arrays (16-byte vectors)
@@ -1129,8 +1130,8 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   g.ni = ni; g.nk = nk; g.RL = RL;
   g.NL = (nk + RL - 1) / RL;
   g.T = ni + g.NL - 1;
-  g.W = (nk + 63) / 64;
-  g.TW = ni + 63;
+  g.W = (nk + LU32_R - 1) / LU32_R;
+  g.TW = ni + LU32_R - 1;
   g.TA = (g.TW + 7) & ~7;
   g.nrec = (long long)g.T * RL * 64;
   long long hc = (long long)RL * 64;
@@ -1140,17 +1141,19 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   g.off_hrf = g.off_hcb + LU_NVEC * hc;
   g.off_hrb = g.off_hrf + (long long)LU_NVEC * ni;
   long long t64 = g.off_hrb + (long long)LU_NVEC * ni;
-  // FP32: the records, cB, the forward results and the backward results,
-  // all in step layout (a 1 KiB row of 64 lanes per vector, block and
-  // step); then the mail of each block's edge lane (one 32-byte entry per
-  // block and step, two 16-byte vectors) and the two ticket counters
+  // FP32, all in step layout (per block and sweep step, a row of 64 lanes):
+  // the forward records (4 vectors of 16 bytes per lane + one 8-byte pair:
+  // 288 16-byte units), the backward records (4 vectors: 256 units), cB,
+  // the forward and the backward results (8 bytes per lane: 32 units each);
+  // then the mail of each block's edge row (one 32-byte entry per block and
+  // step) and the two ticket counters
   g.ncell = (long long)ni * nk;
   g.off_f32 = 0;
-  g.off_b32 = g.off_f32 + 9ll * g.W * g.TA * 64;
-  g.off_cb = g.off_b32 + 8ll * g.W * g.TA * 64;
-  g.off_h = g.off_cb + (long long)g.W * g.TW * 64;
-  g.off_h2 = g.off_h + (long long)g.W * g.TW * 64;
-  g.off_mf = g.off_h2 + (long long)g.W * g.TW * 64;
+  g.off_b32 = g.off_f32 + 288ll * g.W * g.TA;
+  g.off_cb = g.off_b32 + 256ll * g.W * g.TA;
+  g.off_h = g.off_cb + 32ll * g.W * g.TW;
+  g.off_h2 = g.off_h + 32ll * g.W * g.TW;
+  g.off_mf = g.off_h2 + 32ll * g.W * g.TW;
   g.off_mb = g.off_mf + 2ll * g.W * g.TW;
   g.off_tk = g.off_mb + 2ll * g.W * g.TW;
   long long t32 = g.off_tk + 1;
@@ -1163,8 +1166,10 @@ __device__ __forceinline__ LUGeom lu_geom(const DevMesh& M) {
 }
 
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 struct D2 { double x, y; };
 struct F4 { float x, y, z, w; };
+struct F2 { float x, y; };
 
 // buffer descriptor built from wave-uniform values (keeps it in SGPRs, so
 // buffer accesses need no waterfall loop)
@@ -1240,14 +1245,15 @@ __device__ __forceinline__ long long lu_entry(const DevMesh& M, const LUGeom& g,
   return ((long long)t * g.RL + r) * 64 + l;
 }
 
-// FP64 records (reference form, see above)
-// entry of interior cell (j,k) in the FP32 hand-off layout: workgroup
-// w = (k-ks)/64 owns rows ks+64w .. ks+64w+63, lane l = (k-ks)%64; forward
-// step t = (j-js) + l, backward step t = (je-j) + (63-l)
+// FP32 results of interior cell (j,k): workgroup w = (k-ks)/32 owns rows
+// ks+32w .. ks+32w+31, row r = (k-ks)%32 on lanes 2r (components 1, 2) and
+// 2r+1 (3, 4); forward step t = (j-js) + r, backward step t = (je-j) +
+// (31-r).  Returns the 16-byte unit holding the cell's four components in a
+// result array (one 8-byte pair per lane, 32 units per block and step).
 __device__ __forceinline__ long long lu_entry32(const DevMesh& M, const LUGeom& g, int j, int k, int dir) {
-  int kk = k - M.ks, w = kk >> 6, l = kk & 63;
-  int t = dir > 0 ? (j - M.js) + l : (M.je - j) + (63 - l);
-  return ((long long)w * g.TW + t) * 64 + l;
+  int kk = k - M.ks, w = kk / LU32_R, r = kk % LU32_R;
+  int t = dir > 0 ? (j - M.js) + r : (M.je - j) + (LU32_R - 1 - r);
+  return ((long long)w * g.TW + t) * 32 + r;
 }
 
 __global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms) {
@@ -1497,22 +1503,27 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // as 4x4 matrices).  Neighbours in the halo ring hold fixed point-Jacobi
 // values (src/lhs.f:568-598), so their terms are folded into u (forward)
 // and into cB (backward, added by the forward sweep to its output).
-//   F record (9 float4 per cell): u, PA rows 0-3, PB rows 0-3
-//   B record (8 float4 per cell): PA' rows 0-3, PB' rows 0-3
-//   cB (one float4 per cell)
-// all stored by forward step: F vector q of lane l at step t of block w is
-// float4 ((w*TA + t)*9 + q)*64 + l, so a producer reads a step's vector as
-// one coalesced 1 KiB row; B is kept at the cell's forward step too (the
-// backward sweep reaches it at step NT-1-t).
-//   H (one float4 per cell in sweep-step order): s_fwd written by the
-//     forward sweep (the backward producers add cB to it)
-//   H2 (same order): the result, written by the backward sweep
+//
+// A workgroup sweeps a block of LU32_R = 32 rows.  Lane l = 2r + h holds
+// row r and components 2h, 2h+1 ("its pair"); lane l^1 holds the other
+// pair of the same row.  Lane l's view of a row's four components is
+// (its pair, the other pair), so the records store each matrix row's
+// columns in that order and no lane needs a select.  Per lane and sweep
+// step:
+//   forward record  4 rows of 4 floats (PA rows 2h, 2h+1; PB rows 2h,
+//                   2h+1) and the pair of u
+//   backward record 4 rows (PA' rows, PB' rows); cB and the forward result
+//                   as pairs
+// all stored by forward step: for block w and step t the forward record is
+// 16-byte units ((w*TA + t)*288 + q*64 + l) for rows q = 0..3 and bytes
+// 4096 + 8l after the record's start for u, so a producer reads a step as
+// four coalesced 1 KiB rows and one 512-byte row; the backward record (256
+// units) is kept at the cell's forward step too.
+//   H  (one pair per lane and step): s_fwd, written by the forward sweep
+//   H2 (the same): the result, written by the backward sweep
 // The halo-ring s values themselves are scaled in k_lu_finish (the prep
 // reads the unscaled neighbours).
-// Vectors per sweep step in an LDS ring slot: forward u, PA rows, PB rows;
-// backward s_fwd + cB, PA' rows, PB' rows (one more vector carries the
-// mailbox value).
-constexpr int LU32_NV = 9;
+
 // cache-policy operand (sc0 and sc1 bits: system scope) of the result
 // stores of the sweeps
 constexpr int LU32_SYS = 17;
@@ -1532,13 +1543,13 @@ __device__ __forceinline__ void nb_matrix(const LUNb& n, double gm1, double m[16
   for (int w = 0; w < 4; ++w) m[w * 4 + w] += n.spec;
 }
 
-// Prep tiles.  A workgroup of LP_WAVES waves covers 8 rows of a 64-row
-// block and LP_WAVES*8 consecutive sweep steps; row l at step t is cell
-// jr = t - l, so its entries lie in a parallelogram of cells whose bounding
-// box, with the one-cell ring the neighbour terms reach, is LP_RJ x LP_RK
-// cells.  The per-cell quantities of ilu2d (src/lhs.f:524-564) are formed
-// once per cell of that box in LDS; each entry then assembles its operator
-// from the cell and its four neighbours there.
+// Prep tiles.  A workgroup of LP_WAVES waves covers 8 rows of a block and
+// LP_WAVES*8 consecutive sweep steps; row r at step t is cell jr = t - r,
+// so its entries lie in a parallelogram of cells whose bounding box, with
+// the one-cell ring the neighbour terms reach, is LP_RJ x LP_RK cells.  The
+// per-cell quantities of ilu2d (src/lhs.f:524-564) are formed once per cell
+// of that box in LDS; each cell then assembles its operators from itself and
+// its four neighbours there.
 constexpr int LP_WAVES = 4, LP_STEPS = 8 * LP_WAVES;
 constexpr int LP_RJ = LP_STEPS + 7 + 2, LP_RK = 8 + 2, LP_N = LP_RJ * LP_RK;
 
@@ -1550,7 +1561,7 @@ struct LUPrepLds {
 };
 constexpr size_t LP_LDS = sizeof(LUPrepLds);
 
-// neighbour term of an entry from the cell at LDS index n: matrix scaled by
+// neighbour term of a cell from the cell at LDS index n: matrix scaled by
 // f, or folded into c when the neighbour is a halo-ring cell (its s after
 // s*tscale and point Jacobi)
 __device__ __forceinline__ void lu32_term(const LUPrepLds& S, int n, bool halo, int jdir, int dir, double f,
@@ -1574,6 +1585,13 @@ __device__ __forceinline__ void lu32_term(const LUPrepLds& S, int n, bool halo, 
   }
 }
 
+// matrix row `row` of P with its columns in lane pair h's order: its own
+// pair (2h, 2h+1) first, then the other pair
+__device__ __forceinline__ F4 pair_row(const double* P, int row, int h) {
+  const int o = 2 * h, p = 2 - 2 * h;
+  return F4{(float)P[row * 4 + o], (float)P[row * 4 + o + 1], (float)P[row * 4 + p], (float)P[row * 4 + p + 1]};
+}
+
 __global__ __launch_bounds__(64 * LP_WAVES) void k_lu_prep32(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
@@ -1581,18 +1599,17 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_lu_prep32(const DevMesh* __re
   const double eps2 = M.c.epse * 2.;
   const double epsv = 1. + eps2;
   const LUGeom g = lu_geom(M);
-  F4* recf = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_f32);
-  F4* recb = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_b32);
-  F4* cbs = reinterpret_cast<F4*>(reinterpret_cast<D2*>(M.lu) + g.off_cb);
+  D2* lu = reinterpret_cast<D2*>(M.lu);
+  constexpr int RG = LU32_R / 8;   // 8-row groups per block
   const int nthr = 64 * LP_WAVES;
   // tile of this workgroup: block w, rows 8*lb .. 8*lb+7, steps T0 .. T0+LP_STEPS-1
   const int tiles_t = (g.TA + LP_STEPS - 1) / LP_STEPS;
-  if ((int)blockIdx.x >= g.W * 8 * tiles_t) return;
+  if ((int)blockIdx.x >= g.W * RG * tiles_t) return;
   const int tb = (int)blockIdx.x % tiles_t, rest = (int)blockIdx.x / tiles_t;
-  const int lb = rest & 7, w = rest >> 3;
+  const int lb = rest % RG, w = rest / RG;
   const int T0 = tb * LP_STEPS, L0 = lb * 8;
-  // box origin: cell (jr, kk) = (T0 - L0 - 8, 64w + L0 - 1) is LDS cell 0
-  const int jr0 = T0 - L0 - 8, kk0 = 64 * w + L0 - 1;
+  // box origin: cell (jr, kk) = (T0 - L0 - 8, LU32_R*w + L0 - 1) is LDS cell 0
+  const int jr0 = T0 - L0 - 8, kk0 = LU32_R * w + L0 - 1;
   extern __shared__ __attribute__((aligned(16))) double lp_raw[];
   LUPrepLds& S = *reinterpret_cast<LUPrepLds*>(lp_raw);
   for (int n = threadIdx.x; n < LP_N; n += nthr) {
@@ -1609,18 +1626,16 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_lu_prep32(const DevMesh* __re
     for (int q = 0; q < 4; ++q) S.sh[q][n] = sv[q] * c.ts * L.d;
   }
   __syncthreads();
-  // one thread per (step, row) entry: 8 consecutive lanes are 8 consecutive
-  // rows at one step, so each vector of a step row is stored as whole
-  // 128-byte lines.  Entries outside the grid (rows past nk, steps before a
-  // row's first cell or after its last) keep zero records: the workspace is
-  // zero-filled when the case is created and nothing else writes there.
+  // one thread per (step, row) cell, writing the records of the row's two
+  // lanes: 8 consecutive threads are 8 consecutive rows at one step, so
+  // each record row of a step is stored as whole 256-byte runs.  Cells
+  // outside the grid (rows past nk, steps before a row's first cell or
+  // after its last) keep zero records: the workspace is zero-filled when
+  // the case is created and nothing else writes there.
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int l = L0 + (lane & 7), t = T0 + 8 * wave + (lane >> 3);
-  const int kk = 64 * w + l, jr = t - l;
+  const int r = L0 + (lane & 7), t = T0 + 8 * wave + (lane >> 3);
+  const int kk = LU32_R * w + r, jr = t - r;
   if (t >= g.TA || kk >= g.nk || jr < 0 || jr >= g.ni) return;
-  F4* rf = recf + ((long long)(w * g.TA + t) * 9) * 64 + l;
-  F4* rb = recb + ((long long)(w * g.TA + t) * 8) * 64 + l;
-  auto f4 = [](const double* v) { return F4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]}; };
   const int j = js + jr, k = ks + kk;
   const int n0 = (kk - kk0) * LP_RJ + (jr - jr0);
   const double di = S.d[n0], f = (S.ts[n0] * 0.5) * di;
@@ -1631,10 +1646,25 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_lu_prep32(const DevMesh* __re
   lu32_term(S, n0 - LP_RJ, k == ks, 0, +1, f, eps2, gm1, PB, u);
   lu32_term(S, n0 + 1, j == je, 1, -1, -f, eps2, gm1, QA, cb);
   lu32_term(S, n0 + LP_RJ, k == ke, 0, -1, -f, eps2, gm1, QB, cb);
-  rf[0] = f4(u);
-  for (int r = 0; r < 4; ++r) { rf[(1 + r) * 64] = f4(PA + 4 * r); rf[(5 + r) * 64] = f4(PB + 4 * r); }
-  for (int r = 0; r < 4; ++r) { rb[r * 64] = f4(QA + 4 * r); rb[(4 + r) * 64] = f4(QB + 4 * r); }
-  cbs[((long long)w * g.TW + t) * 64 + l] = f4(cb);
+  const long long rec = (long long)w * g.TA + t;
+  F4* rf = reinterpret_cast<F4*>(lu + g.off_f32 + rec * 288);
+  F4* rb = reinterpret_cast<F4*>(lu + g.off_b32 + rec * 256);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int l = 2 * r + h;
+    rf[0 * 64 + l] = pair_row(PA, 2 * h, h);
+    rf[1 * 64 + l] = pair_row(PA, 2 * h + 1, h);
+    rf[2 * 64 + l] = pair_row(PB, 2 * h, h);
+    rf[3 * 64 + l] = pair_row(PB, 2 * h + 1, h);
+    rb[0 * 64 + l] = pair_row(QA, 2 * h, h);
+    rb[1 * 64 + l] = pair_row(QA, 2 * h + 1, h);
+    rb[2 * 64 + l] = pair_row(QB, 2 * h, h);
+    rb[3 * 64 + l] = pair_row(QB, 2 * h + 1, h);
+  }
+  // u of lanes 2r, 2r+1 and cB of the cell (both lanes' pairs, one unit)
+  rf[256 + r] = F4{(float)u[0], (float)u[1], (float)u[2], (float)u[3]};
+  reinterpret_cast<F4*>(lu + g.off_cb)[((long long)w * g.TW + t) * 32 + r] =
+      F4{(float)cb[0], (float)cb[1], (float)cb[2], (float)cb[3]};
 }
 
 // Bounded waits.  Every wait of the FP32 sweeps gives up after LU32_WAIT
@@ -1660,84 +1690,63 @@ __device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_fla
   return false;
 }
 
-// Inter-workgroup mail.  The edge lane of block w stores its result of
+// Inter-workgroup mail.  The edge row of block w stores its result of
 // sweep step t as entry w*TW + t of its direction's mail array: four 8-byte
 // words, word q holding component q's bits in the low half and the sweep's
-// tag in the high half, each written by one 8-byte atomic store.  A reader
-// accepts an entry only when all four words, each read by one 8-byte atomic
-// load, carry the running sweep's tag.  An entry left by an earlier sweep
-// (an older tag) or one still being written (an older tag in some word) is
+// tag in the high half, each written by one aligned 8-byte store (a
+// buffer store with the system-scope cache policy, the instruction an
+// 8-byte relaxed system-scope atomic store compiles to).  A reader accepts
+// an entry only when all four words, each read by one 8-byte atomic load,
+// carry the running sweep's tag.  An entry left by an earlier sweep (an
+// older tag) or one still being written (an older tag in some word) is
 // therefore never taken, and nothing else writes the mail: it needs no
-// reset between sweeps.  Accesses go through global (not generic) pointers:
+// reset between sweeps.  Loads go through global (not generic) pointers:
 // flat instructions would also count against the LDS wait counter and stall
 // the LDS hand-offs behind them.  System scope: writer and reader may sit on
 // different XCDs.
 typedef __attribute__((address_space(1))) unsigned long long gu64;
-
-struct Mail {
-  unsigned long long w[4];
-};
-
-__device__ __forceinline__ void mail_load(const gu64* p, Mail& m) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) m.w[q] = __hip_atomic_load(p + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-__device__ __forceinline__ bool mail_ok(const Mail& m, unsigned tag) {
-  return (unsigned)(m.w[0] >> 32) == tag && (unsigned)(m.w[1] >> 32) == tag && (unsigned)(m.w[2] >> 32) == tag &&
-         (unsigned)(m.w[3] >> 32) == tag;
-}
-
-__device__ __forceinline__ u32x4_t mail_value(const Mail& m) {
-  return u32x4_t{(unsigned)m.w[0], (unsigned)m.w[1], (unsigned)m.w[2], (unsigned)m.w[3]};
-}
-
-// wait until the entry at p carries the running sweep's tag
-__device__ __forceinline__ bool wait_mail(const gu64* p, unsigned tag, Mail& m, int* abort_flag) {
-  const unsigned long long t0 = rt_now();
-  for (;;) {
-    mail_load(p, m);
-    if (mail_ok(m, tag)) return true;
-    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
-    if (rt_now() - t0 > LU32_WAIT) break;
-    __builtin_amdgcn_s_sleep(2);
-  }
-  __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  return false;
-}
-
 __device__ __forceinline__ float4 f4_of(u32x4_t v) { return __builtin_bit_cast(float4, v); }
 
-// DPP wavefront shift whose edge lane (lane 0 for DIR > 0, lane 63 for
-// DIR < 0) receives `edge` instead
+// DPP moves: the other pair of the same row (lane l^1), and the row two
+// lanes up (DIR > 0: lane l-2) or down (DIR < 0: lane l+2) by two
+// wavefront shifts of one lane (lanes shifted in at the edge read zero)
+__device__ __forceinline__ float dpp_pair(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xf, 0xf, true));
+}
 template <int DIR>
-__device__ __forceinline__ float dpp_shift_in(float v, float edge) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v),
-                                                               DIR > 0 ? 0x138 : 0x130, 0xf, 0xf, false));
+__device__ __forceinline__ float dpp_row(float v) {
+  constexpr int ctl = DIR > 0 ? 0x138 : 0x130;   // wave_shr:1 / wave_shl:1
+  const int a = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctl, 0xf, 0xf, true);
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(a, ctl, 0xf, 0xf, true));
 }
 
-// One FP32 sweep.  The rows of an instance are cut into W blocks of 64;
-// block w is one workgroup (on its own CU), lane l owning row ks+64w+l and
-// walking along j one column per step, lane l one step behind lane l-1.
-// Inside the workgroup, wave 0 computes and waves 1..NP stream the records
-// of step t into LDS slot t % S; each producer wave keeps the loads of PD
-// of its steps in flight.  Vector NV of a slot carries, in the lane that
-// takes the wavefront shift's fill value (forward 0, backward 63), the
-// neighbour block's edge value, and in one other lane the slot's tag.
-// Hand-off protocol (LDS, workgroup scope):
-//   tag of slot t % S = t + 1   stored (release) by the producer after the
-//                               records of step t
-//   consumed = c                stored by the computing wave once the reads
-//                               of steps < c have executed, so a producer
+// One FP32 sweep.  The rows of an instance are cut into W blocks of
+// LU32_R; block w is one workgroup, lane l = 2r + h holding row
+// ks + LU32_R*w + r and walking along j one column per step, row r one step
+// behind row r-1 (forward; backward row r one step behind row r+1).
+// Inside the workgroup, wave 0 computes; waves 1..NP stream the records of
+// step t into LDS slot t % S, each keeping the loads of PD of its steps in
+// flight; wave NP+1 reads the neighbour block's mail and puts the fill
+// values into the same slot.  A slot holds four record rows and a fifth
+// row of (u pair, fill pair) per lane: the producer writes the u half, the
+// mail wave the fill half (meaningful in the edge row's two lanes, forward
+// 0-1, backward 62-63: the neighbour block's edge row for the wavefront
+// shift).  Each half has its own tag word.
+// The record producers issue only cached buffer loads; the mail's
+// system-scope polls stay in the mail wave.  A block with no neighbour
+// upstream has zero fills, set up once when the workgroup starts.
+// Hand-off protocol (LDS, workgroup scope; the LDS performs a CU's
+// operations in one order that keeps each wave's issue order):
+//   tags of slot t % S = t + 1  stored (release) by the producer after the
+//                               records and u pair of step t, and by the
+//                               mail wave after the fill pair
+//   consumed = c                stored by the computing wave after it has
+//                               issued the reads of steps < c, so a writer
 //                               may fill slot t % S when consumed >= t-S+1
-// The computing wave reads step t+1's slot while it computes step t and
-// checks the tag when it starts step t+1.  This relies on the LDS
-// executing one wave's operations in issue order: reads issued after the
-// tag's read see the records stored before the tag (the tag is read with
-// them, so a stale tag means the slot is read again), and the reads of a
-// slot execute before the consumed store that frees it.  Compiler barriers
-// (no instructions) keep that program order without the waits a fence
-// would add.
+// The computing wave reads step t+2's slot while it computes step t and
+// checks both tags when it starts step t+2: reads issued after the tags'
+// read see what was stored before them (a stale tag means the slot is read
+// again).
 // Across workgroups: forward, block w waits on block w-1's mail; backward,
 // on block w+1's.  A workgroup takes its block from a ticket counter of the
 // instance (one atomic add at its start, never reset): ticket n is block
@@ -1746,17 +1755,15 @@ __device__ __forceinline__ float dpp_shift_in(float v, float edge) {
 // workgroup that is already running, whatever order the XCDs dispatch in.
 // Every launch takes exactly W tickets per instance (workgroups x < W take
 // one before anything else), and the sweep number + 1 is the mail tag.
+constexpr int LU32_NV = 5;   // 16-byte rows per slot and lane
 template <int DIR, int S, int NP, int PD>
-__global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __restrict__ Ms) {
-  constexpr int NV = LU32_NV;
-  constexpr int MAIL_LANE = DIR > 0 ? 0 : 63, TAG_LANE = DIR > 0 ? 1 : 0;
+__global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __restrict__ Ms) {
+  constexpr int NV = LU32_NV, R = LU32_R;
   extern __shared__ __attribute__((aligned(16))) double lds_raw[];
   __shared__ int consumed, abort_flag, st0, wb_s;
   __shared__ unsigned tag_s;
-  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);   // [S][NV+1][64]
-  auto tag_of = [&](int slot) {
-    return reinterpret_cast<int*>(ring + ((size_t)slot * (NV + 1) + NV) * 64 + TAG_LANE);
-  };
+  u32x4_t* ring = reinterpret_cast<u32x4_t*>(lds_raw);             // [S][NV][64]
+  int2* tags = reinterpret_cast<int2*>(ring + (size_t)S * NV * 64);  // [S]: records, fill
   const DevMesh& M = Ms[blockIdx.y];
   const LUGeom g = lu_geom(M);
   D2* lu = reinterpret_cast<D2*>(M.lu);
@@ -1773,100 +1780,188 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
     }
     st0 = *M.status;
   }
-  if (threadIdx.x < S) *tag_of(threadIdx.x) = 0;
   __syncthreads();
   if (st0 || (int)blockIdx.x >= g.W) return;
   const int wb = wb_s;
   const unsigned tag = tag_s;
   const int w = DIR > 0 ? wb : g.W - 1 - wb;   // row block of this workgroup
+  // a block with no neighbour upstream takes zero fills at every step: its
+  // fill words are zeroed once (nothing else writes them) and its fill tags
+  // read as ready from the start
+  const bool has_in = DIR > 0 ? w > 0 : w < g.W - 1;
+  if (threadIdx.x < S) tags[threadIdx.x] = int2{0, has_in ? 0 : INT_MAX};
+  if (!has_in && threadIdx.x < 2 * S) {
+    const int el = (DIR > 0 ? 0 : 62) + (threadIdx.x & 1);
+    unsigned* f = reinterpret_cast<unsigned*>(ring + (size_t)(threadIdx.x >> 1) * NV * 64 + 4 * 64 + el) + 2;
+    *reinterpret_cast<u32x2_t*>(f) = u32x2_t{0u, 0u};
+  }
+  __syncthreads();
   const int NT = g.TW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
-  // step-ordered result arrays: forward H (row NT-1-t for step t), backward
-  // H2 (row t); the backward producers read H
-  const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, (long long)g.W * g.TW * 64 * 16);
-  const __amdgpu_buffer_rsrc_t hout = DIR > 0 ? hnd : uniform_rsrc(lu + g.off_h2, (long long)g.W * g.TW * 64 * 16);
+  // step-ordered result arrays (one pair per lane): forward H (row NT-1-t
+  // for step t), backward H2 (row t); the backward producers read H
+  const long long hbytes = (long long)g.W * g.TW * 512;
+  const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, hbytes);
+  const __amdgpu_buffer_rsrc_t hout = DIR > 0 ? hnd : uniform_rsrc(lu + g.off_h2, hbytes);
   gu64* const mail = (gu64*)(lu + (DIR > 0 ? g.off_mf : g.off_mb));
   const int base = w * NT;
+  if (wave == NP + 1) {
+    // ---------------------------------------------------------- mail wave
+    // The neighbour block's edge row at this block's step tt: forward, row
+    // R-1 of block w-1 at column j = js+tt (its step tt+R-1); backward, row
+    // 0 of block w+1 at column j = je-tt (its step tt+R-1).  One poll reads
+    // 16 consecutive entries, lane l word l%4 of entry l/4; the entries from
+    // the first one not yet written on wait for the next poll.  Steps
+    // tt >= ni have no neighbour value and get zeros; a block with no
+    // neighbour upstream has its zero fills from the start.
+    if (!has_in) return;
+    const int wn = w - DIR;
+    const int nmail = g.ni;   // steps that take a neighbour value
+    // lane l of a poll writes, for entry l/4, the pair of words l%4 and
+    // l%4+1 (lanes with l%4 = 0, 2) into the edge row's lane of that pair
+    const int dst = (DIR > 0 ? 0 : 62) + ((l >> 1) & 1);
+    const bool writer = (l & 1) == 0;
+    int tt = 0;
+    bool ok = true;
+    unsigned long long t_wait = rt_now();
+    while (ok && tt < NT) {
+      unsigned v = 0u, vn = 0u;
+      int n = 0;   // entries ready from tt on
+      if (tt < nmail) {
+        const int e = tt + (l >> 2);
+        const int ec = e < g.ni ? e : g.ni - 1;
+        const unsigned long long x =
+            __hip_atomic_load(mail + ((long long)wn * NT + ec + R - 1) * 4 + (l & 3), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_SYSTEM);
+        const bool ready = e < g.ni && (unsigned)(x >> 32) == tag;
+        // entry i is ready when lanes 4i..4i+3 all are: count the ready
+        // prefix of 4-lane groups
+        const unsigned long long b = __ballot(ready);
+        // bit 4i: entry i ready; every other bit set, so the lowest clear
+        // bit is at 4 x (the first entry not ready)
+        const unsigned long long nr =
+            ~((b & (b >> 1) & (b >> 2) & (b >> 3) & 0x1111111111111111ull) | 0xEEEEEEEEEEEEEEEEull);
+        n = nr ? (int)(__builtin_ctzll(nr) / 4) : 16;
+        v = (unsigned)x;
+        vn = __builtin_bit_cast(unsigned, __builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, true));   // word of lane l^1
+        if (n == 0) {
+          if (__hip_atomic_load(&abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+          if (rt_now() - t_wait > LU32_WAIT) {
+            __hip_atomic_store(&abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            ok = false;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        t_wait = rt_now();
+      } else {
+        n = NT - tt < 16 ? NT - tt : 16;   // zero fills
+      }
+      // entries tt..tt+n-1 are all neighbour values (tt < nmail) or all
+      // zeros.  The slot of step te is free once consumed >= te-S+1; every
+      // entry whose slot is free is written in one pass (the consumer never
+      // passes an unwritten entry, so consumed <= tt and at most S entries,
+      // in distinct slots, are written at once), then their tags are set.
+      const bool zero = tt >= nmail;
+      for (int done = 0; done < n && ok;) {
+        const int t1 = tt + done;
+        const int c = __hip_atomic_load(&consumed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        int m = c + S - t1;
+        if (m <= 0) {
+          ok = spin_until_ge<4>(&consumed, t1 - S + 1, &abort_flag);
+          continue;
+        }
+        m = m < n - done ? m : n - done;
+        // mail entries: lanes 4i (words 0, 1) and 4i+2 (words 2, 3) of
+        // entry i; zero entries: lanes 2i and 2i+1
+        const int i = zero ? (l >> 1) : (l >> 2);
+        if (i >= done && i < done + m && (zero || writer)) {
+          u32x4_t* d = ring + (size_t)((tt + i) % S) * NV * 64 + 4 * 64;
+          const int dl = zero ? (DIR > 0 ? 0 : 62) + (l & 1) : dst;
+          *reinterpret_cast<u32x2_t*>(reinterpret_cast<unsigned*>(d + dl) + 2) =
+              zero ? u32x2_t{0u, 0u} : u32x2_t{v, vn};
+        }
+        if (l >= done && l < done + m)
+          __hip_atomic_store(&tags[(tt + l) % S].y, tt + l + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        done += m;
+      }
+      tt += n;
+    }
+    return;
+  }
   if (wave > 0) {
     // ---------------------------------------------------------- producers
-    // records in step layout: one coalesced 1 KiB row per vector and step
-    // (lane l's cell at step tt is (js+tt-l, ks+64w+l) forward, the cell of
-    // forward step NT-1-tt backward).  Entries of cells outside the grid
-    // hold zero records: such lanes compute exact zeros from finite
-    // inputs.  Steps past the end are clamped to the last one; the loop
-    // body has no conditional loads.
-    const __amdgpu_buffer_rsrc_t recf = uniform_rsrc(lu + g.off_f32, 9ll * g.W * g.TA * 64 * 16);
-    const __amdgpu_buffer_rsrc_t recb = uniform_rsrc(lu + g.off_b32, 8ll * g.W * g.TA * 64 * 16);
-    const __amdgpu_buffer_rsrc_t cbr = uniform_rsrc(lu + g.off_cb, (long long)g.W * g.TW * 64 * 16);
-    const int krow = 64 * w + l;
-    const bool rowin = krow < g.nk;
-    // the neighbour block's edge value for this block's fill lane at step
-    // tt: forward, lane 63 of block w-1 at column j = js+tt (its step tt+63);
-    // backward, lane 0 of block w+1 at column j = je-tt (its step tt+63).
-    // At the grid edge (no neighbour) the entry read is a valid one that is
-    // ignored.
-    const bool has_in = DIR > 0 ? w > 0 : w < g.W - 1;
-    const int wn = has_in ? w - DIR : w;
-    auto mail_ptr = [&](int tt) -> const gu64* {
-      const int c = tt < 0 ? 0 : (tt >= g.ni ? g.ni - 1 : tt);
-      return mail + ((long long)wn * NT + c + 63) * 4;
-    };
-    constexpr int NB = NV + (DIR < 0 ? 1 : 0);   // backward: + cB
-    auto fetch = [&](int tt, u32x4_t (&v)[NB], Mail& m) {
+    // records in step layout: coalesced rows per step (lane l's cell at
+    // step tt is (js+tt-r, ks+R*w+r) forward, the cell of forward step
+    // NT-1-tt backward).  Entries of cells outside the grid hold zero
+    // records: such lanes compute exact zeros from finite inputs.  Steps past
+    // the end are clamped to the last one; the loop body has no conditional
+    // loads.
+    const __amdgpu_buffer_rsrc_t recf = uniform_rsrc(lu + g.off_f32, 288ll * g.W * g.TA * 16);
+    const __amdgpu_buffer_rsrc_t recb = uniform_rsrc(lu + g.off_b32, 256ll * g.W * g.TA * 16);
+    const __amdgpu_buffer_rsrc_t cbr = uniform_rsrc(lu + g.off_cb, hbytes);
+    const int r = l >> 1;
+    const bool rowin = R * w + r < g.nk;
+    // a step's records: four rows and the u pair (backward: s_fwd and cB
+    // pairs), values in registers
+    auto fetch = [&](int tt, u32x4_t (&v)[4], u32x2_t& u, u32x2_t& cb) {
       const int tc = tt < NT ? tt : NT - 1;
-      const int jr = DIR > 0 ? tc - l : (g.ni - 1) - tc + (63 - l);
-      const bool cin = rowin && (unsigned)jr < (unsigned)g.ni;
       if constexpr (DIR > 0) {
-        const int so = (w * g.TA + tc) * 9 * 1024;
+        const int so = (w * g.TA + tc) * 288 * 16;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(recf, l * 16, so + q * 1024, 0);
+        for (int q = 0; q < 4; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(recf, l * 16, so + q * 1024, 0);
+        u = __builtin_amdgcn_raw_buffer_load_b64(recf, l * 8, so + 4096, 0);
+        cb = u32x2_t{0u, 0u};
       } else {
-        const int so = (w * g.TA + NT - 1 - tc) * 8 * 1024;
-        v[0] = __builtin_amdgcn_raw_buffer_load_b128(hnd, l * 16, (base + tc) * 1024, 0);
+        const int so = (w * g.TA + NT - 1 - tc) * 256 * 16;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[1 + q] = __builtin_amdgcn_raw_buffer_load_b128(recb, l * 16, so + q * 1024, 0);
+        for (int q = 0; q < 4; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(recb, l * 16, so + q * 1024, 0);
+        u = __builtin_amdgcn_raw_buffer_load_b64(hnd, l * 8, (base + tc) * 512, 0);
         // cB of the cell, kept at its forward step NT-1-tc; zero outside
         // the grid (those entries are never written)
-        const u32x4_t cb = __builtin_amdgcn_raw_buffer_load_b128(cbr, l * 16, (base + NT - 1 - tc) * 1024, 0);
-        v[NV] = cin ? cb : u32x4_t{0u, 0u, 0u, 0u};
+        const int jr = (g.ni - 1) - tc + (R - 1 - r);
+        const bool cin = rowin && (unsigned)jr < (unsigned)g.ni;
+        const u32x2_t c = __builtin_amdgcn_raw_buffer_load_b64(cbr, l * 8, (base + NT - 1 - tc) * 512, 0);
+        cb = cin ? c : u32x2_t{0u, 0u};
       }
-      mail_load(mail_ptr(tc), m);
     };
-    auto deliver = [&](int tt, u32x4_t (&v)[NB], Mail& m) -> bool {
+    auto deliver = [&](int tt, const u32x4_t (&v)[4], u32x2_t u, u32x2_t cb) -> bool {
       if (tt >= NT) return true;
-      u32x4_t fill = u32x4_t{0u, 0u, 0u, 0u};
-      if (has_in && tt < g.ni) {
-        // written some 64 steps earlier, the entry is normally already there
-        if (!mail_ok(m, tag))
-          if (!wait_mail(mail_ptr(tt), tag, m, &abort_flag)) return false;
-        fill = mail_value(m);
-      }
+      // whole-vector casts: clang lowers __builtin_bit_cast of one element
+      // of an ext_vector_type lvalue to a load from the vector's first
+      // element, so the .y lane would read .x
+      const F2 uu = __builtin_bit_cast(F2, u);
+      float ux = uu.x, uy = uu.y;
       if constexpr (DIR < 0) {
-        const float4 h = f4_of(v[0]), cb = f4_of(v[NV]);
-        v[0] = __builtin_bit_cast(u32x4_t, F4{h.x + cb.x, h.y + cb.y, h.z + cb.z, h.w + cb.w});
+        const F2 cc = __builtin_bit_cast(F2, cb);
+        ux = ux + cc.x;
+        uy = uy + cc.y;
       }
       if (!spin_until_ge<4>(&consumed, tt - S + 1, &abort_flag)) return false;
-      u32x4_t* d = ring + (size_t)(tt % S) * (NV + 1) * 64;
+      u32x4_t* d = ring + (size_t)(tt % S) * NV * 64;
 #pragma unroll
-      for (int q = 0; q < NV; ++q) d[q * 64 + l] = v[q];
-      if (l == 0) d[NV * 64 + MAIL_LANE] = fill;
-      if (l == 0) __hip_atomic_store(tag_of(tt % S), tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (int q = 0; q < 4; ++q) d[q * 64 + l] = v[q];
+      *reinterpret_cast<u32x2_t*>(d + 4 * 64 + l) =
+          u32x2_t{__builtin_bit_cast(unsigned, ux), __builtin_bit_cast(unsigned, uy)};
+      if (l == 0) __hip_atomic_store(&tags[tt % S].x, tt + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       return true;
     };
     // producer wave p handles steps p-1, p-1+NP, ...; buffer k % PD holds the
     // k-th of them, fetched PD-1 deliveries ahead
-    u32x4_t vb[PD][NB];
-    Mail mb[PD];
+    u32x4_t vb[PD][4];
+    u32x2_t ub[PD], cbb[PD];
     const int t0 = wave - 1;
 #pragma unroll
-    for (int k = 0; k < PD - 1; ++k) fetch(t0 + k * NP, vb[k], mb[k]);
+    for (int k = 0; k < PD - 1; ++k) fetch(t0 + k * NP, vb[k], ub[k], cbb[k]);
     bool ok = true;
     for (int tb = t0; tb < NT && ok; tb += PD * NP) {
 #pragma unroll
       for (int k = 0; k < PD; ++k) {
         const int tt = tb + k * NP;
-        fetch(tt + (PD - 1) * NP, vb[(k + PD - 1) % PD], mb[(k + PD - 1) % PD]);
-        if (ok) ok = deliver(tt, vb[k], mb[k]);
+        const int kf = (k + PD - 1) % PD;
+        fetch(tt + (PD - 1) * NP, vb[kf], ub[kf], cbb[kf]);
+        if (ok) ok = deliver(tt, vb[k], ub[k], cbb[k]);
       }
     }
     return;
@@ -1875,104 +1970,106 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lu_sweep32(const DevMesh* __r
   // Lanes whose cell lies outside the grid compute exact zeros (their
   // records read as zeros), and a lane inside the grid takes its
   // neighbours' values only from cells inside the grid or through zero
-  // matrices (the halo terms are folded into u and cB): no lane needs a
-  // select.
+  // matrices (the halo terms are folded into u and cB).
   __builtin_amdgcn_s_setprio(3);
-  // the edge lane's results are the neighbouring block's mail (entries
-  // before step 63 are never read)
+  // the edge row's results are the neighbouring block's mail (entries
+  // before step R-1 are never read); the opposite edge row takes the fill
   const bool has_out = DIR > 0 ? w < g.W - 1 : w > 0;
-  const bool mail_lane = has_out && l == (DIR > 0 ? 63 : 0);
-  gu64* const mail_out = mail + (long long)w * NT * 4;
-  const unsigned long long tg = (unsigned long long)tag << 32;
-  float sA[4] = {0.f, 0.f, 0.f, 0.f};
-  // k neighbour leaving this lane; the wavefront shift runs with every lane
-  // enabled (a disabled source lane reads as the fill value)
-  float kout[4] = {0.f, 0.f, 0.f, 0.f};
-  // records of the step in `slot`; v[NV] holds the fill value in lane
-  // MAIL_LANE and the tag in lane TAG_LANE
-  auto rd = [&](int slot, u32x4_t (&v)[NV + 1]) {
-    const u32x4_t* s = ring + (size_t)slot * (NV + 1) * 64;
-    v[NV] = s[NV * 64 + l];
-    asm volatile("" ::: "memory");   // the tag's read before the records'
+  const bool mail_lane = has_out && (DIR > 0 ? l >= 62 : l < 2);
+  const bool fill_lane = DIR > 0 ? l < 2 : l >= 62;
+  // the mail words go out by buffer stores from every lane, without a
+  // branch: the other lanes' offset lies past the end of the mail array,
+  // where the hardware drops a buffer store (the range check takes the
+  // lane offset; the step's offset is the scalar part)
+  const __amdgpu_buffer_rsrc_t mailr = uniform_rsrc((void*)mail, (long long)g.W * NT * 32);
+  const int moff = mail_lane ? (w * NT * 4 + 2 * (l & 1)) * 8 : 0x40000000;
+  float a0 = 0.f, a1 = 0.f;   // this lane's pair of the previous step
+  auto rd = [&](int slot, u32x4_t (&v)[NV], int& tv) {
+    const u32x4_t* s = ring + (size_t)slot * NV * 64;
+    const int2 tp = tags[slot];
+    tv = tp.x < tp.y ? tp.x : tp.y;
+    asm volatile("" ::: "memory");   // the tags' read before the records'
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = s[q * 64 + l];
   };
+  auto spin_tags = [&](int slot, int target) -> bool {
+    return spin_until_ge<1>(&tags[slot].x, target, &abort_flag) && spin_until_ge<1>(&tags[slot].y, target, &abort_flag);
+  };
   bool ok = true;
   // step t from slot t % S (= slot, a constant once the loop below is
-  // unrolled); reads step t+1 into N
-  auto step = [&](int slot, int t, u32x4_t (&C)[NV + 1], u32x4_t (&N)[NV + 1]) {
-    const int s1 = slot + 1 < S ? slot + 1 : 0;
-    if (__builtin_amdgcn_readlane((int)C[NV].x, TAG_LANE) < t + 1) {
-      // the records of step t were read before the producer stored them:
-      // wait for the tag and read them again
-      ok = ok && spin_until_ge<1>(tag_of(slot), t + 1, &abort_flag);
-      rd(slot, C);
+  // unrolled), its records read two steps earlier into C; reads step t+2
+  // into N, the registers of step t-1
+  auto step = [&](int slot, int t, u32x4_t (&C)[NV], int& tc, u32x4_t (&N)[NV], int& tn) {
+    const int s2 = slot + 2 < S ? slot + 2 : slot + 2 - S;
+    if (__builtin_expect(__builtin_amdgcn_readfirstlane(tc) < t + 1, 0)) {
+      // the slot of step t was read before its writers finished: wait for
+      // the tags and read it again
+      ok = ok && spin_tags(slot, t + 1);
+      rd(slot, C, tc);
     }
-    // the reads of steps <= t execute before this store
+    // frees the slot of step t.  Its reads were issued before this store
+    // (two steps ago, or just above; the compiler barrier keeps them there)
+    // and the LDS performs a CU's operations in one order that keeps each
+    // wave's issue order, so they have read the slot before a writer that
+    // sees this value refills it: no wait for reads still in flight
     asm volatile("" ::: "memory");
     __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    rd(s1, N);   // past the last step: a harmless read of a stale slot
-    const float4 fill = f4_of(C[NV]);
-    float kin[4];
-    kin[0] = dpp_shift_in<DIR>(kout[0], fill.x);
-    kin[1] = dpp_shift_in<DIR>(kout[1], fill.y);
-    kin[2] = dpp_shift_in<DIR>(kout[2], fill.z);
-    kin[3] = dpp_shift_in<DIR>(kout[3], fill.w);
-    const float4 u = f4_of(C[0]);
-    float sn[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      float4 a = f4_of(C[1 + q]), b = f4_of(C[5 + q]);
-      float uq = q == 0 ? u.x : (q == 1 ? u.y : (q == 2 ? u.z : u.w));
-      float p = __builtin_fmaf(a.x, sA[0], uq);
-      p = __builtin_fmaf(a.y, sA[1], p);
-      p = __builtin_fmaf(a.z, sA[2], p);
-      p = __builtin_fmaf(a.w, sA[3], p);
-      p = __builtin_fmaf(b.x, kin[0], p);
-      p = __builtin_fmaf(b.y, kin[1], p);
-      p = __builtin_fmaf(b.z, kin[2], p);
-      sn[q] = __builtin_fmaf(b.w, kin[3], p);
-    }
+    rd(s2, N, tn);   // past the last step: a harmless read of a stale slot
+    const float4 x = f4_of(C[4]);
+    // this row's other pair, and the neighbouring row's two pairs (the edge
+    // row takes the fill)
+    const float p0 = dpp_pair(a0), p1 = dpp_pair(a1);
+    float k0 = dpp_row<DIR>(a0), k1 = dpp_row<DIR>(a1);
+    k0 = fill_lane ? x.z : k0;
+    k1 = fill_lane ? x.w : k1;
+    const float kp0 = dpp_pair(k0), kp1 = dpp_pair(k1);
+    const float4 A0 = f4_of(C[0]), A1 = f4_of(C[1]), B0 = f4_of(C[2]), B1 = f4_of(C[3]);
+    float o0 = __builtin_fmaf(A0.x, a0, x.x), o1 = __builtin_fmaf(A1.x, a0, x.y);
+    o0 = __builtin_fmaf(A0.y, a1, o0); o1 = __builtin_fmaf(A1.y, a1, o1);
+    o0 = __builtin_fmaf(A0.z, p0, o0); o1 = __builtin_fmaf(A1.z, p0, o1);
+    o0 = __builtin_fmaf(A0.w, p1, o0); o1 = __builtin_fmaf(A1.w, p1, o1);
+    o0 = __builtin_fmaf(B0.x, k0, o0); o1 = __builtin_fmaf(B1.x, k0, o1);
+    o0 = __builtin_fmaf(B0.y, k1, o0); o1 = __builtin_fmaf(B1.y, k1, o1);
+    o0 = __builtin_fmaf(B0.z, kp0, o0); o1 = __builtin_fmaf(B1.z, kp0, o1);
+    o0 = __builtin_fmaf(B0.w, kp1, o0); o1 = __builtin_fmaf(B1.w, kp1, o1);
     // the result of (step, lane), read by the next kernels; entries of
     // cells outside the grid are never read as results.  Written through to
-    // memory (system scope): with plain stores, FP32 batches overlapping
-    // other FP32 batches on other streams diverged more often.
+    // memory (system scope).
     const int orow = DIR > 0 ? NT - 1 - t : t;
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, F4{sn[0], sn[1], sn[2], sn[3]}), hout, l * 16,
-                                           (base + orow) * 1024, LU32_SYS);
-    if (mail_lane) {
-      gu64* p = mail_out + (long long)t * 4;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        __hip_atomic_store(p + q, tg | (unsigned long long)__builtin_bit_cast(unsigned, sn[q]), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      sA[q] = sn[q];
-      kout[q] = sn[q];
-    }
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, F2{o0, o1}), hout, l * 8,
+                                          (base + orow) * 512, LU32_SYS);
+    // mail words {value, tag}, one 8-byte store each
+    __builtin_amdgcn_raw_buffer_store_b64(u32x2_t{__builtin_bit_cast(unsigned, o0), tag}, mailr, moff, t * 32,
+                                          LU32_SYS);
+    __builtin_amdgcn_raw_buffer_store_b64(u32x2_t{__builtin_bit_cast(unsigned, o1), tag}, mailr, moff + 8, t * 32,
+                                          LU32_SYS);
+    a0 = o0;
+    a1 = o1;
   };
-  u32x4_t C0[NV + 1], C1[NV + 1];
-  ok = spin_until_ge<1>(tag_of(0), 1, &abort_flag);
-  rd(0, C0);
-  // S steps per trip (S even), the two register sets alternating, so that
-  // every slot index and LDS offset is a constant; the last, partial trip
-  // checks each step against NT
-  static_assert(S % 2 == 0, "the ring holds an even number of steps");
+  u32x4_t C0[NV], C1[NV], C2[NV];
+  int tc0 = 0, tc1 = 0, tc2 = 0;
+  ok = spin_tags(0, 1);
+  rd(0, C0, tc0);
+  rd(1, C1, tc1);   // checked (and read again if early) at step 1
+  // S steps per trip (S a multiple of 3), the three register sets rotating,
+  // so that every slot index and LDS offset is a constant; the last,
+  // partial trip checks each step against NT
+  static_assert(S % 3 == 0, "the ring holds a multiple of 3 steps");
   int t0 = 0;
   for (; t0 + S <= NT && ok; t0 += S) {
 #pragma unroll
-    for (int i = 0; i < S; i += 2) {
-      step(i, t0 + i, C0, C1);
-      step(i + 1, t0 + i + 1, C1, C0);
+    for (int i = 0; i < S; i += 3) {
+      step(i, t0 + i, C0, tc0, C2, tc2);
+      step(i + 1, t0 + i + 1, C1, tc1, C0, tc0);
+      step(i + 2, t0 + i + 2, C2, tc2, C1, tc1);
     }
   }
   if (ok) {
 #pragma unroll
-    for (int i = 0; i < S; i += 2) {
-      if (t0 + i < NT) step(i, t0 + i, C0, C1);
-      if (t0 + i + 1 < NT) step(i + 1, t0 + i + 1, C1, C0);
+    for (int i = 0; i < S; i += 3) {
+      if (t0 + i < NT) step(i, t0 + i, C0, tc0, C2, tc2);
+      if (t0 + i + 1 < NT) step(i + 1, t0 + i + 1, C1, tc1, C0, tc0);
+      if (t0 + i + 2 < NT) step(i + 2, t0 + i + 2, C2, tc2, C1, tc1);
     }
   }
   if (!ok || __hip_atomic_load(&abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
@@ -2002,7 +2099,8 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
   if (*M.status) return;
   const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
   const LUGeom g = lu_geom(M);
-  const LUArr B = sizeof(T) == 8 ? lu_arr(M, g.off_b, g.nrec) : lu_arr(M, g.off_h2, (long long)g.W * g.TW * 64);
+  const LUArr B = lu_arr(M, g.off_b, g.nrec);
+  const F4* H2 = reinterpret_cast<const F4*>(reinterpret_cast<const D2*>(M.lu) + g.off_h2);
   long long n2 = (long long)jd * kd;
   int bad = 0;
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
@@ -2024,13 +2122,13 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
     double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
     q_roundtrip(q1, q2, q3, q4, M.q[M.nq - 1][i]);
     if (jin && kin) {
-      long long e = sizeof(T) == 8 ? lu_entry(M, g, j, k, -1) : lu_entry32(M, g, j, k, -1);
       double dq[4];
       if constexpr (sizeof(T) == 8) {
+        const long long e = lu_entry(M, g, j, k, -1);
         D2 a = *B.at(0, e), b = *B.at(1, e);
         dq[0] = a.x; dq[1] = a.y; dq[2] = b.x; dq[3] = b.y;
       } else {
-        F4 a = *reinterpret_cast<const F4*>(B.at(0, e));
+        const F4 a = H2[lu_entry32(M, g, j, k, -1)];
         dq[0] = (double)a.x; dq[1] = (double)a.y; dq[2] = (double)a.z; dq[3] = (double)a.w;
       }
       for (int n = 0; n < 4; ++n) M.s[n][i] = dq[n];
@@ -2497,11 +2595,12 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
   return hipSuccess;
 }
 
-// LDS ring slots of the FP32 sweeps: as many slots of (NV+1) KiB as fit in
-// 150 KiB, rounded down to even; producer waves and the steps each keeps in
-// flight
-constexpr int LU32_S = ((150 * 1024) / ((LU32_NV + 1) * 1024)) & ~1;
+// LDS ring slots of the FP32 sweeps (LU32_NV KiB each and two tag words): 14
+// slots, so two workgroups fit one CU's LDS; producer waves and the steps
+// each keeps in flight
+constexpr int LU32_S = 18;
 constexpr int LU32_NP = 3, LU32_PD = 5;
+constexpr size_t LU32_LDS = (size_t)LU32_S * LU32_NV * 1024 + 16 * ((LU32_S * 8 + 15) / 16);
 
 static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, Marks* mk,
                               hipStream_t st) {
@@ -2511,11 +2610,11 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
     mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
     int nk = hM[i].ke - hM[i].ks + 1, ni = hM[i].je - hM[i].js + 1;
     const LUGeom gg = lu_geom(ni, nk, hM[i].lu_rl);
-    const long long nprep = (long long)gg.W * 8 * ((gg.TA + LP_STEPS - 1) / LP_STEPS);   // prep tiles
+    const long long nprep = (long long)gg.W * (LU32_R / 8) * ((gg.TA + LP_STEPS - 1) / LP_STEPS);   // prep tiles
     nc = nprep > nc ? nprep : nc;
     wmax = gg.W > wmax ? gg.W : wmax;
   }
-  const size_t lds = (size_t)LU32_S * (LU32_NV + 1) * 1024;
+  const size_t lds = LU32_LDS;
   static bool attr_set = false;
   if (!attr_set) {
     (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, LU32_S, LU32_NP, LU32_PD>,
@@ -2529,11 +2628,11 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_PREP, st);
-  hipLaunchKernelGGL((k_lu_sweep32<1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), lds, st,
+  hipLaunchKernelGGL((k_lu_sweep32<1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (2 + LU32_NP)), lds, st,
                      dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_FORWARD, st);
-  hipLaunchKernelGGL((k_lu_sweep32<-1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (1 + LU32_NP)), lds, st,
+  hipLaunchKernelGGL((k_lu_sweep32<-1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (2 + LU32_NP)), lds, st,
                      dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_BACKWARD, st);
