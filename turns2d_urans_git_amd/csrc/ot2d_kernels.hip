@@ -1120,7 +1120,6 @@ struct LUGeom {
   long long off_b, off_hcf, off_hcb, off_hrf, off_hrb;   // FP64 arrays (16-byte vectors)
   long long ncell;                 // FP32: interior cells
   long long off_f32, off_b32, off_cb, off_h, off_h2;   // FP32 arrays
-  long long off_mf, off_mb;        // FP32 inter-workgroup mail, forward / backward (2 vectors per entry)
   long long off_tk;                // FP32 workgroup tickets (forward, backward): one vector
   long long total;                 // vectors the workspace must hold (either layout)
 };
@@ -1143,19 +1142,18 @@ __host__ __device__ inline LUGeom lu_geom(int ni, int nk, int RL) {
   long long t64 = g.off_hrb + (long long)LU_NVEC * ni;
   // FP32, all in step layout (per block and sweep step, a row of 64 lanes):
   // the forward records (4 vectors of 16 bytes per lane + one 8-byte pair:
-  // 288 16-byte units), the backward records (4 vectors: 256 units), cB,
-  // the forward and the backward results (8 bytes per lane: 32 units each);
-  // then the mail of each block's edge row (one 32-byte entry per block and
-  // step) and the two ticket counters
+  // 288 16-byte units), the backward records (4 vectors: 256 units), cB (8
+  // bytes per lane: 32 units), the forward and the backward results (16
+  // bytes per lane, each value beside the sweep's tag: 64 units each; the
+  // edge rows' results are also the inter-workgroup mail); then the two
+  // ticket counters
   g.ncell = (long long)ni * nk;
   g.off_f32 = 0;
   g.off_b32 = g.off_f32 + 288ll * g.W * g.TA;
   g.off_cb = g.off_b32 + 256ll * g.W * g.TA;
   g.off_h = g.off_cb + 32ll * g.W * g.TW;
-  g.off_h2 = g.off_h + 32ll * g.W * g.TW;
-  g.off_mf = g.off_h2 + 32ll * g.W * g.TW;
-  g.off_mb = g.off_mf + 2ll * g.W * g.TW;
-  g.off_tk = g.off_mb + 2ll * g.W * g.TW;
+  g.off_h2 = g.off_h + 64ll * g.W * g.TW;
+  g.off_tk = g.off_h2 + 64ll * g.W * g.TW;
   long long t32 = g.off_tk + 1;
   g.total = t64 > t32 ? t64 : t32;
   return g;
@@ -1248,12 +1246,13 @@ __device__ __forceinline__ long long lu_entry(const DevMesh& M, const LUGeom& g,
 // FP32 results of interior cell (j,k): workgroup w = (k-ks)/32 owns rows
 // ks+32w .. ks+32w+31, row r = (k-ks)%32 on lanes 2r (components 1, 2) and
 // 2r+1 (3, 4); forward step t = (j-js) + r, backward step t = (je-j) +
-// (31-r).  Returns the 16-byte unit holding the cell's four components in a
-// result array (one 8-byte pair per lane, 32 units per block and step).
+// (31-r).  Returns the first of the two 16-byte units holding the cell's
+// four components in a result array (16 bytes per lane: component, tag,
+// component, tag; 64 units per block and step).
 __device__ __forceinline__ long long lu_entry32(const DevMesh& M, const LUGeom& g, int j, int k, int dir) {
   int kk = k - M.ks, w = kk / LU32_R, r = kk % LU32_R;
   int t = dir > 0 ? (j - M.js) + r : (M.je - j) + (LU32_R - 1 - r);
-  return ((long long)w * g.TW + t) * 32 + r;
+  return ((long long)w * g.TW + t) * 64 + 2 * r;
 }
 
 __global__ __launch_bounds__(256) void k_lu_prep(const DevMesh* __restrict__ Ms) {
@@ -1690,19 +1689,19 @@ __device__ __forceinline__ bool spin_until_ge(int* w, int target, int* abort_fla
   return false;
 }
 
-// Inter-workgroup mail.  The edge row of block w stores its result of
-// sweep step t as entry w*TW + t of its direction's mail array: four 8-byte
-// words, word q holding component q's bits in the low half and the sweep's
-// tag in the high half, each written by one aligned 8-byte store (a
-// buffer store with the system-scope cache policy, the instruction an
-// 8-byte relaxed system-scope atomic store compiles to).  A reader accepts
-// an entry only when all four words, each read by one 8-byte atomic load,
-// carry the running sweep's tag.  An entry left by an earlier sweep (an
-// older tag) or one still being written (an older tag in some word) is
-// therefore never taken, and nothing else writes the mail: it needs no
-// reset between sweeps.  Loads go through global (not generic) pointers:
-// flat instructions would also count against the LDS wait counter and stall
-// the LDS hand-offs behind them.  System scope: writer and reader may sit on
+// Inter-workgroup mail.  Every lane stores its result of sweep step t as
+// 16 bytes in its direction's result array, two 8-byte words each holding
+// a component's bits in the low half and the sweep's tag in the high half,
+// by one aligned 16-byte store with the system-scope cache policy.  The
+// edge row's two lanes of step t (32 bytes, components 1-4) are the mail
+// entry the neighbour block reads: it accepts the entry only when all four
+// words, each read by one 8-byte system-scope atomic load, carry the
+// running sweep's tag.  An entry left by an earlier sweep (an older tag) or
+// one still being written (an older tag in some word) is therefore never
+// taken, and only the sweep writes these rows: they need no reset between
+// sweeps.  Loads go through global (not generic) pointers: flat
+// instructions would also count against the LDS wait counter and stall the
+// LDS hand-offs behind them.  System scope: writer and reader may sit on
 // different XCDs.
 typedef __attribute__((address_space(1))) unsigned long long gu64;
 __device__ __forceinline__ float4 f4_of(u32x4_t v) { return __builtin_bit_cast(float4, v); }
@@ -1798,12 +1797,13 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
   __syncthreads();
   const int NT = g.TW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
-  // step-ordered result arrays (one pair per lane): forward H (row NT-1-t
-  // for step t), backward H2 (row t); the backward producers read H
-  const long long hbytes = (long long)g.W * g.TW * 512;
+  // step-ordered result arrays (16 bytes per lane: value, tag, value,
+  // tag): forward H (row NT-1-t for step t), backward H2 (row t); the
+  // backward producers read H
+  const long long hbytes = (long long)g.W * g.TW * 1024;
   const __amdgpu_buffer_rsrc_t hnd = uniform_rsrc(lu + g.off_h, hbytes);
   const __amdgpu_buffer_rsrc_t hout = DIR > 0 ? hnd : uniform_rsrc(lu + g.off_h2, hbytes);
-  gu64* const mail = (gu64*)(lu + (DIR > 0 ? g.off_mf : g.off_mb));
+  gu64* const hmail = (gu64*)(lu + (DIR > 0 ? g.off_h : g.off_h2));
   const int base = w * NT;
   if (wave == NP + 1) {
     // ---------------------------------------------------------- mail wave
@@ -1830,9 +1830,13 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
       if (tt < nmail) {
         const int e = tt + (l >> 2);
         const int ec = e < g.ni ? e : g.ni - 1;
+        // the neighbour's step ec+R-1, stored at row NT-1-(ec+R-1)
+        // forward and ec+R-1 backward; its edge lanes 62-63 forward, 0-1
+        // backward (128 words per row)
+        const int srow = DIR > 0 ? NT - 1 - (ec + R - 1) : ec + R - 1;
         const unsigned long long x =
-            __hip_atomic_load(mail + ((long long)wn * NT + ec + R - 1) * 4 + (l & 3), __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_load(hmail + ((long long)wn * NT + srow) * 128 + (DIR > 0 ? 124 : 0) + (l & 3),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         const bool ready = e < g.ni && (unsigned)(x >> 32) == tag;
         // entry i is ready when lanes 4i..4i+3 all are: count the ready
         // prefix of 4-lane groups
@@ -1900,7 +1904,7 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
     // loads.
     const __amdgpu_buffer_rsrc_t recf = uniform_rsrc(lu + g.off_f32, 288ll * g.W * g.TA * 16);
     const __amdgpu_buffer_rsrc_t recb = uniform_rsrc(lu + g.off_b32, 256ll * g.W * g.TA * 16);
-    const __amdgpu_buffer_rsrc_t cbr = uniform_rsrc(lu + g.off_cb, hbytes);
+    const __amdgpu_buffer_rsrc_t cbr = uniform_rsrc(lu + g.off_cb, (long long)g.W * g.TW * 512);
     const int r = l >> 1;
     const bool rowin = R * w + r < g.nk;
     // a step's records: four rows and the u pair (backward: s_fwd and cB
@@ -1917,7 +1921,9 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
         const int so = (w * g.TA + NT - 1 - tc) * 256 * 16;
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = __builtin_amdgcn_raw_buffer_load_b128(recb, l * 16, so + q * 1024, 0);
-        u = __builtin_amdgcn_raw_buffer_load_b64(hnd, l * 8, (base + tc) * 512, 0);
+        // the forward result pair (words 0 and 2 of the lane's 16 bytes)
+        const u32x4_t h = __builtin_amdgcn_raw_buffer_load_b128(hnd, l * 16, (base + tc) * 1024, 0);
+        u = u32x2_t{h.x, h.z};
         // cB of the cell, kept at its forward step NT-1-tc; zero outside
         // the grid (those entries are never written)
         const int jr = (g.ni - 1) - tc + (R - 1 - r);
@@ -1974,15 +1980,7 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
   __builtin_amdgcn_s_setprio(3);
   // the edge row's results are the neighbouring block's mail (entries
   // before step R-1 are never read); the opposite edge row takes the fill
-  const bool has_out = DIR > 0 ? w < g.W - 1 : w > 0;
-  const bool mail_lane = has_out && (DIR > 0 ? l >= 62 : l < 2);
   const bool fill_lane = DIR > 0 ? l < 2 : l >= 62;
-  // the mail words go out by buffer stores from every lane, without a
-  // branch: the other lanes' offset lies past the end of the mail array,
-  // where the hardware drops a buffer store (the range check takes the
-  // lane offset; the step's offset is the scalar part)
-  const __amdgpu_buffer_rsrc_t mailr = uniform_rsrc((void*)mail, (long long)g.W * NT * 32);
-  const int moff = mail_lane ? (w * NT * 4 + 2 * (l & 1)) * 8 : 0x40000000;
   float a0 = 0.f, a1 = 0.f;   // this lane's pair of the previous step
   auto rd = [&](int slot, u32x4_t (&v)[NV], int& tv) {
     const u32x4_t* s = ring + (size_t)slot * NV * 64;
@@ -2032,17 +2030,14 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
     o0 = __builtin_fmaf(B0.y, k1, o0); o1 = __builtin_fmaf(B1.y, k1, o1);
     o0 = __builtin_fmaf(B0.z, kp0, o0); o1 = __builtin_fmaf(B1.z, kp0, o1);
     o0 = __builtin_fmaf(B0.w, kp1, o0); o1 = __builtin_fmaf(B1.w, kp1, o1);
-    // the result of (step, lane), read by the next kernels; entries of
-    // cells outside the grid are never read as results.  Written through to
-    // memory (system scope).
+    // the result of (step, lane), each value beside the tag: read by the
+    // next kernels and, on the edge row, by the neighbour block as mail;
+    // entries of cells outside the grid are never read as results.
+    // Written through to memory (system scope).
     const int orow = DIR > 0 ? NT - 1 - t : t;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, F2{o0, o1}), hout, l * 8,
-                                          (base + orow) * 512, LU32_SYS);
-    // mail words {value, tag}, one 8-byte store each
-    __builtin_amdgcn_raw_buffer_store_b64(u32x2_t{__builtin_bit_cast(unsigned, o0), tag}, mailr, moff, t * 32,
-                                          LU32_SYS);
-    __builtin_amdgcn_raw_buffer_store_b64(u32x2_t{__builtin_bit_cast(unsigned, o1), tag}, mailr, moff + 8, t * 32,
-                                          LU32_SYS);
+    __builtin_amdgcn_raw_buffer_store_b128(
+        u32x4_t{__builtin_bit_cast(unsigned, o0), tag, __builtin_bit_cast(unsigned, o1), tag}, hout, l * 16,
+        (base + orow) * 1024, LU32_SYS);
     a0 = o0;
     a1 = o1;
   };
@@ -2128,8 +2123,9 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
         D2 a = *B.at(0, e), b = *B.at(1, e);
         dq[0] = a.x; dq[1] = a.y; dq[2] = b.x; dq[3] = b.y;
       } else {
-        const F4 a = H2[lu_entry32(M, g, j, k, -1)];
-        dq[0] = (double)a.x; dq[1] = (double)a.y; dq[2] = (double)a.z; dq[3] = (double)a.w;
+        const long long e = lu_entry32(M, g, j, k, -1);
+        const F4 a = H2[e], b = H2[e + 1];   // (value, tag) words
+        dq[0] = (double)a.x; dq[1] = (double)a.z; dq[2] = (double)b.x; dq[3] = (double)b.z;
       }
       for (int n = 0; n < 4; ++n) M.s[n][i] = dq[n];
       if (fuse_update) {
