@@ -27,9 +27,10 @@
 //     result array H; a separate prep kernel resets it to an EMPTY pattern
 //     with plain stores before every sweep, and a reader accepts any
 //     non-EMPTY value.  Workgroup x is block x (dispatch order).
-//   tagged (the current kernel): the mail has its own array of 8-byte words
-//     {value, sweep tag}, written only by the sweep; a reader accepts a word
-//     only with the running sweep's tag.  A workgroup takes its block from
+//   tagged (the current kernel): the mail is 8-byte words {value, sweep
+//     tag} (the kernel keeps them in its result rows, two words per 16-byte
+//     store), written only by the sweep; a reader accepts a word only with
+//     the running sweep's tag.  A workgroup takes its block from
 //     an atomic ticket counter (ticket n: block n % W, sweep n / W, tag
 //     n / W + 1), so it only ever waits on a workgroup that has started.
 //

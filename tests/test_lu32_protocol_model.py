@@ -6,8 +6,8 @@ The model runs the blocked, skewed sweep of k_lu_sweep32 with real threads
 number of CU slots over several XCDs) and compares every result with a
 sequential evaluation, bit for bit.  It contrasts the mailbox protocol the
 kernel used before (a sentinel reset by a separate prep kernel, workgroup x
-= block x) with the one it uses now (mail words tagged with the sweep number
-by 8-byte atomic stores, blocks assigned by an atomic ticket):
+= block x) with the one it uses now (mail words tagged with the sweep number,
+each 8-byte word written whole, blocks assigned by an atomic ticket):
 
 * with the prep's reset written back late across the kernel boundary, the
   sentinel protocol takes a previous sweep's value as mail and the late

@@ -914,10 +914,9 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
     HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, mk, c->stream));
     if (c->iturb == 2) HIPCHK(launch_sst(c->dM, c->hM.data(), nmi(c), mk, c->stream));
     else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), mk, c->stream));
-    HIPCHK(launch_update(c->dM, c->hM.data(), nmi(c), c->stream));
+    // the q update and monitor in one pass (stage "update")
+    HIPCHK(launch_update_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
     mark(mk, OT2D_ST_UPDATE, c->stream);
-    HIPCHK(launch_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
-    mark(mk, OT2D_ST_MONITOR, c->stream);
   }
   // overset meshes exchange fringe values (src/Turns_overset.f:500-503)
   if (c->ndon > 0 || c->nfr > 0) {

@@ -126,8 +126,11 @@ __device__ __forceinline__ void stage_line4(const double* a, const double* b, co
 // s' and c' go to os and oc (os may be ls itself).  The recurrence runs in
 // chunks of THOMAS_CH cells whose LDS reads are issued a chunk ahead into
 // registers, so the dependent chain does not wait on LDS.  Chunk reads past
-// the line's end are clamped to its last cell and never used.
+// the line's end are clamped to its last cell and never used.  Cell t of the
+// line sits at index t*ST of each array (ST > 1: several lines interleaved,
+// one per lane).
 constexpr int THOMAS_CH = 8;
+template <int ST = 1>
 __device__ __forceinline__ void thomas_lds(const double* la, const double* lb, const double* lc, const double* ls,
                                            double* os, double* oc, int n) {
   constexpr int CH = THOMAS_CH;
@@ -138,7 +141,7 @@ __device__ __forceinline__ void thomas_lds(const double* la, const double* lb, c
   auto ld = [&](double (&xa)[CH], double (&xb)[CH], double (&xc)[CH], double (&xs)[CH], int t) {
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-      const int i = t + u < n ? t + u : n - 1;
+      const int i = (t + u < n ? t + u : n - 1) * ST;
       xa[u] = la[i]; xb[u] = lb[i]; xc[u] = lc[i]; xs[u] = ls[i];
     }
   };
@@ -151,7 +154,7 @@ __device__ __forceinline__ void thomas_lds(const double* la, const double* lb, c
         bb = 1. / (xb[u] - ai * cp);
         sp = (xs[u] - ai * sp) * bb;
         cp = xc[u] * bb;
-        os[t + u] = sp; oc[t + u] = cp;
+        os[(t + u) * ST] = sp; oc[(t + u) * ST] = cp;
       }
     }
   };
@@ -172,7 +175,7 @@ __device__ __forceinline__ void thomas_lds(const double* la, const double* lb, c
   auto ldb = [&](double (&xs)[CH], double (&xc)[CH], int t) {
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-      const int i = t - u >= 0 ? t - u : 0;
+      const int i = (t - u >= 0 ? t - u : 0) * ST;
       xs[u] = os[i]; xc[u] = oc[i];
     }
   };
@@ -181,7 +184,7 @@ __device__ __forceinline__ void thomas_lds(const double* la, const double* lb, c
     for (int u = 0; u < CH; ++u) {
       if (u < cnt) {
         sn = xs[u] - xc[u] * sn;
-        os[t - u] = sn;
+        os[(t - u) * ST] = sn;
       }
     }
   };
@@ -248,8 +251,8 @@ hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStrea
 hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st);
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
                      hipStream_t st);
-hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd, int rl);

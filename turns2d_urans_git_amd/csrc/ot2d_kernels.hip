@@ -7,10 +7,13 @@
 //   muscld_new       821-912; norms step.f:53-115
 //   timespectralrhs  src/sources.f:2-27          k_tsrhs
 //   rbc              src/bc.f:88-177             k_rbc            (1 block / instance)
-//   s*tscale, ilu2d  src/step.f:141-159,         k_lu_fwd/k_lu_bwd (1 block / instance,
-//                    src/lhs.f:470-814            one thread per k row, hyperplane sweeps)
-//   q update         src/step.f:205-211          fused in k_lu_bwd or k_update
-//   monitor          src/step.f:332-392          fused / k_monitor
+//   s*tscale, ilu2d  src/step.f:141-159,         FP64: k_lu_prep, k_lu_sweep<double,RL>
+//                    src/lhs.f:470-814            (one wave per instance, skewed rows),
+//                                                 k_lu_finish; FP32 operator: k_lu_prep32,
+//                                                 k_lu_sweep32 (32-row workgroups),
+//                                                 k_lu_finish<float>
+//   q update         src/step.f:205-211          fused in k_lu_finish (Euler) or
+//   monitor          src/step.f:332-392          k_update_monitor; k_monitor alone
 //   initia geometry  src/init.f:2-177, setup.f   k_init_geometry
 //   force2d          src/forces.f:111-398        k_forces
 //
@@ -1518,8 +1521,10 @@ __global__ __launch_bounds__(64) void k_lu_sweep(const DevMesh* __restrict__ Ms)
 // 4096 + 8l after the record's start for u, so a producer reads a step as
 // four coalesced 1 KiB rows and one 512-byte row; the backward record (256
 // units) is kept at the cell's forward step too.
-//   H  (one pair per lane and step): s_fwd, written by the forward sweep
+//   H  (16 bytes per lane and step: the pair, each value beside the
+//      sweep's tag): s_fwd, written by the forward sweep
 //   H2 (the same): the result, written by the backward sweep
+// (the edge row's lanes of H and H2 are also the inter-workgroup mail)
 // The halo-ring s values themselves are scaled in k_lu_finish (the prep
 // reads the unscaled neighbours).
 
@@ -2139,18 +2144,27 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
   if (bad) atomicCAS(M.status, 0, 1);
 }
 
-// q update (src/step.f:205-211) when it cannot be fused
-__global__ __launch_bounds__(256) void k_update(const DevMesh* __restrict__ Ms) {
+// the q update (src/step.f:205-211) and monitor (src/step.f:332-392) in
+// one pass: interior cells are updated, then every cell (halo included) is
+// checked; each cell's update reads only that cell, so this is the update
+// followed by the monitor
+__global__ __launch_bounds__(256) void k_update_monitor(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
-  int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
-  long long ncell = (long long)nj * nk;
-  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
-    int j = M.js + (int)(w % nj), k = M.ks + (int)(w / nj);
-    long long i = cell(M, j, k);
-    double ibp = (double)imax(M.iblank[i], 0);
-    for (int n = 0; n < 4; ++n) M.q[n][i] = M.q[n][i] + M.s[n][i] * ibp;
+  const int jd = M.jd;
+  long long n2 = (long long)jd * M.kd;
+  int bad = 0;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
+    const int j = 1 + (int)(i % jd), k = 1 + (int)(i / jd);
+    double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
+    if (j >= M.js && j <= M.je && k >= M.ks && k <= M.ke) {
+      double ibp = (double)imax(M.iblank[i], 0);
+      q1 = q1 + M.s[0][i] * ibp; q2 = q2 + M.s[1][i] * ibp; q3 = q3 + M.s[2][i] * ibp; q4 = q4 + M.s[3][i] * ibp;
+      M.q[0][i] = q1; M.q[1][i] = q2; M.q[2][i] = q3; M.q[3][i] = q4;
+    }
+    bad |= asq_bad(M.c, q1, q2, q3, q4);
   }
+  if (bad) atomicCAS(M.status, 0, 1);
 }
 
 // monitor (src/step.f:332-392) over all cells
@@ -2670,13 +2684,12 @@ size_t lu_workspace_bytes(int jd, int kd, int rl) {
   return 16 * (size_t)lu_geom(jd - 2, kd - 2, rl).total;
 }
 
-hipError_t launch_update(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
   long long mx = 0;
   for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
-  hipLaunchKernelGGL(k_update, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  hipLaunchKernelGGL(k_update_monitor, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
   return hipGetLastError();
 }
-
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
   long long mx = 0;
   for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;

@@ -10,7 +10,9 @@
 //                                                    two scalar tridiagonal systems, in
 //                                                    gather form (one thread per cell)
 //                                      k_sa_solve_j  tridiagonal solve along j (thread per k)
-//                                      k_sa_solve_k  tridiagonal solve along k (thread per j),
+//                                      k_sa_solve_k8 tridiagonal solves along k, 8 adjacent j
+//                                                    columns per workgroup (k_sa_solve_k: one
+//                                                    column, for lines over 256 cells),
 //                                                    nu-tilde update and eddy viscosity
 //
 // Operand order follows the reference expressions (built with
@@ -730,6 +732,82 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
   }
 }
 
+// lsolvek, the nu-tilde update and c_turm for 8 adjacent j columns
+// (blockIdx.x) at once, for lines of up to SAK_NMAX cells: the workgroup
+// (four waves) stages the 8 k lines as rows of 8 consecutive cells (each row
+// one 64-byte run, where the one-line kernel gathers one element per cache
+// line), lanes 0..7 of wave 0 run the 8 recurrences out of LDS (column c at
+// stride 8), and all threads apply the update and the eddy viscosity, a few
+// cells each with their loads issued together.  Columns past je stage a
+// harmless b = 1 row and are never written.
+constexpr int SAK_TJ = 8, SAK_NMAX = 256, SAK_THREADS = 256;
+__global__ __launch_bounds__(SAK_THREADS) void k_sa_solve_k8(const DevMesh* __restrict__ Ms) {
+  __shared__ double la[SAK_NMAX * SAK_TJ], lb[SAK_NMAX * SAK_TJ], lc[SAK_NMAX * SAK_TJ];
+  __shared__ double ls[SAK_NMAX * SAK_TJ], lcp[SAK_NMAX * SAK_TJ];
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int ks = M.ks, ke = M.ke;
+  const int j0 = M.js + (int)blockIdx.x * SAK_TJ;
+  if (j0 > M.je) return;
+  const int n = ke - ks + 1;
+  const int ne = n * SAK_TJ;
+  const int tid = (int)threadIdx.x;
+  const double* __restrict__ s = M.sa_sj;
+  const double* __restrict__ c = M.sa_ck;
+  const double* __restrict__ a = M.sa_ak;
+  const double* __restrict__ b = M.sa_bb;
+  // element e = k*8 + col of the tile; 4 elements per thread in flight at once
+  constexpr int U = 4;
+  for (int e0 = 0; e0 < ne; e0 += SAK_THREADS * U) {
+    double ra[U], rb[U], rc[U], rs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + tid + SAK_THREADS * u;
+      const int j = j0 + (e & (SAK_TJ - 1));
+      const int k = ks + (e < ne ? e : ne - 1) / SAK_TJ;
+      const long long i = cell(M, j <= M.je ? j : M.je, k);
+      ra[u] = a[i]; rb[u] = b[i]; rc[u] = c[i]; rs[u] = s[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + tid + SAK_THREADS * u;
+      if (e < ne) {
+        const bool in = j0 + (e & (SAK_TJ - 1)) <= M.je;
+        la[e] = in ? ra[u] : 0.0; lb[e] = in ? rb[u] : 1.0; lc[e] = in ? rc[u] : 0.0; ls[e] = in ? rs[u] : 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < SAK_TJ) thomas_lds<SAK_TJ>(la + tid, lb + tid, lc + tid, ls + tid, ls + tid, lcp + tid, n);
+  __syncthreads();
+  // the update and c_turm, U cells per thread with all their loads issued
+  // before the first store
+  const double vnulim = 1.0e-20, relfac = 1.0;
+  for (int e0 = 0; e0 < ne; e0 += SAK_THREADS * U) {
+    long long ii[U];
+    bool ok[U];
+    double q5[U], q1[U], qj[U], vm[U], ib[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + tid + SAK_THREADS * u;
+      const int j = j0 + (e & (SAK_TJ - 1));
+      ok[u] = e < ne && j <= M.je;
+      ii[u] = cell(M, j <= M.je ? j : M.je, ks + (e < ne ? e : ne - 1) / SAK_TJ);
+      q5[u] = M.q[4][ii[u]]; q1[u] = M.q[0][ii[u]]; qj[u] = M.q[M.nq - 1][ii[u]]; vm[u] = M.sa_vmul[ii[u]];
+      ib[u] = (double)imx(M.iblank[ii[u]], 0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      const int e = e0 + tid + SAK_THREADS * u;
+      double dq = relfac * (0. + ls[e]);
+      const double v = fmax((q5[u] + dq * ib[u]), vnulim);
+      M.q[4][ii[u]] = v;
+      M.turmu[ii[u]] = c_turm_val(q1[u], qj[u], vm[u], v);
+    }
+  }
+}
+
 // Line solves for many lines at once: one lane per line, the same
 // reference arithmetic.
 // k_sa_lines_j: lines along j (one per k row), 64 rows per workgroup.  Wave
@@ -1231,7 +1309,9 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sa_update, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
   } else {
-    hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
+    if (mk_ <= SAK_NMAX)
+      hipLaunchKernelGGL(k_sa_solve_k8, dim3((mj + SAK_TJ - 1) / SAK_TJ, nmi), dim3(SAK_THREADS), 0, st, dM);
+    else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
