@@ -846,15 +846,18 @@ static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks*
   const bool later = !c->p.invisc || c->p.timespectral;
   HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms && !later, c->weno, c->stream));
   mark(mk, OT2D_ST_RESIDUAL, c->stream);
+  // without time-spectral terms the residual is complete after visrhs,
+  // which then takes the norm partials itself
+  const bool vis_norms = norms && !c->p.invisc && !c->p.timespectral;
   if (!c->p.invisc) {
-    HIPCHK(launch_visrhs(c->dM, c->hM.data(), nmi(c), c->stream));
+    HIPCHK(launch_visrhs(c->dM, c->hM.data(), nmi(c), c->stream, vis_norms ? 1 : 0));
     mark(mk, OT2D_ST_VISRHS, c->stream);
   }
   if (c->p.timespectral) {
     HIPCHK(launch_tsrhs(c->dM, c->hM.data(), nmi(c), c->stream));
     mark(mk, OT2D_ST_TSRHS, c->stream);
   }
-  if (norms && later) HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
+  if (norms && later && !vis_norms) HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
   if (norms) {
     HIPCHK(launch_norms_final(c->dM, nmi(c), c->stream));
     if (c->conv_on) HIPCHK(launch_conv_check(c->dM, c->d_cinst, c->d_ctol, c->d_status, c->ncase, c->stream));

@@ -212,6 +212,45 @@ constexpr int BC_THREADS = 256;
 __host__ __device__ inline long long cell(const DevMesh& M, int j, int k) {
   return (long long)(j - 1) + (long long)(k - 1) * M.jd;
 }
+
+// residual norms of step (src/step.f:53-115), first stage: the partial sums
+// (sum of squares, rho part, max |s| with its first (k,n,j) position) of the
+// cells of one tile, reduced in a fixed order and stored per tile
+__device__ inline void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool valid, const double* s,
+                                 double* redbuf) {
+  double(*red)[RES_THREADS] = reinterpret_cast<double(*)[RES_THREADS]>(redbuf);
+  const int tid = threadIdx.x;
+  const int jd = M.jd;
+  double rs = 0.0, rr1 = 0.0, rmax = 0.0, key = 0.0;
+  if (valid) {
+    long long i = cell(M, j, k);
+    double ibp = (double)(M.iblank[i] > 0 ? M.iblank[i] : 0);
+    double m1 = s[0] * ibp, m2 = s[1] * ibp, m3 = s[2] * ibp, m4 = s[3] * ibp;
+    rs = m1 * m1 + m2 * m2 + m3 * m3 + m4 * m4;
+    rr1 = s[0] * s[0];
+    // first maximum in (k, n, j) order (src/step.f:67-78)
+    for (int n = 0; n < 4; ++n) {
+      double a = fabs(s[n]) * ibp;
+      if (a > rmax) { rmax = a; key = (double)(((long long)(k - 1) * 4 + n) * jd + (j - 1)); }
+    }
+  }
+  red[0][tid] = rs; red[1][tid] = rr1; red[2][tid] = rmax; red[3][tid] = valid && rmax > 0.0 ? key : 1e300;
+  __syncthreads();
+  for (int off = RES_THREADS / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      red[0][tid] = red[0][tid] + red[0][tid + off];
+      red[1][tid] = red[1][tid] + red[1][tid + off];
+      double a = red[2][tid], b = red[2][tid + off];
+      double ka = red[3][tid], kb = red[3][tid + off];
+      if (b > a || (b == a && kb < ka)) { red[2][tid] = b; red[3][tid] = kb; }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double* P = M.partials + 4ll * T.blk;
+    P[0] = red[0][0]; P[1] = red[1][0]; P[2] = red[2][0]; P[3] = red[3][0];
+  }
+}
 __host__ __device__ inline long long vert(const DevMesh& M, int j, int k) {
   return (long long)(j - 1) + (long long)(k - 1) * M.jmax;
 }
@@ -256,7 +295,7 @@ hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, 
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd, int rl);
-hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int norms = 0);
 hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);

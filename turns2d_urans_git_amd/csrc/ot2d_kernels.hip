@@ -621,8 +621,6 @@ __device__ __forceinline__ void face_states_weno(const double* sf0, int stv, int
   }
 }
 
-__device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool valid, const double* s,
-                          double* redbuf);
 
 // Residual of one 32x8 tile: rhsup for nei=1 (src/eulerrhs.f:2-280), fluxes
 // of the tile's faces computed once into LDS, plus the residual-norm partials
@@ -738,45 +736,6 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
     for (int n = 4; n < M.nv; ++n) M.s[n][i] = 0.0;
   }
   if (do_norms) norm_tile(M, T, j, k, valid, sv, &red[0][0]);
-}
-
-// residual norms of step (src/step.f:53-115), first stage: the partial sums
-// (sum of squares, rho part, max |s| with its first (k,n,j) position) of the
-// cells of one tile, reduced in a fixed order and stored per tile
-__device__ void norm_tile(const DevMesh& M, const ResTile& T, int j, int k, bool valid, const double* s,
-                          double* redbuf) {
-  double(*red)[RES_THREADS] = reinterpret_cast<double(*)[RES_THREADS]>(redbuf);
-  const int tid = threadIdx.x;
-  const int jd = M.jd;
-  double rs = 0.0, rr1 = 0.0, rmax = 0.0, key = 0.0;
-  if (valid) {
-    long long i = cell(M, j, k);
-    double ibp = (double)imax(M.iblank[i], 0);
-    double m1 = s[0] * ibp, m2 = s[1] * ibp, m3 = s[2] * ibp, m4 = s[3] * ibp;
-    rs = m1 * m1 + m2 * m2 + m3 * m3 + m4 * m4;
-    rr1 = sq(s[0]);
-    // first maximum in (k, n, j) order (src/step.f:67-78)
-    for (int n = 0; n < 4; ++n) {
-      double a = fabs(s[n]) * ibp;
-      if (a > rmax) { rmax = a; key = (double)(((long long)(k - 1) * 4 + n) * jd + (j - 1)); }
-    }
-  }
-  red[0][tid] = rs; red[1][tid] = rr1; red[2][tid] = rmax; red[3][tid] = valid && rmax > 0.0 ? key : 1e300;
-  __syncthreads();
-  for (int off = RES_THREADS / 2; off > 0; off >>= 1) {
-    if (tid < off) {
-      red[0][tid] = red[0][tid] + red[0][tid + off];
-      red[1][tid] = red[1][tid] + red[1][tid + off];
-      double a = red[2][tid], b = red[2][tid + off];
-      double ka = red[3][tid], kb = red[3][tid + off];
-      if (b > a || (b == a && kb < ka)) { red[2][tid] = b; red[3][tid] = kb; }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    double* P = M.partials + 4ll * T.blk;
-    P[0] = red[0][0]; P[1] = red[1][0]; P[2] = red[2][0]; P[3] = red[3][0];
-  }
 }
 
 // second stage, one block per instance: the tile partials in a fixed order
@@ -1987,6 +1946,7 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
   // before step R-1 are never read); the opposite edge row takes the fill
   const bool fill_lane = DIR > 0 ? l < 2 : l >= 62;
   float a0 = 0.f, a1 = 0.f;   // this lane's pair of the previous step
+  u32x4_t hv = {0u, tag, 0u, tag};   // the stored words; the tags stay put
   auto rd = [&](int slot, u32x4_t (&v)[NV], int& tv) {
     const u32x4_t* s = ring + (size_t)slot * NV * 64;
     const int2 tp = tags[slot];
@@ -2010,13 +1970,17 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
       ok = ok && spin_tags(slot, t + 1);
       rd(slot, C, tc);
     }
-    // frees the slot of step t.  Its reads were issued before this store
-    // (two steps ago, or just above; the compiler barrier keeps them there)
-    // and the LDS performs a CU's operations in one order that keeps each
-    // wave's issue order, so they have read the slot before a writer that
-    // sees this value refills it: no wait for reads still in flight
-    asm volatile("" ::: "memory");
-    __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // every third step (the last of a register-set rotation) frees the
+    // slots up to step t.  Their reads were issued before this store (two
+    // steps ago, or just above; the compiler barrier keeps them there) and
+    // the LDS performs a CU's operations in one order that keeps each
+    // wave's issue order, so they have read a slot before a writer that
+    // sees this value refills it: no wait for reads still in flight.  The
+    // writers then run up to S-3 steps ahead instead of S-1.
+    if (slot % 3 == 2) {
+      asm volatile("" ::: "memory");
+      __hip_atomic_store(&consumed, t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
     rd(s2, N, tn);   // past the last step: a harmless read of a stale slot
     const float4 x = f4_of(C[4]);
     // this row's other pair, and the neighbouring row's two pairs (the edge
@@ -2040,9 +2004,9 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
     // entries of cells outside the grid are never read as results.
     // Written through to memory (system scope).
     const int orow = DIR > 0 ? NT - 1 - t : t;
-    __builtin_amdgcn_raw_buffer_store_b128(
-        u32x4_t{__builtin_bit_cast(unsigned, o0), tag, __builtin_bit_cast(unsigned, o1), tag}, hout, l * 16,
-        (base + orow) * 1024, LU32_SYS);
+    hv.x = __builtin_bit_cast(unsigned, o0);
+    hv.z = __builtin_bit_cast(unsigned, o1);
+    __builtin_amdgcn_raw_buffer_store_b128(hv, hout, l * 16, (base + orow) * 1024, LU32_SYS);
     a0 = o0;
     a1 = o1;
   };
@@ -2608,7 +2572,7 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
 // LDS ring slots of the FP32 sweeps (LU32_NV KiB each and two tag words): 14
 // slots, so two workgroups fit one CU's LDS; producer waves and the steps
 // each keeps in flight
-constexpr int LU32_S = 18;
+constexpr int LU32_S = 12;
 constexpr int LU32_NP = 3, LU32_PD = 5;
 constexpr size_t LU32_LDS = (size_t)LU32_S * LU32_NV * 1024 + 16 * ((LU32_S * 8 + 15) / 16);
 

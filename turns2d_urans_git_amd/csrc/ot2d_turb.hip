@@ -225,7 +225,13 @@ __device__ __forceinline__ Tri lds_tri(const double (&g)[3][N], int c) {
   return f;
 }
 
-__global__ __launch_bounds__(VIS_THREADS) void k_visrhs(const DevMesh* __restrict__ Ms) {
+// With norms set, the residual is complete once this kernel has added the
+// viscous terms, so each tile also takes its residual-norm partials here
+// (the tiles and thread-to-cell map are the residual's: 32x8 tiles, k-tile
+// major), instead of a separate pass over s.
+__global__ __launch_bounds__(VIS_THREADS) void k_visrhs(const DevMesh* __restrict__ Ms, int norms) {
+  static_assert(VIS_TJ == RES_TJ && VIS_TK == RES_TK, "visrhs tiles are the residual's");
+  __shared__ double red[4][RES_THREADS];
   const DevMesh& M = Ms[blockIdx.y];
   const int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
   const int tiles_j = (nj + VIS_TJ - 1) / VIS_TJ;
@@ -282,66 +288,76 @@ __global__ __launch_bounds__(VIS_THREADS) void k_visrhs(const DevMesh* __restric
 
   const int a = t % VIS_TJ, b = t / VIS_TJ;
   const int j = j0 + a, k = k0 + b;
-  if (j > M.je || k > M.ke) return;
-  const long long i = cell(M, j, k);
-  double s2 = M.s[1][i], s3 = M.s[2][i], s4 = M.s[3][i];
-  const int c = (b + 1) * VIS_RJ + a + 1;
-  const VisPrim p1 = lds_prim(L, c);
-  // ---- eta part (j = const line): faces k-1|k, k|k+1, cross terms
-  {
-    const int e = (b + 1) * VIS_TJ + a;
-    const VisCoef c0 = lds_coef(L.ec, e - VIS_TJ), c1 = lds_coef(L.ec, e), c2 = lds_coef(L.ec, e + VIS_TJ);
-    const Tri fm = vis_face(lds_prim(L, c - VIS_RJ), c0, p1, c1);   // face k-1
-    const Tri fp = vis_face(p1, c1, lds_prim(L, c + VIS_RJ), c2);   // face k
-    const Tri gm = lds_tri(L.eg, e - VIS_TJ), gc = lds_tri(L.eg, e);
-    if (k < M.ke) {
-      const Tri gp = lds_tri(L.eg, e + VIS_TJ);
-      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
-      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
-      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
-    } else {  // one-sided closure at kend (src/visrhs.f:141-150)
-      Tri gq;
-      if (b >= 1) {
-        gq = lds_tri(L.eg, e - 2 * VIS_TJ);
-      } else {  // row k-2 lies below the tile's ring
-        const VisPrim pq = vis_prim(M, cell(M, j, k - 2), c2b, c2bp, prtr);
-        const VisCoef cq = eta_coef(M, j, k - 2, pq, dre, gkpr);
-        gq = eta_cross(cq, div_prim(M, cell(M, j + 1, k - 2)), div_prim(M, cell(M, j - 1, k - 2)));
+  const bool valid = j <= M.je && k <= M.ke;
+  // every thread reaches the one norm_tile call below (its barriers), so
+  // cells outside the grid skip the work instead of returning
+  double sv[4] = {0.0, 0.0, 0.0, 0.0};
+  if (valid) {
+    const long long i = cell(M, j, k);
+    double s2 = M.s[1][i], s3 = M.s[2][i], s4 = M.s[3][i];
+    const int c = (b + 1) * VIS_RJ + a + 1;
+    const VisPrim p1 = lds_prim(L, c);
+    // ---- eta part (j = const line): faces k-1|k, k|k+1, cross terms
+    {
+      const int e = (b + 1) * VIS_TJ + a;
+      const VisCoef c0 = lds_coef(L.ec, e - VIS_TJ), c1 = lds_coef(L.ec, e), c2 = lds_coef(L.ec, e + VIS_TJ);
+      const Tri fm = vis_face(lds_prim(L, c - VIS_RJ), c0, p1, c1);   // face k-1
+      const Tri fp = vis_face(p1, c1, lds_prim(L, c + VIS_RJ), c2);   // face k
+      const Tri gm = lds_tri(L.eg, e - VIS_TJ), gc = lds_tri(L.eg, e);
+      if (k < M.ke) {
+        const Tri gp = lds_tri(L.eg, e + VIS_TJ);
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
+      } else {  // one-sided closure at kend (src/visrhs.f:141-150)
+        Tri gq;
+        if (b >= 1) {
+          gq = lds_tri(L.eg, e - 2 * VIS_TJ);
+        } else {  // row k-2 lies below the tile's ring
+          const VisPrim pq = vis_prim(M, cell(M, j, k - 2), c2b, c2bp, prtr);
+          const VisCoef cq = eta_coef(M, j, k - 2, pq, dre, gkpr);
+          gq = eta_cross(cq, div_prim(M, cell(M, j + 1, k - 2)), div_prim(M, cell(M, j - 1, k - 2)));
+        }
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3. * gc.f4);
       }
-      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
-      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
-      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3. * gc.f4);
     }
-  }
-  // ---- xi part (k = const line)
-  {
-    const int x = b * VIS_RJ + a + 1;
-    const VisCoef c0 = lds_coef(L.xc, x - 1), c1 = lds_coef(L.xc, x), c2 = lds_coef(L.xc, x + 1);
-    const Tri fm = vis_face(lds_prim(L, c - 1), c0, p1, c1);
-    const Tri fp = vis_face(p1, c1, lds_prim(L, c + 1), c2);
-    const Tri gm = lds_tri(L.xg, x - 1), gc = lds_tri(L.xg, x);
-    if (j < M.je) {
-      const Tri gp = lds_tri(L.xg, x + 1);
-      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
-      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
-      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
-    } else {  // one-sided closure at jend (src/visrhs.f:262-271)
-      Tri gq;
-      if (a >= 1) {
-        gq = lds_tri(L.xg, x - 2);
-      } else {  // column j-2 lies left of the tile's ring
-        const VisPrim pq = vis_prim(M, cell(M, j - 2, k), c2b, c2bp, prtr);
-        const VisCoef cq = xi_coef(M, j - 2, k, pq, dre, gkpr);
-        gq = xi_cross(cq, div_prim(M, cell(M, j - 2, k + 1)), div_prim(M, cell(M, j - 2, k - 1)));
+    // ---- xi part (k = const line)
+    {
+      const int x = b * VIS_RJ + a + 1;
+      const VisCoef c0 = lds_coef(L.xc, x - 1), c1 = lds_coef(L.xc, x), c2 = lds_coef(L.xc, x + 1);
+      const Tri fm = vis_face(lds_prim(L, c - 1), c0, p1, c1);
+      const Tri fp = vis_face(p1, c1, lds_prim(L, c + 1), c2);
+      const Tri gm = lds_tri(L.xg, x - 1), gc = lds_tri(L.xg, x);
+      if (j < M.je) {
+        const Tri gp = lds_tri(L.xg, x + 1);
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gp.f2 - gm.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gp.f3 - gm.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gp.f4 - gm.f4);
+      } else {  // one-sided closure at jend (src/visrhs.f:262-271)
+        Tri gq;
+        if (a >= 1) {
+          gq = lds_tri(L.xg, x - 2);
+        } else {  // column j-2 lies left of the tile's ring
+          const VisPrim pq = vis_prim(M, cell(M, j - 2, k), c2b, c2bp, prtr);
+          const VisCoef cq = xi_coef(M, j - 2, k, pq, dre, gkpr);
+          gq = xi_cross(cq, div_prim(M, cell(M, j - 2, k + 1)), div_prim(M, cell(M, j - 2, k - 1)));
+        }
+        s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
+        s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
+        s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3 * gc.f4);
       }
-      s2 = s2 + (fp.f2 - fm.f2) + 0.5 * (gq.f2 - 4. * gm.f2 + 3 * gc.f2);
-      s3 = s3 + (fp.f3 - fm.f3) + 0.5 * (gq.f3 - 4. * gm.f3 + 3 * gc.f3);
-      s4 = s4 + (fp.f4 - fm.f4) + 0.5 * (gq.f4 - 4. * gm.f4 + 3 * gc.f4);
     }
+    M.s[1][i] = s2;
+    M.s[2][i] = s3;
+    M.s[3][i] = s4;
+    sv[0] = M.s[0][i]; sv[1] = s2; sv[2] = s3; sv[3] = s4;
   }
-  M.s[1][i] = s2;
-  M.s[2][i] = s3;
-  M.s[3][i] = s4;
+  if (norms) {
+    const ResTile T{(int)blockIdx.y, j0, k0, (int)blockIdx.x};
+    norm_tile(M, T, j, k, valid, sv, &red[0][0]);
+  }
 }
 
 // ========================================================================= SA
@@ -1250,7 +1266,7 @@ static long long max_cells(const DevMesh* hM, int nmi) {
   return mx;
 }
 
-hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int norms) {
   int tiles = 1;
   for (int i = 0; i < nmi; ++i) {
     const int nj = hM[i].je - hM[i].js + 1, nk = hM[i].ke - hM[i].ks + 1;
@@ -1263,7 +1279,7 @@ hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStrea
     if (e != hipSuccess) return e;
     lds_set = true;
   }
-  hipLaunchKernelGGL(k_visrhs, dim3(tiles, nmi), dim3(VIS_THREADS), VIS_LDS, st, dM);
+  hipLaunchKernelGGL(k_visrhs, dim3(tiles, nmi), dim3(VIS_THREADS), VIS_LDS, st, dM, norms);
   return hipGetLastError();
 }
 
