@@ -172,6 +172,17 @@ int32_t ot2d_batch_create(int32_t ncase, const ot2d_params* params, const ot2d_m
 int32_t ot2d_batch_status(ot2d_case* c, int32_t* status, int32_t* istep0);
 /* Stop advancing case icase (status OT2D_CONVERGED). */
 int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase);
+/* Put a new case into slot icase of a batch, whose case has stopped
+ * (converged, retired or failed): parameters p (same model, precision and
+ * instance count as the batch) and mesh md, which must fit the memory the
+ * slot holds (its cells, vertices, LU workspace and residual tiles, and no
+ * more rows per lane of the FP64 sweep than the batch's).  The slot's
+ * memory is cleared as for a new case, its step counter and status reset,
+ * and the captured step re-made at the next iteration; call ot2d_set_grid
+ * (and ot2d_set_convergence) for it next.  Batches of one mesh per case
+ * without overset connectivity.  For a request loop whose new requests take
+ * the slots of finished ones; no counterpart in the reference. */
+int32_t ot2d_batch_admit(ot2d_case* c, int32_t icase, const ot2d_params* p, const ot2d_mesh_desc* md);
 
 /* --------------------------------------------------------------- serving
  * Primitives of a request loop that keeps many cases resident and admits and

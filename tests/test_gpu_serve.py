@@ -158,3 +158,93 @@ def test_multiblock_fp32_cohorts_overlap():
     assert a.status == b.status and a.steps == b.steps
     np.testing.assert_array_equal(a.state, b.state)
     np.testing.assert_array_equal(a.forces, b.forces)
+
+
+
+def _batch_of(reqs, tol=None):
+    import turns2d_urans_git_amd as t2d
+    ps = [t2d.default_params(**r.deck) for r in reqs]
+    mds = [abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1)) for r in reqs]
+    b = t2d.Batch(ps, mds)
+    for i, r in enumerate(reqs):
+        b.set_grid(i, r.grid.xv, r.grid.yv)
+    if tol:
+        b.set_convergence(tol)
+    return b
+
+
+@pytest.mark.parametrize("fp32", [1, 0])
+def test_slot_admission_equals_alone(fp32):
+    """A case put into the slot of a stopped one (ot2d_batch_admit: a smaller
+    grid, other flow conditions) gives the state it gives alone after the
+    same number of steps, bit for bit, and the case running beside it is
+    not disturbed."""
+    import turns2d_urans_git_amd as t2d
+    first = _polar(0, 0.0, m=0.03, alfa=3.0, jmax=49, kmax=25, fp32=fp32)
+    other = _polar(1, 0.0, alfa=1.0, jmax=41, kmax=21, fp32=fp32)
+    new = _polar(2, 0.0, alfa=-2.0, m=0.01, jmax=45, kmax=21, fp32=fp32)
+    n0, n1 = 200, 300
+    ref = {}
+    for r, n in ((other, n0 + n1), (new, n1)):
+        b = _batch_of([r])
+        rc, _, done = b.iterate(n)
+        assert rc == 0 and done == n
+        ref[r.rid] = (b.get_state(0), b.forces(0))
+        b.close()
+    b = _batch_of([first, other])
+    rc, _, _ = b.iterate(n0)
+    assert rc == 0
+    b.batch_retire(0)
+    b.batch_admit(0, t2d.default_params(**new.deck),
+                  abi.mesh_desc(new.grid.jmax, new.grid.kmax, grid.cgrid_bc(new.grid.jtail1)))
+    b.set_grid(0, new.grid.xv, new.grid.yv)
+    rc, _, _ = b.iterate(n1)
+    assert rc == 0
+    st, ist = b.batch_status()
+    assert list(st) == [0, 0] and list(ist) == [n1, n0 + n1]
+    for i, r in ((0, new), (1, other)):
+        np.testing.assert_array_equal(b.get_state(i), ref[r.rid][0], err_msg=f"case {r.rid}")
+        assert b.forces(i) == ref[r.rid][1]
+    b.close()
+
+
+def test_slot_admission_refuses_what_does_not_fit():
+    """A slot still running, a larger grid than the slot was made for, or
+    another instance count: refused with an error."""
+    import turns2d_urans_git_amd as t2d
+    a = _polar(0, 0.0, jmax=41, kmax=21)
+    b = _batch_of([a, _polar(1, 0.0, alfa=4.0, jmax=41, kmax=21)])
+    big = _polar(2, 0.0, jmax=49, kmax=21)
+    md_big = abi.mesh_desc(big.grid.jmax, big.grid.kmax, grid.cgrid_bc(big.grid.jtail1))
+    md_a = abi.mesh_desc(a.grid.jmax, a.grid.kmax, grid.cgrid_bc(a.grid.jtail1))
+    with pytest.raises(abi.CaseError):      # slot 0 still running
+        b.batch_admit(0, t2d.default_params(**a.deck), md_a)
+    b.batch_retire(0)
+    with pytest.raises(abi.CaseError):      # 49 x 21 does not fit a 41 x 21 slot
+        b.batch_admit(0, t2d.default_params(**big.deck), md_big)
+    pitch = _pitch(3, 0.0, n=3)
+    with pytest.raises(abi.CaseError):      # time-spectral in a steady batch
+        b.batch_admit(0, t2d.default_params(**pitch.deck), md_a)
+    b.close()
+
+
+def test_server_fills_freed_slots():
+    """With memory for one cohort only, a request that arrives while the
+    first runs waits, then takes the first's slot when that stops (no new
+    cohort), with the result it gives alone."""
+    first = _polar(0, 0.0, m=0.03, alfa=3.0, jmax=49, kmax=25)
+    second = _polar(1, 0.001, alfa=-1.0, jmax=41, kmax=21)
+    lib, err = _lib()
+
+    def run(reqs, **kw):
+        srv = serve.Server(lib, steps_per_tick=K, tol=TOL, max_steps=400, keep_state=True, errfn=err, **kw)
+        return srv, srv.run(reqs)
+
+    one = serve.Server(lib, errfn=err).footprint([first])
+    srv, res = run([first, second], budget=int(1.2 * one))
+    assert srv.cohorts_made == 1 and srv.slot_admissions == 1
+    _, alone = run([second])
+    a, b = alone[0], res[1]
+    assert a.status == b.status and a.steps == b.steps
+    np.testing.assert_array_equal(a.state, b.state)
+    np.testing.assert_array_equal(a.forces, b.forces)

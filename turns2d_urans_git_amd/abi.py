@@ -131,6 +131,7 @@ def bind(lib: C.CDLL, prefix: str):
         "batch_create": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, P(C.c_void_p)]),
         "batch_status": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "batch_retire": (C.c_int32, [C.c_void_p, C.c_int32]),
+        "batch_admit": (C.c_int32, [C.c_void_p, C.c_int32, P(Params), P(MeshDesc)]),
         "connect": (C.c_int32, [C.c_void_p, C.c_void_p, P(C.c_int32)]),
         "interpolate": (C.c_int32, [C.c_void_p]),
         "set_connectivity": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
@@ -381,6 +382,17 @@ class CaseAPI:
 
     def batch_retire(self, icase: int):
         return self._check(self._fns["batch_retire"](self.h, int(icase)), "batch_retire")
+
+    def batch_admit(self, icase: int, params: "Params", mesh: "MeshDesc"):
+        """Put a new case (params, mesh) into the stopped slot icase; its grid
+        is set with set_grid next (ot2d_batch_admit)."""
+        rc = self._check(self._fns["batch_admit"](self.h, int(icase), C.byref(params), C.byref(mesh)),
+                         "batch_admit")
+        self.meshes[icase] = mesh
+        jd, kd, nq, nv = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        self._fns["sizes"](C.byref(params), C.byref(mesh), C.byref(jd), C.byref(kd), C.byref(nq), C.byref(nv))
+        self.dims[icase] = (jd.value, kd.value, nq.value, nv.value, mesh.jmax, mesh.kmax)
+        return rc
 
     def forces(self, im: int = 0, nsp: int = 0):
         out = np.zeros(6)

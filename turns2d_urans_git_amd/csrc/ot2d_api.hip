@@ -53,6 +53,11 @@ struct MeshBuf {
   double* sa = nullptr;  // 10 (jd,kd) planes per instance when SA is on
   size_t lu_per = 0;  // doubles of LU workspace per instance
   bool has_grid = false;
+  // what the arrays were allocated for (a batch slot may be refilled with a
+  // smaller mesh: ot2d_batch_admit), and the first norm-partial slot
+  size_t cap_n2 = 0, cap_nv2 = 0, cap_lu = 0;
+  int cap_tiles = 0;   // residual tiles per instance
+  int pofs = 0;
 };
 
 }  // namespace
@@ -369,6 +374,107 @@ static bool same_structure(const ot2d_params* a, const ot2d_params* b) {
 }
 
 // ncase parameter sets; mesh im belongs to case mesh_case[im]
+// DevMesh records of mesh im's instances (c->hM), from its buffers and
+// description (c->m[im]) and its case's parameters
+static int32_t mesh_records(ot2d_case* c, int im, const ot2d_params* pc) {
+  MeshBuf& mb = c->m[im];
+  const ot2d_mesh_desc& md = mb.d;
+  const int ns = c->nspec;
+  const ot2d_params* p = &c->p;
+  int jtail_v = 1;
+  for (int ib = 0; ib < md.nbc; ++ib)
+    if (md.bc[ib].ibtyp == 5) jtail_v = md.bc[ib].jbcs;  // src/init.f:59-66
+  const int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
+  for (int nsp = 0; nsp < ns; ++nsp) {
+    int mi = im * ns + nsp;
+    DevMesh& M = c->hM[mi];
+    memset(&M, 0, sizeof M);
+    M.jd = mb.jd; M.kd = mb.kd; M.jmax = mb.jmax; M.kmax = mb.kmax;
+    M.js = 2; M.je = mb.jd - 1; M.ks = 2; M.ke = mb.kd - 1;
+    M.nq = c->nq; M.nv = c->nv; M.nspec = ns; M.nsp = nsp;
+    M.jtail1 = jtail_v + 1;
+    M.jtail2 = p->half == 0 ? mb.jd - M.jtail1 + 1 : mb.jd - 1;
+    M.has_gv = p->timespectral ? 1 : 0;
+    {
+      double a = 10. / (double)mb.jmax, b = 10. / (double)mb.kmax;
+      M.epsj = a * a * a;  // (10./real(jmax))**3, src/eulerrhs.f:51
+      M.epsk = b * b * b;
+    }
+    M.nbc = md.nbc;
+    for (int ib = 0; ib < md.nbc; ++ib) {
+      M.bc[ib] = fix_bc(md.bc[ib], mb.jmax, mb.kmax, mb.jd, mb.kd);
+      if (!bc_in_range(M.bc[ib], mb.jd, mb.kd))
+        return fail(OT2D_ERR_ARG, "mesh %d boundary segment %d (type %d, dir %d) indexes outside the grid",
+                    im, ib + 1, md.bc[ib].ibtyp, md.bc[ib].ibdir);
+    }
+    size_t o = mb.n2 * nsp, ov = mb.nv2 * nsp;
+    for (int n = 0; n < c->nq; ++n) M.q[n] = mb.q + mb.n2 * ((size_t)n * ns + nsp);
+    for (int n = 0; n < c->nv; ++n) M.s[n] = mb.s + mb.n2 * ((size_t)n * ns + nsp);
+    for (int n = 0; n < c->nq; ++n) M.qall[n] = mb.q + mb.n2 * ((size_t)n * ns);
+    M.inst_stride = (long long)mb.n2;
+    M.x = mb.x + o; M.y = mb.y + o; M.xx = mb.xx + o; M.xy = mb.xy + o; M.yx = mb.yx + o; M.yy = mb.yy + o;
+    M.ug = mb.ug + o; M.vg = mb.vg + o; M.turmu = mb.turmu + o; M.tscale = mb.tscale + o; M.bt = mb.bt + o;
+    M.xv = mb.xv + ov; M.yv = mb.yv + ov; M.ugv = mb.ugv + ov; M.vgv = mb.vgv + ov;
+    M.xv0 = mb.xv; M.yv0 = mb.yv;
+    M.iblank = mb.iblank + o;
+    M.lu = mb.lu + mb.lu_per * nsp;
+    M.lu_rl = c->lu_rl;
+    M.lu_fp32 = p->lu_fp32 ? 1 : 0;
+    M.bodyflag = 0;
+    for (int ib = 0; ib < md.nbc; ++ib)
+      if (md.bc[ib].ibtyp == 5) M.bodyflag = 1;  // src/init.f:59-66
+    if (mb.sa) {
+      const size_t np = c->iturb == 2 ? 20 : 10;
+      double* b = mb.sa + np * mb.n2 * nsp;
+      M.sa_vmul = b; M.sa_u = b + mb.n2; M.sa_v = b + 2 * mb.n2; M.sa_sn = b + 3 * mb.n2;
+      if (c->iturb == 1) {
+        M.sa_sj = b + 4 * mb.n2; M.sa_aj = b + 5 * mb.n2; M.sa_cj = b + 6 * mb.n2;
+        M.sa_ak = b + 7 * mb.n2; M.sa_ck = b + 8 * mb.n2; M.sa_bb = b + 9 * mb.n2;
+      } else {
+        M.kw_f1 = b + 4 * mb.n2; M.kw_x = b + 5 * mb.n2; M.kw_prod = b + 6 * mb.n2; M.kw_cd = b + 7 * mb.n2;
+        for (int n = 0; n < 2; ++n) {
+          double* e = b + (8 + 6 * n) * mb.n2;
+          M.kw_s[n] = e; M.kw_a[n] = e + mb.n2; M.kw_b[n] = e + 2 * mb.n2;
+          M.kw_c[n] = e + 3 * mb.n2; M.kw_ak[n] = e + 4 * mb.n2; M.kw_ck[n] = e + 5 * mb.n2;
+        }
+      }
+    }
+    const int ic = c->mesh_case[im];
+    M.status = c->d_status + ic;
+    M.istep0 = c->d_istep0 + ic;
+    M.stepc = c->d_stepc + ic;
+    M.norms = c->d_norms + 6 * mi;
+    M.tspec = c->d_tspec + (size_t)ic * ns + nsp;
+    M.Ds = c->d_Ds;
+    M.c = make_const(pc, c->iturb, 4);
+    M.c.rotf = (pc->rf != 0.0) ? md.irot_dir * pc->rf : 0.0;  // src/setup.f:96-101
+    M.c.xac = md.xcen; M.c.yac = md.ycen;
+    const int pofs = mb.pofs + nsp * tj * tk;
+    M.res_block0 = pofs;
+    M.res_nblocks = tj * tk;
+    M.partials = c->d_partials + 4 * (size_t)pofs;
+  }
+  return OT2D_OK;
+}
+
+// the residual tiles of every instance, mesh-major (c->hT)
+static void build_tiles(ot2d_case* c) {
+  c->hT.clear();
+  for (int im = 0; im < c->nmesh; ++im) {
+    const MeshBuf& mb = c->m[im];
+    const int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
+    for (int nsp = 0; nsp < c->nspec; ++nsp) {
+      int b = 0;
+      for (int kt = 0; kt < tk; ++kt)
+        for (int jt = 0; jt < tj; ++jt) {
+          ResTile T;
+          T.mesh = im * c->nspec + nsp; T.j0 = 2 + jt * RES_TJ; T.k0 = 2 + kt * RES_TK; T.blk = b++;
+          c->hT.push_back(T);
+        }
+    }
+  }
+}
+
 static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, const ot2d_mesh_desc* md,
                            const int32_t* mesh_case, ot2d_case** out) {
   *out = nullptr;
@@ -488,99 +594,14 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     A(mb.lu, mb.lu_per * ns);
     if (!c->lamin) A(mb.sa, (c->iturb == 2 ? 20 : 10) * mb.n2 * ns);
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
+    mb.cap_n2 = mb.n2; mb.cap_nv2 = mb.nv2; mb.cap_lu = mb.lu_per; mb.cap_tiles = tj * tk;
+    mb.pofs = ntile_total;
     ntile_total += tj * tk * ns;
   }
   A(c->d_partials, 4 * (size_t)ntile_total);
-  c->hT.reserve(ntile_total);
-  int pofs = 0;
-  for (int im = 0; im < nmesh; ++im) {
-    MeshBuf& mb = c->m[im];
-    int jtail_v = 1;
-    for (int ib = 0; ib < md[im].nbc; ++ib)
-      if (md[im].bc[ib].ibtyp == 5) jtail_v = md[im].bc[ib].jbcs;  // src/init.f:59-66
-    for (int nsp = 0; nsp < ns; ++nsp) {
-      int mi = im * ns + nsp;
-      DevMesh& M = c->hM[mi];
-      memset(&M, 0, sizeof M);
-      M.jd = mb.jd; M.kd = mb.kd; M.jmax = mb.jmax; M.kmax = mb.kmax;
-      M.js = 2; M.je = mb.jd - 1; M.ks = 2; M.ke = mb.kd - 1;
-      M.nq = c->nq; M.nv = c->nv; M.nspec = ns; M.nsp = nsp;
-      M.jtail1 = jtail_v + 1;
-      M.jtail2 = p->half == 0 ? mb.jd - M.jtail1 + 1 : mb.jd - 1;
-      M.has_gv = p->timespectral ? 1 : 0;
-      M.epsj = pow(10. / (double)mb.jmax, 3);
-      M.epsk = pow(10. / (double)mb.kmax, 3);
-      {
-        double a = 10. / (double)mb.jmax, b = 10. / (double)mb.kmax;
-        M.epsj = a * a * a;  // (10./real(jmax))**3, src/eulerrhs.f:51
-        M.epsk = b * b * b;
-      }
-      M.nbc = md[im].nbc;
-      for (int ib = 0; ib < md[im].nbc; ++ib) {
-        M.bc[ib] = fix_bc(md[im].bc[ib], mb.jmax, mb.kmax, mb.jd, mb.kd);
-        if (!bc_in_range(M.bc[ib], mb.jd, mb.kd)) {
-          free_case(c);
-          return fail(OT2D_ERR_ARG, "mesh %d boundary segment %d (type %d, dir %d) indexes outside the grid",
-                      im, ib + 1, md[im].bc[ib].ibtyp, md[im].bc[ib].ibdir);
-        }
-      }
-      size_t o = mb.n2 * nsp, ov = mb.nv2 * nsp;
-      for (int n = 0; n < c->nq; ++n) M.q[n] = mb.q + mb.n2 * ((size_t)n * ns + nsp);
-      for (int n = 0; n < c->nv; ++n) M.s[n] = mb.s + mb.n2 * ((size_t)n * ns + nsp);
-      for (int n = 0; n < c->nq; ++n) M.qall[n] = mb.q + mb.n2 * ((size_t)n * ns);
-      M.inst_stride = (long long)mb.n2;
-      M.x = mb.x + o; M.y = mb.y + o; M.xx = mb.xx + o; M.xy = mb.xy + o; M.yx = mb.yx + o; M.yy = mb.yy + o;
-      M.ug = mb.ug + o; M.vg = mb.vg + o; M.turmu = mb.turmu + o; M.tscale = mb.tscale + o; M.bt = mb.bt + o;
-      M.xv = mb.xv + ov; M.yv = mb.yv + ov; M.ugv = mb.ugv + ov; M.vgv = mb.vgv + ov;
-      M.xv0 = mb.xv; M.yv0 = mb.yv;
-      M.iblank = mb.iblank + o;
-      M.lu = mb.lu + mb.lu_per * nsp;
-      M.lu_rl = c->lu_rl;
-      M.lu_fp32 = p->lu_fp32 ? 1 : 0;
-      M.bodyflag = 0;
-      for (int ib = 0; ib < md[im].nbc; ++ib)
-        if (md[im].bc[ib].ibtyp == 5) M.bodyflag = 1;  // src/init.f:59-66
-      if (mb.sa) {
-        const size_t np = c->iturb == 2 ? 20 : 10;
-        double* b = mb.sa + np * mb.n2 * nsp;
-        M.sa_vmul = b; M.sa_u = b + mb.n2; M.sa_v = b + 2 * mb.n2; M.sa_sn = b + 3 * mb.n2;
-        if (c->iturb == 1) {
-          M.sa_sj = b + 4 * mb.n2; M.sa_aj = b + 5 * mb.n2; M.sa_cj = b + 6 * mb.n2;
-          M.sa_ak = b + 7 * mb.n2; M.sa_ck = b + 8 * mb.n2; M.sa_bb = b + 9 * mb.n2;
-        } else {
-          M.kw_f1 = b + 4 * mb.n2; M.kw_x = b + 5 * mb.n2; M.kw_prod = b + 6 * mb.n2; M.kw_cd = b + 7 * mb.n2;
-          for (int n = 0; n < 2; ++n) {
-            double* e = b + (8 + 6 * n) * mb.n2;
-            M.kw_s[n] = e; M.kw_a[n] = e + mb.n2; M.kw_b[n] = e + 2 * mb.n2;
-            M.kw_c[n] = e + 3 * mb.n2; M.kw_ak[n] = e + 4 * mb.n2; M.kw_ck[n] = e + 5 * mb.n2;
-          }
-        }
-      }
-      const int ic = mesh_case[im];
-      const ot2d_params* pc = ps + ic;
-      M.status = c->d_status + ic;
-      M.istep0 = c->d_istep0 + ic;
-      M.stepc = c->d_stepc + ic;
-      M.norms = c->d_norms + 6 * mi;
-      M.tspec = c->d_tspec + (size_t)ic * ns + nsp;
-      M.Ds = c->d_Ds;
-      M.c = make_const(pc, c->iturb, 4);
-      M.c.rotf = (pc->rf != 0.0) ? md[im].irot_dir * pc->rf : 0.0;  // src/setup.f:96-101
-      M.c.xac = md[im].xcen; M.c.yac = md[im].ycen;
-      int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
-      M.res_block0 = pofs;
-      M.res_nblocks = tj * tk;
-      M.partials = c->d_partials + 4 * (size_t)pofs;
-      int b = 0;
-      for (int kt = 0; kt < tk; ++kt)
-        for (int jt = 0; jt < tj; ++jt) {
-          ResTile T;
-          T.mesh = mi; T.j0 = 2 + jt * RES_TJ; T.k0 = 2 + kt * RES_TK; T.blk = b++;
-          c->hT.push_back(T);
-        }
-      pofs += tj * tk;
-    }
-  }
+  for (int im = 0; im < nmesh; ++im)
+    if (int32_t rc = mesh_records(c, im, ps + mesh_case[im])) { free_case(c); return rc; }
+  build_tiles(c);
 #undef A
   if ((e = dalloc(&c->dM, c->hM.size())) != hipSuccess || (e = dalloc(&c->dT, c->hT.size())) != hipSuccess) {
     free_case(c);
@@ -1287,6 +1308,83 @@ extern "C" int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase) {
   if (!c || icase < 0 || icase >= c->ncase) return fail(OT2D_ERR_ARG, "bad case index %d", icase);
   static const int retired = 4;
   HIPCHK(hipMemcpyAsync(c->d_status + icase, &retired, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_batch_admit(ot2d_case* c, int32_t icase, const ot2d_params* p,
+                                    const ot2d_mesh_desc* md) {
+  if (!c || !p || !md) return fail(OT2D_ERR_ARG, "null argument");
+  if (!c->batch || c->nmesh != c->ncase) return fail(OT2D_ERR_ARG, "not a batch of one mesh per case");
+  if (icase < 0 || icase >= c->ncase) return fail(OT2D_ERR_ARG, "bad case index %d", icase);
+  if (c->ndon > 0 || c->nfr > 0) return fail(OT2D_ERR_UNSUPPORTED, "a batch with overset connectivity");
+  if (int32_t rc = check_params(p)) return rc;
+  if (!same_structure(&c->p, p)) return fail(OT2D_ERR_ARG, "model, precision and instance count must match");
+  if (p->ilim == 4 && !c->weno) return fail(OT2D_ERR_ARG, "WENO5 case in a batch made without one");
+  if (int32_t rc = sync(c)) return rc;
+  int st = 0;
+  HIPCHK(hipMemcpy(&st, c->d_status + icase, sizeof(int), hipMemcpyDeviceToHost));
+  if (st == 0) return fail(OT2D_ERR_ARG, "case %d is still running", icase);
+  MeshBuf& mb = c->m[icase];
+  const int ns = c->nspec;
+  const int jmax = md->jmax, kmax = md->kmax, jd = jmax + 1, kd = kmax + 1;
+  const size_t n2 = (size_t)jd * kd, nv2 = (size_t)jmax * kmax;
+  const int rl = (kmax - 1 + 63) / 64;
+  const size_t lu_per = lu_workspace_bytes(jd, kd, c->lu_rl) / sizeof(double);
+  const int tiles = ((jd - 2 + RES_TJ - 1) / RES_TJ) * ((kd - 2 + RES_TK - 1) / RES_TK);
+  if (jmax < 5 || kmax < 5 || (p->ilim == 4 && (jmax < 7 || kmax < 7)))
+    return fail(OT2D_ERR_ARG, "mesh too small");
+  if (md->nbc < 0 || md->nbc > OT2D_MAXBC) return fail(OT2D_ERR_ARG, "nbc");
+  if (n2 > mb.cap_n2 || nv2 > mb.cap_nv2 || lu_per > mb.cap_lu || tiles > mb.cap_tiles || rl > c->lu_rl)
+    return fail(OT2D_ERR_ARG, "a %d x %d mesh does not fit slot %d (allocated for %d x %d)", jmax, kmax, icase,
+                mb.jmax, mb.kmax);
+  if (!c->lamin && (jmax - 1 > SA_LINE_MAX || kmax - 1 > SA_LINE_MAX))
+    return fail(OT2D_ERR_UNSUPPORTED, "the turbulence line solves hold at most %d cells per line", SA_LINE_MAX);
+  if (lu_lds_need(jmax - 1, p->lu_fp32) > 160 * 1024)
+    return fail(OT2D_ERR_UNSUPPORTED, "%d x %d exceeds the LU-SGS sweep's LDS staging", jmax, kmax);
+  if (c->iturb == 2 && !c->lamin)
+    for (int ib = 0; ib < md->nbc; ++ib)
+      if (md->bc[ib].ibtyp == 5) {
+        DevBC bb = fix_bc(md->bc[ib], jmax, kmax, jd, kd);
+        if (abs(bb.idir) != 2 || bb.js < 2 || bb.je > jmax)
+          return fail(OT2D_ERR_UNSUPPORTED, "SST wall segment %d must lie along k on interior columns", ib + 1);
+      }
+  for (int ib = 0; ib < md->nbc; ++ib)
+    if (!bc_in_range(fix_bc(md->bc[ib], jmax, kmax, jd, kd), jd, kd))
+      return fail(OT2D_ERR_ARG, "boundary segment %d indexes outside the grid", ib + 1);
+  // the slot's memory as a new case finds it: zero (the FP32 sweeps' zero
+  // records off the grid included)
+  const size_t cap2 = mb.cap_n2 * ns, capv = mb.cap_nv2 * ns;
+  double* d2[] = {mb.x, mb.y, mb.xx, mb.xy, mb.yx, mb.yy, mb.ug, mb.vg, mb.turmu, mb.tscale, mb.bt};
+  for (double* d : d2) HIPCHK(hipMemsetAsync(d, 0, cap2 * sizeof(double), c->stream));
+  double* dv[] = {mb.xv, mb.yv, mb.ugv, mb.vgv};
+  for (double* d : dv) HIPCHK(hipMemsetAsync(d, 0, capv * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(mb.q, 0, cap2 * c->nq * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(mb.s, 0, cap2 * c->nv * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(mb.iblank, 0, cap2 * sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(mb.lu, 0, mb.cap_lu * ns * sizeof(double), c->stream));
+  if (mb.sa) HIPCHK(hipMemsetAsync(mb.sa, 0, (c->iturb == 2 ? 20 : 10) * cap2 * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_partials + 4 * (size_t)mb.pofs, 0, 4 * (size_t)mb.cap_tiles * ns * sizeof(double),
+                        c->stream));
+  mb.d = *md;
+  mb.jmax = jmax; mb.kmax = kmax; mb.jd = jd; mb.kd = kd; mb.n2 = n2; mb.nv2 = nv2; mb.lu_per = lu_per;
+  mb.has_grid = false;
+  if (int32_t rc = mesh_records(c, icase, p)) return rc;
+  build_tiles(c);
+  // the case's time-spectral instants (src/init.f:71-78); the host buffers
+  // of these copies live until the sync below
+  std::vector<double> tspec(ns, 0.0);
+  const double pi = 4. * atan(1.);
+  for (int n = 0; n < ns; ++n) tspec[n] = (p->rf != 0.0) ? 2 * pi * n / ns / fabs(p->rf) : 0.0;
+  HIPCHK(hipMemcpyAsync(c->d_tspec + (size_t)icase * ns, tspec.data(), sizeof(double) * ns,
+                        hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice, c->stream));
+  // the tile list keeps its allocation: every mesh's count is within its capacity
+  HIPCHK(hipMemcpyAsync(c->dT, c->hT.data(), sizeof(ResTile) * c->hT.size(), hipMemcpyHostToDevice, c->stream));
+  const int zero[1] = {0};
+  HIPCHK(hipMemcpyAsync(c->d_istep0 + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_stepc + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_status + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  c->graph_valid = 0;   // the tiles and launch sizes may have changed
   return sync(c);
 }
 

@@ -3,19 +3,24 @@
 Requests (steady polar points and time-spectral pitching cases) arrive over
 time.  Between iterations the loop
 
-* admits waiting requests, in arrival order, while the device memory they
-  need (``ot2d_footprint``) fits the budget: the requests admitted at one
-  moment that share a structure (model, instance count) become one batch
-  (``ot2d_batch_create``), a *cohort*, with its own HIP stream;
+* admits waiting requests, in arrival order: a request first takes the
+  slot of a finished case in a resident cohort of its structure whose
+  memory it fits (``ot2d_batch_admit``: the slot is cleared and re-made
+  for the new case, the other cases keep running); the rest, while the
+  device memory they need (``ot2d_footprint``) fits the budget, become new
+  batches (``ot2d_batch_create``), one *cohort* per structure (model,
+  precision, instance count) and admission moment, each with its own HIP
+  stream;
 * advances every resident cohort by ``steps_per_tick`` iterations with
   ``ot2d_iterate_async``, so the cohorts' kernels overlap on the device
-  (cohorts with FP32 LU-SGS run one after another: overlapped, they have
-  now and then corrupted a case);
+  (the library orders the FP32 LU-SGS work of different cohorts among
+  themselves; FP64 work overlaps with everything);
 * reads every cohort's per-case states (``ot2d_batch_status``): a case whose
   residual has reached the tolerance stopped on the device (status
   ``CONVERGED``, set by the step itself before its update), and its
-  time-to-converge is recorded; a cohort whose cases have all finished is
-  destroyed and its memory returns to the budget.
+  time-to-converge is recorded and its slot becomes free; a cohort with no
+  case left running, whose free slots no waiting request took, is destroyed
+  and its memory returns to the budget.
 
 State never leaves the device while a case runs.  Cases never interact: a
 case gives the same result alone and in any mix (the kernels of one case
@@ -108,7 +113,31 @@ class _Cohort:
             self.case.set_grid(i, r.grid.xv, r.grid.yv)
         self.case.set_convergence(tol)
         self.open = set(range(len(reqs)))
+        self.free = set()                      # slots whose case has finished
+        # the mesh each slot was allocated for: a request fits a slot when
+        # its grid is no larger in either direction
+        self.cap = [(r.grid.jmax, r.grid.kmax) for r in reqs]
+        self.key = reqs[0].key()
         self.bytes = self.case.bytes()
+        self.lib = lib
+        self.tol = tol
+
+    def fits(self, i: int, r: Request) -> bool:
+        return r.key() == self.key and r.grid.jmax <= self.cap[i][0] and r.grid.kmax <= self.cap[i][1]
+
+    def admit(self, i: int, r: Request):
+        """Put request r into the free slot i (ot2d_batch_admit, then its grid
+        and tolerance)."""
+        p = abi.Params()
+        self.lib.ot2d_default_params(C.byref(p))
+        p.update(**r.deck)
+        md = abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1))
+        self.case.batch_admit(i, p, md)
+        self.case.set_grid(i, r.grid.xv, r.grid.yv)
+        self.case.set_convergence(self.tol, i)
+        self.reqs[i] = r
+        self.free.discard(i)
+        self.open.add(i)
 
     def close(self):
         self.case.close()
@@ -135,6 +164,8 @@ class Server:
         self.errfn = errfn
         self.peak_bytes = 0
         self.ticks = 0
+        self.slot_admissions = 0   # requests that took a finished case's slot
+        self.cohorts_made = 0
 
     def footprint(self, reqs: Sequence[Request]) -> int:
         params = []
@@ -160,7 +191,32 @@ class Server:
                 break
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            # ---- admission: arrival order, while the footprint fits
+            # ---- admission into the free slots of resident cohorts, arrival
+            # order, each request into the smallest slot it fits
+            still: deque = deque()
+            while waiting:
+                r = waiting.popleft()
+                best = None
+                for c in cohorts:
+                    for i in c.free:
+                        if c.fits(i, r):
+                            area = c.cap[i][0] * c.cap[i][1]
+                            if best is None or area < best[0]:
+                                best = (area, c, i)
+                if best is None:
+                    still.append(r)
+                    continue
+                _, c, i = best
+                c.admit(i, r)
+                results[r.rid].admitted = clock() - t0
+                self.slot_admissions += 1
+            waiting = still
+            # cohorts with no case left running give their memory back
+            for c in [c for c in cohorts if not c.open]:
+                held -= c.bytes
+                c.close()
+                cohorts.remove(c)
+            # ---- new cohorts: arrival order, while the footprint fits
             admitted: Dict[tuple, List[Request]] = {}
             while waiting:
                 r = waiting[0]
@@ -174,6 +230,7 @@ class Server:
                 admitted.setdefault(r.key(), []).append(waiting.popleft())
             for group in admitted.values():
                 c = _Cohort(self.lib, group, self.tol, self.errfn)
+                self.cohorts_made += 1
                 held += c.bytes - self.footprint(group)   # the case's own count
                 tnow = clock() - t0
                 for r in group:
@@ -184,13 +241,16 @@ class Server:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
-            # ---- one tick: every cohort advances, their kernels overlapping
-            # on the device (each cohort has its own stream)
+            # ---- one tick: every cohort with a running case advances, their
+            # kernels overlapping on the device (each cohort has its own
+            # stream)
             for c in cohorts:
-                c.case.iterate_async(self.k)
+                if c.open:
+                    c.case.iterate_async(self.k)
             self.ticks += 1
-            done_cohorts = []
             for c in cohorts:
+                if not c.open:
+                    continue
                 st, ist = c.case.batch_status()
                 tnow = clock() - t0
                 for i in list(c.open):
@@ -209,12 +269,13 @@ class Server:
                         if self.keep_state:
                             res.state = c.case.get_state(i)
                         c.open.discard(i)
-                if not c.open:
-                    done_cohorts.append(c)
-            for c in done_cohorts:
-                held -= c.bytes
-                c.close()
-                cohorts.remove(c)
+                        c.free.add(i)
+            if not pending and not waiting:
+                # nothing left to take the free slots
+                for c in [c for c in cohorts if not c.open]:
+                    held -= c.bytes
+                    c.close()
+                    cohorts.remove(c)
         for c in cohorts:
             c.close()
         return [results[r.rid] for r in requests]
