@@ -118,7 +118,7 @@ def batch_groups(cells, ngroups):
     return [sorted(order[g::ngroups]) for g in range(ngroups) if order[g::ngroups]]
 
 
-def converge_batch(t2d, hip, params, meshes, grids, steps, chunk, label, groups=None, first=None):
+def converge_batch(t2d, hip, params, meshes, grids, steps, chunk, label, groups=None, first=None, tune=0):
     """Advance the cases until every one has stopped (rsum <= 1e-10 on the
     device) or `steps` steps; returns (status, steps per case, seconds,
     steps run, forces per case).  With `groups`, each group of cases is one
@@ -126,7 +126,10 @@ def converge_batch(t2d, hip, params, meshes, grids, steps, chunk, label, groups=
     groups' kernels overlap on the device (a latency-bound sweep of one group
     beside bandwidth-bound kernels of another).  With `first` (a list), the
     first step runs synchronously and each case's rsum(1) is appended to it
-    (inside the timed region, which it belongs to)."""
+    (inside the timed region, which it belongs to).  With `tune` > 0 each
+    batch takes its launch choices from ot2d_autotune (`tune` steps per
+    candidate; a cached class is not measured again) before the clock
+    starts."""
     groups = groups or [list(range(len(params)))]
     bs = []
     for grp in groups:
@@ -134,6 +137,8 @@ def converge_batch(t2d, hip, params, meshes, grids, steps, chunk, label, groups=
         for n, i in enumerate(grp):
             b.set_grid(n, grids[i].xv, grids[i].yv)
         b.set_convergence(1e-10)
+        if tune > 0:
+            b.autotune(tune)
         bs.append(b)
     n = len(params)
     st = np.zeros(n, dtype=np.int32)
@@ -272,6 +277,9 @@ def mix_section(args, hip):
     for i, g in enumerate(grids):
         b.set_grid(i, g.xv, g.yv)
     rc, _, _ = b.iterate(10)
+    if args.autotune > 0:
+        sw, sl, _ = b.autotune(args.autotune)
+        out["launch"] = {"sweep": sw, "sa_lane": sl}
     device_sync(hip)
     t0 = time.perf_counter()
     rc, _, d = b.iterate(args.mix_steps)
@@ -304,7 +312,7 @@ def mix_section(args, hip):
     # convergence of the batch
     r1 = []
     st, ist, csecs, total, forces = converge_batch(t2d, hip, p32, meshes, grids, args.mix_converge, 2000,
-                                                   "mix converge", first=r1)
+                                                   "mix converge", first=r1, tune=args.autotune)
     conv = st == abi.CONVERGED
     nconv = int(conv.sum())
     r1 = np.array(r1)
@@ -411,7 +419,7 @@ def run_batch(args, hip):
     n = len(cases)
     groups = batch_groups(cells, args.batch_groups) if args.batch_groups > 1 else None
     st, ist, secs, total, forces = converge_batch(t2d, hip, params, meshes, grids, args.steps, args.batch_chunk, "batch",
-                                                  groups)
+                                                  groups, tune=args.autotune)
     conv = st == abi.CONVERGED
     nconv = int(conv.sum())
     done_work = float((cells * ist).sum())
@@ -622,11 +630,22 @@ def run_serve(args):
     from turns2d_urans_git_amd import serve
     reqs = serve.poisson_stream(args.rate, args.duration, scale=args.serve_scale)
     srv = serve.Server(t2d.load_library(), steps_per_tick=args.serve_tick, tol=1e-10,
-                       max_steps=args.serve_max_steps, errfn=t2d.last_error)
+                       max_steps=args.serve_max_steps, errfn=t2d.last_error, tune_steps=args.autotune,
+                       max_resident=args.serve_max_resident, max_cohort=args.serve_cohort if args.serve_cohort > 0 else 32,
+                       slot_dims=serve.stream_slot_dims(args.serve_scale) if args.serve_cohort > 0 else None)
     print(f"[bench] serve: {len(reqs)} requests over {args.duration} s at {args.rate}/s, grids x{args.serve_scale}",
           file=sys.stderr, flush=True)
+    last = [0.0]
+
+    def progress(d):
+        if d["t"] - last[0] >= 15.0:
+            last[0] = d["t"]
+            print(f"[bench] serve t={d['t']:.0f} s: {d['pending']} to arrive, {d['waiting']} waiting, "
+                  f"{d['resident']} resident in {d['cohorts']} cohorts, {d['finished']} finished",
+                  file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
-    res = srv.run(reqs, time_limit=args.serve_time_limit)
+    res = srv.run(reqs, time_limit=args.serve_time_limit, progress=progress)
     wall = time.perf_counter() - t0
     s = serve.summarize(res, wall)
     out = {
@@ -637,8 +656,11 @@ def run_serve(args):
                  f"time-spectral pitching cases (N in 3,5,7), grids scaled by {args.serve_scale:g}"),
         "config": {"workload": "serve", "rate_per_s": args.rate, "duration_s": args.duration,
                    "grid_scale": args.serve_scale, "steps_per_tick": args.serve_tick,
-                   "max_steps": args.serve_max_steps},
-        "serve": dict(s, peak_bytes=srv.peak_bytes, budget_bytes=srv.budget,
+                   "max_steps": args.serve_max_steps, "autotune_steps": args.autotune,
+                   "max_resident": args.serve_max_resident, "time_limit_s": args.serve_time_limit,
+                   "cohort_slots": args.serve_cohort},
+        "serve": dict(s, peak_bytes=srv.peak_bytes, budget_bytes=srv.budget, cohorts_made=srv.cohorts_made,
+                      slot_admissions=srv.slot_admissions,
                       not_converged=sum(1 for r in res if r.status == -100),
                       failed=sum(1 for r in res if r.status not in (abi.CONVERGED, -100) and r.finished == r.finished)),
     }
@@ -667,6 +689,11 @@ def main():
     ap.add_argument("--serve-tick", type=int, default=200, help="serve: steps between admissions")
     ap.add_argument("--serve-max-steps", type=int, default=30000)
     ap.add_argument("--serve-time-limit", type=float, default=None)
+    ap.add_argument("--serve-cohort", type=int, default=32,
+                    help="serve: slots per cohort, each sized for its structure's largest grid (0: one slot per "
+                         "request, sized to it)")
+    ap.add_argument("--serve-max-resident", type=int, default=None,
+                    help="serve: cases resident at once (default: the memory budget alone)")
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-chunk", type=int, default=1000)
     ap.add_argument("--batch-groups", type=int, default=1,
@@ -683,6 +710,8 @@ def main():
                     help="run a fresh case up to N steps until rsum <= 1e-10; report solves/hour (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size (counter scaling)")
+    ap.add_argument("--autotune", type=int, default=20,
+                    help="steps per candidate of the launch-choice search before the timed run (0: defaults)")
     ap.add_argument("--mix", type=int, default=1,
                     help="sa_513x129: also run the 64-case mix comparison (configuration 4; 0: skip)")
     ap.add_argument("--mix-steps", type=int, default=100, help="mix: graph-replayed batch steps timed")
@@ -724,6 +753,15 @@ def main():
     rc, _, done = case.iterate(args.warmup)
     if rc != 0 or done != args.warmup:
         raise SystemExit(f"warmup stopped: rc={rc} after {done} steps")
+    launch = {"sweep": 0, "sa_lane": -1, "searched": False}
+    if args.autotune > 0:
+        # the launch choices (FP32 sweep ring and producers, SA line
+        # mapping) change speed, never results; the search puts the state back
+        ta = time.perf_counter()
+        sw, sl, measured = case.autotune(args.autotune)
+        launch = {"sweep": sw, "sa_lane": sl, "searched": measured,
+                  "search_s": round(time.perf_counter() - ta, 3)}
+        print(f"[bench] launch choices {launch}", file=sys.stderr, flush=True)
 
     device_sync(hip)
     t0 = time.perf_counter()
@@ -753,11 +791,11 @@ def main():
                     "interior_cells": cells, "model": w["model"],
                     "lu_sgs": "fp32" if w["lu_fp32"] else "fp64", "residual": "fp64",
                     "wall_spacing": w["ds_wall"], "outer_radius": 20.0,
-                    "residual_l2": float(hist[-1, 0])} if not overset else
+                    "residual_l2": float(hist[-1, 0]), "launch": launch} if not overset else
                    {"workload": args.workload, "near_body": list(w["near"]), "background": list(w["back"]),
                     "interior_cells": cells, "fringe_cells": nfr, "model": "sst+weno5",
                     "lu_sgs": "fp32" if w["lu_fp32"] else "fp64", "residual": "fp64",
-                    "residual_l2": [float(v) for v in hist[-1]]}),
+                    "residual_l2": [float(v) for v in hist[-1]], "launch": launch}),
     }
 
     if args.profile_steps > 0:
@@ -785,6 +823,8 @@ def main():
         # src/step.f:60-93) reaches 1e-10; the free-stream start has a
         # near-zero first residual, so the drop is also given from the peak
         c2 = make_gpu_case(w, g)
+        if args.autotune > 0:
+            c2.autotune(args.autotune)   # the headline's class: the cached choice
         steps, conv, peak, chunk, rsum1, low = 0, None, 0.0, 1000, None, np.inf
         device_sync(hip)
         tc0 = time.perf_counter()

@@ -184,6 +184,28 @@ int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase);
  * the slots of finished ones; no counterpart in the reference. */
 int32_t ot2d_batch_admit(ot2d_case* c, int32_t icase, const ot2d_params* p, const ot2d_mesh_desc* md);
 
+/* ------------------------------------------------------------ launch choices
+ * Launch shapes that change speed but not results: sweep (0, 1 or 2) picks
+ * the FP32 LU-SGS sweep's LDS ring depth and producer count (12 slots and
+ * 3 producer waves, 18 and 3, 12 and 4); sa_lane picks the SA line solves'
+ * mapping (-1 by line count, 0 one workgroup per line, 1 one lane per line).
+ * Every choice gives bit-identical states.  No counterpart in the reference,
+ * whose loops have one shape. */
+int32_t ot2d_set_launch(ot2d_case* c, int32_t sweep, int32_t sa_lane);
+int32_t ot2d_get_launch(ot2d_case* c, int32_t* sweep, int32_t* sa_lane);
+/* Measured search over the launch choices for the case's shape class (model,
+ * precision, instances, case count, and cells, jmax and sweep blocks):
+ * each candidate times `steps` captured steps from the case's current state
+ * (HIP events on the case's stream, best of three interleaved rounds), the
+ * state (q, eddy viscosity, step counters, status) is put back after each,
+ * and the fastest is set on the case.  Results are cached per class in
+ * the process and, when OT2D_TUNE_CACHE names a file, in that file (one line
+ * per class, read at the first call, appended after each search); a cached
+ * class is set without measuring.  *measured (may be NULL) is 1 when a
+ * search ran, 0 on a cache hit.  Allocates a copy of the state while it
+ * runs. */
+int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured);
+
 /* --------------------------------------------------------------- serving
  * Primitives of a request loop that keeps many cases resident and admits and
  * retires them between iterations.  No counterpart in the reference (its

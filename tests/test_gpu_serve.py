@@ -248,3 +248,33 @@ def test_server_fills_freed_slots():
     assert a.status == b.status and a.steps == b.steps
     np.testing.assert_array_equal(a.state, b.state)
     np.testing.assert_array_equal(a.forces, b.forces)
+
+
+def test_presized_cohorts_take_later_arrivals():
+    """With slot_dims every cohort has max_cohort slots of its structure's
+    largest grid: requests arriving while it runs take its free slots (FP32
+    sweeps of W = 1 and W = 2 blocks side by side), a second cohort of the
+    structure is made only when the first is full, and each result equals
+    the request run alone, bit for bit."""
+    lib, err = _lib()
+    dims = {"polar": (57, 45), "pitch": (41, 21)}
+    reqs = [_polar(0, 0.0, alfa=1.0), _polar(1, 0.0, jmax=49, kmax=41, alfa=-2.0), _pitch(2, 0.0, n=3),
+            _polar(3, 0.02, m=0.03, alfa=3.0, kmax=37), _polar(4, 0.04, alfa=5.0, jmax=57),
+            _polar(5, 0.06, alfa=0.5, kmax=45), _pitch(6, 0.08, n=3)]
+    srv = serve.Server(lib, steps_per_tick=K, tol=TOL, max_steps=400, keep_state=True, errfn=err, max_cohort=3,
+                       slot_dims=lambda r: dims[r.kind])
+    res = srv.run(reqs)
+    print("cohorts", srv.cohorts_made, "slot admissions", srv.slot_admissions,
+          [(r.rid, r.status, r.steps) for r in res])
+    # polar 0, 1 and pitch 2 start two cohorts; 3 fills the polar cohort's
+    # third slot, 6 the pitch cohort's second; 4 and 5 find the polar cohort
+    # full unless a case has finished
+    assert srv.slot_admissions >= 2
+    assert srv.cohorts_made <= 3
+    for r in res:
+        srv1 = serve.Server(lib, steps_per_tick=K, tol=TOL, max_steps=400, keep_state=True, errfn=err)
+        a = srv1.run([serve.Request(r.rid, 0.0, reqs[r.rid].deck, reqs[r.rid].grid, reqs[r.rid].kind)])[0]
+        assert a.status == r.status and a.steps == r.steps, r.rid
+        assert a.status in (abi.CONVERGED, -100)
+        np.testing.assert_array_equal(a.state, r.state, err_msg=f"request {r.rid}")
+        np.testing.assert_array_equal(a.forces, r.forces, err_msg=f"request {r.rid}")

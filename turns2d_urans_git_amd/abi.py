@@ -132,6 +132,9 @@ def bind(lib: C.CDLL, prefix: str):
         "batch_status": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "batch_retire": (C.c_int32, [C.c_void_p, C.c_int32]),
         "batch_admit": (C.c_int32, [C.c_void_p, C.c_int32, P(Params), P(MeshDesc)]),
+        "set_launch": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+        "get_launch": (C.c_int32, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
+        "autotune": (C.c_int32, [C.c_void_p, C.c_int32, P(C.c_int32)]),
         "connect": (C.c_int32, [C.c_void_p, C.c_void_p, P(C.c_int32)]),
         "interpolate": (C.c_int32, [C.c_void_p]),
         "set_connectivity": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
@@ -393,6 +396,25 @@ class CaseAPI:
         self._fns["sizes"](C.byref(params), C.byref(mesh), C.byref(jd), C.byref(kd), C.byref(nq), C.byref(nv))
         self.dims[icase] = (jd.value, kd.value, nq.value, nv.value, mesh.jmax, mesh.kmax)
         return rc
+
+    def set_launch(self, sweep: int = 0, sa_lane: int = -1):
+        """Launch choices that change speed, not results (ot2d_set_launch)."""
+        return self._check(self._fns["set_launch"](self.h, int(sweep), int(sa_lane)), "set_launch")
+
+    def get_launch(self):
+        """(sweep, sa_lane) in use (ot2d_get_launch)."""
+        sw, sl = C.c_int32(), C.c_int32()
+        self._check(self._fns["get_launch"](self.h, C.byref(sw), C.byref(sl)), "get_launch")
+        return sw.value, sl.value
+
+    def autotune(self, steps: int = 10):
+        """Measured search over the launch choices for this case's shape
+        class, cached per class (ot2d_autotune).  Returns (sweep, sa_lane,
+        measured); measured is False on a cache hit."""
+        m = C.c_int32()
+        self._check(self._fns["autotune"](self.h, int(steps), C.byref(m)), "autotune")
+        sw, sl = self.get_launch()
+        return sw, sl, bool(m.value)
 
     def forces(self, im: int = 0, nsp: int = 0):
         out = np.zeros(6)

@@ -69,6 +69,7 @@ struct ot2d_case {
   int device = 0;
   int lu_rl = 1;             // rows per lane of the LU-SGS sweeps
   int weno = 0;              // some case reconstructs with WENO5 (wider residual tiles)
+  LaunchCfg cfg;             // launch choices that change no result (ot2d_set_launch, ot2d_autotune)
   int ncase = 1;             // independent cases (batch: one per mesh; otherwise 1)
   int batch = 0;             // made by ot2d_batch_create: per-case states persist across calls
   std::vector<int> mesh_case;  // case of each mesh
@@ -907,7 +908,7 @@ extern "C" int32_t ot2d_lhs(ot2d_case* c) {
     Fp32Order order(c);
     HIPCHK(order.error());
     if (int32_t rc = clear_status(c)) return rc;
-    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, nullptr, c->stream));
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, nullptr, c->stream, c->cfg.sweep));
   }
   return sync(c);
 }
@@ -932,12 +933,12 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
   HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));
   mark(mk, OT2D_ST_RBC, c->stream);
   if (c->lamin) {
-    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, mk, c->stream));
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
   } else {
     // vmu_sa reads the state ilu2d leaves (before the q update), src/step.f:189-211
-    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, mk, c->stream));
+    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
     if (c->iturb == 2) HIPCHK(launch_sst(c->dM, c->hM.data(), nmi(c), mk, c->stream));
-    else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), mk, c->stream));
+    else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), mk, c->stream, c->cfg.sa_lane));
     // the q update and monitor in one pass (stage "update")
     HIPCHK(launch_update_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
     mark(mk, OT2D_ST_UPDATE, c->stream);
@@ -955,7 +956,7 @@ extern "C" int32_t ot2d_turbulence(ot2d_case* c) {
   if (c->lamin) return OT2D_OK;
   if (int32_t rc = clear_status(c)) return rc;
   if (c->iturb == 2) HIPCHK(launch_sst(c->dM, c->hM.data(), nmi(c), nullptr, c->stream));
-  else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), nullptr, c->stream));
+  else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), nullptr, c->stream, c->cfg.sa_lane));
   return sync(c);
 }
 
@@ -1386,6 +1387,201 @@ extern "C" int32_t ot2d_batch_admit(ot2d_case* c, int32_t icase, const ot2d_para
   HIPCHK(hipMemcpyAsync(c->d_status + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
   c->graph_valid = 0;   // the tiles and launch sizes may have changed
   return sync(c);
+}
+
+// ================================================================ autotuner
+extern "C" int32_t ot2d_set_launch(ot2d_case* c, int32_t sweep, int32_t sa_lane) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (sweep < 0 || sweep >= LU32_SWEEP_VARIANTS || sa_lane < -1 || sa_lane > 1)
+    return fail(OT2D_ERR_ARG, "launch choice sweep=%d sa_lane=%d", sweep, sa_lane);
+  c->cfg.sweep = sweep;
+  c->cfg.sa_lane = sa_lane;
+  c->graph_valid = 0;
+  return OT2D_OK;
+}
+
+extern "C" int32_t ot2d_get_launch(ot2d_case* c, int32_t* sweep, int32_t* sa_lane) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (sweep) *sweep = c->cfg.sweep;
+  if (sa_lane) *sa_lane = c->cfg.sa_lane;
+  return OT2D_OK;
+}
+
+namespace {
+// a case's shape class: what the launch choices' speed depends on (model,
+// precision, instances, and the number of cases, cells, columns and sweep
+// blocks in powers of two)
+std::string shape_class(const ot2d_case* c) {
+  int maxj = 0, maxk = 0;
+  long long cells = 0;
+  for (const auto& mb : c->m) {
+    maxj = mb.jmax > maxj ? mb.jmax : maxj;
+    maxk = mb.kmax > maxk ? mb.kmax : maxk;
+    cells += (long long)(mb.jd - 2) * (mb.kd - 2);
+  }
+  cells *= c->nspec;
+  auto lg = [](long long v) { int b = 0; while (v > 1) { v >>= 1; ++b; } return b; };
+  char buf[192];
+  snprintf(buf, sizeof buf, "iturb=%d,lamin=%d,fp32=%d,nspec=%d,cases=2^%d,cells=2^%d,jmax=2^%d,blocks=%d", c->iturb,
+           c->lamin, c->p.lu_fp32 ? 1 : 0, c->nspec, lg(c->ncase), lg(cells), lg(maxj), (maxk - 1 + 31) / 32);
+  return buf;
+}
+
+struct TuneCache {
+  std::mutex mu;
+  bool loaded = false;
+  std::vector<std::pair<std::string, LaunchCfg>> entries;
+};
+TuneCache g_tune;
+
+// the file named by OT2D_TUNE_CACHE: one line per class, "class sweep sa_lane"
+void tune_cache_load() {
+  if (g_tune.loaded) return;
+  g_tune.loaded = true;
+  const char* path = getenv("OT2D_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = fopen(path, "r");
+  if (!f) return;
+  char key[256];
+  int sw, sl;
+  while (fscanf(f, "%255s %d %d", key, &sw, &sl) == 3)
+    if (sw >= 0 && sw < LU32_SWEEP_VARIANTS && sl >= -1 && sl <= 1) {
+      LaunchCfg cfg;
+      cfg.sweep = sw;
+      cfg.sa_lane = sl;
+      g_tune.entries.emplace_back(key, cfg);
+    }
+  fclose(f);
+}
+
+void tune_cache_store(const std::string& key, const LaunchCfg& cfg) {
+  g_tune.entries.emplace_back(key, cfg);
+  const char* path = getenv("OT2D_TUNE_CACHE");
+  if (!path || !*path) return;
+  if (FILE* f = fopen(path, "a")) {
+    fprintf(f, "%s %d %d\n", key.c_str(), cfg.sweep, cfg.sa_lane);
+    fclose(f);
+  }
+}
+}  // namespace
+
+extern "C" int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured) {
+  if (!c || steps < 1) return fail(OT2D_ERR_ARG, "bad arguments");
+  for (auto& mb : c->m)
+    if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  const std::string key = shape_class(c);
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    tune_cache_load();
+    for (auto it = g_tune.entries.rbegin(); it != g_tune.entries.rend(); ++it)
+      if (it->first == key) {
+        c->cfg = it->second;
+        c->graph_valid = 0;
+        if (measured) *measured = 0;
+        return OT2D_OK;
+      }
+  }
+  std::vector<LaunchCfg> cand;
+  const int nsw = c->p.lu_fp32 ? LU32_SWEEP_VARIANTS : 1;
+  const bool sa = !c->lamin && c->iturb == 1;
+  for (int sw = 0; sw < nsw; ++sw)
+    for (int sl = sa ? 0 : -1; sl <= (sa ? 1 : -1); ++sl) {
+      LaunchCfg cfg;
+      cfg.sweep = sw;
+      cfg.sa_lane = sl;
+      cand.push_back(cfg);
+    }
+  if (int32_t rc = sync(c)) return rc;
+  if (c->hM[0].hist != nullptr) {   // the timed steps record no history
+    for (auto& M : c->hM) { M.hist = nullptr; M.hist_cap = 0; }
+    HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  // what the steps change, saved so each candidate starts from the same
+  // state and the case ends where it began: q, the eddy viscosity, the
+  // per-case counters and status
+  std::vector<double*> save;
+  auto cleanup = [&]() { for (double* p : save) dfree(p); };
+  const int nc = c->ncase;
+  int* save_i = nullptr;
+  if (hipMalloc(&save_i, 3 * sizeof(int) * nc) != hipSuccess) return fail(OT2D_ERR_NOMEM, "autotune buffers");
+  auto copy_state = [&](bool back) -> hipError_t {
+    hipError_t e = hipSuccess;
+    for (size_t im = 0; im < c->m.size() && e == hipSuccess; ++im) {
+      MeshBuf& mb = c->m[im];
+      const size_t nq = mb.n2 * c->nspec * c->nq, nt = mb.n2 * c->nspec;
+      double* sq = save[2 * im];
+      double* st = save[2 * im + 1];
+      e = hipMemcpyAsync(back ? mb.q : sq, back ? sq : mb.q, nq * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(back ? mb.turmu : st, back ? st : mb.turmu, nt * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream);
+    }
+    int* src[3] = {c->d_status, c->d_istep0, c->d_stepc};
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+      e = hipMemcpyAsync(back ? src[k] : save_i + k * nc, back ? save_i + k * nc : src[k], sizeof(int) * nc,
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+  };
+  for (auto& mb : c->m) {
+    double *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, mb.n2 * c->nspec * c->nq * sizeof(double)) != hipSuccess ||
+        hipMalloc(&b, mb.n2 * c->nspec * sizeof(double)) != hipSuccess) {
+      dfree(a); dfree(save_i); cleanup();
+      return fail(OT2D_ERR_NOMEM, "autotune buffers");
+    }
+    save.push_back(a);
+    save.push_back(b);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = copy_state(false);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const LaunchCfg keep = c->cfg;
+  // three rounds over the candidates, each candidate's best round counts
+  // (the rounds interleave, so a slow spell of the device is shared out)
+  constexpr int ROUNDS = 3;
+  std::vector<float> cand_ms(cand.size(), 3.0e38f);
+  int32_t rc = OT2D_OK;
+  for (int round = 0; round < ROUNDS && e == hipSuccess && rc == OT2D_OK; ++round)
+    for (size_t k = 0; k < cand.size() && e == hipSuccess && rc == OT2D_OK; ++k) {
+      c->cfg = cand[k];
+      c->graph_valid = 0;
+      {
+        Fp32Order order(c);
+        if ((e = order.error()) != hipSuccess) break;
+        if ((rc = clear_status(c)) != OT2D_OK) break;
+        if ((rc = ensure_graph(c)) != OT2D_OK) break;
+        for (int it = 0; it < 2 && e == hipSuccess; ++it) e = hipGraphLaunch(c->graph, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+        for (int it = 0; it < steps && e == hipSuccess; ++it) e = hipGraphLaunch(c->graph, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      }
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess) e = copy_state(true);
+      if (e == hipSuccess && ms < cand_ms[k]) cand_ms[k] = ms;
+    }
+  int best = -1;
+  for (size_t k = 0; k < cand.size(); ++k)
+    if (cand_ms[k] < 1.0e38f && (best < 0 || cand_ms[k] < cand_ms[best])) best = (int)k;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(save_i);
+  cleanup();
+  c->graph_valid = 0;
+  if (rc != OT2D_OK || e != hipSuccess || best < 0) {
+    c->cfg = keep;
+    return rc != OT2D_OK ? rc : fail(OT2D_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+  }
+  c->cfg = cand[best];
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    tune_cache_store(key, c->cfg);
+  }
+  if (measured) *measured = 1;
+  return OT2D_OK;
 }
 
 // ============================================================ overset meshes

@@ -288,15 +288,21 @@ hipError_t launch_conv_check(const DevMesh* dM, const int* inst, const double* t
                              hipStream_t st);
 hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_rbc(const DevMesh* dM, int nmi, hipStream_t st);
+// launch choices that change no result (the autotuner's knobs)
+constexpr int LU32_SWEEP_VARIANTS = 3;   // FP32 sweep instantiations (ring steps, producers, prefetch)
+struct LaunchCfg {
+  int sweep = 0;     // 0: 12 ring steps, 3 producers; 1: 18, 3; 2: 12, 4 (with deeper prefetch)
+  int sa_lane = -1;  // SA line solves: -1 by the number of lines, 0 one wave per line, 1 one lane per line
+};
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
-                     hipStream_t st);
+                     hipStream_t st, int sweep = 0);
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd, int rl);
 hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int norms = 0);
-hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane = -1);
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sst(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);

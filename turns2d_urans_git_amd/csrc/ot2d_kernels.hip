@@ -2572,12 +2572,34 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
 // LDS ring slots of the FP32 sweeps (LU32_NV KiB each and two tag words): 14
 // slots, so two workgroups fit one CU's LDS; producer waves and the steps
 // each keeps in flight
-constexpr int LU32_S = 12;
-constexpr int LU32_NP = 3, LU32_PD = 5;
-constexpr size_t LU32_LDS = (size_t)LU32_S * LU32_NV * 1024 + 16 * ((LU32_S * 8 + 15) / 16);
+// the FP32 sweep's instantiations (ring steps S, producer waves NP,
+// deliveries in flight per producer PD): the same arithmetic in the same
+// order, so the choice (LaunchCfg::sweep, the autotuner's) changes no result
+constexpr size_t lu32_lds(int S) { return (size_t)S * LU32_NV * 1024 + 16 * ((S * 8 + 15) / 16); }
+
+template <int S, int NP, int PD>
+static hipError_t launch_sweeps32(const DevMesh* dM, int wmax, int nmi, Marks* mk, hipStream_t st) {
+  constexpr size_t lds = lu32_lds(S);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, S, NP, PD>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, S, NP, PD>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_lu_sweep32<1, S, NP, PD>), dim3(wmax, nmi), dim3(64 * (2 + NP)), lds, st, dM);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_FORWARD, st);
+  hipLaunchKernelGGL((k_lu_sweep32<-1, S, NP, PD>), dim3(wmax, nmi), dim3(64 * (2 + NP)), lds, st, dM);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark(mk, OT2D_ST_LU_BACKWARD, st);
+  return hipSuccess;
+}
 
 static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, Marks* mk,
-                              hipStream_t st) {
+                              hipStream_t st, int sweep) {
   long long mx = 0, nc = 0;
   int wmax = 1;
   for (int i = 0; i < nmi; ++i) {
@@ -2588,13 +2610,8 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
     nc = nprep > nc ? nprep : nc;
     wmax = gg.W > wmax ? gg.W : wmax;
   }
-  const size_t lds = LU32_LDS;
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<1, LU32_S, LU32_NP, LU32_PD>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)k_lu_sweep32<-1, LU32_S, LU32_NP, LU32_PD>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void*)k_lu_prep32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LP_LDS);
     attr_set = true;
   }
@@ -2602,14 +2619,13 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_PREP, st);
-  hipLaunchKernelGGL((k_lu_sweep32<1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (2 + LU32_NP)), lds, st,
-                     dM);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  mark(mk, OT2D_ST_LU_FORWARD, st);
-  hipLaunchKernelGGL((k_lu_sweep32<-1, LU32_S, LU32_NP, LU32_PD>), dim3(wmax, nmi), dim3(64 * (2 + LU32_NP)), lds, st,
-                     dM);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  mark(mk, OT2D_ST_LU_BACKWARD, st);
+  switch (sweep) {
+    case 0: e = launch_sweeps32<12, 3, 5>(dM, wmax, nmi, mk, st); break;
+    case 1: e = launch_sweeps32<18, 3, 5>(dM, wmax, nmi, mk, st); break;
+    case 2: e = launch_sweeps32<12, 4, 6>(dM, wmax, nmi, mk, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lu_finish<float>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, fuse_update);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_FINISH, st);
@@ -2618,10 +2634,10 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
 
 // rows per lane rl = ceil(rows / 64) of the case (all instances share it)
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
-                     hipStream_t st) {
+                     hipStream_t st, int sweep) {
   const int rl = hM[0].lu_rl;
   if (fp32) {
-    return launch_lu32(dM, hM, nmi, fuse_update, mk, st);
+    return launch_lu32(dM, hM, nmi, fuse_update, mk, st, sweep);
   } else {
     // prefetch depth D chosen so the record ring fits the register file
     switch (rl) {

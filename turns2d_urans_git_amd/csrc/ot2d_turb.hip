@@ -1294,7 +1294,7 @@ hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
 }
 
 // vmu_sa for every instance (src/samodel2d.f:2-197, itrans = 0)
-hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st) {
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane) {
   long long mx = max_cells(hM, nmi);
   int mj = 1, mk_ = 1;
   for (int i = 0; i < nmi; ++i) {
@@ -1315,7 +1315,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   // short in latency; many lines (batches): one lane per line
   long long lines = 0;
   for (int i = 0; i < nmi; ++i) lines += (hM[i].ke - hM[i].ks + 1) + (hM[i].je - hM[i].js + 1);
-  const bool per_lane = lines > SA_LANE_LINES;
+  const bool per_lane = sa_lane < 0 ? lines > SA_LANE_LINES : sa_lane == 1;
   if (per_lane) hipLaunchKernelGGL(k_sa_lines_j, dim3((mk_ + 63) / 64, nmi), dim3(SAJ_THREADS), 0, st, dM);
   else hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;

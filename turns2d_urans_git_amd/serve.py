@@ -98,29 +98,62 @@ def device_memory():
 
 
 # -------------------------------------------------------------------- server
+def _params(lib, deck) -> abi.Params:
+    p = abi.Params()
+    lib.ot2d_default_params(C.byref(p))
+    p.update(**deck)
+    return p
+
+
+def _mesh(g) -> abi.MeshDesc:
+    return abi.mesh_desc(g.jmax, g.kmax, grid.cgrid_bc(g.jtail1))
+
+
 class _Cohort:
-    def __init__(self, lib, reqs: List[Request], tol: float, errfn):
-        self.reqs = reqs
-        params = []
-        for r in reqs:
-            p = abi.Params()
-            lib.ot2d_default_params(C.byref(p))
-            p.update(**r.deck)
-            params.append(p)
-        mds = [abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1)) for r in reqs]
-        self.case = abi.CaseAPI(lib, "ot2d_", params, mds, errfn=errfn, batch=True)
-        for i, r in enumerate(reqs):
-            self.case.set_grid(i, r.grid.xv, r.grid.yv)
-        self.case.set_convergence(tol)
-        self.open = set(range(len(reqs)))
-        self.free = set()                      # slots whose case has finished
-        # the mesh each slot was allocated for: a request fits a slot when
-        # its grid is no larger in either direction
-        self.cap = [(r.grid.jmax, r.grid.kmax) for r in reqs]
-        self.key = reqs[0].key()
-        self.bytes = self.case.bytes()
+    """One batch of a structure (model, precision, instance count) on its own
+    stream.  Without `slot_grid` its slots are the requests that made it,
+    each sized to its request.  With `slot_grid` (the structure's largest
+    grid) it has `slots` slots of that size: they start stopped (the grid
+    set, then retired) and the requests that made it are admitted into the
+    first ones, so later arrivals of any size of the structure find free
+    slots in a resident cohort instead of starting a small one."""
+
+    def __init__(self, lib, reqs: List[Request], tol: float, errfn, tune_steps: int = 0,
+                 slots: int = 0, slot_grid=None):
         self.lib = lib
         self.tol = tol
+        self.key = reqs[0].key()
+        if slot_grid is None:
+            self.reqs = list(reqs)
+            self.case = abi.CaseAPI(lib, "ot2d_", [_params(lib, r.deck) for r in reqs], [_mesh(r.grid) for r in reqs],
+                                    errfn=errfn, batch=True)
+            for i, r in enumerate(reqs):
+                self.case.set_grid(i, r.grid.xv, r.grid.yv)
+            self.case.set_convergence(tol)
+            self.open = set(range(len(reqs)))
+            self.free = set()                      # slots whose case has finished
+            # the mesh each slot was allocated for: a request fits a slot when
+            # its grid is no larger in either direction
+            self.cap = [(r.grid.jmax, r.grid.kmax) for r in reqs]
+        else:
+            n = max(int(slots), len(reqs))
+            self.reqs = [None] * n
+            self.case = abi.CaseAPI(lib, "ot2d_", [_params(lib, reqs[0].deck)] * n, [_mesh(slot_grid)] * n,
+                                    errfn=errfn, batch=True)
+            for i in range(n):
+                self.case.set_grid(i, slot_grid.xv, slot_grid.yv)
+                self.case.batch_retire(i)
+            self.case.set_convergence(tol)
+            self.open = set()
+            self.free = set(range(n))
+            self.cap = [(slot_grid.jmax, slot_grid.kmax)] * n
+            for i, r in enumerate(reqs):
+                self.admit(i, r)
+        if tune_steps > 0:
+            # launch choices of the cohort's shape class (cached per class:
+            # only the first cohort of a class measures)
+            self.case.autotune(tune_steps)
+        self.bytes = self.case.bytes()
 
     def fits(self, i: int, r: Request) -> bool:
         return r.key() == self.key and r.grid.jmax <= self.cap[i][0] and r.grid.kmax <= self.cap[i][1]
@@ -128,11 +161,7 @@ class _Cohort:
     def admit(self, i: int, r: Request):
         """Put request r into the free slot i (ot2d_batch_admit, then its grid
         and tolerance)."""
-        p = abi.Params()
-        self.lib.ot2d_default_params(C.byref(p))
-        p.update(**r.deck)
-        md = abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1))
-        self.case.batch_admit(i, p, md)
+        self.case.batch_admit(i, _params(self.lib, r.deck), _mesh(r.grid))
         self.case.set_grid(i, r.grid.xv, r.grid.yv)
         self.case.set_convergence(self.tol, i)
         self.reqs[i] = r
@@ -148,10 +177,20 @@ class Server:
 
     ``budget``: device bytes the resident cohorts may hold (default: 80 % of
     the free memory at construction).  ``max_steps``: a case still running
-    after that many iterations is retired as not converged."""
+    after that many iterations is retired as not converged.  ``max_resident``:
+    at most that many cases resident at once (None: the memory budget alone
+    limits them); past it, arrivals wait in arrival order, so an overloaded
+    stream queues instead of slowing every resident case.  ``tune_steps``
+    > 0: each new cohort takes its launch choices from ``ot2d_autotune``
+    (that many steps per candidate, measured once per shape class).
+    ``slot_dims``: a function giving a request's structure's largest grid
+    (jmax, kmax); with it every new cohort has ``max_cohort`` slots of that
+    size (see ``_Cohort``), and a cohort left without running cases stays
+    resident while requests are still to arrive."""
 
     def __init__(self, lib, budget: Optional[int] = None, steps_per_tick: int = 50, tol: float = 1e-10,
-                 max_cohort: int = 32, max_steps: int = 100000, keep_state: bool = False, errfn=None):
+                 max_cohort: int = 32, max_steps: int = 100000, keep_state: bool = False, errfn=None,
+                 tune_steps: int = 0, max_resident: Optional[int] = None, slot_dims=None):
         self.lib = lib
         abi.bind(lib, "ot2d_")
         lib.ot2d_default_params.argtypes = [C.POINTER(abi.Params)]
@@ -162,6 +201,10 @@ class Server:
         self.max_steps = int(max_steps)
         self.keep_state = keep_state
         self.errfn = errfn
+        self.tune_steps = int(tune_steps)
+        self.max_resident = max_resident
+        self.slot_dims = slot_dims
+        self._slot_grids: Dict[tuple, grid.CGrid] = {}
         self.peak_bytes = 0
         self.ticks = 0
         self.slot_admissions = 0   # requests that took a finished case's slot
@@ -177,8 +220,26 @@ class Server:
         mds = [abi.mesh_desc(r.grid.jmax, r.grid.kmax, grid.cgrid_bc(r.grid.jtail1)) for r in reqs]
         return abi.footprint(self.lib, "ot2d_", params, mds)
 
+    def _slot_grid(self, r: Request):
+        dims = tuple(self.slot_dims(r))
+        if dims not in self._slot_grids:
+            self._slot_grids[dims] = grid.naca4_cgrid(dims[0], dims[1], ds_wall=1e-6, wake_aspect=1000.0,
+                                                      te_cluster=False)
+        return self._slot_grids[dims]
+
+    def cohort_footprint(self, r: Request) -> int:
+        """Device bytes of a cohort of ``max_cohort`` slots of r's structure's
+        largest grid."""
+        g = self._slot_grid(r)
+        n = self.max_cohort
+        return abi.footprint(self.lib, "ot2d_", [_params(self.lib, r.deck)] * n, [_mesh(g)] * n)
+
     def run(self, requests: Sequence[Request], time_limit: Optional[float] = None,
-            clock=time.perf_counter) -> List[Result]:
+            clock=time.perf_counter, progress=None) -> List[Result]:
+        """Serve `requests` (each admitted no earlier than its arrival time)
+        until all have finished or `time_limit` seconds have passed.
+        `progress(dict)`, if given, is called after every tick with the
+        counts of waiting, resident and finished requests."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         waiting: deque = deque()
         cohorts: List[_Cohort] = []
@@ -211,33 +272,46 @@ class Server:
                 results[r.rid].admitted = clock() - t0
                 self.slot_admissions += 1
             waiting = still
-            # cohorts with no case left running give their memory back
-            for c in [c for c in cohorts if not c.open]:
+            # cohorts with no case left running give their memory back (with
+            # pre-sized slots, once no request is still to arrive)
+            for c in [c for c in cohorts if not c.open and (self.slot_dims is None or not pending)]:
                 held -= c.bytes
                 c.close()
                 cohorts.remove(c)
             # ---- new cohorts: arrival order, while the footprint fits
             admitted: Dict[tuple, List[Request]] = {}
+            resident = sum(len(c.open) for c in cohorts)
             while waiting:
+                if self.max_resident is not None and resident >= self.max_resident:
+                    break
                 r = waiting[0]
                 group = admitted.get(r.key(), [])
                 if len(group) >= self.max_cohort:
                     break
-                need = self.footprint(group + [r]) - (self.footprint(group) if group else 0)
+                if self.slot_dims is None:
+                    need = self.footprint(group + [r]) - (self.footprint(group) if group else 0)
+                else:
+                    need = 0 if group else self.cohort_footprint(r)
                 if held + need > self.budget:
                     break
                 held += need
+                resident += 1
                 admitted.setdefault(r.key(), []).append(waiting.popleft())
             for group in admitted.values():
-                c = _Cohort(self.lib, group, self.tol, self.errfn)
+                if self.slot_dims is None:
+                    c = _Cohort(self.lib, group, self.tol, self.errfn, self.tune_steps)
+                    held += c.bytes - self.footprint(group)   # the case's own count
+                else:
+                    c = _Cohort(self.lib, group, self.tol, self.errfn, self.tune_steps, slots=self.max_cohort,
+                                slot_grid=self._slot_grid(group[0]))
+                    held += c.bytes - self.cohort_footprint(group[0])
                 self.cohorts_made += 1
-                held += c.bytes - self.footprint(group)   # the case's own count
                 tnow = clock() - t0
                 for r in group:
                     results[r.rid].admitted = tnow
                 cohorts.append(c)
             self.peak_bytes = max(self.peak_bytes, held)
-            if not cohorts:
+            if not any(c.open for c in cohorts):
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                 continue
@@ -270,6 +344,10 @@ class Server:
                             res.state = c.case.get_state(i)
                         c.open.discard(i)
                         c.free.add(i)
+            if progress is not None:
+                progress(dict(t=clock() - t0, ticks=self.ticks, pending=len(pending), waiting=len(waiting),
+                              resident=sum(len(c.open) for c in cohorts), cohorts=len(cohorts),
+                              finished=sum(1 for x in results.values() if x.finished == x.finished)))
             if not pending and not waiting:
                 # nothing left to take the free slots
                 for c in [c for c in cohorts if not c.open]:
@@ -315,6 +393,14 @@ def pitch_request(rid, arrival, rng, scale=1.0, lu_fp32=True) -> Request:
     return Request(rid, arrival, deck, g, "pitch")
 
 
+def stream_slot_dims(scale=1.0):
+    """The largest grid of each structure in ``poisson_stream``: polar points
+    (jmax up to 769, kmax up to 193) and pitching cases (385x97), scaled."""
+    polar = (int(769 * scale) // 2 * 2 + 1, int(193 * scale))
+    pitch = (int(385 * scale) // 2 * 2 + 1, int(97 * scale))
+    return lambda r: pitch if r.kind == "pitch" else polar
+
+
 def poisson_stream(lam, duration, seed=20261202, pitch_frac=0.3, scale=1.0, lu_fp32=True) -> List[Request]:
     """Poisson arrivals at ``lam`` requests/s for ``duration`` s."""
     rng = np.random.default_rng(seed)
@@ -329,11 +415,29 @@ def poisson_stream(lam, duration, seed=20261202, pitch_frac=0.3, scale=1.0, lu_f
 
 
 def summarize(results: Sequence[Result], wall: float) -> dict:
+    """Latency (arrival to convergence), its queueing (arrival to admission)
+    and service (admission to convergence) parts over the converged
+    requests, p50/p99; throughput over the wall time; and how many requests
+    were still waiting or running when the loop ended."""
     ok = [r for r in results if r.status == abi.CONVERGED]
-    lat = np.array([r.latency for r in ok]) if ok else np.array([math.nan])
+
+    def pct(v, q):
+        return float(np.percentile(np.array(v), q)) if v else math.nan
+
+    lat = [r.latency for r in ok]
+    wait = [r.admitted - r.arrival for r in ok]
+    svc = [r.finished - r.admitted for r in ok]
     cell_iters = float(sum(r.cells * r.steps for r in results))
     return dict(requests=len(results), converged=len(ok), wall_s=wall,
-                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                p50_s=pct(lat, 50), p99_s=pct(lat, 99),
+                queue_p50_s=pct(wait, 50), queue_p99_s=pct(wait, 99),
+                service_p50_s=pct(svc, 50), service_p99_s=pct(svc, 99),
+                steps_p50=pct([r.steps for r in ok], 50),
+                converged_by_kind={k: sum(1 for r in ok if r.kind == k) for k in ("polar", "pitch")},
+                never_admitted=sum(1 for r in results if r.admitted != r.admitted),
+                failures=[dict(rid=r.rid, kind=r.kind, status=r.status, steps=r.steps) for r in results
+                          if r.finished == r.finished and r.status not in (abi.CONVERGED, -100)],
+                unfinished=sum(1 for r in results if r.admitted == r.admitted and r.finished != r.finished),
                 solves_per_hr=len(ok) / wall * 3600.0 if wall > 0 else 0.0,
                 cell_iter_per_s=cell_iters / wall if wall > 0 else 0.0)
 
@@ -349,14 +453,16 @@ def main(argv=None):
     ap.add_argument("--steps-per-tick", type=int, default=100)
     ap.add_argument("--max-steps", type=int, default=40000)
     ap.add_argument("--time-limit", type=float, default=None)
+    ap.add_argument("--tune-steps", type=int, default=5, help="autotune steps per candidate (0: defaults)")
     a = ap.parse_args(argv)
     reqs = poisson_stream(a.rate, a.duration, scale=a.scale)
     srv = Server(t2d.load_library(), steps_per_tick=a.steps_per_tick, tol=a.tol, max_steps=a.max_steps,
-                 errfn=t2d.last_error)
+                 errfn=t2d.last_error, tune_steps=a.tune_steps)
     t0 = time.perf_counter()
     res = srv.run(reqs, time_limit=a.time_limit)
     out = summarize(res, time.perf_counter() - t0)
-    out.update(rate=a.rate, duration=a.duration, scale=a.scale, tol=a.tol, peak_bytes=srv.peak_bytes)
+    out.update(rate=a.rate, duration=a.duration, scale=a.scale, tol=a.tol, peak_bytes=srv.peak_bytes,
+               cohorts_made=srv.cohorts_made, slot_admissions=srv.slot_admissions)
     print(json.dumps(out))
 
 
