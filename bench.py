@@ -405,6 +405,8 @@ def run_batch(args, hip):
     many steps, and the plain path's time for the same solves is estimated
     from its per-case step time and the batch's steps to convergence."""
     import turns2d_urans_git_amd as t2d
+    if args.fp32_overlap:
+        abi.set_fp32_order(t2d.load_library(), "ot2d_", False)
     cases = batch_cases(args.batch_size)
     params, meshes, grids = [], [], []
     for c in cases:
@@ -629,6 +631,8 @@ def run_serve(args):
     import turns2d_urans_git_amd as t2d
     from turns2d_urans_git_amd import serve
     reqs = serve.poisson_stream(args.rate, args.duration, scale=args.serve_scale)
+    if args.fp32_overlap:
+        abi.set_fp32_order(t2d.load_library(), "ot2d_", False)
     srv = serve.Server(t2d.load_library(), steps_per_tick=args.serve_tick, tol=1e-10,
                        max_steps=args.serve_max_steps, errfn=t2d.last_error, tune_steps=args.autotune,
                        max_resident=args.serve_max_resident, max_cohort=args.serve_cohort if args.serve_cohort > 0 else 32,
@@ -658,7 +662,7 @@ def run_serve(args):
                    "grid_scale": args.serve_scale, "steps_per_tick": args.serve_tick,
                    "max_steps": args.serve_max_steps, "autotune_steps": args.autotune,
                    "max_resident": args.serve_max_resident, "time_limit_s": args.serve_time_limit,
-                   "cohort_slots": args.serve_cohort},
+                   "cohort_slots": args.serve_cohort, "fp32_overlap": bool(args.fp32_overlap)},
         "serve": dict(s, peak_bytes=srv.peak_bytes, budget_bytes=srv.budget, cohorts_made=srv.cohorts_made,
                       slot_admissions=srv.slot_admissions,
                       not_converged=sum(1 for r in res if r.status == -100),
@@ -692,6 +696,8 @@ def main():
     ap.add_argument("--serve-cohort", type=int, default=32,
                     help="serve: slots per cohort, each sized for its structure's largest grid (0: one slot per "
                          "request, sized to it)")
+    ap.add_argument("--fp32-overlap", type=int, default=1,
+                    help="serve, batch64_sa: let FP32 cases on different streams overlap (ot2d_set_fp32_order(0))")
     ap.add_argument("--serve-max-resident", type=int, default=None,
                     help="serve: cases resident at once (default: the memory budget alone)")
     ap.add_argument("--batch-size", type=int, default=64)

@@ -172,6 +172,15 @@ int32_t ot2d_batch_create(int32_t ncase, const ot2d_params* params, const ot2d_m
 int32_t ot2d_batch_status(ot2d_case* c, int32_t* status, int32_t* istep0);
 /* Stop advancing case icase (status OT2D_CONVERGED). */
 int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase);
+/* Launch the batch's following steps over its running cases only: waits
+ * for the case's stream, reads every case's status, and keeps the running
+ * cases' instance records and residual tiles, so stopped cases (converged,
+ * retired, failed) cost no workgroups.  Calls that may make a stopped case
+ * run again (ot2d_set_grid, ot2d_set_state, ot2d_batch_admit) return the
+ * batch to launching over every case.  *running (may be NULL): cases
+ * running.  Batches without overset connectivity.  No counterpart in the
+ * reference; for request loops whose batches hold finished cases. */
+int32_t ot2d_batch_compact(ot2d_case* c, int32_t* running);
 /* Put a new case into slot icase of a batch, whose case has stopped
  * (converged, retired or failed): parameters p (same model, precision and
  * instance count as the batch) and mesh md, which must fit the memory the
@@ -230,6 +239,15 @@ int32_t ot2d_set_convergence(ot2d_case* c, int32_t icase, double rsum_tol);
  * waits for the FP32 work enqueued before it by any case). */
 int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter);
 int32_t ot2d_sync(ot2d_case* c);
+/* *idle = 1 when everything enqueued for the case has finished, else 0;
+ * never waits.  For request loops that advance each batch as soon as its
+ * previous steps are done; no counterpart in the reference. */
+int32_t ot2d_query(ot2d_case* c, int32_t* idle);
+/* on = 1 (the default): the order of FP32 work across cases above; on = 0:
+ * lifted for the whole process, so FP32 cases on different streams overlap
+ * like FP64 ones.  Applies to work enqueued after the call.  No counterpart
+ * in the reference. */
+int32_t ot2d_set_fp32_order(int32_t on);
 
 /* ot2d_iterate without allocating: the same steps, the same outputs (hist,
  * *done, status), but the device-side residual history is staged in a

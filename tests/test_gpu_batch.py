@@ -146,3 +146,54 @@ def test_nonfinite_residual_stops_with_norm():
     assert st[1] == abi.NORM, st
     assert st[0] == abi.OK and st[2] == abi.OK, st
     assert ist[0] == 3 and ist[2] == 3
+
+
+@pytest.mark.parametrize("lu_fp32", [0, 1])
+def test_compacted_batch_runs_only_running_cases(lu_fp32):
+    """ot2d_batch_compact: after a case is retired, the batch's steps launch
+    over the running cases' records and tiles only.  The running cases'
+    histories and states equal those of an uncompacted twin bit for bit, the
+    retired case stays frozen, and setting its grid again returns the batch
+    to every case (the case runs again, as in the twin)."""
+    import turns2d_urans_git_amd as t2d
+    grids, params, meshes = _inputs(lu_fp32)
+
+    def make():
+        b = t2d.Batch(params, meshes)
+        for i, g in enumerate(grids):
+            b.set_grid(i, g.xv, g.yv)
+        b.iterate(8)
+        b.batch_retire(0)
+        return b
+
+    comp, twin = make(), make()
+    q0 = comp.get_state(0).copy()
+    assert comp.batch_compact() == 2
+    assert comp.batch_compact() == 2          # unchanged: nothing to redo
+    r1, h1, d1 = comp.iterate(12)
+    r2, h2, d2 = twin.iterate(12)
+    assert r1 == r2 == 0 and d1 == d2 == 12
+    np.testing.assert_array_equal(h1, h2)
+    assert np.all(h1[:, 0] == 0.0)
+    np.testing.assert_array_equal(comp.get_state(0), q0)
+    for i in (1, 2):
+        np.testing.assert_array_equal(comp.get_state(i), twin.get_state(i), err_msg=f"case {i}")
+    assert list(comp.batch_status()[1]) == list(twin.batch_status()[1]) == [8, 20, 20]
+    # async steps on the compacted records as well
+    comp.iterate_async(5)
+    twin.iterate_async(5)
+    comp.sync()
+    twin.sync()
+    for i in (1, 2):
+        np.testing.assert_array_equal(comp.get_state(i), twin.get_state(i), err_msg=f"case {i}, async")
+    # the retired case runs again after its grid is set: every case launches
+    for b in (comp, twin):
+        b.set_grid(0, grids[0].xv, grids[0].yv)
+        rc, _, _ = b.iterate(6)
+        assert rc == 0
+    for i in range(3):
+        np.testing.assert_array_equal(comp.get_state(i), twin.get_state(i), err_msg=f"case {i}, restarted")
+    # every case running: compaction keeps the full records
+    assert comp.batch_compact() == 3
+    comp.close()
+    twin.close()

@@ -175,13 +175,16 @@ def test_full_mix_batch_equals_alone():
     b.close()
 
 
-def test_overlapped_fp32_batches_equal_one_stream():
+@pytest.mark.parametrize("ordered", [True, False])
+def test_overlapped_fp32_batches_equal_one_stream(ordered):
     """Three FP32 batches of the mix (W = 3 to 6 cases) enqueued on three
     streams at once, against the same three batches each run with nothing
-    beside it, bit for bit, checked every `chunk` steps.  (The library
-    orders FP32 cases' work across streams; the check is that the calls
-    still give the one-stream results.)"""
+    beside it, bit for bit, checked every `chunk` steps: with the library's
+    order of FP32 work across streams (the default) and with it lifted
+    (ot2d_set_fp32_order(0)), where the three batches' sweeps run at the
+    same time."""
     import turns2d_urans_git_amd as t2d
+    lib = t2d.load_library()
     idx, grids, decks, params, meshes = _mix()
     cells = np.array([(g.jmax - 1) * (g.kmax - 1) for g in grids], dtype=np.float64)
     groups = bench.batch_groups(cells, 3)
@@ -195,23 +198,28 @@ def test_overlapped_fp32_batches_equal_one_stream():
     over = [make(g) for g in groups]
     alone = [make(g) for g in groups]
     chunk, nchunk = 250, 8
-    for c in range(nchunk):
-        for b in over:
-            b.iterate_async(chunk)
-        for b in alone:          # one at a time: nothing runs beside it
-            b.iterate_async(chunk)
-            b.sync()
-        for b in over:
-            b.sync()
-        for gi, grp in enumerate(groups):
-            st_o, is_o = over[gi].batch_status()
-            st_a, is_a = alone[gi].batch_status()
-            np.testing.assert_array_equal(st_o, st_a, err_msg=f"status, group {gi}, chunk {c}")
-            np.testing.assert_array_equal(is_o, is_a, err_msg=f"steps, group {gi}, chunk {c}")
-            for n, i in enumerate(grp):
-                qo, qa = over[gi].get_state(n), alone[gi].get_state(n)
-                assert np.array_equal(qo, qa, equal_nan=True), \
-                    f"case {i} (group {gi}) differs after {(c + 1) * chunk} steps"
-        print(f"[lu32 overlap] {(c + 1) * chunk} steps: all {len(idx)} cases equal", flush=True)
+    abi.set_fp32_order(lib, "ot2d_", ordered)
+    try:
+        for c in range(nchunk):
+            for b in over:
+                b.iterate_async(chunk)
+            for b in over:
+                b.sync()
+            for b in alone:          # one at a time: nothing runs beside it
+                b.iterate_async(chunk)
+                b.sync()
+            for gi, grp in enumerate(groups):
+                st_o, is_o = over[gi].batch_status()
+                st_a, is_a = alone[gi].batch_status()
+                np.testing.assert_array_equal(st_o, st_a, err_msg=f"status, group {gi}, chunk {c}")
+                np.testing.assert_array_equal(is_o, is_a, err_msg=f"steps, group {gi}, chunk {c}")
+                for n, i in enumerate(grp):
+                    qo, qa = over[gi].get_state(n), alone[gi].get_state(n)
+                    assert np.array_equal(qo, qa, equal_nan=True), \
+                        f"case {i} (group {gi}) differs after {(c + 1) * chunk} steps (ordered={ordered})"
+            print(f"[lu32 overlap ordered={ordered}] {(c + 1) * chunk} steps: all {len(idx)} cases equal",
+                  flush=True)
+    finally:
+        abi.set_fp32_order(lib, "ot2d_", True)
     for b in over + alone:
         b.close()

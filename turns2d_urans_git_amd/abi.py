@@ -132,6 +132,7 @@ def bind(lib: C.CDLL, prefix: str):
         "batch_status": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "batch_retire": (C.c_int32, [C.c_void_p, C.c_int32]),
         "batch_admit": (C.c_int32, [C.c_void_p, C.c_int32, P(Params), P(MeshDesc)]),
+        "batch_compact": (C.c_int32, [C.c_void_p, P(C.c_int32)]),
         "set_launch": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
         "get_launch": (C.c_int32, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "autotune": (C.c_int32, [C.c_void_p, C.c_int32, P(C.c_int32)]),
@@ -149,6 +150,8 @@ def bind(lib: C.CDLL, prefix: str):
         "set_convergence": (C.c_int32, [C.c_void_p, C.c_int32, C.c_double]),
         "iterate_async": (C.c_int32, [C.c_void_p, C.c_int32]),
         "sync": (C.c_int32, [C.c_void_p]),
+        "set_fp32_order": (C.c_int32, [C.c_int32]),
+        "query": (C.c_int32, [C.c_void_p, P(C.c_int32)]),
         # oracle only: connectivity of one mesh as plain arrays
         "set_mesh_connectivity": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _IP, C.c_int32, _IP, _IP,
                                               C.c_int32, _IP, _DP]),
@@ -257,6 +260,13 @@ class CaseAPI:
 
     def iterate_async(self, niter: int):
         self._check(self._fns["iterate_async"](self.h, int(niter)), "iterate_async")
+
+    def idle(self) -> bool:
+        """True when all work enqueued for the case has finished (ot2d_query;
+        never waits)."""
+        v = C.c_int32()
+        self._check(self._fns["query"](self.h, C.byref(v)), "query")
+        return bool(v.value)
 
     def sync(self):
         self._check(self._fns["sync"](self.h), "sync")
@@ -397,6 +407,13 @@ class CaseAPI:
         self.dims[icase] = (jd.value, kd.value, nq.value, nv.value, mesh.jmax, mesh.kmax)
         return rc
 
+    def batch_compact(self) -> int:
+        """Launch the following steps over the running cases only
+        (ot2d_batch_compact); returns the number of running cases."""
+        n = C.c_int32()
+        self._check(self._fns["batch_compact"](self.h, C.byref(n)), "batch_compact")
+        return n.value
+
     def set_launch(self, sweep: int = 0, sa_lane: int = -1):
         """Launch choices that change speed, not results (ot2d_set_launch)."""
         return self._check(self._fns["set_launch"](self.h, int(sweep), int(sa_lane)), "set_launch")
@@ -495,6 +512,15 @@ def ihc_connect(lib, prefix, meshes, errfn=None) -> IHCLists:
     if rc:
         raise CaseError(rc, "ihc_connect", errfn() if errfn else "")
     return IHCLists(lib, prefix, h, dims, errfn)
+
+
+def set_fp32_order(lib, prefix, on: bool):
+    """Order FP32 LU-SGS work of different cases across streams (on, the
+    default) or let it overlap (off), for the whole process
+    (ot2d_set_fp32_order)."""
+    rc = bind(lib, prefix)["set_fp32_order"](1 if on else 0)
+    if rc:
+        raise CaseError(rc, "set_fp32_order")
 
 
 def footprint(lib, prefix, params_list, meshes) -> int:

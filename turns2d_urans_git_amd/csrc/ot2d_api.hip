@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -98,6 +99,15 @@ struct ot2d_case {
   // captured step
   hipGraphExec_t graph = nullptr;
   int graph_valid = 0;
+  // batches: the running cases' instance records and residual tiles
+  // (ot2d_batch_compact); while act_on the step launches over these only
+  int act_on = 0;
+  std::vector<int> act;        // instance indices, ascending
+  std::vector<DevMesh> hMa;
+  std::vector<ResTile> hTa;
+  DevMesh* dMa = nullptr;
+  ResTile* dTa = nullptr;
+  size_t cap_ta = 0;           // tiles dTa holds
   // overset interpolation lists per time instance (host) and all instances
   // concatenated on the device
   struct HDonor { int mi, jj, kk, pad; double f1, f2; };
@@ -125,11 +135,13 @@ struct ot2d_ihc_lists {
 // one-stream results now and then; the cause is not shown (it is not the
 // sweeps' inter-workgroup mail, whose entries carry the sweep number), so
 // the library keeps such work apart.  FP64 cases still overlap freely.
+// ot2d_set_fp32_order(0) lifts the order for the process.
 namespace {
 constexpr int kMaxDevices = 64;
 struct Fp32Chain {
   std::mutex mu;
   hipEvent_t ev[kMaxDevices] = {};
+  std::atomic<int> on{1};
 };
 Fp32Chain g_fp32;
 
@@ -147,7 +159,7 @@ class Fp32Order {
 }  // namespace
 
 Fp32Order::Fp32Order(const ot2d_case* c) {
-  if (!c->p.lu_fp32) return;
+  if (!c->p.lu_fp32 || !g_fp32.on.load(std::memory_order_relaxed)) return;
   c_ = c;
   g_fp32.mu.lock();
   e_ = hipStreamWaitEvent(c->stream, g_fp32.ev[c->device], 0);
@@ -157,6 +169,14 @@ Fp32Order::~Fp32Order() {
   if (!c_) return;
   (void)hipEventRecord(g_fp32.ev[c_->device], c_->stream);
   g_fp32.mu.unlock();
+}
+
+extern "C" int32_t ot2d_set_fp32_order(int32_t on) {
+  if (on != 0 && on != 1) return fail(OT2D_ERR_ARG, "ot2d_set_fp32_order: on must be 0 or 1");
+  // taken so that no enqueue is half way through its ordered section
+  std::lock_guard<std::mutex> lock(g_fp32.mu);
+  g_fp32.on.store(on, std::memory_order_relaxed);
+  return OT2D_OK;
 }
 
 // the device's ordering event, created with the first FP32 case on it
@@ -336,6 +356,7 @@ static void free_case(ot2d_case* c) {
     dfree(mb.tscale); dfree(mb.bt); dfree(mb.xv); dfree(mb.yv); dfree(mb.ugv);
     dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa);
   }
+  dfree(c->dMa); dfree(c->dTa);
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
   dfree(c->d_cinst); dfree(c->d_ctol);
   dfree(c->d_partials); dfree(c->d_norms); dfree(c->d_tspec);
@@ -732,6 +753,8 @@ static int32_t sync(ot2d_case* c) {
   return OT2D_OK;
 }
 
+static void full_view(ot2d_case* c);
+
 extern "C" int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, const double* yv) {
   if (int32_t rc = check_im(c, im)) return rc;
   if (!xv || !yv) return fail(OT2D_ERR_ARG, "null grid");
@@ -748,6 +771,7 @@ extern "C" int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, con
   if (ic == 0) c->istep0 = 0;
   HIPCHK(hipMemsetAsync(c->d_istep0 + ic, 0, sizeof(int), c->stream));
   HIPCHK(hipMemsetAsync(c->d_status + ic, 0, sizeof(int), c->stream));
+  full_view(c);   // the case runs again
   return sync(c);
 }
 
@@ -756,6 +780,7 @@ extern "C" int32_t ot2d_set_state(ot2d_case* c, int32_t im, const double* q) {
   MeshBuf& mb = c->m[im];
   HIPCHK(hipMemcpyAsync(mb.q, q, sizeof(double) * mb.n2 * c->nspec * c->nq, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->d_status + c->mesh_case[im], 0, sizeof(int), c->stream));
+  full_view(c);   // the case runs again
   return sync(c);
 }
 
@@ -861,27 +886,61 @@ extern "C" int32_t ot2d_bc(ot2d_case* c) {
   return sync(c);
 }
 
+// what the step launches over: every instance record and residual tile, or,
+// for a compacted batch, the running cases' ones
+struct StepView {
+  DevMesh* dM;
+  const DevMesh* hM;
+  int n;
+  const ResTile* dT;
+  int nt;
+};
+static StepView step_view(ot2d_case* c) {
+  if (c->act_on) return {c->dMa, c->hMa.data(), (int)c->hMa.size(), c->dTa, (int)c->hTa.size()};
+  return {c->dM, c->hM.data(), nmi(c), c->dT, (int)c->hT.size()};
+}
+
+// the instance records to the device (and the running cases' copies)
+static int32_t upload_meshes(ot2d_case* c) {
+  HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice, c->stream));
+  if (c->act_on) {
+    for (size_t k = 0; k < c->act.size(); ++k) c->hMa[k] = c->hM[c->act[k]];
+    HIPCHK(hipMemcpyAsync(c->dMa, c->hMa.data(), sizeof(DevMesh) * c->hMa.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  return OT2D_OK;
+}
+
+// back to launching over every instance (a stopped case may run again)
+static void full_view(ot2d_case* c) {
+  if (!c->act_on) return;
+  c->act_on = 0;
+  c->act.clear();
+  c->graph_valid = 0;
+}
+
 static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks* mk = nullptr) {
-  HIPCHK(launch_bc(c->dM, nmi(c), zero_halo_s, c->stream));
+  const StepView v = step_view(c);
+  HIPCHK(launch_bc(v.dM, v.n, zero_halo_s, c->stream));
   mark(mk, OT2D_ST_BC, c->stream);
   // the norms are taken on the complete residual (src/step.f:46-93)
   const bool later = !c->p.invisc || c->p.timespectral;
-  HIPCHK(launch_residual(c->dM, c->dT, (int)c->hT.size(), norms && !later, c->weno, c->stream));
+  HIPCHK(launch_residual(v.dM, v.dT, v.nt, norms && !later, c->weno, c->stream));
   mark(mk, OT2D_ST_RESIDUAL, c->stream);
   // without time-spectral terms the residual is complete after visrhs,
   // which then takes the norm partials itself
   const bool vis_norms = norms && !c->p.invisc && !c->p.timespectral;
   if (!c->p.invisc) {
-    HIPCHK(launch_visrhs(c->dM, c->hM.data(), nmi(c), c->stream, vis_norms ? 1 : 0));
+    HIPCHK(launch_visrhs(v.dM, v.hM, v.n, c->stream, vis_norms ? 1 : 0));
     mark(mk, OT2D_ST_VISRHS, c->stream);
   }
   if (c->p.timespectral) {
-    HIPCHK(launch_tsrhs(c->dM, c->hM.data(), nmi(c), c->stream));
+    HIPCHK(launch_tsrhs(v.dM, v.hM, v.n, c->stream));
     mark(mk, OT2D_ST_TSRHS, c->stream);
   }
-  if (norms && later && !vis_norms) HIPCHK(launch_norms(c->dM, c->dT, (int)c->hT.size(), c->stream));
+  if (norms && later && !vis_norms) HIPCHK(launch_norms(v.dM, v.dT, v.nt, c->stream));
   if (norms) {
-    HIPCHK(launch_norms_final(c->dM, nmi(c), c->stream));
+    HIPCHK(launch_norms_final(v.dM, v.n, c->stream));
     if (c->conv_on) HIPCHK(launch_conv_check(c->dM, c->d_cinst, c->d_ctol, c->d_status, c->ncase, c->stream));
     mark(mk, OT2D_ST_NORMS, c->stream);
   }
@@ -926,21 +985,22 @@ extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
 
 // one step (src/step.f:2-218) + monitor, enqueued without synchronisation
 static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
+  const StepView v = step_view(c);
   // the step opens (istep0, step counters) inside the time-step kernel
-  HIPCHK(launch_time_step(c->dM, c->hM.data(), nmi(c), c->stream, c->d_istep0, c->d_status, c->d_stepc, c->ncase));
+  HIPCHK(launch_time_step(v.dM, v.hM, v.n, c->stream, c->d_istep0, c->d_status, c->d_stepc, c->ncase));
   mark(mk, OT2D_ST_TIME_STEP, c->stream);
   if (int32_t rc = enqueue_residual(c, 1, 1, mk)) return rc;
-  HIPCHK(launch_rbc(c->dM, nmi(c), c->stream));
+  HIPCHK(launch_rbc(v.dM, v.n, c->stream));
   mark(mk, OT2D_ST_RBC, c->stream);
   if (c->lamin) {
-    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
+    HIPCHK(launch_lu(v.dM, v.hM, v.n, 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
   } else {
     // vmu_sa reads the state ilu2d leaves (before the q update), src/step.f:189-211
-    HIPCHK(launch_lu(c->dM, c->hM.data(), nmi(c), 0, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
-    if (c->iturb == 2) HIPCHK(launch_sst(c->dM, c->hM.data(), nmi(c), mk, c->stream));
-    else HIPCHK(launch_sa(c->dM, c->hM.data(), nmi(c), mk, c->stream, c->cfg.sa_lane));
+    HIPCHK(launch_lu(v.dM, v.hM, v.n, 0, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
+    if (c->iturb == 2) HIPCHK(launch_sst(v.dM, v.hM, v.n, mk, c->stream));
+    else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane));
     // the q update and monitor in one pass (stage "update")
-    HIPCHK(launch_update_monitor(c->dM, c->hM.data(), nmi(c), c->stream));
+    HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream));
     mark(mk, OT2D_ST_UPDATE, c->stream);
   }
   // overset meshes exchange fringe values (src/Turns_overset.f:500-503)
@@ -1061,7 +1121,7 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
       c->hM[i].hist_stride = per;
       c->hM[i].hist_slot = i;
     }
-    HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * per, hipMemcpyHostToDevice, c->stream));
+    if (int32_t rc = upload_meshes(c)) return rc;
   }
   {
     Fp32Order order(c);
@@ -1127,7 +1187,7 @@ static int32_t set_history(ot2d_case* c, double* h, int niter) {
     c->hM[i].hist_stride = per;
     c->hM[i].hist_slot = i;
   }
-  HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * per, hipMemcpyHostToDevice, c->stream));
+  if (int32_t rc = upload_meshes(c)) return rc;
   return OT2D_OK;
 }
 
@@ -1195,7 +1255,7 @@ extern "C" int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter) {
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
   if (c->hM[0].hist != nullptr) {
     for (auto& M : c->hM) { M.hist = nullptr; M.hist_cap = 0; }
-    HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice, c->stream));
+    if (int32_t rc = upload_meshes(c)) return rc;
   }
   Fp32Order order(c);
   HIPCHK(order.error());
@@ -1208,6 +1268,16 @@ extern "C" int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter) {
 extern "C" int32_t ot2d_sync(ot2d_case* c) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
   return sync(c);
+}
+
+extern "C" int32_t ot2d_query(ot2d_case* c, int32_t* idle) {
+  if (!c || !idle) return fail(OT2D_ERR_ARG, "null argument");
+  const hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess || e == hipErrorNotReady) {
+    *idle = e == hipSuccess;
+    return OT2D_OK;
+  }
+  return fail(OT2D_ERR_HIP, "stream query: %s", hipGetErrorString(e));
 }
 
 extern "C" int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]) {
@@ -1312,6 +1382,60 @@ extern "C" int32_t ot2d_batch_retire(ot2d_case* c, int32_t icase) {
   return sync(c);
 }
 
+extern "C" int32_t ot2d_batch_compact(ot2d_case* c, int32_t* running) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (!c->batch) return fail(OT2D_ERR_ARG, "not a batch");
+  if (c->ndon > 0 || c->nfr > 0) return fail(OT2D_ERR_UNSUPPORTED, "a batch with overset connectivity");
+  std::vector<int> st(c->ncase);
+  HIPCHK(hipMemcpyAsync(st.data(), c->d_status, sizeof(int) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  std::vector<int> act;
+  int nrun = 0;
+  for (int ic = 0; ic < c->ncase; ++ic) nrun += st[ic] == 0;
+  for (int im = 0; im < c->nmesh; ++im)
+    if (st[c->mesh_case[im]] == 0)
+      for (int n = 0; n < c->nspec; ++n) act.push_back(im * c->nspec + n);
+  if (running) *running = nrun;
+  // every case running, or none (then nothing is launched for them anyway
+  // past each kernel's status check): the full records
+  if (nrun == 0 || (int)act.size() == nmi(c)) {
+    full_view(c);
+    return OT2D_OK;
+  }
+  if (c->act_on && act == c->act) return OT2D_OK;
+  if (!c->dMa) {
+    size_t cap = 0;
+    for (const auto& mb : c->m) cap += (size_t)mb.cap_tiles * c->nspec;
+    hipError_t e = dalloc(&c->dMa, c->hM.size());
+    if (e == hipSuccess) e = dalloc(&c->dTa, cap);
+    if (e != hipSuccess) {
+      dfree(c->dMa);
+      c->dMa = nullptr;
+      return fail(OT2D_ERR_NOMEM, "running-case records: %s", hipGetErrorString(e));
+    }
+    c->cap_ta = cap;
+    c->bytes += sizeof(DevMesh) * c->hM.size() + sizeof(ResTile) * cap;
+  }
+  // position of every running instance in the compacted records
+  std::vector<int> pos(c->hM.size(), -1);
+  for (size_t k = 0; k < act.size(); ++k) pos[act[k]] = (int)k;
+  c->hTa.clear();
+  for (const ResTile& T : c->hT)
+    if (pos[T.mesh] >= 0) {
+      ResTile t = T;
+      t.mesh = pos[T.mesh];
+      c->hTa.push_back(t);
+    }
+  if (c->hTa.size() > c->cap_ta) return fail(OT2D_ERR_STATE, "running-case tiles exceed their capacity");
+  c->act = act;
+  c->hMa.resize(act.size());
+  c->act_on = 1;
+  c->graph_valid = 0;
+  if (int32_t rc = upload_meshes(c)) return rc;
+  HIPCHK(hipMemcpyAsync(c->dTa, c->hTa.data(), sizeof(ResTile) * c->hTa.size(), hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
 extern "C" int32_t ot2d_batch_admit(ot2d_case* c, int32_t icase, const ot2d_params* p,
                                     const ot2d_mesh_desc* md) {
   if (!c || !p || !md) return fail(OT2D_ERR_ARG, "null argument");
@@ -1369,6 +1493,7 @@ extern "C" int32_t ot2d_batch_admit(ot2d_case* c, int32_t icase, const ot2d_para
   mb.d = *md;
   mb.jmax = jmax; mb.kmax = kmax; mb.jd = jd; mb.kd = kd; mb.n2 = n2; mb.nv2 = nv2; mb.lu_per = lu_per;
   mb.has_grid = false;
+  full_view(c);   // the new case runs: every instance is launched again
   if (int32_t rc = mesh_records(c, icase, p)) return rc;
   build_tiles(c);
   // the case's time-spectral instants (src/init.f:71-78); the host buffers
@@ -1494,7 +1619,7 @@ extern "C" int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured)
   if (int32_t rc = sync(c)) return rc;
   if (c->hM[0].hist != nullptr) {   // the timed steps record no history
     for (auto& M : c->hM) { M.hist = nullptr; M.hist_cap = 0; }
-    HIPCHK(hipMemcpyAsync(c->dM, c->hM.data(), sizeof(DevMesh) * c->hM.size(), hipMemcpyHostToDevice, c->stream));
+    if (int32_t rc = upload_meshes(c)) return rc;
   }
   // what the steps change, saved so each candidate starts from the same
   // state and the case ends where it began: q, the eddy viscosity, the

@@ -11,13 +11,15 @@ time.  Between iterations the loop
   batches (``ot2d_batch_create``), one *cohort* per structure (model,
   precision, instance count) and admission moment, each with its own HIP
   stream;
-* advances every resident cohort by ``steps_per_tick`` iterations with
-  ``ot2d_iterate_async``, so the cohorts' kernels overlap on the device
-  (the library orders the FP32 LU-SGS work of different cohorts among
-  themselves; FP64 work overlaps with everything);
-* reads every cohort's per-case states (``ot2d_batch_status``): a case whose
-  residual has reached the tolerance stopped on the device (status
-  ``CONVERGED``, set by the step itself before its update), and its
+* advances each resident cohort by ``steps_per_tick`` iterations with
+  ``ot2d_iterate_async`` as soon as its previous ones are done
+  (``ot2d_query`` never waits), so cohorts advance independently and their
+  kernels overlap on the device (the library orders the FP32 LU-SGS work of
+  different cohorts among themselves unless ``ot2d_set_fp32_order(0)``
+  lifts that; FP64 work overlaps with everything);
+* reads a cohort's per-case states (``ot2d_batch_status``) once its steps
+  are done: a case whose residual has reached the tolerance stopped on the
+  device (status ``CONVERGED``, set by the step itself before its update), and its
   time-to-converge is recorded and its slot becomes free; a cohort with no
   case left running, whose free slots no waiting request took, is destroyed
   and its memory returns to the budget.
@@ -149,6 +151,8 @@ class _Cohort:
             self.cap = [(slot_grid.jmax, slot_grid.kmax)] * n
             for i, r in enumerate(reqs):
                 self.admit(i, r)
+        self.busy = False                          # steps enqueued and not yet done
+        self.dirty = True                          # cases stopped or admitted since the last compaction
         if tune_steps > 0:
             # launch choices of the cohort's shape class (cached per class:
             # only the first cohort of a class measures)
@@ -167,6 +171,7 @@ class _Cohort:
         self.reqs[i] = r
         self.free.discard(i)
         self.open.add(i)
+        self.dirty = True
 
     def close(self):
         self.case.close()
@@ -252,13 +257,23 @@ class Server:
                 break
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            # ---- admission into the free slots of resident cohorts, arrival
-            # order, each request into the smallest slot it fits
+            # ---- cohorts whose enqueued steps are done: read their cases'
+            # states (no wait: ot2d_query said the stream is idle)
+            collected = False
+            for c in cohorts:
+                if c.busy and c.case.idle():
+                    c.busy = False
+                    self._collect(c, results, clock() - t0)
+                    collected = True
+            # ---- admission into the free slots of idle resident cohorts,
+            # arrival order, each request into the smallest slot it fits
             still: deque = deque()
             while waiting:
                 r = waiting.popleft()
                 best = None
                 for c in cohorts:
+                    if c.busy:
+                        continue
                     for i in c.free:
                         if c.fits(i, r):
                             area = c.cap[i][0] * c.cap[i][1]
@@ -274,29 +289,41 @@ class Server:
             waiting = still
             # cohorts with no case left running give their memory back (with
             # pre-sized slots, once no request is still to arrive)
-            for c in [c for c in cohorts if not c.open and (self.slot_dims is None or not pending)]:
+            for c in [c for c in cohorts if not c.busy and not c.open and (self.slot_dims is None or not pending)]:
                 held -= c.bytes
                 c.close()
                 cohorts.remove(c)
-            # ---- new cohorts: arrival order, while the footprint fits
+            # ---- new cohorts: arrival order, while the footprint fits.  A
+            # request that fits a free slot of a cohort still running its
+            # steps waits for that slot instead (one request per slot)
+            spare = [(c, i) for c in cohorts if c.busy for i in c.free]
             admitted: Dict[tuple, List[Request]] = {}
             resident = sum(len(c.open) for c in cohorts)
-            while waiting:
-                if self.max_resident is not None and resident >= self.max_resident:
-                    break
-                r = waiting[0]
+            remaining: deque = deque()
+            blocked = False
+            for r in waiting:
+                if blocked:
+                    remaining.append(r)
+                    continue
+                slot = next((k for k, (c, i) in enumerate(spare) if c.fits(i, r)), None)
+                if slot is not None:
+                    spare.pop(slot)
+                    remaining.append(r)
+                    continue
                 group = admitted.get(r.key(), [])
-                if len(group) >= self.max_cohort:
-                    break
                 if self.slot_dims is None:
                     need = self.footprint(group + [r]) - (self.footprint(group) if group else 0)
                 else:
                     need = 0 if group else self.cohort_footprint(r)
-                if held + need > self.budget:
-                    break
+                if ((self.max_resident is not None and resident >= self.max_resident)
+                        or len(group) >= self.max_cohort or held + need > self.budget):
+                    blocked = True
+                    remaining.append(r)
+                    continue
                 held += need
                 resident += 1
-                admitted.setdefault(r.key(), []).append(waiting.popleft())
+                admitted.setdefault(r.key(), []).append(r)
+            waiting = remaining
             for group in admitted.values():
                 if self.slot_dims is None:
                     c = _Cohort(self.lib, group, self.tol, self.errfn, self.tune_steps)
@@ -311,52 +338,63 @@ class Server:
                     results[r.rid].admitted = tnow
                 cohorts.append(c)
             self.peak_bytes = max(self.peak_bytes, held)
-            if not any(c.open for c in cohorts):
-                if pending:
-                    time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
-                continue
-            # ---- one tick: every cohort with a running case advances, their
-            # kernels overlapping on the device (each cohort has its own
-            # stream)
+            # ---- every idle cohort with a running case takes its next
+            # steps_per_tick steps at once: cohorts advance independently,
+            # their kernels overlapping on the device (each has its own stream)
+            launched = False
             for c in cohorts:
-                if c.open:
+                if not c.busy and c.open:
+                    if c.dirty:
+                        # launch over the running cases only (finished and
+                        # empty slots cost no workgroups)
+                        c.case.batch_compact()
+                        c.dirty = False
                     c.case.iterate_async(self.k)
-            self.ticks += 1
-            for c in cohorts:
-                if not c.open:
-                    continue
-                st, ist = c.case.batch_status()
-                tnow = clock() - t0
-                for i in list(c.open):
-                    res = results[c.reqs[i].rid]
-                    res.steps = int(ist[i])
-                    if st[i] != abi.OK or ist[i] >= self.max_steps:
-                        if st[i] == abi.OK:
-                            c.case.batch_retire(i)
-                            res.status = -100          # not converged within max_steps
-                        else:
-                            res.status = int(st[i])
-                        res.finished = tnow
-                        n = c.reqs[i].nspec
-                        res.forces = np.array([c.case.forces(i, s)[k] for s in range(n)
-                                               for k in ("cl", "cd", "cm", "cpower", "cfx", "cfy")]).reshape(n, 6)
-                        if self.keep_state:
-                            res.state = c.case.get_state(i)
-                        c.open.discard(i)
-                        c.free.add(i)
-            if progress is not None:
+                    c.busy = True
+                    launched = True
+                    self.ticks += 1
+            if (launched or collected) and progress is not None:
                 progress(dict(t=clock() - t0, ticks=self.ticks, pending=len(pending), waiting=len(waiting),
                               resident=sum(len(c.open) for c in cohorts), cohorts=len(cohorts),
                               finished=sum(1 for x in results.values() if x.finished == x.finished)))
             if not pending and not waiting:
                 # nothing left to take the free slots
-                for c in [c for c in cohorts if not c.open]:
+                for c in [c for c in cohorts if not c.busy and not c.open]:
                     held -= c.bytes
                     c.close()
                     cohorts.remove(c)
+            if not launched and not collected:
+                if any(c.busy for c in cohorts):
+                    time.sleep(0.001)
+                elif pending:
+                    time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
         for c in cohorts:
             c.close()
         return [results[r.rid] for r in requests]
+
+    def _collect(self, c: "_Cohort", results: Dict[int, Result], tnow: float):
+        """Per-case states of a cohort whose steps are done: a case that
+        stopped (converged, failed) or reached max_steps finishes, and its
+        slot becomes free."""
+        st, ist = c.case.batch_status()
+        for i in list(c.open):
+            res = results[c.reqs[i].rid]
+            res.steps = int(ist[i])
+            if st[i] != abi.OK or ist[i] >= self.max_steps:
+                if st[i] == abi.OK:
+                    c.case.batch_retire(i)
+                    res.status = -100          # not converged within max_steps
+                else:
+                    res.status = int(st[i])
+                res.finished = tnow
+                n = c.reqs[i].nspec
+                res.forces = np.array([c.case.forces(i, s)[k] for s in range(n)
+                                       for k in ("cl", "cd", "cm", "cpower", "cfx", "cfy")]).reshape(n, 6)
+                if self.keep_state:
+                    res.state = c.case.get_state(i)
+                c.open.discard(i)
+                c.free.add(i)
+                c.dirty = True
 
 
 # ---------------------------------------------------------------- streams
