@@ -268,8 +268,12 @@ class Server:
             # ---- admission into the free slots of idle resident cohorts,
             # arrival order, each request into the smallest slot it fits
             still: deque = deque()
+            resident = sum(len(c.open) for c in cohorts)
             while waiting:
                 r = waiting.popleft()
+                if self.max_resident is not None and resident >= self.max_resident:
+                    still.append(r)
+                    continue
                 best = None
                 for c in cohorts:
                     if c.busy:
@@ -284,6 +288,7 @@ class Server:
                     continue
                 _, c, i = best
                 c.admit(i, r)
+                resident += 1
                 results[r.rid].admitted = clock() - t0
                 self.slot_admissions += 1
             waiting = still
