@@ -115,6 +115,17 @@ def driver_deck():
     return g, dict(SA, nsteps=c["nsteps"], nrest=c["nrest"], npnorm=c["npnorm"])
 
 
+def _program_outputs(d, names):
+    """The named output files of a run in directory d, as bytes (absent: not
+    stored)."""
+    out = {}
+    for n in names:
+        path = os.path.join(d, n)
+        if os.path.exists(path):
+            out["file:" + n] = np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
+    return out
+
+
 def run_driver_case():
     g, deck = driver_deck()
     with tempfile.TemporaryDirectory() as d:
@@ -124,9 +135,60 @@ def run_driver_case():
         norms = np.loadtxt(os.path.join(d, "fort.71"), ndmin=2)
         forces = np.loadtxt(os.path.join(d, "fort.11"), ndmin=2)
         q8 = open(os.path.join(d, "fort.8"), "rb").read()
+        files = _program_outputs(d, ["fort.9", "cp_TS1.dat", "SA.dat"])
     np.savez_compressed(os.path.join(HERE, "driver_sa_41x21.npz"), norms=norms, forces=forces,
-                        fort8=np.frombuffer(q8, dtype=np.uint8))
+                        fort8=np.frombuffer(q8, dtype=np.uint8), **files)
     print("driver_sa_41x21 norms", norms[-1])
+
+
+# The reference program on a time-spectral deck (N = 3 instances of a
+# pitching SA case) and on an overset deck (SST + WENO5 near-body C-grid in a
+# Cartesian background): every output file the run leaves.
+DRIVER_TS = dict(jmax=41, kmax=21, ds_wall=2e-4, nsteps=12, nrest=5, npnorm=4,
+                 deck=dict(SA, fsmach=0.3, alfa=1.0, rey=4.0e6, dt=5.0, timespectral=True, nspec=3,
+                           motiontype=1, rf=0.2, angmax=3.0))
+DRIVER_OVERSET = dict(near=(97, 25), back=(61, 61), nsteps=8, nrest=5, npnorm=2,
+                      deck=dict(SA, iturb=2, ilim=4, fsmach=0.3, alfa=6.0, rey=3.0e6, dt=5.0))
+
+
+def driver_ts_deck():
+    c = DRIVER_TS
+    g = grid.naca4_cgrid(c["jmax"], c["kmax"], ds_wall=c["ds_wall"])
+    return g, dict(c["deck"], nsteps=c["nsteps"], nrest=c["nrest"], npnorm=c["npnorm"])
+
+
+def driver_overset_deck():
+    c = DRIVER_OVERSET
+    meshes = grid.airfoil_overset(near=c["near"], back=c["back"], ds_wall=1e-4)
+    return meshes, dict(c["deck"], nsteps=c["nsteps"], nrest=c["nrest"], npnorm=c["npnorm"])
+
+
+def run_driver_ts_case():
+    g, deck = driver_ts_deck()
+    n = deck["nspec"]
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_case(d, g, **deck)
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([PROGRAM], cwd=d, stdin=fin, check=True, stdout=subprocess.DEVNULL)
+        names = ["fort.71", "fort.11"] + [f"{s}{i}{e}" for i in range(1, n + 1)
+                                          for s, e in (("fort_TS", ".8"), ("fort_TS", ".9"), ("cp_TS", ".dat"),
+                                                       ("SA_TS", ".dat"))]
+        files = _program_outputs(d, names)
+    np.savez_compressed(os.path.join(HERE, "driver_ts3_sa_41x21.npz"), **files)
+    print("driver_ts3_sa_41x21", sorted(files))
+
+
+def run_driver_overset_case():
+    meshes, deck = driver_overset_deck()
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_overset_case(d, meshes, **deck)
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([PROGRAM], cwd=d, stdin=fin, check=True, stdout=subprocess.DEVNULL,
+                           env=dict(os.environ, **ZERO_MALLOC))
+        files = _program_outputs(d, ["fort.71", "fort.72", "fort.11", "fort.8", "fort.9", "cp_TS1.dat",
+                                     "komega.dat"])
+    np.savez_compressed(os.path.join(HERE, "driver_overset_sst_97x25_61x61.npz"), **files)
+    print("driver_overset_sst_97x25_61x61", sorted(files))
 
 
 # Overset fixtures (num_grids = 2): connectivity by the reference's connect2d
@@ -188,6 +250,10 @@ if __name__ == "__main__":
             run_ts_case(name, *spec)
     if not only or "driver_sa_41x21" in only:
         run_driver_case()
+    if not only or "driver_ts3_sa_41x21" in only:
+        run_driver_ts_case()
+    if not only or "driver_overset_sst_97x25_61x61" in only:
+        run_driver_overset_case()
     for name, spec in OVERSET_CASES.items():
         if not only or name in only:
             run_overset_case(name, *spec)

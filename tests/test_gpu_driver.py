@@ -1,11 +1,20 @@
 """The Fortran driver (turns2d_urans_git_amd/fortran/ot2d_run) on the GPU
 against the reference program itself.
 
-The fixture tests/golden/driver_sa_41x21.npz holds fort.71 (norms), fort.11
-(forces) and fort.8 (restart solution) that the reference program wrote for
-the same deck (tests/golden/make_golden.py).  ot2d_run reads the same case
-files and must write the same numbers: fort.71/fort.11 to their printed
-precision (10 significant digits), fort.8 to 1e-10 relative per value.
+The fixtures hold the output files the reference program wrote for the same
+decks (tests/golden/make_golden.py): a steady SA case, a time-spectral SA
+case (N = 3 pitching instances) and an overset SST + WENO5 case (near-body
+C-grid in a Cartesian background).  ot2d_run reads the same case files and
+must write the same files:
+* norms (fort.7<m>) and forces (fort.11): the integer columns exactly, the
+  values to their printed precision (10 significant digits; FP32 LU-SGS:
+  1e-4, the defect-correction variant);
+* restart solutions and grids (fort.8/fort.9 and fort_TS<n>.8/.9): the same
+  records, integer records (mesh count, dimensions, istep0) and iblank
+  exactly, values within 1e-10 of the record's largest (FP32 LU-SGS: 1e-4);
+* surface pressure (cp_TS<n>.dat) and turbulence files (SA*.dat,
+  komega.dat): the same text where it is text, iblank exactly, values within
+  1e-9 of the column's largest.
 """
 import os
 import subprocess
@@ -22,6 +31,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "turns2d_urans_git_amd", "fortran", "ot2d_run")
 sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+needs_driver = pytest.mark.skipif(not os.path.exists(DRIVER), reason="ot2d_run not built")
 
 
 def fortran_records(raw: bytes):
@@ -34,7 +44,75 @@ def fortran_records(raw: bytes):
     return out
 
 
-@pytest.mark.skipif(not os.path.exists(DRIVER), reason="ot2d_run not built")
+def run_driver(d, lu="fp64"):
+    env = dict(os.environ, OT2D_LU=lu)
+    with open(os.path.join(d, "inputs")) as fin:
+        r = subprocess.run([DRIVER], cwd=d, stdin=fin, env=env, capture_output=True, text=True, timeout=120)
+    assert "completed successfully" in r.stdout + r.stderr, r.stdout + r.stderr
+
+
+def compare_columns(got_path, ref_bytes, rtol, ints):
+    """Printed columns: the first `ints` exactly, the rest to rtol."""
+    got = np.loadtxt(got_path, ndmin=2)
+    ref = np.loadtxt(ref_bytes.tobytes().decode().splitlines(), ndmin=2)
+    assert got.shape == ref.shape, (got_path, got.shape, ref.shape)
+    np.testing.assert_array_equal(got[:, :ints], ref[:, :ints])
+    np.testing.assert_allclose(got[:, ints:], ref[:, ints:], rtol=rtol, atol=1e-13)
+
+
+def compare_unformatted(got_path, ref_bytes, tol, nmesh=1, grid_file=False):
+    """Records of a plot3d-style file: the integer records ([nmesh],
+    dimensions, and istep0 last in a solution file) exactly, value records
+    within tol of their largest.  In a grid file of several meshes each
+    mesh record ends with its iblank (int32), compared exactly."""
+    got = fortran_records(open(got_path, "rb").read())
+    ref = fortran_records(ref_bytes.tobytes())
+    assert len(got) == len(ref), (got_path, len(got), len(ref))
+    head = 2 if nmesh > 1 else 1
+    tail = 0 if grid_file else 1
+    dims = np.frombuffer(ref[head - 1], dtype=np.int32).reshape(-1, 2)
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert len(a) == len(b), (got_path, i)
+        if i < head or i >= len(ref) - tail:
+            assert a == b, (got_path, i)
+            continue
+        if grid_file and nmesh > 1:
+            n = int(dims[i - head][0]) * int(dims[i - head][1])
+            assert a[16 * n:] == b[16 * n:], (got_path, "iblank", i)
+            a, b = a[:16 * n], b[:16 * n]
+        va, vb = np.frombuffer(a, dtype=np.float64), np.frombuffer(b, dtype=np.float64)
+        err = np.abs(va - vb).max() / max(np.abs(vb).max(), 1e-300)
+        assert err <= tol, (got_path, i, err)
+
+
+def compare_text(got_path, ref_bytes, tol):
+    """Token by token: words and integers exactly, reals within tol of the
+    largest real of the file."""
+    got = open(got_path).read().split()
+    ref = ref_bytes.tobytes().decode().split()
+    assert len(got) == len(ref), (got_path, len(got), len(ref))
+    ga, ra = [], []
+    for a, b in zip(got, ref):
+        try:
+            int(b)
+            assert a == b, (got_path, a, b)
+            continue
+        except ValueError:
+            pass
+        try:
+            rb = float(b)
+        except ValueError:
+            assert a == b, (got_path, a, b)
+            continue
+        ga.append(float(a))
+        ra.append(rb)
+    ga, ra = np.array(ga), np.array(ra)
+    if ra.size:
+        err = np.abs(ga - ra).max() / max(np.abs(ra).max(), 1e-300)
+        assert err <= tol, (got_path, err)
+
+
+@needs_driver
 @pytest.mark.parametrize("lu", ["fp64", "fp32"])
 def test_driver_matches_reference_program(lu):
     import make_golden as mg
@@ -42,18 +120,18 @@ def test_driver_matches_reference_program(lu):
     g, deck = mg.driver_deck()
     with tempfile.TemporaryDirectory() as d:
         grid.write_case(d, g, **deck)
-        env = dict(os.environ, OT2D_LU=lu)
-        with open(os.path.join(d, "inputs")) as fin:
-            r = subprocess.run([DRIVER], cwd=d, stdin=fin, env=env, capture_output=True, text=True,
-                               timeout=120)
-        assert "completed successfully" in r.stdout + r.stderr, r.stdout + r.stderr
+        run_driver(d, lu)
         norms = np.loadtxt(os.path.join(d, "fort.71"), ndmin=2)
         forces = np.loadtxt(os.path.join(d, "fort.11"), ndmin=2)
         rec = fortran_records(open(os.path.join(d, "fort.8"), "rb").read())
+        # FP64 LU: the reference's arithmetic, agreement to print precision;
+        # FP32 LU: the defect-correction variant, close to the reference
+        rtol = 1e-9 if lu == "fp64" else 1e-4
+        if lu == "fp64":
+            compare_unformatted(os.path.join(d, "fort.9"), z["file:fort.9"], 1e-12, grid_file=True)
+            compare_text(os.path.join(d, "cp_TS1.dat"), z["file:cp_TS1.dat"], 1e-9)
+            compare_text(os.path.join(d, "SA.dat"), z["file:SA.dat"], 1e-9)
     ref = fortran_records(z["fort8"].tobytes())
-    # FP64 LU: the reference's arithmetic, agreement to print precision;
-    # FP32 LU: the defect-correction variant, close to the reference
-    rtol = 1e-9 if lu == "fp64" else 1e-4
     assert norms.shape == z["norms"].shape
     np.testing.assert_array_equal(norms[:, 0], z["norms"][:, 0])
     np.testing.assert_allclose(norms[:, 1:4], z["norms"][:, 1:4], rtol=rtol)
@@ -67,3 +145,45 @@ def test_driver_matches_reference_program(lu):
         va, vb = np.frombuffer(a, dtype=np.float64), np.frombuffer(b, dtype=np.float64)
         tol = 1e-10 if lu == "fp64" else 1e-4
         assert np.abs(va - vb).max() <= tol * np.abs(vb).max(), np.abs(va - vb).max() / np.abs(vb).max()
+
+
+@needs_driver
+def test_driver_time_spectral_matches_reference_program():
+    """N = 3 pitching instances: per-instance norm lines (and the last
+    instance's angle as theta_col), per-instance force lines, and one
+    restart, grid, surface pressure and SA file per instance."""
+    import make_golden as mg
+    z = np.load(os.path.join(ROOT, "tests", "golden", "driver_ts3_sa_41x21.npz"))
+    g, deck = mg.driver_ts_deck()
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_case(d, g, **deck)
+        run_driver(d)
+        compare_columns(os.path.join(d, "fort.71"), z["file:fort.71"], 1e-9, 1)
+        compare_columns(os.path.join(d, "fort.11"), z["file:fort.11"], 1e-9, 2)
+        for n in range(1, deck["nspec"] + 1):
+            compare_unformatted(os.path.join(d, f"fort_TS{n}.8"), z[f"file:fort_TS{n}.8"], 1e-10)
+            compare_unformatted(os.path.join(d, f"fort_TS{n}.9"), z[f"file:fort_TS{n}.9"], 1e-12, grid_file=True)
+            compare_text(os.path.join(d, f"cp_TS{n}.dat"), z[f"file:cp_TS{n}.dat"], 1e-9)
+            compare_text(os.path.join(d, f"SA_TS{n}.dat"), z[f"file:SA_TS{n}.dat"], 1e-9)
+
+
+@needs_driver
+def test_driver_overset_matches_reference_program():
+    """Two meshes (SST + WENO5 near-body C-grid, Cartesian background): the
+    ihcbc.<n> files read and connect2d run through ot2d_connect, one norm
+    file per mesh, multizone restart and grid files (iblank in the grid),
+    the k-omega restart records the reference writes (twice per mesh) and
+    komega.dat."""
+    import make_golden as mg
+    z = np.load(os.path.join(ROOT, "tests", "golden", "driver_overset_sst_97x25_61x61.npz"))
+    meshes, deck = mg.driver_overset_deck()
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_overset_case(d, meshes, **deck)
+        run_driver(d)
+        compare_columns(os.path.join(d, "fort.71"), z["file:fort.71"], 1e-8, 1)
+        compare_columns(os.path.join(d, "fort.72"), z["file:fort.72"], 1e-8, 1)
+        compare_columns(os.path.join(d, "fort.11"), z["file:fort.11"], 1e-8, 1)
+        compare_unformatted(os.path.join(d, "fort.8"), z["file:fort.8"], 1e-9, nmesh=2)
+        compare_unformatted(os.path.join(d, "fort.9"), z["file:fort.9"], 1e-12, nmesh=2, grid_file=True)
+        compare_text(os.path.join(d, "cp_TS1.dat"), z["file:cp_TS1.dat"], 1e-8)
+        compare_text(os.path.join(d, "komega.dat"), z["file:komega.dat"], 1e-8)

@@ -97,6 +97,13 @@ module ot2d_capi
       real(c_double), intent(out) :: out(*)
     end function ot2d_get_field
 
+    integer(c_int32_t) function ot2d_get_iblank(case, im, iblank) bind(C, name='ot2d_get_iblank')
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: case
+      integer(c_int32_t), value :: im
+      integer(c_int32_t), intent(out) :: iblank(*)
+    end function ot2d_get_iblank
+
     integer(c_int32_t) function ot2d_set_istep0(case, istep0) bind(C, name='ot2d_set_istep0')
       import :: c_int32_t, c_ptr
       type(c_ptr), value :: case
