@@ -145,6 +145,21 @@ int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop);       /* src/step.f:332     
  * be NULL).  istep0 is advanced by one first. */
 int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax);
 
+/* Time-accurate runs (itnmax > 1 Newton sub-iterations, or ntac = 2, 3 for
+ * BDF2 / BDF3 time accuracy; fixed grids, iunst = 0, not time-spectral):
+ * every step of ot2d_step / ot2d_iterate is one time step of the
+ * reference's driver loop (src/Turns_overset.f:357-510): the time levels
+ * shift (stqol, src/time.f:109-172), then itnmax sub-iterations of
+ * time_step, step (with the Newton time term, src/time.f:174-209) and
+ * monitor.  The norms ot2d_step returns are the last sub-iteration's.
+ * ot2d_set_dtpseudo sets the dual time of sub-iterations 1..n
+ * (dtpseudo(itn) of the deck; default: dualtime for all, n <= 16). */
+int32_t ot2d_set_dtpseudo(ot2d_case* c, int32_t n, const double* dtpseudo);
+/* Norms of every sub-iteration of the last time step:
+ * norms[(itn*nmesh*nspec + i)*3 + m] (rsum, resrho, resmax) and argmax in
+ * the same order (j, k, n), itn = 0..itnmax-1.  Either may be NULL. */
+int32_t ot2d_subiter_norms(ot2d_case* c, double* norms, int32_t* argmax);
+
 /* niter steps without host synchronisation in between; the residual
  * history (rsum per step per mesh/instance) is written to hist
  * (niter*nmesh*nspec doubles, may be NULL).  Stops early on MSTOP/NORM and

@@ -191,6 +191,37 @@ def run_driver_overset_case():
     print("driver_overset_sst_97x25_61x61", sorted(files))
 
 
+# Time-accurate decks (fixed grids, iunst = 0): Newton sub-iterations with
+# the time term of src/time.f:174-209 and BDF2/BDF3 levels (stqol,
+# src/time.f:109-172); the reference program's output files.
+# name: (jmax, kmax, ds_wall, deck, nsteps, nrest, npnorm, zero-filled allocations)
+DRIVER_TIME = {
+    "driver_newton_sa_41x21": (41, 21, 1e-4, dict(SA, itnmax=3, ntac=2, dt=0.5, iconstant_cfl=0), 8, 5, 2, False),
+    "driver_dual_sst_41x21": (41, 21, 1e-4, dict(SA, iturb=2, itnmax=4, ntac=3, idual=1, dt=0.2,
+                                                  dtpseudo=[2.0, 1.5, 1.0, 0.5]), 7, 4, 1, True),
+    "driver_newton_euler_33x17": (33, 17, 1e-3, dict(EULER, itnmax=2, ntac=1, dt=1.0), 6, 4, 2, False),
+}
+
+
+def driver_time_deck(name):
+    jm, km, ds, deck, nsteps, nrest, npnorm, _ = DRIVER_TIME[name]
+    g = grid.naca4_cgrid(jm, km, m=0.02, p=0.4, t=0.12, ds_wall=ds)
+    return g, dict(deck, nsteps=nsteps, nrest=nrest, npnorm=npnorm)
+
+
+def run_driver_time_case(name):
+    g, deck = driver_time_deck(name)
+    zero = DRIVER_TIME[name][-1]
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_case(d, g, **deck)
+        env = dict(os.environ, **ZERO_MALLOC) if zero else dict(os.environ)
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([PROGRAM], cwd=d, stdin=fin, check=True, stdout=subprocess.DEVNULL, env=env)
+        files = _program_outputs(d, ["fort.71", "fort.11", "fort.8", "fort.9", "cp_TS1.dat", "SA.dat", "komega.dat"])
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **files)
+    print(name, sorted(files))
+
+
 # Overset fixtures (num_grids = 2): connectivity by the reference's connect2d
 # and, for the small pair, a trajectory with do_interpolations after every
 # iteration.  name: (near-body jmax, kmax, background jmax, kmax, deck, nsteps)
@@ -254,6 +285,9 @@ if __name__ == "__main__":
         run_driver_ts_case()
     if not only or "driver_overset_sst_97x25_61x61" in only:
         run_driver_overset_case()
+    for name in DRIVER_TIME:
+        if not only or name in only:
+            run_driver_time_case(name)
     for name, spec in OVERSET_CASES.items():
         if not only or name in only:
             run_overset_case(name, *spec)

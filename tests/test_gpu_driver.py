@@ -187,3 +187,38 @@ def test_driver_overset_matches_reference_program():
         compare_unformatted(os.path.join(d, "fort.9"), z["file:fort.9"], 1e-12, nmesh=2, grid_file=True)
         compare_text(os.path.join(d, "cp_TS1.dat"), z["file:cp_TS1.dat"], 1e-8)
         compare_text(os.path.join(d, "komega.dat"), z["file:komega.dat"], 1e-8)
+
+
+@needs_driver
+@pytest.mark.parametrize("name", ["driver_newton_sa_41x21", "driver_dual_sst_41x21", "driver_newton_euler_33x17"])
+def test_driver_time_accurate_matches_reference_program(name):
+    """Time-accurate decks on fixed grids: Newton sub-iterations (itnmax 2-4)
+    with the time term of newton (src/time.f:174-209), BDF2 / BDF3 levels
+    (ntac 2, 3; stqol), dual time per sub-iteration (idual = 1 with
+    dtpseudo 2.0, 1.5, 1.0, 0.5) and SST's time-accuracy factor.  One norm
+    line per sub-iteration, forces, restart and grid files, surface pressure
+    and turbulence files as the reference program writes them."""
+    import make_golden as mg
+    z = np.load(os.path.join(ROOT, "tests", "golden", f"{name}.npz"))
+    g, deck = mg.driver_time_deck(name)
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_case(d, g, **deck)
+        run_driver(d)
+        compare_columns(os.path.join(d, "fort.71"), z["file:fort.71"], 1e-8, 1)
+        compare_columns(os.path.join(d, "fort.11"), z["file:fort.11"], 1e-8, 1)
+        compare_unformatted(os.path.join(d, "fort.8"), z["file:fort.8"], 1e-9)
+        compare_unformatted(os.path.join(d, "fort.9"), z["file:fort.9"], 1e-12, grid_file=True)
+        compare_text(os.path.join(d, "cp_TS1.dat"), z["file:cp_TS1.dat"], 1e-8)
+        for f in ("SA.dat", "komega.dat"):
+            if "file:" + f in z:
+                compare_text(os.path.join(d, f), z["file:" + f], 1e-8)
+
+
+def test_subiteration_norms_need_a_time_accurate_case():
+    from tests import helpers as H
+    from turns2d_urans_git_amd import abi
+    c = H.gpu_case(H.cgrid(33, 17), dict(invisc=1, lamin=1, fsmach=0.5, alfa=1.25, irhsy=-3, ilim=1, ilhs=1,
+                                         iconstant_cfl=1, dt=10.0, epse=0.01, iunst=0, ntac=1, itnmax=1))
+    with pytest.raises(abi.CaseError):
+        c.subiter_norms(1)
+    c.close()

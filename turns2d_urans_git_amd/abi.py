@@ -133,6 +133,8 @@ def bind(lib: C.CDLL, prefix: str):
         "batch_retire": (C.c_int32, [C.c_void_p, C.c_int32]),
         "batch_admit": (C.c_int32, [C.c_void_p, C.c_int32, P(Params), P(MeshDesc)]),
         "batch_compact": (C.c_int32, [C.c_void_p, P(C.c_int32)]),
+        "set_dtpseudo": (C.c_int32, [C.c_void_p, C.c_int32, _DP]),
+        "subiter_norms": (C.c_int32, [C.c_void_p, _DP, _IP]),
         "set_launch": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
         "get_launch": (C.c_int32, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "autotune": (C.c_int32, [C.c_void_p, C.c_int32, P(C.c_int32)]),
@@ -406,6 +408,20 @@ class CaseAPI:
         self._fns["sizes"](C.byref(params), C.byref(mesh), C.byref(jd), C.byref(kd), C.byref(nq), C.byref(nv))
         self.dims[icase] = (jd.value, kd.value, nq.value, nv.value, mesh.jmax, mesh.kmax)
         return rc
+
+    def set_dtpseudo(self, values):
+        """dtpseudo(itn) of sub-iterations 1..len(values) (ot2d_set_dtpseudo)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        return self._check(self._fns["set_dtpseudo"](self.h, len(v), v), "set_dtpseudo")
+
+    def subiter_norms(self, itnmax: int):
+        """Norms (itnmax, instances, 3) and argmax of every sub-iteration of
+        the last time step (ot2d_subiter_norms)."""
+        per = len(self.meshes) * self.nspec
+        n = np.zeros(itnmax * per * 3)
+        a = np.zeros(itnmax * per * 3, dtype=np.int32)
+        self._check(self._fns["subiter_norms"](self.h, n, a), "subiter_norms")
+        return n.reshape(itnmax, per, 3), a.reshape(itnmax, per, 3)
 
     def batch_compact(self) -> int:
         """Launch the following steps over the running cases only

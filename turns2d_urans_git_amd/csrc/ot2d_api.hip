@@ -52,6 +52,7 @@ struct MeshBuf {
   int* iblank = nullptr;
   double* lu = nullptr;
   double* sa = nullptr;  // 10 (jd,kd) planes per instance when SA is on
+  double* ta = nullptr;  // time-accurate levels qtn, qtnm1, qnewt: 3 nv (jd,kd) planes per instance
   size_t lu_per = 0;  // doubles of LU workspace per instance
   bool has_grid = false;
   // what the arrays were allocated for (a batch slot may be refilled with a
@@ -84,6 +85,10 @@ struct ot2d_case {
   int* d_status = nullptr;
   int* d_istep0 = nullptr;
   int* d_stepc = nullptr;
+  int* d_istep = nullptr;      // time steps of this run per case (time-accurate steps)
+  int tacc = 0;                // time-accurate steps: stored levels, Newton sub-iterations
+  int itnmax = 1;
+  double* d_itnorms = nullptr; // norms of every sub-iteration of the last step (itnmax x 6 per instance)
   int* d_cinst = nullptr;      // first, one-past-last instance record of every case
   double* d_ctol = nullptr;    // convergence tolerance on rsum per case (0: off)
   int conv_on = 0;             // some case has a tolerance: the step checks it
@@ -276,6 +281,8 @@ static DevConst make_const(const ot2d_params* p, int iturb, int nmv) {
   c.iturb = iturb;
   c.nmv = nmv;
   c.motiontype = p->motiontype;
+  c.itnmax = p->itnmax;
+  for (int i = 0; i < OT2D_MAX_ITN; ++i) c.dtps[i] = p->dualtime;
   // src/setup.f:1232-1234 (itrans = 0), after rey = rey/fsmach
   c.tkeinf = 1e-6 * (p->fsmach * p->fsmach);
   c.tomegainf = 5 * p->fsmach / c.rey;
@@ -354,9 +361,9 @@ static void free_case(ot2d_case* c) {
     dfree(mb.q); dfree(mb.s); dfree(mb.x); dfree(mb.y); dfree(mb.xx); dfree(mb.xy);
     dfree(mb.yx); dfree(mb.yy); dfree(mb.ug); dfree(mb.vg); dfree(mb.turmu);
     dfree(mb.tscale); dfree(mb.bt); dfree(mb.xv); dfree(mb.yv); dfree(mb.ugv);
-    dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa);
+    dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa); dfree(mb.ta);
   }
-  dfree(c->dMa); dfree(c->dTa);
+  dfree(c->dMa); dfree(c->dTa); dfree(c->d_istep); dfree(c->d_itnorms);
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
   dfree(c->d_cinst); dfree(c->d_ctol);
   dfree(c->d_partials); dfree(c->d_norms); dfree(c->d_tspec);
@@ -382,8 +389,11 @@ static int32_t check_params(const ot2d_params* p) {
     return fail(OT2D_ERR_UNSUPPORTED, "ilim=%d not supported (1: Koren MUSCL, 4: WENO5)", p->ilim);
   if (!p->timespectral && p->iunst != 0)
     return fail(OT2D_ERR_UNSUPPORTED, "iunst=%d: time-accurate motion is not supported", p->iunst);
-  if (p->itnmax != 1) return fail(OT2D_ERR_UNSUPPORTED, "itnmax=%d: Newton sub-iterations not supported", p->itnmax);
-  if (p->ntac != 1) return fail(OT2D_ERR_UNSUPPORTED, "ntac=%d not supported", p->ntac);
+  if (p->itnmax < 1 || p->itnmax > OT2D_MAX_ITN)
+    return fail(OT2D_ERR_UNSUPPORTED, "itnmax=%d: 1 to %d Newton sub-iterations", p->itnmax, OT2D_MAX_ITN);
+  if (p->ntac < 1 || p->ntac > 3) return fail(OT2D_ERR_UNSUPPORTED, "ntac=%d: time accuracy 1, 2 or 3", p->ntac);
+  if (p->timespectral && (p->itnmax > 1 || p->ntac > 1))
+    return fail(OT2D_ERR_UNSUPPORTED, "Newton sub-iterations and time accuracy of time-spectral cases");
   if (p->timespectral && (p->nspec < 1 || p->nspec > 64)) return fail(OT2D_ERR_ARG, "nspec=%d", p->nspec);
   return OT2D_OK;
 }
@@ -392,7 +402,8 @@ static int32_t check_params(const ot2d_params* p) {
 static bool same_structure(const ot2d_params* a, const ot2d_params* b) {
   return a->invisc == b->invisc && a->lamin == b->lamin && a->iturb == b->iturb && a->itrans == b->itrans &&
          a->timespectral == b->timespectral && (a->timespectral ? a->nspec == b->nspec : true) &&
-         a->lu_fp32 == b->lu_fp32 && a->half == b->half;
+         a->lu_fp32 == b->lu_fp32 && a->half == b->half && a->itnmax == b->itnmax &&
+         (a->itnmax > 1 || a->ntac > 1) == (b->itnmax > 1 || b->ntac > 1);
 }
 
 // ncase parameter sets; mesh im belongs to case mesh_case[im]
@@ -464,6 +475,8 @@ static int32_t mesh_records(ot2d_case* c, int im, const ot2d_params* pc) {
     const int ic = c->mesh_case[im];
     M.status = c->d_status + ic;
     M.istep0 = c->d_istep0 + ic;
+    M.istep = c->d_istep + ic;
+    M.ta = mb.ta ? mb.ta + 3 * (size_t)c->nv * mb.n2 * nsp : nullptr;
     M.stepc = c->d_stepc + ic;
     M.norms = c->d_norms + 6 * mi;
     M.tspec = c->d_tspec + (size_t)ic * ns + nsp;
@@ -533,7 +546,10 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
 #define A(ptr, n) do { if ((e = dalloc(&(ptr), (n))) != hipSuccess) { free_case(c); \
     return fail(OT2D_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e)); } \
     c->bytes += (size_t)(n) * sizeof(*(ptr)); } while (0)
-  A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase);
+  c->itnmax = p->itnmax;
+  c->tacc = p->itnmax > 1 || p->ntac > 1;
+  A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase); A(c->d_istep, ncase);
+  if (c->tacc) A(c->d_itnorms, 6 * (size_t)nmi * c->itnmax);
   A(c->d_cinst, 2 * (size_t)ncase); A(c->d_ctol, ncase);
   A(c->d_norms, 6 * nmi); A(c->d_tspec, (size_t)ns * ncase); A(c->d_Ds, ns * ns);
   A(c->d_forces, 16);
@@ -615,6 +631,7 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     mb.lu_per = lu_workspace_bytes(mb.jd, mb.kd, c->lu_rl) / sizeof(double);
     A(mb.lu, mb.lu_per * ns);
     if (!c->lamin) A(mb.sa, (c->iturb == 2 ? 20 : 10) * mb.n2 * ns);
+    if (c->tacc) A(mb.ta, 3 * (size_t)c->nv * mb.n2 * ns);
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
     mb.cap_n2 = mb.n2; mb.cap_nv2 = mb.nv2; mb.cap_lu = mb.lu_per; mb.cap_tiles = tj * tk;
     mb.pofs = ntile_total;
@@ -666,12 +683,15 @@ static size_t footprint(const ot2d_params* ps, int32_t ncase, int32_t nmesh, con
   const size_t ns = ps->timespectral ? (size_t)ps->nspec : 1;
   int rl = 1;
   for (int im = 0; im < nmesh; ++im) rl = (md[im].kmax - 1 + 63) / 64 > rl ? (md[im].kmax - 1 + 63) / 64 : rl;
-  size_t b = (size_t)ncase * (3 * sizeof(int) + 2 * sizeof(int) + sizeof(double)) +
+  const bool tacc = ps->itnmax > 1 || ps->ntac > 1;
+  size_t b = (size_t)ncase * (4 * sizeof(int) + 2 * sizeof(int) + sizeof(double)) +
              (6 * (size_t)nmesh * ns + ns * ncase + ns * ns + 16) * sizeof(double);
+  if (tacc) b += 6 * (size_t)nmesh * ns * ps->itnmax * sizeof(double);
   for (int im = 0; im < nmesh; ++im) {
     const size_t jd = md[im].jmax + 1, kd = md[im].kmax + 1, n2 = jd * kd, nv2 = (size_t)md[im].jmax * md[im].kmax;
     size_t d = n2 * (nq + nv) + 11 * n2 + 4 * nv2;
     if (!lamin) d += (iturb == 2 ? 20 : 10) * n2;
+    if (tacc) d += 3 * (size_t)nv * n2;
     const size_t tiles = ((jd - 2 + RES_TJ - 1) / RES_TJ) * ((kd - 2 + RES_TK - 1) / RES_TK);
     b += ns * (d * sizeof(double) + n2 * sizeof(int) + lu_workspace_bytes((int)jd, (int)kd, rl) +
                tiles * (4 * sizeof(double) + sizeof(ResTile)) + sizeof(DevMesh));
@@ -771,6 +791,9 @@ extern "C" int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, con
   if (ic == 0) c->istep0 = 0;
   HIPCHK(hipMemsetAsync(c->d_istep0 + ic, 0, sizeof(int), c->stream));
   HIPCHK(hipMemsetAsync(c->d_status + ic, 0, sizeof(int), c->stream));
+  // a time-accurate run starts over: no earlier time levels
+  HIPCHK(hipMemsetAsync(c->d_istep + ic, 0, sizeof(int), c->stream));
+  if (mb.ta) HIPCHK(hipMemsetAsync(mb.ta, 0, 3 * (size_t)c->nv * mb.cap_n2 * c->nspec * sizeof(double), c->stream));
   full_view(c);   // the case runs again
   return sync(c);
 }
@@ -919,14 +942,17 @@ static void full_view(ot2d_case* c) {
   c->graph_valid = 0;
 }
 
-static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks* mk = nullptr) {
+// newton: the time term of a Newton sub-iteration joins the residual after
+// the inviscid terms and before the viscous ones (src/step.f:266-277)
+static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks* mk = nullptr, bool newton = false) {
   const StepView v = step_view(c);
   HIPCHK(launch_bc(v.dM, v.n, zero_halo_s, c->stream));
   mark(mk, OT2D_ST_BC, c->stream);
   // the norms are taken on the complete residual (src/step.f:46-93)
-  const bool later = !c->p.invisc || c->p.timespectral;
+  const bool later = !c->p.invisc || c->p.timespectral || newton;
   HIPCHK(launch_residual(v.dM, v.dT, v.nt, norms && !later, c->weno, c->stream));
   mark(mk, OT2D_ST_RESIDUAL, c->stream);
+  if (newton) HIPCHK(launch_newton(v.dM, v.hM, v.n, c->stream));
   // without time-spectral terms the residual is complete after visrhs,
   // which then takes the norm partials itself
   const bool vis_norms = norms && !c->p.invisc && !c->p.timespectral;
@@ -984,7 +1010,42 @@ extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
 }
 
 // one step (src/step.f:2-218) + monitor, enqueued without synchronisation
-static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr) {
+static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr);
+
+// a time step of a time-accurate run (src/Turns_overset.f:357-510): the
+// step opens, the time levels shift (stqol), then itnmax Newton
+// sub-iterations of time_step, step and monitor, each with the fringe
+// exchange of overset meshes; every sub-iteration's norms are kept
+static int32_t enqueue_time_step(ot2d_case* c, Marks* mk) {
+  const StepView v = step_view(c);
+  HIPCHK(launch_open_step(c->d_status, c->d_istep0, c->d_stepc, c->d_istep, c->ncase, c->stream));
+  HIPCHK(launch_stqol(v.dM, v.hM, v.n, c->stream));
+  for (int itn = 1; itn <= c->itnmax; ++itn) {
+    HIPCHK(launch_time_step(v.dM, v.hM, v.n, c->stream, nullptr, nullptr, nullptr, 0, itn));
+    mark(mk, OT2D_ST_TIME_STEP, c->stream);
+    if (int32_t rc = enqueue_residual(c, 1, 1, mk, c->itnmax > 1)) return rc;
+    HIPCHK(launch_rbc(v.dM, v.n, c->stream));
+    mark(mk, OT2D_ST_RBC, c->stream);
+    if (c->lamin) {
+      HIPCHK(launch_lu(v.dM, v.hM, v.n, 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
+    } else {
+      HIPCHK(launch_lu(v.dM, v.hM, v.n, 0, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
+      if (c->iturb == 2) HIPCHK(launch_sst(v.dM, v.hM, v.n, mk, c->stream));
+      else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane));
+      HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream));
+      mark(mk, OT2D_ST_UPDATE, c->stream);
+    }
+    if (c->ndon > 0 || c->nfr > 0) {
+      HIPCHK(launch_interp(c->dM, c->d_don, c->ndon, c->d_fr, c->nfr, c->d_qbc, c->nv, c->stream));
+      mark(mk, OT2D_ST_INTERP, c->stream);
+    }
+    HIPCHK(launch_copy_doubles(c->d_itnorms + (size_t)(itn - 1) * 6 * nmi(c), c->d_norms, 6 * nmi(c), c->stream));
+  }
+  return OT2D_OK;
+}
+
+static int32_t enqueue_step(ot2d_case* c, Marks* mk) {
+  if (c->tacc) return enqueue_time_step(c, mk);
   const StepView v = step_view(c);
   // the step opens (istep0, step counters) inside the time-step kernel
   HIPCHK(launch_time_step(v.dM, v.hM, v.n, c->stream, c->d_istep0, c->d_status, c->d_stepc, c->ncase));
@@ -1280,6 +1341,29 @@ extern "C" int32_t ot2d_query(ot2d_case* c, int32_t* idle) {
   return fail(OT2D_ERR_HIP, "stream query: %s", hipGetErrorString(e));
 }
 
+extern "C" int32_t ot2d_set_dtpseudo(ot2d_case* c, int32_t n, const double* dtpseudo) {
+  if (!c || !dtpseudo || n < 1 || n > OT2D_MAX_ITN) return fail(OT2D_ERR_ARG, "bad arguments");
+  for (auto& M : c->hM)
+    for (int i = 0; i < n; ++i) M.c.dtps[i] = dtpseudo[i];
+  if (int32_t rc = upload_meshes(c)) return rc;
+  return sync(c);
+}
+
+extern "C" int32_t ot2d_subiter_norms(ot2d_case* c, double* norms, int32_t* argmax) {
+  if (!c) return fail(OT2D_ERR_ARG, "null case");
+  if (!c->tacc) return fail(OT2D_ERR_STATE, "not a time-accurate case (itnmax = 1 and ntac = 1)");
+  const int per = nmi(c);
+  std::vector<double> b(6 * (size_t)per * c->itnmax);
+  HIPCHK(hipMemcpyAsync(b.data(), c->d_itnorms, sizeof(double) * b.size(), hipMemcpyDeviceToHost, c->stream));
+  if (int32_t rc = sync(c)) return rc;
+  for (size_t k = 0; k < (size_t)per * c->itnmax; ++k)
+    for (int m = 0; m < 3; ++m) {
+      if (norms) norms[3 * k + m] = b[6 * k + m];
+      if (argmax) argmax[3 * k + m] = (int32_t)b[6 * k + 3 + m];
+    }
+  return OT2D_OK;
+}
+
 extern "C" int32_t ot2d_forces(ot2d_case* c, int32_t im, int32_t nsp, double out[6]) {
   if (int32_t rc = check_im(c, im)) return rc;
   if (nsp < 0 || nsp >= c->nspec || !out) return fail(OT2D_ERR_ARG, "bad instance");
@@ -1509,6 +1593,8 @@ extern "C" int32_t ot2d_batch_admit(ot2d_case* c, int32_t icase, const ot2d_para
   const int zero[1] = {0};
   HIPCHK(hipMemcpyAsync(c->d_istep0 + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_stepc + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_istep + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  if (mb.ta) HIPCHK(hipMemsetAsync(mb.ta, 0, 3 * (size_t)c->nv * mb.cap_n2 * ns * sizeof(double), c->stream));
   HIPCHK(hipMemcpyAsync(c->d_status + icase, zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
   c->graph_valid = 0;   // the tiles and launch sizes may have changed
   return sync(c);
@@ -1628,35 +1714,41 @@ extern "C" int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured)
   auto cleanup = [&]() { for (double* p : save) dfree(p); };
   const int nc = c->ncase;
   int* save_i = nullptr;
-  if (hipMalloc(&save_i, 3 * sizeof(int) * nc) != hipSuccess) return fail(OT2D_ERR_NOMEM, "autotune buffers");
+  if (hipMalloc(&save_i, 4 * sizeof(int) * nc) != hipSuccess) return fail(OT2D_ERR_NOMEM, "autotune buffers");
   auto copy_state = [&](bool back) -> hipError_t {
     hipError_t e = hipSuccess;
     for (size_t im = 0; im < c->m.size() && e == hipSuccess; ++im) {
       MeshBuf& mb = c->m[im];
       const size_t nq = mb.n2 * c->nspec * c->nq, nt = mb.n2 * c->nspec;
-      double* sq = save[2 * im];
-      double* st = save[2 * im + 1];
+      double* sq = save[3 * im];
+      double* st = save[3 * im + 1];
+      double* sl = save[3 * im + 2];
       e = hipMemcpyAsync(back ? mb.q : sq, back ? sq : mb.q, nq * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(back ? mb.turmu : st, back ? st : mb.turmu, nt * sizeof(double), hipMemcpyDeviceToDevice,
                            c->stream);
+      if (e == hipSuccess && mb.ta)
+        e = hipMemcpyAsync(back ? mb.ta : sl, back ? sl : mb.ta, 3 * c->nv * nt * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->stream);
     }
-    int* src[3] = {c->d_status, c->d_istep0, c->d_stepc};
-    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+    int* src[4] = {c->d_status, c->d_istep0, c->d_stepc, c->d_istep};
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
       e = hipMemcpyAsync(back ? src[k] : save_i + k * nc, back ? save_i + k * nc : src[k], sizeof(int) * nc,
                          hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     return e;
   };
   for (auto& mb : c->m) {
-    double *a = nullptr, *b = nullptr;
+    double *a = nullptr, *b = nullptr, *l = nullptr;
     if (hipMalloc(&a, mb.n2 * c->nspec * c->nq * sizeof(double)) != hipSuccess ||
-        hipMalloc(&b, mb.n2 * c->nspec * sizeof(double)) != hipSuccess) {
-      dfree(a); dfree(save_i); cleanup();
+        hipMalloc(&b, mb.n2 * c->nspec * sizeof(double)) != hipSuccess ||
+        (mb.ta && hipMalloc(&l, 3 * c->nv * mb.n2 * c->nspec * sizeof(double)) != hipSuccess)) {
+      dfree(a); dfree(b); dfree(save_i); cleanup();
       return fail(OT2D_ERR_NOMEM, "autotune buffers");
     }
     save.push_back(a);
     save.push_back(b);
+    save.push_back(l);
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = copy_state(false);

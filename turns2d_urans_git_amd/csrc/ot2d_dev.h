@@ -27,6 +27,8 @@ struct DevBC {
 
 // Case constants derived as read_inputs does (src/setup.f:1162-1248) plus
 // the scheme switches the kernels branch on.
+constexpr int OT2D_MAX_ITN = 16;   // Newton sub-iterations per time step, at most
+
 struct DevConst {
   double gamma, gm1, ggm1, einf, pinf, rinf, uinf, vinf, rey, rmue, h;
   double th, qt, epse, Mp, dualtime, timeac, jac_factor, pr, tinf;
@@ -36,6 +38,8 @@ struct DevConst {
   double fmm, half, ca, sa;  // force normalisation (src/forces.f:27-31, 166-172)
   int invisc, iread, ilim, iconstant_cfl, idual, iprecon, timespectral, ntac;
   int iturb, nmv, motiontype;
+  int itnmax;                // Newton sub-iterations per time step
+  double dtps[OT2D_MAX_ITN]; // dtpseudo(itn), the dual time of sub-iteration itn
 };
 
 struct DevMesh {
@@ -68,6 +72,8 @@ struct DevMesh {
   // per-case bookkeeping (shared by all instances of a case)
   int* status;               // 0 ok, 1 mstop, 2 norm
   int* istep0;               // device copy of istep0
+  int* istep;                // time steps of this run (the driver's istep; time-accurate steps)
+  double* ta;                // time-accurate levels: qtn, qtnm1, qnewt (nv planes each), or null
   double* partials;          // norm partials of this instance's tiles (res_nblocks * 4 doubles)
   double* norms;             // rsum, resrho, resmax, jmax, kmax, nmax
   int* stepc;                // steps started in the current ot2d_iterate call
@@ -209,6 +215,17 @@ constexpr int RES_THREADS = RES_TJ * RES_TK;
 // one after another
 constexpr int BC_THREADS = 256;
 
+// oat of the time-accuracy order (src/time.f:22-25): the reference's istep
+// counts the time steps of the run
+__device__ inline double time_oat(const DevMesh& M) {
+  double oat = 1.0;
+  if (M.c.ntac < 2) return oat;
+  const int istep = *M.istep;
+  if (M.c.ntac >= 2 && istep > 1) oat = 2. / 3.;
+  if (M.c.ntac == 3 && istep > 2) oat = 6. / 11.;
+  return oat;
+}
+
 __host__ __device__ inline long long cell(const DevMesh& M, int j, int k) {
   return (long long)(j - 1) + (long long)(k - 1) * M.jd;
 }
@@ -278,7 +295,12 @@ hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st);
 // with ncase > 0 also opens the step of every running case (istep0 and the
 // step counter)
 hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int* istep0 = nullptr,
-                            const int* status = nullptr, int* stepc = nullptr, int ncase = 0);
+                            const int* status = nullptr, int* stepc = nullptr, int ncase = 0, int itn = 0);
+// time-accurate steps with Newton sub-iterations (src/time.f:109-209)
+hipError_t launch_open_step(const int* status, int* istep0, int* stepc, int* istep, int ncase, hipStream_t st);
+hipError_t launch_stqol(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_newton(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_copy_doubles(double* dst, const double* src, int n, hipStream_t st);
 hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, int weno,
                            hipStream_t st);

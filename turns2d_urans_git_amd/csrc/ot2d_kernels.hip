@@ -33,9 +33,12 @@ __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
 // =================================================================== time_step
 // src/time.f:2-107 (one value per cell, all cells incl. halo)
-__device__ inline void time_step_cell(const DevMesh& M, long long i) {
+// itn > 0: Newton sub-iteration itn of a time-accurate step, whose dual time
+// is dtpseudo(itn) (src/Turns_overset.f:478)
+__device__ inline void time_step_cell(const DevMesh& M, long long i, int itn) {
   const DevConst& C = M.c;
-  double oat = 1.0;  // ntac = 1 (steady); ntac>=2 handled on the host (rejected)
+  const double oat = time_oat(M);
+  const double dualtime = itn > 0 ? C.dtps[itn - 1] : C.dualtime;
   double ibp = (double)imax(M.iblank[i], 0);
   double qj = M.q[M.nq - 1][i];
   if (C.iconstant_cfl == 0) {
@@ -46,7 +49,7 @@ __device__ inline void time_step_cell(const DevMesh& M, long long i) {
     } else {
       double t = ibp * (1.0 + C.jac_factor * sqrt(qj)) / (1.0 + sqrt(qj));
       if (C.iprecon) t = 100. * ibp * (1.0 + 0.005 * sqrt(qj)) / (1.0 + sqrt(qj));
-      t = t * C.dualtime;
+      t = t * dualtime;
       M.bt[i] = 1. / (1. + t / C.h / oat);
       M.tscale[i] = t / (1. + t / C.h / oat);
     }
@@ -68,7 +71,7 @@ __device__ inline void time_step_cell(const DevMesh& M, long long i) {
     M.tscale[i] = t * ibp;
     M.bt[i] = 1.;
   } else {
-    double hnew = C.dualtime / (lam_j + lam_k);
+    double hnew = dualtime / (lam_j + lam_k);
     double t = hnew * ibp;
     t = t / (1. + t / C.h / oat);
     M.tscale[i] = t;
@@ -80,7 +83,7 @@ __device__ inline void time_step_cell(const DevMesh& M, long long i) {
 // case (istep0 and the step counter; src/Turns_overset.f:364): the
 // boundary conditions launched next are the first readers of istep0.
 __global__ __launch_bounds__(256) void k_time_step(const DevMesh* __restrict__ Ms, int* istep0, const int* status,
-                                                   int* stepc, int ncase) {
+                                                   int* stepc, int ncase, int itn) {
   if (blockIdx.x == 0 && blockIdx.y == 0)
     for (int c = threadIdx.x; c < ncase; c += 256)
       if (status[c] == 0) {
@@ -91,7 +94,71 @@ __global__ __launch_bounds__(256) void k_time_step(const DevMesh* __restrict__ M
   if (*M.status) return;
   long long n2 = (long long)M.jd * M.kd;
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x)
-    time_step_cell(M, i);
+    time_step_cell(M, i, itn);
+}
+
+// ====================================================== time-accurate steps
+// A time step of a time-accurate run opens here: istep0, the call's step
+// counter and the run's time-step counter istep of every running case
+// (src/Turns_overset.f:362-365); its Newton sub-iterations follow.
+__global__ __launch_bounds__(256) void k_open_step(const int* status, int* istep0, int* stepc, int* istep,
+                                                   int ncase) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < ncase && status[c] == 0) {
+    istep0[c] += 1;
+    stepc[c] += 1;
+    istep[c] += 1;
+  }
+}
+
+// stqol (src/time.f:109-172): the Newton target qnewt (the state, plus the
+// BDF2/BDF3 extrapolation once there are earlier levels) and the shift of
+// the stored levels, every cell and variable 1..nv
+__global__ __launch_bounds__(256) void k_stqol(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const long long n2 = (long long)M.jd * M.kd;
+  const int istep = *M.istep, ntac = M.c.ntac, nv = M.nv;
+  const bool bdf2 = (ntac == 2 && istep > 1) || (ntac == 3 && istep == 2);
+  const bool bdf3 = ntac == 3 && istep > 2;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x)
+    for (int n = 0; n < nv; ++n) {
+      double* qtn = M.ta + (long long)n * n2;
+      double* qtnm1 = M.ta + (long long)(nv + n) * n2;
+      double* qnewt = M.ta + (long long)(2 * nv + n) * n2;
+      const double q = M.q[n][i], a = qtn[i], b = qtnm1[i];
+      double w = q;
+      if (bdf2) w = w + (q - a) / 3.;
+      if (bdf3) w = w + 7. * (q - a) / 11. - 2. * (a - b) / 11.;
+      qnewt[i] = w;
+      qtnm1[i] = a;
+      qtn[i] = q;
+    }
+}
+
+// newton (src/time.f:174-209): the time term of the Newton iteration on the
+// interior residual, s - (q - qnewt) / (oat h), variables 1..nv
+__global__ __launch_bounds__(256) void k_newton(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
+  const long long ncell = (long long)nj * nk;
+  const long long n2 = (long long)M.jd * M.kd;
+  const double tac = 1. / (time_oat(M) * M.c.h);
+  for (long long w = blockIdx.x * 256ll + threadIdx.x; w < ncell; w += 256ll * gridDim.x) {
+    const long long i = cell(M, M.js + (int)(w % nj), M.ks + (int)(w / nj));
+    for (int n = 0; n < M.nv; ++n) {
+      const double* qnewt = M.ta + (long long)(2 * M.nv + n) * n2;
+      M.s[n][i] = M.s[n][i] - (M.q[n][i] - qnewt[i]) * tac;
+    }
+  }
+}
+
+// n doubles from src to dst (a graph holds no copy nodes)
+__global__ __launch_bounds__(256) void k_copy_doubles(double* __restrict__ dst, const double* __restrict__ src,
+                                                      int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 // ======================================================================= BCs
@@ -2466,10 +2533,35 @@ hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st) {
 }
 
 hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int* istep0,
-                            const int* status, int* stepc, int ncase) {
+                            const int* status, int* stepc, int ncase, int itn) {
   long long mx = 0;
   for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
-  hipLaunchKernelGGL(k_time_step, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, istep0, status, stepc, ncase);
+  hipLaunchKernelGGL(k_time_step, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, istep0, status, stepc, ncase,
+                     itn);
+  return hipGetLastError();
+}
+
+hipError_t launch_open_step(const int* status, int* istep0, int* stepc, int* istep, int ncase, hipStream_t st) {
+  hipLaunchKernelGGL(k_open_step, dim3((ncase + 255) / 256), dim3(256), 0, st, status, istep0, stepc, istep, ncase);
+  return hipGetLastError();
+}
+
+hipError_t launch_stqol(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  hipLaunchKernelGGL(k_stqol, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_newton(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  hipLaunchKernelGGL(k_newton, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_doubles(double* dst, const double* src, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_doubles, dim3((n + 255) / 256), dim3(256), 0, st, dst, src, n);
   return hipGetLastError();
 }
 

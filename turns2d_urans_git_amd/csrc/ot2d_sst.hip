@@ -365,11 +365,11 @@ __global__ __launch_bounds__(256) void k_sst_rhs(const DevMesh* __restrict__ Ms)
         bj[n] = bj[n] + (ay + cy) / rho;
       }
     }
-    // ---- pseudo-time scaling (oat = 1)
+    // ---- pseudo-time scaling (src/komegasst.f:100-118)
     double tscal = M.tscale[i];
     if (C.timeac == 1.) {
       double qj = M.q[M.nq - 1][i];
-      const double dtpseudo_turb = 10.0, oat = 1.0;
+      const double dtpseudo_turb = 10.0, oat = time_oat(M);
       tscal = (double)imk(M.iblank[i], 0) * (1.0 + 0.002 * sqrt(qj)) / (1. + sqrt(qj));
       tscal = tscal * dtpseudo_turb;
       tscal = tscal / (1. + tscal / C.h / oat);

@@ -97,6 +97,22 @@ module ot2d_capi
       real(c_double), intent(out) :: out(*)
     end function ot2d_get_field
 
+    ! time-accurate runs: dual time per Newton sub-iteration, and the norms of
+    ! every sub-iteration of the last time step
+    integer(c_int32_t) function ot2d_set_dtpseudo(case, n, dtpseudo) bind(C, name='ot2d_set_dtpseudo')
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: case
+      integer(c_int32_t), value :: n
+      real(c_double), intent(in) :: dtpseudo(*)
+    end function ot2d_set_dtpseudo
+
+    integer(c_int32_t) function ot2d_subiter_norms(case, norms, argmax) bind(C, name='ot2d_subiter_norms')
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: case
+      real(c_double), intent(out) :: norms(*)
+      integer(c_int32_t), intent(out) :: argmax(*)
+    end function ot2d_subiter_norms
+
     integer(c_int32_t) function ot2d_get_iblank(case, im, iblank) bind(C, name='ot2d_get_iblank')
       import :: c_int32_t, c_ptr
       type(c_ptr), value :: case

@@ -74,10 +74,10 @@ program ot2d_run
   type(mesh_t), allocatable :: ms(:)
   type(c_ptr) :: case
   integer(c_int32_t) :: rc, jd, kd, nq, nv, done, istep0c, nfr
-  real(8), allocatable :: norms(:)
-  integer(c_int32_t), allocatable :: argmax(:)
+  real(8), allocatable :: norms(:), itnorms(:)
+  integer(c_int32_t), allocatable :: argmax(:), itargmax(:)
   real(8) :: f(6), theta_col, rey_a
-  integer :: im, ib, istep, istep0, nrc2, nxt, ios, nmesh, ns, nsp, i, nm1
+  integer :: im, ib, istep, istep0, nrc2, nxt, ios, nmesh, ns, nsp, i, nm1, itn
   character(len=16) :: luenv
   character(len=64) :: fname
 
@@ -169,6 +169,10 @@ program ot2d_run
     call ot2d_check(ot2d_get_field(case, int(im - 1, c_int32_t), OT2D_F_Y, ms(im)%y), 'ot2d_get_field')
   end do
   allocate(norms(3 * nmesh * ns), argmax(3 * nmesh * ns))
+  allocate(itnorms(3 * nmesh * ns * itnmax), itargmax(3 * nmesh * ns * itnmax))
+  ! the dual time of each Newton sub-iteration (src/Turns_overset.f:478)
+  if (itnmax > 1) call ot2d_check(ot2d_set_dtpseudo(case, int(min(itnmax, 16), c_int32_t), dtpseudo), &
+       'ot2d_set_dtpseudo')
 
   rey_a = merge(rey / fsmach, rey / fmtip, fsmach /= 0.d0)
   istep0 = 0
@@ -232,12 +236,22 @@ program ot2d_run
     call ot2d_check(rc, 'ot2d_step')
     istep = istep + 1; istep0 = istep0 + 1
     if (mod(istep0, npnorm) == 0) then
-      do im = 1, nmesh
-        do nsp = 1, ns
-          i = 3 * ((im - 1) * ns + nsp - 1)
-          write(70 + im, '(i7,5(e18.10))') istep0, norms(i + 1), norms(i + 2), norms(i + 3), totime, theta_col
-          write(6, '(a,3i4,3(x,e12.4))') '  j,k,n,rmax,l2rho,l2 =', argmax(i + 1), argmax(i + 2), &
-               argmax(i + 3), norms(i + 3), norms(i + 2), norms(i + 1)
+      ! one line per Newton sub-iteration of the step (src/step.f:52-101)
+      if (itnmax > 1) then
+        call ot2d_check(ot2d_subiter_norms(case, itnorms, itargmax), 'ot2d_subiter_norms')
+      else
+        itnorms = norms
+        itargmax = argmax
+      end if
+      do itn = 1, itnmax
+        do im = 1, nmesh
+          do nsp = 1, ns
+            i = 3 * (((itn - 1) * nmesh + im - 1) * ns + nsp - 1)
+            write(70 + im, '(i7,5(e18.10))') istep0, itnorms(i + 1), itnorms(i + 2), itnorms(i + 3), totime, &
+                 theta_col
+            write(6, '(a,3i4,3(x,e12.4))') '  j,k,n,rmax,l2rho,l2 =', itargmax(i + 1), itargmax(i + 2), &
+                 itargmax(i + 3), itnorms(i + 3), itnorms(i + 2), itnorms(i + 1)
+          end do
         end do
       end do
     end if
