@@ -145,13 +145,16 @@ int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop);       /* src/step.f:332     
  * be NULL).  istep0 is advanced by one first. */
 int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax);
 
-/* Time-accurate runs (itnmax > 1 Newton sub-iterations, or ntac = 2, 3 for
- * BDF2 / BDF3 time accuracy; fixed grids, iunst = 0, not time-spectral):
- * every step of ot2d_step / ot2d_iterate is one time step of the
- * reference's driver loop (src/Turns_overset.f:357-510): the time levels
- * shift (stqol, src/time.f:109-172), then itnmax sub-iterations of
- * time_step, step (with the Newton time term, src/time.f:174-209) and
- * monitor.  The norms ot2d_step returns are the last sub-iteration's.
+/* Time-accurate runs (itnmax > 1 Newton sub-iterations, ntac = 2, 3 for
+ * BDF2 / BDF3 time accuracy, or a pitching grid, iunst = 1 with
+ * motiontype = 1; not time-spectral): every step of ot2d_step /
+ * ot2d_iterate is one time step of the reference's driver loop
+ * (src/Turns_overset.f:357-510): a pitching grid turns by the step's angle
+ * (pitch, src/motion.f:546-632) and its metrics, grid velocities and wall
+ * distances follow, the time levels shift (stqol, src/time.f:109-172), then
+ * itnmax sub-iterations of time_step, step (with the Newton time term,
+ * src/time.f:174-209, and the swept-area face speeds of the moving grid,
+ * src/eulerrhs.f:118-136) and monitor.  The norms ot2d_step returns are the last sub-iteration's.
  * ot2d_set_dtpseudo sets the dual time of sub-iterations 1..n
  * (dtpseudo(itn) of the deck; default: dualtime for all, n <= 16). */
 int32_t ot2d_set_dtpseudo(ot2d_case* c, int32_t n, const double* dtpseudo);

@@ -200,6 +200,14 @@ DRIVER_TIME = {
     "driver_dual_sst_41x21": (41, 21, 1e-4, dict(SA, iturb=2, itnmax=4, ntac=3, idual=1, dt=0.2,
                                                   dtpseudo=[2.0, 1.5, 1.0, 0.5]), 7, 4, 1, True),
     "driver_newton_euler_33x17": (33, 17, 1e-3, dict(EULER, itnmax=2, ntac=1, dt=1.0), 6, 4, 2, False),
+    # pitching grids (iunst = 1, motiontype = 1, src/motion.f:546-632)
+    "driver_pitch_sa_41x21": (41, 21, 1e-4, dict(SA, iunst=1, motiontype=1, rf=0.2, angmax=2.0, dt=0.05,
+                                                  iconstant_cfl=0), 8, 5, 2, False),
+    "driver_pitch_euler_33x17": (33, 17, 1e-3, dict(EULER, iunst=1, motiontype=1, rf=0.3, angmax=3.0, dt=0.1,
+                                                     ntac=2, itnmax=3, iconstant_cfl=0), 7, 4, 1, False),
+    "driver_pitch_sst_41x21": (41, 21, 1e-4, dict(SA, iturb=2, iunst=1, motiontype=1, rf=0.25, angmax=1.5,
+                                                   dt=0.05, ntac=2, itnmax=2, idual=1, dtpseudo=[1.0, 0.5]),
+                               6, 4, 2, True),
 }
 
 

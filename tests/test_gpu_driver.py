@@ -190,14 +190,18 @@ def test_driver_overset_matches_reference_program():
 
 
 @needs_driver
-@pytest.mark.parametrize("name", ["driver_newton_sa_41x21", "driver_dual_sst_41x21", "driver_newton_euler_33x17"])
+@pytest.mark.parametrize("name", ["driver_newton_sa_41x21", "driver_dual_sst_41x21", "driver_newton_euler_33x17",
+                                  "driver_pitch_sa_41x21", "driver_pitch_euler_33x17", "driver_pitch_sst_41x21"])
 def test_driver_time_accurate_matches_reference_program(name):
-    """Time-accurate decks on fixed grids: Newton sub-iterations (itnmax 2-4)
-    with the time term of newton (src/time.f:174-209), BDF2 / BDF3 levels
-    (ntac 2, 3; stqol), dual time per sub-iteration (idual = 1 with
-    dtpseudo 2.0, 1.5, 1.0, 0.5) and SST's time-accuracy factor.  One norm
-    line per sub-iteration, forces, restart and grid files, surface pressure
-    and turbulence files as the reference program writes them."""
+    """Time-accurate decks: Newton sub-iterations (itnmax 2-4) with the time
+    term of newton (src/time.f:174-209), BDF2 / BDF3 levels (ntac 2, 3;
+    stqol), dual time per sub-iteration (idual = 1 with dtpseudo 2.0, 1.5,
+    1.0, 0.5) and SST's time-accuracy factor, on fixed grids and on pitching
+    grids (iunst = 1: the grid turns every step, metrics, grid velocities and
+    the swept-area face speeds of first and second order follow).  One norm
+    line per sub-iteration, forces (with the pitch angle), restart and grid
+    files, surface pressure and turbulence files as the reference program
+    writes them."""
     import make_golden as mg
     z = np.load(os.path.join(ROOT, "tests", "golden", f"{name}.npz"))
     g, deck = mg.driver_time_deck(name)

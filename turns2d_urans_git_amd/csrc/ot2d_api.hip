@@ -53,6 +53,7 @@ struct MeshBuf {
   double* lu = nullptr;
   double* sa = nullptr;  // 10 (jd,kd) planes per instance when SA is on
   double* ta = nullptr;  // time-accurate levels qtn, qtnm1, qnewt: 3 nv (jd,kd) planes per instance
+  double *xold = nullptr, *yold = nullptr, *xole = nullptr, *yole = nullptr;  // moving grids: earlier vertices
   size_t lu_per = 0;  // doubles of LU workspace per instance
   bool has_grid = false;
   // what the arrays were allocated for (a batch slot may be refilled with a
@@ -86,7 +87,8 @@ struct ot2d_case {
   int* d_istep0 = nullptr;
   int* d_stepc = nullptr;
   int* d_istep = nullptr;      // time steps of this run per case (time-accurate steps)
-  int tacc = 0;                // time-accurate steps: stored levels, Newton sub-iterations
+  int tacc = 0;                // time-accurate steps: stored levels, Newton sub-iterations, motion
+  int moving = 0;              // the grid pitches every time step (iunst = 1)
   int itnmax = 1;
   double* d_itnorms = nullptr; // norms of every sub-iteration of the last step (itnmax x 6 per instance)
   int* d_cinst = nullptr;      // first, one-past-last instance record of every case
@@ -282,6 +284,7 @@ static DevConst make_const(const ot2d_params* p, int iturb, int nmv) {
   c.nmv = nmv;
   c.motiontype = p->motiontype;
   c.itnmax = p->itnmax;
+  c.iunst = p->timespectral ? 0 : p->iunst;
   for (int i = 0; i < OT2D_MAX_ITN; ++i) c.dtps[i] = p->dualtime;
   // src/setup.f:1232-1234 (itrans = 0), after rey = rey/fsmach
   c.tkeinf = 1e-6 * (p->fsmach * p->fsmach);
@@ -362,6 +365,7 @@ static void free_case(ot2d_case* c) {
     dfree(mb.yx); dfree(mb.yy); dfree(mb.ug); dfree(mb.vg); dfree(mb.turmu);
     dfree(mb.tscale); dfree(mb.bt); dfree(mb.xv); dfree(mb.yv); dfree(mb.ugv);
     dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa); dfree(mb.ta);
+    dfree(mb.xold); dfree(mb.yold); dfree(mb.xole); dfree(mb.yole);
   }
   dfree(c->dMa); dfree(c->dTa); dfree(c->d_istep); dfree(c->d_itnorms);
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
@@ -387,8 +391,9 @@ static int32_t check_params(const ot2d_params* p) {
   // (src/eulerrhs.f:85-97); ilim = -4 reaches neither
   if (!(abs(p->ilim) == 1 || p->ilim == 4))
     return fail(OT2D_ERR_UNSUPPORTED, "ilim=%d not supported (1: Koren MUSCL, 4: WENO5)", p->ilim);
-  if (!p->timespectral && p->iunst != 0)
-    return fail(OT2D_ERR_UNSUPPORTED, "iunst=%d: time-accurate motion is not supported", p->iunst);
+  if (!p->timespectral && p->iunst != 0 && !(p->iunst == 1 && p->motiontype == 1))
+    return fail(OT2D_ERR_UNSUPPORTED, "iunst=%d motiontype=%d: of moving grids, pitching (iunst 1, motiontype 1)",
+                p->iunst, p->motiontype);
   if (p->itnmax < 1 || p->itnmax > OT2D_MAX_ITN)
     return fail(OT2D_ERR_UNSUPPORTED, "itnmax=%d: 1 to %d Newton sub-iterations", p->itnmax, OT2D_MAX_ITN);
   if (p->ntac < 1 || p->ntac > 3) return fail(OT2D_ERR_UNSUPPORTED, "ntac=%d: time accuracy 1, 2 or 3", p->ntac);
@@ -403,12 +408,73 @@ static bool same_structure(const ot2d_params* a, const ot2d_params* b) {
   return a->invisc == b->invisc && a->lamin == b->lamin && a->iturb == b->iturb && a->itrans == b->itrans &&
          a->timespectral == b->timespectral && (a->timespectral ? a->nspec == b->nspec : true) &&
          a->lu_fp32 == b->lu_fp32 && a->half == b->half && a->itnmax == b->itnmax &&
-         (a->itnmax > 1 || a->ntac > 1) == (b->itnmax > 1 || b->ntac > 1);
+         (a->itnmax > 1 || a->ntac > 1) == (b->itnmax > 1 || b->ntac > 1) &&
+         (a->timespectral ? 0 : a->iunst) == (b->timespectral ? 0 : b->iunst);
 }
 
 // ncase parameter sets; mesh im belongs to case mesh_case[im]
 // DevMesh records of mesh im's instances (c->hM), from its buffers and
 // description (c->m[im]) and its case's parameters
+// Cells a boundary segment writes and reads (src/bc.f, src/samodel2d.f
+// turbc): every type writes its halo cells and reads along the inward
+// normal at the same tangential index, the wake cut also at the mirrored
+// index; periodic segments read across the mesh.
+struct CellBox { int j0, j1, k0, k1; };
+static bool boxes_meet(const CellBox& a, const CellBox& b) {
+  return a.j0 <= b.j1 && b.j0 <= a.j1 && a.k0 <= b.k1 && b.k0 <= a.k1;
+}
+static bool bc_cells(const DevBC& b, int jd, std::vector<CellBox>& w, std::vector<CellBox>& r) {
+  const int iadd = b.idir >= 0 ? 1 : -1, iadir = b.idir >= 0 ? b.idir : -b.idir;
+  auto span = [](int a, int c) { return a < c ? std::make_pair(a, c) : std::make_pair(c, a); };
+  if (b.type == 46) {
+    w.push_back({b.js, b.je, b.ks, b.ke});
+    return true;
+  }
+  if (b.type == 22 || (iadir != 1 && iadir != 2)) return false;   // anywhere
+  if (iadir == 2) {
+    const int k = b.idir == 2 ? b.ke : b.ks;
+    const auto kr = span(k + iadd, k + 2 * iadd);
+    w.push_back({b.js, b.je, k, k});
+    r.push_back({b.js, b.je, kr.first, kr.second});
+    if (b.type == 51) {   // the mirrored side of the wake cut
+      w.push_back({jd - b.je + 1, jd - b.js + 1, k, k});
+      r.push_back({jd - b.je + 1, jd - b.js + 1, kr.first, kr.second});
+    }
+    return true;
+  }
+  if (b.type == 51) return false;
+  const int j = b.idir == 1 ? b.je : b.js;
+  const auto jr = span(j + iadd, j + 2 * iadd);
+  w.push_back({j, j, b.ks, b.ke});
+  r.push_back({jr.first, jr.second, b.ks, b.ke});
+  return true;
+}
+
+// phase of every segment: one past the latest earlier segment whose cells
+// it shares (written by either, or written by one and read by the other)
+static int bc_phases(DevBC* bc, int nbc, int jd) {
+  std::vector<std::vector<CellBox>> w(nbc), r(nbc);
+  std::vector<char> local(nbc);
+  int nph = 0;
+  for (int s = 0; s < nbc; ++s) {
+    local[s] = bc_cells(bc[s], jd, w[s], r[s]);
+    int ph = 0;
+    for (int t = 0; t < s; ++t) {
+      bool meet = !local[s] || !local[t];
+      for (const auto& a : w[s])
+        for (const auto& b : w[t]) meet = meet || boxes_meet(a, b);
+      for (const auto& a : w[s])
+        for (const auto& b : r[t]) meet = meet || boxes_meet(a, b);
+      for (const auto& a : r[s])
+        for (const auto& b : w[t]) meet = meet || boxes_meet(a, b);
+      if (meet) ph = bc[t].phase + 1 > ph ? bc[t].phase + 1 : ph;
+    }
+    bc[s].phase = ph;
+    nph = ph + 1 > nph ? ph + 1 : nph;
+  }
+  return nph;
+}
+
 static int32_t mesh_records(ot2d_case* c, int im, const ot2d_params* pc) {
   MeshBuf& mb = c->m[im];
   const ot2d_mesh_desc& md = mb.d;
@@ -427,7 +493,7 @@ static int32_t mesh_records(ot2d_case* c, int im, const ot2d_params* pc) {
     M.nq = c->nq; M.nv = c->nv; M.nspec = ns; M.nsp = nsp;
     M.jtail1 = jtail_v + 1;
     M.jtail2 = p->half == 0 ? mb.jd - M.jtail1 + 1 : mb.jd - 1;
-    M.has_gv = p->timespectral ? 1 : 0;
+    M.has_gv = (p->timespectral || p->iunst == 1) ? 1 : 0;
     {
       double a = 10. / (double)mb.jmax, b = 10. / (double)mb.kmax;
       M.epsj = a * a * a;  // (10./real(jmax))**3, src/eulerrhs.f:51
@@ -440,6 +506,7 @@ static int32_t mesh_records(ot2d_case* c, int im, const ot2d_params* pc) {
         return fail(OT2D_ERR_ARG, "mesh %d boundary segment %d (type %d, dir %d) indexes outside the grid",
                     im, ib + 1, md.bc[ib].ibtyp, md.bc[ib].ibdir);
     }
+    M.nbc_phase = bc_phases(M.bc, md.nbc, mb.jd);
     size_t o = mb.n2 * nsp, ov = mb.nv2 * nsp;
     for (int n = 0; n < c->nq; ++n) M.q[n] = mb.q + mb.n2 * ((size_t)n * ns + nsp);
     for (int n = 0; n < c->nv; ++n) M.s[n] = mb.s + mb.n2 * ((size_t)n * ns + nsp);
@@ -449,6 +516,7 @@ static int32_t mesh_records(ot2d_case* c, int im, const ot2d_params* pc) {
     M.ug = mb.ug + o; M.vg = mb.vg + o; M.turmu = mb.turmu + o; M.tscale = mb.tscale + o; M.bt = mb.bt + o;
     M.xv = mb.xv + ov; M.yv = mb.yv + ov; M.ugv = mb.ugv + ov; M.vgv = mb.vgv + ov;
     M.xv0 = mb.xv; M.yv0 = mb.yv;
+    if (mb.xold) { M.xold = mb.xold + ov; M.yold = mb.yold + ov; M.xole = mb.xole + ov; M.yole = mb.yole + ov; }
     M.iblank = mb.iblank + o;
     M.lu = mb.lu + mb.lu_per * nsp;
     M.lu_rl = c->lu_rl;
@@ -547,7 +615,12 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     return fail(OT2D_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e)); } \
     c->bytes += (size_t)(n) * sizeof(*(ptr)); } while (0)
   c->itnmax = p->itnmax;
-  c->tacc = p->itnmax > 1 || p->ntac > 1;
+  c->moving = !p->timespectral && p->iunst == 1;
+  c->tacc = p->itnmax > 1 || p->ntac > 1 || c->moving;
+  if (c->moving && nmesh > 1 && !(ncase > 1)) {
+    free_case(c);
+    return fail(OT2D_ERR_UNSUPPORTED, "moving overset meshes (connectivity re-cut every step)");
+  }
   A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase); A(c->d_istep, ncase);
   if (c->tacc) A(c->d_itnorms, 6 * (size_t)nmi * c->itnmax);
   A(c->d_cinst, 2 * (size_t)ncase); A(c->d_ctol, ncase);
@@ -632,6 +705,7 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     A(mb.lu, mb.lu_per * ns);
     if (!c->lamin) A(mb.sa, (c->iturb == 2 ? 20 : 10) * mb.n2 * ns);
     if (c->tacc) A(mb.ta, 3 * (size_t)c->nv * mb.n2 * ns);
+    if (c->moving) { A(mb.xold, mb.nv2 * ns); A(mb.yold, mb.nv2 * ns); A(mb.xole, mb.nv2 * ns); A(mb.yole, mb.nv2 * ns); }
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
     mb.cap_n2 = mb.n2; mb.cap_nv2 = mb.nv2; mb.cap_lu = mb.lu_per; mb.cap_tiles = tj * tk;
     mb.pofs = ntile_total;
@@ -683,7 +757,7 @@ static size_t footprint(const ot2d_params* ps, int32_t ncase, int32_t nmesh, con
   const size_t ns = ps->timespectral ? (size_t)ps->nspec : 1;
   int rl = 1;
   for (int im = 0; im < nmesh; ++im) rl = (md[im].kmax - 1 + 63) / 64 > rl ? (md[im].kmax - 1 + 63) / 64 : rl;
-  const bool tacc = ps->itnmax > 1 || ps->ntac > 1;
+  const bool tacc = ps->itnmax > 1 || ps->ntac > 1 || (!ps->timespectral && ps->iunst == 1);
   size_t b = (size_t)ncase * (4 * sizeof(int) + 2 * sizeof(int) + sizeof(double)) +
              (6 * (size_t)nmesh * ns + ns * ncase + ns * ns + 16) * sizeof(double);
   if (tacc) b += 6 * (size_t)nmesh * ns * ps->itnmax * sizeof(double);
@@ -692,6 +766,7 @@ static size_t footprint(const ot2d_params* ps, int32_t ncase, int32_t nmesh, con
     size_t d = n2 * (nq + nv) + 11 * n2 + 4 * nv2;
     if (!lamin) d += (iturb == 2 ? 20 : 10) * n2;
     if (tacc) d += 3 * (size_t)nv * n2;
+    if (!ps->timespectral && ps->iunst == 1) d += 4 * nv2;
     const size_t tiles = ((jd - 2 + RES_TJ - 1) / RES_TJ) * ((kd - 2 + RES_TK - 1) / RES_TK);
     b += ns * (d * sizeof(double) + n2 * sizeof(int) + lu_workspace_bytes((int)jd, (int)kd, rl) +
                tiles * (4 * sizeof(double) + sizeof(ResTile)) + sizeof(DevMesh));
@@ -791,8 +866,16 @@ extern "C" int32_t ot2d_set_grid(ot2d_case* c, int32_t im, const double* xv, con
   if (ic == 0) c->istep0 = 0;
   HIPCHK(hipMemsetAsync(c->d_istep0 + ic, 0, sizeof(int), c->stream));
   HIPCHK(hipMemsetAsync(c->d_status + ic, 0, sizeof(int), c->stream));
-  // a time-accurate run starts over: no earlier time levels
+  // a time-accurate run starts over: no earlier time levels; a moving grid's
+  // earlier vertices are the grid itself (src/init.f:92-103)
   HIPCHK(hipMemsetAsync(c->d_istep + ic, 0, sizeof(int), c->stream));
+  if (mb.xold) {
+    const size_t nb = sizeof(double) * mb.nv2 * c->nspec;
+    HIPCHK(hipMemcpyAsync(mb.xold, mb.xv, nb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(mb.yold, mb.yv, nb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(mb.xole, mb.xv, nb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(mb.yole, mb.yv, nb, hipMemcpyDeviceToDevice, c->stream));
+  }
   if (mb.ta) HIPCHK(hipMemsetAsync(mb.ta, 0, 3 * (size_t)c->nv * mb.cap_n2 * c->nspec * sizeof(double), c->stream));
   full_view(c);   // the case runs again
   return sync(c);
@@ -905,7 +988,7 @@ extern "C" int32_t ot2d_time_step(ot2d_case* c) {
 extern "C" int32_t ot2d_bc(ot2d_case* c) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
   if (int32_t rc = clear_status(c)) return rc;
-  HIPCHK(launch_bc(c->dM, nmi(c), 0, c->stream));
+  HIPCHK(launch_bc(c->dM, c->hM.data(), nmi(c), 0, c->stream));
   return sync(c);
 }
 
@@ -946,7 +1029,7 @@ static void full_view(ot2d_case* c) {
 // the inviscid terms and before the viscous ones (src/step.f:266-277)
 static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks* mk = nullptr, bool newton = false) {
   const StepView v = step_view(c);
-  HIPCHK(launch_bc(v.dM, v.n, zero_halo_s, c->stream));
+  HIPCHK(launch_bc(v.dM, v.hM, v.n, zero_halo_s, c->stream));
   mark(mk, OT2D_ST_BC, c->stream);
   // the norms are taken on the complete residual (src/step.f:46-93)
   const bool later = !c->p.invisc || c->p.timespectral || newton;
@@ -1019,7 +1102,14 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr);
 static int32_t enqueue_time_step(ot2d_case* c, Marks* mk) {
   const StepView v = step_view(c);
   HIPCHK(launch_open_step(c->d_status, c->d_istep0, c->d_stepc, c->d_istep, c->ncase, c->stream));
-  HIPCHK(launch_stqol(v.dM, v.hM, v.n, c->stream));
+  if (c->moving) {
+    // the grid turns, then its metrics and the SA wall distance follow
+    // (src/Turns_overset.f:368-391, src/samodel2d.f:83)
+    HIPCHK(launch_pitch(v.dM, v.hM, v.n, c->stream));
+    HIPCHK(launch_init_geometry(v.dM, v.n, c->stream, 1));
+    if (!c->lamin) HIPCHK(launch_sa_dist(v.dM, v.hM, v.n, c->stream));
+  }
+  if (c->itnmax > 1) HIPCHK(launch_stqol(v.dM, v.hM, v.n, c->stream));
   for (int itn = 1; itn <= c->itnmax; ++itn) {
     HIPCHK(launch_time_step(v.dM, v.hM, v.n, c->stream, nullptr, nullptr, nullptr, 0, itn));
     mark(mk, OT2D_ST_TIME_STEP, c->stream);

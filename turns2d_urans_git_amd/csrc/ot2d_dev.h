@@ -23,6 +23,10 @@ namespace ot2d {
 // (src/bc.f:21-35): js..je, ks..ke are 1-based cell indices incl. halo.
 struct DevBC {
   int type, idir, js, je, ks, ke;
+  // segments of one phase touch disjoint cells and run side by side; the
+  // phases run in order, so segments that share cells keep the reference's
+  // segment order
+  int phase;
 };
 
 // Case constants derived as read_inputs does (src/setup.f:1162-1248) plus
@@ -39,6 +43,7 @@ struct DevConst {
   int invisc, iread, ilim, iconstant_cfl, idual, iprecon, timespectral, ntac;
   int iturb, nmv, motiontype;
   int itnmax;                // Newton sub-iterations per time step
+  int iunst;                 // 1: the grid pitches every time step (motiontype 1)
   double dtps[OT2D_MAX_ITN]; // dtpseudo(itn), the dual time of sub-iteration itn
 };
 
@@ -46,6 +51,7 @@ struct DevMesh {
   int jd, kd, jmax, kmax;
   int js, je, ks, ke;        // jbeg, jend, kbeg, kend (1-based)
   int nq, nv, nbc, nspec, nsp;
+  int nbc_phase;             // phases of the boundary segments
   int jtail1, jtail2;
   int has_gv;                // grid velocities present (time-spectral)
   int res_block0, res_nblocks;  // this instance's slice of the norm partials
@@ -56,6 +62,7 @@ struct DevMesh {
   double *x, *y, *xx, *xy, *yx, *yy, *ug, *vg, *turmu, *tscale, *bt;
   double *xv, *yv, *ugv, *vgv;  // vertex arrays (jmax,kmax) of this instance
   const double *xv0, *yv0;      // instance-1 vertices (time-spectral rotation)
+  double *xold, *yold, *xole, *yole;  // vertices one and two time steps back (moving grids), or null
   int* iblank;
   int bodyflag;              // mesh has a wall segment (type 5)
   // SA workspace (jd,kd) planes: laminar viscosity, u, v, wall distance and
@@ -291,7 +298,11 @@ inline void mark(Marks* m, int s, hipStream_t st) {
   }
 }
 
-hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st);
+// metrics_only: metfv alone on moved vertices (no cell centres, no
+// free-stream state), as the time loop of a moving grid does
+hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st, int metrics_only = 0);
+// pitch (src/motion.f:546-632): the grid turns by this time step's angle
+hipError_t launch_pitch(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 // with ncase > 0 also opens the step of every running case (istep0 and the
 // step counter)
 hipError_t launch_time_step(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int* istep0 = nullptr,
@@ -301,7 +312,7 @@ hipError_t launch_open_step(const int* status, int* istep0, int* stepc, int* ist
 hipError_t launch_stqol(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_newton(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_copy_doubles(double* dst, const double* src, int n, hipStream_t st);
-hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st);
+hipError_t launch_bc(const DevMesh* dM, const DevMesh* hM, int nmi, int zero_s, hipStream_t st);
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, int weno,
                            hipStream_t st);
 hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st);

@@ -28,6 +28,24 @@ namespace ot2d {
 __device__ __forceinline__ double sq(double a) { return a * a; }
 __device__ __forceinline__ double cube(double a) { return a * a * a; }
 __device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+// face speed of a moving grid from the area the face sweeps in one time step
+// (src/eulerrhs.f:118-136, 225-243): face between vertices p and m, sgn -1
+// for xi faces and +1 for eta faces; second order with the earlier step
+// when ntac = 2
+__device__ inline double swept_tj(const DevMesh& M, long long p, long long m, double sgn) {
+  const double dt = M.c.h;
+  double dx1 = M.xv[p] - M.xold[m], dy1 = M.yv[p] - M.yold[m];
+  double dx2 = M.xold[p] - M.xv[m], dy2 = M.yold[p] - M.yv[m];
+  double tj = sgn * 0.5 * (dx1 * dy2 - dx2 * dy1) / dt;
+  if (M.c.ntac == 2 && *M.istep > 1) {
+    dx1 = M.xold[p] - M.xole[m]; dy1 = M.yold[p] - M.yole[m];
+    dx2 = M.xole[p] - M.xold[m]; dy2 = M.yole[p] - M.yold[m];
+    const double temp = sgn * 0.5 * (dx1 * dy2 - dx2 * dy1) / dt;
+    tj = 1.5 * tj - 0.5 * temp;
+  }
+  return tj;
+}
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
@@ -448,15 +466,17 @@ __device__ void zero_halo_s(const DevMesh& M) {
   }
 }
 
-__global__ __launch_bounds__(BC_THREADS) void k_bc(const DevMesh* __restrict__ Ms, int zero_s) {
-  const DevMesh& M = Ms[blockIdx.x];
+// bc (src/bc.f:2-85): the segments of one phase of every instance, one
+// workgroup per segment (segments of a phase touch disjoint cells; phases
+// run in the reference's segment order); the first workgroup of phase 0
+// also clears s on the halo ring
+__global__ __launch_bounds__(BC_THREADS) void k_bc(const DevMesh* __restrict__ Ms, int zero_s, int phase) {
+  const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
-  int istep0 = *M.istep0;
-  if (zero_s) zero_halo_s(M);
-  for (int ib = 0; ib < M.nbc; ++ib) {
-    bc_segment(M, M.bc[ib], istep0);
-    __syncthreads();
-  }
+  const int ib = blockIdx.x;
+  if (zero_s && phase == 0 && ib == 0) zero_halo_s(M);
+  if (ib >= M.nbc || M.bc[ib].phase != phase) return;
+  bc_segment(M, M.bc[ib], *M.istep0);
 }
 
 // =================================================================== residual
@@ -755,7 +775,9 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
     double xa = syv[r + 1][cc] - syv[r][cc];
     double ya = -(sxv[r + 1][cc] - sxv[r][cc]);
     double tj = 0.0;
-    if (M.has_gv) {
+    if (C.iunst > 0) {
+      tj = swept_tj(M, vert(M, fi, k), vert(M, fi, k - 1), -1.);
+    } else if (M.has_gv) {
       long long a = vert(M, fi, k), b = vert(M, fi, k - 1);
       tj = -0.5 * (M.ugv[a] + M.ugv[b]) * xa - 0.5 * (M.vgv[a] + M.vgv[b]) * ya;
     }
@@ -777,7 +799,9 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
     double xa = -(syv[r][cc + 1] - syv[r][cc]);
     double ya = (sxv[r][cc + 1] - sxv[r][cc]);
     double tj = 0.0;
-    if (M.has_gv) {
+    if (C.iunst > 0) {
+      tj = swept_tj(M, vert(M, j, fk), vert(M, j - 1, fk), 1.);
+    } else if (M.has_gv) {
       long long a = vert(M, j, fk), b = vert(M, j - 1, fk);
       tj = -0.5 * (M.ugv[a] + M.ugv[b]) * xa - 0.5 * (M.vgv[a] + M.vgv[b]) * ya;
     }
@@ -2214,7 +2238,7 @@ __global__ __launch_bounds__(256) void k_monitor(const DevMesh* __restrict__ Ms)
 // find_cellcenter (setup.f:111-345), metfv (setup.f:440-724),
 // find_timemetrics_TS (setup.f:727-786), qzero (init.f:209-277), qdivj
 // (init.f:180-206).  One block per instance, phases separated by barriers.
-__global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict__ Ms) {
+__global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict__ Ms, int metrics_only) {
   const DevMesh& M = Ms[blockIdx.x];
   const DevConst& C = M.c;
   const int t = threadIdx.x, nt = blockDim.x;
@@ -2225,6 +2249,7 @@ __global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict
   double* xv = M.xv;
   double* yv = M.yv;
   const double pi = 4.0 * atan(1.0);
+  if (metrics_only) goto metrics;
   // find_coord_TS for instances 2..nspec (src/setup.f:348-393)
   if (M.nsp > 0 && C.timespectral) {
     double tspec = *M.tspec;
@@ -2320,6 +2345,7 @@ __global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict
     __syncthreads();
   }
   // ---- metfv
+metrics:
   double* qj = M.q[M.nq - 1];
   for (long long w = t; w < (long long)(jmax - 1) * (kmax - 1); w += nt) {
     int j = 2 + (int)(w % (jmax - 1)), k = 2 + (int)(w / (jmax - 1));
@@ -2379,6 +2405,7 @@ __global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict
     __syncthreads();
   }
 #undef CPY
+  if (metrics_only) return;
   // ---- find_timemetrics_TS (src/setup.f:727-786)
   if (C.timespectral) {
     double tspec = *M.tspec, rotf = C.rotf, am = C.angmax;
@@ -2421,6 +2448,50 @@ __global__ __launch_bounds__(256) void k_init_geometry(const DevMesh* __restrict
 #undef YV
 #undef XC
 #undef YC
+}
+
+// ============================================================ pitching grid
+// pitch (src/motion.f:546-632) at time step istep of the run: the cell
+// centres and vertices turn about (xac, yac) by this step's angle, the grid
+// velocity of every cell follows from the turn (first order, or second order
+// with the previous turn when ntac > 1), and the vertex levels shift (xole
+// <- xold <- xv).  The metrics follow from the turned vertices (metfv).
+__global__ __launch_bounds__(256) void k_pitch(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const DevConst& C = M.c;
+  const double pi = 4.0 * atan(1.0);
+  const double dt = C.h;
+  const int istep = *M.istep;
+  const double dtheta = -C.angmax * (cos(C.rotf * istep * dt) - cos(C.rotf * (istep - 1) * dt));
+  const double sn = sin(dtheta * pi / 180.), cn = cos(dtheta * pi / 180.);
+  const double snold = sin(-1. * dtheta * pi / 180.), cnold = cos(-1. * dtheta * pi / 180.);
+  const long long n2 = (long long)M.jd * M.kd, nv2 = (long long)M.jmax * M.kmax;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
+    const double x = M.x[i], y = M.y[i];
+    const double xnew = (x - C.xac) * cn + (y - C.yac) * sn + C.xac;
+    const double ynew = -(x - C.xac) * sn + (y - C.yac) * cn + C.yac;
+    if (C.ntac == 1) {
+      M.ug[i] = (xnew - x) / dt;
+      M.vg[i] = (ynew - y) / dt;
+    } else {
+      const double xo = (x - C.xac) * cnold + (y - C.yac) * snold + C.xac;
+      const double yo = -(x - C.xac) * snold + (y - C.yac) * cnold + C.yac;
+      M.ug[i] = (1.5 * xnew - 2. * x + 0.5 * xo) / dt;
+      M.vg[i] = (1.5 * ynew - 2. * y + 0.5 * yo) / dt;
+    }
+    M.x[i] = xnew;
+    M.y[i] = ynew;
+  }
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < nv2; i += 256ll * gridDim.x) {
+    M.xole[i] = M.xold[i];
+    M.yole[i] = M.yold[i];
+    const double xo = M.xv[i], yo = M.yv[i];
+    M.xold[i] = xo;
+    M.yold[i] = yo;
+    M.xv[i] = (xo - C.xac) * cn + (yo - C.yac) * sn + C.xac;
+    M.yv[i] = -(xo - C.xac) * sn + (yo - C.yac) * cn + C.yac;
+  }
 }
 
 // ================================================================== forces
@@ -2527,8 +2598,15 @@ static inline int nblk(long long n, int per) {
   return (int)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
 }
 
-hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st) {
-  hipLaunchKernelGGL(k_init_geometry, dim3(nmi), dim3(256), 0, st, dM);
+hipError_t launch_init_geometry(const DevMesh* dM, int nmi, hipStream_t st, int metrics_only) {
+  hipLaunchKernelGGL(k_init_geometry, dim3(nmi), dim3(256), 0, st, dM, metrics_only);
+  return hipGetLastError();
+}
+
+hipError_t launch_pitch(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  hipLaunchKernelGGL(k_pitch, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
   return hipGetLastError();
 }
 
@@ -2565,9 +2643,18 @@ hipError_t launch_copy_doubles(double* dst, const double* src, int n, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_bc(const DevMesh* dM, int nmi, int zero_s, hipStream_t st) {
-  hipLaunchKernelGGL(k_bc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, zero_s);
-  return hipGetLastError();
+hipError_t launch_bc(const DevMesh* dM, const DevMesh* hM, int nmi, int zero_s, hipStream_t st) {
+  int nbc = 1, nph = 1;
+  for (int i = 0; i < nmi; ++i) {
+    nbc = hM[i].nbc > nbc ? hM[i].nbc : nbc;
+    nph = hM[i].nbc_phase > nph ? hM[i].nbc_phase : nph;
+  }
+  for (int ph = 0; ph < nph; ++ph) {
+    hipLaunchKernelGGL(k_bc, dim3(nbc, nmi), dim3(BC_THREADS), 0, st, dM, zero_s, ph);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, int weno,

@@ -5,7 +5,9 @@
 //                                                    cross terms once into LDS, then each
 //                                                    cell's thin-layer fluxes from there
 //   vmu_sa (src/samodel2d.f:2-197)     k_sa_prim     laminar viscosity, u, v (all cells)
-//                                      k_turbc       SA boundary conditions (1 block / instance)
+//                                      k_turbc       SA boundary conditions (1 block / segment,
+//                                                    one launch per phase of the segments)
+//                                      k_turm_ring   eddy viscosity of the halo ring
 //                                      k_sa_rhs      convection, diffusion, source and the
 //                                                    two scalar tridiagonal systems, in
 //                                                    gather form (one thread per cell)
@@ -409,17 +411,13 @@ __device__ __forceinline__ void c_turm_cell(const DevMesh& M, long long i) {
   M.turmu[i] = c_turm_val(M.q[0][i], M.q[M.nq - 1][i], M.sa_vmul[i], M.q[4][i]);
 }
 
-// turbc (src/samodel2d.f:827-1303) for one instance; with eddy=1 the eddy
-// viscosity of the halo ring follows (c_turm after the final turbc)
-__global__ __launch_bounds__(BC_THREADS) void k_turbc(const DevMesh* __restrict__ Ms, int eddy) {
-  const DevMesh& M = Ms[blockIdx.x];
-  if (*M.status) return;
+// turbc (src/samodel2d.f:827-1303) for one boundary segment
+__device__ void turbc_segment(const DevMesh& M, const DevBC& b) {
   double* q5 = M.q[4];
   const double* u = M.sa_u;
   const double* v = M.sa_v;
   const int t = threadIdx.x, nt = blockDim.x;
-  for (int ib = 0; ib < M.nbc; ++ib) {
-    const DevBC& b = M.bc[ib];
+  {
     int js = b.js, je = b.je, ks = b.ks, ke = b.ke, idir = b.idir;
     int iadd = idir >= 0 ? 1 : -1, iadir = idir >= 0 ? idir : -idir;
     int ty = b.type;
@@ -498,12 +496,27 @@ __global__ __launch_bounds__(BC_THREADS) void k_turbc(const DevMesh* __restrict_
         }
       }
     }
-    __syncthreads();
   }
-  if (!eddy) return;
+}
+
+// the segments of one phase of every instance, one workgroup per segment
+// (segments of a phase touch disjoint cells; phases run in the reference's
+// segment order)
+__global__ __launch_bounds__(BC_THREADS) void k_turbc(const DevMesh* __restrict__ Ms, int phase) {
+  const DevMesh& M = Ms[blockIdx.y];
+  if (*M.status) return;
+  const int ib = blockIdx.x;
+  if (ib >= M.nbc || M.bc[ib].phase != phase) return;
+  turbc_segment(M, M.bc[ib]);
+}
+
+// the eddy viscosity of the halo ring after the final turbc (c_turm)
+__global__ __launch_bounds__(BC_THREADS) void k_turm_ring(const DevMesh* __restrict__ Ms) {
+  const DevMesh& M = Ms[blockIdx.x];
+  if (*M.status) return;
   const int jd = M.jd, kd = M.kd;
-  int nring = 2 * jd + 2 * (kd - 2);
-  for (int w = t; w < nring; w += nt) {
+  const int nring = 2 * jd + 2 * (kd - 2);
+  for (int w = threadIdx.x; w < nring; w += blockDim.x) {
     int j, k;
     if (w < jd) { j = w + 1; k = 1; }
     else if (w < 2 * jd) { j = w - jd + 1; k = kd; }
@@ -1294,6 +1307,21 @@ hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
 }
 
 // vmu_sa for every instance (src/samodel2d.f:2-197, itrans = 0)
+// turbc over every instance: one launch per phase of the segments
+static hipError_t launch_turbc(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+  int nbc = 1, nph = 1;
+  for (int i = 0; i < nmi; ++i) {
+    nbc = hM[i].nbc > nbc ? hM[i].nbc : nbc;
+    nph = hM[i].nbc_phase > nph ? hM[i].nbc_phase : nph;
+  }
+  for (int ph = 0; ph < nph; ++ph) {
+    hipLaunchKernelGGL(k_turbc, dim3(nbc, nmi), dim3(BC_THREADS), 0, st, dM, ph);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane) {
   long long mx = max_cells(hM, nmi);
   int mj = 1, mk_ = 1;
@@ -1305,8 +1333,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_PRIM, st);
-  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, 0);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_turbc(dM, hM, nmi, st)) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_BC, st);
   hipLaunchKernelGGL(k_sa_rhs, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1331,7 +1358,8 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
-  hipLaunchKernelGGL(k_turbc, dim3(nmi), dim3(BC_THREADS), 0, st, dM, 1);
+  if ((e = launch_turbc(dM, hM, nmi, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_turm_ring, dim3(nmi), dim3(BC_THREADS), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_EDDY, st);
   return hipSuccess;

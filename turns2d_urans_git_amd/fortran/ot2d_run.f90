@@ -106,7 +106,9 @@ program ot2d_run
   else
     nspec = 1
   end if
-  if (iunst /= 0) stop 'ot2d_run: moving grids (iunst > 0) are not supported by this driver'
+  if (iunst /= 0 .and. .not. (iunst == 1 .and. motiontype == 1)) &
+       stop 'ot2d_run: of moving grids, only pitching (iunst = 1, motiontype = 1) is supported'
+  if (iunst /= 0 .and. num_grids > 1) stop 'ot2d_run: moving overset meshes are not supported'
   if (invisc) lamin = .true.
   if (lamin) iturb = -1
   nmesh = num_grids
@@ -214,6 +216,7 @@ program ot2d_run
   end do
   nrc2 = nsteps - istep0
   istep = 0
+  totime = 0.d0
   do while (istep < nrc2)
     ! next step whose end needs the host: a norm line (istep0), a force
     ! line (istep), a restart write (istep0 = 2 or a multiple of nrest)
@@ -225,6 +228,9 @@ program ot2d_run
     end do
     if (nxt - istep > 1) then
       rc = ot2d_iterate(case, int(nxt - istep - 1, c_int32_t), c_null_ptr, done)
+      do i = 1, done
+        call advance_time(istep + i)
+      end do
       istep = istep + done; istep0 = istep0 + done
       call ot2d_check(rc, 'ot2d_iterate')
       if (rc /= OT2D_OK) then
@@ -235,6 +241,7 @@ program ot2d_run
     rc = ot2d_step(case, norms, argmax)
     call ot2d_check(rc, 'ot2d_step')
     istep = istep + 1; istep0 = istep0 + 1
+    call advance_time(istep)
     if (mod(istep0, npnorm) == 0) then
       ! one line per Newton sub-iteration of the step (src/step.f:52-101)
       if (itnmax > 1) then
@@ -272,6 +279,23 @@ program ot2d_run
 
 contains
 
+  ! the cell centres of mesh m as they are now (a moving grid turns them)
+  subroutine cell_centres(m)
+    integer, intent(in) :: m
+    call ot2d_check(ot2d_get_field(case, int(m - 1, c_int32_t), OT2D_F_X, ms(m)%x), 'ot2d_get_field')
+    call ot2d_check(ot2d_get_field(case, int(m - 1, c_int32_t), OT2D_F_Y, ms(m)%y), 'ot2d_get_field')
+  end subroutine cell_centres
+
+  ! the time and pitch angle of a moving grid at time step n of the run
+  ! (src/Turns_overset.f:362-379)
+  subroutine advance_time(n)
+    integer, intent(in) :: n
+    if (iunst /= 0) then
+      totime = totime + dt
+      theta_col = alfa + angmax * (1.d0 - cos(md(1)%irot_dir * rf * n * dt))
+    end if
+  end subroutine advance_time
+
   ! negative speed of sound ends the loop (src/Turns_overset.f:497); an
   ! out-of-bounds norm stops the run (src/step.f:104-110)
   subroutine stopped(st)
@@ -289,11 +313,16 @@ contains
   subroutine forces()
     integer :: n
     call ot2d_check(ot2d_get_state(case, 0_c_int32_t, ms(1)%q), 'ot2d_get_state')
+    if (iunst /= 0) call cell_centres(1)
     open(11, file='fort.11', position='append', form='formatted')
     do n = 1, ns
       call ot2d_check(ot2d_forces(case, 0_c_int32_t, int(n - 1, c_int32_t), f), 'ot2d_forces')
       if (.not. timespectral) then
-        write(11, '(i6,4(x,e18.10))') istep0, f(1), f(2), f(3), f(4)
+        if (iunst == 0) then
+          write(11, '(i6,4(x,e18.10))') istep0, f(1), f(2), f(3), f(4)
+        else
+          write(11, '(i6,5(x,e18.10))') istep0, f(1), f(2), f(3), f(4), theta_col
+        end if
       else
         write(11, '(2i6,4(x,e18.10))') istep0, n, f(1), f(2), f(3), f(4)
       end if
@@ -387,6 +416,7 @@ contains
         jj = m%jd; kk = m%kd
         call ot2d_check(ot2d_get_state(case, int(im - 1, c_int32_t), m%q), 'ot2d_get_state')
         call ot2d_check(ot2d_get_iblank(case, int(im - 1, c_int32_t), m%ib), 'ot2d_get_iblank')
+        if (iunst /= 0) call cell_centres(im)
         do n = 1, ns
           do k = 1, kk
             do j = 1, jj
