@@ -1027,7 +1027,9 @@ static void full_view(ot2d_case* c) {
 
 // newton: the time term of a Newton sub-iteration joins the residual after
 // the inviscid terms and before the viscous ones (src/step.f:266-277)
-static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks* mk = nullptr, bool newton = false) {
+// rbc: the residual boundary conditions join the norms' last launch
+static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks* mk = nullptr, bool newton = false,
+                                int rbc = 0) {
   const StepView v = step_view(c);
   HIPCHK(launch_bc(v.dM, v.hM, v.n, zero_halo_s, c->stream));
   mark(mk, OT2D_ST_BC, c->stream);
@@ -1049,7 +1051,7 @@ static int32_t enqueue_residual(ot2d_case* c, int zero_halo_s, int norms, Marks*
   }
   if (norms && later && !vis_norms) HIPCHK(launch_norms(v.dM, v.dT, v.nt, c->stream));
   if (norms) {
-    HIPCHK(launch_norms_final(v.dM, v.n, c->stream));
+    HIPCHK(launch_norms_final(v.dM, v.n, c->stream, rbc));
     if (c->conv_on) HIPCHK(launch_conv_check(c->dM, c->d_cinst, c->d_ctol, c->d_status, c->ncase, c->stream));
     mark(mk, OT2D_ST_NORMS, c->stream);
   }
@@ -1113,15 +1115,15 @@ static int32_t enqueue_time_step(ot2d_case* c, Marks* mk) {
   for (int itn = 1; itn <= c->itnmax; ++itn) {
     HIPCHK(launch_time_step(v.dM, v.hM, v.n, c->stream, nullptr, nullptr, nullptr, 0, itn));
     mark(mk, OT2D_ST_TIME_STEP, c->stream);
-    if (int32_t rc = enqueue_residual(c, 1, 1, mk, c->itnmax > 1)) return rc;
-    HIPCHK(launch_rbc(v.dM, v.n, c->stream));
+    if (int32_t rc = enqueue_residual(c, 1, 1, mk, c->itnmax > 1, 1)) return rc;
     mark(mk, OT2D_ST_RBC, c->stream);
     if (c->lamin) {
       HIPCHK(launch_lu(v.dM, v.hM, v.n, 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
     } else {
-      HIPCHK(launch_lu(v.dM, v.hM, v.n, 0, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
+      const int sa = c->iturb == 1 ? 2 : 0;
+      HIPCHK(launch_lu(v.dM, v.hM, v.n, sa, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
       if (c->iturb == 2) HIPCHK(launch_sst(v.dM, v.hM, v.n, mk, c->stream));
-      else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane));
+      else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane, 1));
       HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream));
       mark(mk, OT2D_ST_UPDATE, c->stream);
     }
@@ -1140,16 +1142,17 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk) {
   // the step opens (istep0, step counters) inside the time-step kernel
   HIPCHK(launch_time_step(v.dM, v.hM, v.n, c->stream, c->d_istep0, c->d_status, c->d_stepc, c->ncase));
   mark(mk, OT2D_ST_TIME_STEP, c->stream);
-  if (int32_t rc = enqueue_residual(c, 1, 1, mk)) return rc;
-  HIPCHK(launch_rbc(v.dM, v.n, c->stream));
+  if (int32_t rc = enqueue_residual(c, 1, 1, mk, false, 1)) return rc;
   mark(mk, OT2D_ST_RBC, c->stream);
   if (c->lamin) {
     HIPCHK(launch_lu(v.dM, v.hM, v.n, 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
   } else {
     // vmu_sa reads the state ilu2d leaves (before the q update), src/step.f:189-211
-    HIPCHK(launch_lu(v.dM, v.hM, v.n, 0, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
+    // SA: the laminar viscosity and velocities come out of k_lu_finish
+    const int sa = c->iturb == 1 ? 2 : 0;
+    HIPCHK(launch_lu(v.dM, v.hM, v.n, sa, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
     if (c->iturb == 2) HIPCHK(launch_sst(v.dM, v.hM, v.n, mk, c->stream));
-    else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane));
+    else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane, 1));
     // the q update and monitor in one pass (stage "update")
     HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream));
     mark(mk, OT2D_ST_UPDATE, c->stream);

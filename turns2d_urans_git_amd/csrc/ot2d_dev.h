@@ -233,6 +233,19 @@ __device__ inline double time_oat(const DevMesh& M) {
   return oat;
 }
 
+// laminar viscosity (lamvis, src/samodel2d.f:200-238) and the velocities
+// vmu_sa keeps (src/samodel2d.f:61-72) of one cell from its state
+__device__ inline void sa_prim_cell(const DevMesh& M, long long i, double q1, double q2, double q3, double q4) {
+  const double c2b = 198.6 / M.c.tinf, c2bp = c2b + 1.;
+  double ra = 1. / q1;
+  double uvel = q2 * ra, vvel = q3 * ra;
+  double ei = q4 * ra - 0.5 * (uvel * uvel + vvel * vvel);
+  double tt = M.c.ggm1 * ei;
+  M.sa_vmul[i] = c2bp * tt * sqrt(tt) / (c2b + tt);
+  M.sa_u[i] = q2 / q1;
+  M.sa_v[i] = q3 / q1;
+}
+
 __host__ __device__ inline long long cell(const DevMesh& M, int j, int k) {
   return (long long)(j - 1) + (long long)(k - 1) * M.jd;
 }
@@ -316,7 +329,8 @@ hipError_t launch_bc(const DevMesh* dM, const DevMesh* hM, int nmi, int zero_s, 
 hipError_t launch_residual(const DevMesh* dM, const ResTile* tiles, int ntiles, int do_norms, int weno,
                            hipStream_t st);
 hipError_t launch_norms(const DevMesh* dM, const ResTile* tiles, int ntiles, hipStream_t st);
-hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st);
+// rbc: the residual boundary conditions follow in the same launch
+hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st, int rbc = 0);
 hipError_t launch_conv_check(const DevMesh* dM, const int* inst, const double* tol, int* status, int ncase,
                              hipStream_t st);
 hipError_t launch_zero_s(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
@@ -327,6 +341,9 @@ struct LaunchCfg {
   int sweep = 0;     // 0: 12 ring steps, 3 producers; 1: 18, 3; 2: 12, 4 (with deeper prefetch)
   int sa_lane = -1;  // SA line solves: -1 by the number of lines, 0 one wave per line, 1 one lane per line
 };
+// fuse_update: 1 the q update and monitor in k_lu_finish (Euler), | 2
+// k_lu_finish also leaves the SA model's laminar viscosity and velocities
+// (what k_sa_prim computes) from the state it writes
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
                      hipStream_t st, int sweep = 0);
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
@@ -335,7 +352,10 @@ hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st)
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd, int rl);
 hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int norms = 0);
-hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane = -1);
+// prim_done: the laminar viscosity and velocities are already in place
+// (k_lu_finish with flag 2)
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane = -1,
+                     int prim_done = 0);
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sst(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);

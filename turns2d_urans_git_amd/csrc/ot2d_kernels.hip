@@ -832,8 +832,13 @@ __global__ __launch_bounds__(RES_THREADS) void k_residual(const DevMesh* __restr
 // second stage, one block per instance: the tile partials in a fixed order
 // (strided per-thread sums, then an LDS tree), the norms, the history slot
 // and the "norm" status (src/step.f:104-110)
-__global__ __launch_bounds__(RES_THREADS) void k_norms_final(const DevMesh* __restrict__ Ms) {
+__device__ void rbc_segment(const DevMesh& M, const DevBC& b);
+
+// with rbc = 1 the residual boundary conditions of the instance follow in
+// the same workgroup (rbc, src/step.f:133-136), unless the norm stopped it
+__global__ __launch_bounds__(RES_THREADS) void k_norms_final(const DevMesh* __restrict__ Ms, int rbc) {
   __shared__ double red[4][RES_THREADS];
+  __shared__ int stop_s;
   const DevMesh& M = Ms[blockIdx.x];
   if (*M.status) return;
   const int tid = threadIdx.x;
@@ -880,6 +885,14 @@ __global__ __launch_bounds__(RES_THREADS) void k_norms_final(const DevMesh* __re
       if (sc >= 1 && sc <= M.hist_cap) M.hist[(long long)(sc - 1) * M.hist_stride + M.hist_slot] = rsum;
     }
     if (rsum > 1000.0) atomicCAS(M.status, 0, 2);
+    stop_s = rsum > 1000.0;
+  }
+  if (!rbc) return;
+  __syncthreads();
+  if (stop_s) return;
+  for (int ib = 0; ib < M.nbc; ++ib) {
+    rbc_segment(M, M.bc[ib]);
+    __syncthreads();
   }
 }
 
@@ -2149,9 +2162,13 @@ __device__ __forceinline__ bool asq_bad(const DevConst& C, double q1, double q2,
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ Ms, int fuse_update) {
+// flags: 1 the q update and monitor follow here (Euler); 2 the SA model's
+// laminar viscosity and velocities of every cell follow from the state
+// written here (k_sa_prim's values)
+__global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ Ms, int flags) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
+  const int fuse_update = flags & 1, sa_prim = flags & 2;
   const int jd = M.jd, kd = M.kd, js = M.js, je = M.je, ks = M.ks, ke = M.ke;
   const LUGeom g = lu_geom(M);
   const LUArr B = lu_arr(M, g.off_b, g.nrec);
@@ -2195,6 +2212,7 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
     }
     M.q[0][i] = q1; M.q[1][i] = q2; M.q[2][i] = q3; M.q[3][i] = q4;
     if (fuse_update) bad |= asq_bad(M.c, q1, q2, q3, q4);
+    if (sa_prim) sa_prim_cell(M, i, q1, q2, q3, q4);
   }
   if (bad) atomicCAS(M.status, 0, 1);
 }
@@ -2697,8 +2715,8 @@ hipError_t launch_conv_check(const DevMesh* dM, const int* inst, const double* t
   return hipGetLastError();
 }
 
-hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st) {
-  hipLaunchKernelGGL(k_norms_final, dim3(nmi), dim3(RES_THREADS), 0, st, dM);
+hipError_t launch_norms_final(const DevMesh* dM, int nmi, hipStream_t st, int rbc) {
+  hipLaunchKernelGGL(k_norms_final, dim3(nmi), dim3(RES_THREADS), 0, st, dM, rbc);
   return hipGetLastError();
 }
 

@@ -376,18 +376,9 @@ __device__ __forceinline__ double ip6(double x) { return ((((x * x) * x) * x) * 
 __global__ __launch_bounds__(256) void k_sa_prim(const DevMesh* __restrict__ Ms) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
-  const double c2b = 198.6 / M.c.tinf, c2bp = c2b + 1.;
   long long n2 = (long long)M.jd * M.kd;
-  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
-    double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
-    double ra = 1. / q1;
-    double uvel = q2 * ra, vvel = q3 * ra;
-    double ei = q4 * ra - 0.5 * (sqd(uvel) + sqd(vvel));
-    double tt = M.c.ggm1 * ei;
-    M.sa_vmul[i] = c2bp * tt * sqrt(tt) / (c2b + tt);
-    M.sa_u[i] = q2 / q1;
-    M.sa_v[i] = q3 / q1;
-  }
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x)
+    sa_prim_cell(M, i, M.q[0][i], M.q[1][i], M.q[2][i], M.q[3][i]);
 }
 
 __device__ __forceinline__ int sa_periodic_src(int idir, int js, int je, int ks, int ke, int jd, int kd, int jk) {
@@ -1322,16 +1313,19 @@ static hipError_t launch_turbc(const DevMesh* dM, const DevMesh* hM, int nmi, hi
   return hipSuccess;
 }
 
-hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane) {
+hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane,
+                     int prim_done) {
   long long mx = max_cells(hM, nmi);
   int mj = 1, mk_ = 1;
   for (int i = 0; i < nmi; ++i) {
     mj = hM[i].je - hM[i].js + 1 > mj ? hM[i].je - hM[i].js + 1 : mj;
     mk_ = hM[i].ke - hM[i].ks + 1 > mk_ ? hM[i].ke - hM[i].ks + 1 : mk_;
   }
-  hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
+  if (!prim_done) {
+    hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   mark(mk, OT2D_ST_SA_PRIM, st);
   if ((e = launch_turbc(dM, hM, nmi, st)) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_BC, st);
