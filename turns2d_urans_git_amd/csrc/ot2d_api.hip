@@ -53,6 +53,7 @@ struct MeshBuf {
   double* lu = nullptr;
   double* sa = nullptr;  // 10 (jd,kd) planes per instance when SA is on
   double* ta = nullptr;  // time-accurate levels qtn, qtnm1, qnewt: 3 nv (jd,kd) planes per instance
+  double* turmu_n = nullptr;  // SA beside ilu2d: the new eddy viscosity
   double *xold = nullptr, *yold = nullptr, *xole = nullptr, *yole = nullptr;  // moving grids: earlier vertices
   size_t lu_per = 0;  // doubles of LU workspace per instance
   bool has_grid = false;
@@ -89,6 +90,10 @@ struct ot2d_case {
   int* d_istep = nullptr;      // time steps of this run per case (time-accurate steps)
   int tacc = 0;                // time-accurate steps: stored levels, Newton sub-iterations, motion
   int moving = 0;              // the grid pitches every time step (iunst = 1)
+  // SA cases: the turbulence model runs on a second stream beside ilu2d's
+  // prep and sweeps (fork and join events), joining before k_lu_finish
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int itnmax = 1;
   double* d_itnorms = nullptr; // norms of every sub-iteration of the last step (itnmax x 6 per instance)
   int* d_cinst = nullptr;      // first, one-past-last instance record of every case
@@ -132,6 +137,12 @@ struct ot2d_ihc_lists {
   ot2d::ihc::Result r;
   std::vector<int> jmax, kmax;
 };
+
+// The SA model beside ilu2d on a second stream (enqueue_turbulent_implicit):
+// switched off.  Its first run of the serving tests on the device ended in
+// an illegal memory access whose cause is not found; without the second
+// stream the step runs the model after ilu2d, as before.
+constexpr bool kSaBesideLu = false;
 
 // ------------------------------------------------ order of FP32 LU-SGS work
 // Cases whose LU-SGS sweeps run in FP32 (lu_fp32) never run at the same time
@@ -365,13 +376,16 @@ static void free_case(ot2d_case* c) {
     dfree(mb.yx); dfree(mb.yy); dfree(mb.ug); dfree(mb.vg); dfree(mb.turmu);
     dfree(mb.tscale); dfree(mb.bt); dfree(mb.xv); dfree(mb.yv); dfree(mb.ugv);
     dfree(mb.vgv); dfree(mb.iblank); dfree(mb.lu); dfree(mb.sa); dfree(mb.ta);
-    dfree(mb.xold); dfree(mb.yold); dfree(mb.xole); dfree(mb.yole);
+    dfree(mb.xold); dfree(mb.yold); dfree(mb.xole); dfree(mb.yole); dfree(mb.turmu_n);
   }
   dfree(c->dMa); dfree(c->dTa); dfree(c->d_istep); dfree(c->d_itnorms);
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
   dfree(c->d_cinst); dfree(c->d_ctol);
   dfree(c->d_partials); dfree(c->d_norms); dfree(c->d_tspec);
   dfree(c->d_Ds); dfree(c->d_hist); dfree(c->d_forces);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -514,6 +528,7 @@ static int32_t mesh_records(ot2d_case* c, int im, const ot2d_params* pc) {
     M.inst_stride = (long long)mb.n2;
     M.x = mb.x + o; M.y = mb.y + o; M.xx = mb.xx + o; M.xy = mb.xy + o; M.yx = mb.yx + o; M.yy = mb.yy + o;
     M.ug = mb.ug + o; M.vg = mb.vg + o; M.turmu = mb.turmu + o; M.tscale = mb.tscale + o; M.bt = mb.bt + o;
+    M.turmu_n = mb.turmu_n ? mb.turmu_n + o : nullptr;
     M.xv = mb.xv + ov; M.yv = mb.yv + ov; M.ugv = mb.ugv + ov; M.vgv = mb.vgv + ov;
     M.xv0 = mb.xv; M.yv0 = mb.yv;
     if (mb.xold) { M.xold = mb.xold + ov; M.yold = mb.yold + ov; M.xole = mb.xole + ov; M.yole = mb.yole + ov; }
@@ -606,6 +621,14 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
   if (e != hipSuccess) { free_case(c); return fail(OT2D_ERR_HIP, "stream: %s", hipGetErrorString(e)); }
   if (p->lu_fp32)
     if (int32_t rc = fp32_chain_init(c->device)) { free_case(c); return rc; }
+  if (kSaBesideLu && !c->lamin && c->iturb == 1) {
+    if ((e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess) {
+      free_case(c);
+      return fail(OT2D_ERR_HIP, "second stream: %s", hipGetErrorString(e));
+    }
+  }
   int ns = c->nspec;
   c->m.resize(nmesh);
   int nmi = nmesh * ns;
@@ -704,6 +727,7 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     mb.lu_per = lu_workspace_bytes(mb.jd, mb.kd, c->lu_rl) / sizeof(double);
     A(mb.lu, mb.lu_per * ns);
     if (!c->lamin) A(mb.sa, (c->iturb == 2 ? 20 : 10) * mb.n2 * ns);
+    if (!c->lamin && c->iturb == 1) A(mb.turmu_n, mb.n2 * ns);
     if (c->tacc) A(mb.ta, 3 * (size_t)c->nv * mb.n2 * ns);
     if (c->moving) { A(mb.xold, mb.nv2 * ns); A(mb.yold, mb.nv2 * ns); A(mb.xole, mb.nv2 * ns); A(mb.yole, mb.nv2 * ns); }
     int tj = (mb.jd - 2 + RES_TJ - 1) / RES_TJ, tk = (mb.kd - 2 + RES_TK - 1) / RES_TK;
@@ -765,6 +789,7 @@ static size_t footprint(const ot2d_params* ps, int32_t ncase, int32_t nmesh, con
     const size_t jd = md[im].jmax + 1, kd = md[im].kmax + 1, n2 = jd * kd, nv2 = (size_t)md[im].jmax * md[im].kmax;
     size_t d = n2 * (nq + nv) + 11 * n2 + 4 * nv2;
     if (!lamin) d += (iturb == 2 ? 20 : 10) * n2;
+    if (!lamin && iturb == 1) d += n2;
     if (tacc) d += 3 * (size_t)nv * n2;
     if (!ps->timespectral && ps->iunst == 1) d += 4 * nv2;
     const size_t tiles = ((jd - 2 + RES_TJ - 1) / RES_TJ) * ((kd - 2 + RES_TK - 1) / RES_TK);
@@ -1095,6 +1120,35 @@ extern "C" int32_t ot2d_monitor(ot2d_case* c, int32_t* mstop) {
 }
 
 // one step (src/step.f:2-218) + monitor, enqueued without synchronisation
+// ilu2d, the turbulence model (vmu_sa reads the state ilu2d leaves, before
+// the q update, src/step.f:189-211) and the update with monitor.  For SA
+// (outside per-stage profiling) the model runs on a second stream beside
+// ilu2d's prep and sweeps, which do not touch what it reads and writes
+// (q5, its workspace, turmu_n); it takes q1-q4 through ilu2d's primitive
+// round trip itself, and k_lu_finish (which writes that round trip) and the
+// update (which takes turmu_n) wait for it.
+static int32_t enqueue_turbulent_implicit(ot2d_case* c, const StepView& v, Marks* mk) {
+  if (c->iturb == 1 && !mk && c->st2) {
+    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+    HIPCHK(launch_sa(v.dM, v.hM, v.n, nullptr, c->st2, c->cfg.sa_lane, 0, 1));
+    HIPCHK(hipEventRecord(c->ev_join, c->st2));
+    HIPCHK(launch_lu(v.dM, v.hM, v.n, 4, c->p.lu_fp32, nullptr, c->stream, c->cfg.sweep));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    HIPCHK(launch_lu_finish(v.dM, v.hM, v.n, 0, c->p.lu_fp32, c->stream));
+    HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream, 1));
+    return OT2D_OK;
+  }
+  // SA: the laminar viscosity and velocities come out of k_lu_finish
+  const int sa = c->iturb == 1 ? 2 : 0;
+  HIPCHK(launch_lu(v.dM, v.hM, v.n, sa, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
+  if (c->iturb == 2) HIPCHK(launch_sst(v.dM, v.hM, v.n, mk, c->stream));
+  else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane, 1));
+  // the q update and monitor in one pass (stage "update", marked by the caller)
+  HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream));
+  return OT2D_OK;
+}
+
 static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr);
 
 // a time step of a time-accurate run (src/Turns_overset.f:357-510): the
@@ -1120,11 +1174,7 @@ static int32_t enqueue_time_step(ot2d_case* c, Marks* mk) {
     if (c->lamin) {
       HIPCHK(launch_lu(v.dM, v.hM, v.n, 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
     } else {
-      const int sa = c->iturb == 1 ? 2 : 0;
-      HIPCHK(launch_lu(v.dM, v.hM, v.n, sa, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
-      if (c->iturb == 2) HIPCHK(launch_sst(v.dM, v.hM, v.n, mk, c->stream));
-      else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane, 1));
-      HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream));
+      if (int32_t rc = enqueue_turbulent_implicit(c, v, mk)) return rc;
       mark(mk, OT2D_ST_UPDATE, c->stream);
     }
     if (c->ndon > 0 || c->nfr > 0) {
@@ -1148,13 +1198,7 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk) {
     HIPCHK(launch_lu(v.dM, v.hM, v.n, 1, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
   } else {
     // vmu_sa reads the state ilu2d leaves (before the q update), src/step.f:189-211
-    // SA: the laminar viscosity and velocities come out of k_lu_finish
-    const int sa = c->iturb == 1 ? 2 : 0;
-    HIPCHK(launch_lu(v.dM, v.hM, v.n, sa, c->p.lu_fp32, mk, c->stream, c->cfg.sweep));
-    if (c->iturb == 2) HIPCHK(launch_sst(v.dM, v.hM, v.n, mk, c->stream));
-    else HIPCHK(launch_sa(v.dM, v.hM, v.n, mk, c->stream, c->cfg.sa_lane, 1));
-    // the q update and monitor in one pass (stage "update")
-    HIPCHK(launch_update_monitor(v.dM, v.hM, v.n, c->stream));
+    if (int32_t rc = enqueue_turbulent_implicit(c, v, mk)) return rc;
     mark(mk, OT2D_ST_UPDATE, c->stream);
   }
   // overset meshes exchange fringe values (src/Turns_overset.f:500-503)

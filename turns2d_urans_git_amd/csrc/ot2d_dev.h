@@ -60,6 +60,7 @@ struct DevMesh {
   double* q[OT2D_MAXQ];      // q(:,:,nsp,n) planes, n = 1..nq  (index n-1)
   double* s[OT2D_MAXQ];      // s(:,:,nsp,n) planes
   double *x, *y, *xx, *xy, *yx, *yy, *ug, *vg, *turmu, *tscale, *bt;
+  double* turmu_n;           // SA beside ilu2d: the new eddy viscosity until the update takes it
   double *xv, *yv, *ugv, *vgv;  // vertex arrays (jmax,kmax) of this instance
   const double *xv0, *yv0;      // instance-1 vertices (time-spectral rotation)
   double *xold, *yold, *xole, *yole;  // vertices one and two time steps back (moving grids), or null
@@ -233,6 +234,16 @@ __device__ inline double time_oat(const DevMesh& M) {
   return oat;
 }
 
+// round trip of ilu2d's in-place primitive conversion (src/lhs.f:524-531,
+// 805-811): the reference leaves q(j,k,1:4) with these bits
+__device__ __forceinline__ void q_roundtrip(double& q1, double& q2, double& q3, double& q4, double qj) {
+  double dj = 1.0 / q1;
+  double u = q2 * dj, v = q3 * dj, e = q4 * dj;
+  double r = q1 * qj;
+  q1 = r / qj;
+  q2 = u * q1; q3 = v * q1; q4 = e * q1;
+}
+
 // laminar viscosity (lamvis, src/samodel2d.f:200-238) and the velocities
 // vmu_sa keeps (src/samodel2d.f:61-72) of one cell from its state
 __device__ inline void sa_prim_cell(const DevMesh& M, long long i, double q1, double q2, double q3, double q4) {
@@ -347,15 +358,20 @@ struct LaunchCfg {
 hipError_t launch_lu(const DevMesh* dM, const DevMesh* hM, int nmi, int fuse_update, int fp32, Marks* mk,
                      hipStream_t st, int sweep = 0);
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
-hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
+hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int take_turmu = 0);
+// k_lu_finish alone (launch_lu with fuse_update | 4 leaves it out)
+hipError_t launch_lu_finish(const DevMesh* dM, const DevMesh* hM, int nmi, int flags, int fp32, hipStream_t st);
 hipError_t launch_forces(const DevMesh* dM, int mi, double* out, hipStream_t st);
 hipError_t launch_tsrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 size_t lu_workspace_bytes(int jd, int kd, int rl);
 hipError_t launch_visrhs(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int norms = 0);
 // prim_done: the laminar viscosity and velocities are already in place
 // (k_lu_finish with flag 2)
+// ovl: the SA model runs beside ilu2d (another stream, before k_lu_finish):
+// it takes q1-q4 through ilu2d's primitive round trip itself and leaves the
+// eddy viscosity in turmu_n, which k_update_monitor copies
 hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane = -1,
-                     int prim_done = 0);
+                     int prim_done = 0, int ovl = 0);
 hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st);
 hipError_t launch_sst(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st);

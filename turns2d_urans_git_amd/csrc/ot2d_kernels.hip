@@ -2144,16 +2144,6 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
     if (l == 0) atomicCAS(M.status, 0, 3);
 }
 
-// round trip of ilu2d's in-place primitive conversion (src/lhs.f:524-531,
-// 805-811): the reference leaves q(j,k,1:4) with these bits
-__device__ __forceinline__ void q_roundtrip(double& q1, double& q2, double& q3, double& q4, double qj) {
-  double dj = 1.0 / q1;
-  double u = q2 * dj, v = q3 * dj, e = q4 * dj;
-  double r = q1 * qj;
-  q1 = r / qj;
-  q2 = u * q1; q3 = v * q1; q4 = e * q1;
-}
-
 __device__ __forceinline__ bool asq_bad(const DevConst& C, double q1, double q2, double q3, double q4) {
   double rho = q1;
   double qq = sq(q2) + sq(q3);
@@ -2221,7 +2211,9 @@ __global__ __launch_bounds__(256) void k_lu_finish(const DevMesh* __restrict__ M
 // one pass: interior cells are updated, then every cell (halo included) is
 // checked; each cell's update reads only that cell, so this is the update
 // followed by the monitor
-__global__ __launch_bounds__(256) void k_update_monitor(const DevMesh* __restrict__ Ms) {
+// take_turmu: the eddy viscosity the SA model left in turmu_n (it ran beside
+// ilu2d) becomes turmu
+__global__ __launch_bounds__(256) void k_update_monitor(const DevMesh* __restrict__ Ms, int take_turmu) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int jd = M.jd;
@@ -2235,6 +2227,7 @@ __global__ __launch_bounds__(256) void k_update_monitor(const DevMesh* __restric
       q1 = q1 + M.s[0][i] * ibp; q2 = q2 + M.s[1][i] * ibp; q3 = q3 + M.s[2][i] * ibp; q4 = q4 + M.s[3][i] * ibp;
       M.q[0][i] = q1; M.q[1][i] = q2; M.q[2][i] = q3; M.q[3][i] = q4;
     }
+    if (take_turmu) M.turmu[i] = M.turmu_n[i];
     bad |= asq_bad(M.c, q1, q2, q3, q4);
   }
   if (bad) atomicCAS(M.status, 0, 1);
@@ -2760,6 +2753,7 @@ static hipError_t launch_lu_t(const DevMesh* dM, const DevMesh* hM, int nmi, int
   hipLaunchKernelGGL((k_lu_sweep<T, -1, RL, D>), dim3(nmi), dim3(64), lds, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_BACKWARD, st);
+  if (fuse_update & 4) return hipSuccess;
   hipLaunchKernelGGL(k_lu_finish<T>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, fuse_update);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_FINISH, st);
@@ -2823,10 +2817,19 @@ static hipError_t launch_lu32(const DevMesh* dM, const DevMesh* hM, int nmi, int
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
+  if (fuse_update & 4) return hipSuccess;
   hipLaunchKernelGGL(k_lu_finish<float>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, fuse_update);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_LU_FINISH, st);
   return hipSuccess;
+}
+
+hipError_t launch_lu_finish(const DevMesh* dM, const DevMesh* hM, int nmi, int flags, int fp32, hipStream_t st) {
+  long long mx = 0;
+  for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
+  if (fp32) hipLaunchKernelGGL(k_lu_finish<float>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, flags);
+  else hipLaunchKernelGGL(k_lu_finish<double>, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, flags);
+  return hipGetLastError();
 }
 
 // rows per lane rl = ceil(rows / 64) of the case (all instances share it)
@@ -2861,10 +2864,10 @@ size_t lu_workspace_bytes(int jd, int kd, int rl) {
   return 16 * (size_t)lu_geom(jd - 2, kd - 2, rl).total;
 }
 
-hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
+hipError_t launch_update_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st, int take_turmu) {
   long long mx = 0;
   for (int i = 0; i < nmi; ++i) mx = hM[i].jd * (long long)hM[i].kd > mx ? hM[i].jd * (long long)hM[i].kd : mx;
-  hipLaunchKernelGGL(k_update_monitor, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM);
+  hipLaunchKernelGGL(k_update_monitor, dim3(nblk(mx, 256), nmi), dim3(256), 0, st, dM, take_turmu);
   return hipGetLastError();
 }
 hipError_t launch_monitor(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {

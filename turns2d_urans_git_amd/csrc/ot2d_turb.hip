@@ -7,7 +7,8 @@
 //   vmu_sa (src/samodel2d.f:2-197)     k_sa_prim     laminar viscosity, u, v (all cells)
 //                                      k_turbc       SA boundary conditions (1 block / segment,
 //                                                    one launch per phase of the segments)
-//                                      k_turm_ring   eddy viscosity of the halo ring
+//                                      k_turbc_eddy  the final SA boundary conditions and the
+//                                                    eddy viscosity of the halo ring
 //                                      k_sa_rhs      convection, diffusion, source and the
 //                                                    two scalar tridiagonal systems, in
 //                                                    gather form (one thread per cell)
@@ -373,12 +374,16 @@ __device__ __forceinline__ double ip6(double x) { return ((((x * x) * x) * x) * 
 
 // laminar viscosity (lamvis, src/samodel2d.f:200-238) and the velocities
 // vmu_sa keeps (src/samodel2d.f:61-72), all cells
-__global__ __launch_bounds__(256) void k_sa_prim(const DevMesh* __restrict__ Ms) {
+// ovl: the state as ilu2d's primitive round trip leaves it
+__global__ __launch_bounds__(256) void k_sa_prim(const DevMesh* __restrict__ Ms, int ovl) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   long long n2 = (long long)M.jd * M.kd;
-  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x)
-    sa_prim_cell(M, i, M.q[0][i], M.q[1][i], M.q[2][i], M.q[3][i]);
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += 256ll * gridDim.x) {
+    double q1 = M.q[0][i], q2 = M.q[1][i], q3 = M.q[2][i], q4 = M.q[3][i];
+    if (ovl) q_roundtrip(q1, q2, q3, q4, M.q[M.nq - 1][i]);
+    sa_prim_cell(M, i, q1, q2, q3, q4);
+  }
 }
 
 __device__ __forceinline__ int sa_periodic_src(int idir, int js, int je, int ks, int ke, int jd, int kd, int jk) {
@@ -398,8 +403,17 @@ __device__ __forceinline__ double c_turm_val(double q1, double qn, double vmul, 
   double fv1 = ip3(chi) / (ip3(chi) + ip3(SA_cv1));
   return q1 * qn * q5 * fv1;
 }
-__device__ __forceinline__ void c_turm_cell(const DevMesh& M, long long i) {
-  M.turmu[i] = c_turm_val(M.q[0][i], M.q[M.nq - 1][i], M.sa_vmul[i], M.q[4][i]);
+// the eddy viscosity of cell i; with ovl the SA model runs beside ilu2d, so
+// q1 is taken through ilu2d's round trip, (q1 qj) / qj, and the value goes
+// to turmu_n
+__device__ __forceinline__ void put_turm(const DevMesh& M, long long i, double q1, double qj, double vmul, double q5,
+                                         int ovl) {
+  if (ovl) M.turmu_n[i] = c_turm_val((q1 * qj) / qj, qj, vmul, q5);
+  else M.turmu[i] = c_turm_val(q1, qj, vmul, q5);
+}
+
+__device__ __forceinline__ void c_turm_cell(const DevMesh& M, long long i, int ovl) {
+  put_turm(M, i, M.q[0][i], M.q[M.nq - 1][i], M.sa_vmul[i], M.q[4][i], ovl);
 }
 
 // turbc (src/samodel2d.f:827-1303) for one boundary segment
@@ -501,10 +515,16 @@ __global__ __launch_bounds__(BC_THREADS) void k_turbc(const DevMesh* __restrict_
   turbc_segment(M, M.bc[ib]);
 }
 
-// the eddy viscosity of the halo ring after the final turbc (c_turm)
-__global__ __launch_bounds__(BC_THREADS) void k_turm_ring(const DevMesh* __restrict__ Ms) {
+// the final turbc and the eddy viscosity of the halo ring after it
+// (c_turm), one workgroup per instance: every segment in order (one launch
+// here costs less than a launch per phase and one for the ring)
+__global__ __launch_bounds__(BC_THREADS) void k_turbc_eddy(const DevMesh* __restrict__ Ms, int ovl) {
   const DevMesh& M = Ms[blockIdx.x];
   if (*M.status) return;
+  for (int ib = 0; ib < M.nbc; ++ib) {
+    turbc_segment(M, M.bc[ib]);
+    __syncthreads();
+  }
   const int jd = M.jd, kd = M.kd;
   const int nring = 2 * jd + 2 * (kd - 2);
   for (int w = threadIdx.x; w < nring; w += blockDim.x) {
@@ -513,7 +533,7 @@ __global__ __launch_bounds__(BC_THREADS) void k_turm_ring(const DevMesh* __restr
     else if (w < 2 * jd) { j = w - jd + 1; k = kd; }
     else if (w < 2 * jd + kd - 2) { j = 1; k = w - 2 * jd + 2; }
     else { j = jd; k = w - 2 * jd - (kd - 2) + 2; }
-    c_turm_cell(M, cell(M, j, k));
+    c_turm_cell(M, cell(M, j, k), ovl);
   }
 }
 
@@ -722,7 +742,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ M
 // lsolvek (src/samodel2d.f:1347-1385), s2 and the nu-tilde update with its
 // clip (src/samodel2d.f:157-188), then the interior eddy viscosity (c_turm)
 // for the j column blockIdx.x
-__global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ Ms) {
+__global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ Ms, int ovl) {
   __shared__ double la[SA_LINE_MAX], lb[SA_LINE_MAX], lc[SA_LINE_MAX], ls[SA_LINE_MAX];
   __shared__ double lcp[SA_LINE_MAX], lsp[SA_LINE_MAX];
   const DevMesh& M = Ms[blockIdx.y];
@@ -748,7 +768,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
     double q5 = M.q[4][i];
     q5 = fmax((q5 + dq * (double)imx(M.iblank[i], 0)), vnulim);
     M.q[4][i] = q5;
-    c_turm_cell(M, i);
+    c_turm_cell(M, i, ovl);
   }
 }
 
@@ -761,7 +781,7 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
 // cells each with their loads issued together.  Columns past je stage a
 // harmless b = 1 row and are never written.
 constexpr int SAK_TJ = 8, SAK_NMAX = 256, SAK_THREADS = 256;
-__global__ __launch_bounds__(SAK_THREADS) void k_sa_solve_k8(const DevMesh* __restrict__ Ms) {
+__global__ __launch_bounds__(SAK_THREADS) void k_sa_solve_k8(const DevMesh* __restrict__ Ms, int ovl) {
   __shared__ double la[SAK_NMAX * SAK_TJ], lb[SAK_NMAX * SAK_TJ], lc[SAK_NMAX * SAK_TJ];
   __shared__ double ls[SAK_NMAX * SAK_TJ], lcp[SAK_NMAX * SAK_TJ];
   const DevMesh& M = Ms[blockIdx.y];
@@ -823,7 +843,7 @@ __global__ __launch_bounds__(SAK_THREADS) void k_sa_solve_k8(const DevMesh* __re
       double dq = relfac * (0. + ls[e]);
       const double v = fmax((q5[u] + dq * ib[u]), vnulim);
       M.q[4][ii[u]] = v;
-      M.turmu[ii[u]] = c_turm_val(q1[u], qj[u], vm[u], v);
+      put_turm(M, ii[u], q1[u], qj[u], vm[u], v, ovl);
     }
   }
 }
@@ -1218,7 +1238,7 @@ __global__ __launch_bounds__(64) void k_sa_lines(const DevMesh* __restrict__ Ms)
 // eddy viscosity after the lane-per-line k solve: a pointwise pass over the
 // interior cells, so its loads stream at full bandwidth instead of waiting
 // on each line's recurrence
-__global__ __launch_bounds__(256) void k_sa_update(const DevMesh* __restrict__ Ms) {
+__global__ __launch_bounds__(256) void k_sa_update(const DevMesh* __restrict__ Ms, int ovl) {
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int nj = M.je - M.js + 1, nk = M.ke - M.ks + 1;
@@ -1230,7 +1250,7 @@ __global__ __launch_bounds__(256) void k_sa_update(const DevMesh* __restrict__ M
     const double dq = M.sa_sj[i];
     const double v = fmax((M.q[4][i] + relfac * (0. + dq) * (double)imx(M.iblank[i], 0)), vnulim);
     M.q[4][i] = v;
-    M.turmu[i] = c_turm_val(M.q[0][i], M.q[M.nq - 1][i], M.sa_vmul[i], v);
+    put_turm(M, i, M.q[0][i], M.q[M.nq - 1][i], M.sa_vmul[i], v, ovl);
   }
 }
 
@@ -1293,11 +1313,10 @@ hipError_t launch_sa_dist(const DevMesh* dM, const DevMesh* hM, int nmi, hipStre
 }
 
 hipError_t launch_sa_prim(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
-  hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(max_cells(hM, nmi), 256), nmi), dim3(256), 0, st, dM);
+  hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(max_cells(hM, nmi), 256), nmi), dim3(256), 0, st, dM, 0);
   return hipGetLastError();
 }
 
-// vmu_sa for every instance (src/samodel2d.f:2-197, itrans = 0)
 // turbc over every instance: one launch per phase of the segments
 static hipError_t launch_turbc(const DevMesh* dM, const DevMesh* hM, int nmi, hipStream_t st) {
   int nbc = 1, nph = 1;
@@ -1313,8 +1332,9 @@ static hipError_t launch_turbc(const DevMesh* dM, const DevMesh* hM, int nmi, hi
   return hipSuccess;
 }
 
+// vmu_sa for every instance (src/samodel2d.f:2-197, itrans = 0)
 hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, hipStream_t st, int sa_lane,
-                     int prim_done) {
+                     int prim_done, int ovl) {
   long long mx = max_cells(hM, nmi);
   int mj = 1, mk_ = 1;
   for (int i = 0; i < nmi; ++i) {
@@ -1323,7 +1343,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   }
   hipError_t e = hipSuccess;
   if (!prim_done) {
-    hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
+    hipLaunchKernelGGL(k_sa_prim, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM, ovl);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   mark(mk, OT2D_ST_SA_PRIM, st);
@@ -1344,16 +1364,15 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   if (per_lane) {
     hipLaunchKernelGGL(k_sa_lines<0>, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sa_update, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM);
+    hipLaunchKernelGGL(k_sa_update, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM, ovl);
   } else {
     if (mk_ <= SAK_NMAX)
-      hipLaunchKernelGGL(k_sa_solve_k8, dim3((mj + SAK_TJ - 1) / SAK_TJ, nmi), dim3(SAK_THREADS), 0, st, dM);
-    else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM);
+      hipLaunchKernelGGL(k_sa_solve_k8, dim3((mj + SAK_TJ - 1) / SAK_TJ, nmi), dim3(SAK_THREADS), 0, st, dM, ovl);
+    else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM, ovl);
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
-  if ((e = launch_turbc(dM, hM, nmi, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_turm_ring, dim3(nmi), dim3(BC_THREADS), 0, st, dM);
+  hipLaunchKernelGGL(k_turbc_eddy, dim3(nmi), dim3(BC_THREADS), 0, st, dM, ovl);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_EDDY, st);
   return hipSuccess;
