@@ -13,8 +13,8 @@
 //                                                    two scalar tridiagonal systems, in
 //                                                    gather form (one thread per cell)
 //                                      k_sa_solve_j  tridiagonal solve along j (thread per k)
-//                                      k_sa_solve_k8 tridiagonal solves along k, 8 adjacent j
-//                                                    columns per workgroup (k_sa_solve_k: one
+//                                      k_sa_solve_kt tridiagonal solves along k, 16 or 8 adjacent
+//                                                    j columns per workgroup (k_sa_solve_k: one
 //                                                    column, for lines over 256 cells),
 //                                                    nu-tilde update and eddy viscosity
 //
@@ -772,74 +772,82 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
   }
 }
 
-// lsolvek, the nu-tilde update and c_turm for 8 adjacent j columns
-// (blockIdx.x) at once, for lines of up to SAK_NMAX cells: the workgroup
-// (four waves) stages the 8 k lines as rows of 8 consecutive cells (each row
-// one 64-byte run, where the one-line kernel gathers one element per cache
-// line), lanes 0..7 of wave 0 run the 8 recurrences out of LDS (column c at
-// stride 8), and all threads apply the update and the eddy viscosity, a few
-// cells each with their loads issued together.  Columns past je stage a
-// harmless b = 1 row and are never written.
-constexpr int SAK_TJ = 8, SAK_NMAX = 256, SAK_THREADS = 256;
-__global__ __launch_bounds__(SAK_THREADS) void k_sa_solve_k8(const DevMesh* __restrict__ Ms, int ovl) {
-  __shared__ double la[SAK_NMAX * SAK_TJ], lb[SAK_NMAX * SAK_TJ], lc[SAK_NMAX * SAK_TJ];
-  __shared__ double ls[SAK_NMAX * SAK_TJ], lcp[SAK_NMAX * SAK_TJ];
+// lsolvek, the nu-tilde update and c_turm for TJ adjacent j columns
+// (blockIdx.x) at once, for lines of up to nmax cells: the workgroup (32*TJ
+// threads) stages the TJ k lines as rows of TJ consecutive cells (each row one
+// TJ*8-byte run, where the one-line kernel gathers one element per cache
+// line; TJ = 16 makes each run a whole 128-byte line), lanes 0..TJ-1 of wave
+// 0 run the TJ recurrences out of LDS (column c at stride TJ), and all
+// threads apply the update and the eddy viscosity, a few cells each with
+// their loads issued together.  Columns past je stage a harmless b = 1 row
+// and are never written.  LDS: five arrays of nmax*TJ doubles (dynamic).
+constexpr int SAK_TJ = 8, SAK_NMAX = 256;
+constexpr int SAK_TJ16 = 16, SAK_NMAX16 = 192;   // 16 columns for lines of up to 192 cells
+template <int TJ>
+__global__ __launch_bounds__(32 * TJ) void k_sa_solve_kt(const DevMesh* __restrict__ Ms, int ovl, int nmax) {
+  extern __shared__ __attribute__((aligned(16))) double sak_lds[];
+  double* const la = sak_lds;
+  double* const lb = la + (size_t)nmax * TJ;
+  double* const lc = lb + (size_t)nmax * TJ;
+  double* const ls = lc + (size_t)nmax * TJ;
+  double* const lcp = ls + (size_t)nmax * TJ;
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int ks = M.ks, ke = M.ke;
-  const int j0 = M.js + (int)blockIdx.x * SAK_TJ;
+  const int j0 = M.js + (int)blockIdx.x * TJ;
   if (j0 > M.je) return;
   const int n = ke - ks + 1;
-  const int ne = n * SAK_TJ;
+  const int ne = n * TJ;
+  constexpr int NT = 32 * TJ;   // threads
   const int tid = (int)threadIdx.x;
   const double* __restrict__ s = M.sa_sj;
   const double* __restrict__ c = M.sa_ck;
   const double* __restrict__ a = M.sa_ak;
   const double* __restrict__ b = M.sa_bb;
-  // element e = k*8 + col of the tile; 4 elements per thread in flight at once
+  // element e = k*TJ + col of the tile; 4 elements per thread in flight at once
   constexpr int U = 4;
-  for (int e0 = 0; e0 < ne; e0 += SAK_THREADS * U) {
+  for (int e0 = 0; e0 < ne; e0 += NT * U) {
     double ra[U], rb[U], rc[U], rs[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = e0 + tid + SAK_THREADS * u;
-      const int j = j0 + (e & (SAK_TJ - 1));
-      const int k = ks + (e < ne ? e : ne - 1) / SAK_TJ;
+      const int e = e0 + tid + NT * u;
+      const int j = j0 + (e & (TJ - 1));
+      const int k = ks + (e < ne ? e : ne - 1) / TJ;
       const long long i = cell(M, j <= M.je ? j : M.je, k);
       ra[u] = a[i]; rb[u] = b[i]; rc[u] = c[i]; rs[u] = s[i];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = e0 + tid + SAK_THREADS * u;
+      const int e = e0 + tid + NT * u;
       if (e < ne) {
-        const bool in = j0 + (e & (SAK_TJ - 1)) <= M.je;
+        const bool in = j0 + (e & (TJ - 1)) <= M.je;
         la[e] = in ? ra[u] : 0.0; lb[e] = in ? rb[u] : 1.0; lc[e] = in ? rc[u] : 0.0; ls[e] = in ? rs[u] : 0.0;
       }
     }
   }
   __syncthreads();
-  if (tid < SAK_TJ) thomas_lds<SAK_TJ>(la + tid, lb + tid, lc + tid, ls + tid, ls + tid, lcp + tid, n);
+  if (tid < TJ) thomas_lds<TJ>(la + tid, lb + tid, lc + tid, ls + tid, ls + tid, lcp + tid, n);
   __syncthreads();
   // the update and c_turm, U cells per thread with all their loads issued
   // before the first store
   const double vnulim = 1.0e-20, relfac = 1.0;
-  for (int e0 = 0; e0 < ne; e0 += SAK_THREADS * U) {
+  for (int e0 = 0; e0 < ne; e0 += NT * U) {
     long long ii[U];
     bool ok[U];
     double q5[U], q1[U], qj[U], vm[U], ib[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = e0 + tid + SAK_THREADS * u;
-      const int j = j0 + (e & (SAK_TJ - 1));
+      const int e = e0 + tid + NT * u;
+      const int j = j0 + (e & (TJ - 1));
       ok[u] = e < ne && j <= M.je;
-      ii[u] = cell(M, j <= M.je ? j : M.je, ks + (e < ne ? e : ne - 1) / SAK_TJ);
+      ii[u] = cell(M, j <= M.je ? j : M.je, ks + (e < ne ? e : ne - 1) / TJ);
       q5[u] = M.q[4][ii[u]]; q1[u] = M.q[0][ii[u]]; qj[u] = M.q[M.nq - 1][ii[u]]; vm[u] = M.sa_vmul[ii[u]];
       ib[u] = (double)imx(M.iblank[ii[u]], 0);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (!ok[u]) continue;
-      const int e = e0 + tid + SAK_THREADS * u;
+      const int e = e0 + tid + NT * u;
       double dq = relfac * (0. + ls[e]);
       const double v = fmax((q5[u] + dq * ib[u]), vnulim);
       M.q[4][ii[u]] = v;
@@ -1366,9 +1374,29 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sa_update, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM, ovl);
   } else {
-    if (mk_ <= SAK_NMAX)
-      hipLaunchKernelGGL(k_sa_solve_k8, dim3((mj + SAK_TJ - 1) / SAK_TJ, nmi), dim3(SAK_THREADS), 0, st, dM, ovl);
-    else hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM, ovl);
+    if (mk_ <= SAK_NMAX16) {
+      const size_t lds = (size_t)5 * mk_ * SAK_TJ16 * sizeof(double);
+      static bool attr = false;
+      if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_sa_solve_kt<SAK_TJ16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(5 * SAK_NMAX16 * SAK_TJ16 * sizeof(double)));
+        attr = true;
+      }
+      hipLaunchKernelGGL(k_sa_solve_kt<SAK_TJ16>, dim3((mj + SAK_TJ16 - 1) / SAK_TJ16, nmi), dim3(32 * SAK_TJ16), lds,
+                         st, dM, ovl, mk_);
+    } else if (mk_ <= SAK_NMAX) {
+      const size_t lds = (size_t)5 * mk_ * SAK_TJ * sizeof(double);
+      static bool attr = false;
+      if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_sa_solve_kt<SAK_TJ>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(5 * SAK_NMAX * SAK_TJ * sizeof(double)));
+        attr = true;
+      }
+      hipLaunchKernelGGL(k_sa_solve_kt<SAK_TJ>, dim3((mj + SAK_TJ - 1) / SAK_TJ, nmi), dim3(32 * SAK_TJ), lds, st,
+                         dM, ovl, mk_);
+    } else {
+      hipLaunchKernelGGL(k_sa_solve_k, dim3(mj, nmi), dim3(64), 0, st, dM, ovl);
+    }
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_K, st);
