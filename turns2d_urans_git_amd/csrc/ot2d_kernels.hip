@@ -2073,11 +2073,6 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lu_sweep32(const DevMesh* __r
       // the tags and read it again
       ok = ok && spin_tags(slot, t + 1);
       rd(slot, C, tc);
-      // wait for the re-read here (lgkmcnt(0)): the compiler's wait before a
-      // use takes the worst case over the paths that reach it, and a re-read
-      // still in flight at the join would make every step wait for the reads
-      // issued one step earlier instead of two
-      __builtin_amdgcn_s_waitcnt(0xC07F);
     }
     // every third step (the last of a register-set rotation) frees the
     // slots up to step t.  Their reads were issued before this store (two
