@@ -12,7 +12,8 @@
 //                                      k_sa_rhs      convection, diffusion, source and the
 //                                                    two scalar tridiagonal systems, in
 //                                                    gather form (one thread per cell)
-//                                      k_sa_solve_j  tridiagonal solve along j (thread per k)
+//                                      k_sa_solve_j  tridiagonal solve along j from both ends
+//                                                    (two lanes per k row)
 //                                      k_sa_solve_kt tridiagonal solves along k, 16 or 8 adjacent
 //                                                    j columns per workgroup (k_sa_solve_k: one
 //                                                    column, for lines over 256 cells),
@@ -710,13 +711,120 @@ __global__ __launch_bounds__(256) void k_sa_rhs(const DevMesh* __restrict__ Ms) 
 // The two scalar tridiagonal solves of vmu_sa are sequential recurrences
 // along each grid line.  One wave owns one line: all lanes stage the line's
 // coefficients in LDS (stage_line4: every load in flight at once, coalesced
-// along j, gathered along k), lane 0 runs the reference's Thomas recurrence
-// out of LDS (thomas_lds: its reads run a chunk ahead of the dependent
-// chain), and all lanes write the results.
-// The arithmetic is the reference's, operation for operation.
+// along j, gathered along k), lanes run the recurrence out of LDS (thomas_lds:
+// its reads run a chunk ahead of the dependent chain), and all lanes write the
+// results.  Along k the arithmetic is the reference's Thomas recurrence
+// (lsolvek), operation for operation.
+//
+// Along j (lsolvej) the line is eliminated from both ends at once, which
+// halves the dependent chain.  With 32-cell tiles, ntile = ceil(n/32) and
+// nr = floor(ntile/2) (0 for ntile = 1), the right part is the last 32*nr
+// cells and the left part the first m = n - 32*nr:
+//   left  (t = 0 .. m-1, up):   bb = 1/(b - a c'),  s' = (s - a s') bb,  c' = c bb
+//   right (t = n-1 .. m, down): bb = 1/(b - c a'),  s' = (s - c s') bb,  a' = a bb
+// (the first cell of each side: bb = 1/b), then at the meeting cells each side
+// takes its own value from its (s', c') and the other side's:
+//   x = (s' - c' s'_other) / (1 - c' c'_other)
+// and substitutes back towards its end: x = s' - c' x_next.  The reference's
+// recurrence is the case nr = 0.  Both j kernels (one wave per line, one
+// lane per line) evaluate exactly these operations in this order, so they
+// give bit-identical results; against the one-sided recurrence the results
+// differ in rounding only.
+constexpr int SAJ_TILE = 32;
+__device__ __forceinline__ void saj_split(int n, int& nr, int& m) {
+  const int ntile = (n + SAJ_TILE - 1) / SAJ_TILE;
+  nr = ntile >= 2 ? ntile / 2 : 0;
+  m = n - SAJ_TILE * nr;
+}
 
-// lsolvej (src/samodel2d.f:1306-1344) and s1, sjmat*bjmat of vmu_sa
-// (src/samodel2d.f:148-152) for the k row blockIdx.x
+// The two-ended elimination of one line in LDS, run by lanes h = 0 (left)
+// and h = 1 (right) of one wave (nr > 0), or by lane 0 alone (nr = 0).
+// s' and the solution go to os, c' / a' to oc.
+__device__ __forceinline__ void thomas2_lds(const double* la, const double* lb, const double* lc, const double* ls,
+                                            double* os, double* oc, int n, int h) {
+  constexpr int CH = THOMAS_CH;
+  int nr, m;
+  saj_split(n, nr, m);
+  const int cnt = h ? n - m : m;
+  const int i0 = h ? n - 1 : 0, st = h ? -1 : 1;   // cell of step u: i0 + st*u
+  const double* lo = h ? lc : la;
+  const double* hi = h ? la : lc;
+  double bb = 1. / lb[i0];
+  double sp = ls[i0] * bb, cp = hi[i0] * bb;
+  os[i0] = sp; oc[i0] = cp;
+  double pa[CH], pb[CH], pc[CH], ps[CH], qa[CH], qb[CH], qc[CH], qs[CH];
+  auto ld = [&](double (&xa)[CH], double (&xb)[CH], double (&xc)[CH], double (&xs)[CH], int u0) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int i = i0 + st * (u0 + u < cnt ? u0 + u : cnt - 1);
+      xa[u] = lo[i]; xb[u] = lb[i]; xc[u] = hi[i]; xs[u] = ls[i];
+    }
+  };
+  auto run = [&](const double (&xa)[CH], const double (&xb)[CH], const double (&xc)[CH], const double (&xs)[CH],
+                 int u0, int k) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (u < k) {
+        const double ai = xa[u];
+        bb = 1. / (xb[u] - ai * cp);
+        sp = (xs[u] - ai * sp) * bb;
+        cp = xc[u] * bb;
+        const int i = i0 + st * (u0 + u);
+        os[i] = sp; oc[i] = cp;
+      }
+    }
+  };
+  int u = 1;
+  ld(pa, pb, pc, ps, u);
+  for (; u + 2 * CH <= cnt; u += 2 * CH) {
+    ld(qa, qb, qc, qs, u + CH);
+    run(pa, pb, pc, ps, u, CH);
+    ld(pa, pb, pc, ps, u + 2 * CH);
+    run(qa, qb, qc, qs, u + CH, CH);
+  }
+  ld(qa, qb, qc, qs, u + CH);
+  run(pa, pb, pc, ps, u, cnt - u);
+  run(qa, qb, qc, qs, u + CH, cnt - u - CH);
+  // the meeting cells
+  double sn = sp;
+  if (nr > 0) {
+    const double so = __shfl_xor(sp, 1), co = __shfl_xor(cp, 1);
+    sn = (sp - cp * so) / (1. - cp * co);
+  }
+  os[i0 + st * (cnt - 1)] = sn;
+  // back substitution towards this side's end, chunks of CH steps
+  double ys[CH], yc[CH], zs[CH], zc[CH];
+  auto ldb = [&](double (&xs)[CH], double (&xc)[CH], int u1) {
+#pragma unroll
+    for (int v = 0; v < CH; ++v) {
+      const int i = i0 + st * (u1 - v >= 0 ? u1 - v : 0);
+      xs[v] = os[i]; xc[v] = oc[i];
+    }
+  };
+  auto runb = [&](const double (&xs)[CH], const double (&xc)[CH], int u1, int k) {
+#pragma unroll
+    for (int v = 0; v < CH; ++v) {
+      if (v < k) {
+        sn = xs[v] - xc[v] * sn;
+        os[i0 + st * (u1 - v)] = sn;
+      }
+    }
+  };
+  int b = cnt - 2;
+  ldb(ys, yc, b);
+  for (; b - 2 * CH + 1 >= 0; b -= 2 * CH) {
+    ldb(zs, zc, b - CH);
+    runb(ys, yc, b, CH);
+    ldb(ys, yc, b - 2 * CH);
+    runb(zs, zc, b - CH, CH);
+  }
+  ldb(zs, zc, b - CH);
+  runb(ys, yc, b, b + 1);
+  runb(zs, zc, b - CH, b - CH + 1);
+}
+
+// lsolvej (src/samodel2d.f:1306-1344), eliminated from both ends, and s1,
+// sjmat*bjmat of vmu_sa (src/samodel2d.f:148-152) for the k row blockIdx.x
 __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ Ms) {
   __shared__ double la[SA_LINE_MAX], lb[SA_LINE_MAX], lc[SA_LINE_MAX], ls[SA_LINE_MAX];
   __shared__ double lcp[SA_LINE_MAX], lsp[SA_LINE_MAX];
@@ -733,7 +841,9 @@ __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ M
   const double* __restrict__ b = M.sa_bb;
   stage_line4(a, b, c, s, i0, 1, n, la, lb, lc, ls);
   __syncthreads();
-  if (threadIdx.x == 0) thomas_lds(la, lb, lc, ls, lsp, lcp, n);
+  int nr, m;
+  saj_split(n, nr, m);
+  if (threadIdx.x < (nr > 0 ? 2u : 1u)) thomas2_lds(la, lb, lc, ls, lsp, lcp, n, (int)threadIdx.x);
   __syncthreads();
   // sjmat = (s1 - s2) * bjmat with s2 = 0
   for (int t = threadIdx.x; t < n; t += 64) s[i0 + t] = (0. + lsp[t] - 0.) * lb[t];
@@ -856,18 +966,22 @@ __global__ __launch_bounds__(32 * TJ) void k_sa_solve_kt(const DevMesh* __restri
   }
 }
 
-// Line solves for many lines at once: one lane per line, the same
-// reference arithmetic.
-// k_sa_lines_j: lines along j (one per k row), 64 rows per workgroup.  Wave
-// 0 runs the 64 recurrences; waves 1..SAJ_MW move the coefficient tiles
-// (SA_TJ consecutive cells of each of the 64 rows, read and written as
-// row-contiguous segments) between global memory and two LDS slots.  While
-// wave 0 works on tile x in one slot, the movers write tile x-1's results
-// back from the other slot and refill it with tile x+1, whose loads they
-// issued SAJ_D tiles earlier, so the recurrence does not wait on memory.
-// The forward pass leaves c' and s' in place (as lsolvej does), the
-// backward pass writes s = (s1 - s2) * bjmat.
-constexpr int SA_TJ = 32, SAJ_MW = 4, SAJ_D = 2;
+// Line solves for many lines at once: one lane per line (along j: one lane
+// per side of a line), the same arithmetic as the one-wave kernels.
+// k_sa_lines_j: lines along j (one per k row), 32 rows per workgroup, lane
+// l = 32h + r of wave 0 running side h of row r's two-ended elimination
+// (h = 0 the left part from j = js, h = 1 the right part from j = je).
+// Each side is cut into 32-cell tiles, counted from its own end; tile x of
+// side 1 is stored reversed (position p holds cell n-1-32x-p), so both
+// sides walk positions upwards in the elimination and downwards in the
+// back substitution.  Waves 1..SAJ_MW move the tiles (row-contiguous
+// segments) between global memory and two LDS slots: while wave 0 works on
+// tile x in one slot, the movers write tile x-1's results back from the
+// other slot and refill it with tile x+1, whose loads they issued SAJ_D
+// tiles earlier, so the recurrences do not wait on memory.  The elimination
+// leaves s' and c' (a' on the right) in place, the back substitution writes
+// s = (s1 - s2) * bjmat.
+constexpr int SA_TJ = SAJ_TILE, SAJ_MW = 4, SAJ_D = 2;
 constexpr int SAJ_THREADS = 64 * (1 + SAJ_MW);
 constexpr int SAJ_PER = 64 * SA_TJ / (64 * SAJ_MW);   // elements per mover lane and array
 // total lines (both directions, all instances) above which the solves run one lane per line
@@ -878,29 +992,38 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  const int k0 = ks + 64 * blockIdx.x;
+  const int k0 = ks + 32 * blockIdx.x;
   if (k0 > ke) return;
-  const int nrow = ke - k0 + 1 < 64 ? ke - k0 + 1 : 64;
+  const int nrow = ke - k0 + 1 < 32 ? ke - k0 + 1 : 32;
   const int n = je - js + 1;
-  const int ntile = (n + SA_TJ - 1) / SA_TJ;
+  int nr, m;
+  saj_split(n, nr, m);
+  const int nl = (m + SA_TJ - 1) / SA_TJ;   // tiles of the left side (nl >= nr)
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, ml = (int)threadIdx.x - 64;
+  const int h = l >> 5, rr = l & 31;         // the solver lane's side and row
+  // tiles of side h, and the last position of its last tile
+  const int ntl = h ? nr : nl;
+  const int lastp = h ? SA_TJ - 1 : m - 1 - SA_TJ * (nl - 1);
   gdouble* __restrict__ sS = gptr(M.sa_sj);
   gdouble* __restrict__ sC = gptr(M.sa_cj);
   const gdouble* __restrict__ sA = gptr(M.sa_aj);
   const gdouble* __restrict__ sB = gptr(M.sa_bb);
-  // element p of mover lane ml in a tile: e = ml + 256 p, row e / SA_TJ,
-  // column e % SA_TJ; the global index is clamped into the grid, so every
-  // load is unconditional and no loaded value merges across a branch
-  auto gidx = [&](int t, int p, int& r, int& cc, bool& ok) {
+  // element p of mover lane ml in a tile: e = ml + 256 p, slot row e / SA_TJ
+  // (side and row as for the solver lanes), position e % SA_TJ; the global
+  // index is clamped into the grid, so every load is unconditional and no
+  // loaded value merges across a branch
+  auto gidx = [&](int x, int p, int& r, int& cc, bool& ok) {
     const int e = ml + 64 * SAJ_MW * p;
     r = e / SA_TJ; cc = e % SA_TJ;
-    const int j = js + t * SA_TJ + cc;
-    ok = r < nrow && j <= je;
-    return cell(M, j < je ? j : je, k0 + (r < nrow ? r : 0));
+    const int hh = r >> 5, rw = r & 31;
+    const int t = hh ? n - 1 - SA_TJ * x - cc : SA_TJ * x + cc;   // cell of the line
+    ok = rw < nrow && (hh ? x < nr : t < m);
+    const int tc = t < 0 ? 0 : (t < n ? t : n - 1);
+    return cell(M, js + tc, k0 + (rw < nrow ? rw : 0));
   };
   double ra[SAJ_D][SAJ_PER], rb[SAJ_D][SAJ_PER], rc[SAJ_D][SAJ_PER], rs[SAJ_D][SAJ_PER];
 
-  // ---- forward elimination
+  // ---- elimination from both ends
   auto fetch_f = [&](int t, double (&a)[SAJ_PER], double (&b)[SAJ_PER], double (&c)[SAJ_PER], double (&s)[SAJ_PER]) {
 #pragma unroll
     for (int p = 0; p < SAJ_PER; ++p) {
@@ -910,8 +1033,9 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
       a[p] = sA[i]; b[p] = sB[i]; c[p] = sC[i]; s[p] = sS[i];
     }
   };
-  // elements outside the grid (rows past the last, columns past je in the
-  // last tile) hold copies of clamped cells: the recurrences never read them
+  // elements outside a side (rows past the last, cells past the side's end
+  // in its last tile) hold copies of clamped cells: the recurrences never
+  // read them
   auto put_f = [&](int slot, const double (&a)[SAJ_PER], const double (&b)[SAJ_PER], const double (&c)[SAJ_PER],
                    const double (&s)[SAJ_PER]) {
 #pragma unroll
@@ -920,7 +1044,7 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
       ta[slot][r][cc] = a[p]; tb[slot][r][cc] = b[p]; tc[slot][r][cc] = c[p]; ts[slot][r][cc] = s[p];
     }
   };
-  auto back_f = [&](int t, int slot) {   // c' and s' of tile t to global memory
+  auto back_f = [&](int t, int slot) {   // s' and c' (a') of tile t to global memory
 #pragma unroll
     for (int p = 0; p < SAJ_PER; ++p) {
       int r, cc; bool ok;
@@ -929,26 +1053,26 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     }
   };
   double cp = 0.0, sp = 0.0;
-  auto solve_f = [&](int t, int slot) {
-    if (l >= nrow) return;
-    // a full tile is one straight-line unrolled block, so its LDS reads run
-    // ahead of the recurrence; the line's first cell and a partial last
-    // tile take separate paths
-    const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
-    double (*A)[SA_TJ + 1] = ta[slot];
+  // side h's lower and upper coefficients: (a, c) on the left, (c, a) on the
+  // right; c' / a' goes where c was
+  auto solve_f = [&](int x, int slot) {
+    if (rr >= nrow || x >= ntl) return;
+    const int cn = x == ntl - 1 ? lastp + 1 : SA_TJ;   // positions of this tile
+    double (*LO)[SA_TJ + 1] = h ? tc[slot] : ta[slot];
+    double (*HI)[SA_TJ + 1] = h ? ta[slot] : tc[slot];
     double (*B)[SA_TJ + 1] = tb[slot];
     double (*Cc)[SA_TJ + 1] = tc[slot];
     double (*S)[SA_TJ + 1] = ts[slot];
     auto elem = [&](int cc) {
-      double ai = A[l][cc];
+      double ai = LO[l][cc];
       double bb = 1. / (B[l][cc] - ai * cp);
       sp = (S[l][cc] - ai * sp) * bb;
-      cp = Cc[l][cc] * bb;
+      cp = HI[l][cc] * bb;
       S[l][cc] = sp; Cc[l][cc] = cp;
     };
-    if (c0 == 0) {
+    if (x == 0) {
       double bb = 1. / B[l][0];
-      sp = S[l][0] * bb; cp = Cc[l][0] * bb;
+      sp = S[l][0] * bb; cp = HI[l][0] * bb;
       S[l][0] = sp; Cc[l][0] = cp;
 #pragma unroll
       for (int cc = 1; cc < SA_TJ; ++cc)
@@ -961,7 +1085,7 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
       auto ld = [&](int q, int c0) {
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-          xa[q][u] = A[l][c0 + u]; xb[q][u] = B[l][c0 + u]; xc[q][u] = Cc[l][c0 + u]; xs[q][u] = S[l][c0 + u];
+          xa[q][u] = LO[l][c0 + u]; xb[q][u] = B[l][c0 + u]; xc[q][u] = HI[l][c0 + u]; xs[q][u] = S[l][c0 + u];
         }
       };
       ld(0, 0);
@@ -985,23 +1109,23 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     }
   };
   // Tile x sits in register set x % SAJ_D.  Every fetch is unconditional
-  // (tile indices past the line are clamped to its last tile) and the
+  // (tile indices past the sides are clamped to the last tile) and the
   // movers' main loop runs whole groups of SAJ_D tiles in code of their own,
   // so no loaded value merges across a branch and each load is waited for
   // only where it is stored to LDS.  Solver and movers pass the same
   // sequence of barriers: one after each pass's prologue, one per tile, and
   // one between the passes.
-  auto clampt = [&](int t) { return t < ntile ? t : ntile - 1; };
+  auto clampt = [&](int t) { return t < nl ? t : nl - 1; };
   auto step_f = [&](int t, double (&a)[SAJ_PER], double (&b)[SAJ_PER], double (&c)[SAJ_PER],
                     double (&s)[SAJ_PER]) {
     if (t >= 1) back_f(t - 1, (t - 1) & 1);
-    if (t + 1 < ntile) put_f((t + 1) & 1, a, b, c, s);
+    if (t + 1 < nl) put_f((t + 1) & 1, a, b, c, s);
     fetch_f(clampt(t + 1 + SAJ_D), a, b, c, s);
     __syncthreads();
   };
   if (wave == 0) {
     __syncthreads();
-    for (int t = 0; t < ntile; ++t) {
+    for (int t = 0; t < nl; ++t) {
       solve_f(t, t & 1);
       __syncthreads();
     }
@@ -1012,7 +1136,7 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     fetch_f(clampt(SAJ_D), ra[0], rb[0], rc[0], rs[0]);
     __syncthreads();
     int t0 = 0;
-    for (; t0 + SAJ_D <= ntile; t0 += SAJ_D) {
+    for (; t0 + SAJ_D <= nl; t0 += SAJ_D) {
 #pragma unroll
       for (int d = 0; d < SAJ_D; ++d) {
         const int d1 = (d + 1) % SAJ_D;
@@ -1022,14 +1146,22 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
 #pragma unroll
     for (int d = 0; d < SAJ_D; ++d) {
       const int d1 = (d + 1) % SAJ_D;
-      if (t0 + d < ntile) step_f(t0 + d, ra[d1], rb[d1], rc[d1], rs[d1]);
+      if (t0 + d < nl) step_f(t0 + d, ra[d1], rb[d1], rc[d1], rs[d1]);
     }
-    back_f(ntile - 1, (ntile - 1) & 1);
+    back_f(nl - 1, (nl - 1) & 1);
+  }
+  // the meeting cells: each side's value from its own (s', c') and the other
+  // side's (lanes l and l ^ 32 hold the two sides of one row)
+  double sn = sp;
+  if (wave == 0 && nr > 0) {
+    const double so = __shfl_xor(sp, 32), co = __shfl_xor(cp, 32);
+    sn = (sp - cp * so) / (1. - cp * co);
   }
 
-  // ---- back substitution, tiles in reverse (iteration x takes tile
-  // ntile-1-x); s = (0 + sn - 0) * bjmat.  Each mover lane reloads the
-  // elements it wrote itself, so program order makes them visible.
+  // ---- back substitution, tiles in reverse (iteration y takes tile
+  // nl-1-y of each side); s = (0 + sn - 0) * bjmat.  Each mover lane
+  // reloads the elements it wrote itself, so program order makes them
+  // visible.
   auto fetch_b = [&](int t, double (&b)[SAJ_PER], double (&c)[SAJ_PER], double (&s)[SAJ_PER]) {
 #pragma unroll
     for (int p = 0; p < SAJ_PER; ++p) {
@@ -1054,10 +1186,8 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
       if (ok) sS[i] = ts[slot][r][cc];
     }
   };
-  double sn = 0.0;
-  auto solve_b = [&](int t, int slot) {
-    if (l >= nrow) return;
-    const int c0 = t * SA_TJ, cn = n - c0 < SA_TJ ? n - c0 : SA_TJ;
+  auto solve_b = [&](int x, int slot) {
+    if (rr >= nrow || x >= ntl) return;
     double (*B)[SA_TJ + 1] = tb[slot];
     double (*Cc)[SA_TJ + 1] = tc[slot];
     double (*S)[SA_TJ + 1] = ts[slot];
@@ -1065,13 +1195,12 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
       sn = S[l][cc] - Cc[l][cc] * sn;
       S[l][cc] = (0. + sn - 0.) * B[l][cc];
     };
-    if (t == ntile - 1) {
-      // the line's last cell starts the recurrence
-      sn = S[l][cn - 1];
-      S[l][cn - 1] = (0. + sn - 0.) * B[l][cn - 1];
+    if (x == ntl - 1) {
+      // the side's meeting cell starts the substitution
+      S[l][lastp] = (0. + sn - 0.) * B[l][lastp];
 #pragma unroll
       for (int cc = SA_TJ - 2; cc >= 0; --cc)
-        if (cc < cn - 1) elem(cc);
+        if (cc < lastp) elem(cc);
     } else {
       // chunks of THOMAS_CH cells from the tile's end, reads a chunk ahead
       constexpr int CH = THOMAS_CH;
@@ -1096,19 +1225,19 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     }
   };
   __syncthreads();
-  // iteration x takes tile ntile-1-x; register set x % SAJ_D, tiles past
-  // the line's start clamped to tile 0
-  auto rtile = [&](int x) { return x < ntile ? ntile - 1 - x : 0; };
-  auto step_b = [&](int x, double (&b)[SAJ_PER], double (&c)[SAJ_PER], double (&s)[SAJ_PER]) {
-    if (x >= 1) back_b(ntile - x, (x - 1) & 1);
-    if (x + 1 < ntile) put_b((x + 1) & 1, b, c, s);
-    fetch_b(rtile(x + 1 + SAJ_D), b, c, s);
+  // iteration y takes tile nl-1-y; register set y % SAJ_D, tiles past the
+  // sides' starts clamped to tile 0
+  auto rtile = [&](int y) { return y < nl ? nl - 1 - y : 0; };
+  auto step_b = [&](int y, double (&b)[SAJ_PER], double (&c)[SAJ_PER], double (&s)[SAJ_PER]) {
+    if (y >= 1) back_b(nl - y, (y - 1) & 1);
+    if (y + 1 < nl) put_b((y + 1) & 1, b, c, s);
+    fetch_b(rtile(y + 1 + SAJ_D), b, c, s);
     __syncthreads();
   };
   if (wave == 0) {
     __syncthreads();
-    for (int x = 0; x < ntile; ++x) {
-      solve_b(ntile - 1 - x, x & 1);
+    for (int y = 0; y < nl; ++y) {
+      solve_b(nl - 1 - y, y & 1);
       __syncthreads();
     }
   } else {
@@ -1117,20 +1246,20 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
     put_b(0, rb[0], rc[0], rs[0]);
     fetch_b(rtile(SAJ_D), rb[0], rc[0], rs[0]);
     __syncthreads();
-    int x0 = 0;
-    for (; x0 + SAJ_D <= ntile; x0 += SAJ_D) {
+    int y0 = 0;
+    for (; y0 + SAJ_D <= nl; y0 += SAJ_D) {
 #pragma unroll
       for (int d = 0; d < SAJ_D; ++d) {
         const int d1 = (d + 1) % SAJ_D;
-        step_b(x0 + d, rb[d1], rc[d1], rs[d1]);
+        step_b(y0 + d, rb[d1], rc[d1], rs[d1]);
       }
     }
 #pragma unroll
     for (int d = 0; d < SAJ_D; ++d) {
       const int d1 = (d + 1) % SAJ_D;
-      if (x0 + d < ntile) step_b(x0 + d, rb[d1], rc[d1], rs[d1]);
+      if (y0 + d < nl) step_b(y0 + d, rb[d1], rc[d1], rs[d1]);
     }
-    back_b(0, (ntile - 1) & 1);
+    back_b(0, (nl - 1) & 1);
   }
 }
 
@@ -1139,14 +1268,14 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
 //   JL = 0: lines along k (one per j column), lanes on consecutive columns,
 //           so every step reads one contiguous row segment; the nu-tilde
 //           update follows in k_sa_update.
-//   JL = 1: lines along j (one per k row); the lanes' rows are jd apart,
-//           but each lane's next cells lie in lines it has already read.
-//           The forward pass leaves c' and s' in place (as lsolvej does),
-//           the backward pass writes s = (s1 - s2) * bjmat.
+//   JL = 1: lines along j (one per k row) with the one-sided recurrence;
+//           not launched (the j lines are eliminated from both ends, in
+//           k_sa_lines_j and k_sa_solve_j).
 constexpr int SA_PK = 16;
 
 template <int JL>
 __global__ __launch_bounds__(64) void k_sa_lines(const DevMesh* __restrict__ Ms) {
+  static_assert(JL == 0, "lines along j run in k_sa_lines_j");
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
@@ -1365,7 +1494,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   long long lines = 0;
   for (int i = 0; i < nmi; ++i) lines += (hM[i].ke - hM[i].ks + 1) + (hM[i].je - hM[i].js + 1);
   const bool per_lane = sa_lane < 0 ? lines > SA_LANE_LINES : sa_lane == 1;
-  if (per_lane) hipLaunchKernelGGL(k_sa_lines_j, dim3((mk_ + 63) / 64, nmi), dim3(SAJ_THREADS), 0, st, dM);
+  if (per_lane) hipLaunchKernelGGL(k_sa_lines_j, dim3((mk_ + 31) / 32, nmi), dim3(SAJ_THREADS), 0, st, dM);
   else hipLaunchKernelGGL(k_sa_solve_j, dim3(mk_, nmi), dim3(64), 0, st, dM);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_J, st);
