@@ -731,22 +731,24 @@ __global__ __launch_bounds__(256) void k_sa_rhs(const DevMesh* __restrict__ Ms) 
 // give bit-identical results; against the one-sided recurrence the results
 // differ in rounding only.
 constexpr int SAJ_TILE = 32;
-__device__ __forceinline__ void saj_split(int n, int& nr, int& m) {
+__device__ __forceinline__ void sa_split(int n, int& nr, int& m) {
   const int ntile = (n + SAJ_TILE - 1) / SAJ_TILE;
   nr = ntile >= 2 ? ntile / 2 : 0;
   m = n - SAJ_TILE * nr;
 }
 
-// The two-ended elimination of one line in LDS, run by lanes h = 0 (left)
-// and h = 1 (right) of one wave (nr > 0), or by lane 0 alone (nr = 0).
-// s' and the solution go to os, c' / a' to oc.
+// The two-ended elimination of one line in LDS, run by two lanes of one
+// wave, h = 0 (left) and h = 1 (right), lane numbers XM apart (nr > 0), or
+// by the h = 0 lane alone (nr = 0).  Cell t of the line sits at index t*ST.
+// s' and the solution go to os (which may be ls), c' / a' to oc.
+template <int ST, int XM>
 __device__ __forceinline__ void thomas2_lds(const double* la, const double* lb, const double* lc, const double* ls,
                                             double* os, double* oc, int n, int h) {
   constexpr int CH = THOMAS_CH;
   int nr, m;
-  saj_split(n, nr, m);
+  sa_split(n, nr, m);
   const int cnt = h ? n - m : m;
-  const int i0 = h ? n - 1 : 0, st = h ? -1 : 1;   // cell of step u: i0 + st*u
+  const int i0 = (h ? n - 1 : 0) * ST, st = (h ? -1 : 1) * ST;   // index of step u: i0 + st*u
   const double* lo = h ? lc : la;
   const double* hi = h ? la : lc;
   double bb = 1. / lb[i0];
@@ -788,7 +790,7 @@ __device__ __forceinline__ void thomas2_lds(const double* la, const double* lb, 
   // the meeting cells
   double sn = sp;
   if (nr > 0) {
-    const double so = __shfl_xor(sp, 1), co = __shfl_xor(cp, 1);
+    const double so = __shfl_xor(sp, XM), co = __shfl_xor(cp, XM);
     sn = (sp - cp * so) / (1. - cp * co);
   }
   os[i0 + st * (cnt - 1)] = sn;
@@ -842,8 +844,8 @@ __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ M
   stage_line4(a, b, c, s, i0, 1, n, la, lb, lc, ls);
   __syncthreads();
   int nr, m;
-  saj_split(n, nr, m);
-  if (threadIdx.x < (nr > 0 ? 2u : 1u)) thomas2_lds(la, lb, lc, ls, lsp, lcp, n, (int)threadIdx.x);
+  sa_split(n, nr, m);
+  if (threadIdx.x < (nr > 0 ? 2u : 1u)) thomas2_lds<1, 1>(la, lb, lc, ls, lsp, lcp, n, (int)threadIdx.x);
   __syncthreads();
   // sjmat = (s1 - s2) * bjmat with s2 = 0
   for (int t = threadIdx.x; t < n; t += 64) s[i0 + t] = (0. + lsp[t] - 0.) * lb[t];
@@ -869,7 +871,9 @@ __global__ __launch_bounds__(64) void k_sa_solve_k(const DevMesh* __restrict__ M
   const double* __restrict__ b = M.sa_bb;
   stage_line4(a, b, c, s, i0, st, n, la, lb, lc, ls);
   __syncthreads();
-  if (threadIdx.x == 0) thomas_lds(la, lb, lc, ls, lsp, lcp, n);
+  int nr, m;
+  sa_split(n, nr, m);
+  if (threadIdx.x < (nr > 0 ? 2u : 1u)) thomas2_lds<1, 1>(la, lb, lc, ls, lsp, lcp, n, (int)threadIdx.x);
   __syncthreads();
   const double vnulim = 1.0e-20, relfac = 1.0;
   for (int t = threadIdx.x; t < n; t += 64) {
@@ -936,7 +940,14 @@ __global__ __launch_bounds__(32 * TJ) void k_sa_solve_kt(const DevMesh* __restri
     }
   }
   __syncthreads();
-  if (tid < TJ) thomas_lds<TJ>(la + tid, lb + tid, lc + tid, ls + tid, ls + tid, lcp + tid, n);
+  {
+    // column tid & 31 of the tile, side tid >> 5 (lanes 0..TJ-1 and 32..32+TJ-1 of wave 0)
+    int nr, m;
+    sa_split(n, nr, m);
+    const int cl = tid & 31, sd = tid >> 5;
+    if (cl < TJ && (sd == 0 || (sd == 1 && nr > 0)))
+      thomas2_lds<TJ, 32>(la + cl, lb + cl, lc + cl, ls + cl, ls + cl, lcp + cl, n, sd);
+  }
   __syncthreads();
   // the update and c_turm, U cells per thread with all their loads issued
   // before the first store
@@ -997,7 +1008,7 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
   const int nrow = ke - k0 + 1 < 32 ? ke - k0 + 1 : 32;
   const int n = je - js + 1;
   int nr, m;
-  saj_split(n, nr, m);
+  sa_split(n, nr, m);
   const int nl = (m + SA_TJ - 1) / SA_TJ;   // tiles of the left side (nl >= nr)
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, ml = (int)threadIdx.x - 64;
   const int h = l >> 5, rr = l & 31;         // the solver lane's side and row
@@ -1263,14 +1274,13 @@ __global__ __launch_bounds__(SAJ_THREADS) void k_sa_lines_j(const DevMesh* __res
   }
 }
 
-// k_sa_lines<JL>: one lane per line, each lane walking its line with the
-// coefficients of the next SA_PK cells loaded ahead into registers.
-//   JL = 0: lines along k (one per j column), lanes on consecutive columns,
-//           so every step reads one contiguous row segment; the nu-tilde
-//           update follows in k_sa_update.
-//   JL = 1: lines along j (one per k row) with the one-sided recurrence;
-//           not launched (the j lines are eliminated from both ends, in
-//           k_sa_lines_j and k_sa_solve_j).
+// k_sa_lines<0>: lines along k (one per j column), one lane per side of a
+// line: 32 columns per workgroup, lane l = 32h + c walking side h of column
+// c's two-ended elimination (h = 0 up from ks, h = 1 down from ke; the same
+// operations as thomas2_lds), with the coefficients of its next SA_PK cells
+// loaded ahead into registers.  Lanes on consecutive columns, so every step
+// reads one contiguous row segment; the nu-tilde update follows in
+// k_sa_update.
 constexpr int SA_PK = 16;
 
 template <int JL>
@@ -1279,96 +1289,107 @@ __global__ __launch_bounds__(64) void k_sa_lines(const DevMesh* __restrict__ Ms)
   const DevMesh& M = Ms[blockIdx.y];
   if (*M.status) return;
   const int js = M.js, je = M.je, ks = M.ks, ke = M.ke;
-  const int l0 = JL ? ks : js, l1 = JL ? ke : je;   // line index range
-  if (l0 + 64 * (int)blockIdx.x > l1) return;
-  const int line = l0 + 64 * blockIdx.x + threadIdx.x;
-  const bool lane_ok = line <= l1;
-  const int ll = lane_ok ? line : l1;
-  const int n = JL ? je - js + 1 : ke - ks + 1;
-  const long long st = JL ? 1 : M.jd;
-  const long long i0 = JL ? cell(M, js, ll) : cell(M, ll, ks);
+  if (js + 32 * (int)blockIdx.x > je) return;
+  const int h = (int)threadIdx.x >> 5;
+  const int line = js + 32 * blockIdx.x + (threadIdx.x & 31);
+  const bool lane_ok = line <= je;
+  const int ll = lane_ok ? line : je;
+  const int n = ke - ks + 1;
+  int nr, m;
+  sa_split(n, nr, m);
+  // this lane's side: cnt cells, step u at base + u*step (both in elements);
+  // a side with no cells (h = 1, nr = 0) walks the left side's cells and
+  // stores nothing
+  const bool side_ok = h == 0 || nr > 0;
+  const bool st_ok = lane_ok && side_ok;
+  const int hs = side_ok ? h : 0;
+  const int cnt = hs ? n - m : m;
+  const long long st = M.jd;
+  const long long base = cell(M, ll, ks) + (hs ? (long long)(n - 1) * st : 0);
+  const long long step = hs ? -st : st;
   // global (not generic) pointers: the compiler then counts each load's
   // completion exactly and the loads ahead stay in flight
   gdouble* __restrict__ sS = gptr(M.sa_sj);
-  gdouble* __restrict__ sC = gptr(JL ? M.sa_cj : M.sa_ck);
-  const gdouble* __restrict__ sA = gptr(JL ? M.sa_aj : M.sa_ak);
+  gdouble* __restrict__ sC = gptr(M.sa_ck);
+  const gdouble* __restrict__ sA = gptr(M.sa_ak);
   const gdouble* __restrict__ sB = gptr(M.sa_bb);
-  // Loads ahead are unconditional (indices clamped to the line) and the
-  // unrolled loops run whole chunks, the remainder separately, so no loaded
-  // value merges across a branch: the compiler then waits for each value
-  // exactly where it is used.  Slot d of a prefetch array holds the cell
-  // whose index is congruent to d modulo SA_PK.
-  auto at = [&](int t) { return i0 + (long long)(t < n ? t : n - 1) * st; };
+  // lower and upper coefficients of the side: (a, c) going up, (c, a) down;
+  // c' / a' goes where c was
+  const gdouble* __restrict__ sLo = hs ? (const gdouble*)sC : sA;
+  const gdouble* __restrict__ sHi = hs ? sA : (const gdouble*)sC;
+  // Loads ahead are unconditional (steps clamped to the side) and the
+  // unrolled loops run whole chunks, the remainder separately.  Slot d of a
+  // prefetch array holds the step congruent to d modulo SA_PK.
+  auto at = [&](int u) { return base + (long long)(u < cnt ? u : cnt - 1) * step; };
   double pa[SA_PK], pb[SA_PK], pc[SA_PK], ps[SA_PK];
 #pragma unroll
   for (int d = 0; d < SA_PK; ++d) {
     const long long i = at(d);
-    pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i];
+    pa[d] = sLo[i]; pb[d] = sB[i]; pc[d] = sHi[i]; ps[d] = sS[i];
   }
   double cp, sp;
   {
-    // first cell (its a is not used)
+    // first cell of the side (its lower coefficient is not used)
     const double b = pb[0], c = pc[0], s = ps[0];
     const long long i = at(SA_PK);
-    pa[0] = sA[i]; pb[0] = sB[i]; pc[0] = sC[i]; ps[0] = sS[i];
+    pa[0] = sLo[i]; pb[0] = sB[i]; pc[0] = sHi[i]; ps[0] = sS[i];
     double bb = 1. / b;
     sp = s * bb; cp = c * bb;
-    if (lane_ok) { sS[i0] = sp; sC[i0] = cp; }
+    if (st_ok) { sS[base] = sp; sC[base] = cp; }
   }
-  auto fwd = [&](int t, int d) {
+  auto fwd = [&](int u, int d) {
     const double a = pa[d], b = pb[d], c = pc[d], s = ps[d];
-    const long long i = at(t + SA_PK);
-    pa[d] = sA[i]; pb[d] = sB[i]; pc[d] = sC[i]; ps[d] = sS[i];
+    const long long i = at(u + SA_PK);
+    pa[d] = sLo[i]; pb[d] = sB[i]; pc[d] = sHi[i]; ps[d] = sS[i];
     double bb = 1. / (b - a * cp);
     sp = (s - a * sp) * bb;
     cp = c * bb;
-    if (lane_ok) { const long long o = i0 + (long long)t * st; sS[o] = sp; sC[o] = cp; }
+    if (st_ok) { const long long o = base + (long long)u * step; sS[o] = sp; sC[o] = cp; }
   };
   int tb = 1;
-  for (; tb + SA_PK <= n; tb += SA_PK) {
+  for (; tb + SA_PK <= cnt; tb += SA_PK) {
 #pragma unroll
     for (int e = 0; e < SA_PK; ++e) fwd(tb + e, (1 + e) % SA_PK);
   }
 #pragma unroll
   for (int e = 0; e < SA_PK; ++e)
-    if (tb + e < n) fwd(tb + e, (1 + e) % SA_PK);
-  // back substitution, cells in reverse (the values this lane just wrote),
-  // the next cells loaded ahead; slot d holds u = n-1-t congruent to d
-  auto atb = [&](int u) { return i0 + (long long)(u < n ? n - 1 - u : 0) * st; };
-  double qs[SA_PK], qc[SA_PK], qb[SA_PK];
+    if (tb + e < cnt) fwd(tb + e, (1 + e) % SA_PK);
+  // the meeting cells (lanes l and l ^ 32 hold the two sides of a column)
+  double sn = sp;
+  if (nr > 0) {
+    const double so = __shfl_xor(sp, 32), co = __shfl_xor(cp, 32);
+    sn = (sp - cp * so) / (1. - cp * co);
+  }
+  // back substitution towards the side's start (the values this lane just
+  // wrote), the next cells loaded ahead; slot d holds back-step v congruent
+  // to d (back-step v is step cnt-1-v); dq = s2 goes back into sjmat
+  auto atb = [&](int v) { return base + (long long)(v < cnt ? cnt - 1 - v : 0) * step; };
+  double qs[SA_PK], qc[SA_PK];
 #pragma unroll
   for (int d = 0; d < SA_PK; ++d) {
     const long long i = atb(d);
     qs[d] = sS[i]; qc[d] = sC[i];
-    if (JL) qb[d] = sB[i];
   }
-  double sn = sp;
   {
-    // last cell: sn is its s'
-    const double b = JL ? qb[0] : 0.0;
     const long long i = atb(SA_PK);
     qs[0] = sS[i]; qc[0] = sC[i];
-    if (JL) qb[0] = sB[i];
-    // JL: sjmat = (s1 - s2) * bjmat with s2 = 0; k lines: dq = s2 goes
-    // back into sjmat
-    if (lane_ok) sS[i0 + (long long)(n - 1) * st] = JL ? (0. + sn - 0.) * b : sn;
+    if (st_ok) sS[base + (long long)(cnt - 1) * step] = sn;
   }
-  auto bwd = [&](int u, int d) {
-    const double s = qs[d], c = qc[d], b = JL ? qb[d] : 0.0;
-    const long long i = atb(u + SA_PK);
+  auto bwd = [&](int v, int d) {
+    const double s = qs[d], c = qc[d];
+    const long long i = atb(v + SA_PK);
     qs[d] = sS[i]; qc[d] = sC[i];
-    if (JL) qb[d] = sB[i];
     sn = s - c * sn;
-    if (lane_ok) sS[i0 + (long long)(n - 1 - u) * st] = JL ? (0. + sn - 0.) * b : sn;
+    if (st_ok) sS[base + (long long)(cnt - 1 - v) * step] = sn;
   };
   int ub = 1;
-  for (; ub + SA_PK <= n; ub += SA_PK) {
+  for (; ub + SA_PK <= cnt; ub += SA_PK) {
 #pragma unroll
     for (int e = 0; e < SA_PK; ++e) bwd(ub + e, (1 + e) % SA_PK);
   }
 #pragma unroll
   for (int e = 0; e < SA_PK; ++e)
-    if (ub + e < n) bwd(ub + e, (1 + e) % SA_PK);
+    if (ub + e < cnt) bwd(ub + e, (1 + e) % SA_PK);
 }
 
 // nu-tilde update with its clip (src/samodel2d.f:157-188) and the interior
@@ -1499,7 +1520,7 @@ hipError_t launch_sa(const DevMesh* dM, const DevMesh* hM, int nmi, Marks* mk, h
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark(mk, OT2D_ST_SA_SOLVE_J, st);
   if (per_lane) {
-    hipLaunchKernelGGL(k_sa_lines<0>, dim3((mj + 63) / 64, nmi), dim3(64), 0, st, dM);
+    hipLaunchKernelGGL(k_sa_lines<0>, dim3((mj + 31) / 32, nmi), dim3(64), 0, st, dM);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sa_update, dim3(nblk2(mx, 256), nmi), dim3(256), 0, st, dM, ovl);
   } else {
