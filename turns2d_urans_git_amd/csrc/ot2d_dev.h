@@ -215,6 +215,110 @@ __device__ __forceinline__ void thomas_lds(const double* la, const double* lb, c
   runb(zs, zc, b - CH, b - CH + 1);
 }
 
+// Two-ended elimination of a tridiagonal line (the turbulence models' line
+// solves): with 32-cell tiles, ntile = ceil(n/32) and nr = floor(ntile/2)
+// (0 for ntile = 1), the right part is the last 32*nr cells and the left
+// part the first m = n - 32*nr:
+//   left  (t = 0 .. m-1, up):   bb = 1/(b - a c'),  s' = (s - a s') bb,  c' = c bb
+//   right (t = n-1 .. m, down): bb = 1/(b - c a'),  s' = (s - c s') bb,  a' = a bb
+// (the first cell of each side: bb = 1/b); at the meeting cells each side
+// takes x = (s' - c' s'_other) / (1 - c' c'_other) and substitutes back
+// towards its end, x = s' - c' x_next.  nr = 0 is the one-sided recurrence.
+constexpr int SAJ_TILE = 32;
+__device__ __forceinline__ void sa_split(int n, int& nr, int& m) {
+  const int ntile = (n + SAJ_TILE - 1) / SAJ_TILE;
+  nr = ntile >= 2 ? ntile / 2 : 0;
+  m = n - SAJ_TILE * nr;
+}
+
+// The two-ended elimination of one line in LDS, run by two lanes of one
+// wave, h = 0 (left) and h = 1 (right), lane numbers XM apart (nr > 0), or
+// by the h = 0 lane alone (nr = 0).  Cell t of the line sits at index t*ST.
+// s' and the solution go to os (which may be ls), c' / a' to oc.
+template <int ST, int XM>
+__device__ __forceinline__ void thomas2_lds(const double* la, const double* lb, const double* lc, const double* ls,
+                                            double* os, double* oc, int n, int h) {
+  constexpr int CH = THOMAS_CH;
+  int nr, m;
+  sa_split(n, nr, m);
+  const int cnt = h ? n - m : m;
+  const int i0 = (h ? n - 1 : 0) * ST, st = (h ? -1 : 1) * ST;   // index of step u: i0 + st*u
+  const double* lo = h ? lc : la;
+  const double* hi = h ? la : lc;
+  double bb = 1. / lb[i0];
+  double sp = ls[i0] * bb, cp = hi[i0] * bb;
+  os[i0] = sp; oc[i0] = cp;
+  double pa[CH], pb[CH], pc[CH], ps[CH], qa[CH], qb[CH], qc[CH], qs[CH];
+  auto ld = [&](double (&xa)[CH], double (&xb)[CH], double (&xc)[CH], double (&xs)[CH], int u0) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int i = i0 + st * (u0 + u < cnt ? u0 + u : cnt - 1);
+      xa[u] = lo[i]; xb[u] = lb[i]; xc[u] = hi[i]; xs[u] = ls[i];
+    }
+  };
+  auto run = [&](const double (&xa)[CH], const double (&xb)[CH], const double (&xc)[CH], const double (&xs)[CH],
+                 int u0, int k) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (u < k) {
+        const double ai = xa[u];
+        bb = 1. / (xb[u] - ai * cp);
+        sp = (xs[u] - ai * sp) * bb;
+        cp = xc[u] * bb;
+        const int i = i0 + st * (u0 + u);
+        os[i] = sp; oc[i] = cp;
+      }
+    }
+  };
+  int u = 1;
+  ld(pa, pb, pc, ps, u);
+  for (; u + 2 * CH <= cnt; u += 2 * CH) {
+    ld(qa, qb, qc, qs, u + CH);
+    run(pa, pb, pc, ps, u, CH);
+    ld(pa, pb, pc, ps, u + 2 * CH);
+    run(qa, qb, qc, qs, u + CH, CH);
+  }
+  ld(qa, qb, qc, qs, u + CH);
+  run(pa, pb, pc, ps, u, cnt - u);
+  run(qa, qb, qc, qs, u + CH, cnt - u - CH);
+  // the meeting cells
+  double sn = sp;
+  if (nr > 0) {
+    const double so = __shfl_xor(sp, XM), co = __shfl_xor(cp, XM);
+    sn = (sp - cp * so) / (1. - cp * co);
+  }
+  os[i0 + st * (cnt - 1)] = sn;
+  // back substitution towards this side's end, chunks of CH steps
+  double ys[CH], yc[CH], zs[CH], zc[CH];
+  auto ldb = [&](double (&xs)[CH], double (&xc)[CH], int u1) {
+#pragma unroll
+    for (int v = 0; v < CH; ++v) {
+      const int i = i0 + st * (u1 - v >= 0 ? u1 - v : 0);
+      xs[v] = os[i]; xc[v] = oc[i];
+    }
+  };
+  auto runb = [&](const double (&xs)[CH], const double (&xc)[CH], int u1, int k) {
+#pragma unroll
+    for (int v = 0; v < CH; ++v) {
+      if (v < k) {
+        sn = xs[v] - xc[v] * sn;
+        os[i0 + st * (u1 - v)] = sn;
+      }
+    }
+  };
+  int b = cnt - 2;
+  ldb(ys, yc, b);
+  for (; b - 2 * CH + 1 >= 0; b -= 2 * CH) {
+    ldb(zs, zc, b - CH);
+    runb(ys, yc, b, CH);
+    ldb(ys, yc, b - 2 * CH);
+    runb(zs, zc, b - CH, CH);
+  }
+  ldb(zs, zc, b - CH);
+  runb(ys, yc, b, b + 1);
+  runb(zs, zc, b - CH, b - CH + 1);
+}
+
 constexpr int RES_TJ = 32;
 constexpr int RES_TK = 8;
 constexpr int RES_THREADS = RES_TJ * RES_TK;

@@ -386,12 +386,15 @@ __global__ __launch_bounds__(256) void k_sst_rhs(const DevMesh* __restrict__ Ms)
   }
 }
 
-// Thomas recurrence of one line held in LDS (lane 0), the reference's
-// arithmetic (lsolvej2 / lsolvek2); on return ls holds the solution
+// lsolvej2 / lsolvek2 of one line held in LDS, eliminated from both ends
+// (lanes 0 and 1; the reference's one-sided recurrence for lines of up to
+// 32 cells); on return ls holds the solution
 __device__ __forceinline__ void kw_line_solve(const double* la, const double* lb, const double* lc, double* ls,
                                               double* lcp, int n) {
-  if (threadIdx.x != 0) return;
-  thomas_lds(la, lb, lc, ls, ls, lcp, n);
+  int nr, m;
+  sa_split(n, nr, m);
+  if (threadIdx.x >= (nr > 0 ? 2u : 1u)) return;
+  thomas2_lds<1, 1>(la, lb, lc, ls, ls, lcp, n, (int)threadIdx.x);
 }
 
 // lsolvej2 for k row blockIdx.x / 2, equation blockIdx.x % 2, then

@@ -709,122 +709,16 @@ __global__ __launch_bounds__(256) void k_sa_rhs(const DevMesh* __restrict__ Ms) 
 }
 
 // The two scalar tridiagonal solves of vmu_sa are sequential recurrences
-// along each grid line.  One wave owns one line: all lanes stage the line's
-// coefficients in LDS (stage_line4: every load in flight at once, coalesced
-// along j, gathered along k), lanes run the recurrence out of LDS (thomas_lds:
-// its reads run a chunk ahead of the dependent chain), and all lanes write the
-// results.  Along k the arithmetic is the reference's Thomas recurrence
-// (lsolvek), operation for operation.
-//
-// Along j (lsolvej) the line is eliminated from both ends at once, which
-// halves the dependent chain.  With 32-cell tiles, ntile = ceil(n/32) and
-// nr = floor(ntile/2) (0 for ntile = 1), the right part is the last 32*nr
-// cells and the left part the first m = n - 32*nr:
-//   left  (t = 0 .. m-1, up):   bb = 1/(b - a c'),  s' = (s - a s') bb,  c' = c bb
-//   right (t = n-1 .. m, down): bb = 1/(b - c a'),  s' = (s - c s') bb,  a' = a bb
-// (the first cell of each side: bb = 1/b), then at the meeting cells each side
-// takes its own value from its (s', c') and the other side's:
-//   x = (s' - c' s'_other) / (1 - c' c'_other)
-// and substitutes back towards its end: x = s' - c' x_next.  The reference's
-// recurrence is the case nr = 0.  Both j kernels (one wave per line, one
-// lane per line) evaluate exactly these operations in this order, so they
-// give bit-identical results; against the one-sided recurrence the results
-// differ in rounding only.
-constexpr int SAJ_TILE = 32;
-__device__ __forceinline__ void sa_split(int n, int& nr, int& m) {
-  const int ntile = (n + SAJ_TILE - 1) / SAJ_TILE;
-  nr = ntile >= 2 ? ntile / 2 : 0;
-  m = n - SAJ_TILE * nr;
-}
-
-// The two-ended elimination of one line in LDS, run by two lanes of one
-// wave, h = 0 (left) and h = 1 (right), lane numbers XM apart (nr > 0), or
-// by the h = 0 lane alone (nr = 0).  Cell t of the line sits at index t*ST.
-// s' and the solution go to os (which may be ls), c' / a' to oc.
-template <int ST, int XM>
-__device__ __forceinline__ void thomas2_lds(const double* la, const double* lb, const double* lc, const double* ls,
-                                            double* os, double* oc, int n, int h) {
-  constexpr int CH = THOMAS_CH;
-  int nr, m;
-  sa_split(n, nr, m);
-  const int cnt = h ? n - m : m;
-  const int i0 = (h ? n - 1 : 0) * ST, st = (h ? -1 : 1) * ST;   // index of step u: i0 + st*u
-  const double* lo = h ? lc : la;
-  const double* hi = h ? la : lc;
-  double bb = 1. / lb[i0];
-  double sp = ls[i0] * bb, cp = hi[i0] * bb;
-  os[i0] = sp; oc[i0] = cp;
-  double pa[CH], pb[CH], pc[CH], ps[CH], qa[CH], qb[CH], qc[CH], qs[CH];
-  auto ld = [&](double (&xa)[CH], double (&xb)[CH], double (&xc)[CH], double (&xs)[CH], int u0) {
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const int i = i0 + st * (u0 + u < cnt ? u0 + u : cnt - 1);
-      xa[u] = lo[i]; xb[u] = lb[i]; xc[u] = hi[i]; xs[u] = ls[i];
-    }
-  };
-  auto run = [&](const double (&xa)[CH], const double (&xb)[CH], const double (&xc)[CH], const double (&xs)[CH],
-                 int u0, int k) {
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      if (u < k) {
-        const double ai = xa[u];
-        bb = 1. / (xb[u] - ai * cp);
-        sp = (xs[u] - ai * sp) * bb;
-        cp = xc[u] * bb;
-        const int i = i0 + st * (u0 + u);
-        os[i] = sp; oc[i] = cp;
-      }
-    }
-  };
-  int u = 1;
-  ld(pa, pb, pc, ps, u);
-  for (; u + 2 * CH <= cnt; u += 2 * CH) {
-    ld(qa, qb, qc, qs, u + CH);
-    run(pa, pb, pc, ps, u, CH);
-    ld(pa, pb, pc, ps, u + 2 * CH);
-    run(qa, qb, qc, qs, u + CH, CH);
-  }
-  ld(qa, qb, qc, qs, u + CH);
-  run(pa, pb, pc, ps, u, cnt - u);
-  run(qa, qb, qc, qs, u + CH, cnt - u - CH);
-  // the meeting cells
-  double sn = sp;
-  if (nr > 0) {
-    const double so = __shfl_xor(sp, XM), co = __shfl_xor(cp, XM);
-    sn = (sp - cp * so) / (1. - cp * co);
-  }
-  os[i0 + st * (cnt - 1)] = sn;
-  // back substitution towards this side's end, chunks of CH steps
-  double ys[CH], yc[CH], zs[CH], zc[CH];
-  auto ldb = [&](double (&xs)[CH], double (&xc)[CH], int u1) {
-#pragma unroll
-    for (int v = 0; v < CH; ++v) {
-      const int i = i0 + st * (u1 - v >= 0 ? u1 - v : 0);
-      xs[v] = os[i]; xc[v] = oc[i];
-    }
-  };
-  auto runb = [&](const double (&xs)[CH], const double (&xc)[CH], int u1, int k) {
-#pragma unroll
-    for (int v = 0; v < CH; ++v) {
-      if (v < k) {
-        sn = xs[v] - xc[v] * sn;
-        os[i0 + st * (u1 - v)] = sn;
-      }
-    }
-  };
-  int b = cnt - 2;
-  ldb(ys, yc, b);
-  for (; b - 2 * CH + 1 >= 0; b -= 2 * CH) {
-    ldb(zs, zc, b - CH);
-    runb(ys, yc, b, CH);
-    ldb(ys, yc, b - 2 * CH);
-    runb(zs, zc, b - CH, CH);
-  }
-  ldb(zs, zc, b - CH);
-  runb(ys, yc, b, b + 1);
-  runb(zs, zc, b - CH, b - CH + 1);
-}
-
+// along each grid line, both eliminated from the line's two ends at once
+// (thomas2_lds), which halves the dependent chain; lines of up to 32 cells
+// keep the reference's one-sided recurrence (lsolvej / lsolvek).  One wave
+// owns one line: all lanes stage the line's coefficients in LDS (stage_line4:
+// every load in flight at once, coalesced along j, gathered along k), two
+// lanes run the two sides out of LDS (reads a chunk ahead of the dependent
+// chain), and all lanes write the results.  The wave, tile and lane kernels
+// of a direction evaluate the same operations in the same order, so every
+// mapping gives bit-identical results; against the one-sided recurrence the
+// results differ in rounding only.
 // lsolvej (src/samodel2d.f:1306-1344), eliminated from both ends, and s1,
 // sjmat*bjmat of vmu_sa (src/samodel2d.f:148-152) for the k row blockIdx.x
 __global__ __launch_bounds__(64) void k_sa_solve_j(const DevMesh* __restrict__ Ms) {
