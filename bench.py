@@ -756,18 +756,20 @@ def main():
         cells = (g.jmax - 1) * (g.kmax - 1)
         case = make_gpu_case(w, g)
         print(f"[bench] {args.workload}: {g.jmax}x{g.kmax}, warmup {args.warmup}", file=sys.stderr, flush=True)
-    rc, _, done = case.iterate(args.warmup)
-    if rc != 0 or done != args.warmup:
-        raise SystemExit(f"warmup stopped: rc={rc} after {done} steps")
     launch = {"sweep": 0, "sa_lane": -1, "searched": False}
     if args.autotune > 0:
         # the launch choices (FP32 sweep ring and producers, SA line
-        # mapping) change speed, never results; the search puts the state back
+        # mapping) change speed, never results; the search puts the state
+        # back.  It runs before the warm-up, so the warm-up steps (with the
+        # chosen step graph) run right before the timed ones.
         ta = time.perf_counter()
         sw, sl, measured = case.autotune(args.autotune)
         launch = {"sweep": sw, "sa_lane": sl, "searched": measured,
                   "search_s": round(time.perf_counter() - ta, 3)}
         print(f"[bench] launch choices {launch}", file=sys.stderr, flush=True)
+    rc, _, done = case.iterate(args.warmup)
+    if rc != 0 or done != args.warmup:
+        raise SystemExit(f"warmup stopped: rc={rc} after {done} steps")
 
     device_sync(hip)
     t0 = time.perf_counter()

@@ -228,9 +228,10 @@ int32_t ot2d_get_launch(ot2d_case* c, int32_t* sweep, int32_t* sa_lane);
  * and the fastest is set on the case.  Results are cached per class in
  * the process and, when OT2D_TUNE_CACHE names a file, in that file (one line
  * per class, read at the first call, appended after each search); a cached
- * class is set without measuring.  *measured (may be NULL) is 1 when a
- * search ran, 0 on a cache hit.  Allocates a copy of the state while it
- * runs. */
+ * class is set without measuring.  Either way the chosen step is captured
+ * before the call returns, so the next ot2d_iterate starts at once.
+ * *measured (may be NULL) is 1 when a search ran, 0 on a cache hit.
+ * Allocates a copy of the state while it runs. */
 int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured);
 
 /* --------------------------------------------------------------- serving

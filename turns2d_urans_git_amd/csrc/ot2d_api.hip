@@ -88,6 +88,8 @@ struct ot2d_case {
   int* d_istep0 = nullptr;
   int* d_stepc = nullptr;
   int* d_istep = nullptr;      // time steps of this run per case (time-accurate steps)
+  int* h_small = nullptr;      // pinned host copy of d_stepc and d_status (2 * ncase ints)
+  double* h_hist = nullptr;    // pinned host copy of d_hist (hist_cap steps)
   int tacc = 0;                // time-accurate steps: stored levels, Newton sub-iterations, motion
   int moving = 0;              // the grid pitches every time step (iunst = 1)
   // SA cases: the turbulence model runs on a second stream beside ilu2d's
@@ -380,6 +382,8 @@ static void free_case(ot2d_case* c) {
   }
   dfree(c->dMa); dfree(c->dTa); dfree(c->d_istep); dfree(c->d_itnorms);
   dfree(c->dM); dfree(c->dT); dfree(c->d_status); dfree(c->d_istep0); dfree(c->d_stepc);
+  if (c->h_small) (void)hipHostFree(c->h_small);
+  if (c->h_hist) (void)hipHostFree(c->h_hist);
   dfree(c->d_cinst); dfree(c->d_ctol);
   dfree(c->d_partials); dfree(c->d_norms); dfree(c->d_tspec);
   dfree(c->d_Ds); dfree(c->d_hist); dfree(c->d_forces);
@@ -645,6 +649,11 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
     return fail(OT2D_ERR_UNSUPPORTED, "moving overset meshes (connectivity re-cut every step)");
   }
   A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase); A(c->d_istep, ncase);
+  if ((e = hipHostMalloc((void**)&c->h_small, sizeof(int) * 2 * (size_t)ncase, hipHostMallocDefault)) != hipSuccess) {
+    c->h_small = nullptr;
+    free_case(c);
+    return fail(OT2D_ERR_NOMEM, "pinned host allocation failed: %s", hipGetErrorString(e));
+  }
   if (c->tacc) A(c->d_itnorms, 6 * (size_t)nmi * c->itnmax);
   A(c->d_cinst, 2 * (size_t)ncase); A(c->d_ctol, ncase);
   A(c->d_norms, 6 * nmi); A(c->d_tspec, (size_t)ns * ncase); A(c->d_Ds, ns * ns);
@@ -1234,13 +1243,23 @@ static int32_t read_status(ot2d_case* c, int* st) {
   return OT2D_OK;
 }
 
-// steps done by the furthest case in the current call
-static int32_t read_steps(ot2d_case* c, int* steps) {
-  std::vector<int> all(c->ncase);
-  HIPCHK(hipMemcpyAsync(all.data(), c->d_stepc, sizeof(int) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+// steps done by the furthest case in the current call and the status as
+// read_status gives it, with one synchronisation (both copies into the
+// case's pinned buffer)
+static int32_t read_steps_status(ot2d_case* c, int* steps, int* st) {
+  const size_t nb = sizeof(int) * (size_t)c->ncase;
+  HIPCHK(hipMemcpyAsync(c->h_small, c->d_stepc, nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_small + c->ncase, c->d_status, nb, hipMemcpyDeviceToHost, c->stream));
   if (int32_t rc = sync(c)) return rc;
   *steps = 0;
-  for (int v : all) *steps = v > *steps ? v : *steps;
+  for (int i = 0; i < c->ncase; ++i) *steps = c->h_small[i] > *steps ? c->h_small[i] : *steps;
+  if (!c->batch) {
+    *st = c->h_small[c->ncase];
+  } else {
+    *st = 0;
+    for (int i = 0; i < c->ncase; ++i)
+      if (c->h_small[c->ncase + i] == 3) *st = 3;
+  }
   return OT2D_OK;
 }
 
@@ -1306,8 +1325,15 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
   if (hist && niter > c->hist_cap) {
     dfree(c->d_hist);
     c->d_hist = nullptr;
+    if (c->h_hist) (void)hipHostFree(c->h_hist);
+    c->h_hist = nullptr;
+    c->hist_cap = 0;
     hipError_t e = dalloc(&c->d_hist, (size_t)niter * per);
-    if (e != hipSuccess) return fail(OT2D_ERR_NOMEM, "history buffer: %s", hipGetErrorString(e));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_hist, sizeof(double) * (size_t)niter * per, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      c->h_hist = nullptr;
+      return fail(OT2D_ERR_NOMEM, "history buffer: %s", hipGetErrorString(e));
+    }
     c->hist_cap = niter;
     grown = true;  // the allocator may hand back the old address: refresh the capacity too
   }
@@ -1329,10 +1355,13 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
     if (int32_t rc = ensure_graph(c)) return rc;
     for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
   }
+  // the history comes back with the counters, under one synchronisation
+  if (hist && niter > 0)
+    HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, sizeof(double) * (size_t)niter * per, hipMemcpyDeviceToHost,
+                          c->stream));
   int st = 0, steps = 0;
-  if (int32_t rc = read_steps(c, &steps)) return rc;
-  if (int32_t rc = read_status(c, &st)) return rc;
-  if (hist && steps > 0) HIPCHK(hipMemcpy(hist, c->d_hist, sizeof(double) * (size_t)steps * per, hipMemcpyDeviceToHost));
+  if (int32_t rc = read_steps_status(c, &steps, &st)) return rc;
+  if (hist && steps > 0) memcpy(hist, c->h_hist, sizeof(double) * (size_t)steps * per);
   c->istep0 += steps;
   if (done) *done = steps;
   if (st == 3) return fail(OT2D_ERR_HIP, "LU-SGS sweep pipeline stopped (wait limit)");
@@ -1567,7 +1596,7 @@ extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, 
         break;
       }
     }
-    if (rc == OT2D_OK && (rc = read_steps(c, &steps)) == OT2D_OK) rc = read_status(c, &st);
+    if (rc == OT2D_OK) rc = read_steps_status(c, &steps, &st);
   }
   for (int i = 0; i < created; ++i) (void)hipEventDestroy(mk.ev[i]);
   (void)hipEventDestroy(ev0);
@@ -1818,6 +1847,7 @@ extern "C" int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured)
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
   const std::string key = shape_class(c);
+  bool cached = false;
   {
     std::lock_guard<std::mutex> lk(g_tune.mu);
     tune_cache_load();
@@ -1825,9 +1855,14 @@ extern "C" int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured)
       if (it->first == key) {
         c->cfg = it->second;
         c->graph_valid = 0;
-        if (measured) *measured = 0;
-        return OT2D_OK;
+        cached = true;
+        break;
       }
+  }
+  if (cached) {
+    if (measured) *measured = 0;
+    // the chosen step is captured here, not inside the next ot2d_iterate
+    return ensure_graph(c);
   }
   std::vector<LaunchCfg> cand;
   const int nsw = c->p.lu_fp32 ? LU32_SWEEP_VARIANTS : 1;
@@ -1935,7 +1970,7 @@ extern "C" int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured)
     tune_cache_store(key, c->cfg);
   }
   if (measured) *measured = 1;
-  return OT2D_OK;
+  return ensure_graph(c);
 }
 
 // ============================================================ overset meshes
