@@ -76,6 +76,34 @@ def test_sa_fp32_lusgs():
 WENO_EULER = dict(H.EULER, ilim=4, fsmach=0.6)
 
 
+
+@pytest.mark.parametrize("jmax,kmax", [(35, 34), (65, 33), (67, 66), (99, 34)])
+def test_sa_line_split_edges_vs_oracle(jmax, kmax):
+    """Line lengths (jmax-1 along j, kmax-1 along k) at the edges of the
+    two-ended line solves' split: 32 cells (one-sided), 33 (a near side of
+    one cell), 34, 64, 65, 66 and 98 (uneven tiles).
+    Both mappings (one wave per line, one lane per side of a line) agree bit
+    for bit, and both stay within 1e-10 of the oracle's one-sided recurrence
+    over 30 steps."""
+    g = H.cgrid(jmax, kmax, ds_wall=1e-4)
+    print(f"[sa split] {jmax}x{kmax}", flush=True)
+    o = H.oracle_case(g, SA_DECK)
+    rc_o, h_o, n_o = o.iterate(30)
+    assert rc_o == 0 and n_o == 30
+    res = []
+    for sl in (0, 1):
+        d = H.gpu_case(g, SA_DECK)
+        d.set_launch(0, sl)
+        rc_d, h_d, n_d = d.iterate(30)
+        assert rc_d == 0 and n_d == 30
+        res.append((h_d.copy(), d.get_state(0), d.get_field(0, abi.F_TURMU)))
+        d.close()
+    for a, b, what in zip(res[0], res[1], ("history", "q", "turmu")):
+        np.testing.assert_array_equal(a, b, err_msg=f"{what}: wave-per-line against lane-per-side")
+    np.testing.assert_allclose(res[0][0], h_o, rtol=1e-10)
+    H.assert_cellwise(res[0][1], o.get_state(0), 1e-10, "q")
+    H.assert_cellwise(res[0][2], o.get_field(0, abi.F_TURMU), 1e-10, "turmu")
+
 @pytest.mark.parametrize("deck", ["euler", "sa", "sa_fp32", "first_order"])
 def test_weno5_larger_grid_vs_oracle(deck):
     """WENO5 reconstruction (ilim=4) on a grid with several residual tiles
