@@ -21,7 +21,8 @@
 //
 // Operand order follows the reference expressions (built with
 // -ffp-contract=off), so results agree with the reference to the rounding of
-// the device's sqrt/div/pow.
+// the device's sqrt/div/pow; the line solves of lines over 32 cells run from
+// both ends (thomas2_lds), which changes their rounding only.
 #include "ot2d_dev.h"
 
 namespace ot2d {
