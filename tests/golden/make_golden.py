@@ -191,6 +191,33 @@ def run_driver_overset_case():
     print("driver_overset_sst_97x25_61x61", sorted(files))
 
 
+# The reference program on a moving overset deck: the near-body SA C-grid
+# pitches (iunst = 1, motiontype = 1) inside the fixed Cartesian background,
+# and connect2d runs again after every step's motion
+# (src/Turns_overset.f:411-465).
+DRIVER_OVERSET_PITCH = dict(near=(97, 25), back=(61, 61), nsteps=6, nrest=4, npnorm=1,
+                            deck=dict(SA, fsmach=0.3, alfa=2.0, rey=3.0e6, iunst=1, motiontype=1, rf=1.0,
+                                      angmax=4.0, dt=0.1, iconstant_cfl=0))
+
+
+def driver_overset_pitch_deck():
+    c = DRIVER_OVERSET_PITCH
+    meshes = grid.airfoil_overset(near=c["near"], back=c["back"], ds_wall=1e-4)
+    return meshes, dict(c["deck"], nsteps=c["nsteps"], nrest=c["nrest"], npnorm=c["npnorm"])
+
+
+def run_driver_overset_pitch_case():
+    meshes, deck = driver_overset_pitch_deck()
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_overset_case(d, meshes, **deck)
+        with open(os.path.join(d, "inputs")) as fin:
+            subprocess.run([PROGRAM], cwd=d, stdin=fin, check=True, stdout=subprocess.DEVNULL,
+                           env=dict(os.environ, **ZERO_MALLOC))
+        files = _program_outputs(d, ["fort.71", "fort.72", "fort.11", "fort.8", "fort.9", "cp_TS1.dat", "SA.dat"])
+    np.savez_compressed(os.path.join(HERE, "driver_pitch_overset_sa_97x25_61x61.npz"), **files)
+    print("driver_pitch_overset_sa_97x25_61x61", sorted(files))
+
+
 # Time-accurate decks (fixed grids, iunst = 0): Newton sub-iterations with
 # the time term of src/time.f:174-209 and BDF2/BDF3 levels (stqol,
 # src/time.f:109-172); the reference program's output files.
@@ -296,6 +323,8 @@ if __name__ == "__main__":
     for name in DRIVER_TIME:
         if not only or name in only:
             run_driver_time_case(name)
+    if not only or "driver_pitch_overset_sa_97x25_61x61" in only:
+        run_driver_overset_pitch_case()
     for name, spec in OVERSET_CASES.items():
         if not only or name in only:
             run_overset_case(name, *spec)

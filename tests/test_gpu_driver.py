@@ -3,8 +3,9 @@ against the reference program itself.
 
 The fixtures hold the output files the reference program wrote for the same
 decks (tests/golden/make_golden.py): a steady SA case, a time-spectral SA
-case (N = 3 pitching instances) and an overset SST + WENO5 case (near-body
-C-grid in a Cartesian background).  ot2d_run reads the same case files and
+case (N = 3 pitching instances), an overset SST + WENO5 case (near-body
+C-grid in a Cartesian background), time-accurate and pitching decks, and a
+moving overset deck.  ot2d_run reads the same case files and
 must write the same files:
 * norms (fort.7<m>) and forces (fort.11): the integer columns exactly, the
   values to their printed precision (10 significant digits; FP32 LU-SGS:
@@ -187,6 +188,30 @@ def test_driver_overset_matches_reference_program():
         compare_unformatted(os.path.join(d, "fort.9"), z["file:fort.9"], 1e-12, nmesh=2, grid_file=True)
         compare_text(os.path.join(d, "cp_TS1.dat"), z["file:cp_TS1.dat"], 1e-8)
         compare_text(os.path.join(d, "komega.dat"), z["file:komega.dat"], 1e-8)
+
+
+@needs_driver
+def test_driver_moving_overset_matches_reference_program():
+    """A moving overset deck: the near-body SA C-grid pitches (iunst = 1,
+    motiontype = 1) inside the fixed Cartesian background, and the
+    connectivity is found again after every step's motion
+    (src/Turns_overset.f:411-465; ot2d_iterate re-cuts on the host with the
+    ihcbc rows of ot2d_connect).  Norms of both meshes, forces, restart and
+    grid files (the turned grid, the last cut's iblank), surface pressure and
+    SA.dat as the reference program writes them."""
+    import make_golden as mg
+    z = np.load(os.path.join(ROOT, "tests", "golden", "driver_pitch_overset_sa_97x25_61x61.npz"))
+    meshes, deck = mg.driver_overset_pitch_deck()
+    with tempfile.TemporaryDirectory() as d:
+        grid.write_overset_case(d, meshes, **deck)
+        run_driver(d)
+        compare_columns(os.path.join(d, "fort.71"), z["file:fort.71"], 1e-8, 1)
+        compare_columns(os.path.join(d, "fort.72"), z["file:fort.72"], 1e-8, 1)
+        compare_columns(os.path.join(d, "fort.11"), z["file:fort.11"], 1e-8, 1)
+        compare_unformatted(os.path.join(d, "fort.8"), z["file:fort.8"], 1e-9, nmesh=2)
+        compare_unformatted(os.path.join(d, "fort.9"), z["file:fort.9"], 1e-12, nmesh=2, grid_file=True)
+        compare_text(os.path.join(d, "cp_TS1.dat"), z["file:cp_TS1.dat"], 1e-8)
+        compare_text(os.path.join(d, "SA.dat"), z["file:SA.dat"], 1e-8)
 
 
 @needs_driver

@@ -92,6 +92,9 @@ struct ot2d_case {
   double* h_hist = nullptr;    // pinned host copy of d_hist (hist_cap steps)
   int tacc = 0;                // time-accurate steps: stored levels, Newton sub-iterations, motion
   int moving = 0;              // the grid pitches every time step (iunst = 1)
+  // the ihcbc rows of the last ot2d_connect: a moving overset case finds its
+  // connectivity again with them at every time step
+  std::vector<ot2d_ihcbc> ihcbc_saved;
   // SA cases: the turbulence model runs on a second stream beside ilu2d's
   // prep and sweeps (fork and join events), joining before k_lu_finish
   hipStream_t st2 = nullptr;
@@ -644,10 +647,6 @@ static int32_t create_impl(const ot2d_params* ps, int32_t ncase, int32_t nmesh, 
   c->itnmax = p->itnmax;
   c->moving = !p->timespectral && p->iunst == 1;
   c->tacc = p->itnmax > 1 || p->ntac > 1 || c->moving;
-  if (c->moving && nmesh > 1 && !(ncase > 1)) {
-    free_case(c);
-    return fail(OT2D_ERR_UNSUPPORTED, "moving overset meshes (connectivity re-cut every step)");
-  }
   A(c->d_status, ncase); A(c->d_istep0, ncase); A(c->d_stepc, ncase); A(c->d_istep, ncase);
   if ((e = hipHostMalloc((void**)&c->h_small, sizeof(int) * 2 * (size_t)ncase, hipHostMallocDefault)) != hipSuccess) {
     c->h_small = nullptr;
@@ -1164,16 +1163,25 @@ static int32_t enqueue_step(ot2d_case* c, Marks* mk = nullptr);
 // step opens, the time levels shift (stqol), then itnmax Newton
 // sub-iterations of time_step, step and monitor, each with the fringe
 // exchange of overset meshes; every sub-iteration's norms are kept
-static int32_t enqueue_time_step(ot2d_case* c, Marks* mk) {
+// part 0: the whole time step; part 1: its opening and the grid motion
+// only; part 2: the rest (a moving overset case finds its connectivity
+// between the two)
+static int32_t enqueue_time_step(ot2d_case* c, Marks* mk, int part = 0) {
   const StepView v = step_view(c);
-  HIPCHK(launch_open_step(c->d_status, c->d_istep0, c->d_stepc, c->d_istep, c->ncase, c->stream));
-  if (c->moving) {
-    // the grid turns, then its metrics and the SA wall distance follow
-    // (src/Turns_overset.f:368-391, src/samodel2d.f:83)
-    HIPCHK(launch_pitch(v.dM, v.hM, v.n, c->stream));
-    HIPCHK(launch_init_geometry(v.dM, v.n, c->stream, 1));
-    if (!c->lamin) HIPCHK(launch_sa_dist(v.dM, v.hM, v.n, c->stream));
+  if (part != 2) {
+    HIPCHK(launch_open_step(c->d_status, c->d_istep0, c->d_stepc, c->d_istep, c->ncase, c->stream));
+    if (c->moving) {
+      // the grid turns, then its metrics and the SA wall distance follow
+      // (src/Turns_overset.f:368-391, src/samodel2d.f:83); of an overset
+      // case's meshes only the first moves (ii = 1), the others keep their
+      // grid and metrics
+      const int nmv = (c->ncase == 1 && c->nmesh > 1) ? c->nspec : v.n;
+      HIPCHK(launch_pitch(v.dM, v.hM, nmv, c->stream));
+      HIPCHK(launch_init_geometry(v.dM, nmv, c->stream, 1));
+      if (!c->lamin) HIPCHK(launch_sa_dist(v.dM, v.hM, nmv, c->stream));
+    }
   }
+  if (part == 1) return OT2D_OK;
   if (c->itnmax > 1) HIPCHK(launch_stqol(v.dM, v.hM, v.n, c->stream));
   for (int itn = 1; itn <= c->itnmax; ++itn) {
     HIPCHK(launch_time_step(v.dM, v.hM, v.n, c->stream, nullptr, nullptr, nullptr, 0, itn));
@@ -1263,17 +1271,50 @@ static int32_t read_steps_status(ot2d_case* c, int* steps, int* st) {
   return OT2D_OK;
 }
 
+// A moving overset case (iunst = 1 with several meshes, after ot2d_connect)
+// finds its connectivity again after every step's motion, as the reference
+// does (src/Turns_overset.f:411-465): the hole cut runs on the host, so its
+// steps are enqueued one at a time around it instead of as a captured graph.
+static bool recut_active(const ot2d_case* c) {
+  return c->moving && c->nmesh > 1 && c->ncase == 1 && !c->ihcbc_saved.empty();
+}
+static int32_t connect_impl(ot2d_case* c, const ot2d_ihcbc* ihcbc, int32_t* nfringe_total);
+static int32_t run_recut_steps(ot2d_case* c, int32_t niter) {
+  for (int it = 0; it < niter; ++it) {
+    {
+      Fp32Order order(c);
+      HIPCHK(order.error());
+      if (int32_t rc = enqueue_time_step(c, nullptr, 1)) return rc;
+    }
+    if (int32_t rc = sync(c)) return rc;
+    int st = 0;
+    HIPCHK(hipMemcpy(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) break;   // a stopped case takes no further step
+    if (int32_t rc = connect_impl(c, c->ihcbc_saved.data(), nullptr)) return rc;
+    {
+      Fp32Order order(c);
+      HIPCHK(order.error());
+      if (int32_t rc = enqueue_time_step(c, nullptr, 2)) return rc;
+    }
+  }
+  return OT2D_OK;
+}
+
 extern "C" int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax) {
   if (!c) return fail(OT2D_ERR_ARG, "null case");
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  const bool recut = recut_active(c);
   {
     Fp32Order order(c);
     HIPCHK(order.error());
     if (int32_t rc = clear_status(c)) return rc;
     HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
-    if (int32_t rc = enqueue_step(c)) return rc;
+    if (!recut)
+      if (int32_t rc = enqueue_step(c)) return rc;
   }
+  if (recut)
+    if (int32_t rc = run_recut_steps(c, 1)) return rc;
   int st = 0;
   if (int32_t rc = read_status(c, &st)) return rc;
   std::vector<double> nb(6 * nmi(c));
@@ -1347,14 +1388,19 @@ extern "C" int32_t ot2d_iterate(ot2d_case* c, int32_t niter, double* hist, int32
     }
     if (int32_t rc = upload_meshes(c)) return rc;
   }
+  const bool recut = recut_active(c);
   {
     Fp32Order order(c);
     HIPCHK(order.error());
     if (int32_t rc = clear_status(c)) return rc;
     HIPCHK(hipMemsetAsync(c->d_stepc, 0, sizeof(int) * c->ncase, c->stream));
-    if (int32_t rc = ensure_graph(c)) return rc;
-    for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    if (!recut) {
+      if (int32_t rc = ensure_graph(c)) return rc;
+      for (int it = 0; it < niter; ++it) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    }
   }
+  if (recut)
+    if (int32_t rc = run_recut_steps(c, niter)) return rc;
   // the history comes back with the counters, under one synchronisation
   if (hist && niter > 0)
     HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, sizeof(double) * (size_t)niter * per, hipMemcpyDeviceToHost,
@@ -1422,6 +1468,9 @@ extern "C" int32_t ot2d_iterate_noalloc(ot2d_case* c, int32_t niter, double* his
   if (!c || niter < 0) return fail(OT2D_ERR_ARG, "bad arguments");
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  if (recut_active(c))
+    return fail(OT2D_ERR_UNSUPPORTED, "%s: a moving overset case re-cuts its connectivity on the host every time "
+                "step; step it with ot2d_step or ot2d_iterate", "ot2d_iterate_noalloc");
   const int per = nmi(c);
   if (hist && (long long)niter * per > (long long)OT2D_NOALLOC_HIST_MAX)
     return fail(OT2D_ERR_ARG,
@@ -1480,6 +1529,9 @@ extern "C" int32_t ot2d_iterate_async(ot2d_case* c, int32_t niter) {
   if (!c || niter < 0) return fail(OT2D_ERR_ARG, "bad arguments");
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  if (recut_active(c))
+    return fail(OT2D_ERR_UNSUPPORTED, "%s: a moving overset case re-cuts its connectivity on the host every time "
+                "step; step it with ot2d_step or ot2d_iterate", "ot2d_iterate_async");
   if (c->hM[0].hist != nullptr) {
     for (auto& M : c->hM) { M.hist = nullptr; M.hist_cap = 0; }
     if (int32_t rc = upload_meshes(c)) return rc;
@@ -1554,6 +1606,9 @@ extern "C" int32_t ot2d_profile(ot2d_case* c, int32_t nsteps, double* stage_ms, 
   if (!c || nsteps < 0 || !stage_ms) return fail(OT2D_ERR_ARG, "bad arguments");
   for (auto& mb : c->m)
     if (!mb.has_grid) return fail(OT2D_ERR_STATE, "ot2d_set_grid not called for every mesh");
+  if (recut_active(c))
+    return fail(OT2D_ERR_UNSUPPORTED, "%s: a moving overset case re-cuts its connectivity on the host every time "
+                "step; step it with ot2d_step or ot2d_iterate", "ot2d_profile");
   for (int i = 0; i < OT2D_NSTAGE; ++i) stage_ms[i] = 0.0;
   // the history pointer baked into the device meshes stays as it is; the
   // step counter restarts so that history slots are not overrun
@@ -1859,10 +1914,12 @@ extern "C" int32_t ot2d_autotune(ot2d_case* c, int32_t steps, int32_t* measured)
         break;
       }
   }
-  if (cached) {
+  if (cached || c->moving) {
+    // a moving grid is not searched (its steps would move the grid, which the
+    // search does not put back): it keeps the cached or default choices
     if (measured) *measured = 0;
     // the chosen step is captured here, not inside the next ot2d_iterate
-    return ensure_graph(c);
+    return recut_active(c) ? OT2D_OK : ensure_graph(c);
   }
   std::vector<LaunchCfg> cand;
   const int nsw = c->p.lu_fp32 ? LU32_SWEEP_VARIANTS : 1;
@@ -2138,6 +2195,15 @@ extern "C" int32_t ot2d_connect(ot2d_case* c, const ot2d_ihcbc* ihcbc, int32_t* 
   if (c->nmesh < 2) return fail(OT2D_ERR_ARG, "ot2d_connect: a single mesh has no overset partner");
   for (int im = 0; im < c->nmesh; ++im)
     if (!c->m[im].has_grid) return fail(OT2D_ERR_STATE, "mesh %d has no grid", im);
+  if (int32_t rc = sync(c)) return rc;
+  if (int32_t rc = connect_impl(c, ihcbc, nfringe_total)) return rc;
+  c->ihcbc_saved.assign(ihcbc, ihcbc + c->nmesh);
+  return OT2D_OK;
+}
+
+// the hole cut of every instance from the grids on the device (the case's
+// stream has finished)
+static int32_t connect_impl(ot2d_case* c, const ot2d_ihcbc* ihcbc, int32_t* nfringe_total) {
   int32_t total = 0;
   for (int nsp = 0; nsp < c->nspec; ++nsp) {
     std::vector<std::vector<double>> xv(c->nmesh), yv(c->nmesh), xc(c->nmesh), yc(c->nmesh), vol(c->nmesh);

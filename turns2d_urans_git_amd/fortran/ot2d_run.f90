@@ -108,7 +108,6 @@ program ot2d_run
   end if
   if (iunst /= 0 .and. .not. (iunst == 1 .and. motiontype == 1)) &
        stop 'ot2d_run: of moving grids, only pitching (iunst = 1, motiontype = 1) is supported'
-  if (iunst /= 0 .and. num_grids > 1) stop 'ot2d_run: moving overset meshes are not supported'
   if (invisc) lamin = .true.
   if (lamin) iturb = -1
   nmesh = num_grids
