@@ -155,6 +155,12 @@ int32_t ot2d_step(ot2d_case* c, double* norms, int32_t* argmax);
  * itnmax sub-iterations of time_step, step (with the Newton time term,
  * src/time.f:174-209, and the swept-area face speeds of the moving grid,
  * src/eulerrhs.f:118-136) and monitor.  The norms ot2d_step returns are the last sub-iteration's.
+ * A pitching overset case (several meshes, after ot2d_connect) moves its
+ * first mesh only, and ot2d_step / ot2d_iterate find its connectivity again
+ * on the host after every step's motion, with the ihcbc rows of the last
+ * ot2d_connect (src/Turns_overset.f:411-465); ot2d_iterate_async,
+ * ot2d_iterate_noalloc and ot2d_profile refuse such a case
+ * (OT2D_ERR_UNSUPPORTED).
  * ot2d_set_dtpseudo sets the dual time of sub-iterations 1..n
  * (dtpseudo(itn) of the deck; default: dualtime for all, n <= 16). */
 int32_t ot2d_set_dtpseudo(ot2d_case* c, int32_t n, const double* dtpseudo);
@@ -366,7 +372,8 @@ int32_t ot2d_ihc_free(ot2d_ihc_lists* l);
 int32_t ot2d_set_connectivity(ot2d_case* c, int32_t nsp, const ot2d_ihc_lists* l);
 /* Gathers every instance's grid from the device, runs ot2d_ihc_connect with
  * ihcbc[nmesh] and installs the result (src/Turns_overset.f:294-353).
- * *nfringe_total (may be NULL) receives the fringe cells of all meshes. */
+ * *nfringe_total (may be NULL) receives the fringe cells of all meshes.  The
+ * ihcbc rows are kept: a pitching case cuts again with them every step. */
 int32_t ot2d_connect(ot2d_case* c, const ot2d_ihcbc* ihcbc, int32_t* nfringe_total);
 /* do_interpolations once, outside a step. */
 int32_t ot2d_interpolate(ot2d_case* c);
