@@ -80,3 +80,50 @@ def test_interpolate_entry_point_matches_oracle():
     d.interpolate()
     for im in range(2):
         H.assert_cellwise(d.get_state(im), o.get_state(im), 1e-14, f"mesh {im}")
+
+
+def test_pitching_overset_iterate_equals_steps():
+    """A pitching overset case (the near-body mesh turns every step, the
+    connectivity is cut again after each step's motion): ot2d_iterate over
+    four steps and four ot2d_step calls give the same states, grid and
+    iblank bit for bit; the near-body grid turns and the background's does
+    not; the iblank the steps leave is that of a fresh cut of the turned
+    grids; the graph-only entry points refuse the case, and the launch search
+    leaves its grid alone (no search)."""
+    import turns2d_urans_git_amd as t2d
+    from turns2d_urans_git_amd import grid
+    gms = grid.airfoil_overset(near=(97, 25), back=(61, 61), ds_wall=1e-4)
+    ms = [(m.xv, m.yv, m.bc, m.ihc) for m in gms]
+    deck = dict(fsmach=0.3, alfa=2.0, rey=3.0e6, invisc=0, lamin=0, iturb=1, vnuinf=3.0, irhsy=-3, ilim=1,
+                ilhs=1, iconstant_cfl=0, dt=0.1, epse=0.01, iunst=1, motiontype=1, rf=1.0, angmax=4.0, ntac=1,
+                itnmax=1, timeac=1.0)
+    a, b = _gpu(ms, deck), _gpu(ms, deck)
+    for c in (a, b):
+        c.connect([m[3] for m in ms])
+    xv0 = [a.get_field(im, abi.F_XV).copy() for im in (0, 1)]
+    print("[pitch overset] iterate 4 / step x4", flush=True)
+    rc, _, done = a.iterate(4)
+    assert rc == 0 and done == 4
+    for _ in range(4):
+        b.step()
+    for im in (0, 1):
+        np.testing.assert_array_equal(a.get_state(im), b.get_state(im), err_msg=f"q of mesh {im}")
+        np.testing.assert_array_equal(a.get_iblank(im), b.get_iblank(im), err_msg=f"iblank of mesh {im}")
+        np.testing.assert_array_equal(a.get_field(im, abi.F_XV), b.get_field(im, abi.F_XV))
+    assert not np.array_equal(a.get_field(0, abi.F_XV), xv0[0]), "the near-body mesh did not turn"
+    np.testing.assert_array_equal(a.get_field(1, abi.F_XV), xv0[1], err_msg="the background moved")
+    # the cut the steps left is the cut of the turned grid
+    c = _gpu(ms, deck)
+    for im in (0, 1):
+        c.set_grid(im, a.get_field(im, abi.F_XV), a.get_field(im, abi.F_YV))
+    c.connect([m[3] for m in ms])
+    for im in (0, 1):
+        np.testing.assert_array_equal(a.get_iblank(im), c.get_iblank(im), err_msg=f"cut of the turned mesh {im}")
+    c.close()
+    with pytest.raises(abi.CaseError):
+        a.iterate_async(1)
+    sw, sl, measured = a.autotune(steps=2)
+    assert not measured
+    np.testing.assert_array_equal(a.get_field(0, abi.F_XV), b.get_field(0, abi.F_XV))
+    a.close()
+    b.close()
