@@ -80,7 +80,11 @@ typedef struct ot2d_params {
   /* extensions (no reference counterpart): arithmetic of the LU-SGS
    * operator, 0 = binary64 (reference), 1 = binary32 sweeps */
   int32_t lu_fp32;
-  int32_t reserved[7];
+  /* static_conn of &inputs (src/modules.f:186, 204): 1 keeps the first
+   * connectivity of a moving overset case instead of cutting again after
+   * every step's motion (src/Turns_overset.f:413) */
+  int32_t static_conn;
+  int32_t reserved[6];
 } ot2d_params;
 
 typedef struct ot2d_case ot2d_case;

@@ -200,22 +200,29 @@ DRIVER_OVERSET_PITCH = dict(near=(97, 25), back=(61, 61), nsteps=6, nrest=4, npn
                                       angmax=4.0, dt=0.1, iconstant_cfl=0))
 
 
-def driver_overset_pitch_deck():
+def driver_overset_pitch_deck(static_conn=False):
     c = DRIVER_OVERSET_PITCH
     meshes = grid.airfoil_overset(near=c["near"], back=c["back"], ds_wall=1e-4)
-    return meshes, dict(c["deck"], nsteps=c["nsteps"], nrest=c["nrest"], npnorm=c["npnorm"])
+    extra = dict(static_conn=True) if static_conn else {}
+    return meshes, dict(c["deck"], nsteps=c["nsteps"], nrest=c["nrest"], npnorm=c["npnorm"], **extra)
 
 
-def run_driver_overset_pitch_case():
-    meshes, deck = driver_overset_pitch_deck()
+# static_conn = .true.: the reference keeps the first connectivity while the
+# near-body mesh moves (src/Turns_overset.f:413)
+OVERSET_PITCH_FIXTURES = {"driver_pitch_overset_sa_97x25_61x61": False,
+                          "driver_pitch_overset_static_sa_97x25_61x61": True}
+
+
+def run_driver_overset_pitch_case(name):
+    meshes, deck = driver_overset_pitch_deck(OVERSET_PITCH_FIXTURES[name])
     with tempfile.TemporaryDirectory() as d:
         grid.write_overset_case(d, meshes, **deck)
         with open(os.path.join(d, "inputs")) as fin:
             subprocess.run([PROGRAM], cwd=d, stdin=fin, check=True, stdout=subprocess.DEVNULL,
                            env=dict(os.environ, **ZERO_MALLOC))
         files = _program_outputs(d, ["fort.71", "fort.72", "fort.11", "fort.8", "fort.9", "cp_TS1.dat", "SA.dat"])
-    np.savez_compressed(os.path.join(HERE, "driver_pitch_overset_sa_97x25_61x61.npz"), **files)
-    print("driver_pitch_overset_sa_97x25_61x61", sorted(files))
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **files)
+    print(name, sorted(files))
 
 
 # Time-accurate decks (fixed grids, iunst = 0): Newton sub-iterations with
@@ -323,8 +330,9 @@ if __name__ == "__main__":
     for name in DRIVER_TIME:
         if not only or name in only:
             run_driver_time_case(name)
-    if not only or "driver_pitch_overset_sa_97x25_61x61" in only:
-        run_driver_overset_pitch_case()
+    for name in OVERSET_PITCH_FIXTURES:
+        if not only or name in only:
+            run_driver_overset_pitch_case(name)
     for name, spec in OVERSET_CASES.items():
         if not only or name in only:
             run_overset_case(name, *spec)

@@ -191,17 +191,19 @@ def test_driver_overset_matches_reference_program():
 
 
 @needs_driver
-def test_driver_moving_overset_matches_reference_program():
+@pytest.mark.parametrize("name", ["driver_pitch_overset_sa_97x25_61x61", "driver_pitch_overset_static_sa_97x25_61x61"])
+def test_driver_moving_overset_matches_reference_program(name):
     """A moving overset deck: the near-body SA C-grid pitches (iunst = 1,
     motiontype = 1) inside the fixed Cartesian background, and the
     connectivity is found again after every step's motion
     (src/Turns_overset.f:411-465; ot2d_iterate re-cuts on the host with the
-    ihcbc rows of ot2d_connect).  Norms of both meshes, forces, restart and
-    grid files (the turned grid, the last cut's iblank), surface pressure and
-    SA.dat as the reference program writes them."""
+    ihcbc rows of ot2d_connect), or, with static_conn = .true., kept from the
+    first cut.  Norms of both meshes, forces, restart and grid files (the
+    turned grid, the last cut's iblank), surface pressure and SA.dat as the
+    reference program writes them."""
     import make_golden as mg
-    z = np.load(os.path.join(ROOT, "tests", "golden", "driver_pitch_overset_sa_97x25_61x61.npz"))
-    meshes, deck = mg.driver_overset_pitch_deck()
+    z = np.load(os.path.join(ROOT, "tests", "golden", f"{name}.npz"))
+    meshes, deck = mg.driver_overset_pitch_deck(mg.OVERSET_PITCH_FIXTURES[name])
     with tempfile.TemporaryDirectory() as d:
         grid.write_overset_case(d, meshes, **deck)
         run_driver(d)

@@ -44,11 +44,11 @@ _PARAM_DBLS = ["fsmach", "alfa", "rey", "vnuinf", "tuinf", "dt", "timeac", "epse
 class Params(C.Structure):
     _fields_ = ([(n, C.c_int32) for n in _PARAM_INTS]
                 + [(n, C.c_double) for n in _PARAM_DBLS]
-                + [("lu_fp32", C.c_int32), ("reserved", C.c_int32 * 7)])
+                + [("lu_fp32", C.c_int32), ("static_conn", C.c_int32), ("reserved", C.c_int32 * 6)])
 
     def update(self, **kw):
         for k, v in kw.items():
-            if k in _PARAM_INTS or k == "lu_fp32":
+            if k in _PARAM_INTS or k in ("lu_fp32", "static_conn"):
                 setattr(self, k, int(bool(v)) if isinstance(v, bool) else int(v))
             elif k in _PARAM_DBLS:
                 setattr(self, k, float(v))

@@ -1276,7 +1276,7 @@ static int32_t read_steps_status(ot2d_case* c, int* steps, int* st) {
 // does (src/Turns_overset.f:411-465): the hole cut runs on the host, so its
 // steps are enqueued one at a time around it instead of as a captured graph.
 static bool recut_active(const ot2d_case* c) {
-  return c->moving && c->nmesh > 1 && c->ncase == 1 && !c->ihcbc_saved.empty();
+  return c->moving && c->nmesh > 1 && c->ncase == 1 && !c->ihcbc_saved.empty() && !c->p.static_conn;
 }
 static int32_t connect_impl(ot2d_case* c, const ot2d_ihcbc* ihcbc, int32_t* nfringe_total);
 static int32_t run_recut_steps(ot2d_case* c, int32_t niter) {

@@ -34,7 +34,8 @@ module ot2d_capi
     real(c_double) :: rf, angmax, fmtip, rartio, totime
     real(c_double) :: gamma, pr, rmue, tinf
     integer(c_int32_t) :: lu_fp32
-    integer(c_int32_t) :: reserved(7)
+    integer(c_int32_t) :: static_conn
+    integer(c_int32_t) :: reserved(6)
   end type ot2d_params
 
   ! one ihcbc.<n> file (ihc/src/ihc.f90:147-208)

@@ -122,6 +122,7 @@ program ot2d_run
   p%fsmach = fsmach; p%alfa = alfa; p%rey = rey; p%vnuinf = vnuinf; p%tuinf = tuinf
   p%dt = dt; p%timeac = timeac; p%epse = epse; p%Mp = Mp
   p%rf = rf; p%angmax = angmax; p%fmtip = fmtip; p%rartio = rartio; p%totime = totime
+  p%static_conn = merge(1, 0, static_conn)
   call get_environment_variable('OT2D_LU', luenv)
   p%lu_fp32 = merge(1, 0, trim(luenv) == 'fp32')
 
