@@ -46,8 +46,8 @@ def _c_layout():
 #include "ot2d.h"
 int main(void) {
   printf("%zu %zu %zu\n", sizeof(ot2d_params), sizeof(ot2d_mesh_desc), sizeof(ot2d_bcseg));
-  printf("%zu %zu %zu %zu\n", offsetof(ot2d_params, fsmach), offsetof(ot2d_params, tinf),
-         offsetof(ot2d_params, lu_fp32), offsetof(ot2d_mesh_desc, bc));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(ot2d_params, fsmach), offsetof(ot2d_params, tinf),
+         offsetof(ot2d_params, lu_fp32), offsetof(ot2d_params, static_conn), offsetof(ot2d_mesh_desc, bc));
   printf("%d %d\n", OT2D_NSTAGE, OT2D_F_COUNT);
   return 0;
 }
@@ -62,13 +62,14 @@ int main(void) {
 
 
 def test_ctypes_mirror_matches_header_layout():
-    (sp, sm, sb, o_fs, o_tinf, o_lu, o_bc, nstage, nfield) = _c_layout()
+    (sp, sm, sb, o_fs, o_tinf, o_lu, o_sc, o_bc, nstage, nfield) = _c_layout()
     assert C.sizeof(abi.Params) == sp
     assert C.sizeof(abi.MeshDesc) == sm
     assert C.sizeof(abi.BCSeg) == sb
     assert abi.Params.fsmach.offset == o_fs
     assert abi.Params.tinf.offset == o_tinf
     assert abi.Params.lu_fp32.offset == o_lu
+    assert abi.Params.static_conn.offset == o_sc
     assert abi.MeshDesc.bc.offset == o_bc
     assert abi.NSTAGE == nstage
     assert abi.F_VGV + 1 == nfield
